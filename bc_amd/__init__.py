@@ -1,0 +1,362 @@
+"""bc_amd — MI355X-native betweenness / closeness engine (Python host mirror).
+
+A thin ctypes layer over the C-ABI in ``include/cent.h`` (``bc_amd/libcent.so``).
+The names mirror the reference's operator interface for the hot path:
+
+=====================================  ======================================================
+``buildGraph(path)`` / ``createCSR``   Lib/AdjList/AdjList.c:13-150, Lib/CSR/CSR.c:9-108
+``computeBC(csr, BCs)``                Seq_Brandes/Brandes.c:224-372 (BCs accumulated, +=)
+``computeCC(csr, CCs)``                Seq_CC/CC.cpp:37-117 (CCs assigned per source id)
+``compute_D1_AP_CC_shareBased_DegreeOrder(csr, CCs)``  Seq_CC/CC.cpp:1728-2052
+=====================================  ======================================================
+
+Everything runs on the HIP engine; there is no CPU fallback.  If the library
+is not built this module raises ImportError, and engine calls on a machine
+without a gfx950 device raise CentError(CENT_E_DEVICE).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcent.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'` "
+        "or `make -C bc_amd`")
+
+_lib = C.CDLL(LIB_PATH)
+
+# ---------------------------------------------------------------- statuses
+CENT_OK, CENT_E_INVAL, CENT_E_OOM, CENT_E_DEVICE, CENT_E_OVERFLOW, CENT_E_IO = 0, 1, 2, 3, 4, 5
+CENT_E_BUSY, CENT_E_CANCELLED, CENT_E_NOTFOUND, CENT_E_UNSUPPORTED = 6, 7, 8, 9
+CENT_BC_TEXTBOOK, CENT_BC_REF_COMPAT = 0, 1
+CENT_OPT_TIMING = 1
+CENT_CSR_LCC = 1
+
+_lib.cent_strerror.restype = C.c_char_p
+_lib.cent_strerror.argtypes = [C.c_int]
+
+
+class CentError(RuntimeError):
+    def __init__(self, status: int, where: str, detail: str = ""):
+        self.status = status
+        msg = f"{where}: {_lib.cent_strerror(status).decode()} (status {status})"
+        if detail:
+            msg += f" [{detail}]"
+        super().__init__(msg)
+
+
+def _check(status: int, where: str, detail: str = "") -> None:
+    if status != CENT_OK:
+        raise CentError(status, where, detail)
+
+
+# ---------------------------------------------------------------- CSR
+class _CSR(C.Structure):
+    _fields_ = [
+        ("csrV", C.POINTER(C.c_int32)),
+        ("csrE", C.POINTER(C.c_int32)),
+        ("csrNodesDegree", C.POINTER(C.c_int32)),
+        ("csrESize", C.c_int64),
+        ("csrVSize", C.c_int32),
+        ("startNodeID", C.c_int32),
+        ("endNodeID", C.c_int32),
+        ("totalNodeNumber", C.c_int32),
+        ("maxDegree", C.c_int32),
+        ("startAtZero", C.c_int32),
+    ]
+
+
+_P_CSR = C.POINTER(_CSR)
+_I32P = C.POINTER(C.c_int32)
+_I64P = C.POINTER(C.c_int64)
+_F64P = C.POINTER(C.c_double)
+
+_lib.cent_csr_from_edges.argtypes = [C.c_int32, _I32P, _I32P, C.c_int64, C.c_int32,
+                                     C.POINTER(_P_CSR), C.POINTER(_I32P)]
+_lib.cent_csr_from_arrays.argtypes = [C.c_int32, _I32P, _I32P, C.POINTER(_P_CSR)]
+_lib.cent_csr_free.argtypes = [_P_CSR]
+_lib.cent_csr_free.restype = None
+_lib.cent_free.argtypes = [C.c_void_p]
+_lib.cent_free.restype = None
+_lib.cent_load_edgelist.argtypes = [C.c_char_p, C.POINTER(_P_CSR)]
+_lib.cent_gen_rmat.argtypes = [C.c_int32, C.c_int64, C.c_double, C.c_double, C.c_double,
+                               C.c_uint64, C.c_int32, C.POINTER(_P_CSR)]
+_lib.cent_gen_grid.argtypes = [C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_uint64,
+                               C.POINTER(_P_CSR)]
+_lib.cent_gen_blocktree.argtypes = [C.c_int32, C.c_double, C.c_int32, C.c_int32, C.c_double,
+                                    C.c_double, C.c_double, C.c_uint64, C.POINTER(_P_CSR)]
+_lib.cent_sample_sources.argtypes = [C.c_int32, C.c_int32, C.c_uint64, _I32P]
+
+
+def _ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+class CSR:
+    """Owned handle on a library ``struct CSR`` (0-based, immutable).
+
+    ``row_ptr``/``col`` are zero-copy numpy views of the library's arrays.
+    """
+
+    def __init__(self, ptr: _P_CSR):
+        if not ptr:
+            raise ValueError("null CSR")
+        self._p = ptr
+        s = ptr.contents
+        self.n = int(s.csrVSize) - 1
+        self.nnz = int(s.csrESize)
+        self.startNodeID = int(s.startNodeID)
+        self.endNodeID = int(s.endNodeID)
+        self.totalNodeNumber = int(s.totalNodeNumber)
+        self.maxDegree = int(s.maxDegree)
+        self.startAtZero = int(s.startAtZero)
+        self.row_ptr = np.ctypeslib.as_array(s.csrV, shape=(self.n + 1,))
+        self.col = (np.ctypeslib.as_array(s.csrE, shape=(self.nnz,)) if self.nnz
+                    else np.zeros(0, np.int32))
+        self.degree = (np.ctypeslib.as_array(s.csrNodesDegree, shape=(self.n,)) if self.n
+                       else np.zeros(0, np.int32))
+
+    @property
+    def m(self) -> int:
+        """Undirected edge count."""
+        return self.nnz // 2
+
+    def __del__(self):
+        p = getattr(self, "_p", None)
+        if p:
+            _lib.cent_csr_free(p)
+            self._p = None
+
+    @classmethod
+    def from_edges(cls, n: int, u, v, lcc: bool = False):
+        u = np.ascontiguousarray(u, dtype=np.int32)
+        v = np.ascontiguousarray(v, dtype=np.int32)
+        if u.shape != v.shape:
+            raise ValueError("u and v differ in length")
+        out = _P_CSR()
+        old = _I32P()
+        _check(_lib.cent_csr_from_edges(n, _ptr(u, C.c_int32), _ptr(v, C.c_int32), len(u),
+                                        CENT_CSR_LCC if lcc else 0, C.byref(out),
+                                        C.byref(old) if lcc else None), "cent_csr_from_edges")
+        g = cls(out)
+        if lcc and old:
+            g.old_id = np.ctypeslib.as_array(old, shape=(g.n,)).copy() if g.n else np.zeros(0, np.int32)
+            _lib.cent_free(old)
+        return g
+
+    @classmethod
+    def from_arrays(cls, row_ptr, col):
+        rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
+        cl = np.ascontiguousarray(col, dtype=np.int32)
+        out = _P_CSR()
+        _check(_lib.cent_csr_from_arrays(len(rp) - 1, _ptr(rp, C.c_int32), _ptr(cl, C.c_int32),
+                                         C.byref(out)), "cent_csr_from_arrays")
+        return cls(out)
+
+    def sources(self) -> np.ndarray:
+        """The reference's source range startNodeID..endNodeID."""
+        return np.arange(self.startNodeID, self.endNodeID + 1, dtype=np.int32)
+
+
+def buildGraph(path: str) -> CSR:
+    """Load a reference-format edge list (Lib/AdjList/AdjList.c:13-150)."""
+    out = _P_CSR()
+    _check(_lib.cent_load_edgelist(os.fsencode(path), C.byref(out)), "buildGraph", path)
+    return CSR(out)
+
+
+def createCSR(graph: CSR) -> CSR:
+    """The loader already yields a CSR (Lib/CSR/CSR.c:9); kept for call-site parity."""
+    return graph
+
+
+def gen_rmat(scale: int, samples: int, a=0.57, b=0.19, c=0.19, seed=1, lcc=True) -> CSR:
+    out = _P_CSR()
+    _check(_lib.cent_gen_rmat(scale, samples, a, b, c, seed, CENT_CSR_LCC if lcc else 0,
+                              C.byref(out)), "cent_gen_rmat")
+    return CSR(out)
+
+
+def gen_grid(rows: int, cols: int, del_frac=0.05, shortcuts=1000, seed=4) -> CSR:
+    out = _P_CSR()
+    _check(_lib.cent_gen_grid(rows, cols, del_frac, shortcuts, seed, C.byref(out)), "cent_gen_grid")
+    return CSR(out)
+
+
+def gen_blocktree(n_blocks=2000, alpha=1.5, smin=8, smax=50000, avg_deg=6.0,
+                  pendant_frac=0.3, p_geo=0.5, seed=3) -> CSR:
+    out = _P_CSR()
+    _check(_lib.cent_gen_blocktree(n_blocks, alpha, smin, smax, avg_deg, pendant_frac, p_geo,
+                                   seed, C.byref(out)), "cent_gen_blocktree")
+    return CSR(out)
+
+
+def sample_sources(n: int, k: int, seed: int) -> np.ndarray:
+    out = np.empty(k, np.int32)
+    _check(_lib.cent_sample_sources(n, k, seed, _ptr(out, C.c_int32)), "cent_sample_sources")
+    return out
+
+
+# ---------------------------------------------------------------- engine
+class _Opts(C.Structure):
+    _fields_ = [("bc_mode", C.c_int32), ("flags", C.c_int32), ("block_size", C.c_int32),
+                ("grid_waves", C.c_int32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("ms_fwd", C.c_double), ("ms_bwd", C.c_double), ("ms_total", C.c_double),
+                ("launches_fwd", C.c_int64), ("launches_bwd", C.c_int64),
+                ("batches", C.c_int64), ("levels", C.c_int64), ("dag_edges", C.c_int64),
+                ("reached", C.c_int64), ("edges_reached", C.c_int64)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_lib.cent_ctx_create.argtypes = [C.c_int32, C.POINTER(C.c_void_p)]
+_lib.cent_ctx_destroy.argtypes = [C.c_void_p]
+_lib.cent_ctx_destroy.restype = None
+_lib.cent_ctx_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
+_lib.cent_ctx_reset_stats.argtypes = [C.c_void_p]
+_lib.cent_ctx_last_error.argtypes = [C.c_void_p]
+_lib.cent_ctx_last_error.restype = C.c_char_p
+_lib.cent_graph_from_csr.argtypes = [C.c_void_p, _I32P, _I32P, C.c_int32, C.c_int64,
+                                     C.POINTER(C.c_void_p)]
+_lib.cent_graph_free.argtypes = [C.c_void_p]
+_lib.cent_graph_free.restype = None
+_lib.cent_bc.argtypes = [C.c_void_p, C.c_void_p, _I32P, C.c_int32, _F64P, _I64P, _I64P,
+                         C.POINTER(_Opts)]
+_lib.cent_cc.argtypes = [C.c_void_p, C.c_void_p, _I32P, C.c_int32, _I64P, _I64P,
+                         C.POINTER(_Opts)]
+_lib.computeBC.argtypes = [_P_CSR, _F64P]
+_lib.computeBC_compat.argtypes = [_P_CSR, _F64P]
+_lib.computeCC.argtypes = [_P_CSR, _I64P]
+
+
+def _opts(mode=CENT_BC_TEXTBOOK, timing=False, grid_waves=0) -> _Opts:
+    o = _Opts()
+    o.bc_mode = mode
+    o.flags = CENT_OPT_TIMING if timing else 0
+    o.grid_waves = grid_waves
+    return o
+
+
+class DeviceGraph:
+    def __init__(self, ctx: "Context", csr: CSR):
+        self.ctx = ctx
+        self.n = csr.n
+        self.nnz = csr.nnz
+        self._csr = csr
+        h = C.c_void_p()
+        _check(_lib.cent_graph_from_csr(ctx._h, _ptr(csr.row_ptr, C.c_int32),
+                                        _ptr(csr.col, C.c_int32) if csr.nnz else None,
+                                        csr.n, csr.nnz, C.byref(h)),
+               "cent_graph_from_csr", ctx.last_error())
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.cent_graph_free(h)
+            self._h = None
+
+
+class Context:
+    """One HIP device + stream + workspace.  Not thread-safe: one per thread."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _check(_lib.cent_ctx_create(device, C.byref(h)), "cent_ctx_create",
+               "needs a gfx950 (MI355X) device")
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.cent_ctx_destroy(h)
+            self._h = None
+
+    def last_error(self) -> str:
+        return _lib.cent_ctx_last_error(self._h).decode()
+
+    def upload(self, csr: CSR) -> DeviceGraph:
+        return DeviceGraph(self, csr)
+
+    def stats(self) -> dict:
+        s = Stats()
+        _check(_lib.cent_ctx_stats(self._h, C.byref(s)), "cent_ctx_stats")
+        return s.as_dict()
+
+    def reset_stats(self) -> None:
+        _check(_lib.cent_ctx_reset_stats(self._h), "cent_ctx_reset_stats")
+
+    @staticmethod
+    def _src(g: DeviceGraph, sources):
+        if sources is None:
+            return None, -1, g.n
+        s = np.ascontiguousarray(sources, dtype=np.int32)
+        return s, len(s), len(s)
+
+    def bc(self, g: DeviceGraph, sources=None, mode: int = CENT_BC_TEXTBOOK, bc=None,
+           farness: bool = True, dist_sum: bool = False, timing: bool = False,
+           grid_waves: int = 0):
+        """Brandes BC from ``sources`` (None: all).  Returns (bc, farness, dist_sum)."""
+        s, k, kk = self._src(g, sources)
+        out = np.zeros(g.n, np.float64) if bc is None else bc
+        far = np.zeros(max(kk, 1), np.int64) if farness else None
+        ds = np.zeros(max(g.n, 1), np.int64) if dist_sum else None
+        o = _opts(mode, timing, grid_waves)
+        _check(_lib.cent_bc(self._h, g._h, _ptr(s, C.c_int32) if s is not None and k else None, k,
+                            _ptr(out, C.c_double) if g.n else None,
+                            _ptr(far, C.c_int64) if far is not None else None,
+                            _ptr(ds, C.c_int64) if ds is not None else None, C.byref(o)),
+               "cent_bc", self.last_error())
+        return out, (far[:kk] if far is not None else None), (ds[:g.n] if ds is not None else None)
+
+    def cc(self, g: DeviceGraph, sources=None, dist_sum: bool = False, timing: bool = False,
+           grid_waves: int = 0):
+        """Closeness (farness) from ``sources`` (None: all).  Returns (farness, dist_sum)."""
+        s, k, kk = self._src(g, sources)
+        far = np.zeros(max(kk, 1), np.int64)
+        ds = np.zeros(max(g.n, 1), np.int64) if dist_sum else None
+        o = _opts(0, timing, grid_waves)
+        _check(_lib.cent_cc(self._h, g._h, _ptr(s, C.c_int32) if s is not None and k else None, k,
+                            _ptr(far, C.c_int64),
+                            _ptr(ds, C.c_int64) if ds is not None else None, C.byref(o)),
+               "cent_cc", self.last_error())
+        return far[:kk], (ds[:g.n] if ds is not None else None)
+
+
+# ------------------------------------------- reference-shaped entry points
+def computeBC(csr: CSR, BCs: np.ndarray, compat: bool = False) -> np.ndarray:
+    """Brandes BC over sources startNodeID..endNodeID, accumulated into BCs
+    (float64, length n) like Seq_Brandes/Brandes.c:224 accumulates into its
+    caller-zeroed float array.  compat=True uses the reference's per-predecessor
+    accumulation (Brandes.c:317-327)."""
+    if BCs.dtype != np.float64 or len(BCs) < csr.n or not BCs.flags.c_contiguous:
+        raise ValueError("BCs must be a contiguous float64 array of length n")
+    fn = _lib.computeBC_compat if compat else _lib.computeBC
+    _check(fn(csr._p, _ptr(BCs, C.c_double)), "computeBC")
+    return BCs
+
+
+def computeCC(csr: CSR, CCs: np.ndarray) -> np.ndarray:
+    """Farness of every source startNodeID..endNodeID, assigned into CCs[id]
+    (int64), like Seq_CC/CC.cpp:37-117."""
+    if CCs.dtype != np.int64 or len(CCs) < csr.n or not CCs.flags.c_contiguous:
+        raise ValueError("CCs must be a contiguous int64 array of length n")
+    _check(_lib.computeCC(csr._p, _ptr(CCs, C.c_int64)), "computeCC")
+    return CCs
+
+
+__all__ = [
+    "CSR", "CentError", "Context", "DeviceGraph", "buildGraph", "createCSR", "computeBC",
+    "computeCC", "gen_rmat", "gen_grid", "gen_blocktree", "sample_sources", "LIB_PATH",
+    "CENT_BC_TEXTBOOK", "CENT_BC_REF_COMPAT",
+]

@@ -1,0 +1,190 @@
+/*
+ * cent.h — C-ABI of the MI355X centrality engine (betweenness + closeness on
+ * undirected, unweighted graphs).
+ *
+ * Every entry point returns an int status (CENT_OK == 0) and never exits the
+ * process.  Graph ids are 0-based int32; edge offsets are int32 (the engine
+ * rejects graphs with more than INT32_MAX directed edges).  Host buffers are
+ * owned by the caller; device buffers are owned by the engine.  Inputs are
+ * never mutated.
+ *
+ * Reference interfaces replaced (paths relative to the XPPGX/BC tree):
+ *   struct CSR / createCSR          Lib/CSR/CSR.h:34-97, Lib/CSR/CSR.c:9-108
+ *   buildGraph (edge-list loader)   Lib/AdjList/AdjList.h:53, AdjList.c:13-150
+ *   D1Folding                       Lib/D1Process/D1Process.c:5-154
+ *   AP_detection / AP_Copy_And_Split / rebuildGraph
+ *                                   Lib/AP_Process/AP_Process.c:6-225,468-1065,1141-1321
+ *   computeBC                       Seq_Brandes/Brandes.c:224-372
+ *   computeCC                       Seq_CC/CC.cpp:37-117
+ *   compute_D1_AP_CC_shareBased_DegreeOrder
+ *                                   Seq_CC/CC.cpp:1728-2052
+ */
+#ifndef CENT_H
+#define CENT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+enum {
+    CENT_OK          = 0,
+    CENT_E_INVAL     = 1,   /* bad argument (null pointer, id out of range, ...) */
+    CENT_E_OOM       = 2,   /* host or device allocation failed / over budget   */
+    CENT_E_DEVICE    = 3,   /* HIP runtime error, or no usable gfx950 device     */
+    CENT_E_OVERFLOW  = 4,   /* shortest-path count left the FP64 range          */
+    CENT_E_IO        = 5,   /* file could not be opened or parsed               */
+    CENT_E_BUSY      = 6,   /* server queue full                                */
+    CENT_E_CANCELLED = 7,   /* request cancelled                                */
+    CENT_E_NOTFOUND  = 8,   /* unknown request / graph id                       */
+    CENT_E_UNSUPPORTED = 9  /* valid request the engine does not handle         */
+};
+
+const char* cent_strerror(int status);
+
+/* ------------------------------------------------------------------- graph */
+/*
+ * Own CSR.  Field names follow the reference's struct CSR (Lib/CSR/CSR.h:34-95)
+ * where they carry the same meaning; the layout is 0-based and immutable:
+ * vertex v's neighbours are csrE[csrV[v] .. csrV[v+1]).  Self-loops and
+ * duplicate edges are removed on construction, both directions are stored.
+ * For a 1-based input file vertex 0 exists with degree 0 (startNodeID = 1),
+ * so outputs stay indexed by the file's own ids.
+ */
+struct CSR {
+    int32_t* csrV;            /* csrVSize = n + 1 row offsets                  */
+    int32_t* csrE;            /* csrESize column ids                           */
+    int32_t* csrNodesDegree;  /* n degrees                                     */
+    int64_t  csrESize;        /* number of directed edges (2 x undirected)     */
+    int32_t  csrVSize;        /* n + 1                                         */
+    int32_t  startNodeID;     /* first id present in the input (0 or 1)        */
+    int32_t  endNodeID;       /* last id (n - 1)                               */
+    int32_t  totalNodeNumber; /* endNodeID - startNodeID + 1                   */
+    int32_t  maxDegree;
+    int32_t  startAtZero;     /* 1 if the input ids start at 0                 */
+};
+
+/* Build from an undirected edge list (u[i], v[i]), ids in [0, n).
+ * flags: CENT_CSR_LCC keeps only the largest connected component and
+ * relabels it to 0..n'-1 preserving id order; old_id (optional, length n')
+ * receives the original id of every kept vertex when non-null after the
+ * call (allocated by the library, free with cent_free).  Replaces createCSR
+ * (Lib/CSR/CSR.c:9-108), which also did no cleaning. */
+#define CENT_CSR_LCC 1
+int  cent_csr_from_edges(int32_t n, const int32_t* u, const int32_t* v, int64_t m,
+                         int32_t flags, struct CSR** out, int32_t** old_id);
+/* Copy of existing CSR arrays (validated: offsets monotone, ids in range). */
+int  cent_csr_from_arrays(int32_t n, const int32_t* row_ptr, const int32_t* col,
+                          struct CSR** out);
+void cent_csr_free(struct CSR* csr);
+void cent_free(void* p);
+
+/* Loader for the reference's text edge-list format ("n m" header, then m
+ * lines "u v"; 0- or 1-based ids detected as in Lib/AdjList/AdjList.c:39-86).
+ * Replaces buildGraph + createCSR. */
+int  cent_load_edgelist(const char* path, struct CSR** out);
+
+/* Seeded synthetic generators (host C, deterministic for a given seed). */
+/* R-MAT, 2^scale ids, n_samples edge draws with quadrant probabilities
+ * a, b, c (d = 1-a-b-c), random id permutation, cleaned and symmetrised. */
+int  cent_gen_rmat(int32_t scale, int64_t n_samples, double a, double b, double c,
+                   uint64_t seed, int32_t flags, struct CSR** out);
+/* rows x cols 4-neighbour grid, del_frac of the edges removed uniformly at
+ * random, largest component kept, then n_shortcuts random extra edges. */
+int  cent_gen_grid(int32_t rows, int32_t cols, double del_frac, int32_t n_shortcuts,
+                   uint64_t seed, struct CSR** out);
+/* Block tree: n_blocks biconnected blocks (size Pareto(alpha) clipped to
+ * [smin, smax], ER of average degree avg_deg plus a Hamiltonian cycle),
+ * joined in a random tree through shared articulation vertices, then
+ * pendant_frac x (block vertices) extra vertices as pendant chains whose
+ * lengths are Geometric(p_geo) on {1, 2, ...}. */
+int  cent_gen_blocktree(int32_t n_blocks, double alpha, int32_t smin, int32_t smax,
+                        double avg_deg, double pendant_frac, double p_geo,
+                        uint64_t seed, struct CSR** out);
+
+/* Source sampling: k ids drawn uniformly without replacement from [0, n)
+ * (k <= n), in draw order. */
+int  cent_sample_sources(int32_t n, int32_t k, uint64_t seed, int32_t* out);
+
+/* ---------------------------------------------------------- engine (GPU) */
+typedef struct cent_ctx   cent_ctx;    /* one HIP device + stream + workspace  */
+typedef struct cent_graph cent_graph;  /* device-resident CSR                  */
+
+/* BC accumulation formula. TEXTBOOK: BC[w] += delta_s[w] once per source
+ * (Brandes 2001).  REF_COMPAT: BC[w] += delta_s[w] once per shortest-path
+ * predecessor of w, the formula of Seq_Brandes/Brandes.c:317-327. */
+#define CENT_BC_TEXTBOOK   0
+#define CENT_BC_REF_COMPAT 1
+
+/* opts.flags */
+#define CENT_OPT_TIMING    1   /* record per-kernel HIP events (stats)         */
+
+typedef struct cent_opts {
+    int32_t bc_mode;      /* CENT_BC_TEXTBOOK or CENT_BC_REF_COMPAT          */
+    int32_t flags;        /* CENT_OPT_*                                      */
+    int32_t block_size;   /* 0 = tuned default                               */
+    int32_t grid_waves;   /* 0 = tuned default (waves per level launch)     */
+} cent_opts;
+
+typedef struct cent_stats {
+    double  ms_fwd;           /* summed duration of forward level kernels    */
+    double  ms_bwd;           /* summed duration of backward level kernels   */
+    double  ms_total;         /* wall time of the last call (host clock)     */
+    int64_t launches_fwd;
+    int64_t launches_bwd;
+    int64_t batches;          /* 64-lane source batches processed            */
+    int64_t levels;           /* summed BFS depth over batches               */
+    int64_t dag_edges;        /* sum over sources of |shortest-path DAG|     */
+    int64_t reached;          /* sum over sources of reached vertices        */
+    int64_t edges_reached;    /* sum over sources of directed edges of the
+                                 reached component (2 x m_cc(s))             */
+} cent_stats;
+
+int  cent_ctx_create(int32_t device, cent_ctx** out);
+void cent_ctx_destroy(cent_ctx* ctx);
+int  cent_ctx_stats(const cent_ctx* ctx, cent_stats* out);
+int  cent_ctx_reset_stats(cent_ctx* ctx);
+/* Text of the last HIP error seen by this context (empty if none). */
+const char* cent_ctx_last_error(const cent_ctx* ctx);
+
+int  cent_graph_upload(cent_ctx* ctx, const struct CSR* csr, cent_graph** out);
+int  cent_graph_from_csr(cent_ctx* ctx, const int32_t* row_ptr, const int32_t* col,
+                         int32_t n, int64_t nnz, cent_graph** out);
+void cent_graph_free(cent_graph* g);
+
+/*
+ * Brandes betweenness from the k sources src[0..k) (k == -1: every vertex),
+ * computed 64 sources per wavefront in FP64.  bc (length n) is ACCUMULATED
+ * (+=) like computeBC (Seq_Brandes/Brandes.c:224, caller-zeroed output).
+ * Optional outputs, may be null:
+ *   farness[i]  = sum of BFS distances from src[i] over its component
+ *                 (assigned, int64)                 — Seq_CC/CC.cpp:94
+ *   dist_sum[v] += sum over the sources of d(src, v) (int64, push form)
+ * Returns CENT_E_OVERFLOW if a path count leaves the FP64 range.
+ */
+int  cent_bc(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k,
+             double* bc, int64_t* farness, int64_t* dist_sum, const cent_opts* opts);
+
+/*
+ * Closeness by bit-parallel multi-source BFS (64 sources per uint64 word):
+ * farness[i] assigned for every source (k == -1: all vertices, indexed by
+ * vertex), dist_sum accumulated, both int64.  Replaces computeCC
+ * (Seq_CC/CC.cpp:37-117).
+ */
+int  cent_cc(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k,
+             int64_t* farness, int64_t* dist_sum, const cent_opts* opts);
+
+/* ------------------------------------------- reference-shaped entry points */
+/* Drop-in shapes of the reference functions over the own struct CSR, running
+ * on a process-wide default context (device 0).  Ids startNodeID..endNodeID
+ * are the sources.  BCs is accumulated, CCs is assigned. */
+int  computeBC(const struct CSR* csr, double* BCs);                      /* Brandes.c:224 */
+int  computeBC_compat(const struct CSR* csr, double* BCs);               /* Brandes.c:317-327 formula */
+int  computeCC(const struct CSR* csr, int64_t* CCs);                     /* CC.cpp:37 */
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CENT_H */

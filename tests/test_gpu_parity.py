@@ -1,0 +1,143 @@
+"""GPU parity: the HIP engine (through the C-ABI) against the CPU oracle.
+
+Bar: farness / distance sums / reached counts bit-exact (int64); BC within
+1e-9 relative (FP64, summation order differs from the oracle's).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import _oracle as O
+import bc_amd
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+BC_RTOL = 1e-9
+
+
+def assert_bc_close(got, want, what=""):
+    scale = max(1.0, float(np.abs(want).max()) if len(want) else 1.0)
+    np.testing.assert_allclose(got, want, rtol=BC_RTOL, atol=BC_RTOL * scale, err_msg=what)
+
+
+FIXTURES = ["karate.txt", "dolphins.txt", "football.txt", "polbooks.txt", "testdata.txt",
+            "testdata1.txt", "testdata2.txt", "ia-fb-messages.mtx"]
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_fixture_all_sources(ctx, name):
+    print(f"[gpu] fixture {name}", flush=True)
+    g = bc_amd.buildGraph(O.dataset_path(name))
+    dg = ctx.upload(g)
+    src = g.sources()
+    for mode in (bc_amd.CENT_BC_TEXTBOOK, bc_amd.CENT_BC_REF_COMPAT):
+        bc, far, ds = ctx.bc(dg, src, mode=mode, dist_sum=True)
+        obc, ofar, ods, _ = O.brandes(g, src, mode=mode)
+        assert_bc_close(bc, obc, f"{name} mode {mode}")
+        assert np.array_equal(far, ofar)
+        assert np.array_equal(ds, ods)
+    cfar, cds = ctx.cc(dg, src, dist_sum=True)
+    assert np.array_equal(cfar, ofar)
+    assert np.array_equal(cds, ods)
+
+
+def test_reference_answers_on_gpu(ctx):
+    """karate_BC_Score.txt (reference formula, FP32 print) and CCs[7] = 86."""
+    g = bc_amd.buildGraph(O.dataset_path("karate.txt"))
+    BCs = np.zeros(g.n)
+    bc_amd.computeBC(g, BCs, compat=True)
+    lines = open(os.path.join(GOLDEN, "karate_BC_Score.txt")).read().split()
+    ref = np.array([float(x) for x in lines[2:]])
+    np.testing.assert_allclose(BCs[1:35], ref, rtol=2e-6, atol=2e-6)
+    CCs = np.zeros(g.n, np.int64)
+    bc_amd.computeCC(g, CCs)
+    assert CCs[7] == 86
+    ref = json.load(open(os.path.join(GOLDEN, "ref_golden.json")))["karate.txt"]
+    assert CCs[1:35].tolist() == ref["cc"]
+
+
+def test_musae_git_known_answer(ctx):
+    g = bc_amd.buildGraph(O.dataset_path("musae_git.txt"))
+    dg = ctx.upload(g)
+    src = np.array([7, 0, 1, 37699], np.int32)
+    far, _ = ctx.cc(dg, src)
+    assert far[0] == 117726
+    assert np.array_equal(far, O.farness(g, src))
+
+
+def test_rmat14_all_sources(ctx):
+    """R-MAT 2^14, avg degree 8, LCC, all sources: exact BC + farness."""
+    g = bc_amd.gen_rmat(14, 8 << 14, seed=1)
+    print(f"[gpu] rmat14 n={g.n} m={g.m}", flush=True)
+    dg = ctx.upload(g)
+    bc, far, ds = ctx.bc(dg, None, dist_sum=True)
+    obc, ofar, ods, _ = O.brandes(g, None, threads=min(16, O.max_threads()))
+    assert_bc_close(bc, obc, "rmat14")
+    assert np.array_equal(far, ofar) and np.array_equal(ds, ods)
+    cfar, cds = ctx.cc(dg, None, dist_sum=True)
+    assert np.array_equal(cfar, ofar) and np.array_equal(cds, ods)
+
+
+def test_sampled_ragged_batches(ctx):
+    g = bc_amd.gen_rmat(16, 16 << 16, seed=5)
+    dg = ctx.upload(g)
+    for k in (1, 63, 64, 65, 200):
+        src = bc_amd.sample_sources(g.n, k, 100 + k)
+        bc, far, ds = ctx.bc(dg, src, dist_sum=True)
+        obc, ofar, ods, _ = O.brandes(g, src, threads=min(16, O.max_threads()))
+        assert_bc_close(bc, obc, f"k={k}")
+        assert np.array_equal(far, ofar) and np.array_equal(ds, ods)
+
+
+def test_edge_cases(ctx):
+    # disconnected graph with an isolated vertex; duplicate sources; empty source list
+    g = bc_amd.CSR.from_edges(9, [0, 1, 3, 4, 4, 6], [1, 2, 4, 5, 6, 7])
+    dg = ctx.upload(g)
+    src = np.array([8, 0, 0, 4, 7, 2], np.int32)
+    bc, far, ds = ctx.bc(dg, src, dist_sum=True)
+    obc, ofar, ods, _ = O.brandes(g, src)
+    assert_bc_close(bc, obc)
+    assert np.array_equal(far, ofar) and np.array_equal(ds, ods)
+    assert far[0] == 0
+    bc, far, ds = ctx.bc(dg, np.zeros(0, np.int32), dist_sum=True)
+    assert np.all(bc == 0) and len(far) == 0 and np.all(ds == 0)
+    with pytest.raises(bc_amd.CentError) as e:
+        ctx.bc(dg, np.array([9], np.int32))
+    assert e.value.status == bc_amd.CENT_E_INVAL
+    # BC accumulates into the caller's array (Brandes.c:224 contract)
+    acc = np.full(g.n, 1.5)
+    ctx.bc(dg, src, bc=acc)
+    assert_bc_close(acc - 1.5, obc)
+
+
+def test_deterministic(ctx):
+    g = bc_amd.gen_rmat(15, 16 << 15, seed=9)
+    dg = ctx.upload(g)
+    src = bc_amd.sample_sources(g.n, 130, 1)
+    a = ctx.bc(dg, src)[0]
+    b = ctx.bc(dg, src)[0]
+    assert np.array_equal(a, b)
+
+
+def test_large_rmat_properties(ctx):
+    """R-MAT 2^20 (the sampled-BC workload's graph) — size-independent checks:
+    sum identity per source set, and a 64-source subset bit-exact vs the oracle."""
+    g = bc_amd.gen_rmat(20, 16 << 20, seed=2)
+    print(f"[gpu] rmat20 n={g.n} m={g.m}", flush=True)
+    dg = ctx.upload(g)
+    src = bc_amd.sample_sources(g.n, 256, 2)
+    ctx.reset_stats()
+    bc, far, ds = ctx.bc(dg, src, dist_sum=True)
+    st = ctx.stats()
+    reached = st["reached"]
+    # sum_v delta_s(v) = sum_{t != s} (d(s,t) - 1) = farness(s) - (reached_s - 1)
+    assert np.isclose(bc.sum(), far.sum() - (reached - len(src)), rtol=1e-9)
+    assert ds.sum() == far.sum()
+    sub = src[:64]
+    obc, ofar, ods, odag = O.brandes(g, sub, threads=min(16, O.max_threads()))
+    bc2, far2, ds2 = ctx.bc(dg, sub, dist_sum=True)
+    assert_bc_close(bc2, obc, "rmat20 subset")
+    assert np.array_equal(far2, ofar) and np.array_equal(ds2, ods)
