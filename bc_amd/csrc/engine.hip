@@ -42,8 +42,9 @@ struct cent_ctx {
     cent_stats stats{};
     // workspace
     DevBuf<uint64_t> vis, F;
-    DevBuf<double> sigma, coef, bc;
-    DevBuf<uint32_t> npred;
+    DevBuf<double> sigma, coef, bc, hp_acc;
+    DevBuf<uint32_t> npred, hp_cnt;
+    DevBuf<uint64_t> hp_or;
     DevBuf<int64_t> dist_sum, far, part_far;
     DevBuf<unsigned long long> part_cnt, cnt;
     DevBuf<int> src, flags;
@@ -53,6 +54,10 @@ struct cent_ctx {
     size_t ev_used = 0;
 };
 
+// Rows longer than kHubDeg are scanned as kHubDeg-edge segments, one
+// wavefront each (see msbfs_kernels.h).
+constexpr int kHubDeg = 256;
+
 struct cent_graph {
     cent_ctx* ctx = nullptr;
     int32_t n = 0;
@@ -61,6 +66,11 @@ struct cent_graph {
     int* col = nullptr;
     std::vector<int32_t> comp;        // component id per vertex (host)
     std::vector<int64_t> comp_nnz;    // directed edges per component (host)
+    // hub segments (device): seg_v/seg_beg/seg_end [nseg], hub_v [nhub], hub_seg_off [nhub+1]
+    int hub_deg = kHubDeg;
+    int nseg = 0, nhub = 0;
+    int* hub_buf = nullptr;           // one allocation holding the five arrays
+    int *seg_v = nullptr, *seg_beg = nullptr, *seg_end = nullptr, *hub_v = nullptr, *hub_seg_off = nullptr;
 };
 
 static void set_error(cent_ctx* ctx, hipError_t e, const char* what) {
@@ -116,6 +126,7 @@ extern "C" void cent_ctx_destroy(cent_ctx* ctx) {
     release(ctx->vis); release(ctx->F); release(ctx->sigma); release(ctx->coef); release(ctx->bc);
     release(ctx->npred); release(ctx->dist_sum); release(ctx->far); release(ctx->part_far);
     release(ctx->part_cnt); release(ctx->cnt); release(ctx->src); release(ctx->flags);
+    release(ctx->hp_acc); release(ctx->hp_cnt); release(ctx->hp_or);
     if (ctx->h_flags) (void)hipHostFree(ctx->h_flags);
     for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
@@ -178,10 +189,47 @@ extern "C" int cent_graph_from_csr(cent_ctx* ctx, const int32_t* row_ptr, const 
     }
     if (hipMemcpy(g->rp, row_ptr, sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice) != hipSuccess ||
         (nnz > 0 && hipMemcpy(g->col, col, sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice) != hipSuccess)) {
-        (void)hipFree(g->rp);
-        (void)hipFree(g->col);
-        delete g;
+        cent_graph_free(g);
         return CENT_E_DEVICE;
+    }
+    // hub segments
+    std::vector<int> sv, sb, se, hv, ho;
+    ho.push_back(0);
+    for (int32_t v = 0; v < n; ++v) {
+        const int b = row_ptr[v], e = row_ptr[v + 1];
+        if (e - b <= g->hub_deg) continue;
+        hv.push_back(v);
+        for (int x = b; x < e; x += g->hub_deg) {
+            sv.push_back(v);
+            sb.push_back(x);
+            se.push_back(x + g->hub_deg < e ? x + g->hub_deg : e);
+        }
+        ho.push_back((int)sv.size());
+    }
+    g->nseg = (int)sv.size();
+    g->nhub = (int)hv.size();
+    if (g->nhub) {
+        const size_t tot = 3 * sv.size() + hv.size() + ho.size();
+        if (hipMalloc((void**)&g->hub_buf, sizeof(int) * tot) != hipSuccess) {
+            cent_graph_free(g);
+            return CENT_E_OOM;
+        }
+        g->seg_v = g->hub_buf;
+        g->seg_beg = g->seg_v + g->nseg;
+        g->seg_end = g->seg_beg + g->nseg;
+        g->hub_v = g->seg_end + g->nseg;
+        g->hub_seg_off = g->hub_v + g->nhub;
+        std::vector<int> all;
+        all.reserve(tot);
+        all.insert(all.end(), sv.begin(), sv.end());
+        all.insert(all.end(), sb.begin(), sb.end());
+        all.insert(all.end(), se.begin(), se.end());
+        all.insert(all.end(), hv.begin(), hv.end());
+        all.insert(all.end(), ho.begin(), ho.end());
+        if (hipMemcpy(g->hub_buf, all.data(), sizeof(int) * tot, hipMemcpyHostToDevice) != hipSuccess) {
+            cent_graph_free(g);
+            return CENT_E_DEVICE;
+        }
     }
     *out = g;
     return CENT_OK;
@@ -197,6 +245,7 @@ extern "C" void cent_graph_free(cent_graph* g) {
     if (g->ctx) (void)hipSetDevice(g->ctx->device);
     if (g->rp) (void)hipFree(g->rp);
     if (g->col) (void)hipFree(g->col);
+    if (g->hub_buf) (void)hipFree(g->hub_buf);
     delete g;
 }
 
@@ -252,6 +301,21 @@ static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
     if ((st = ensure(ctx, ctx->cnt, 2))) return st;
     if ((st = ensure(ctx, ctx->src, (size_t)(k > 0 ? k : 1)))) return st;
     const int max_levels = n + 2;
+    // hub segment partials
+    HubArgs ha{};
+    int Gseg = 1, Ghub = 1;
+    if (g->nhub) {
+        if ((st = ensure(ctx, ctx->hp_acc, (size_t)g->nseg * kWave))) return st;
+        if ((st = ensure(ctx, ctx->hp_cnt, (size_t)g->nseg * kWave))) return st;
+        if ((st = ensure(ctx, ctx->hp_or, (size_t)g->nseg))) return st;
+        ha.nseg = g->nseg; ha.seg_v = g->seg_v; ha.seg_beg = g->seg_beg; ha.seg_end = g->seg_end;
+        ha.nhub = g->nhub; ha.hub_v = g->hub_v; ha.hub_seg_off = g->hub_seg_off;
+        ha.part_acc = ctx->hp_acc.p; ha.part_cnt = ctx->hp_cnt.p; ha.part_or = ctx->hp_or.p;
+        Gseg = (g->nseg + kWavesPerBlock - 1) / kWavesPerBlock;
+        if (Gseg > 4096) Gseg = 4096;
+        Ghub = (g->nhub + kWavesPerBlock - 1) / kWavesPerBlock;
+        if (Ghub > G) Ghub = G;  // k_fwd_hubfin flushes into the G per-block slots
+    }
     if ((st = ensure(ctx, ctx->flags, 2))) return st;
     if (!ctx->h_flags) {
         HIPTRY(hipHostMalloc((void**)&ctx->h_flags, sizeof(int) * 2, hipHostMallocDefault));
@@ -305,7 +369,8 @@ static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
             }
             HIPTRY(hipMemsetAsync(ctx->flags.p, 0, sizeof(int), s));
             FwdArgs fa;
-            fa.n = n; fa.rp = g->rp; fa.col = g->col; fa.vis = ctx->vis.p; fa.Fd = Fd; fa.Fn = Fn;
+            fa.n = n; fa.hub_deg = g->hub_deg; fa.rp = g->rp; fa.col = g->col; fa.vis = ctx->vis.p;
+            fa.Fd = Fd; fa.Fn = Fn;
             fa.sigma = with_bc ? ctx->sigma.p : nullptr;
             fa.npred = compat ? ctx->npred.p : nullptr;
             fa.dist_sum = dist_sum ? ctx->dist_sum.p : nullptr;
@@ -322,6 +387,17 @@ static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
                 k_fwd<false, false><<<G, kBlock, 0, s>>>(fa);
             }
             HIPTRY(hipGetLastError());
+            if (g->nhub) {
+                if (with_bc) {
+                    k_fwd_hubseg<true><<<Gseg, kBlock, 0, s>>>(fa, ha);
+                    if (compat) k_fwd_hubfin<true, true><<<Ghub, kBlock, 0, s>>>(fa, ha);
+                    else k_fwd_hubfin<true, false><<<Ghub, kBlock, 0, s>>>(fa, ha);
+                } else {
+                    k_fwd_hubseg<false><<<Gseg, kBlock, 0, s>>>(fa, ha);
+                    k_fwd_hubfin<false, false><<<Ghub, kBlock, 0, s>>>(fa, ha);
+                }
+                HIPTRY(hipGetLastError());
+            }
             if (timing_ok) { HIPTRY(hipEventRecord(e1, s)); tm.fwd.push_back({e0, e1}); }
             ctx->stats.launches_fwd++;
             HIPTRY(hipMemcpyAsync(ctx->h_flags, ctx->flags.p, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
@@ -335,7 +411,7 @@ static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
         if (with_bc) {
             for (int lv = d; lv >= 1; --lv) {
                 BwdArgs ba;
-                ba.n = n; ba.rp = g->rp; ba.col = g->col;
+                ba.n = n; ba.hub_deg = g->hub_deg; ba.rp = g->rp; ba.col = g->col;
                 ba.Fd = ctx->F.p + N * (size_t)lv;
                 ba.Fn = ctx->F.p + N * (size_t)(lv + 1);
                 ba.sigma = ctx->sigma.p; ba.coef = ctx->coef.p;
@@ -346,11 +422,17 @@ static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
                 if (compat) k_bwd<true><<<G, kBlock, 0, s>>>(ba);
                 else k_bwd<false><<<G, kBlock, 0, s>>>(ba);
                 HIPTRY(hipGetLastError());
+                if (g->nhub) {
+                    k_bwd_hubseg<<<Gseg, kBlock, 0, s>>>(ba, ha);
+                    if (compat) k_bwd_hubfin<true><<<Ghub, kBlock, 0, s>>>(ba, ha);
+                    else k_bwd_hubfin<false><<<Ghub, kBlock, 0, s>>>(ba, ha);
+                    HIPTRY(hipGetLastError());
+                }
                 if (timing_ok) { HIPTRY(hipEventRecord(e1, s)); tm.bwd.push_back({e0, e1}); }
                 ctx->stats.launches_bwd++;
             }
         }
-        k_reduce_batch<<<1, kWave, 0, s>>>(ctx->part_far.p, ctx->part_cnt.p, G, kb, ctx->far.p + b0, ctx->cnt.p);
+        k_reduce_batch<<<1, 1024, 0, s>>>(ctx->part_far.p, ctx->part_cnt.p, G, kb, ctx->far.p + b0, ctx->cnt.p);
         HIPTRY(hipGetLastError());
         ++batches;
     }

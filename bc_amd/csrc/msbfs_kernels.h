@@ -19,6 +19,11 @@
 // form) and coef = (1 + delta) / sigma for the level above.  The BC of v gets
 // the lane sum of delta (or of delta * #predecessors in the reference's
 // per-predecessor formula), one deterministic butterfly reduction per level.
+//
+// Load balance: rows longer than the hub threshold are cut into fixed edge
+// segments, each scanned by its own wavefront (k_*_hubseg) into a per-segment
+// lane partial; k_*_hubfin then adds a hub's partials in segment order, so
+// the sums do not depend on scheduling.  The main kernels skip those rows.
 #pragma once
 #include "wave.h"
 
@@ -26,9 +31,24 @@ namespace cent {
 
 constexpr int kBlock = 256;               // 4 waves per workgroup
 constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kUnroll = 4;                // row gathers in flight per wave
+
+struct HubArgs {
+    int nseg;                       // number of hub edge segments
+    const int* __restrict__ seg_v;  // [nseg] owning vertex
+    const int* __restrict__ seg_beg;
+    const int* __restrict__ seg_end;
+    int nhub;
+    const int* __restrict__ hub_v;        // [nhub]
+    const int* __restrict__ hub_seg_off;  // [nhub + 1]
+    double* __restrict__ part_acc;        // [nseg][64]
+    uint32_t* __restrict__ part_cnt;      // [nseg][64]
+    uint64_t* __restrict__ part_or;       // [nseg]
+};
 
 struct FwdArgs {
     int n;
+    int hub_deg;                          // rows longer than this are hub rows
     const int* __restrict__ rp;
     const int* __restrict__ col;
     uint64_t* __restrict__ vis;
@@ -46,109 +66,9 @@ struct FwdArgs {
     int* __restrict__ overflow;           // set to 1 when a path count is inf
 };
 
-template <bool SIGMA, bool COMPAT>
-__global__ __launch_bounds__(kBlock) void k_fwd(FwdArgs a) {
-    __shared__ int64_t s_far[kWavesPerBlock][kWave];
-    __shared__ unsigned long long s_cnt[kWavesPerBlock][2];
-    const int lane = lane_id();
-    const int wid = threadIdx.x >> 6;
-    const int gw = blockIdx.x * kWavesPerBlock + wid;
-    const int nw = gridDim.x * kWavesPerBlock;
-    const uint64_t A = a.A;
-    int64_t my_far = 0;
-    unsigned long long my_new = 0, my_dag = 0;
-    bool ovf = false;
-
-    for (int base = gw * kWave; base < a.n; base += nw * kWave) {
-        const int vl = base + lane;
-        uint64_t visl = A;
-        if (vl < a.n) visl = a.vis[vl];
-        uint64_t work = ballot(visl != A);
-        if (!work) continue;
-        int rbl = 0, rel = 0;
-        if (vl < a.n && visl != A) { rbl = a.rp[vl]; rel = a.rp[vl + 1]; }
-        while (work) {
-            const int b = __builtin_ctzll(work);
-            work &= work - 1;
-            const int v = base + b;
-            const uint64_t old = rl64(visl, b);
-            const uint64_t cand = A & ~old;
-            const int beg = rl32(rbl, b), end = rl32(rel, b);
-            double acc = 0.0;
-            uint32_t cnt = 0;
-            uint64_t orf = 0;
-            for (int j0 = beg; j0 < end; j0 += kWave) {
-                const int j = j0 + lane;
-                int u = 0;
-                uint64_t f = 0;
-                if (j < end) {
-                    u = a.col[j];
-                    f = a.Fd[u] & cand;
-                }
-                if constexpr (SIGMA) {
-                    uint64_t ball = ballot(f != 0);
-                    while (ball) {
-                        const int c = __builtin_ctzll(ball);
-                        ball &= ball - 1;
-                        const int uc = rl32(u, c);
-                        const uint64_t fc = rl64(f, c);
-                        if ((fc >> lane) & 1) {
-                            acc += a.sigma[(size_t)uc * kWave + lane];
-                            ++cnt;
-                        }
-                    }
-                } else {
-                    orf |= f;
-                }
-            }
-            uint64_t nm;
-            if constexpr (SIGMA) nm = ballot(cnt != 0);
-            else nm = wave_or(orf);
-            if (nm) {
-                if ((nm >> lane) & 1) {
-                    if constexpr (SIGMA) {
-                        a.sigma[(size_t)v * kWave + lane] = acc;
-                        if (__builtin_isinf(acc)) ovf = true;
-                    }
-                    if constexpr (COMPAT) a.npred[(size_t)v * kWave + lane] = cnt;
-                    my_far += a.weight ? (int64_t)a.dnext * a.weight[v] : (int64_t)a.dnext;
-                    my_dag += cnt;
-                }
-                if (lane == 0) {
-                    a.Fn[v] = nm;
-                    a.vis[v] = old | nm;
-                    const int pc = __popcll(nm);
-                    if (a.dist_sum) a.dist_sum[v] += (int64_t)a.dnext * pc;
-                    my_new += (unsigned long long)pc;
-                }
-            }
-        }
-    }
-    // block-level reduction of the lane partials, one plain store per block
-    s_far[wid][lane] = my_far;
-    const unsigned long long dag = (unsigned long long)wave_sum_i64((int64_t)my_dag);
-    if (lane == 0) { s_cnt[wid][0] = my_new; s_cnt[wid][1] = dag; }
-    if (ovf) *a.overflow = 1;
-    __syncthreads();
-    if (wid == 0) {
-        int64_t f = 0;
-#pragma unroll
-        for (int w = 0; w < kWavesPerBlock; ++w) f += s_far[w][lane];
-        if (f) a.part_far[(size_t)blockIdx.x * kWave + lane] += f;
-        if (lane < 2) {
-            unsigned long long c = 0;
-#pragma unroll
-            for (int w = 0; w < kWavesPerBlock; ++w) c += s_cnt[w][lane];
-            if (c) {
-                a.part_cnt[(size_t)blockIdx.x * 2 + lane] += c;
-                if (lane == 0) *a.flag = 1;
-            }
-        }
-    }
-}
-
 struct BwdArgs {
     int n;
+    int hub_deg;
     const int* __restrict__ rp;
     const int* __restrict__ col;
     const uint64_t* __restrict__ Fd;      // level d masks
@@ -159,58 +79,300 @@ struct BwdArgs {
     double* __restrict__ bc;
 };
 
-template <bool COMPAT>
-__global__ __launch_bounds__(kBlock) void k_bwd(BwdArgs a) {
-    const int lane = lane_id();
-    const int wid = threadIdx.x >> 6;
-    const int gw = blockIdx.x * kWavesPerBlock + wid;
-    const int nw = gridDim.x * kWavesPerBlock;
-    for (int base = gw * kWave; base < a.n; base += nw * kWave) {
-        const int vl = base + lane;
-        uint64_t fl = 0;
-        if (vl < a.n) fl = a.Fd[vl];
-        uint64_t work = ballot(fl != 0);
-        if (!work) continue;
-        int rbl = 0, rel = 0;
-        if (fl) { rbl = a.rp[vl]; rel = a.rp[vl + 1]; }
-        while (work) {
-            const int b = __builtin_ctzll(work);
-            work &= work - 1;
-            const int v = base + b;
-            const uint64_t m = rl64(fl, b);
-            const int beg = rl32(rbl, b), end = rl32(rel, b);
-            double acc = 0.0;
-            for (int j0 = beg; j0 < end; j0 += kWave) {
-                const int j = j0 + lane;
-                int w = 0;
-                uint64_t g = 0;
-                if (j < end) {
-                    w = a.col[j];
-                    g = a.Fn[w] & m;
+// ---------------------------------------------------------------- gathers
+// Forward scan of edges [beg, end): lane s sums sigma[u][s] (SIGMA) over the
+// neighbours u with bit s in Fd[u] & cand, or ORs the masks (closeness only).
+template <bool SIGMA>
+__device__ __forceinline__ void gather_fwd(const int* __restrict__ col, const uint64_t* __restrict__ Fd,
+                                           const double* __restrict__ sigma, int beg, int end,
+                                           uint64_t cand, int lane, double& acc, uint32_t& cnt,
+                                           uint64_t& orf) {
+    for (int j0 = beg; j0 < end; j0 += kWave) {
+        const int j = j0 + lane;
+        int u = 0;
+        uint64_t f = 0;
+        if (j < end) {
+            u = col[j];
+            f = Fd[u] & cand;
+        }
+        if constexpr (SIGMA) {
+            uint64_t ball = ballot(f != 0);
+            while (ball) {
+                double x[kUnroll];
+#pragma unroll
+                for (int t = 0; t < kUnroll; ++t) {
+                    x[t] = 0.0;
+                    if (ball) {
+                        const int c = __builtin_ctzll(ball);
+                        ball &= ball - 1;
+                        const int uc = rl32(u, c);
+                        const uint64_t fc = rl64(f, c);
+                        if ((fc >> lane) & 1) {
+                            x[t] = sigma[(size_t)uc * kWave + lane];
+                            ++cnt;
+                        }
+                    }
                 }
-                uint64_t ball = ballot(g != 0);
-                while (ball) {
-                    const int c = __builtin_ctzll(ball);
-                    ball &= ball - 1;
-                    const int wc = rl32(w, c);
-                    const uint64_t gc = rl64(g, c);
-                    if ((gc >> lane) & 1) acc += a.coef[(size_t)wc * kWave + lane];
-                }
+#pragma unroll
+                for (int t = 0; t < kUnroll; ++t) acc += x[t];
             }
-            double contrib = 0.0;
-            if ((m >> lane) & 1) {
-                const size_t idx = (size_t)v * kWave + lane;
-                const double sg = a.sigma[idx];
-                const double dl = sg * acc;
-                a.coef[idx] = (1.0 + dl) / sg;
-                contrib = COMPAT ? dl * (double)a.npred[idx] : dl;
-            }
-            const double tot = wave_sum(contrib);
-            if (lane == 0) a.bc[v] += tot;
+        } else {
+            orf |= f;
         }
     }
 }
 
+// Backward scan of edges [beg, end): lane s sums coef[w][s] over neighbours w
+// with bit s in Fn[w] & m.
+__device__ __forceinline__ double gather_bwd(const int* __restrict__ col, const uint64_t* __restrict__ Fn,
+                                             const double* __restrict__ coef, int beg, int end,
+                                             uint64_t m, int lane) {
+    double acc = 0.0;
+    for (int j0 = beg; j0 < end; j0 += kWave) {
+        const int j = j0 + lane;
+        int w = 0;
+        uint64_t g = 0;
+        if (j < end) {
+            w = col[j];
+            g = Fn[w] & m;
+        }
+        uint64_t ball = ballot(g != 0);
+        while (ball) {
+            double x[kUnroll];
+#pragma unroll
+            for (int t = 0; t < kUnroll; ++t) {
+                x[t] = 0.0;
+                if (ball) {
+                    const int c = __builtin_ctzll(ball);
+                    ball &= ball - 1;
+                    const int wc = rl32(w, c);
+                    const uint64_t gc = rl64(g, c);
+                    if ((gc >> lane) & 1) x[t] = coef[(size_t)wc * kWave + lane];
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < kUnroll; ++t) acc += x[t];
+        }
+    }
+    return acc;
+}
+
+// ---------------------------------------------------------------- epilogues
+struct FwdAccum {
+    int64_t far = 0;
+    unsigned long long nnew = 0, dag = 0;
+    bool ovf = false;
+};
+
+template <bool SIGMA, bool COMPAT>
+__device__ __forceinline__ void fwd_vertex_done(const FwdArgs& a, int v, uint64_t old, double acc,
+                                                uint32_t cnt, uint64_t orf_all, int lane, FwdAccum& r) {
+    uint64_t nm;
+    if constexpr (SIGMA) nm = ballot(cnt != 0);
+    else nm = orf_all;
+    if (!nm) return;
+    if ((nm >> lane) & 1) {
+        if constexpr (SIGMA) {
+            a.sigma[(size_t)v * kWave + lane] = acc;
+            if (__builtin_isinf(acc)) r.ovf = true;
+        }
+        if constexpr (COMPAT) a.npred[(size_t)v * kWave + lane] = cnt;
+        r.far += a.weight ? (int64_t)a.dnext * a.weight[v] : (int64_t)a.dnext;
+        r.dag += cnt;
+    }
+    if (lane == 0) {
+        a.Fn[v] = nm;
+        a.vis[v] = old | nm;
+        const int pc = __popcll(nm);
+        if (a.dist_sum) a.dist_sum[v] += (int64_t)a.dnext * pc;
+        r.nnew += (unsigned long long)pc;
+    }
+}
+
+// Per-block reduction of the forward partials: one plain add per block slot.
+__device__ __forceinline__ void fwd_block_flush(const FwdArgs& a, const FwdAccum& r, int slot) {
+    __shared__ int64_t s_far[kWavesPerBlock][kWave];
+    __shared__ unsigned long long s_cnt[kWavesPerBlock][2];
+    const int lane = lane_id();
+    const int wid = threadIdx.x >> 6;
+    s_far[wid][lane] = r.far;
+    const unsigned long long dag = (unsigned long long)wave_sum_i64((int64_t)r.dag);
+    const unsigned long long nn = (unsigned long long)wave_sum_i64((int64_t)r.nnew);
+    if (lane == 0) { s_cnt[wid][0] = nn; s_cnt[wid][1] = dag; }
+    if (r.ovf) *a.overflow = 1;
+    __syncthreads();
+    if (wid == 0) {
+        int64_t f = 0;
+#pragma unroll
+        for (int w = 0; w < kWavesPerBlock; ++w) f += s_far[w][lane];
+        if (f) a.part_far[(size_t)slot * kWave + lane] += f;
+        if (lane < 2) {
+            unsigned long long c = 0;
+#pragma unroll
+            for (int w = 0; w < kWavesPerBlock; ++w) c += s_cnt[w][lane];
+            if (c) {
+                a.part_cnt[(size_t)slot * 2 + lane] += c;
+                if (lane == 0) *a.flag = 1;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void bwd_vertex_done(const BwdArgs& a, int v, uint64_t m, double acc,
+                                                int lane, bool compat) {
+    double contrib = 0.0;
+    if ((m >> lane) & 1) {
+        const size_t idx = (size_t)v * kWave + lane;
+        const double sg = a.sigma[idx];
+        const double dl = sg * acc;
+        a.coef[idx] = (1.0 + dl) / sg;
+        contrib = compat ? dl * (double)a.npred[idx] : dl;
+    }
+    const double tot = wave_sum(contrib);
+    if (lane == 0) a.bc[v] += tot;
+}
+
+// ---------------------------------------------------------------- forward
+template <bool SIGMA, bool COMPAT>
+__global__ __launch_bounds__(kBlock) void k_fwd(FwdArgs a) {
+    const int lane = lane_id();
+    const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const int nw = gridDim.x * kWavesPerBlock;
+    const uint64_t A = a.A;
+    FwdAccum r;
+    for (int base = gw * kWave; base < a.n; base += nw * kWave) {
+        const int vl = base + lane;
+        uint64_t visl = A;
+        int rbl = 0, rel = 0;
+        if (vl < a.n) {
+            visl = a.vis[vl];
+            if (visl != A) {
+                rbl = a.rp[vl];
+                rel = a.rp[vl + 1];
+                if (rel - rbl > a.hub_deg) visl = A;  // hub rows: k_fwd_hubseg
+            }
+        }
+        uint64_t work = ballot(visl != A);
+        while (work) {
+            const int b = __builtin_ctzll(work);
+            work &= work - 1;
+            const int v = base + b;
+            const uint64_t old = rl64(visl, b);
+            double acc = 0.0;
+            uint32_t cnt = 0;
+            uint64_t orf = 0;
+            gather_fwd<SIGMA>(a.col, a.Fd, a.sigma, rl32(rbl, b), rl32(rel, b), A & ~old, lane, acc, cnt, orf);
+            fwd_vertex_done<SIGMA, COMPAT>(a, v, old, acc, cnt, SIGMA ? 0 : wave_or(orf), lane, r);
+        }
+    }
+    fwd_block_flush(a, r, blockIdx.x);
+}
+
+template <bool SIGMA>
+__global__ __launch_bounds__(kBlock) void k_fwd_hubseg(FwdArgs a, HubArgs h) {
+    const int lane = lane_id();
+    const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const int nw = gridDim.x * kWavesPerBlock;
+    for (int i = gw; i < h.nseg; i += nw) {
+        const int v = h.seg_v[i];
+        const uint64_t cand = a.A & ~a.vis[v];
+        if (!cand) continue;  // k_fwd_hubfin skips the vertex too
+        double acc = 0.0;
+        uint32_t cnt = 0;
+        uint64_t orf = 0;
+        gather_fwd<SIGMA>(a.col, a.Fd, a.sigma, h.seg_beg[i], h.seg_end[i], cand, lane, acc, cnt, orf);
+        if constexpr (SIGMA) {
+            h.part_acc[(size_t)i * kWave + lane] = acc;
+            h.part_cnt[(size_t)i * kWave + lane] = cnt;
+        } else {
+            orf = wave_or(orf);
+            if (lane == 0) h.part_or[i] = orf;
+        }
+    }
+}
+
+template <bool SIGMA, bool COMPAT>
+__global__ __launch_bounds__(kBlock) void k_fwd_hubfin(FwdArgs a, HubArgs h) {
+    const int lane = lane_id();
+    const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const int nw = gridDim.x * kWavesPerBlock;
+    FwdAccum r;
+    for (int k = gw; k < h.nhub; k += nw) {
+        const int v = h.hub_v[k];
+        const uint64_t old = a.vis[v];
+        if (!(a.A & ~old)) continue;
+        double acc = 0.0;
+        uint32_t cnt = 0;
+        uint64_t orf = 0;
+        for (int i = h.hub_seg_off[k]; i < h.hub_seg_off[k + 1]; ++i) {
+            if constexpr (SIGMA) {
+                acc += h.part_acc[(size_t)i * kWave + lane];
+                cnt += h.part_cnt[(size_t)i * kWave + lane];
+            } else {
+                orf |= h.part_or[i];
+            }
+        }
+        fwd_vertex_done<SIGMA, COMPAT>(a, v, old, acc, cnt, orf, lane, r);
+    }
+    fwd_block_flush(a, r, blockIdx.x);
+}
+
+// ---------------------------------------------------------------- backward
+template <bool COMPAT>
+__global__ __launch_bounds__(kBlock) void k_bwd(BwdArgs a) {
+    const int lane = lane_id();
+    const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const int nw = gridDim.x * kWavesPerBlock;
+    for (int base = gw * kWave; base < a.n; base += nw * kWave) {
+        const int vl = base + lane;
+        uint64_t fl = 0;
+        int rbl = 0, rel = 0;
+        if (vl < a.n) {
+            fl = a.Fd[vl];
+            if (fl) {
+                rbl = a.rp[vl];
+                rel = a.rp[vl + 1];
+                if (rel - rbl > a.hub_deg) fl = 0;  // hub rows: k_bwd_hubseg
+            }
+        }
+        uint64_t work = ballot(fl != 0);
+        while (work) {
+            const int b = __builtin_ctzll(work);
+            work &= work - 1;
+            const uint64_t m = rl64(fl, b);
+            const double acc = gather_bwd(a.col, a.Fn, a.coef, rl32(rbl, b), rl32(rel, b), m, lane);
+            bwd_vertex_done(a, base + b, m, acc, lane, COMPAT);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bwd_hubseg(BwdArgs a, HubArgs h) {
+    const int lane = lane_id();
+    const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const int nw = gridDim.x * kWavesPerBlock;
+    for (int i = gw; i < h.nseg; i += nw) {
+        const uint64_t m = a.Fd[h.seg_v[i]];
+        if (!m) continue;
+        h.part_acc[(size_t)i * kWave + lane] = gather_bwd(a.col, a.Fn, a.coef, h.seg_beg[i], h.seg_end[i], m, lane);
+    }
+}
+
+template <bool COMPAT>
+__global__ __launch_bounds__(kBlock) void k_bwd_hubfin(BwdArgs a, HubArgs h) {
+    const int lane = lane_id();
+    const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const int nw = gridDim.x * kWavesPerBlock;
+    for (int k = gw; k < h.nhub; k += nw) {
+        const int v = h.hub_v[k];
+        const uint64_t m = a.Fd[v];
+        if (!m) continue;
+        double acc = 0.0;
+        for (int i = h.hub_seg_off[k]; i < h.hub_seg_off[k + 1]; ++i) acc += h.part_acc[(size_t)i * kWave + lane];
+        bwd_vertex_done(a, v, m, acc, lane, COMPAT);
+    }
+}
+
+// ---------------------------------------------------------------- batch setup
 // Seeds a batch: lane s starts at src[s] (duplicates allowed, each its own lane).
 __global__ void k_seed(const int* __restrict__ src, int k, uint64_t* __restrict__ F0,
                        uint64_t* __restrict__ vis, double* __restrict__ sigma, bool with_sigma) {
@@ -223,17 +385,34 @@ __global__ void k_seed(const int* __restrict__ src, int k, uint64_t* __restrict_
 }
 
 // Sums the per-block lane partials of a batch into far[0..k) and the counters.
-__global__ void k_reduce_batch(const int64_t* __restrict__ part_far,
-                               const unsigned long long* __restrict__ part_cnt, int nblocks, int k,
-                               int64_t* __restrict__ far, unsigned long long* __restrict__ cnt_out) {
-    const int lane = threadIdx.x;  // 64 threads
+// 1024 threads: thread t sums lane (t % 64) over blocks t/64, t/64 + 16, ...
+__global__ __launch_bounds__(1024) void k_reduce_batch(const int64_t* __restrict__ part_far,
+                                                       const unsigned long long* __restrict__ part_cnt,
+                                                       int nblocks, int k, int64_t* __restrict__ far,
+                                                       unsigned long long* __restrict__ cnt_out) {
+    constexpr int kRows = 1024 / kWave;
+    __shared__ int64_t s_far[kRows][kWave];
+    __shared__ unsigned long long s_c[kRows][2];
+    const int lane = threadIdx.x & 63;
+    const int row = threadIdx.x >> 6;
     int64_t f = 0;
-    for (int b = 0; b < nblocks; ++b) f += part_far[(size_t)b * kWave + lane];
-    if (lane < k) far[lane] = f;
-    if (lane < 2) {
-        unsigned long long c = 0;
-        for (int b = 0; b < nblocks; ++b) c += part_cnt[(size_t)b * 2 + lane];
-        cnt_out[lane] += c;
+    unsigned long long c = 0;
+    for (int b = row; b < nblocks; b += kRows) {
+        f += part_far[(size_t)b * kWave + lane];
+        if (lane < 2) c += part_cnt[(size_t)b * 2 + lane];
+    }
+    s_far[row][lane] = f;
+    if (lane < 2) s_c[row][lane] = c;
+    __syncthreads();
+    if (row == 0) {
+        int64_t t = 0;
+        for (int r = 0; r < kRows; ++r) t += s_far[r][lane];
+        if (lane < k) far[lane] = t;
+        if (lane < 2) {
+            unsigned long long u = 0;
+            for (int r = 0; r < kRows; ++r) u += s_c[r][lane];
+            cnt_out[lane] += u;
+        }
     }
 }
 
