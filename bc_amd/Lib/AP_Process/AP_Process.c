@@ -1,0 +1,345 @@
+/*
+ * Articulation-point split of the folded graph — the role of AP_detection,
+ * AP_Copy_And_Split and rebuildGraph (Lib/AP_Process/AP_Process.c:6-225,
+ * 468-1065, 1141-1321), by a different, linear-time route:
+ *
+ *  1. Hopcroft–Tarjan biconnected components (iterative DFS with an edge
+ *     stack) over the unfolded vertices.  Blocks come out in DFS post-order,
+ *     each headed by the vertex it hangs from (its parent articulation
+ *     vertex; the head of a component's last block is the DFS root).
+ *  2. Block-cut tree weights.  With the tree rooted at each component's last
+ *     block, a copy x of vertex v in block B gets
+ *         W(x) = w[v] + sum of sub_w(C) over child blocks C hanging at v
+ *     and the copy of B's parent articulation vertex gets the rest of the
+ *     component, Wcomp - sub_w(B).  The reference's per-part "represent"
+ *     values (getPartsInfo, AP_Process.c:383-462) play this role.
+ *  3. Packing: each block gets a private, contiguous id range with its own
+ *     copy of every articulation vertex, so the packed graph is a disjoint
+ *     union of blocks.
+ *  4. Work items: consecutive small blocks are grouped into items of at most
+ *     64 sources; a larger block gives ceil(size / 64) items over its range.
+ *
+ * cent_reduced_finish then turns the GPU's weighted distance sums into
+ * farness with the block constants K_B (see reduce.h), computed bottom-up and
+ * top-down over the block-cut tree — the quantity the reference obtains per
+ * articulation point as CC[AP] = total_ff + ff[AP] (AP_Process.c:741) and
+ * through the clones' ff values.
+ */
+#include <stdio.h>
+#include "../common.h"
+#include "../reduce.h"
+
+void cent_reduced_free(struct cent_reduced* r) {
+    if (!r) return;
+    if (r->dev && r->dev_free) r->dev_free(r->dev);
+    cent_d1_free(&r->d1);
+    free(r->prp); free(r->pcol); free(r->pw); free(r->porig); free(r->pblock);
+    free(r->boff); free(r->bhead); free(r->bparent); free(r->rep); free(r->vcomp);
+    free(r->comp_w); free(r->items);
+    free(r);
+}
+
+typedef struct { int32_t u, v; } edge_t;
+
+int cent_reduced_build(const struct CSR* g, struct cent_reduced** out) {
+    if (!g || !out) return CENT_E_INVAL;
+    *out = NULL;
+    const int32_t n = g->csrVSize - 1;
+    const size_t N = (size_t)(n > 0 ? n : 1);
+    const int32_t* rp = g->csrV;
+    const int32_t* col = g->csrE;
+    struct cent_reduced* r = (struct cent_reduced*)calloc(1, sizeof(*r));
+    if (!r) return CENT_E_OOM;
+    r->n = n;
+    r->m = g->csrESize / 2;
+    int st = cent_d1_fold(g, &r->d1);
+    if (st) { free(r); return st; }
+    const int32_t* parent1 = r->d1.parent;
+    const int64_t* w = r->d1.w;
+#define ALIVE(x) (parent1[(x)] < 0)
+
+    /* ---- connected components (of the input graph) and their sizes */
+    r->vcomp = (int32_t*)malloc(N * sizeof(int32_t));
+    r->comp_w = (int64_t*)malloc(N * sizeof(int64_t));
+    r->rep = (int32_t*)malloc(N * sizeof(int32_t));
+    if (!r->vcomp || !r->comp_w || !r->rep) { cent_reduced_free(r); return CENT_E_OOM; }
+    r->ncomp = cent_components(g, r->vcomp, r->comp_w);
+    if (r->ncomp < 0) { cent_reduced_free(r); return CENT_E_OOM; }
+    for (int32_t v = 0; v < n; ++v) r->rep[v] = -1;
+
+    /* ---- Hopcroft–Tarjan over unfolded vertices with rdeg > 0 */
+    int32_t* disc = (int32_t*)malloc(N * sizeof(int32_t));
+    int32_t* low = (int32_t*)malloc(N * sizeof(int32_t));
+    int32_t* dpar = (int32_t*)malloc(N * sizeof(int32_t));
+    int32_t* it = (int32_t*)malloc(N * sizeof(int32_t));
+    int32_t* fstack = (int32_t*)malloc(N * sizeof(int32_t));
+    edge_t* estack = (edge_t*)malloc((size_t)(g->csrESize / 2 + 1) * sizeof(edge_t));
+    int32_t* stamp = (int32_t*)malloc(N * sizeof(int32_t));
+    int32_t* pblock_of = (int32_t*)malloc(N * sizeof(int32_t)); /* block holding v's tree edge from its DFS parent */
+    cent_ivec bverts = {0}, bstart = {0}, bheadv = {0}, bedges_u = {0}, bedges_v = {0}, bestart = {0};
+    if (!disc || !low || !dpar || !it || !fstack || !estack || !stamp || !pblock_of) {
+        st = CENT_E_OOM;
+        goto fail;
+    }
+    for (int32_t v = 0; v < n; ++v) { disc[v] = -1; stamp[v] = -1; pblock_of[v] = -1; }
+    int32_t timer = 0, nblk = 0;
+    int64_t esp = 0;
+    for (int32_t root = 0; root < n; ++root) {
+        if (!ALIVE(root) || r->d1.rdeg[root] == 0 || disc[root] >= 0) continue;
+        int32_t fsp = 0;
+        fstack[fsp++] = root;
+        disc[root] = low[root] = timer++;
+        dpar[root] = -1;
+        it[root] = rp[root];
+        while (fsp > 0) {
+            const int32_t v = fstack[fsp - 1];
+            if (it[v] < rp[v + 1]) {
+                const int32_t x = col[it[v]++];
+                if (!ALIVE(x)) continue;
+                if (disc[x] < 0) {
+                    estack[esp].u = v; estack[esp].v = x; ++esp;
+                    dpar[x] = v;
+                    disc[x] = low[x] = timer++;
+                    it[x] = rp[x];
+                    fstack[fsp++] = x;
+                } else if (x != dpar[v] && disc[x] < disc[v]) {
+                    estack[esp].u = v; estack[esp].v = x; ++esp;
+                    if (disc[x] < low[v]) low[v] = disc[x];
+                }
+            } else {
+                --fsp;
+                const int32_t p = dpar[v];
+                if (p < 0) continue;
+                if (low[v] < low[p]) low[p] = low[v];
+                if (low[v] >= disc[p]) {
+                    /* pop the block ending at tree edge (p, v); head = p */
+                    if ((st = cent_ivec_push(&bstart, (int32_t)bverts.n)) ||
+                        (st = cent_ivec_push(&bestart, (int32_t)bedges_u.n)) ||
+                        (st = cent_ivec_push(&bheadv, p)))
+                        goto fail;
+                    for (;;) {
+                        --esp;
+                        const int32_t a = estack[esp].u, b = estack[esp].v;
+                        if ((st = cent_ivec_push(&bedges_u, a)) || (st = cent_ivec_push(&bedges_v, b))) goto fail;
+                        if (stamp[a] != nblk) { stamp[a] = nblk; if ((st = cent_ivec_push(&bverts, a))) goto fail; }
+                        if (stamp[b] != nblk) { stamp[b] = nblk; if ((st = cent_ivec_push(&bverts, b))) goto fail; }
+                        if (dpar[b] == a) pblock_of[b] = nblk;  /* tree edge into b */
+                        if (a == p && b == v) break;
+                    }
+                    ++nblk;
+                }
+            }
+        }
+        /* the root's own parent block: the last block it heads */
+        pblock_of[root] = nblk - 1;
+    }
+    if ((st = cent_ivec_push(&bstart, (int32_t)bverts.n)) || (st = cent_ivec_push(&bestart, (int32_t)bedges_u.n)))
+        goto fail;
+
+    /* ---- block-cut tree: parent block of each block; children are earlier */
+    r->nblocks = nblk;
+    r->boff = (int32_t*)malloc(((size_t)nblk + 1) * sizeof(int32_t));
+    r->bhead = (int32_t*)malloc(((size_t)nblk + 1) * sizeof(int32_t));
+    r->bparent = (int32_t*)malloc(((size_t)nblk + 1) * sizeof(int32_t));
+    int64_t* sub_w = (int64_t*)calloc((size_t)nblk + 1, sizeof(int64_t));
+    int64_t* hang_w = (int64_t*)malloc(N * sizeof(int64_t)); /* w[v] + sum sub_w of child blocks at v */
+    if (!r->boff || !r->bhead || !r->bparent || !sub_w || !hang_w) { free(sub_w); free(hang_w); st = CENT_E_OOM; goto fail; }
+    for (int32_t v = 0; v < n; ++v) hang_w[v] = w[v];
+    for (int32_t b = 0; b < nblk; ++b) {
+        const int32_t h = bheadv.a[b];
+        r->bparent[b] = (pblock_of[h] == b) ? -1 : pblock_of[h];
+    }
+    /* bottom-up (emission order = children first): sub_w(B) = sum of hang_w over B minus its head */
+    for (int32_t b = 0; b < nblk; ++b) {
+        const int32_t h = bheadv.a[b];
+        int64_t s = 0;
+        for (int32_t i = bstart.a[b]; i < bstart.a[b + 1]; ++i) {
+            const int32_t x = bverts.a[i];
+            if (x != h || r->bparent[b] < 0) s += hang_w[x];
+        }
+        if (r->bparent[b] >= 0) { sub_w[b] = s; hang_w[h] += s; }
+        else sub_w[b] = s;  /* root block: the whole component */
+    }
+
+    /* ---- packing: blocks in emission order, ids contiguous per block */
+    r->np = (int32_t)bverts.n;
+    int64_t ne = bedges_u.n;
+    r->nnzp = 2 * ne;
+    if (r->nnzp > INT32_MAX) { free(sub_w); free(hang_w); st = CENT_E_INVAL; goto fail; }
+    const size_t NP = (size_t)(r->np > 0 ? r->np : 1);
+    r->prp = (int32_t*)calloc(NP + 1, sizeof(int32_t));
+    r->pcol = (int32_t*)malloc((size_t)(r->nnzp > 0 ? r->nnzp : 1) * sizeof(int32_t));
+    r->pw = (int64_t*)malloc(NP * sizeof(int64_t));
+    r->porig = (int32_t*)malloc(NP * sizeof(int32_t));
+    r->pblock = (int32_t*)malloc(NP * sizeof(int32_t));
+    int32_t* local = (int32_t*)malloc(N * sizeof(int32_t));
+    int32_t* fill = (int32_t*)malloc(NP * sizeof(int32_t));
+    if (!r->prp || !r->pcol || !r->pw || !r->porig || !r->pblock || !local || !fill) {
+        free(sub_w); free(hang_w); free(local); free(fill);
+        st = CENT_E_OOM;
+        goto fail;
+    }
+    for (int32_t b = 0; b < nblk; ++b) {
+        const int32_t h = bheadv.a[b];
+        r->boff[b] = bstart.a[b];
+        r->bhead[b] = -1;
+        const int64_t wc = r->comp_w[r->vcomp[h]];
+        for (int32_t i = bstart.a[b]; i < bstart.a[b + 1]; ++i) {
+            const int32_t x = bverts.a[i];
+            local[x] = i;
+            r->porig[i] = x;
+            r->pblock[i] = b;
+            if (x == h && r->bparent[b] >= 0) {
+                r->pw[i] = wc - sub_w[b];
+                r->bhead[b] = i;
+            } else {
+                r->pw[i] = hang_w[x];
+            }
+            if (r->rep[x] < 0 && (x != h || r->bparent[b] < 0)) r->rep[x] = i;
+        }
+        for (int32_t e = bestart.a[b]; e < bestart.a[b + 1]; ++e) {
+            r->prp[local[bedges_u.a[e]] + 1]++;
+            r->prp[local[bedges_v.a[e]] + 1]++;
+        }
+    }
+    r->boff[nblk] = r->np;
+    for (int32_t i = 0; i < r->np; ++i) r->prp[i + 1] += r->prp[i];
+    for (int32_t i = 0; i < r->np; ++i) fill[i] = r->prp[i];
+    for (int32_t b = 0; b < nblk; ++b) {
+        for (int32_t i = bstart.a[b]; i < bstart.a[b + 1]; ++i) local[bverts.a[i]] = i;
+        for (int32_t e = bestart.a[b]; e < bestart.a[b + 1]; ++e) {
+            const int32_t a = local[bedges_u.a[e]], c = local[bedges_v.a[e]];
+            r->pcol[fill[a]++] = c;
+            r->pcol[fill[c]++] = a;
+        }
+    }
+    free(local); free(fill); free(sub_w); free(hang_w);
+
+    /* ---- articulation vertices: copies in more than one block */
+    {
+        int32_t* cnt = (int32_t*)calloc(N, sizeof(int32_t));
+        if (!cnt) { st = CENT_E_OOM; goto fail; }
+        for (int32_t i = 0; i < r->np; ++i) cnt[r->porig[i]]++;
+        r->n_ap = 0;
+        for (int32_t v = 0; v < n; ++v) r->n_ap += cnt[v] > 1;
+        free(cnt);
+    }
+
+    /* ---- work items: group consecutive small blocks up to 64 sources */
+    {
+        int64_t cap = r->np / 8 + nblk + 16;
+        r->items = (struct cent_item*)malloc((size_t)cap * sizeof(struct cent_item));
+        if (!r->items) { st = CENT_E_OOM; goto fail; }
+        int32_t ni = 0;
+        int32_t b = 0;
+        while (b < nblk) {
+            const int32_t sz = r->boff[b + 1] - r->boff[b];
+            if (sz > 64) {
+                for (int32_t s0 = r->boff[b]; s0 < r->boff[b + 1]; s0 += 64) {
+                    if (ni == cap) break;
+                    const int32_t ns = (r->boff[b + 1] - s0) < 64 ? (r->boff[b + 1] - s0) : 64;
+                    r->items[ni].lo = r->boff[b]; r->items[ni].hi = r->boff[b + 1];
+                    r->items[ni].s0 = s0; r->items[ni].ns = ns;
+                    ++ni;
+                }
+                ++b;
+            } else {
+                int32_t e = b, tot = 0;
+                while (e < nblk && tot + (r->boff[e + 1] - r->boff[e]) <= 64) { tot += r->boff[e + 1] - r->boff[e]; ++e; }
+                if (ni == cap) break;
+                r->items[ni].lo = r->boff[b]; r->items[ni].hi = r->boff[e];
+                r->items[ni].s0 = r->boff[b]; r->items[ni].ns = tot;
+                ++ni;
+                b = e;
+            }
+        }
+        r->nitems = ni;
+    }
+    st = CENT_OK;
+fail:
+    free(disc); free(low); free(dpar); free(it); free(fstack); free(estack); free(stamp); free(pblock_of);
+    cent_ivec_free(&bverts); cent_ivec_free(&bstart); cent_ivec_free(&bheadv);
+    cent_ivec_free(&bedges_u); cent_ivec_free(&bedges_v); cent_ivec_free(&bestart);
+    if (st) { cent_reduced_free(r); return st; }
+    *out = r;
+#undef ALIVE
+    return CENT_OK;
+}
+
+/*
+ * Block constants.  For a copy x in block B, F(x) is the far-field of x seen
+ * from B: the sum over the vertices hanging at x outside B (x included) of
+ * ff[y] + d(x, y) * w[y].  Then K_B = sum_{x in B} F(x) and
+ *   child-side copy x (not B's head):  F(x) = ff[x] + sum_{C child block at x} down(C)
+ *   down(C) = D(head copy of C) + sum_{z in C, z != head} F(z)
+ *   head copy h of B:                  F(h) = farness(v_h) - down(B),
+ *   farness(v_h) = D(copy of v_h in its parent block P) + K_P.
+ * Blocks are stored children-first, so one forward pass gives down(), one
+ * backward pass gives K.
+ */
+int cent_reduced_finish(const struct cent_reduced* r, const int64_t* D, int64_t* farness) {
+    if (!r || (!D && r->np > 0) || !farness) return CENT_E_INVAL;
+    const int32_t n = r->n, nb = r->nblocks;
+    const size_t N = (size_t)(n > 0 ? n : 1);
+    int64_t* down = (int64_t*)calloc((size_t)nb + 1, sizeof(int64_t));
+    int64_t* K = (int64_t*)calloc((size_t)nb + 1, sizeof(int64_t));
+    int64_t* hang_ff = (int64_t*)malloc(N * sizeof(int64_t)); /* ff[v] + sum down(C) of child blocks at v */
+    if (!down || !K || !hang_ff) { free(down); free(K); free(hang_ff); return CENT_E_OOM; }
+    for (int32_t v = 0; v < n; ++v) hang_ff[v] = r->d1.ff[v];
+    /* bottom-up */
+    for (int32_t b = 0; b < nb; ++b) {
+        if (r->bparent[b] < 0) continue;
+        const int32_t h = r->bhead[b];
+        int64_t s = D[h];
+        for (int32_t i = r->boff[b]; i < r->boff[b + 1]; ++i)
+            if (i != h) s += hang_ff[r->porig[i]];
+        down[b] = s;
+        hang_ff[r->porig[h]] += s;
+    }
+    /* top-down (parents before children) */
+    for (int32_t b = nb - 1; b >= 0; --b) {
+        int64_t s = 0;
+        const int32_t h = r->bhead[b];
+        for (int32_t i = r->boff[b]; i < r->boff[b + 1]; ++i)
+            if (i != h) s += hang_ff[r->porig[i]];
+        if (r->bparent[b] < 0) {
+            K[b] = s;
+        } else {
+            /* farness of the head vertex from its copy in the parent block */
+            const int32_t P = r->bparent[b];
+            const int32_t xp = r->rep[r->porig[h]];  /* the head vertex's copy in P */
+            if (xp < 0 || r->pblock[xp] != P) { free(down); free(K); free(hang_ff); return CENT_E_INVAL; }
+            const int64_t far_h = D[xp] + K[P];
+            K[b] = (far_h - down[b]) + s;
+        }
+    }
+    /* unfolded vertices */
+    for (int32_t v = 0; v < n; ++v) {
+        if (r->d1.parent[v] >= 0) continue;
+        const int32_t x = r->rep[v];
+        farness[v] = (x >= 0) ? D[x] + K[r->pblock[x]] : r->d1.ff[v];  /* lone: fully folded component */
+    }
+    /* folded vertices, reverse fold order (Seq_CC/CC.cpp:2037-2049) */
+    for (int32_t i = r->d1.nfold - 1; i >= 0; --i) {
+        const int32_t d = r->d1.order[i];
+        const int32_t p = r->d1.parent[d];
+        farness[d] = farness[p] + r->comp_w[r->vcomp[d]] - 2 * r->d1.w[d];
+    }
+    free(down); free(K); free(hang_ff);
+    return CENT_OK;
+}
+
+/* ---- C-ABI wrappers (include/cent.h) */
+int cent_reduce(const struct CSR* csr, cent_reduced** out) { return cent_reduced_build(csr, out); }
+
+int cent_reduced_get_view(const cent_reduced* r, cent_reduced_view* v) {
+    if (!r || !v) return CENT_E_INVAL;
+    v->n = r->n; v->np = r->np; v->nnzp = r->nnzp; v->nblocks = r->nblocks; v->nitems = r->nitems;
+    v->nfold = r->d1.nfold; v->n_ap = r->n_ap; v->ncomp = r->ncomp;
+    v->prp = r->prp; v->pcol = r->pcol; v->pw = r->pw; v->porig = r->porig; v->boff = r->boff;
+    return CENT_OK;
+}
+
+int cent_reduced_farness(const cent_reduced* r, const int64_t* D, int64_t* farness) {
+    return cent_reduced_finish(r, D, farness);
+}

@@ -203,6 +203,64 @@ def sample_sources(n: int, k: int, seed: int) -> np.ndarray:
     return out
 
 
+# ---------------------------------------------------------------- reduced form
+class _RView(C.Structure):
+    _fields_ = [("n", C.c_int32), ("np", C.c_int32), ("nnzp", C.c_int64), ("nblocks", C.c_int32),
+                ("nitems", C.c_int32), ("nfold", C.c_int32), ("n_ap", C.c_int32),
+                ("ncomp", C.c_int32), ("prp", _I32P), ("pcol", _I32P), ("pw", _I64P),
+                ("porig", _I32P), ("boff", _I32P)]
+
+
+_lib.cent_reduce.argtypes = [_P_CSR, C.POINTER(C.c_void_p)]
+_lib.cent_reduced_get_view.argtypes = [C.c_void_p, C.POINTER(_RView)]
+_lib.cent_reduced_farness.argtypes = [C.c_void_p, _I64P, _I64P]
+_lib.cent_reduced_free.argtypes = [C.c_void_p]
+_lib.cent_reduced_free.restype = None
+CENT_SCHED_PERSISTENT, CENT_SCHED_PER_ITEM = 0, 1
+
+
+class Reduced:
+    """Degree-1 folded, biconnected-block packed form of a graph (host C).
+
+    Mirrors the reduction steps of compute_D1_AP_CC_shareBased_DegreeOrder
+    (Seq_CC/CC.cpp:1742-1776: D1Folding, AP_detection + AP_Copy_And_Split,
+    rebuildGraph).  Arrays are zero-copy views of the library's buffers.
+    """
+
+    def __init__(self, csr: CSR):
+        self._csr = csr
+        h = C.c_void_p()
+        _check(_lib.cent_reduce(csr._p, C.byref(h)), "cent_reduce")
+        self._h = h
+        v = _RView()
+        _check(_lib.cent_reduced_get_view(h, C.byref(v)), "cent_reduced_get_view")
+        self.n, self.np, self.nnzp = v.n, v.np, v.nnzp
+        self.nblocks, self.nitems, self.nfold, self.n_ap, self.ncomp = (
+            v.nblocks, v.nitems, v.nfold, v.n_ap, v.ncomp)
+        arr = np.ctypeslib.as_array
+        self.prp = arr(v.prp, shape=(self.np + 1,)) if v.prp else np.zeros(1, np.int32)
+        self.pcol = arr(v.pcol, shape=(self.nnzp,)) if self.nnzp else np.zeros(0, np.int32)
+        self.pw = arr(v.pw, shape=(self.np,)) if self.np else np.zeros(0, np.int64)
+        self.porig = arr(v.porig, shape=(self.np,)) if self.np else np.zeros(0, np.int32)
+        self.boff = arr(v.boff, shape=(self.nblocks + 1,)) if v.boff else np.zeros(1, np.int32)
+
+    def farness_from(self, D) -> np.ndarray:
+        """farness of every input vertex from the packed weighted distance sums D."""
+        D = np.ascontiguousarray(D, dtype=np.int64)
+        if len(D) != self.np:
+            raise ValueError("D must have one entry per packed vertex")
+        out = np.zeros(max(self.n, 1), np.int64)
+        _check(_lib.cent_reduced_farness(self._h, _ptr(D, C.c_int64) if self.np else None,
+                                         _ptr(out, C.c_int64)), "cent_reduced_farness")
+        return out[:self.n]
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.cent_reduced_free(h)
+            self._h = None
+
+
 # ---------------------------------------------------------------- engine
 class _Opts(C.Structure):
     _fields_ = [("bc_mode", C.c_int32), ("flags", C.c_int32), ("block_size", C.c_int32),
@@ -228,12 +286,18 @@ _lib.cent_ctx_last_error.argtypes = [C.c_void_p]
 _lib.cent_ctx_last_error.restype = C.c_char_p
 _lib.cent_graph_from_csr.argtypes = [C.c_void_p, _I32P, _I32P, C.c_int32, C.c_int64,
                                      C.POINTER(C.c_void_p)]
+_lib.cent_graph_create.argtypes = [C.c_void_p, _I32P, _I32P, C.c_int32, C.c_int64, C.c_int32,
+                                   C.POINTER(C.c_void_p)]
+CENT_GRAPH_KEEP_ORDER = 1
 _lib.cent_graph_free.argtypes = [C.c_void_p]
 _lib.cent_graph_free.restype = None
 _lib.cent_bc.argtypes = [C.c_void_p, C.c_void_p, _I32P, C.c_int32, _F64P, _I64P, _I64P,
                          C.POINTER(_Opts)]
 _lib.cent_cc.argtypes = [C.c_void_p, C.c_void_p, _I32P, C.c_int32, _I64P, _I64P,
                          C.POINTER(_Opts)]
+_lib.cent_probe_copy_bandwidth.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.POINTER(C.c_double)]
+_lib.cent_cc_reduced.argtypes = [C.c_void_p, C.c_void_p, _I64P, C.c_int32, C.POINTER(_Opts)]
+_lib.compute_D1_AP_CC_shareBased_DegreeOrder.argtypes = [_P_CSR, _I64P]
 _lib.computeBC.argtypes = [_P_CSR, _F64P]
 _lib.computeBC_compat.argtypes = [_P_CSR, _F64P]
 _lib.computeCC.argtypes = [_P_CSR, _I64P]
@@ -248,16 +312,17 @@ def _opts(mode=CENT_BC_TEXTBOOK, timing=False, grid_waves=0) -> _Opts:
 
 
 class DeviceGraph:
-    def __init__(self, ctx: "Context", csr: CSR):
+    def __init__(self, ctx: "Context", csr: CSR, keep_order: bool = False):
         self.ctx = ctx
         self.n = csr.n
         self.nnz = csr.nnz
         self._csr = csr
         h = C.c_void_p()
-        _check(_lib.cent_graph_from_csr(ctx._h, _ptr(csr.row_ptr, C.c_int32),
-                                        _ptr(csr.col, C.c_int32) if csr.nnz else None,
-                                        csr.n, csr.nnz, C.byref(h)),
-               "cent_graph_from_csr", ctx.last_error())
+        _check(_lib.cent_graph_create(ctx._h, _ptr(csr.row_ptr, C.c_int32),
+                                      _ptr(csr.col, C.c_int32) if csr.nnz else None,
+                                      csr.n, csr.nnz, CENT_GRAPH_KEEP_ORDER if keep_order else 0,
+                                      C.byref(h)),
+               "cent_graph_create", ctx.last_error())
         self._h = h
 
     def __del__(self):
@@ -285,8 +350,15 @@ class Context:
     def last_error(self) -> str:
         return _lib.cent_ctx_last_error(self._h).decode()
 
-    def upload(self, csr: CSR) -> DeviceGraph:
-        return DeviceGraph(self, csr)
+    def probe_copy_bandwidth(self, nbytes: int = 1 << 30, iters: int = 10) -> float:
+        """Read+write GB/s of a plain device copy (HBM ceiling / counter calibration)."""
+        out = C.c_double()
+        _check(_lib.cent_probe_copy_bandwidth(self._h, nbytes, iters, C.byref(out)),
+               "cent_probe_copy_bandwidth", self.last_error())
+        return out.value
+
+    def upload(self, csr: CSR, keep_order: bool = False) -> DeviceGraph:
+        return DeviceGraph(self, csr, keep_order)
 
     def stats(self) -> dict:
         s = Stats()
@@ -333,7 +405,26 @@ class Context:
         return far[:kk], (ds[:g.n] if ds is not None else None)
 
 
+    def cc_reduced(self, red: Reduced, sched: int = CENT_SCHED_PERSISTENT, timing: bool = False):
+        """Farness of every vertex through the D1/AP-reduced form."""
+        out = np.zeros(max(red.n, 1), np.int64)
+        o = _opts(0, timing)
+        _check(_lib.cent_cc_reduced(self._h, red._h, _ptr(out, C.c_int64), sched, C.byref(o)),
+               "cent_cc_reduced", self.last_error())
+        return out[:red.n]
+
+
 # ------------------------------------------- reference-shaped entry points
+def compute_D1_AP_CC_shareBased_DegreeOrder(csr: CSR, CCs: np.ndarray) -> np.ndarray:
+    """Farness of every vertex via degree-1 folding + articulation-point split,
+    assigned into CCs (int64) like Seq_CC/CC.cpp:1728-2052."""
+    if CCs.dtype != np.int64 or len(CCs) < csr.n or not CCs.flags.c_contiguous:
+        raise ValueError("CCs must be a contiguous int64 array of length n")
+    _check(_lib.compute_D1_AP_CC_shareBased_DegreeOrder(csr._p, _ptr(CCs, C.c_int64)),
+           "compute_D1_AP_CC_shareBased_DegreeOrder")
+    return CCs
+
+
 def computeBC(csr: CSR, BCs: np.ndarray, compat: bool = False) -> np.ndarray:
     """Brandes BC over sources startNodeID..endNodeID, accumulated into BCs
     (float64, length n) like Seq_Brandes/Brandes.c:224 accumulates into its
@@ -357,6 +448,6 @@ def computeCC(csr: CSR, CCs: np.ndarray) -> np.ndarray:
 
 __all__ = [
     "CSR", "CentError", "Context", "DeviceGraph", "buildGraph", "createCSR", "computeBC",
-    "computeCC", "gen_rmat", "gen_grid", "gen_blocktree", "sample_sources", "LIB_PATH",
+    "computeCC", "compute_D1_AP_CC_shareBased_DegreeOrder", "Reduced", "gen_rmat", "gen_grid", "gen_blocktree", "sample_sources", "LIB_PATH",
     "CENT_BC_TEXTBOOK", "CENT_BC_REF_COMPAT",
 ]

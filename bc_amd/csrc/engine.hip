@@ -1,5 +1,6 @@
 // Host side of the centrality engine: contexts, device graphs, the level
-// loop of the 64-source batches, and the C-ABI declared in include/cent.h.
+// loop of the 64-source batches, the reduced (D1 + AP) closeness schedule,
+// and the C-ABI declared in include/cent.h.
 //
 // A call processes its sources in batches of 64 (one wavefront lane per
 // source).  Per batch: seed, forward levels until a level reaches nothing
@@ -8,6 +9,7 @@
 // context's stream, so the per-vertex BC sums are accumulated in a fixed
 // order and the result does not depend on timing.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -16,6 +18,8 @@
 #include <vector>
 
 #include "../../include/cent.h"
+#include "../Lib/reduce.h"
+#include "items_kernels.h"
 #include "msbfs_kernels.h"
 
 using namespace cent;
@@ -27,6 +31,12 @@ using namespace cent;
             set_error(ctx, e_, #expr);                                         \
             return (e_ == hipErrorOutOfMemory) ? CENT_E_OOM : CENT_E_DEVICE;   \
         }                                                                      \
+    } while (0)
+
+#define TRY(expr)                  \
+    do {                           \
+        int st_ = (expr);          \
+        if (st_) return st_;       \
     } while (0)
 
 template <class T>
@@ -41,15 +51,15 @@ struct cent_ctx {
     char last_error[256] = {0};
     cent_stats stats{};
     // workspace
-    DevBuf<uint64_t> vis, F;
+    DevBuf<uint64_t> vis, F, hp_or;
     DevBuf<double> sigma, coef, bc, hp_acc;
     DevBuf<uint32_t> npred, hp_cnt;
-    DevBuf<uint64_t> hp_or;
     DevBuf<int64_t> dist_sum, far, part_far;
     DevBuf<unsigned long long> part_cnt, cnt;
     DevBuf<int> src, flags;
+    size_t F_levels = 0;     // levels the F buffer holds for the current N
+    size_t F_n = 0;
     int* h_flags = nullptr;  // pinned
-    size_t h_flags_cap = 0;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
 };
@@ -57,21 +67,8 @@ struct cent_ctx {
 // Rows longer than kHubDeg are scanned as kHubDeg-edge segments, one
 // wavefront each (see msbfs_kernels.h).
 constexpr int kHubDeg = 256;
-
-struct cent_graph {
-    cent_ctx* ctx = nullptr;
-    int32_t n = 0;
-    int64_t nnz = 0;
-    int* rp = nullptr;
-    int* col = nullptr;
-    std::vector<int32_t> comp;        // component id per vertex (host)
-    std::vector<int64_t> comp_nnz;    // directed edges per component (host)
-    // hub segments (device): seg_v/seg_beg/seg_end [nseg], hub_v [nhub], hub_seg_off [nhub+1]
-    int hub_deg = kHubDeg;
-    int nseg = 0, nhub = 0;
-    int* hub_buf = nullptr;           // one allocation holding the five arrays
-    int *seg_v = nullptr, *seg_beg = nullptr, *seg_end = nullptr, *hub_v = nullptr, *hub_seg_off = nullptr;
-};
+// Edge budget of one wavefront's vertex chunk in the main level kernels.
+constexpr int kChunkEdges = 1024;
 
 static void set_error(cent_ctx* ctx, hipError_t e, const char* what) {
     if (!ctx) return;
@@ -100,6 +97,144 @@ static int check_device(cent_ctx* ctx) {
     return CENT_OK;
 }
 
+// ------------------------------------------------------------ device CSR
+// Device copy of a CSR plus its scheduling metadata: vertex chunks (<= 64
+// vertices, <= kChunkEdges non-hub edges; optionally restarted at given
+// block boundaries) and hub segments.  Host copies of the chunk and hub
+// offsets allow sub-range views.
+struct DevCSR {
+    int n = 0;
+    int64_t nnz = 0;
+    int* rp = nullptr;
+    int* col = nullptr;
+    int hub_deg = kHubDeg;
+    int nchunk = 0;
+    int* chunk_beg = nullptr;
+    int nseg = 0, nhub = 0;
+    int* hub_buf = nullptr;
+    int *seg_v = nullptr, *seg_beg = nullptr, *seg_end = nullptr, *hub_v = nullptr, *hub_seg_off = nullptr;
+    std::vector<int> h_chunk_beg, h_hub_v, h_hub_seg_off;
+};
+
+static void devcsr_free(DevCSR& d) {
+    if (d.rp) (void)hipFree(d.rp);
+    if (d.col) (void)hipFree(d.col);
+    if (d.chunk_beg) (void)hipFree(d.chunk_beg);
+    if (d.hub_buf) (void)hipFree(d.hub_buf);
+    d = DevCSR();
+}
+
+template <class T>
+static int upload(cent_ctx* ctx, T** dst, const T* src, size_t n) {
+    HIPTRY(hipMalloc((void**)dst, sizeof(T) * (n > 0 ? n : 1)));
+    if (n) HIPTRY(hipMemcpy(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice));
+    return CENT_OK;
+}
+
+// bounds: optional sorted block starts (chunks never straddle a boundary)
+static int devcsr_build(cent_ctx* ctx, const int32_t* rp, const int32_t* col, int32_t n, int64_t nnz,
+                        const int32_t* bounds, int32_t nbounds, DevCSR& d) {
+    d.n = n;
+    d.nnz = nnz;
+    TRY(upload(ctx, &d.rp, rp, (size_t)n + 1));
+    TRY(upload(ctx, &d.col, col, (size_t)nnz));
+    // chunks
+    std::vector<int>& cb = d.h_chunk_beg;
+    cb.clear();
+    cb.push_back(0);
+    int cnt = 0, bi = 0;
+    int64_t edges = 0;
+    for (int32_t v = 0; v < n; ++v) {
+        bool brk = false;
+        while (bounds && bi < nbounds && bounds[bi] <= v) {
+            if (bounds[bi] == v && v > 0) brk = true;
+            ++bi;
+        }
+        int dg = rp[v + 1] - rp[v];
+        if (dg > d.hub_deg) dg = 0;
+        if (cnt > 0 && (brk || cnt == kWave || edges + dg > kChunkEdges)) {
+            cb.push_back(v);
+            cnt = 0;
+            edges = 0;
+        }
+        ++cnt;
+        edges += dg;
+    }
+    if (n > 0) cb.push_back(n);
+    d.nchunk = (int)cb.size() - 1;
+    TRY(upload(ctx, &d.chunk_beg, cb.data(), cb.size()));
+    // hub segments
+    std::vector<int> sv, sb, se;
+    d.h_hub_v.clear();
+    d.h_hub_seg_off.assign(1, 0);
+    for (int32_t v = 0; v < n; ++v) {
+        const int b = rp[v], e = rp[v + 1];
+        if (e - b <= d.hub_deg) continue;
+        d.h_hub_v.push_back(v);
+        for (int x = b; x < e; x += d.hub_deg) {
+            sv.push_back(v);
+            sb.push_back(x);
+            se.push_back(x + d.hub_deg < e ? x + d.hub_deg : e);
+        }
+        d.h_hub_seg_off.push_back((int)sv.size());
+    }
+    d.nseg = (int)sv.size();
+    d.nhub = (int)d.h_hub_v.size();
+    if (d.nhub) {
+        std::vector<int> all;
+        all.insert(all.end(), sv.begin(), sv.end());
+        all.insert(all.end(), sb.begin(), sb.end());
+        all.insert(all.end(), se.begin(), se.end());
+        all.insert(all.end(), d.h_hub_v.begin(), d.h_hub_v.end());
+        all.insert(all.end(), d.h_hub_seg_off.begin(), d.h_hub_seg_off.end());
+        TRY(upload(ctx, &d.hub_buf, all.data(), all.size()));
+        d.seg_v = d.hub_buf;
+        d.seg_beg = d.seg_v + d.nseg;
+        d.seg_end = d.seg_beg + d.nseg;
+        d.hub_v = d.seg_end + d.nseg;
+        d.hub_seg_off = d.hub_v + d.nhub;
+    }
+    return CENT_OK;
+}
+
+// A sub-range [lo, hi) of a DevCSR whose boundaries are chunk boundaries.
+struct GView {
+    const DevCSR* d;
+    int lo, hi;
+    int c0, nchunk;       // chunk range
+    int s0, nseg;         // hub segment range
+    int h0, nhub;         // hub range
+};
+
+static GView full_view(const DevCSR& d) {
+    return GView{&d, 0, d.n, 0, d.nchunk, 0, d.nseg, 0, d.nhub};
+}
+
+static GView sub_view(const DevCSR& d, int lo, int hi) {
+    GView v{&d, lo, hi, 0, 0, 0, 0, 0, 0};
+    const auto& cb = d.h_chunk_beg;
+    v.c0 = (int)(std::lower_bound(cb.begin(), cb.end(), lo) - cb.begin());
+    const int c1 = (int)(std::lower_bound(cb.begin(), cb.end(), hi) - cb.begin());
+    v.nchunk = c1 - v.c0;
+    const auto& hv = d.h_hub_v;
+    v.h0 = (int)(std::lower_bound(hv.begin(), hv.end(), lo) - hv.begin());
+    const int h1 = (int)(std::lower_bound(hv.begin(), hv.end(), hi) - hv.begin());
+    v.nhub = h1 - v.h0;
+    v.s0 = d.h_hub_seg_off[(size_t)v.h0];
+    v.nseg = d.h_hub_seg_off[(size_t)h1] - v.s0;
+    return v;
+}
+
+struct cent_graph {
+    cent_ctx* ctx = nullptr;
+    int32_t n = 0;
+    int64_t nnz = 0;
+    DevCSR d;
+    std::vector<int32_t> comp;        // component id per vertex (host, caller ids)
+    std::vector<int64_t> comp_nnz;    // directed edges per component
+    std::vector<int32_t> new_id;      // device id of each caller id (empty: identity)
+};
+
 extern "C" int cent_ctx_create(int32_t device, cent_ctx** out) {
     if (!out) return CENT_E_INVAL;
     *out = nullptr;
@@ -115,6 +250,11 @@ extern "C" int cent_ctx_create(int32_t device, cent_ctx** out) {
         delete ctx;
         return CENT_E_DEVICE;
     }
+    if (hipHostMalloc((void**)&ctx->h_flags, sizeof(int) * 2, hipHostMallocDefault) != hipSuccess) {
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return CENT_E_DEVICE;
+    }
     *out = ctx;
     return CENT_OK;
 }
@@ -123,10 +263,10 @@ extern "C" void cent_ctx_destroy(cent_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    release(ctx->vis); release(ctx->F); release(ctx->sigma); release(ctx->coef); release(ctx->bc);
-    release(ctx->npred); release(ctx->dist_sum); release(ctx->far); release(ctx->part_far);
-    release(ctx->part_cnt); release(ctx->cnt); release(ctx->src); release(ctx->flags);
-    release(ctx->hp_acc); release(ctx->hp_cnt); release(ctx->hp_or);
+    release(ctx->vis); release(ctx->F); release(ctx->hp_or); release(ctx->sigma); release(ctx->coef);
+    release(ctx->bc); release(ctx->hp_acc); release(ctx->npred); release(ctx->hp_cnt);
+    release(ctx->dist_sum); release(ctx->far); release(ctx->part_far); release(ctx->part_cnt);
+    release(ctx->cnt); release(ctx->src); release(ctx->flags);
     if (ctx->h_flags) (void)hipHostFree(ctx->h_flags);
     for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
@@ -139,22 +279,40 @@ extern "C" int cent_ctx_stats(const cent_ctx* ctx, cent_stats* out) {
     return CENT_OK;
 }
 
+extern "C" int cent_ctx_reset_stats(cent_ctx* ctx) {
+    if (!ctx) return CENT_E_INVAL;
+    ctx->stats = cent_stats{};
+    return CENT_OK;
+}
+
 extern "C" const char* cent_ctx_last_error(const cent_ctx* ctx) {
     return ctx ? ctx->last_error : "null context";
 }
 
-extern "C" int cent_graph_from_csr(cent_ctx* ctx, const int32_t* row_ptr, const int32_t* col,
-                                   int32_t n, int64_t nnz, cent_graph** out) {
-    if (!ctx || !out || n < 0 || nnz < 0 || nnz > INT32_MAX || !row_ptr || (nnz > 0 && !col))
-        return CENT_E_INVAL;
-    *out = nullptr;
+// ------------------------------------------------------------ graphs
+static int validate_csr(const int32_t* row_ptr, const int32_t* col, int32_t n, int64_t nnz) {
+    if (n < 0 || nnz < 0 || nnz > INT32_MAX || !row_ptr || (nnz > 0 && !col)) return CENT_E_INVAL;
     if (row_ptr[0] != 0 || row_ptr[n] != nnz) return CENT_E_INVAL;
     for (int32_t v = 0; v < n; ++v)
         if (row_ptr[v + 1] < row_ptr[v]) return CENT_E_INVAL;
     for (int64_t j = 0; j < nnz; ++j)
         if (col[j] < 0 || col[j] >= n) return CENT_E_INVAL;
-    int st = check_device(ctx);
-    if (st) return st;
+    return CENT_OK;
+}
+
+extern "C" void cent_graph_free(cent_graph* g) {
+    if (!g) return;
+    if (g->ctx) (void)hipSetDevice(g->ctx->device);
+    devcsr_free(g->d);
+    delete g;
+}
+
+extern "C" int cent_graph_create(cent_ctx* ctx, const int32_t* row_ptr, const int32_t* col, int32_t n,
+                                 int64_t nnz, int32_t flags, cent_graph** out) {
+    if (!ctx || !out) return CENT_E_INVAL;
+    *out = nullptr;
+    TRY(validate_csr(row_ptr, col, n, nnz));
+    TRY(check_device(ctx));
     cent_graph* g = new (std::nothrow) cent_graph();
     if (!g) return CENT_E_OOM;
     g->ctx = ctx;
@@ -179,60 +337,46 @@ extern "C" int cent_graph_from_csr(cent_ctx* ctx, const int32_t* row_ptr, const 
         g->comp_nnz.push_back(e);
         ++nc;
     }
-    hipError_t e1 = hipMalloc((void**)&g->rp, sizeof(int) * ((size_t)n + 1));
-    hipError_t e2 = hipMalloc((void**)&g->col, sizeof(int) * (size_t)(nnz > 0 ? nnz : 1));
-    if (e1 != hipSuccess || e2 != hipSuccess) {
-        if (g->rp) (void)hipFree(g->rp);
-        if (g->col) (void)hipFree(g->col);
-        delete g;
-        return CENT_E_OOM;
+    // Device layout: vertices relabelled by descending degree (ties by id), so
+    // the rows of high-degree vertices — the most gathered sigma/coef rows —
+    // share a small, cache-resident address range.  Rows are kept sorted by
+    // the new ids.  CENT_GRAPH_KEEP_ORDER keeps the caller's ids.
+    std::vector<int32_t> drp, dcol;
+    const int32_t* urp = row_ptr;
+    const int32_t* ucol = col;
+    if (!(flags & CENT_GRAPH_KEEP_ORDER) && n > 1) {
+        std::vector<int32_t> order((size_t)n);
+        for (int32_t v = 0; v < n; ++v) order[(size_t)v] = v;
+        std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+            return (row_ptr[x + 1] - row_ptr[x]) > (row_ptr[y + 1] - row_ptr[y]);
+        });
+        g->new_id.resize((size_t)n);
+        for (int32_t i = 0; i < n; ++i) g->new_id[(size_t)order[(size_t)i]] = i;
+        drp.resize((size_t)n + 1);
+        dcol.resize((size_t)(nnz > 0 ? nnz : 1));
+        drp[0] = 0;
+        for (int32_t i = 0; i < n; ++i) {
+            const int32_t v = order[(size_t)i];
+            int32_t o = drp[(size_t)i];
+            for (int32_t j = row_ptr[v]; j < row_ptr[v + 1]; ++j) dcol[(size_t)o++] = g->new_id[(size_t)col[j]];
+            std::sort(dcol.begin() + drp[(size_t)i], dcol.begin() + o);
+            drp[(size_t)i + 1] = o;
+        }
+        urp = drp.data();
+        ucol = dcol.data();
     }
-    if (hipMemcpy(g->rp, row_ptr, sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice) != hipSuccess ||
-        (nnz > 0 && hipMemcpy(g->col, col, sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice) != hipSuccess)) {
+    int st = devcsr_build(ctx, urp, ucol, n, nnz, nullptr, 0, g->d);
+    if (st) {
         cent_graph_free(g);
-        return CENT_E_DEVICE;
-    }
-    // hub segments
-    std::vector<int> sv, sb, se, hv, ho;
-    ho.push_back(0);
-    for (int32_t v = 0; v < n; ++v) {
-        const int b = row_ptr[v], e = row_ptr[v + 1];
-        if (e - b <= g->hub_deg) continue;
-        hv.push_back(v);
-        for (int x = b; x < e; x += g->hub_deg) {
-            sv.push_back(v);
-            sb.push_back(x);
-            se.push_back(x + g->hub_deg < e ? x + g->hub_deg : e);
-        }
-        ho.push_back((int)sv.size());
-    }
-    g->nseg = (int)sv.size();
-    g->nhub = (int)hv.size();
-    if (g->nhub) {
-        const size_t tot = 3 * sv.size() + hv.size() + ho.size();
-        if (hipMalloc((void**)&g->hub_buf, sizeof(int) * tot) != hipSuccess) {
-            cent_graph_free(g);
-            return CENT_E_OOM;
-        }
-        g->seg_v = g->hub_buf;
-        g->seg_beg = g->seg_v + g->nseg;
-        g->seg_end = g->seg_beg + g->nseg;
-        g->hub_v = g->seg_end + g->nseg;
-        g->hub_seg_off = g->hub_v + g->nhub;
-        std::vector<int> all;
-        all.reserve(tot);
-        all.insert(all.end(), sv.begin(), sv.end());
-        all.insert(all.end(), sb.begin(), sb.end());
-        all.insert(all.end(), se.begin(), se.end());
-        all.insert(all.end(), hv.begin(), hv.end());
-        all.insert(all.end(), ho.begin(), ho.end());
-        if (hipMemcpy(g->hub_buf, all.data(), sizeof(int) * tot, hipMemcpyHostToDevice) != hipSuccess) {
-            cent_graph_free(g);
-            return CENT_E_DEVICE;
-        }
+        return st;
     }
     *out = g;
     return CENT_OK;
+}
+
+extern "C" int cent_graph_from_csr(cent_ctx* ctx, const int32_t* row_ptr, const int32_t* col,
+                                   int32_t n, int64_t nnz, cent_graph** out) {
+    return cent_graph_create(ctx, row_ptr, col, n, nnz, 0, out);
 }
 
 extern "C" int cent_graph_upload(cent_ctx* ctx, const struct CSR* csr, cent_graph** out) {
@@ -240,16 +384,7 @@ extern "C" int cent_graph_upload(cent_ctx* ctx, const struct CSR* csr, cent_grap
     return cent_graph_from_csr(ctx, csr->csrV, csr->csrE, csr->csrVSize - 1, csr->csrESize, out);
 }
 
-extern "C" void cent_graph_free(cent_graph* g) {
-    if (!g) return;
-    if (g->ctx) (void)hipSetDevice(g->ctx->device);
-    if (g->rp) (void)hipFree(g->rp);
-    if (g->col) (void)hipFree(g->col);
-    if (g->hub_buf) (void)hipFree(g->hub_buf);
-    delete g;
-}
-
-// ------------------------------------------------------------------ timing
+// ------------------------------------------------------------ timing
 static hipEvent_t next_event(cent_ctx* ctx) {
     if (ctx->ev_used == ctx->ev_pool.size()) {
         hipEvent_t e;
@@ -260,12 +395,39 @@ static hipEvent_t next_event(cent_ctx* ctx) {
 }
 
 struct KernelTimer {
+    bool on = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> fwd, bwd;
 };
 
-// ------------------------------------------------------------------ core
-static int grid_for(int n, const cent_opts* o) {
-    int chunks = (n + kWave - 1) / kWave;
+static int timer_begin(cent_ctx* ctx, KernelTimer& tm, hipEvent_t* e0) {
+    *e0 = nullptr;
+    if (!tm.on) return CENT_OK;
+    *e0 = next_event(ctx);
+    if (!*e0) return CENT_E_DEVICE;
+    HIPTRY(hipEventRecord(*e0, ctx->stream));
+    return CENT_OK;
+}
+
+static int timer_end(cent_ctx* ctx, KernelTimer& tm, hipEvent_t e0, bool fwd) {
+    if (!tm.on) return CENT_OK;
+    hipEvent_t e1 = next_event(ctx);
+    if (!e1) return CENT_E_DEVICE;
+    HIPTRY(hipEventRecord(e1, ctx->stream));
+    (fwd ? tm.fwd : tm.bwd).push_back({e0, e1});
+    return CENT_OK;
+}
+
+static void timer_collect(cent_ctx* ctx, KernelTimer& tm) {
+    float ms;
+    for (auto& p : tm.fwd) if (hipEventElapsedTime(&ms, p.first, p.second) == hipSuccess) ctx->stats.ms_fwd += ms;
+    for (auto& p : tm.bwd) if (hipEventElapsedTime(&ms, p.first, p.second) == hipSuccess) ctx->stats.ms_bwd += ms;
+    tm.fwd.clear();
+    tm.bwd.clear();
+    ctx->ev_used = 0;
+}
+
+// ------------------------------------------------------------ level loop
+static int grid_for(int chunks, const cent_opts* o) {
     int blocks = (chunks + kWavesPerBlock - 1) / kWavesPerBlock;
     int cap = (o && o->grid_waves > 0) ? (o->grid_waves + kWavesPerBlock - 1) / kWavesPerBlock : 2048;
     if (blocks > cap) blocks = cap;
@@ -273,168 +435,212 @@ static int grid_for(int n, const cent_opts* o) {
     return blocks;
 }
 
-// Runs all batches of one call.  with_bc: Brandes (sigma + backward);
-// otherwise closeness only (bitmask BFS).
-static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k,
-                       bool with_bc, int bc_mode, double* bc, int64_t* farness, int64_t* dist_sum,
-                       const cent_opts* opts) {
-    const int n = g->n;
-    const bool compat = with_bc && bc_mode == CENT_BC_REF_COMPAT;
-    const bool timing = opts && (opts->flags & CENT_OPT_TIMING);
-    const int G = grid_for(n, opts);
-    int st;
-    const size_t N = (size_t)(n > 0 ? n : 1);
-    int Lcap = with_bc ? 16 : 2;
-    if ((st = ensure(ctx, ctx->vis, N))) return st;
-    if ((st = ensure(ctx, ctx->F, N * (size_t)(Lcap + 1)))) return st;
-    Lcap = (int)(ctx->F.cap / N) - 1;
-    if (with_bc) {
-        if ((st = ensure(ctx, ctx->sigma, N * kWave))) return st;
-        if ((st = ensure(ctx, ctx->coef, N * kWave))) return st;
-        if ((st = ensure(ctx, ctx->bc, N))) return st;
-        if (compat && (st = ensure(ctx, ctx->npred, N * kWave))) return st;
+struct BatchSpec {
+    bool with_bc = false;
+    bool compat = false;
+    const int64_t* weight = nullptr;  // device, per vertex of the DevCSR, or null
+    int64_t* dist_sum = nullptr;      // device [n] or null
+    double* bc = nullptr;             // device [n] (with_bc)
+    const cent_opts* opts = nullptr;
+};
+
+// Workspace for a DevCSR with N vertices; the F store grows on demand.
+static int ensure_workspace(cent_ctx* ctx, const DevCSR& d, const BatchSpec& sp, int G) {
+    const size_t N = (size_t)(d.n > 0 ? d.n : 1);
+    TRY(ensure(ctx, ctx->vis, N));
+    const size_t want_levels = sp.with_bc ? 16 : 2;
+    if (ctx->F_n != N || ctx->F.cap < N * (want_levels + 1)) {
+        TRY(ensure(ctx, ctx->F, N * (want_levels + 1)));
+        ctx->F_n = N;
     }
-    if (dist_sum && (st = ensure(ctx, ctx->dist_sum, N))) return st;
-    if ((st = ensure(ctx, ctx->far, (size_t)(k > 0 ? k : 1)))) return st;
-    if ((st = ensure(ctx, ctx->part_far, (size_t)G * kWave))) return st;
-    if ((st = ensure(ctx, ctx->part_cnt, (size_t)G * 2))) return st;
-    if ((st = ensure(ctx, ctx->cnt, 2))) return st;
-    if ((st = ensure(ctx, ctx->src, (size_t)(k > 0 ? k : 1)))) return st;
-    const int max_levels = n + 2;
-    // hub segment partials
+    ctx->F_levels = ctx->F.cap / N - 1;
+    if (sp.with_bc) {
+        TRY(ensure(ctx, ctx->sigma, N * kWave));
+        TRY(ensure(ctx, ctx->coef, N * kWave));
+        if (sp.compat) TRY(ensure(ctx, ctx->npred, N * kWave));
+    }
+    TRY(ensure(ctx, ctx->part_far, (size_t)G * kWave));
+    TRY(ensure(ctx, ctx->part_cnt, (size_t)G * 2));
+    TRY(ensure(ctx, ctx->cnt, 2));
+    TRY(ensure(ctx, ctx->flags, 2));
+    if (d.nhub) {
+        TRY(ensure(ctx, ctx->hp_acc, (size_t)d.nseg * kWave));
+        TRY(ensure(ctx, ctx->hp_cnt, (size_t)d.nseg * kWave));
+        TRY(ensure(ctx, ctx->hp_or, (size_t)d.nseg));
+    }
+    return CENT_OK;
+}
+
+static int grow_levels(cent_ctx* ctx, size_t N, int keep_levels) {
+    const size_t ncap = ctx->F_levels * 2;
+    uint64_t* nf = nullptr;
+    HIPTRY(hipMalloc((void**)&nf, sizeof(uint64_t) * N * (ncap + 1)));
+    HIPTRY(hipMemcpyAsync(nf, ctx->F.p, sizeof(uint64_t) * N * (size_t)keep_levels, hipMemcpyDeviceToDevice,
+                          ctx->stream));
+    HIPTRY(hipStreamSynchronize(ctx->stream));
+    HIPTRY(hipFree(ctx->F.p));
+    ctx->F.p = nf;
+    ctx->F.cap = N * (ncap + 1);
+    ctx->F_levels = ncap;
+    return CENT_OK;
+}
+
+// One batch of kb <= 64 sources (device ids d_src[0..kb)) whose BFS stays
+// inside the view's vertex range.  far_out (device, kb entries) gets the
+// weighted distance sums.  Adds the batch's depth to *levels.
+static int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d_src, int kb,
+                       int64_t* far_out, KernelTimer& tm, int64_t* levels) {
+    const DevCSR& d = *v.d;
+    hipStream_t s = ctx->stream;
+    const size_t N = (size_t)(d.n > 0 ? d.n : 1);
+    const size_t R = (size_t)(v.hi - v.lo);
+    const int G = grid_for(v.nchunk, sp.opts);
+    TRY(ensure_workspace(ctx, d, sp, G));
+    const uint64_t A = (kb == 64) ? ~0ull : ((1ull << kb) - 1);
+    HIPTRY(hipMemsetAsync(ctx->vis.p + v.lo, 0, sizeof(uint64_t) * R, s));
+    HIPTRY(hipMemsetAsync(ctx->F.p + v.lo, 0, sizeof(uint64_t) * R, s));
+    HIPTRY(hipMemsetAsync(ctx->F.p + N + v.lo, 0, sizeof(uint64_t) * R, s));
+    HIPTRY(hipMemsetAsync(ctx->part_far.p, 0, sizeof(int64_t) * (size_t)G * kWave, s));
+    HIPTRY(hipMemsetAsync(ctx->part_cnt.p, 0, sizeof(unsigned long long) * (size_t)G * 2, s));
+    HIPTRY(hipMemsetAsync(ctx->flags.p, 0, sizeof(int) * 2, s));
+    k_seed<<<1, kWave, 0, s>>>(d_src, kb, ctx->F.p, ctx->vis.p, sp.with_bc ? ctx->sigma.p : nullptr, sp.with_bc);
+    HIPTRY(hipGetLastError());
+
     HubArgs ha{};
     int Gseg = 1, Ghub = 1;
-    if (g->nhub) {
-        if ((st = ensure(ctx, ctx->hp_acc, (size_t)g->nseg * kWave))) return st;
-        if ((st = ensure(ctx, ctx->hp_cnt, (size_t)g->nseg * kWave))) return st;
-        if ((st = ensure(ctx, ctx->hp_or, (size_t)g->nseg))) return st;
-        ha.nseg = g->nseg; ha.seg_v = g->seg_v; ha.seg_beg = g->seg_beg; ha.seg_end = g->seg_end;
-        ha.nhub = g->nhub; ha.hub_v = g->hub_v; ha.hub_seg_off = g->hub_seg_off;
+    if (v.nhub) {
+        ha.nseg = v.nseg; ha.seg_v = d.seg_v + v.s0; ha.seg_beg = d.seg_beg + v.s0; ha.seg_end = d.seg_end + v.s0;
+        ha.seg_base = v.s0;
+        ha.nhub = v.nhub; ha.hub_v = d.hub_v + v.h0; ha.hub_seg_off = d.hub_seg_off + v.h0;
         ha.part_acc = ctx->hp_acc.p; ha.part_cnt = ctx->hp_cnt.p; ha.part_or = ctx->hp_or.p;
-        Gseg = (g->nseg + kWavesPerBlock - 1) / kWavesPerBlock;
-        if (Gseg > 4096) Gseg = 4096;
-        Ghub = (g->nhub + kWavesPerBlock - 1) / kWavesPerBlock;
-        if (Ghub > G) Ghub = G;  // k_fwd_hubfin flushes into the G per-block slots
+        Gseg = std::min((v.nseg + kWavesPerBlock - 1) / kWavesPerBlock, 4096);
+        Ghub = std::min((v.nhub + kWavesPerBlock - 1) / kWavesPerBlock, G);  // hubfin flushes into G slots
     }
-    if ((st = ensure(ctx, ctx->flags, 2))) return st;
-    if (!ctx->h_flags) {
-        HIPTRY(hipHostMalloc((void**)&ctx->h_flags, sizeof(int) * 2, hipHostMallocDefault));
-        ctx->h_flags_cap = 2;
-    }
-    hipStream_t s = ctx->stream;
-    if (k > 0) HIPTRY(hipMemcpyAsync(ctx->src.p, src, sizeof(int) * (size_t)k, hipMemcpyHostToDevice, s));
-    if (with_bc) HIPTRY(hipMemsetAsync(ctx->bc.p, 0, sizeof(double) * N, s));
-    if (dist_sum) HIPTRY(hipMemsetAsync(ctx->dist_sum.p, 0, sizeof(int64_t) * N, s));
-    HIPTRY(hipMemsetAsync(ctx->cnt.p, 0, sizeof(unsigned long long) * 2, s));
-
-    ctx->ev_used = 0;
-    KernelTimer tm;
-    int64_t levels_total = 0;
-    int64_t batches = 0;
-    const bool timing_ok = timing;
-
-    for (int32_t b0 = 0; b0 < k; b0 += kWave) {
-        const int kb = (k - b0) < kWave ? (k - b0) : kWave;
-        const uint64_t A = (kb == 64) ? ~0ull : ((1ull << kb) - 1);
-        HIPTRY(hipMemsetAsync(ctx->vis.p, 0, sizeof(uint64_t) * N, s));
-        HIPTRY(hipMemsetAsync(ctx->F.p, 0, sizeof(uint64_t) * N * 2, s));
-        HIPTRY(hipMemsetAsync(ctx->part_far.p, 0, sizeof(int64_t) * (size_t)G * kWave, s));
-        HIPTRY(hipMemsetAsync(ctx->part_cnt.p, 0, sizeof(unsigned long long) * (size_t)G * 2, s));
-        HIPTRY(hipMemsetAsync(ctx->flags.p, 0, sizeof(int) * 2, s));
-        k_seed<<<1, kWave, 0, s>>>(ctx->src.p + b0, kb, ctx->F.p, ctx->vis.p,
-                                   with_bc ? ctx->sigma.p : nullptr, with_bc);
-        int d = 0;  // current frontier distance
-        for (;;) {
-            if (d + 1 >= max_levels) break;
-            // frontier buffers: with_bc keeps every level, closeness ping-pongs two
-            if (with_bc && d + 2 > Lcap) {
-                // grow the level store, keeping levels 0..d+1
-                int ncap = Lcap * 2;
-                uint64_t* nf = nullptr;
-                HIPTRY(hipMalloc((void**)&nf, sizeof(uint64_t) * N * (size_t)(ncap + 1)));
-                HIPTRY(hipMemcpyAsync(nf, ctx->F.p, sizeof(uint64_t) * N * (size_t)(d + 1),
-                                      hipMemcpyDeviceToDevice, s));
-                HIPTRY(hipStreamSynchronize(s));
-                HIPTRY(hipFree(ctx->F.p));
-                ctx->F.p = nf;
-                ctx->F.cap = N * (size_t)(ncap + 1);
-                Lcap = ncap;
-            }
-            const uint64_t* Fd = ctx->F.p + N * (size_t)(with_bc ? d : (d & 1));
-            uint64_t* Fn = ctx->F.p + N * (size_t)(with_bc ? d + 1 : ((d + 1) & 1));
-            if (with_bc) {
-                if (d + 1 > 1) HIPTRY(hipMemsetAsync(Fn, 0, sizeof(uint64_t) * N, s));
+    const bool bc = sp.with_bc, compat = sp.compat;
+    int dl = 0;  // current frontier distance
+    for (;;) {
+        if (bc && (size_t)(dl + 2) > ctx->F_levels) TRY(grow_levels(ctx, N, dl + 1));
+        const uint64_t* Fd = ctx->F.p + N * (size_t)(bc ? dl : (dl & 1));
+        uint64_t* Fn = ctx->F.p + N * (size_t)(bc ? dl + 1 : ((dl + 1) & 1));
+        if (!bc || dl + 1 > 1) HIPTRY(hipMemsetAsync(Fn + v.lo, 0, sizeof(uint64_t) * R, s));
+        HIPTRY(hipMemsetAsync(ctx->flags.p, 0, sizeof(int), s));
+        FwdArgs fa;
+        fa.n = d.n; fa.hub_deg = d.hub_deg; fa.nchunk = v.nchunk; fa.chunk_beg = d.chunk_beg + v.c0;
+        fa.rp = d.rp; fa.col = d.col; fa.vis = ctx->vis.p; fa.Fd = Fd; fa.Fn = Fn;
+        fa.sigma = bc ? ctx->sigma.p : nullptr;
+        fa.npred = compat ? ctx->npred.p : nullptr;
+        fa.dist_sum = sp.dist_sum;
+        fa.weight = sp.weight;
+        fa.A = A; fa.dnext = dl + 1;
+        fa.part_far = ctx->part_far.p; fa.part_cnt = ctx->part_cnt.p;
+        fa.flag = ctx->flags.p; fa.overflow = ctx->flags.p + 1;
+        hipEvent_t e0;
+        TRY(timer_begin(ctx, tm, &e0));
+        if (bc) {
+            if (compat) k_fwd<true, true><<<G, kBlock, 0, s>>>(fa);
+            else k_fwd<true, false><<<G, kBlock, 0, s>>>(fa);
+        } else {
+            k_fwd<false, false><<<G, kBlock, 0, s>>>(fa);
+        }
+        HIPTRY(hipGetLastError());
+        if (v.nhub) {
+            if (bc) {
+                k_fwd_hubseg<true><<<Gseg, kBlock, 0, s>>>(fa, ha);
+                if (compat) k_fwd_hubfin<true, true><<<Ghub, kBlock, 0, s>>>(fa, ha);
+                else k_fwd_hubfin<true, false><<<Ghub, kBlock, 0, s>>>(fa, ha);
             } else {
-                HIPTRY(hipMemsetAsync(Fn, 0, sizeof(uint64_t) * N, s));
-            }
-            HIPTRY(hipMemsetAsync(ctx->flags.p, 0, sizeof(int), s));
-            FwdArgs fa;
-            fa.n = n; fa.hub_deg = g->hub_deg; fa.rp = g->rp; fa.col = g->col; fa.vis = ctx->vis.p;
-            fa.Fd = Fd; fa.Fn = Fn;
-            fa.sigma = with_bc ? ctx->sigma.p : nullptr;
-            fa.npred = compat ? ctx->npred.p : nullptr;
-            fa.dist_sum = dist_sum ? ctx->dist_sum.p : nullptr;
-            fa.weight = nullptr;
-            fa.A = A; fa.dnext = d + 1;
-            fa.part_far = ctx->part_far.p; fa.part_cnt = ctx->part_cnt.p;
-            fa.flag = ctx->flags.p; fa.overflow = ctx->flags.p + 1;
-            hipEvent_t e0 = nullptr, e1 = nullptr;
-            if (timing_ok) { e0 = next_event(ctx); e1 = next_event(ctx); HIPTRY(hipEventRecord(e0, s)); }
-            if (with_bc) {
-                if (compat) k_fwd<true, true><<<G, kBlock, 0, s>>>(fa);
-                else k_fwd<true, false><<<G, kBlock, 0, s>>>(fa);
-            } else {
-                k_fwd<false, false><<<G, kBlock, 0, s>>>(fa);
+                k_fwd_hubseg<false><<<Gseg, kBlock, 0, s>>>(fa, ha);
+                k_fwd_hubfin<false, false><<<Ghub, kBlock, 0, s>>>(fa, ha);
             }
             HIPTRY(hipGetLastError());
-            if (g->nhub) {
-                if (with_bc) {
-                    k_fwd_hubseg<true><<<Gseg, kBlock, 0, s>>>(fa, ha);
-                    if (compat) k_fwd_hubfin<true, true><<<Ghub, kBlock, 0, s>>>(fa, ha);
-                    else k_fwd_hubfin<true, false><<<Ghub, kBlock, 0, s>>>(fa, ha);
-                } else {
-                    k_fwd_hubseg<false><<<Gseg, kBlock, 0, s>>>(fa, ha);
-                    k_fwd_hubfin<false, false><<<Ghub, kBlock, 0, s>>>(fa, ha);
-                }
+        }
+        TRY(timer_end(ctx, tm, e0, true));
+        ctx->stats.launches_fwd++;
+        HIPTRY(hipMemcpyAsync(ctx->h_flags, ctx->flags.p, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+        HIPTRY(hipStreamSynchronize(s));
+        if (ctx->h_flags[1]) return CENT_E_OVERFLOW;
+        if (!ctx->h_flags[0]) break;
+        ++dl;
+    }
+    *levels += dl;  // deepest non-empty level
+    if (bc) {
+        for (int lv = dl; lv >= 1; --lv) {
+            BwdArgs ba;
+            ba.n = d.n; ba.hub_deg = d.hub_deg; ba.nchunk = v.nchunk; ba.chunk_beg = d.chunk_beg + v.c0;
+            ba.rp = d.rp; ba.col = d.col;
+            ba.Fd = ctx->F.p + N * (size_t)lv;
+            ba.Fn = ctx->F.p + N * (size_t)(lv + 1);
+            ba.sigma = ctx->sigma.p; ba.coef = ctx->coef.p;
+            ba.npred = compat ? ctx->npred.p : nullptr;
+            ba.bc = sp.bc;
+            hipEvent_t e0;
+            TRY(timer_begin(ctx, tm, &e0));
+            if (compat) k_bwd<true><<<G, kBlock, 0, s>>>(ba);
+            else k_bwd<false><<<G, kBlock, 0, s>>>(ba);
+            HIPTRY(hipGetLastError());
+            if (v.nhub) {
+                k_bwd_hubseg<<<Gseg, kBlock, 0, s>>>(ba, ha);
+                if (compat) k_bwd_hubfin<true><<<Ghub, kBlock, 0, s>>>(ba, ha);
+                else k_bwd_hubfin<false><<<Ghub, kBlock, 0, s>>>(ba, ha);
                 HIPTRY(hipGetLastError());
             }
-            if (timing_ok) { HIPTRY(hipEventRecord(e1, s)); tm.fwd.push_back({e0, e1}); }
-            ctx->stats.launches_fwd++;
-            HIPTRY(hipMemcpyAsync(ctx->h_flags, ctx->flags.p, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+            TRY(timer_end(ctx, tm, e0, false));
+            ctx->stats.launches_bwd++;
+        }
+    }
+    k_reduce_batch<<<1, 1024, 0, s>>>(ctx->part_far.p, ctx->part_cnt.p, G, kb, far_out, ctx->cnt.p);
+    HIPTRY(hipGetLastError());
+    ctx->stats.batches++;
+    return CENT_OK;
+}
+
+// Runs all batches of one plain call.  with_bc: Brandes (sigma + backward);
+// otherwise closeness only (bitmask BFS).
+static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k, bool with_bc,
+                       int bc_mode, double* bc, int64_t* farness, int64_t* dist_sum, const cent_opts* opts) {
+    const int n = g->n;
+    const size_t N = (size_t)(n > 0 ? n : 1);
+    hipStream_t s = ctx->stream;
+    BatchSpec sp;
+    sp.with_bc = with_bc;
+    sp.compat = with_bc && bc_mode == CENT_BC_REF_COMPAT;
+    sp.opts = opts;
+    if (with_bc) {
+        TRY(ensure(ctx, ctx->bc, N));
+        sp.bc = ctx->bc.p;
+        HIPTRY(hipMemsetAsync(ctx->bc.p, 0, sizeof(double) * N, s));
+    }
+    if (dist_sum) {
+        TRY(ensure(ctx, ctx->dist_sum, N));
+        sp.dist_sum = ctx->dist_sum.p;
+        HIPTRY(hipMemsetAsync(ctx->dist_sum.p, 0, sizeof(int64_t) * N, s));
+    }
+    TRY(ensure(ctx, ctx->far, (size_t)(k > 0 ? k : 1)));
+    TRY(ensure(ctx, ctx->src, (size_t)(k > 0 ? k : 1)));
+    TRY(ensure(ctx, ctx->cnt, 2));
+    HIPTRY(hipMemsetAsync(ctx->cnt.p, 0, sizeof(unsigned long long) * 2, s));
+    std::vector<int32_t> dsrc;
+    const int32_t* up_src = src;
+    if (!g->new_id.empty() && k > 0) {
+        dsrc.resize((size_t)k);
+        for (int32_t i = 0; i < k; ++i) dsrc[(size_t)i] = g->new_id[(size_t)src[i]];
+        up_src = dsrc.data();
+    }
+    if (k > 0) HIPTRY(hipMemcpyAsync(ctx->src.p, up_src, sizeof(int) * (size_t)k, hipMemcpyHostToDevice, s));
+    HIPTRY(hipStreamSynchronize(s));  // dsrc is a pageable host buffer
+
+    KernelTimer tm;
+    tm.on = opts && (opts->flags & CENT_OPT_TIMING);
+    ctx->ev_used = 0;
+    const GView view = full_view(g->d);
+    int64_t levels = 0;
+    for (int32_t b0 = 0; b0 < k; b0 += kWave) {
+        const int kb = (k - b0) < kWave ? (k - b0) : kWave;
+        TRY(level_batch(ctx, view, sp, ctx->src.p + b0, kb, ctx->far.p + b0, tm, &levels));
+        if (tm.on && ctx->ev_used > 4096) {  // keep the event pool bounded
             HIPTRY(hipStreamSynchronize(s));
-            if (ctx->h_flags[1]) return CENT_E_OVERFLOW;
-            if (!ctx->h_flags[0]) break;
-            ++d;
+            timer_collect(ctx, tm);
         }
-        // d = deepest non-empty level
-        levels_total += d;
-        if (with_bc) {
-            for (int lv = d; lv >= 1; --lv) {
-                BwdArgs ba;
-                ba.n = n; ba.hub_deg = g->hub_deg; ba.rp = g->rp; ba.col = g->col;
-                ba.Fd = ctx->F.p + N * (size_t)lv;
-                ba.Fn = ctx->F.p + N * (size_t)(lv + 1);
-                ba.sigma = ctx->sigma.p; ba.coef = ctx->coef.p;
-                ba.npred = compat ? ctx->npred.p : nullptr;
-                ba.bc = ctx->bc.p;
-                hipEvent_t e0 = nullptr, e1 = nullptr;
-                if (timing_ok) { e0 = next_event(ctx); e1 = next_event(ctx); HIPTRY(hipEventRecord(e0, s)); }
-                if (compat) k_bwd<true><<<G, kBlock, 0, s>>>(ba);
-                else k_bwd<false><<<G, kBlock, 0, s>>>(ba);
-                HIPTRY(hipGetLastError());
-                if (g->nhub) {
-                    k_bwd_hubseg<<<Gseg, kBlock, 0, s>>>(ba, ha);
-                    if (compat) k_bwd_hubfin<true><<<Ghub, kBlock, 0, s>>>(ba, ha);
-                    else k_bwd_hubfin<false><<<Ghub, kBlock, 0, s>>>(ba, ha);
-                    HIPTRY(hipGetLastError());
-                }
-                if (timing_ok) { HIPTRY(hipEventRecord(e1, s)); tm.bwd.push_back({e0, e1}); }
-                ctx->stats.launches_bwd++;
-            }
-        }
-        k_reduce_batch<<<1, 1024, 0, s>>>(ctx->part_far.p, ctx->part_cnt.p, G, kb, ctx->far.p + b0, ctx->cnt.p);
-        HIPTRY(hipGetLastError());
-        ++batches;
     }
     // results back to the host
     std::vector<double> hb;
@@ -452,20 +658,16 @@ static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
         HIPTRY(hipMemcpyAsync(farness, ctx->far.p, sizeof(int64_t) * (size_t)k, hipMemcpyDeviceToHost, s));
     HIPTRY(hipMemcpyAsync(hc, ctx->cnt.p, sizeof(hc), hipMemcpyDeviceToHost, s));
     HIPTRY(hipStreamSynchronize(s));
+    const bool remap = !g->new_id.empty();
     if (with_bc && bc)
-        for (int v = 0; v < n; ++v) bc[v] += hb[v];
+        for (int v = 0; v < n; ++v) bc[v] += hb[remap ? (size_t)g->new_id[(size_t)v] : (size_t)v];
     if (dist_sum)
-        for (int v = 0; v < n; ++v) dist_sum[v] += hd[v];
-    if (timing_ok) {
-        float ms;
-        for (auto& p : tm.fwd) if (hipEventElapsedTime(&ms, p.first, p.second) == hipSuccess) ctx->stats.ms_fwd += ms;
-        for (auto& p : tm.bwd) if (hipEventElapsedTime(&ms, p.first, p.second) == hipSuccess) ctx->stats.ms_bwd += ms;
-    }
-    ctx->stats.batches += batches;
-    ctx->stats.levels += levels_total;
+        for (int v = 0; v < n; ++v) dist_sum[v] += hd[remap ? (size_t)g->new_id[(size_t)v] : (size_t)v];
+    timer_collect(ctx, tm);
+    ctx->stats.levels += levels;
     ctx->stats.reached += (int64_t)hc[0] + k;
     ctx->stats.dag_edges += (int64_t)hc[1];
-    for (int32_t i = 0; i < k; ++i) ctx->stats.edges_reached += g->comp_nnz[(size_t)g->comp[src[i]]];
+    for (int32_t i = 0; i < k; ++i) ctx->stats.edges_reached += g->comp_nnz[(size_t)g->comp[(size_t)src[i]]];
     return CENT_OK;
 }
 
@@ -494,11 +696,10 @@ extern "C" int cent_bc(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
     std::vector<int32_t> all;
     const int32_t* s;
     int32_t kk;
-    int st = prepare_sources(g, src, k, all, &s, &kk);
-    if (st) return st;
-    if ((st = check_device(ctx))) return st;
+    TRY(prepare_sources(g, src, k, all, &s, &kk));
+    TRY(check_device(ctx));
     auto t0 = std::chrono::steady_clock::now();
-    st = run_batches(ctx, g, s, kk, true, mode, bc, farness, dist_sum, opts);
+    int st = run_batches(ctx, g, s, kk, true, mode, bc, farness, dist_sum, opts);
     ctx->stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return st;
 }
@@ -509,27 +710,181 @@ extern "C" int cent_cc(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
     std::vector<int32_t> all;
     const int32_t* s;
     int32_t kk;
-    int st = prepare_sources(g, src, k, all, &s, &kk);
-    if (st) return st;
-    if ((st = check_device(ctx))) return st;
+    TRY(prepare_sources(g, src, k, all, &s, &kk));
+    TRY(check_device(ctx));
     auto t0 = std::chrono::steady_clock::now();
-    st = run_batches(ctx, g, s, kk, false, 0, nullptr, farness, dist_sum, opts);
+    int st = run_batches(ctx, g, s, kk, false, 0, nullptr, farness, dist_sum, opts);
     ctx->stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return st;
 }
 
-extern "C" int cent_ctx_reset_stats(cent_ctx* ctx) {
-    if (!ctx) return CENT_E_INVAL;
-    ctx->stats = cent_stats{};
+// ------------------------------------------------------------ reduced closeness
+// Device copy of a packed block graph, cached on the cent_reduced object
+// (freed through its dev_free hook).
+struct RedDev {
+    cent_ctx* ctx = nullptr;
+    DevCSR d;
+    int64_t* w = nullptr;     // [np]
+    int* iota = nullptr;      // [np] packed ids 0..np-1 (batch sources)
+    int4* items = nullptr;    // [nitems]
+    int* order = nullptr;     // [nsmall] small items, largest range first
+    int* head = nullptr;      // queue counter
+    int64_t* D = nullptr;     // [np]
+    int nsmall = 0;
+    std::vector<int> big;     // items handled by the level-synchronous kernels
+};
+
+static void reddev_free(void* p) {
+    RedDev* rd = (RedDev*)p;
+    if (!rd) return;
+    if (rd->ctx) (void)hipSetDevice(rd->ctx->device);
+    devcsr_free(rd->d);
+    if (rd->w) (void)hipFree(rd->w);
+    if (rd->iota) (void)hipFree(rd->iota);
+    if (rd->items) (void)hipFree(rd->items);
+    if (rd->order) (void)hipFree(rd->order);
+    if (rd->head) (void)hipFree(rd->head);
+    if (rd->D) (void)hipFree(rd->D);
+    delete rd;
+}
+
+static int reddev_get(cent_ctx* ctx, cent_reduced* r, RedDev** out) {
+    if (r->dev) {
+        RedDev* rd = (RedDev*)r->dev;
+        if (rd->ctx != ctx) return CENT_E_INVAL;  // one context per reduced graph
+        *out = rd;
+        return CENT_OK;
+    }
+    RedDev* rd = new (std::nothrow) RedDev();
+    if (!rd) return CENT_E_OOM;
+    rd->ctx = ctx;
+    r->dev = rd;
+    r->dev_free = reddev_free;
+    const size_t NP = (size_t)(r->np > 0 ? r->np : 1);
+    TRY(devcsr_build(ctx, r->prp, r->pcol, r->np, r->nnzp, r->boff, r->nblocks, rd->d));
+    TRY(upload(ctx, &rd->w, r->pw, (size_t)r->np));
+    std::vector<int> iota(NP);
+    for (size_t i = 0; i < NP; ++i) iota[i] = (int)i;
+    TRY(upload(ctx, &rd->iota, iota.data(), NP));
+    std::vector<int4> items((size_t)(r->nitems > 0 ? r->nitems : 1));
+    std::vector<int> small;
+    for (int i = 0; i < r->nitems; ++i) {
+        const cent_item& it = r->items[i];
+        items[(size_t)i] = make_int4(it.lo, it.hi, it.s0, it.ns);
+        if (it.hi - it.lo <= kItemLds) small.push_back(i);
+        else rd->big.push_back(i);
+    }
+    std::stable_sort(small.begin(), small.end(), [&](int a, int b) {
+        return (r->items[a].hi - r->items[a].lo) > (r->items[b].hi - r->items[b].lo);
+    });
+    rd->nsmall = (int)small.size();
+    TRY(upload(ctx, &rd->items, items.data(), (size_t)r->nitems));
+    TRY(upload(ctx, &rd->order, small.data(), small.size()));
+    int zero = 0;
+    TRY(upload(ctx, &rd->head, &zero, 1));
+    HIPTRY(hipMalloc((void**)&rd->D, sizeof(int64_t) * NP));
+    *out = rd;
     return CENT_OK;
 }
 
-// ------------------------------------------- reference-shaped entry points
+extern "C" int cent_cc_reduced(cent_ctx* ctx, const cent_reduced* rc, int64_t* farness, int32_t sched,
+                               const cent_opts* opts) {
+    if (!ctx || !rc || !farness) return CENT_E_INVAL;
+    if (sched != CENT_SCHED_PERSISTENT && sched != CENT_SCHED_PER_ITEM) return CENT_E_INVAL;
+    cent_reduced* r = const_cast<cent_reduced*>(rc);  // the device cache lives on the object
+    TRY(check_device(ctx));
+    auto t0 = std::chrono::steady_clock::now();
+    RedDev* rd;
+    TRY(reddev_get(ctx, r, &rd));
+    hipStream_t s = ctx->stream;
+    const size_t NP = (size_t)(r->np > 0 ? r->np : 1);
+    HIPTRY(hipMemsetAsync(rd->D, 0, sizeof(int64_t) * NP, s));
+    TRY(ensure(ctx, ctx->cnt, 2));
+    HIPTRY(hipMemsetAsync(ctx->cnt.p, 0, sizeof(unsigned long long) * 2, s));
+    KernelTimer tm;
+    tm.on = opts && (opts->flags & CENT_OPT_TIMING);
+    ctx->ev_used = 0;
+    BatchSpec sp;
+    sp.weight = rd->w;
+    sp.opts = opts;
+    int64_t levels = 0;
+    auto run_item = [&](int i) -> int {
+        const cent_item& it = r->items[i];
+        const GView v = sub_view(rd->d, it.lo, it.hi);
+        return level_batch(ctx, v, sp, rd->iota + it.s0, it.ns, rd->D + it.s0, tm, &levels);
+    };
+    if (sched == CENT_SCHED_PER_ITEM) {
+        for (int i = 0; i < r->nitems; ++i) TRY(run_item(i));
+    } else {
+        if (rd->nsmall) {
+            HIPTRY(hipMemsetAsync(rd->head, 0, sizeof(int), s));
+            ItemArgs ia;
+            ia.rp = rd->d.rp; ia.col = rd->d.col; ia.weight = rd->w; ia.items = rd->items; ia.order = rd->order;
+            ia.nitems = rd->nsmall; ia.head = rd->head; ia.D = rd->D;
+            const int grid = std::min(rd->nsmall, 2048);
+            hipEvent_t e0;
+            TRY(timer_begin(ctx, tm, &e0));
+            k_cc_items<<<grid, kItemBlock, 0, s>>>(ia);
+            HIPTRY(hipGetLastError());
+            TRY(timer_end(ctx, tm, e0, true));
+            ctx->stats.launches_fwd++;
+            ctx->stats.batches += rd->nsmall;
+        }
+        for (int i : rd->big) TRY(run_item(i));
+    }
+    std::vector<int64_t> D(NP);
+    HIPTRY(hipMemcpyAsync(D.data(), rd->D, sizeof(int64_t) * NP, hipMemcpyDeviceToHost, s));
+    HIPTRY(hipStreamSynchronize(s));
+    timer_collect(ctx, tm);
+    TRY(cent_reduced_finish(r, D.data(), farness));
+    ctx->stats.levels += levels;
+    ctx->stats.reached += (int64_t)r->n;  // every vertex is a source
+    ctx->stats.edges_reached += 2 * r->m;
+    ctx->stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return CENT_OK;
+}
+
+// ------------------------------------------------------------ probe
+// Plain streaming copy, 16 B per lane per step: the calibration kernel for
+// HBM counter readings and the measured bandwidth ceiling of the device.
+__global__ __launch_bounds__(256) void k_copy_probe(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                     size_t n16) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
+extern "C" int cent_probe_copy_bandwidth(cent_ctx* ctx, int64_t bytes, int32_t iters, double* gbs) {
+    if (!ctx || bytes < 1024 || iters < 1 || !gbs) return CENT_E_INVAL;
+    TRY(check_device(ctx));
+    const size_t n16 = (size_t)bytes / 16;
+    uint4 *a = nullptr, *b = nullptr;
+    HIPTRY(hipMalloc((void**)&a, n16 * 16));
+    if (hipMalloc((void**)&b, n16 * 16) != hipSuccess) { (void)hipFree(a); return CENT_E_OOM; }
+    ctx->ev_used = 0;
+    hipEvent_t e0 = next_event(ctx), e1 = next_event(ctx);
+    int rc = CENT_OK;
+    if (!e0 || !e1 || hipMemsetAsync(a, 1, n16 * 16, ctx->stream) != hipSuccess) rc = CENT_E_DEVICE;
+    const int grid = 256 * 32;
+    if (!rc) {
+        k_copy_probe<<<grid, 256, 0, ctx->stream>>>(a, b, n16);  // warm
+        (void)hipEventRecord(e0, ctx->stream);
+        for (int i = 0; i < iters; ++i) k_copy_probe<<<grid, 256, 0, ctx->stream>>>(a, b, n16);
+        (void)hipEventRecord(e1, ctx->stream);
+        float ms = 0;
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = CENT_E_DEVICE;
+        else *gbs = 2.0 * (double)(n16 * 16) * iters / (ms * 1e-3) / 1e9;
+    }
+    ctx->ev_used = 0;
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return rc;
+}
+
+// ------------------------------------------------------------ reference shapes
 static std::mutex g_default_mu;
 static cent_ctx* g_default = nullptr;
 
 static int default_ctx(cent_ctx** out) {
-    std::lock_guard<std::mutex> lk(g_default_mu);
     if (!g_default) {
         int st = cent_ctx_create(0, &g_default);
         if (st) return st;
@@ -540,17 +895,17 @@ static int default_ctx(cent_ctx** out) {
 
 static int shim_run(const struct CSR* csr, bool with_bc, int mode, double* BCs, int64_t* CCs) {
     if (!csr || (with_bc && !BCs) || (!with_bc && !CCs)) return CENT_E_INVAL;
-    cent_ctx* ctx;
-    int st = default_ctx(&ctx);
-    if (st) return st;
     std::lock_guard<std::mutex> lk(g_default_mu);
+    cent_ctx* ctx;
+    TRY(default_ctx(&ctx));
     cent_graph* g = nullptr;
-    if ((st = cent_graph_upload(ctx, csr, &g))) return st;
+    TRY(cent_graph_upload(ctx, csr, &g));
     int32_t k = csr->endNodeID - csr->startNodeID + 1;
     std::vector<int32_t> src((size_t)(k > 0 ? k : 0));
     for (int32_t i = 0; i < k; ++i) src[(size_t)i] = csr->startNodeID + i;
     cent_opts o{};
     o.bc_mode = mode;
+    int st;
     if (with_bc) {
         st = cent_bc(ctx, g, src.data(), k, BCs, nullptr, nullptr, &o);
     } else {
@@ -566,3 +921,19 @@ static int shim_run(const struct CSR* csr, bool with_bc, int mode, double* BCs, 
 extern "C" int computeBC(const struct CSR* csr, double* BCs) { return shim_run(csr, true, CENT_BC_TEXTBOOK, BCs, nullptr); }
 extern "C" int computeBC_compat(const struct CSR* csr, double* BCs) { return shim_run(csr, true, CENT_BC_REF_COMPAT, BCs, nullptr); }
 extern "C" int computeCC(const struct CSR* csr, int64_t* CCs) { return shim_run(csr, false, 0, nullptr, CCs); }
+
+extern "C" int compute_D1_AP_CC_shareBased_DegreeOrder(const struct CSR* csr, int64_t* CCs) {
+    if (!csr || !CCs) return CENT_E_INVAL;
+    std::lock_guard<std::mutex> lk(g_default_mu);
+    cent_ctx* ctx;
+    TRY(default_ctx(&ctx));
+    cent_reduced* r = nullptr;
+    TRY(cent_reduce(csr, &r));
+    const int32_t n = csr->csrVSize - 1;
+    std::vector<int64_t> far((size_t)(n > 0 ? n : 1));
+    int st = cent_cc_reduced(ctx, r, far.data(), CENT_SCHED_PERSISTENT, nullptr);
+    if (!st)
+        for (int32_t v = csr->startNodeID; v <= csr->endNodeID; ++v) CCs[v] = far[(size_t)v];
+    cent_reduced_free(r);
+    return st;
+}

@@ -20,7 +20,9 @@
 // the lane sum of delta (or of delta * #predecessors in the reference's
 // per-predecessor formula), one deterministic butterfly reduction per level.
 //
-// Load balance: rows longer than the hub threshold are cut into fixed edge
+// Load balance: a wavefront takes one vertex chunk at a time — at most 64
+// consecutive vertices whose rows hold at most a fixed edge budget — and
+// rows longer than the hub threshold are cut into fixed edge
 // segments, each scanned by its own wavefront (k_*_hubseg) into a per-segment
 // lane partial; k_*_hubfin then adds a hub's partials in segment order, so
 // the sums do not depend on scheduling.  The main kernels skip those rows.
@@ -38,6 +40,7 @@ struct HubArgs {
     const int* __restrict__ seg_v;  // [nseg] owning vertex
     const int* __restrict__ seg_beg;
     const int* __restrict__ seg_end;
+    int seg_base;                         // hub_seg_off value of segment 0 of this view
     int nhub;
     const int* __restrict__ hub_v;        // [nhub]
     const int* __restrict__ hub_seg_off;  // [nhub + 1]
@@ -49,6 +52,8 @@ struct HubArgs {
 struct FwdArgs {
     int n;
     int hub_deg;                          // rows longer than this are hub rows
+    int nchunk;                           // vertex chunks: <= 64 vertices, bounded edges
+    const int* __restrict__ chunk_beg;    // [nchunk + 1]
     const int* __restrict__ rp;
     const int* __restrict__ col;
     uint64_t* __restrict__ vis;
@@ -69,6 +74,8 @@ struct FwdArgs {
 struct BwdArgs {
     int n;
     int hub_deg;
+    int nchunk;
+    const int* __restrict__ chunk_beg;
     const int* __restrict__ rp;
     const int* __restrict__ col;
     const uint64_t* __restrict__ Fd;      // level d masks
@@ -240,11 +247,12 @@ __global__ __launch_bounds__(kBlock) void k_fwd(FwdArgs a) {
     const int nw = gridDim.x * kWavesPerBlock;
     const uint64_t A = a.A;
     FwdAccum r;
-    for (int base = gw * kWave; base < a.n; base += nw * kWave) {
+    for (int c = gw; c < a.nchunk; c += nw) {
+        const int base = a.chunk_beg[c];
         const int vl = base + lane;
         uint64_t visl = A;
         int rbl = 0, rel = 0;
-        if (vl < a.n) {
+        if (vl < a.chunk_beg[c + 1]) {
             visl = a.vis[vl];
             if (visl != A) {
                 rbl = a.rp[vl];
@@ -304,7 +312,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_hubfin(FwdArgs a, HubArgs h) {
         double acc = 0.0;
         uint32_t cnt = 0;
         uint64_t orf = 0;
-        for (int i = h.hub_seg_off[k]; i < h.hub_seg_off[k + 1]; ++i) {
+        for (int i = h.hub_seg_off[k] - h.seg_base; i < h.hub_seg_off[k + 1] - h.seg_base; ++i) {
             if constexpr (SIGMA) {
                 acc += h.part_acc[(size_t)i * kWave + lane];
                 cnt += h.part_cnt[(size_t)i * kWave + lane];
@@ -323,11 +331,12 @@ __global__ __launch_bounds__(kBlock) void k_bwd(BwdArgs a) {
     const int lane = lane_id();
     const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const int nw = gridDim.x * kWavesPerBlock;
-    for (int base = gw * kWave; base < a.n; base += nw * kWave) {
+    for (int c = gw; c < a.nchunk; c += nw) {
+        const int base = a.chunk_beg[c];
         const int vl = base + lane;
         uint64_t fl = 0;
         int rbl = 0, rel = 0;
-        if (vl < a.n) {
+        if (vl < a.chunk_beg[c + 1]) {
             fl = a.Fd[vl];
             if (fl) {
                 rbl = a.rp[vl];
@@ -367,7 +376,8 @@ __global__ __launch_bounds__(kBlock) void k_bwd_hubfin(BwdArgs a, HubArgs h) {
         const uint64_t m = a.Fd[v];
         if (!m) continue;
         double acc = 0.0;
-        for (int i = h.hub_seg_off[k]; i < h.hub_seg_off[k + 1]; ++i) acc += h.part_acc[(size_t)i * kWave + lane];
+        for (int i = h.hub_seg_off[k] - h.seg_base; i < h.hub_seg_off[k + 1] - h.seg_base; ++i)
+            acc += h.part_acc[(size_t)i * kWave + lane];
         bwd_vertex_done(a, v, m, acc, lane, COMPAT);
     }
 }

@@ -51,6 +51,10 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=-1,
                     help="sources for the CPU baseline (default: 8 per core)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--keep-order", action="store_true",
+                    help="keep the generator's vertex ids on the device (no degree relabel)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also time a plain 1 GiB device copy (HBM ceiling, counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -69,7 +73,7 @@ def main() -> int:
           f"(generated in {time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
 
     ctx = bc_amd.Context(0)
-    dg = ctx.upload(g)
+    dg = ctx.upload(g, keep_order=args.keep_order)
     bc = np.zeros(g.n)
     for i in range(args.warmup):
         ctx.bc(dg, src, bc=bc, dist_sum=True)
@@ -106,6 +110,9 @@ def main() -> int:
         "fwd_GBs": round(b_fwd / (ms_fwd * 1e-3) / 1e9, 1) if ms_fwd else None,
         "bwd_GBs": round(b_bwd / (ms_bwd * 1e-3) / 1e9, 1) if ms_bwd else None,
     }
+
+    if args.copy_probe:
+        roofline["copy_probe_GBs"] = round(ctx.probe_copy_bandwidth(1 << 30, 10), 1)
 
     cpu = None
     if not args.no_cpu_baseline:

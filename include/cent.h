@@ -148,7 +148,18 @@ int  cent_ctx_stats(const cent_ctx* ctx, cent_stats* out);
 int  cent_ctx_reset_stats(cent_ctx* ctx);
 /* Text of the last HIP error seen by this context (empty if none). */
 const char* cent_ctx_last_error(const cent_ctx* ctx);
+/* Measurement helper (no reference counterpart): times `iters` launches of a
+ * plain device copy of `bytes` bytes and returns read+write GB/s.  Serves as
+ * the calibration kernel for HBM counter readings. */
+int  cent_probe_copy_bandwidth(cent_ctx* ctx, int64_t bytes, int32_t iters, double* gbs);
 
+/* Uploads a CSR (validated; ids in range, offsets monotone).  By default the
+ * device copy is relabelled by descending degree for cache locality; every
+ * input and output of the engine stays in the caller's ids either way.
+ * flags: CENT_GRAPH_KEEP_ORDER keeps the caller's ids on the device too. */
+#define CENT_GRAPH_KEEP_ORDER 1
+int  cent_graph_create(cent_ctx* ctx, const int32_t* row_ptr, const int32_t* col,
+                       int32_t n, int64_t nnz, int32_t flags, cent_graph** out);
 int  cent_graph_upload(cent_ctx* ctx, const struct CSR* csr, cent_graph** out);
 int  cent_graph_from_csr(cent_ctx* ctx, const int32_t* row_ptr, const int32_t* col,
                          int32_t n, int64_t nnz, cent_graph** out);
@@ -176,6 +187,52 @@ int  cent_bc(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k,
 int  cent_cc(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k,
              int64_t* farness, int64_t* dist_sum, const cent_opts* opts);
 
+/* ------------------------------------------------ reduced (D1 + AP) form */
+/*
+ * Closeness through degree-1 folding and articulation-point splitting, the
+ * reduction pipeline of compute_D1_AP_CC_shareBased_DegreeOrder
+ * (Seq_CC/CC.cpp:1728-2052: D1Folding, AP_detection, AP_Copy_And_Split,
+ * rebuildGraph, then per-component traversal and D1 recovery).  The host
+ * folds leaves, splits the folded graph into biconnected blocks (linear
+ * time) and packs every block as its own id range with a copy of each
+ * articulation vertex; the GPU computes weighted BFS distance sums for every
+ * packed vertex, and the host turns them into the farness of every input
+ * vertex.
+ */
+typedef struct cent_reduced cent_reduced;
+
+typedef struct cent_reduced_view {
+    int32_t n;             /* input vertices                                   */
+    int32_t np;            /* packed vertices (block copies)                   */
+    int64_t nnzp;          /* packed directed edges                            */
+    int32_t nblocks;
+    int32_t nitems;        /* (block range, <= 64 sources) work items          */
+    int32_t nfold;         /* folded degree-1 vertices                         */
+    int32_t n_ap;          /* articulation vertices of the folded graph        */
+    int32_t ncomp;         /* connected components of the input                */
+    const int32_t* prp;    /* [np + 1] packed CSR                              */
+    const int32_t* pcol;   /* [nnzp]                                           */
+    const int64_t* pw;     /* [np] copy weights                                */
+    const int32_t* porig;  /* [np] input vertex of each copy                   */
+    const int32_t* boff;   /* [nblocks + 1] packed id range of each block      */
+} cent_reduced_view;
+
+int  cent_reduce(const struct CSR* csr, cent_reduced** out);
+int  cent_reduced_get_view(const cent_reduced* r, cent_reduced_view* out);
+/* farness[v] (assigned, length n) from D[x] = sum_y d(x, y) * pw[y] over each
+ * packed vertex x's block. */
+int  cent_reduced_farness(const cent_reduced* r, const int64_t* D, int64_t* farness);
+void cent_reduced_free(cent_reduced* r);
+
+/* Schedules for cent_cc_reduced. */
+#define CENT_SCHED_PERSISTENT 0  /* one persistent launch over the work-item queue */
+#define CENT_SCHED_PER_ITEM   1  /* level-synchronous launches per work item       */
+
+/* farness (length n, assigned) of every input vertex through the reduced
+ * form; equals cent_cc over all sources on every graph. */
+int  cent_cc_reduced(cent_ctx* ctx, const cent_reduced* r, int64_t* farness, int32_t sched,
+                     const cent_opts* opts);
+
 /* ------------------------------------------- reference-shaped entry points */
 /* Drop-in shapes of the reference functions over the own struct CSR, running
  * on a process-wide default context (device 0).  Ids startNodeID..endNodeID
@@ -183,6 +240,7 @@ int  cent_cc(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k,
 int  computeBC(const struct CSR* csr, double* BCs);                      /* Brandes.c:224 */
 int  computeBC_compat(const struct CSR* csr, double* BCs);               /* Brandes.c:317-327 formula */
 int  computeCC(const struct CSR* csr, int64_t* CCs);                     /* CC.cpp:37 */
+int  compute_D1_AP_CC_shareBased_DegreeOrder(const struct CSR* csr, int64_t* CCs); /* CC.cpp:1728 */
 
 #ifdef __cplusplus
 }

@@ -233,3 +233,43 @@ int oracle_max_threads(void) {
     return 1;
 #endif
 }
+
+/* Weighted BFS distance sums: far[i] = sum over reached v of d(src[i], v) * w[v].
+ * Checker for the packed block graph of the reduced closeness path. */
+int oracle_weighted_farness(int32_t n, const int32_t* rp, const int32_t* col, const int64_t* w,
+                            const int32_t* src, int32_t k, int64_t* far) {
+    int err = 0;
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+    {
+        int32_t* dist = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+        int32_t* queue = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+        if (!dist || !queue) {
+#ifdef _OPENMP
+#pragma omp atomic write
+#endif
+            err = 1;
+        } else {
+            for (int32_t v = 0; v < n; ++v) dist[v] = -1;
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 16)
+#endif
+            for (int32_t i = 0; i < k; ++i) {
+                int64_t h = 0, t = 0, f = 0;
+                queue[t++] = src[i]; dist[src[i]] = 0;
+                while (h < t) {
+                    int32_t x = queue[h++];
+                    for (int32_t j = rp[x]; j < rp[x + 1]; ++j) {
+                        int32_t y = col[j];
+                        if (dist[y] < 0) { dist[y] = dist[x] + 1; queue[t++] = y; f += (int64_t)dist[y] * w[y]; }
+                    }
+                }
+                far[i] = f;
+                for (int64_t q = 0; q < t; ++q) dist[queue[q]] = -1;
+            }
+        }
+        free(dist); free(queue);
+    }
+    return err ? 2 : 0;
+}

@@ -141,3 +141,16 @@ def test_large_rmat_properties(ctx):
     bc2, far2, ds2 = ctx.bc(dg, sub, dist_sum=True)
     assert_bc_close(bc2, obc, "rmat20 subset")
     assert np.array_equal(far2, ofar) and np.array_equal(ds2, ods)
+
+
+def test_device_order_does_not_change_results(ctx):
+    """Degree-relabelled device layout (default) vs the caller's order."""
+    g = bc_amd.gen_rmat(15, 16 << 15, seed=4)
+    src = bc_amd.sample_sources(g.n, 100, 4)
+    a = ctx.bc(ctx.upload(g), src, dist_sum=True)
+    b = ctx.bc(ctx.upload(g, keep_order=True), src, dist_sum=True)
+    assert_bc_close(a[0], b[0])
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+    obc, ofar, ods, _ = O.brandes(g, src)
+    assert_bc_close(a[0], obc)
+    assert np.array_equal(a[1], ofar) and np.array_equal(a[2], ods)

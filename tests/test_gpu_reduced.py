@@ -1,0 +1,65 @@
+"""GPU parity of the reduced (degree-1 folding + articulation split) closeness
+path against plain BFS farness — the invariance that pins it: both schedules
+(persistent work queue, per-item launches) must give the plain farness of
+every vertex, bit-exact in int64."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import _oracle as O
+import bc_amd
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+SCHEDS = (bc_amd.CENT_SCHED_PERSISTENT, bc_amd.CENT_SCHED_PER_ITEM)
+
+
+def plain_farness(g):
+    return O.weighted_farness(g.n, g.row_ptr, g.col, np.ones(g.n, np.int64))
+
+
+def check(ctx, g, what):
+    print(f"[gpu] reduced {what}: n={g.n} m={g.m}", flush=True)
+    red = bc_amd.Reduced(g)
+    want = plain_farness(g)
+    for sched in SCHEDS:
+        got = ctx.cc_reduced(red, sched)
+        bad = np.nonzero(got != want)[0]
+        assert len(bad) == 0, (what, sched, bad[:10], got[bad[:5]], want[bad[:5]])
+    return red
+
+
+@pytest.mark.parametrize("name", ["karate.txt", "dolphins.txt", "football.txt", "polbooks.txt",
+                                  "testdata.txt", "testdata1.txt", "testdata2.txt",
+                                  "ia-fb-messages.mtx"])
+def test_reduced_fixtures(ctx, name):
+    check(ctx, bc_amd.buildGraph(O.dataset_path(name)), name)
+
+
+def test_reference_shim_and_known_answers(ctx):
+    g = bc_amd.buildGraph(O.dataset_path("karate.txt"))
+    CCs = np.zeros(g.n, np.int64)
+    bc_amd.compute_D1_AP_CC_shareBased_DegreeOrder(g, CCs)
+    assert CCs[7] == 86
+    ref = json.load(open(os.path.join(GOLDEN, "ref_golden.json")))["karate.txt"]
+    assert CCs[1:35].tolist() == ref["d1ap"]
+    g = bc_amd.buildGraph(O.dataset_path("musae_git.txt"))
+    red = check(ctx, g, "musae_git")
+    far = ctx.cc_reduced(red)
+    assert far[7] == 117726
+
+
+def test_reduced_block_trees(ctx):
+    # many small blocks (persistent queue) + pendant chains
+    check(ctx, bc_amd.gen_blocktree(300, 1.5, 8, 2000, 6.0, 0.3, 0.5, seed=3), "blocktree-300")
+    # blocks larger than the LDS item range (level-synchronous path)
+    check(ctx, bc_amd.gen_blocktree(6, 1.5, 2500, 6000, 6.0, 0.3, 0.5, seed=8), "blocktree-big")
+
+
+def test_reduced_power_law_and_grid(ctx):
+    # giant block with hub rows + many components, trees and pendants
+    check(ctx, bc_amd.gen_rmat(13, 8 << 13, seed=12, lcc=False), "rmat13-all-components")
+    check(ctx, bc_amd.gen_grid(40, 40, 0.1, 4, seed=4), "grid40")

@@ -1,0 +1,91 @@
+"""Host reduction (degree-1 folding + biconnected-block split + block-cut tree
+constants) checked on the CPU: the packed weighted distance sums D are
+computed by the oracle, cent_reduced_farness turns them into farness, and
+the result must equal plain BFS farness exactly (int64) on every graph —
+connected or not, trees, cycles, block trees, power-law graphs."""
+import itertools
+
+import numpy as np
+import pytest
+
+import _oracle as O
+import bc_amd
+
+
+def reduced_farness_cpu(g):
+    r = bc_amd.Reduced(g)
+    D = O.weighted_farness(r.np, r.prp, r.pcol, r.pw)
+    return r, r.farness_from(D)
+
+
+def check(g):
+    r, far = reduced_farness_cpu(g)
+    want = O.farness(g)
+    assert np.array_equal(far, want), np.nonzero(far != want)[0][:10]
+    # every block's weights add up to its component size
+    for b in range(r.nblocks):
+        lo, hi = r.boff[b], r.boff[b + 1]
+        assert hi > lo
+    return r
+
+
+@pytest.mark.parametrize("name", ["karate.txt", "dolphins.txt", "football.txt", "polbooks.txt",
+                                  "testdata.txt", "testdata1.txt", "testdata2.txt",
+                                  "ia-fb-messages.mtx"])
+def test_fixtures(name):
+    check(bc_amd.buildGraph(O.dataset_path(name)))
+
+
+def test_reference_known_answers_through_reduction():
+    g = bc_amd.buildGraph(O.dataset_path("karate.txt"))
+    _, far = reduced_farness_cpu(g)
+    assert far[7] == 86
+    g = bc_amd.buildGraph(O.dataset_path("musae_git.txt"))
+    r, far = reduced_farness_cpu(g)
+    assert far[7] == 117726
+    assert r.nfold > 0 and r.n_ap > 0
+
+
+def path(n):
+    return bc_amd.CSR.from_edges(n, np.arange(n - 1), np.arange(1, n))
+
+
+def test_shapes():
+    check(path(2))
+    check(path(17))                                       # tree: folds to one vertex
+    star = bc_amd.CSR.from_edges(9, np.zeros(8), np.arange(1, 9))
+    check(star)
+    cyc = bc_amd.CSR.from_edges(10, np.arange(10), (np.arange(10) + 1) % 10)
+    r = check(cyc)
+    assert r.nfold == 0 and r.nblocks == 1
+    # two triangles joined by a path (bridges), with pendant chains
+    u = [0, 1, 2, 2, 3, 4, 5, 6, 6, 4, 7, 8]
+    v = [1, 2, 0, 3, 4, 5, 6, 7, 5, 9, 9, 10]
+    check(bc_amd.CSR.from_edges(11, u, v))
+    # bowtie: AP shared by two cycles + isolated vertex + isolated edge
+    u = [0, 1, 2, 2, 3, 4, 6]
+    v = [1, 2, 0, 3, 4, 2, 7]
+    check(bc_amd.CSR.from_edges(9, u, v))
+    u, v = zip(*itertools.combinations(range(6), 2))
+    check(bc_amd.CSR.from_edges(6, u, v))
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_sparse_graphs(seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(5, 80))
+    m = int(rng.integers(n // 2, 2 * n))
+    u = rng.integers(0, n, m)
+    v = rng.integers(0, n, m)
+    check(bc_amd.CSR.from_edges(n, u, v))
+
+
+def test_block_tree_and_rmat():
+    t = bc_amd.gen_blocktree(60, 1.5, 8, 400, 6.0, 0.3, 0.5, seed=3)
+    r = check(t)
+    assert r.n_ap > 10 and r.nfold > 0
+    g = bc_amd.gen_rmat(11, 8 << 11, seed=7, lcc=False)
+    r = check(g)
+    assert r.ncomp > 1
+    grid = bc_amd.gen_grid(20, 20, 0.1, 5, seed=4)
+    check(grid)
