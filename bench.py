@@ -40,6 +40,96 @@ def cpu_model() -> str:
     return platform.processor()
 
 
+def log(msg: str) -> None:
+    print(f"# {msg}", file=sys.stderr, flush=True)
+
+
+def timed(fn, reps: int):
+    fn()
+    ts = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t)
+    return sum(ts) / len(ts)
+
+
+def extra_blocktree(ctx, args):
+    """Block-tree workload: 2000 blocks (Pareto(1.5) sizes in [8, 50000], ER avg
+    degree 6 + Hamiltonian cycle), random tree of shared articulation vertices,
+    +30% pendant chains (Geometric(0.5) lengths), seed 3; all-source closeness
+    through D1 folding + block split.  Compares one persistent launch over the
+    work-item queue with level-synchronous launches per work item."""
+    import bc_amd
+    t0 = time.time()
+    g = bc_amd.gen_blocktree(2000, 1.5, 8, 50000, 6.0, 0.3, 0.5, seed=3)
+    red = bc_amd.Reduced(g)
+    log(f"blocktree: n={g.n} m={g.m} blocks={red.nblocks} aps={red.n_ap} folded={red.nfold} "
+        f"packed={red.np} items={red.nitems} (built in {time.time() - t0:.1f}s)")
+    res = {}
+    far_p = ctx.cc_reduced(red, bc_amd.CENT_SCHED_PERSISTENT)
+    far_i = ctx.cc_reduced(red, bc_amd.CENT_SCHED_PER_ITEM)
+    assert (far_p == far_i).all()
+    tp = timed(lambda: ctx.cc_reduced(red, bc_amd.CENT_SCHED_PERSISTENT), 5)
+    ti = timed(lambda: ctx.cc_reduced(red, bc_amd.CENT_SCHED_PER_ITEM), 2)
+    dg = ctx.upload(g)
+    tplain = timed(lambda: ctx.cc(dg, None), 1)
+    log(f"blocktree: persistent {tp * 1e3:.2f} ms, per-item {ti * 1e3:.1f} ms, "
+        f"plain all-source MS-BFS {tplain * 1e3:.1f} ms")
+    n, m = g.n, g.m
+    res.update({
+        "workload": "all-source closeness, block tree 2000 blocks (D1 + AP reduced), seed 3",
+        "n": n, "m": m, "blocks": red.nblocks, "articulation": red.n_ap, "folded": red.nfold,
+        "packed_vertices": red.np, "items": red.nitems,
+        "persistent_ms": round(tp * 1e3, 3), "per_item_ms": round(ti * 1e3, 3),
+        "plain_unreduced_ms": round(tplain * 1e3, 3),
+        "queries_per_s_persistent": round(1.0 / tp, 2), "queries_per_s_per_item": round(1.0 / ti, 2),
+        "speedup_vs_per_item": round(ti / tp, 2),
+        "effective_TEPS_persistent": round(n * m / tp, 1),
+    })
+    return res
+
+
+def extra_blocktree_bc(ctx, args):
+    """All-source Brandes BC + closeness on the block-tree graph (unreduced)."""
+    import bc_amd
+    g = bc_amd.gen_blocktree(2000, 1.5, 8, 50000, 6.0, 0.3, 0.5, seed=3)
+    dg = ctx.upload(g)
+    ctx.reset_stats()
+    t = timed(lambda: ctx.bc(dg, None), 1)
+    st = ctx.stats()
+    log(f"blocktree-bc: {t * 1e3:.1f} ms, levels/batch {st['levels'] / max(1, st['batches']):.1f}")
+    return {"workload": "all-source BC + closeness, block tree (unreduced)", "n": g.n, "m": g.m,
+            "ms": round(t * 1e3, 2), "TEPS": round(g.n * g.m / t, 1),
+            "levels_per_batch": round(st["levels"] / max(1, st["batches"]), 2)}
+
+
+def extra_grid(ctx, args):
+    """2048x2048 4-neighbour grid, 5% edges removed, largest component, +1000
+    shortcuts, seed 4; 1024 sampled sources, BC + closeness."""
+    import bc_amd
+    t0 = time.time()
+    g = bc_amd.gen_grid(2048, 2048, 0.05, 1000, seed=4)
+    src = bc_amd.sample_sources(g.n, 1024, 4)
+    log(f"grid: n={g.n} m={g.m} (built in {time.time() - t0:.1f}s)")
+    dg = ctx.upload(g)
+    ctx.reset_stats()
+    t1 = time.perf_counter()
+    ctx.bc(dg, src, timing=True)
+    t = time.perf_counter() - t1
+    st = ctx.stats()
+    log(f"grid: {t * 1e3:.1f} ms, levels/batch {st['levels'] / max(1, st['batches']):.1f}, "
+        f"fwd {st['ms_fwd']:.1f} ms bwd {st['ms_bwd']:.1f} ms")
+    return {"workload": "sampled BC + closeness, 2048^2 grid (5% removed, LCC, +1000 shortcuts), "
+                        "1024 sources", "n": g.n, "m": g.m, "ms": round(t * 1e3, 2),
+            "TEPS": round(1024 * g.m / t, 1),
+            "levels_per_batch": round(st["levels"] / max(1, st["batches"]), 2),
+            "fwd_ms": round(st["ms_fwd"], 2), "bwd_ms": round(st["ms_bwd"], 2)}
+
+
+EXTRA = {"blocktree": extra_blocktree, "blocktree-bc": extra_blocktree_bc, "grid": extra_grid}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -55,6 +145,11 @@ def main() -> int:
                     help="keep the generator's vertex ids on the device (no degree relabel)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also time a plain 1 GiB device copy (HBM ceiling, counter calibration)")
+    ap.add_argument("--extra", default="",
+                    help="comma list of extra workloads reported under 'workloads': "
+                         "blocktree (reduced closeness, persistent vs per-item launches), "
+                         "blocktree-bc (plain BC+CC on the block tree), grid (sampled BC+CC on "
+                         "the 2048^2 grid)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -132,6 +227,10 @@ def main() -> int:
                "sample": f"first {ks} of the {k} sources (same graph), {tc:.2f} s, "
                          f"CPU {cpu_model()}"}
 
+    workloads = {}
+    for name in [x for x in args.extra.split(",") if x]:
+        workloads[name] = EXTRA[name](ctx, args)
+
     out = {
         "metric": "TEPS (sources×edges/s) BC+CC, batched/served vs plain; p50/p99 query latency",
         "value": round(teps, 1), "unit": "TEPS", "n_gpus": 1, "steps": K, "warmup": args.warmup,
@@ -144,6 +243,8 @@ def main() -> int:
                    "dag_edges_per_step": int(dag), "parallelism": "single GPU"},
         "roofline": roofline, "cpu_baseline": cpu,
     }
+    if workloads:
+        out["workloads"] = workloads
     print(json.dumps(out), flush=True)
     return 0
 
