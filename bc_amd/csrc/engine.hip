@@ -21,6 +21,7 @@
 #include "../Lib/reduce.h"
 #include "items_kernels.h"
 #include "msbfs_kernels.h"
+#include "sparse_kernels.h"
 
 using namespace cent;
 
@@ -57,6 +58,9 @@ struct cent_ctx {
     DevBuf<int64_t> dist_sum, far, part_far;
     DevBuf<unsigned long long> part_cnt, cnt;
     DevBuf<int> src, flags;
+    // frontier-queue mode
+    DevBuf<uint64_t> Fd2, QF, Adev;
+    DevBuf<int> Q, qoff, cand, stamp, ctl;
     size_t F_levels = 0;     // levels the F buffer holds for the current N
     size_t F_n = 0;
     int* h_flags = nullptr;  // pinned
@@ -233,7 +237,20 @@ struct cent_graph {
     std::vector<int32_t> comp;        // component id per vertex (host, caller ids)
     std::vector<int64_t> comp_nnz;    // directed edges per component
     std::vector<int32_t> new_id;      // device id of each caller id (empty: identity)
+    int32_t ecc_probe = 0;            // BFS depth from the highest-degree vertex
 };
+
+// Frontier mode for a plain call: the frontier-queue kernels win when BFS runs
+// many levels with small frontiers.  The probe is the BFS depth from the
+// highest-degree vertex, measured once at upload.
+constexpr int kSparseDepth = 48;
+
+static bool use_sparse(const cent_graph* g, const cent_opts* o) {
+    const int m = o ? o->frontier : CENT_FRONTIER_AUTO;
+    if (m == CENT_FRONTIER_DENSE) return false;
+    if (m == CENT_FRONTIER_SPARSE) return true;
+    return g->ecc_probe > kSparseDepth;
+}
 
 extern "C" int cent_ctx_create(int32_t device, cent_ctx** out) {
     if (!out) return CENT_E_INVAL;
@@ -267,6 +284,8 @@ extern "C" void cent_ctx_destroy(cent_ctx* ctx) {
     release(ctx->bc); release(ctx->hp_acc); release(ctx->npred); release(ctx->hp_cnt);
     release(ctx->dist_sum); release(ctx->far); release(ctx->part_far); release(ctx->part_cnt);
     release(ctx->cnt); release(ctx->src); release(ctx->flags);
+    release(ctx->Fd2); release(ctx->QF); release(ctx->Adev); release(ctx->Q); release(ctx->qoff);
+    release(ctx->cand); release(ctx->stamp); release(ctx->ctl);
     if (ctx->h_flags) (void)hipHostFree(ctx->h_flags);
     for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
@@ -364,6 +383,22 @@ extern "C" int cent_graph_create(cent_ctx* ctx, const int32_t* row_ptr, const in
         }
         urp = drp.data();
         ucol = dcol.data();
+    }
+    // eccentricity probe from the highest-degree vertex (host BFS, O(n + m))
+    if (n > 0) {
+        int32_t s0 = 0;
+        for (int32_t v = 1; v < n; ++v)
+            if (row_ptr[v + 1] - row_ptr[v] > row_ptr[s0 + 1] - row_ptr[s0]) s0 = v;
+        std::vector<int32_t> dist((size_t)n, -1);
+        size_t h = 0, t = 0;
+        q[t++] = s0;
+        dist[(size_t)s0] = 0;
+        while (h < t) {
+            const int32_t x = q[h++];
+            for (int32_t j = row_ptr[x]; j < row_ptr[x + 1]; ++j)
+                if (dist[(size_t)col[j]] < 0) { dist[(size_t)col[j]] = dist[(size_t)x] + 1; q[t++] = col[j]; }
+        }
+        g->ecc_probe = dist[(size_t)q[t - 1]];
     }
     int st = devcsr_build(ctx, urp, ucol, n, nnz, nullptr, 0, g->d);
     if (st) {
@@ -594,6 +629,132 @@ static int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const
     return CENT_OK;
 }
 
+
+// ------------------------------------------------------------ frontier-queue mode
+constexpr int kSpReplay = 16;  // levels per hipGraph replay
+
+struct SpPlan {
+    SpArgs a{};
+    int G = 1;                 // blocks of the list kernels
+    hipGraph_t gf = nullptr, gb = nullptr;
+    hipGraphExec_t xf = nullptr, xb = nullptr;
+    ~SpPlan() {
+        if (xf) (void)hipGraphExecDestroy(xf);
+        if (xb) (void)hipGraphExecDestroy(xb);
+        if (gf) (void)hipGraphDestroy(gf);
+        if (gb) (void)hipGraphDestroy(gb);
+    }
+};
+
+static int sp_prepare(cent_ctx* ctx, const DevCSR& d, const BatchSpec& sp, SpPlan& pl) {
+    const size_t N = (size_t)(d.n > 0 ? d.n : 1);
+    pl.G = std::max(1, std::min((int)((N + kBlock - 1) / kBlock), 1024));
+    const bool fresh_stamp = ctx->stamp.cap < N;
+    TRY(ensure(ctx, ctx->vis, N));
+    TRY(ensure(ctx, ctx->Fd2, 2 * N));
+    const size_t qcap = std::min<size_t>(N * kWave, (size_t)INT32_MAX);
+    TRY(ensure(ctx, ctx->Q, qcap));
+    TRY(ensure(ctx, ctx->QF, qcap));
+    TRY(ensure(ctx, ctx->qoff, N + 3));
+    TRY(ensure(ctx, ctx->cand, N));
+    TRY(ensure(ctx, ctx->stamp, N));
+    TRY(ensure(ctx, ctx->ctl, kCtlSlots));
+    TRY(ensure(ctx, ctx->Adev, 1));
+    TRY(ensure(ctx, ctx->part_far, (size_t)pl.G * kWave));
+    TRY(ensure(ctx, ctx->part_cnt, (size_t)pl.G * 2));
+    TRY(ensure(ctx, ctx->cnt, 2));
+    if (sp.with_bc) {
+        TRY(ensure(ctx, ctx->sigma, N * kWave));
+        TRY(ensure(ctx, ctx->coef, N * kWave));
+        if (sp.compat) TRY(ensure(ctx, ctx->npred, N * kWave));
+    }
+    hipStream_t s = ctx->stream;
+    // all-zero vis / masks between batches (batches clear what they touched)
+    HIPTRY(hipMemsetAsync(ctx->vis.p, 0, sizeof(uint64_t) * N, s));
+    HIPTRY(hipMemsetAsync(ctx->Fd2.p, 0, sizeof(uint64_t) * 2 * N, s));
+    if (fresh_stamp) HIPTRY(hipMemsetAsync(ctx->stamp.p, 0xff, sizeof(int) * ctx->stamp.cap, s));
+    HIPTRY(hipMemsetAsync(ctx->ctl.p, 0, sizeof(int) * kCtlSlots, s));
+    SpArgs& a = pl.a;
+    a.n = d.n; a.rp = d.rp; a.col = d.col; a.vis = ctx->vis.p; a.Fd2 = ctx->Fd2.p;
+    a.sigma = sp.with_bc ? ctx->sigma.p : nullptr; a.coef = sp.with_bc ? ctx->coef.p : nullptr;
+    a.npred = sp.compat ? ctx->npred.p : nullptr; a.bc = sp.bc; a.dist_sum = sp.dist_sum;
+    a.weight = sp.weight; a.Q = ctx->Q.p; a.QF = ctx->QF.p; a.qoff = ctx->qoff.p;
+    a.qcap = (int)qcap; a.qoff_cap = (int)(N + 3); a.cand = ctx->cand.p; a.stamp = ctx->stamp.p;
+    a.ctl = ctx->ctl.p; a.Aptr = ctx->Adev.p; a.part_far = ctx->part_far.p; a.part_cnt = ctx->part_cnt.p;
+    // capture kSpReplay levels of the forward and of the backward once per call
+    const int G = pl.G;
+    HIPTRY(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+    for (int l = 0; l < kSpReplay; ++l) {
+        k_sp_expand<<<G, kBlock, 0, s>>>(a);
+        if (sp.with_bc) {
+            if (sp.compat) k_sp_pull<true, true><<<G, kBlock, 0, s>>>(a);
+            else k_sp_pull<true, false><<<G, kBlock, 0, s>>>(a);
+        } else {
+            k_sp_pull<false, false><<<G, kBlock, 0, s>>>(a);
+        }
+        k_sp_advance<<<1, 1, 0, s>>>(a);
+    }
+    HIPTRY(hipStreamEndCapture(s, &pl.gf));
+    HIPTRY(hipGraphInstantiate(&pl.xf, pl.gf, nullptr, nullptr, 0));
+    if (sp.with_bc) {
+        HIPTRY(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+        for (int l = 0; l < kSpReplay; ++l) {
+            if (sp.compat) k_sp_bwd<true><<<G, kBlock, 0, s>>>(a);
+            else k_sp_bwd<false><<<G, kBlock, 0, s>>>(a);
+            k_sp_bwd_next<<<G, kBlock, 0, s>>>(a);
+            k_sp_bwd_step<<<1, 1, 0, s>>>(a);
+        }
+        HIPTRY(hipStreamEndCapture(s, &pl.gb));
+        HIPTRY(hipGraphInstantiate(&pl.xb, pl.gb, nullptr, nullptr, 0));
+    }
+    return CENT_OK;
+}
+
+// One batch of kb <= 64 sources in frontier-queue mode (whole graph).
+static int level_batch_sparse(cent_ctx* ctx, SpPlan& pl, const BatchSpec& sp, const int* d_src, int kb,
+                              int64_t* far_out, KernelTimer& tm, int64_t* levels) {
+    hipStream_t s = ctx->stream;
+    const int G = pl.G;
+    const uint64_t A = (kb == 64) ? ~0ull : ((1ull << kb) - 1);
+    HIPTRY(hipMemsetAsync(ctx->part_far.p, 0, sizeof(int64_t) * (size_t)G * kWave, s));
+    HIPTRY(hipMemsetAsync(ctx->part_cnt.p, 0, sizeof(unsigned long long) * (size_t)G * 2, s));
+    k_sp_seed<<<1, kWave, 0, s>>>(pl.a, d_src, kb, sp.with_bc, A);
+    HIPTRY(hipGetLastError());
+    int deepest = 0;
+    for (;;) {
+        hipEvent_t e0;
+        TRY(timer_begin(ctx, tm, &e0));
+        HIPTRY(hipGraphLaunch(pl.xf, s));
+        TRY(timer_end(ctx, tm, e0, true));
+        ctx->stats.launches_fwd += kSpReplay;
+        HIPTRY(hipMemcpyAsync(ctx->h_flags, ctx->ctl.p + kCtlDone, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+        HIPTRY(hipStreamSynchronize(s));
+        if (ctx->h_flags[0]) break;
+    }
+    int ctl_h[kCtlSlots];
+    HIPTRY(hipMemcpyAsync(ctl_h, ctx->ctl.p, sizeof(ctl_h), hipMemcpyDeviceToHost, s));
+    HIPTRY(hipStreamSynchronize(s));
+    if (ctl_h[kCtlOverflow]) return CENT_E_OVERFLOW;
+    deepest = ctl_h[kCtlDeepest];
+    *levels += deepest;
+    if (sp.with_bc && deepest >= 1) {
+        k_sp_bwd_init<<<1, 1, 0, s>>>(pl.a);
+        const int replays = (deepest + kSpReplay - 1) / kSpReplay;
+        for (int r = 0; r < replays; ++r) {
+            hipEvent_t e0;
+            TRY(timer_begin(ctx, tm, &e0));
+            HIPTRY(hipGraphLaunch(pl.xb, s));
+            TRY(timer_end(ctx, tm, e0, false));
+            ctx->stats.launches_bwd += kSpReplay;
+        }
+    }
+    k_sp_clear<<<G, kBlock, 0, s>>>(pl.a);
+    k_reduce_batch<<<1, 1024, 0, s>>>(ctx->part_far.p, ctx->part_cnt.p, G, kb, far_out, ctx->cnt.p);
+    HIPTRY(hipGetLastError());
+    ctx->stats.batches++;
+    return CENT_OK;
+}
+
 // Runs all batches of one plain call.  with_bc: Brandes (sigma + backward);
 // otherwise closeness only (bitmask BFS).
 static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k, bool with_bc,
@@ -634,9 +795,13 @@ static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
     ctx->ev_used = 0;
     const GView view = full_view(g->d);
     int64_t levels = 0;
+    const bool sparse = use_sparse(g, opts) && k > 0;
+    SpPlan plan;
+    if (sparse) TRY(sp_prepare(ctx, g->d, sp, plan));
     for (int32_t b0 = 0; b0 < k; b0 += kWave) {
         const int kb = (k - b0) < kWave ? (k - b0) : kWave;
-        TRY(level_batch(ctx, view, sp, ctx->src.p + b0, kb, ctx->far.p + b0, tm, &levels));
+        if (sparse) TRY(level_batch_sparse(ctx, plan, sp, ctx->src.p + b0, kb, ctx->far.p + b0, tm, &levels));
+        else TRY(level_batch(ctx, view, sp, ctx->src.p + b0, kb, ctx->far.p + b0, tm, &levels));
         if (tm.on && ctx->ev_used > 4096) {  // keep the event pool bounded
             HIPTRY(hipStreamSynchronize(s));
             timer_collect(ctx, tm);

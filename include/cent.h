@@ -121,11 +121,21 @@ typedef struct cent_graph cent_graph;  /* device-resident CSR                  *
 /* opts.flags */
 #define CENT_OPT_TIMING    1   /* record per-kernel HIP events (stats)         */
 
+/* opts.frontier: how a level finds its work.  DENSE scans every vertex
+ * (pull; best for low-diameter graphs), SPARSE keeps compacted frontier
+ * lists and replays hipGraph-captured runs of levels (best for hundreds of
+ * levels with small frontiers), AUTO picks from the graph's BFS-depth probe. */
+#define CENT_FRONTIER_AUTO   0
+#define CENT_FRONTIER_DENSE  1
+#define CENT_FRONTIER_SPARSE 2
+
 typedef struct cent_opts {
     int32_t bc_mode;      /* CENT_BC_TEXTBOOK or CENT_BC_REF_COMPAT          */
     int32_t flags;        /* CENT_OPT_*                                      */
-    int32_t block_size;   /* 0 = tuned default                               */
-    int32_t grid_waves;   /* 0 = tuned default (waves per level launch)     */
+    int32_t block_size;   /* reserved, 0                                     */
+    int32_t grid_waves;   /* 0 = default cap on waves per dense level launch */
+    int32_t frontier;     /* CENT_FRONTIER_*                                 */
+    int32_t reserved;
 } cent_opts;
 
 typedef struct cent_stats {

@@ -154,3 +154,31 @@ def test_device_order_does_not_change_results(ctx):
     obc, ofar, ods, _ = O.brandes(g, src)
     assert_bc_close(a[0], obc)
     assert np.array_equal(a[1], ofar) and np.array_equal(a[2], ods)
+
+
+@pytest.mark.parametrize("frontier", ["dense", "sparse"])
+def test_frontier_modes_agree(ctx, frontier):
+    """Dense pull levels and frontier-queue (hipGraph-replayed) levels give the
+    oracle's results on a power-law graph, a high-diameter grid, a disconnected
+    graph with duplicate sources, and in the reference-compat formula."""
+    cases = [("rmat13", bc_amd.gen_rmat(13, 16 << 13, seed=3), 150),
+             ("grid", bc_amd.gen_grid(64, 96, 0.05, 3, seed=4), 130),
+             ("path", bc_amd.CSR.from_edges(300, np.arange(299), np.arange(1, 300)), 70)]
+    for name, g, k in cases:
+        print(f"[gpu] frontier={frontier} {name} n={g.n}", flush=True)
+        dg = ctx.upload(g)
+        src = bc_amd.sample_sources(g.n, min(k, g.n), 7)
+        src = np.concatenate([src, src[:3]])  # duplicate sources in the last batch
+        for mode in (bc_amd.CENT_BC_TEXTBOOK, bc_amd.CENT_BC_REF_COMPAT):
+            bc, far, ds = ctx.bc(dg, src, mode=mode, dist_sum=True, frontier=frontier)
+            obc, ofar, ods, _ = O.brandes(g, src, mode=mode, threads=min(16, O.max_threads()))
+            assert_bc_close(bc, obc, f"{name} {frontier} mode {mode}")
+            assert np.array_equal(far, ofar) and np.array_equal(ds, ods)
+        cfar, cds = ctx.cc(dg, src, dist_sum=True, frontier=frontier)
+        assert np.array_equal(cfar, ofar) and np.array_equal(cds, ods)
+    g = bc_amd.CSR.from_edges(9, [0, 1, 3, 4, 4, 6], [1, 2, 4, 5, 6, 7])
+    src = np.array([8, 0, 0, 4, 7, 2], np.int32)
+    bc, far, ds = ctx.bc(ctx.upload(g), src, dist_sum=True, frontier=frontier)
+    obc, ofar, ods, _ = O.brandes(g, src)
+    assert_bc_close(bc, obc)
+    assert np.array_equal(far, ofar) and np.array_equal(ds, ods)
