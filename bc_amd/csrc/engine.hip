@@ -649,7 +649,6 @@ struct SpPlan {
 static int sp_prepare(cent_ctx* ctx, const DevCSR& d, const BatchSpec& sp, SpPlan& pl) {
     const size_t N = (size_t)(d.n > 0 ? d.n : 1);
     pl.G = std::max(1, std::min((int)((N + kBlock - 1) / kBlock), 1024));
-    const bool fresh_stamp = ctx->stamp.cap < N;
     TRY(ensure(ctx, ctx->vis, N));
     TRY(ensure(ctx, ctx->Fd2, 2 * N));
     const size_t qcap = std::min<size_t>(N * kWave, (size_t)INT32_MAX);
@@ -672,7 +671,8 @@ static int sp_prepare(cent_ctx* ctx, const DevCSR& d, const BatchSpec& sp, SpPla
     // all-zero vis / masks between batches (batches clear what they touched)
     HIPTRY(hipMemsetAsync(ctx->vis.p, 0, sizeof(uint64_t) * N, s));
     HIPTRY(hipMemsetAsync(ctx->Fd2.p, 0, sizeof(uint64_t) * 2 * N, s));
-    if (fresh_stamp) HIPTRY(hipMemsetAsync(ctx->stamp.p, 0xff, sizeof(int) * ctx->stamp.cap, s));
+    // stamps restart with the call's level counter (ctl), so reset them too
+    HIPTRY(hipMemsetAsync(ctx->stamp.p, 0xff, sizeof(int) * N, s));
     HIPTRY(hipMemsetAsync(ctx->ctl.p, 0, sizeof(int) * kCtlSlots, s));
     SpArgs& a = pl.a;
     a.n = d.n; a.rp = d.rp; a.col = d.col; a.vis = ctx->vis.p; a.Fd2 = ctx->Fd2.p;
