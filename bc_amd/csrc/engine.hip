@@ -238,18 +238,20 @@ struct cent_graph {
     std::vector<int64_t> comp_nnz;    // directed edges per component
     std::vector<int32_t> new_id;      // device id of each caller id (empty: identity)
     int32_t ecc_probe = 0;            // BFS depth from the highest-degree vertex
+    int32_t width_probe = 0;          // widest level of that BFS
 };
 
 // Frontier mode for a plain call: the frontier-queue kernels win when BFS runs
-// many levels with small frontiers.  The probe is the BFS depth from the
-// highest-degree vertex, measured once at upload.
+// many levels and the 64 lanes' frontiers together stay a small part of the
+// graph.  The probe is one BFS from the highest-degree vertex at upload:
+// its depth and its widest level.
 constexpr int kSparseDepth = 48;
 
 static bool use_sparse(const cent_graph* g, const cent_opts* o) {
     const int m = o ? o->frontier : CENT_FRONTIER_AUTO;
     if (m == CENT_FRONTIER_DENSE) return false;
     if (m == CENT_FRONTIER_SPARSE) return true;
-    return g->ecc_probe > kSparseDepth;
+    return g->ecc_probe > kSparseDepth && (int64_t)g->width_probe * kWave * 4 < (int64_t)g->n;
 }
 
 extern "C" int cent_ctx_create(int32_t device, cent_ctx** out) {
@@ -399,6 +401,9 @@ extern "C" int cent_graph_create(cent_ctx* ctx, const int32_t* row_ptr, const in
                 if (dist[(size_t)col[j]] < 0) { dist[(size_t)col[j]] = dist[(size_t)x] + 1; q[t++] = col[j]; }
         }
         g->ecc_probe = dist[(size_t)q[t - 1]];
+        std::vector<int32_t> width((size_t)g->ecc_probe + 1, 0);
+        for (size_t i = 0; i < t; ++i) width[(size_t)dist[(size_t)q[i]]]++;
+        g->width_probe = *std::max_element(width.begin(), width.end());
     }
     int st = devcsr_build(ctx, urp, ucol, n, nnz, nullptr, 0, g->d);
     if (st) {
