@@ -208,6 +208,17 @@ def main() -> int:
 
     if args.copy_probe:
         roofline["copy_probe_GBs"] = round(ctx.probe_copy_bandwidth(1 << 30, 10), 1)
+    # HBM-side bytes per launch come from the rocprofv3 counter passes of this
+    # same workload (profiles/rmat20_traffic.json, FETCH_SIZE / WRITE_SIZE
+    # scaled by the copy-kernel calibration); they are not measured live.
+    tpath = os.path.join(ROOT, "profiles", "rmat20_traffic.json")
+    if (args.scale, args.edge_factor, args.sources) == (20, 16, 4096) and os.path.exists(tpath):
+        tr = json.load(open(tpath))
+        key = "bwd_pass" if kname.startswith("k_bwd") else "fwd_pass"
+        roofline["traffic"] = tr[key]["traffic_bytes_per_launch"]
+        roofline["traffic_unit"] = "bytes per launch of the level pass"
+        roofline["algorithmic_bytes_per_launch"] = int(b / launches)
+        roofline["traffic_source"] = tr["source"]
 
     cpu = None
     if not args.no_cpu_baseline:
