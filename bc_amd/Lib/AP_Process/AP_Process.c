@@ -343,3 +343,10 @@ int cent_reduced_get_view(const cent_reduced* r, cent_reduced_view* v) {
 int cent_reduced_farness(const cent_reduced* r, const int64_t* D, int64_t* farness) {
     return cent_reduced_finish(r, D, farness);
 }
+
+void cent_reduced_drop_device(cent_reduced* r) {
+    if (!r) return;
+    if (r->dev && r->dev_free) r->dev_free(r->dev);
+    r->dev = NULL;
+    r->dev_free = NULL;
+}

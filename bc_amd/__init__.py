@@ -419,6 +419,115 @@ class Context:
         return out[:red.n]
 
 
+# ---------------------------------------------------------------- serving
+CENT_SERVER_DRYRUN = 1
+CENT_Q_BC, CENT_Q_CC = 0, 1
+REQ_STATE = {0: "queued", 1: "running", 2: "done", 3: "failed", 4: "cancelled"}
+
+
+class _ServerOpts(C.Structure):
+    _fields_ = [("device", C.c_int32), ("slots", C.c_int32), ("hbm_budget", C.c_int64),
+                ("max_queue", C.c_int32), ("flags", C.c_int32)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("state", C.c_int32), ("status", C.c_int32), ("slot", C.c_int32),
+                ("reserved", C.c_int32), ("t_submit_ms", C.c_double), ("t_start_ms", C.c_double),
+                ("t_end_ms", C.c_double), ("footprint", C.c_int64)]
+
+    def as_dict(self) -> dict:
+        d = {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+        d["state"] = REQ_STATE.get(self.state, self.state)
+        return d
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(f, C.c_int64) for f in (
+        "submitted", "completed", "failed", "cancelled", "rejected", "queued", "running",
+        "max_queued", "max_running", "graph_uploads", "graph_hits", "graph_evictions",
+        "used_workspace", "used_graph")]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_lib.cent_server_create.argtypes = [C.POINTER(_ServerOpts), C.POINTER(C.c_void_p)]
+_lib.cent_server_destroy.argtypes = [C.c_void_p]
+_lib.cent_server_destroy.restype = None
+_lib.cent_server_add_graph.argtypes = [C.c_void_p, _P_CSR, C.POINTER(C.c_int32)]
+_lib.cent_server_submit.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _I32P, C.c_int32,
+                                    C.POINTER(C.c_int64)]
+_lib.cent_server_wait.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.POINTER(RequestInfo)]
+_lib.cent_server_result.argtypes = [C.c_void_p, C.c_int64, _F64P, _I64P, C.POINTER(RequestInfo)]
+_lib.cent_server_cancel.argtypes = [C.c_void_p, C.c_int64]
+_lib.cent_server_get_stats.argtypes = [C.c_void_p, C.POINTER(ServerStats)]
+
+
+class Server:
+    """Continuous-batching query server (see cent_server_* in include/cent.h)."""
+
+    def __init__(self, slots: int = 4, hbm_budget: int = 64 << 30, max_queue: int = 100000,
+                 device: int = 0, dryrun: bool = False):
+        o = _ServerOpts(device, slots, hbm_budget, max_queue, CENT_SERVER_DRYRUN if dryrun else 0)
+        h = C.c_void_p()
+        _check(_lib.cent_server_create(C.byref(o), C.byref(h)), "cent_server_create")
+        self._h = h
+        self._graphs = []
+        self._meta = {}
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.cent_server_destroy(h)
+            self._h = None
+
+    __del__ = close
+
+    def add_graph(self, csr: CSR) -> int:
+        gid = C.c_int32()
+        _check(_lib.cent_server_add_graph(self._h, csr._p, C.byref(gid)), "cent_server_add_graph")
+        self._graphs.append(csr.n)
+        return gid.value
+
+    def submit(self, gid: int, kind: int, sources=None) -> int:
+        rid = C.c_int64()
+        if sources is None:
+            st = _lib.cent_server_submit(self._h, gid, kind, None, -1, C.byref(rid))
+            k = -1
+        else:
+            src = np.ascontiguousarray(sources, dtype=np.int32)
+            st = _lib.cent_server_submit(self._h, gid, kind, _ptr(src, C.c_int32) if len(src) else None,
+                                         len(src), C.byref(rid))
+            k = len(src)
+        _check(st, "cent_server_submit")
+        self._meta[rid.value] = (gid, kind, k)
+        return rid.value
+
+    def wait(self, rid: int, timeout_ms: int = -1) -> dict:
+        info = RequestInfo()
+        _check(_lib.cent_server_wait(self._h, rid, timeout_ms, C.byref(info)), "cent_server_wait")
+        return info.as_dict()
+
+    def result(self, rid: int):
+        """(status, bc or None, farness, info) — forgets the request."""
+        gid, kind, k = self._meta.pop(rid)
+        n = self._graphs[gid]
+        bc = np.zeros(max(n, 1)) if kind == CENT_Q_BC else None
+        far = np.zeros(max(n if k == -1 else k, 1), np.int64)
+        info = RequestInfo()
+        st = _lib.cent_server_result(self._h, rid, _ptr(bc, C.c_double) if bc is not None else None,
+                                     _ptr(far, C.c_int64), C.byref(info))
+        return st, (bc[:n] if bc is not None else None), far[:(n if k == -1 else k)], info.as_dict()
+
+    def cancel(self, rid: int) -> None:
+        _check(_lib.cent_server_cancel(self._h, rid), "cent_server_cancel")
+
+    def stats(self) -> dict:
+        s = ServerStats()
+        _check(_lib.cent_server_get_stats(self._h, C.byref(s)), "cent_server_get_stats")
+        return s.as_dict()
+
+
 # ------------------------------------------- reference-shaped entry points
 def compute_D1_AP_CC_shareBased_DegreeOrder(csr: CSR, CCs: np.ndarray) -> np.ndarray:
     """Farness of every vertex via degree-1 folding + articulation-point split,
@@ -453,6 +562,6 @@ def computeCC(csr: CSR, CCs: np.ndarray) -> np.ndarray:
 
 __all__ = [
     "CSR", "CentError", "Context", "DeviceGraph", "buildGraph", "createCSR", "computeBC",
-    "computeCC", "compute_D1_AP_CC_shareBased_DegreeOrder", "Reduced", "gen_rmat", "gen_grid", "gen_blocktree", "sample_sources", "LIB_PATH",
+    "computeCC", "compute_D1_AP_CC_shareBased_DegreeOrder", "Reduced", "Server", "gen_rmat", "gen_grid", "gen_blocktree", "sample_sources", "LIB_PATH",
     "CENT_BC_TEXTBOOK", "CENT_BC_REF_COMPAT",
 ]

@@ -61,6 +61,9 @@ struct cent_ctx {
     // frontier-queue mode
     DevBuf<uint64_t> Fd2, QF, Adev;
     DevBuf<int> Q, qoff, cand, stamp, ctl;
+    // reduced closeness per-run buffers
+    DevBuf<int64_t> red_D;
+    DevBuf<int> red_head;
     size_t F_levels = 0;     // levels the F buffer holds for the current N
     size_t F_n = 0;
     int* h_flags = nullptr;  // pinned
@@ -288,6 +291,7 @@ extern "C" void cent_ctx_destroy(cent_ctx* ctx) {
     release(ctx->cnt); release(ctx->src); release(ctx->flags);
     release(ctx->Fd2); release(ctx->QF); release(ctx->Adev); release(ctx->Q); release(ctx->qoff);
     release(ctx->cand); release(ctx->stamp); release(ctx->ctl);
+    release(ctx->red_D); release(ctx->red_head);
     if (ctx->h_flags) (void)hipHostFree(ctx->h_flags);
     for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
@@ -303,6 +307,32 @@ extern "C" int cent_ctx_stats(const cent_ctx* ctx, cent_stats* out) {
 extern "C" int cent_ctx_reset_stats(cent_ctx* ctx) {
     if (!ctx) return CENT_E_INVAL;
     ctx->stats = cent_stats{};
+    return CENT_OK;
+}
+
+extern "C" int64_t cent_ctx_workspace_bytes(const cent_ctx* ctx) {
+    if (!ctx) return 0;
+    return (int64_t)(8 * (ctx->vis.cap + ctx->F.cap + ctx->hp_or.cap + ctx->sigma.cap + ctx->coef.cap +
+                          ctx->bc.cap + ctx->hp_acc.cap + ctx->dist_sum.cap + ctx->far.cap + ctx->part_far.cap +
+                          ctx->part_cnt.cap + ctx->cnt.cap + ctx->Fd2.cap + ctx->QF.cap + ctx->Adev.cap +
+                          ctx->red_D.cap) +
+                     4 * (ctx->npred.cap + ctx->hp_cnt.cap + ctx->src.cap + ctx->flags.cap + ctx->Q.cap +
+                          ctx->qoff.cap + ctx->cand.cap + ctx->stamp.cap + ctx->ctl.cap + ctx->red_head.cap));
+}
+
+extern "C" int cent_ctx_trim(cent_ctx* ctx) {
+    if (!ctx) return CENT_E_INVAL;
+    TRY(check_device(ctx));
+    HIPTRY(hipStreamSynchronize(ctx->stream));
+    release(ctx->vis); release(ctx->F); release(ctx->hp_or); release(ctx->sigma); release(ctx->coef);
+    release(ctx->bc); release(ctx->hp_acc); release(ctx->npred); release(ctx->hp_cnt);
+    release(ctx->dist_sum); release(ctx->far); release(ctx->part_far); release(ctx->part_cnt);
+    release(ctx->cnt); release(ctx->src); release(ctx->flags);
+    release(ctx->Fd2); release(ctx->QF); release(ctx->Adev); release(ctx->Q); release(ctx->qoff);
+    release(ctx->cand); release(ctx->stamp); release(ctx->ctl);
+    release(ctx->red_D); release(ctx->red_head);
+    ctx->F_levels = 0;
+    ctx->F_n = 0;
     return CENT_OK;
 }
 
@@ -860,7 +890,7 @@ static int prepare_sources(const cent_graph* g, const int32_t* src, int32_t k, s
 
 extern "C" int cent_bc(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k, double* bc,
                        int64_t* farness, int64_t* dist_sum, const cent_opts* opts) {
-    if (!ctx || !g || g->ctx != ctx) return CENT_E_INVAL;
+    if (!ctx || !g || g->ctx->device != ctx->device) return CENT_E_INVAL;
     int mode = opts ? opts->bc_mode : CENT_BC_TEXTBOOK;
     if (mode != CENT_BC_TEXTBOOK && mode != CENT_BC_REF_COMPAT) return CENT_E_INVAL;
     std::vector<int32_t> all;
@@ -876,7 +906,7 @@ extern "C" int cent_bc(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
 
 extern "C" int cent_cc(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k, int64_t* farness,
                        int64_t* dist_sum, const cent_opts* opts) {
-    if (!ctx || !g || g->ctx != ctx) return CENT_E_INVAL;
+    if (!ctx || !g || g->ctx->device != ctx->device) return CENT_E_INVAL;
     std::vector<int32_t> all;
     const int32_t* s;
     int32_t kk;
@@ -898,8 +928,6 @@ struct RedDev {
     int* iota = nullptr;      // [np] packed ids 0..np-1 (batch sources)
     int4* items = nullptr;    // [nitems]
     int* order = nullptr;     // [nsmall] small items, largest range first
-    int* head = nullptr;      // queue counter
-    int64_t* D = nullptr;     // [np]
     int nsmall = 0;
     std::vector<int> big;     // items handled by the level-synchronous kernels
 };
@@ -913,15 +941,17 @@ static void reddev_free(void* p) {
     if (rd->iota) (void)hipFree(rd->iota);
     if (rd->items) (void)hipFree(rd->items);
     if (rd->order) (void)hipFree(rd->order);
-    if (rd->head) (void)hipFree(rd->head);
-    if (rd->D) (void)hipFree(rd->D);
+
     delete rd;
 }
 
+static std::mutex g_reddev_mu;  // first-use creation of a reduced graph's device copy
+
 static int reddev_get(cent_ctx* ctx, cent_reduced* r, RedDev** out) {
+    std::lock_guard<std::mutex> lk(g_reddev_mu);
     if (r->dev) {
         RedDev* rd = (RedDev*)r->dev;
-        if (rd->ctx != ctx) return CENT_E_INVAL;  // one context per reduced graph
+        if (rd->ctx->device != ctx->device) return CENT_E_INVAL;
         *out = rd;
         return CENT_OK;
     }
@@ -950,9 +980,6 @@ static int reddev_get(cent_ctx* ctx, cent_reduced* r, RedDev** out) {
     rd->nsmall = (int)small.size();
     TRY(upload(ctx, &rd->items, items.data(), (size_t)r->nitems));
     TRY(upload(ctx, &rd->order, small.data(), small.size()));
-    int zero = 0;
-    TRY(upload(ctx, &rd->head, &zero, 1));
-    HIPTRY(hipMalloc((void**)&rd->D, sizeof(int64_t) * NP));
     *out = rd;
     return CENT_OK;
 }
@@ -968,7 +995,11 @@ extern "C" int cent_cc_reduced(cent_ctx* ctx, const cent_reduced* rc, int64_t* f
     TRY(reddev_get(ctx, r, &rd));
     hipStream_t s = ctx->stream;
     const size_t NP = (size_t)(r->np > 0 ? r->np : 1);
-    HIPTRY(hipMemsetAsync(rd->D, 0, sizeof(int64_t) * NP, s));
+    // per-run buffers live in the context, so several contexts can share rd
+    TRY(ensure(ctx, ctx->red_D, NP));
+    TRY(ensure(ctx, ctx->red_head, 1));
+    int64_t* Dd = ctx->red_D.p;
+    HIPTRY(hipMemsetAsync(Dd, 0, sizeof(int64_t) * NP, s));
     TRY(ensure(ctx, ctx->cnt, 2));
     HIPTRY(hipMemsetAsync(ctx->cnt.p, 0, sizeof(unsigned long long) * 2, s));
     KernelTimer tm;
@@ -981,16 +1012,16 @@ extern "C" int cent_cc_reduced(cent_ctx* ctx, const cent_reduced* rc, int64_t* f
     auto run_item = [&](int i) -> int {
         const cent_item& it = r->items[i];
         const GView v = sub_view(rd->d, it.lo, it.hi);
-        return level_batch(ctx, v, sp, rd->iota + it.s0, it.ns, rd->D + it.s0, tm, &levels);
+        return level_batch(ctx, v, sp, rd->iota + it.s0, it.ns, Dd + it.s0, tm, &levels);
     };
     if (sched == CENT_SCHED_PER_ITEM) {
         for (int i = 0; i < r->nitems; ++i) TRY(run_item(i));
     } else {
         if (rd->nsmall) {
-            HIPTRY(hipMemsetAsync(rd->head, 0, sizeof(int), s));
+            HIPTRY(hipMemsetAsync(ctx->red_head.p, 0, sizeof(int), s));
             ItemArgs ia;
             ia.rp = rd->d.rp; ia.col = rd->d.col; ia.weight = rd->w; ia.items = rd->items; ia.order = rd->order;
-            ia.nitems = rd->nsmall; ia.head = rd->head; ia.D = rd->D;
+            ia.nitems = rd->nsmall; ia.head = ctx->red_head.p; ia.D = Dd;
             const int grid = std::min(rd->nsmall, 2048);
             hipEvent_t e0;
             TRY(timer_begin(ctx, tm, &e0));
@@ -1003,7 +1034,7 @@ extern "C" int cent_cc_reduced(cent_ctx* ctx, const cent_reduced* rc, int64_t* f
         for (int i : rd->big) TRY(run_item(i));
     }
     std::vector<int64_t> D(NP);
-    HIPTRY(hipMemcpyAsync(D.data(), rd->D, sizeof(int64_t) * NP, hipMemcpyDeviceToHost, s));
+    HIPTRY(hipMemcpyAsync(D.data(), Dd, sizeof(int64_t) * NP, hipMemcpyDeviceToHost, s));
     HIPTRY(hipStreamSynchronize(s));
     timer_collect(ctx, tm);
     TRY(cent_reduced_finish(r, D.data(), farness));

@@ -1,0 +1,406 @@
+// Centrality query server: continuous batching of BC / closeness requests on
+// one GPU.  (The reference has no serving path; this is the request loop
+// around the engine's C-ABI.)
+//
+// - Requests (graph, kind, source list) queue FIFO up to max_queue.
+// - `slots` worker threads, each with its own engine context and HIP stream,
+//   take the head request as soon as a slot is free AND its estimated device
+//   footprint (workspace + the graph's device copy if not cached) fits in the
+//   HBM budget next to everything already admitted.  Requests therefore run
+//   concurrently on the device, and a new one is admitted the moment a slot
+//   and memory free up, whatever the others are doing.
+// - Graphs are registered once (host copy) and uploaded on first use; device
+//   copies stay cached and are evicted least-recently-used, when no running
+//   request holds them, if the budget needs the room.
+// - A request whose footprint alone exceeds the budget fails with CENT_E_OOM
+//   instead of blocking the queue.
+// - Results are per request and do not depend on what else runs: each
+//   request's kernels run in order on its slot's stream, with its own
+//   workspace, so its sums are accumulated in the same order alone or mixed.
+//
+// The DRYRUN flag replaces the engine calls by a sleep of the modelled
+// duration (no device is touched, no results are produced); it exists to
+// test admission, ordering and accounting on machines without a GPU.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/cent.h"
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+struct GraphEntry {
+    struct CSR* host = nullptr;    // owned copy
+    int64_t n = 0, nnz = 0;
+    cent_graph* dev = nullptr;     // cached device CSR
+    cent_reduced* red = nullptr;   // host reduction (closeness queries)
+    int64_t dev_bytes = 0;         // estimated device bytes of dev + red
+    int refs = 0;                  // running requests using the device copy
+    uint64_t last_use = 0;
+    std::mutex build_mu;           // serialises upload / reduction
+};
+
+struct Request {
+    int64_t id = 0;
+    int32_t gid = 0, kind = 0;
+    std::vector<int32_t> src;
+    bool all = false;
+    int32_t state = CENT_REQ_QUEUED;
+    int32_t status = CENT_OK;
+    bool cancel = false;
+    double t_submit = 0, t_start = 0, t_end = 0;
+    int64_t footprint = 0;     // admitted workspace bytes
+    int64_t graph_bytes = 0;   // graph bytes added by this request's upload (0 if cached)
+    int32_t slot = -1;
+    std::vector<double> bc;
+    std::vector<int64_t> far;
+};
+
+}  // namespace
+
+struct cent_server {
+    cent_server_opts o{};
+    std::vector<cent_ctx*> ctx;
+    std::vector<std::thread> workers;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request*> queue;
+    std::unordered_map<int64_t, Request*> reqs;
+    std::vector<GraphEntry*> graphs;
+    int64_t used_ws = 0, used_graph = 0;
+    int running = 0;
+    bool stop = false;
+    int64_t next_id = 1;
+    uint64_t tick = 0;
+    clk::time_point t0 = clk::now();
+    cent_server_stats st{};
+};
+
+static double now_ms(const cent_server* s) {
+    return std::chrono::duration<double, std::milli>(clk::now() - s->t0).count();
+}
+
+// Device bytes a request needs besides the graph (the context workspace the
+// engine sizes for it: per vertex 2 x 512 B of path counts / coefficients,
+// frontier masks, partials).
+static int64_t workspace_estimate(const GraphEntry* g, int32_t kind, bool all, int64_t k) {
+    const int64_t n = std::max<int64_t>(g->n, 1);
+    if (kind == CENT_Q_BC) return n * (1024 + 8 + 17 * 8 + 8 + 64) + k * 16 + (64 << 20);
+    if (all && g->red) {
+        cent_reduced_view v;
+        cent_reduced_get_view(g->red, &v);
+        return (int64_t)v.np * (8 + 8 + 3 * 8 + 64) + (16 << 20);
+    }
+    return n * (8 + 3 * 8 + 8 + 64) + k * 16 + (16 << 20);
+}
+
+static int64_t graph_estimate(const GraphEntry* g) {
+    return 4 * (g->n + 1) + 4 * g->nnz + g->n / 4 + g->nnz / 16 + (1 << 20);
+}
+
+static int64_t reduced_estimate(const cent_reduced* r) {
+    cent_reduced_view v;
+    cent_reduced_get_view(r, &v);
+    return 4 * ((int64_t)v.np + 1) + 4 * v.nnzp + 12 * (int64_t)v.np + 16 * (int64_t)v.nitems + (1 << 20);
+}
+
+// Drops least-recently-used unreferenced device copies until `need` more
+// bytes fit.  Called with s->mu held.  Returns true if they fit.
+static bool make_room(cent_server* s, int64_t need, const GraphEntry* keep) {
+    while (s->used_ws + s->used_graph + need > s->o.hbm_budget) {
+        GraphEntry* victim = nullptr;
+        for (GraphEntry* g : s->graphs)
+            if (g != keep && g->dev_bytes > 0 && g->refs == 0 && (!victim || g->last_use < victim->last_use))
+                victim = g;
+        if (!victim) return false;
+        if (victim->dev) cent_graph_free(victim->dev);
+        if (victim->red) cent_reduced_drop_device(victim->red);
+        victim->dev = nullptr;
+        s->used_graph -= victim->dev_bytes;
+        victim->dev_bytes = 0;
+        s->st.graph_evictions++;
+    }
+    return true;
+}
+
+static void run_request(cent_server* s, int slot, Request* r, GraphEntry* g) {
+    int st = CENT_OK;
+    const bool dry = (s->o.flags & CENT_SERVER_DRYRUN) != 0;
+    cent_ctx* ctx = dry ? nullptr : s->ctx[(size_t)slot];
+    {
+        std::lock_guard<std::mutex> lk(g->build_mu);
+        if (!dry && !g->dev) st = cent_graph_upload(ctx, g->host, &g->dev);
+        if (!st && r->kind == CENT_Q_CC && r->all && !g->red) st = cent_reduce(g->host, &g->red);
+    }
+    const int64_t k = r->all ? g->n : (int64_t)r->src.size();
+    if (!st) {
+        if (dry) {
+            // modelled duration: sources x directed edges at 1e9 edge visits/s
+            const double us = (double)k * (double)std::max<int64_t>(g->nnz, 1) / 1e3;
+            std::this_thread::sleep_for(std::chrono::microseconds((int64_t)std::min(us, 2e6)));
+        } else if (r->kind == CENT_Q_BC) {
+            r->bc.assign((size_t)g->n, 0.0);
+            r->far.assign((size_t)std::max<int64_t>(k, 1), 0);
+            st = cent_bc(ctx, g->dev, r->all ? nullptr : r->src.data(), r->all ? -1 : (int32_t)k, r->bc.data(),
+                         r->far.data(), nullptr, nullptr);
+        } else if (r->all) {
+            r->far.assign((size_t)std::max<int64_t>(g->n, 1), 0);
+            st = cent_cc_reduced(ctx, g->red, r->far.data(), CENT_SCHED_PERSISTENT, nullptr);
+        } else {
+            r->far.assign((size_t)std::max<int64_t>(k, 1), 0);
+            st = cent_cc(ctx, g->dev, r->src.data(), (int32_t)k, r->far.data(), nullptr, nullptr);
+        }
+    }
+    // keep a slot's cached workspace within its share of the budget
+    if (!dry && cent_ctx_workspace_bytes(ctx) > s->o.hbm_budget / std::max(1, s->o.slots) / 2) cent_ctx_trim(ctx);
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (!dry && g->red && r->kind == CENT_Q_CC && r->all) {
+        // the reduced device copy now exists; account for it once
+        const int64_t rb = reduced_estimate(g->red);
+        if (g->dev_bytes < graph_estimate(g) + rb) {
+            s->used_graph += graph_estimate(g) + rb - g->dev_bytes;
+            g->dev_bytes = graph_estimate(g) + rb;
+        }
+    }
+    r->status = st;
+    r->t_end = now_ms(s);
+    r->state = r->cancel ? CENT_REQ_CANCELLED : (st ? CENT_REQ_FAILED : CENT_REQ_DONE);
+    if (r->state == CENT_REQ_DONE) s->st.completed++;
+    else if (r->state == CENT_REQ_FAILED) s->st.failed++;
+    else s->st.cancelled++;
+    s->used_ws -= r->footprint;
+    g->refs--;
+    g->last_use = ++s->tick;
+    s->running--;
+    s->cv_done.notify_all();
+    s->cv_work.notify_all();
+}
+
+static void worker(cent_server* s, int slot) {
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        Request* r = nullptr;
+        GraphEntry* g = nullptr;
+        while (!s->stop) {
+            if (!s->queue.empty()) {
+                Request* h = s->queue.front();
+                GraphEntry* hg = s->graphs[(size_t)h->gid];
+                const int64_t ws = workspace_estimate(hg, h->kind, h->all, h->all ? hg->n : (int64_t)h->src.size());
+                const int64_t gb = hg->dev_bytes > 0 ? 0 : graph_estimate(hg);
+                if (ws + graph_estimate(hg) > s->o.hbm_budget) {
+                    // can never fit: fail it instead of blocking the queue
+                    s->queue.pop_front();
+                    h->state = CENT_REQ_FAILED;
+                    h->status = CENT_E_OOM;
+                    h->t_start = h->t_end = now_ms(s);
+                    s->st.failed++;
+                    s->cv_done.notify_all();
+                    continue;
+                }
+                if (make_room(s, ws + gb, hg)) {
+                    s->queue.pop_front();
+                    r = h;
+                    g = hg;
+                    r->footprint = ws;
+                    r->graph_bytes = gb;
+                    s->used_ws += ws;
+                    if (gb) {
+                        s->used_graph += gb;
+                        g->dev_bytes = gb;
+                        s->st.graph_uploads++;
+                    } else {
+                        s->st.graph_hits++;
+                    }
+                    g->refs++;
+                    g->last_use = ++s->tick;
+                    r->state = CENT_REQ_RUNNING;
+                    r->slot = slot;
+                    r->t_start = now_ms(s);
+                    s->running++;
+                    if (s->running > s->st.max_running) s->st.max_running = s->running;
+                    break;
+                }
+            }
+            s->cv_work.wait(lk);
+        }
+        if (!r) return;  // stopping
+        lk.unlock();
+        run_request(s, slot, r, g);
+        lk.lock();
+    }
+}
+
+extern "C" int cent_server_create(const cent_server_opts* opts, cent_server** out) {
+    if (!opts || !out || opts->slots < 1 || opts->slots > 64 || opts->hbm_budget <= 0 || opts->max_queue < 1)
+        return CENT_E_INVAL;
+    *out = nullptr;
+    cent_server* s = new (std::nothrow) cent_server();
+    if (!s) return CENT_E_OOM;
+    s->o = *opts;
+    if (!(opts->flags & CENT_SERVER_DRYRUN)) {
+        for (int i = 0; i < opts->slots; ++i) {
+            cent_ctx* c = nullptr;
+            int st = cent_ctx_create(opts->device, &c);
+            if (st) {
+                for (cent_ctx* x : s->ctx) cent_ctx_destroy(x);
+                delete s;
+                return st;
+            }
+            s->ctx.push_back(c);
+        }
+    }
+    for (int i = 0; i < opts->slots; ++i) s->workers.emplace_back(worker, s, i);
+    *out = s;
+    return CENT_OK;
+}
+
+extern "C" void cent_server_destroy(cent_server* s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    for (auto& t : s->workers) t.join();
+    for (auto& kv : s->reqs) delete kv.second;
+    for (GraphEntry* g : s->graphs) {
+        if (g->dev) cent_graph_free(g->dev);
+        if (g->red) cent_reduced_free(g->red);
+        cent_csr_free(g->host);
+        delete g;
+    }
+    for (cent_ctx* c : s->ctx) cent_ctx_destroy(c);
+    delete s;
+}
+
+extern "C" int cent_server_add_graph(cent_server* s, const struct CSR* csr, int32_t* gid) {
+    if (!s || !csr || !gid) return CENT_E_INVAL;
+    struct CSR* copy = nullptr;
+    int st = cent_csr_from_arrays(csr->csrVSize - 1, csr->csrV, csr->csrE, &copy);
+    if (st) return st;
+    copy->startNodeID = csr->startNodeID;
+    copy->endNodeID = csr->endNodeID;
+    copy->totalNodeNumber = csr->totalNodeNumber;
+    copy->startAtZero = csr->startAtZero;
+    GraphEntry* g = new (std::nothrow) GraphEntry();
+    if (!g) { cent_csr_free(copy); return CENT_E_OOM; }
+    g->host = copy;
+    g->n = copy->csrVSize - 1;
+    g->nnz = copy->csrESize;
+    std::lock_guard<std::mutex> lk(s->mu);
+    *gid = (int32_t)s->graphs.size();
+    s->graphs.push_back(g);
+    return CENT_OK;
+}
+
+extern "C" int cent_server_submit(cent_server* s, int32_t gid, int32_t kind, const int32_t* src, int32_t k,
+                                  int64_t* rid) {
+    if (!s || !rid || (kind != CENT_Q_BC && kind != CENT_Q_CC) || k < -1 || (k > 0 && !src)) return CENT_E_INVAL;
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (gid < 0 || (size_t)gid >= s->graphs.size()) return CENT_E_NOTFOUND;
+    GraphEntry* g = s->graphs[(size_t)gid];
+    for (int32_t i = 0; i < k; ++i)
+        if (src[i] < 0 || src[i] >= g->n) return CENT_E_INVAL;
+    if ((int)s->queue.size() >= s->o.max_queue) {
+        s->st.rejected++;
+        return CENT_E_BUSY;
+    }
+    Request* r = new (std::nothrow) Request();
+    if (!r) return CENT_E_OOM;
+    r->id = s->next_id++;
+    r->gid = gid;
+    r->kind = kind;
+    r->all = (k == -1);
+    if (k > 0) r->src.assign(src, src + k);
+    r->t_submit = now_ms(s);
+    s->reqs[r->id] = r;
+    s->queue.push_back(r);
+    s->st.submitted++;
+    if ((int64_t)s->queue.size() > s->st.max_queued) s->st.max_queued = (int64_t)s->queue.size();
+    *rid = r->id;
+    lk.unlock();
+    s->cv_work.notify_one();
+    return CENT_OK;
+}
+
+static void fill_info(const Request* r, cent_request_info* info) {
+    info->state = r->state;
+    info->status = r->status;
+    info->slot = r->slot;
+    info->t_submit_ms = r->t_submit;
+    info->t_start_ms = r->t_start;
+    info->t_end_ms = r->t_end;
+    info->footprint = r->footprint;
+}
+
+extern "C" int cent_server_wait(cent_server* s, int64_t rid, int32_t timeout_ms, cent_request_info* info) {
+    if (!s) return CENT_E_INVAL;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(rid);
+    if (it == s->reqs.end()) return CENT_E_NOTFOUND;
+    Request* r = it->second;
+    auto done = [&] { return r->state >= CENT_REQ_DONE; };
+    if (timeout_ms < 0) s->cv_done.wait(lk, done);
+    else s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), done);
+    if (info) fill_info(r, info);
+    return CENT_OK;
+}
+
+extern "C" int cent_server_result(cent_server* s, int64_t rid, double* bc, int64_t* farness,
+                                  cent_request_info* info) {
+    if (!s) return CENT_E_INVAL;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(rid);
+    if (it == s->reqs.end()) return CENT_E_NOTFOUND;
+    Request* r = it->second;
+    if (r->state < CENT_REQ_DONE) return CENT_E_BUSY;
+    if (info) fill_info(r, info);
+    const int st = (r->state == CENT_REQ_CANCELLED) ? CENT_E_CANCELLED : r->status;
+    if (r->state == CENT_REQ_DONE) {
+        if (bc && !r->bc.empty()) std::memcpy(bc, r->bc.data(), sizeof(double) * r->bc.size());
+        const GraphEntry* g = s->graphs[(size_t)r->gid];
+        const size_t nf = r->all ? (size_t)g->n : r->src.size();
+        if (farness && !r->far.empty()) std::memcpy(farness, r->far.data(), sizeof(int64_t) * nf);
+    }
+    s->reqs.erase(it);
+    delete r;
+    return st;
+}
+
+extern "C" int cent_server_cancel(cent_server* s, int64_t rid) {
+    if (!s) return CENT_E_INVAL;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(rid);
+    if (it == s->reqs.end()) return CENT_E_NOTFOUND;
+    Request* r = it->second;
+    if (r->state == CENT_REQ_QUEUED) {
+        s->queue.erase(std::find(s->queue.begin(), s->queue.end(), r));
+        r->state = CENT_REQ_CANCELLED;
+        r->t_start = r->t_end = now_ms(s);
+        s->st.cancelled++;
+        s->cv_done.notify_all();
+    } else if (r->state == CENT_REQ_RUNNING) {
+        r->cancel = true;  // its kernels finish; the result is discarded
+    }
+    return CENT_OK;
+}
+
+extern "C" int cent_server_get_stats(cent_server* s, cent_server_stats* out) {
+    if (!s || !out) return CENT_E_INVAL;
+    std::lock_guard<std::mutex> lk(s->mu);
+    *out = s->st;
+    out->queued = (int64_t)s->queue.size();
+    out->running = s->running;
+    out->used_workspace = s->used_ws;
+    out->used_graph = s->used_graph;
+    return CENT_OK;
+}

@@ -127,7 +127,111 @@ def extra_grid(ctx, args):
             "fwd_ms": round(st["ms_fwd"], 2), "bwd_ms": round(st["ms_bwd"], 2)}
 
 
-EXTRA = {"blocktree": extra_blocktree, "blocktree-bc": extra_blocktree_bc, "grid": extra_grid}
+def serve_pool(seed: int, size: int):
+    """Seeded graph pool: vertex counts log-uniform in [1e3, 1e6]; even entries
+    R-MAT (avg degree 16, LCC), odd entries block trees (about 31 vertices per
+    block with the 30% pendant chains)."""
+    import numpy as np
+    import bc_amd
+    rng = np.random.default_rng(seed)
+    pool = []
+    for i in range(size):
+        n = float(np.exp(rng.uniform(np.log(1e3), np.log(1e6))))
+        if i % 2 == 0:
+            scale = max(10, int(round(np.log2(n))))
+            g = bc_amd.gen_rmat(scale, 8 << scale, seed=seed * 1000 + i)
+            pool.append(("rmat", g))
+        else:
+            nb = max(8, int(n / 31))
+            g = bc_amd.gen_blocktree(nb, 1.5, 8, 50000, 6.0, 0.3, 0.5, seed=seed * 1000 + i)
+            pool.append(("blocktree", g))
+    return pool
+
+
+def run_stream(srv, gids, pool, reqs, rate, deadline_s):
+    """Open-loop Poisson submission; returns per-request (kind, latency_ms) and
+    the wall time from the first submission to the last completion."""
+    import numpy as np
+    import bc_amd
+    rng = np.random.default_rng(55)
+    gaps = rng.exponential(1.0 / rate, len(reqs))
+    t0 = time.perf_counter()
+    t_next = t0
+    rids = []
+    for (j, kind, src), gap in zip(reqs, gaps):
+        t_next += gap
+        d = t_next - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        rids.append((srv.submit(gids[j], kind, src), kind))
+    lat, done_t = [], []
+    end = t0 + deadline_s
+    for rid, kind in rids:
+        left = int(max(0.0, end - time.perf_counter()) * 1000)
+        info = srv.wait(rid, left)
+        if info["state"] not in ("done", "failed", "cancelled"):
+            srv.cancel(rid)
+            info = srv.wait(rid)
+        st, _, _, info = srv.result(rid)
+        if info["state"] == "done":
+            lat.append((kind, info["t_end_ms"] - info["t_submit_ms"]))
+            done_t.append(info["t_end_ms"])
+    return lat, time.perf_counter() - t0
+
+
+def extra_serve(ctx, args):
+    """Serving workload: Poisson arrivals at 200 requests/s; each request picks a
+    graph from the pool; half sampled BC (k log-uniform in [64, 1024]), half
+    all-vertex closeness (through the D1/AP-reduced path); seed 5.  Reports
+    p50/p99 latency and completed queries/s for the continuous-batching server
+    (4 slots) and for a single-slot server (one request at a time)."""
+    import numpy as np
+    import bc_amd
+    pool = serve_pool(5, args.serve_pool)
+    rng = np.random.default_rng(5)
+    reqs = []
+    for i in range(args.serve_requests):
+        j = int(rng.integers(0, len(pool)))
+        g = pool[j][1]
+        if rng.random() < 0.5:
+            k = int(min(g.n, np.exp(rng.uniform(np.log(64), np.log(1024)))))
+            reqs.append((j, bc_amd.CENT_Q_BC, bc_amd.sample_sources(g.n, k, 1000 + i)))
+        else:
+            reqs.append((j, bc_amd.CENT_Q_CC, None))
+    log(f"serve: pool {[(kind, g.n) for kind, g in pool]}, {len(reqs)} requests")
+    res = {"workload": f"Poisson {args.serve_rate:g} req/s, {len(reqs)} requests over a pool of "
+                       f"{len(pool)} graphs (n log-uniform in [1e3, 1e6], half R-MAT deg 16, half "
+                       "block tree), 50% sampled BC k in [64,1024], 50% all-vertex closeness",
+           "pool_n": [g.n for _, g in pool]}
+    for label, slots in (("served", 4), ("one_at_a_time", 1)):
+        srv = bc_amd.Server(slots=slots, hbm_budget=200 << 30)
+        gids = [srv.add_graph(g) for _, g in pool]
+        # warm the graph cache and reductions
+        for gid in gids:
+            srv.wait(srv.submit(gid, bc_amd.CENT_Q_BC, [0]))
+        lat, wall = run_stream(srv, gids, pool, reqs, args.serve_rate, args.serve_deadline)
+        st = srv.stats()
+        srv.close()
+        ms = np.array([x for _, x in lat]) if lat else np.zeros(1)
+        out = {"completed": len(lat), "of": len(reqs), "wall_s": round(wall, 3),
+               "queries_per_s": round(len(lat) / wall, 2),
+               "p50_ms": round(float(np.percentile(ms, 50)), 2),
+               "p99_ms": round(float(np.percentile(ms, 99)), 2),
+               "max_running": st["max_running"], "max_queued": st["max_queued"]}
+        for kind, name in ((bc_amd.CENT_Q_BC, "bc"), (bc_amd.CENT_Q_CC, "cc")):
+            x = np.array([v for kd, v in lat if kd == kind])
+            if len(x):
+                out[f"{name}_p50_ms"] = round(float(np.percentile(x, 50)), 2)
+                out[f"{name}_p99_ms"] = round(float(np.percentile(x, 99)), 2)
+        log(f"serve[{label}]: {out}")
+        res[label] = out
+    res["speedup_queries_per_s"] = round(res["served"]["queries_per_s"] /
+                                         max(1e-9, res["one_at_a_time"]["queries_per_s"]), 2)
+    return res
+
+
+EXTRA = {"blocktree": extra_blocktree, "blocktree-bc": extra_blocktree_bc, "grid": extra_grid,
+         "serve": extra_serve}
 
 
 def main() -> int:
@@ -150,6 +254,11 @@ def main() -> int:
                          "blocktree (reduced closeness, persistent vs per-item launches), "
                          "blocktree-bc (plain BC+CC on the block tree), grid (sampled BC+CC on "
                          "the 2048^2 grid)")
+    ap.add_argument("--serve-requests", type=int, default=1000)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-pool", type=int, default=12)
+    ap.add_argument("--serve-deadline", type=float, default=90.0,
+                    help="seconds after the first submission; unfinished requests are cancelled")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

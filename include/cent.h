@@ -156,6 +156,10 @@ int  cent_ctx_create(int32_t device, cent_ctx** out);
 void cent_ctx_destroy(cent_ctx* ctx);
 int  cent_ctx_stats(const cent_ctx* ctx, cent_stats* out);
 int  cent_ctx_reset_stats(cent_ctx* ctx);
+/* Device bytes the context's workspace currently holds, and releasing them
+ * (the next call re-allocates what it needs). */
+int64_t cent_ctx_workspace_bytes(const cent_ctx* ctx);
+int  cent_ctx_trim(cent_ctx* ctx);
 /* Text of the last HIP error seen by this context (empty if none). */
 const char* cent_ctx_last_error(const cent_ctx* ctx);
 /* Measurement helper (no reference counterpart): times `iters` launches of a
@@ -233,6 +237,8 @@ int  cent_reduced_get_view(const cent_reduced* r, cent_reduced_view* out);
  * packed vertex x's block. */
 int  cent_reduced_farness(const cent_reduced* r, const int64_t* D, int64_t* farness);
 void cent_reduced_free(cent_reduced* r);
+/* Frees the device copy the engine caches on r (re-created on next use). */
+void cent_reduced_drop_device(cent_reduced* r);
 
 /* Schedules for cent_cc_reduced. */
 #define CENT_SCHED_PERSISTENT 0  /* one persistent launch over the work-item queue */
@@ -242,6 +248,71 @@ void cent_reduced_free(cent_reduced* r);
  * form; equals cent_cc over all sources on every graph. */
 int  cent_cc_reduced(cent_ctx* ctx, const cent_reduced* r, int64_t* farness, int32_t sched,
                      const cent_opts* opts);
+
+/* ---------------------------------------------------------------- serving */
+/*
+ * Query server: BC / closeness requests on registered graphs, continuously
+ * batched on one GPU.  `slots` requests run at once, each on its own context
+ * and HIP stream; the FIFO queue admits the next request as soon as a slot is
+ * free and its estimated device footprint fits the HBM budget next to the
+ * admitted requests and cached graphs (least-recently-used unreferenced
+ * graphs are evicted to make room).  A request's result does not depend on
+ * what runs beside it.  (No reference counterpart: the reference has no
+ * serving path.)
+ */
+typedef struct cent_server cent_server;
+
+#define CENT_SERVER_DRYRUN 1   /* admission/scheduling only: requests sleep for a
+                                  modelled time, nothing runs on a device */
+
+typedef struct cent_server_opts {
+    int32_t device;
+    int32_t slots;        /* concurrent requests (1..64)                      */
+    int64_t hbm_budget;   /* bytes for request workspaces + cached graphs     */
+    int32_t max_queue;    /* queued requests before submit returns CENT_E_BUSY */
+    int32_t flags;        /* CENT_SERVER_*                                    */
+} cent_server_opts;
+
+#define CENT_Q_BC 0       /* Brandes BC (+ farness of the sources)            */
+#define CENT_Q_CC 1       /* closeness (farness); k == -1: every vertex,
+                             through the D1/AP-reduced path                   */
+
+#define CENT_REQ_QUEUED    0
+#define CENT_REQ_RUNNING   1
+#define CENT_REQ_DONE      2
+#define CENT_REQ_FAILED    3
+#define CENT_REQ_CANCELLED 4
+
+typedef struct cent_request_info {
+    int32_t state;        /* CENT_REQ_*                                       */
+    int32_t status;       /* engine status of a finished request              */
+    int32_t slot;
+    int32_t reserved;
+    double  t_submit_ms, t_start_ms, t_end_ms;   /* server clock               */
+    int64_t footprint;    /* admitted workspace bytes                         */
+} cent_request_info;
+
+typedef struct cent_server_stats {
+    int64_t submitted, completed, failed, cancelled, rejected;
+    int64_t queued, running, max_queued, max_running;
+    int64_t graph_uploads, graph_hits, graph_evictions;
+    int64_t used_workspace, used_graph;
+} cent_server_stats;
+
+int  cent_server_create(const cent_server_opts* opts, cent_server** out);
+void cent_server_destroy(cent_server* s);
+int  cent_server_add_graph(cent_server* s, const struct CSR* csr, int32_t* gid);
+/* k > 0: sources src[0..k); k == -1: every vertex. */
+int  cent_server_submit(cent_server* s, int32_t gid, int32_t kind, const int32_t* src, int32_t k,
+                        int64_t* rid);
+/* Waits up to timeout_ms (< 0: no limit) for the request to finish. */
+int  cent_server_wait(cent_server* s, int64_t rid, int32_t timeout_ms, cent_request_info* info);
+/* Copies the result out (bc: n doubles for CENT_Q_BC; farness: one per source,
+ * or n for k == -1) and forgets the request.  Returns the request's status. */
+int  cent_server_result(cent_server* s, int64_t rid, double* bc, int64_t* farness,
+                        cent_request_info* info);
+int  cent_server_cancel(cent_server* s, int64_t rid);
+int  cent_server_get_stats(cent_server* s, cent_server_stats* out);
 
 /* ------------------------------------------- reference-shaped entry points */
 /* Drop-in shapes of the reference functions over the own struct CSR, running

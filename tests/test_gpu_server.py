@@ -1,0 +1,57 @@
+"""Serving on the GPU: every request's result equals the engine run alone
+(bit-identical — each request's kernels run in order on its own stream and
+workspace) and the oracle, with several requests overlapping on the device."""
+import numpy as np
+import pytest
+
+import _oracle as O
+import bc_amd
+
+pytestmark = pytest.mark.gpu
+
+
+def test_results_same_alone_and_in_mix(ctx):
+    graphs = [bc_amd.gen_rmat(12, 16 << 12, seed=21), bc_amd.gen_rmat(14, 16 << 14, seed=22),
+              bc_amd.gen_blocktree(120, 1.5, 8, 3000, 6.0, 0.3, 0.5, seed=23),
+              bc_amd.gen_grid(40, 50, 0.05, 3, seed=24)]
+    srv = bc_amd.Server(slots=4, hbm_budget=32 << 30)
+    gids = [srv.add_graph(g) for g in graphs]
+    rng = np.random.default_rng(5)
+    reqs = []
+    for i in range(24):
+        j = int(rng.integers(0, len(graphs)))
+        g = graphs[j]
+        if i % 2 == 0:
+            src = bc_amd.sample_sources(g.n, int(rng.integers(1, 200)), 100 + i)
+            reqs.append((srv.submit(gids[j], bc_amd.CENT_Q_BC, src), j, "bc", src))
+        elif i % 4 == 1:
+            reqs.append((srv.submit(gids[j], bc_amd.CENT_Q_CC, None), j, "cc_all", None))
+        else:
+            src = bc_amd.sample_sources(g.n, int(rng.integers(1, 100)), 200 + i)
+            reqs.append((srv.submit(gids[j], bc_amd.CENT_Q_CC, src), j, "cc", src))
+    print("[gpu] server: 24 requests submitted", flush=True)
+    dgs = [ctx.upload(g) for g in graphs]
+    plain_far = {}
+    for rid, j, kind, src in reqs:
+        info = srv.wait(rid)
+        assert info["state"] == "done", info
+        st, bc, far, info = srv.result(rid)
+        assert st == 0
+        g = graphs[j]
+        if kind == "bc":
+            abc, afar, _ = ctx.bc(dgs[j], src)
+            assert np.array_equal(bc, abc) and np.array_equal(far, afar)   # alone == in the mix
+            obc, ofar, _, _ = O.brandes(g, src, threads=8)
+            np.testing.assert_allclose(bc, obc, rtol=1e-9, atol=1e-9 * max(1.0, np.abs(obc).max()))
+            assert np.array_equal(far, ofar)
+        elif kind == "cc":
+            assert np.array_equal(far, ctx.cc(dgs[j], src)[0])
+            assert np.array_equal(far, O.farness(g, src))
+        else:
+            if j not in plain_far:
+                plain_far[j] = O.weighted_farness(g.n, g.row_ptr, g.col, np.ones(g.n, np.int64))
+            assert np.array_equal(far, plain_far[j])
+    st = srv.stats()
+    print(f"[gpu] server stats {st}", flush=True)
+    assert st["completed"] == 24 and st["max_running"] >= 2
+    srv.close()
