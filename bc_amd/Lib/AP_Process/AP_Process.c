@@ -350,3 +350,52 @@ void cent_reduced_drop_device(cent_reduced* r) {
     r->dev = NULL;
     r->dev_free = NULL;
 }
+
+/*
+ * Betweenness through the reduction (no reference counterpart: the reference
+ * reduces the graph for closeness only).  For a vertex v of a component with
+ * N vertices, let the pieces of v be the connected parts of the component
+ * once v is removed.  Every shortest path between two different pieces runs
+ * through v; a shortest path between two vertices of the same piece can only
+ * run through v inside a biconnected block that holds v, entering and leaving
+ * that block at its copies' weighted vertices.  Hence
+ *   BC(v) = [(N-1)^2 - sum of squared piece sizes]
+ *         + sum over the blocks B that hold v of the weighted in-block term
+ *           sum_{a != b, both != v} W(a) W(b) sigma_ab(v) / sigma_ab,
+ * the second part being what the GPU accumulates at v's copy in B (weighted
+ * Brandes from every copy, source and target weights W).  The pieces of v are
+ * the trees folded into v (sizes w[c] of its fold children c) and, for every
+ * block B holding v, the rest of the component behind B: N - W(v's copy in B)
+ * (for a folded vertex: N - w[v]).  Ordered pairs, like computeBC summed over
+ * all sources (Seq_Brandes/Brandes.c:249).
+ */
+int cent_reduced_finish_bc(const struct cent_reduced* r, const double* bcp, double* bc) {
+    if (!r || (!bcp && r->np > 0) || !bc) return CENT_E_INVAL;
+    const int32_t n = r->n;
+    const size_t N = (size_t)(n > 0 ? n : 1);
+    double* sumsq = (double*)calloc(N, sizeof(double));  /* sum of squared piece sizes */
+    double* inblock = (double*)calloc(N, sizeof(double));
+    if (!sumsq || !inblock) { free(sumsq); free(inblock); return CENT_E_OOM; }
+    for (int32_t i = 0; i < r->d1.nfold; ++i) {
+        const int32_t c = r->d1.order[i];
+        const double wc = (double)r->d1.w[c];
+        sumsq[r->d1.parent[c]] += wc * wc;
+        const double rest = (double)(r->comp_w[r->vcomp[c]] - r->d1.w[c]);
+        sumsq[c] += rest * rest;
+    }
+    for (int32_t x = 0; x < r->np; ++x) {
+        const int32_t v = r->porig[x];
+        const double rest = (double)(r->comp_w[r->vcomp[v]] - r->pw[x]);
+        sumsq[v] += rest * rest;
+        inblock[v] += bcp[x];
+    }
+    for (int32_t v = 0; v < n; ++v) {
+        const double s = (double)(r->comp_w[r->vcomp[v]] - 1);
+        bc[v] += (s * s - sumsq[v]) + inblock[v];
+    }
+    free(sumsq);
+    free(inblock);
+    return CENT_OK;
+}
+
+int cent_reduced_bc(const cent_reduced* r, const double* bcp, double* bc) { return cent_reduced_finish_bc(r, bcp, bc); }

@@ -79,6 +79,11 @@ void cent_reduced_free(struct cent_reduced* r);
  * vertices (recovered from their parents) and isolated / fully folded ones. */
 int  cent_reduced_finish(const struct cent_reduced* r, const int64_t* D, int64_t* farness);
 
+/* Given bcp[x] = sum over sources a in x's block of W(a) * delta^W_a(x)
+ * (weighted Brandes on the packed graph), adds the betweenness of every
+ * input vertex into bc (ordered pairs, like computeBC over all sources). */
+int  cent_reduced_finish_bc(const struct cent_reduced* r, const double* bcp, double* bc);
+
 #ifdef __cplusplus
 }
 #endif

@@ -214,6 +214,7 @@ class _RView(C.Structure):
 _lib.cent_reduce.argtypes = [_P_CSR, C.POINTER(C.c_void_p)]
 _lib.cent_reduced_get_view.argtypes = [C.c_void_p, C.POINTER(_RView)]
 _lib.cent_reduced_farness.argtypes = [C.c_void_p, _I64P, _I64P]
+_lib.cent_reduced_bc.argtypes = [C.c_void_p, _F64P, _F64P]
 _lib.cent_reduced_free.argtypes = [C.c_void_p]
 _lib.cent_reduced_free.restype = None
 CENT_SCHED_PERSISTENT, CENT_SCHED_PER_ITEM = 0, 1
@@ -252,6 +253,16 @@ class Reduced:
         out = np.zeros(max(self.n, 1), np.int64)
         _check(_lib.cent_reduced_farness(self._h, _ptr(D, C.c_int64) if self.np else None,
                                          _ptr(out, C.c_int64)), "cent_reduced_farness")
+        return out[:self.n]
+
+    def bc_from(self, bcp) -> np.ndarray:
+        """betweenness of every input vertex from the packed weighted dependencies bcp."""
+        bcp = np.ascontiguousarray(bcp, dtype=np.float64)
+        if len(bcp) != self.np:
+            raise ValueError("bcp must have one entry per packed vertex")
+        out = np.zeros(max(self.n, 1))
+        _check(_lib.cent_reduced_bc(self._h, _ptr(bcp, C.c_double) if self.np else None,
+                                    _ptr(out, C.c_double)), "cent_reduced_bc")
         return out[:self.n]
 
     def __del__(self):

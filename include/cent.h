@@ -237,6 +237,11 @@ int  cent_reduced_get_view(const cent_reduced* r, cent_reduced_view* out);
  * packed vertex x's block. */
 int  cent_reduced_farness(const cent_reduced* r, const int64_t* D, int64_t* farness);
 void cent_reduced_free(cent_reduced* r);
+/* bc[v] += betweenness of every input vertex (all sources, ordered pairs,
+ * textbook formula) from bcp[x] = weighted Brandes dependencies at each packed
+ * vertex x (source and target weights pw).  No reference counterpart: the
+ * reference reduces the graph for closeness only. */
+int  cent_reduced_bc(const cent_reduced* r, const double* bcp, double* bc);
 /* Frees the device copy the engine caches on r (re-created on next use). */
 void cent_reduced_drop_device(cent_reduced* r);
 

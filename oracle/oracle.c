@@ -273,3 +273,44 @@ int oracle_weighted_farness(int32_t n, const int32_t* rp, const int32_t* col, co
     }
     return err ? 2 : 0;
 }
+
+/* Weighted Brandes (checker for the reduced BC path): vertex weights w act as
+ * multiplicities of sources and targets:
+ *   bc[v] += w[s] * delta_s(v),  delta_s(v) = sum_{u: v in pred_s(u)} sigma_v/sigma_u (w[u] + delta_s(u)).
+ * With w == 1 this is oracle_brandes mode 0.  Serial, bc accumulated. */
+int oracle_weighted_brandes(int32_t n, const int32_t* rp, const int32_t* col, const int64_t* w,
+                            const int32_t* src, int32_t k, double* bc, int64_t* far) {
+    ws_t ws;
+    if (!ws_alloc(&ws, n)) { ws_free(&ws); return 2; }
+    for (int32_t v = 0; v < n; ++v) ws.dist[v] = -1;
+    for (int32_t i = 0; i < k; ++i) {
+        const int32_t s = src[i];
+        int64_t h = 0, t = 0, f = 0;
+        ws.queue[t++] = s; ws.dist[s] = 0; ws.sigma[s] = 1.0;
+        while (h < t) {
+            int32_t x = ws.queue[h++];
+            for (int32_t j = rp[x]; j < rp[x + 1]; ++j) {
+                int32_t y = col[j];
+                if (ws.dist[y] < 0) {
+                    ws.dist[y] = ws.dist[x] + 1; ws.sigma[y] = 0.0; ws.queue[t++] = y;
+                    f += (int64_t)ws.dist[y] * w[y];
+                }
+                if (ws.dist[y] == ws.dist[x] + 1) ws.sigma[y] += ws.sigma[x];
+            }
+        }
+        if (far) far[i] = f;
+        for (int64_t q = t - 1; q >= 0; --q) {
+            int32_t x = ws.queue[q];
+            double acc = 0.0;
+            for (int32_t j = rp[x]; j < rp[x + 1]; ++j) {
+                int32_t y = col[j];
+                if (ws.dist[y] == ws.dist[x] + 1) acc += ((double)w[y] + ws.delta[y]) / ws.sigma[y];
+            }
+            ws.delta[x] = ws.sigma[x] * acc;
+            if (x != s) bc[x] += (double)w[s] * ws.delta[x];
+        }
+        for (int64_t q = 0; q < t; ++q) ws.dist[ws.queue[q]] = -1;
+    }
+    ws_free(&ws);
+    return 0;
+}

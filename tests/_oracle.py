@@ -31,6 +31,7 @@ def _load():
     lib.oracle_brandes_f32.argtypes = [C.c_int32, I32P, I32P, I32P, C.c_int32, F32P]
     lib.oracle_max_threads.restype = C.c_int
     lib.oracle_weighted_farness.argtypes = [C.c_int32, I32P, I32P, I64P, I32P, C.c_int32, I64P]
+    lib.oracle_weighted_brandes.argtypes = [C.c_int32, I32P, I32P, I64P, I32P, C.c_int32, F64P, I64P]
     return lib
 
 
@@ -91,6 +92,20 @@ def weighted_farness(n, rp, col, w, sources=None):
     assert _lib.oracle_weighted_farness(n, _p(rp, C.c_int32), _p(col, C.c_int32), _p(w, C.c_int64),
                                         _p(src, C.c_int32), len(src), _p(far, C.c_int64)) == 0
     return far[:len(src)]
+
+
+def weighted_brandes(n, rp, col, w, sources=None):
+    """(bc, weighted farness) with source/target weights w."""
+    rp = np.ascontiguousarray(rp, np.int32)
+    col = np.ascontiguousarray(col, np.int32) if len(col) else np.zeros(1, np.int32)
+    w = np.ascontiguousarray(w, np.int64) if len(w) else np.zeros(1, np.int64)
+    src = np.arange(n, dtype=np.int32) if sources is None else np.ascontiguousarray(sources, np.int32)
+    bc = np.zeros(max(n, 1))
+    far = np.zeros(max(len(src), 1), np.int64)
+    assert _lib.oracle_weighted_brandes(n, _p(rp, C.c_int32), _p(col, C.c_int32), _p(w, C.c_int64),
+                                        _p(src, C.c_int32), len(src), _p(bc, C.c_double),
+                                        _p(far, C.c_int64)) == 0
+    return bc[:n], far[:len(src)]
 
 
 def max_threads():

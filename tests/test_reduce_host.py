@@ -89,3 +89,48 @@ def test_block_tree_and_rmat():
     assert r.ncomp > 1
     grid = bc_amd.gen_grid(20, 20, 0.1, 5, seed=4)
     check(grid)
+
+
+def check_bc(g):
+    """Reduced betweenness (combinatorial piece term + weighted in-block Brandes
+    from the oracle) must equal plain all-source Brandes."""
+    r = bc_amd.Reduced(g)
+    bcp, D = O.weighted_brandes(r.np, r.prp, r.pcol, r.pw)
+    bc = r.bc_from(bcp)
+    want, far, _, _ = O.brandes(g, None)
+    np.testing.assert_allclose(bc, want, rtol=1e-10, atol=1e-9 * max(1.0, np.abs(want).max()))
+    assert np.array_equal(r.farness_from(D), far)
+
+
+def test_weighted_oracle_reduces_to_plain():
+    g = bc_amd.buildGraph(O.dataset_path("karate.txt"))
+    bc, far = O.weighted_brandes(g.n, g.row_ptr, g.col, np.ones(g.n, np.int64))
+    want, wfar, _, _ = O.brandes(g, None)
+    np.testing.assert_allclose(bc, want, rtol=1e-12)
+    assert np.array_equal(far, wfar)
+
+
+@pytest.mark.parametrize("name", ["karate.txt", "dolphins.txt", "football.txt", "polbooks.txt",
+                                  "testdata.txt", "testdata1.txt", "testdata2.txt",
+                                  "ia-fb-messages.mtx"])
+def test_reduced_bc_fixtures(name):
+    check_bc(bc_amd.buildGraph(O.dataset_path(name)))
+
+
+def test_reduced_bc_shapes_and_random():
+    check_bc(path(2))
+    check_bc(path(15))
+    check_bc(bc_amd.CSR.from_edges(9, np.zeros(8), np.arange(1, 9)))
+    u = [0, 1, 2, 2, 3, 4, 5, 6, 6, 4, 7, 8]
+    v = [1, 2, 0, 3, 4, 5, 6, 7, 5, 9, 9, 10]
+    check_bc(bc_amd.CSR.from_edges(11, u, v))
+    u = [0, 1, 2, 2, 3, 4, 6]
+    v = [1, 2, 0, 3, 4, 2, 7]
+    check_bc(bc_amd.CSR.from_edges(9, u, v))
+    for seed in range(10):
+        rng = np.random.default_rng(100 + seed)
+        n = int(rng.integers(5, 70))
+        m = int(rng.integers(n // 2, 2 * n))
+        check_bc(bc_amd.CSR.from_edges(n, rng.integers(0, n, m), rng.integers(0, n, m)))
+    check_bc(bc_amd.gen_blocktree(40, 1.5, 8, 200, 6.0, 0.3, 0.5, seed=3))
+    check_bc(bc_amd.gen_rmat(9, 8 << 9, seed=7, lcc=False))
