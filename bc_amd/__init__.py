@@ -312,6 +312,7 @@ _lib.cent_cc.argtypes = [C.c_void_p, C.c_void_p, _I32P, C.c_int32, _I64P, _I64P,
                          C.POINTER(_Opts)]
 _lib.cent_probe_copy_bandwidth.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.POINTER(C.c_double)]
 _lib.cent_cc_reduced.argtypes = [C.c_void_p, C.c_void_p, _I64P, C.c_int32, C.POINTER(_Opts)]
+_lib.cent_bc_reduced.argtypes = [C.c_void_p, C.c_void_p, _F64P, _I64P, C.c_int32, C.POINTER(_Opts)]
 _lib.compute_D1_AP_CC_shareBased_DegreeOrder.argtypes = [_P_CSR, _I64P]
 _lib.computeBC.argtypes = [_P_CSR, _F64P]
 _lib.computeBC_compat.argtypes = [_P_CSR, _F64P]
@@ -428,6 +429,15 @@ class Context:
         _check(_lib.cent_cc_reduced(self._h, red._h, _ptr(out, C.c_int64), sched, C.byref(o)),
                "cent_cc_reduced", self.last_error())
         return out[:red.n]
+
+    def bc_reduced(self, red: Reduced, sched: int = CENT_SCHED_PERSISTENT, timing: bool = False):
+        """(bc, farness) of every vertex, all sources, through the reduced form."""
+        bc = np.zeros(max(red.n, 1))
+        far = np.zeros(max(red.n, 1), np.int64)
+        o = _opts(0, timing)
+        _check(_lib.cent_bc_reduced(self._h, red._h, _ptr(bc, C.c_double), _ptr(far, C.c_int64),
+                                    sched, C.byref(o)), "cent_bc_reduced", self.last_error())
+        return bc[:red.n], far[:red.n]
 
 
 # ---------------------------------------------------------------- serving

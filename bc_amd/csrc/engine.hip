@@ -61,9 +61,12 @@ struct cent_ctx {
     // frontier-queue mode
     DevBuf<uint64_t> Fd2, QF, Adev;
     DevBuf<int> Q, qoff, cand, stamp, ctl;
-    // reduced closeness per-run buffers
+    // reduced closeness / betweenness per-run buffers
     DevBuf<int64_t> red_D;
     DevBuf<int> red_head;
+    DevBuf<double> red_part, red_bcp, it_sig, it_coef;
+    DevBuf<uint8_t> it_lvl;
+    DevBuf<int> red_deep;
     size_t F_levels = 0;     // levels the F buffer holds for the current N
     size_t F_n = 0;
     int* h_flags = nullptr;  // pinned
@@ -292,6 +295,8 @@ extern "C" void cent_ctx_destroy(cent_ctx* ctx) {
     release(ctx->Fd2); release(ctx->QF); release(ctx->Adev); release(ctx->Q); release(ctx->qoff);
     release(ctx->cand); release(ctx->stamp); release(ctx->ctl);
     release(ctx->red_D); release(ctx->red_head);
+    release(ctx->red_part); release(ctx->red_bcp); release(ctx->it_sig); release(ctx->it_coef);
+    release(ctx->it_lvl); release(ctx->red_deep);
     if (ctx->h_flags) (void)hipHostFree(ctx->h_flags);
     for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
@@ -315,9 +320,11 @@ extern "C" int64_t cent_ctx_workspace_bytes(const cent_ctx* ctx) {
     return (int64_t)(8 * (ctx->vis.cap + ctx->F.cap + ctx->hp_or.cap + ctx->sigma.cap + ctx->coef.cap +
                           ctx->bc.cap + ctx->hp_acc.cap + ctx->dist_sum.cap + ctx->far.cap + ctx->part_far.cap +
                           ctx->part_cnt.cap + ctx->cnt.cap + ctx->Fd2.cap + ctx->QF.cap + ctx->Adev.cap +
-                          ctx->red_D.cap) +
+                          ctx->red_D.cap + ctx->red_part.cap + ctx->red_bcp.cap + ctx->it_sig.cap +
+                          ctx->it_coef.cap) + ctx->it_lvl.cap +
                      4 * (ctx->npred.cap + ctx->hp_cnt.cap + ctx->src.cap + ctx->flags.cap + ctx->Q.cap +
-                          ctx->qoff.cap + ctx->cand.cap + ctx->stamp.cap + ctx->ctl.cap + ctx->red_head.cap));
+                          ctx->qoff.cap + ctx->cand.cap + ctx->stamp.cap + ctx->ctl.cap + ctx->red_head.cap +
+                          ctx->red_deep.cap));
 }
 
 extern "C" int cent_ctx_trim(cent_ctx* ctx) {
@@ -331,6 +338,8 @@ extern "C" int cent_ctx_trim(cent_ctx* ctx) {
     release(ctx->Fd2); release(ctx->QF); release(ctx->Adev); release(ctx->Q); release(ctx->qoff);
     release(ctx->cand); release(ctx->stamp); release(ctx->ctl);
     release(ctx->red_D); release(ctx->red_head);
+    release(ctx->red_part); release(ctx->red_bcp); release(ctx->it_sig); release(ctx->it_coef);
+    release(ctx->it_lvl); release(ctx->red_deep);
     ctx->F_levels = 0;
     ctx->F_n = 0;
     return CENT_OK;
@@ -511,6 +520,7 @@ struct BatchSpec {
     const int64_t* weight = nullptr;  // device, per vertex of the DevCSR, or null
     int64_t* dist_sum = nullptr;      // device [n] or null
     double* bc = nullptr;             // device [n] (with_bc)
+    const int64_t* bwgt = nullptr;    // weighted dependencies (reduced graphs), or null
     const cent_opts* opts = nullptr;
 };
 
@@ -643,6 +653,8 @@ static int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const
             ba.sigma = ctx->sigma.p; ba.coef = ctx->coef.p;
             ba.npred = compat ? ctx->npred.p : nullptr;
             ba.bc = sp.bc;
+            ba.wgt = sp.bwgt;
+            ba.bsrc = d_src;
             hipEvent_t e0;
             TRY(timer_begin(ctx, tm, &e0));
             if (compat) k_bwd<true><<<G, kBlock, 0, s>>>(ba);
@@ -930,6 +942,9 @@ struct RedDev {
     int* order = nullptr;     // [nsmall] small items, largest range first
     int nsmall = 0;
     std::vector<int> big;     // items handled by the level-synchronous kernels
+    int64_t* poff = nullptr;  // [nitems] offsets of the items' BC partials (small items)
+    int64_t part_total = 0;
+    std::vector<int64_t> h_poff;
 };
 
 static void reddev_free(void* p) {
@@ -941,6 +956,7 @@ static void reddev_free(void* p) {
     if (rd->iota) (void)hipFree(rd->iota);
     if (rd->items) (void)hipFree(rd->items);
     if (rd->order) (void)hipFree(rd->order);
+    if (rd->poff) (void)hipFree(rd->poff);
 
     delete rd;
 }
@@ -980,6 +996,12 @@ static int reddev_get(cent_ctx* ctx, cent_reduced* r, RedDev** out) {
     rd->nsmall = (int)small.size();
     TRY(upload(ctx, &rd->items, items.data(), (size_t)r->nitems));
     TRY(upload(ctx, &rd->order, small.data(), small.size()));
+    rd->h_poff.assign((size_t)(r->nitems > 0 ? r->nitems : 1), -1);
+    for (int i : small) {
+        rd->h_poff[(size_t)i] = rd->part_total;
+        rd->part_total += r->items[i].hi - r->items[i].lo;
+    }
+    TRY(upload(ctx, &rd->poff, rd->h_poff.data(), rd->h_poff.size()));
     *out = rd;
     return CENT_OK;
 }
@@ -1040,6 +1062,116 @@ extern "C" int cent_cc_reduced(cent_ctx* ctx, const cent_reduced* rc, int64_t* f
     TRY(cent_reduced_finish(r, D.data(), farness));
     ctx->stats.levels += levels;
     ctx->stats.reached += (int64_t)r->n;  // every vertex is a source
+    ctx->stats.edges_reached += 2 * r->m;
+    ctx->stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return CENT_OK;
+}
+
+
+// Betweenness (+ closeness) through the reduction: weighted Brandes from every
+// packed vertex inside its block, then the host adds the piece-size terms.
+constexpr int kBcItemGrid = 512;
+
+extern "C" int cent_bc_reduced(cent_ctx* ctx, const cent_reduced* rc, double* bc, int64_t* farness,
+                               int32_t sched, const cent_opts* opts) {
+    if (!ctx || !rc || !bc) return CENT_E_INVAL;
+    if (sched != CENT_SCHED_PERSISTENT && sched != CENT_SCHED_PER_ITEM) return CENT_E_INVAL;
+    cent_reduced* r = const_cast<cent_reduced*>(rc);
+    TRY(check_device(ctx));
+    auto t0 = std::chrono::steady_clock::now();
+    RedDev* rd;
+    TRY(reddev_get(ctx, r, &rd));
+    hipStream_t s = ctx->stream;
+    const size_t NP = (size_t)(r->np > 0 ? r->np : 1);
+    TRY(ensure(ctx, ctx->red_D, NP));
+    TRY(ensure(ctx, ctx->red_head, 2));
+    TRY(ensure(ctx, ctx->red_bcp, NP));
+    TRY(ensure(ctx, ctx->red_deep, (size_t)(r->nitems > 0 ? r->nitems : 1)));
+    TRY(ensure(ctx, ctx->cnt, 2));
+    int64_t* Dd = ctx->red_D.p;
+    HIPTRY(hipMemsetAsync(Dd, 0, sizeof(int64_t) * NP, s));
+    HIPTRY(hipMemsetAsync(ctx->red_bcp.p, 0, sizeof(double) * NP, s));
+    HIPTRY(hipMemsetAsync(ctx->cnt.p, 0, sizeof(unsigned long long) * 2, s));
+    KernelTimer tm;
+    tm.on = opts && (opts->flags & CENT_OPT_TIMING);
+    ctx->ev_used = 0;
+    BatchSpec sp;
+    sp.with_bc = true;
+    sp.weight = rd->w;
+    sp.bwgt = rd->w;
+    sp.bc = ctx->red_bcp.p;
+    sp.opts = opts;
+    int64_t levels = 0;
+    auto run_item = [&](int i) -> int {
+        const cent_item& it = r->items[i];
+        const GView v = sub_view(rd->d, it.lo, it.hi);
+        return level_batch(ctx, v, sp, rd->iota + it.s0, it.ns, Dd + it.s0, tm, &levels);
+    };
+    std::vector<int> deep;
+    const bool persistent = (sched == CENT_SCHED_PERSISTENT) && rd->nsmall > 0;
+    if (persistent) {
+        const int grid = std::min(rd->nsmall, kBcItemGrid);
+        const size_t slot = (size_t)kItemLds * kWave;
+        TRY(ensure(ctx, ctx->it_sig, slot * grid));
+        TRY(ensure(ctx, ctx->it_coef, slot * grid));
+        TRY(ensure(ctx, ctx->it_lvl, slot * grid));
+        TRY(ensure(ctx, ctx->red_part, (size_t)(rd->part_total > 0 ? rd->part_total : 1)));
+        TRY(ensure(ctx, ctx->flags, 2));
+        HIPTRY(hipMemsetAsync(ctx->red_head.p, 0, sizeof(int) * 2, s));
+        HIPTRY(hipMemsetAsync(ctx->flags.p, 0, sizeof(int) * 2, s));
+        BcItemArgs ba;
+        ba.rp = rd->d.rp; ba.col = rd->d.col; ba.weight = rd->w; ba.items = rd->items; ba.order = rd->order;
+        ba.poff = rd->poff; ba.nitems = rd->nsmall; ba.head = ctx->red_head.p; ba.D = Dd;
+        ba.bcpart = ctx->red_part.p; ba.sig = ctx->it_sig.p; ba.coef = ctx->it_coef.p; ba.lvl = ctx->it_lvl.p;
+        ba.deep = ctx->red_deep.p; ba.ndeep = ctx->red_head.p + 1; ba.overflow = ctx->flags.p + 1;
+        hipEvent_t e0;
+        TRY(timer_begin(ctx, tm, &e0));
+        k_bc_items<<<grid, kItemBlock, 0, s>>>(ba);
+        HIPTRY(hipGetLastError());
+        TRY(timer_end(ctx, tm, e0, true));
+        ctx->stats.launches_fwd++;
+        ctx->stats.batches += rd->nsmall;
+        int hh[2];
+        HIPTRY(hipMemcpyAsync(hh, ctx->red_head.p, sizeof(hh), hipMemcpyDeviceToHost, s));
+        HIPTRY(hipMemcpyAsync(ctx->h_flags, ctx->flags.p, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+        HIPTRY(hipStreamSynchronize(s));
+        if (ctx->h_flags[1]) return CENT_E_OVERFLOW;
+        if (hh[1] > 0) {
+            deep.resize((size_t)hh[1]);
+            HIPTRY(hipMemcpy(deep.data(), ctx->red_deep.p, sizeof(int) * (size_t)hh[1], hipMemcpyDeviceToHost));
+            std::sort(deep.begin(), deep.end());
+        }
+        for (int i : rd->big) TRY(run_item(i));
+        for (int i : deep) TRY(run_item(i));
+    } else {
+        for (int i = 0; i < r->nitems; ++i) TRY(run_item(i));
+    }
+    std::vector<int64_t> D(NP);
+    std::vector<double> bcp(NP);
+    std::vector<double> part;
+    HIPTRY(hipMemcpyAsync(D.data(), Dd, sizeof(int64_t) * NP, hipMemcpyDeviceToHost, s));
+    HIPTRY(hipMemcpyAsync(bcp.data(), ctx->red_bcp.p, sizeof(double) * NP, hipMemcpyDeviceToHost, s));
+    if (persistent && rd->part_total > 0) {
+        part.resize((size_t)rd->part_total);
+        HIPTRY(hipMemcpyAsync(part.data(), ctx->red_part.p, sizeof(double) * part.size(), hipMemcpyDeviceToHost, s));
+    }
+    HIPTRY(hipStreamSynchronize(s));
+    timer_collect(ctx, tm);
+    if (persistent) {
+        // add the small items' partials in item order (deep items were redone above)
+        std::vector<char> is_deep((size_t)(r->nitems > 0 ? r->nitems : 1), 0);
+        for (int i : deep) is_deep[(size_t)i] = 1;
+        for (int i = 0; i < r->nitems; ++i) {
+            const int64_t o = rd->h_poff[(size_t)i];
+            if (o < 0 || is_deep[(size_t)i]) continue;
+            const cent_item& it = r->items[i];
+            for (int x = it.lo; x < it.hi; ++x) bcp[(size_t)x] += part[(size_t)(o + x - it.lo)];
+        }
+    }
+    TRY(cent_reduced_finish_bc(r, bcp.data(), bc));
+    if (farness) TRY(cent_reduced_finish(r, D.data(), farness));
+    ctx->stats.levels += levels;
+    ctx->stats.reached += (int64_t)r->n;
     ctx->stats.edges_reached += 2 * r->m;
     ctx->stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return CENT_OK;

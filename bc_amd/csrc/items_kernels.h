@@ -111,3 +111,228 @@ __global__ __launch_bounds__(kItemBlock) void k_cc_items(ItemArgs a) {
 }
 
 }  // namespace cent
+
+namespace cent {
+
+// Persistent weighted Brandes over the same work items (reduced graphs).
+// Frontier masks stay in LDS as in k_cc_items; the per-(vertex, lane) path
+// counts, coefficients and levels of the item's range live in a per-
+// workgroup slice of HBM scratch (kItemLds x 64 entries each).  Outputs:
+//   D[s]                      weighted farness of every source (assigned)
+//   bcpart[poff[item] + i]    the item's weighted dependencies at vertex lo+i
+//                             (added across the item's levels; the host sums
+//                             the items of a block in item order)
+// An item whose BFS gets deeper than 254 levels is abandoned and listed in
+// deep[] for the host to run on the level-synchronous kernels.
+struct BcItemArgs {
+    const int* __restrict__ rp;
+    const int* __restrict__ col;
+    const int64_t* __restrict__ weight;
+    const int4* __restrict__ items;
+    const int* __restrict__ order;
+    const int64_t* __restrict__ poff;     // per item index
+    int nitems;
+    int* __restrict__ head;
+    int64_t* __restrict__ D;
+    double* __restrict__ bcpart;
+    double* __restrict__ sig;             // [grid][kItemLds * 64]
+    double* __restrict__ coef;            // [grid][kItemLds * 64]
+    uint8_t* __restrict__ lvl;            // [grid][kItemLds * 64], 255 = unreached
+    int* __restrict__ deep;               // [nitems] item indices
+    int* __restrict__ ndeep;
+    int* __restrict__ overflow;           // set when a path count is inf
+};
+
+constexpr int kItemMaxDepth = 254;
+
+__global__ __launch_bounds__(kItemBlock) void k_bc_items(BcItemArgs a) {
+    __shared__ uint64_t s_vis[kItemLds];
+    __shared__ uint64_t s_F[2][kItemLds];
+    __shared__ int64_t s_far[kItemBlock / kWave][kWave];
+    __shared__ int s_item;
+    __shared__ int s_flag;
+    __shared__ int s_deep;
+    const int tid = threadIdx.x;
+    const int lane = lane_id();
+    const int wid = tid >> 6;
+    constexpr int kW = kItemBlock / kWave;
+    double* __restrict__ sig = a.sig + (size_t)blockIdx.x * kItemLds * kWave;
+    double* __restrict__ coef = a.coef + (size_t)blockIdx.x * kItemLds * kWave;
+    uint8_t* __restrict__ lvl = a.lvl + (size_t)blockIdx.x * kItemLds * kWave;
+    for (;;) {
+        if (tid == 0) s_item = atomicAdd(a.head, 1);
+        __syncthreads();
+        const int q = s_item;
+        if (q >= a.nitems) break;
+        const int itx = a.order[q];
+        const int4 it = a.items[itx];
+        const int lo = it.x, R = it.y - it.x, s0 = it.z, ns = it.w;
+        const uint64_t A = (ns >= 64) ? ~0ull : ((1ull << ns) - 1);
+        double* __restrict__ part = a.bcpart + a.poff[itx];
+        for (int i = tid; i < R; i += kItemBlock) { s_vis[i] = 0; s_F[0][i] = 0; s_F[1][i] = 0; part[i] = 0.0; }
+        for (int i = tid; i < R * (kWave / 8); i += kItemBlock) reinterpret_cast<uint64_t*>(lvl)[i] = ~0ull;
+        if (tid == 0) s_deep = 0;
+        __syncthreads();
+        if (tid < ns) {
+            const int i = s0 - lo + tid;
+            s_F[0][i] = 1ull << tid;
+            s_vis[i] = 1ull << tid;
+            sig[(size_t)i * kWave + tid] = 1.0;
+            lvl[(size_t)i * kWave + tid] = 0;
+        }
+        int64_t far = 0;
+        int cur = 0, depth = 0;
+        // ---- forward
+        for (int d = 0;; ++d) {
+            if (tid == 0) s_flag = 0;
+            __syncthreads();
+            const uint64_t* Fc = s_F[cur];
+            uint64_t* Fn = s_F[cur ^ 1];
+            bool any = false;
+            for (int base = wid * kWave; base < R; base += kW * kWave) {
+                const int i = base + lane;
+                uint64_t visl = A;
+                int rb = 0, re = 0;
+                if (i < R) {
+                    visl = s_vis[i];
+                    if (visl != A) { rb = a.rp[lo + i]; re = a.rp[lo + i + 1]; }
+                }
+                uint64_t work = ballot(visl != A);
+                while (work) {
+                    const int b = __builtin_ctzll(work);
+                    work &= work - 1;
+                    const uint64_t old = rl64(visl, b);
+                    const uint64_t cand = A & ~old;
+                    const int beg = rl32(rb, b), end = rl32(re, b);
+                    double acc = 0.0;
+                    uint32_t cnt = 0;
+                    for (int j0 = beg; j0 < end; j0 += kWave) {
+                        const int j = j0 + lane;
+                        int u = 0;
+                        uint64_t f = 0;
+                        if (j < end) {
+                            u = a.col[j] - lo;
+                            f = Fc[u] & cand;
+                        }
+                        uint64_t ball = ballot(f != 0);
+                        while (ball) {
+                            const int c = __builtin_ctzll(ball);
+                            ball &= ball - 1;
+                            const int uc = rl32(u, c);
+                            const uint64_t fc = rl64(f, c);
+                            if ((fc >> lane) & 1) {
+                                acc += sig[(size_t)uc * kWave + lane];
+                                ++cnt;
+                            }
+                        }
+                    }
+                    const uint64_t nm = ballot(cnt != 0);
+                    if (nm) {
+                        const int vi = base + b;
+                        if ((nm >> lane) & 1) {
+                            sig[(size_t)vi * kWave + lane] = acc;
+                            if (__builtin_isinf(acc)) *a.overflow = 1;
+                            lvl[(size_t)vi * kWave + lane] = (uint8_t)(d + 1);
+                            far += (int64_t)(d + 1) * a.weight[lo + vi];
+                        }
+                        if (lane == 0) {
+                            Fn[vi] = nm;
+                            s_vis[vi] = old | nm;
+                        }
+                        any = true;
+                    }
+                }
+            }
+            if (any && lane == 0) s_flag = 1;
+            __syncthreads();
+            if (!s_flag) break;
+            depth = d + 1;
+            if (depth >= kItemMaxDepth) {
+                if (tid == 0) s_deep = 1;
+                __syncthreads();
+                break;
+            }
+            for (int i = tid; i < R; i += kItemBlock) s_F[cur][i] = 0;
+            cur ^= 1;
+            __syncthreads();
+        }
+        if (s_deep) {
+            if (tid == 0) a.deep[atomicAdd(a.ndeep, 1)] = itx;
+            __syncthreads();
+            continue;
+        }
+        // Fc = s_F[cur] holds level `depth`, the other buffer is all zero (level depth + 1)
+        // ---- backward, level d uses masks of d (Fd) and d + 1 (Fu)
+        for (int d = depth; d >= 1; --d) {
+            const uint64_t* Fd = s_F[cur];
+            const uint64_t* Fu = s_F[cur ^ 1];
+            for (int base = wid * kWave; base < R; base += kW * kWave) {
+                const int i = base + lane;
+                uint64_t fl = 0;
+                int rb = 0, re = 0;
+                if (i < R) {
+                    fl = Fd[i];
+                    if (fl) { rb = a.rp[lo + i]; re = a.rp[lo + i + 1]; }
+                }
+                uint64_t work = ballot(fl != 0);
+                while (work) {
+                    const int b = __builtin_ctzll(work);
+                    work &= work - 1;
+                    const uint64_t m = rl64(fl, b);
+                    const int beg = rl32(rb, b), end = rl32(re, b);
+                    double acc = 0.0;
+                    for (int j0 = beg; j0 < end; j0 += kWave) {
+                        const int j = j0 + lane;
+                        int w = 0;
+                        uint64_t g = 0;
+                        if (j < end) {
+                            w = a.col[j] - lo;
+                            g = Fu[w] & m;
+                        }
+                        uint64_t ball = ballot(g != 0);
+                        while (ball) {
+                            const int c = __builtin_ctzll(ball);
+                            ball &= ball - 1;
+                            const int wc = rl32(w, c);
+                            const uint64_t gc = rl64(g, c);
+                            if ((gc >> lane) & 1) acc += coef[(size_t)wc * kWave + lane];
+                        }
+                    }
+                    const int vi = base + b;
+                    double contrib = 0.0;
+                    if ((m >> lane) & 1) {
+                        const size_t idx = (size_t)vi * kWave + lane;
+                        const double sg = sig[idx];
+                        const double dl = sg * acc;
+                        coef[idx] = ((double)a.weight[lo + vi] + dl) / sg;
+                        contrib = dl * (double)a.weight[s0 + lane];
+                    }
+                    const double tot = wave_sum(contrib);
+                    if (lane == 0) part[vi] += tot;
+                }
+            }
+            __syncthreads();
+            // the buffer of level d + 1 becomes level d - 1, rebuilt from lvl
+            uint64_t* Fnew = s_F[cur ^ 1];
+            for (int base = wid * kWave; base < R; base += kW * kWave) {
+                for (int b = 0; b < kWave && base + b < R; ++b) {
+                    const uint64_t mk = ballot(lane < ns && lvl[(size_t)(base + b) * kWave + lane] == (uint8_t)(d - 1));
+                    if (lane == 0) Fnew[base + b] = mk;
+                }
+            }
+            cur ^= 1;
+            __syncthreads();
+        }
+        s_far[wid][lane] = far;
+        __syncthreads();
+        if (wid == 0 && lane < ns) {
+            int64_t t = 0;
+#pragma unroll
+            for (int w = 0; w < kW; ++w) t += s_far[w][lane];
+            a.D[s0 + lane] = t;
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace cent

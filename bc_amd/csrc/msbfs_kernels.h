@@ -84,6 +84,11 @@ struct BwdArgs {
     double* __restrict__ coef;
     const uint32_t* __restrict__ npred;
     double* __restrict__ bc;
+    // weighted dependencies (reduced graphs): target weight wgt[w] replaces the
+    // 1 in (1 + delta) / sigma, and lane s's contribution is scaled by its
+    // source's weight wgt[bsrc[s]].  Null: unweighted.
+    const int64_t* __restrict__ wgt;
+    const int* __restrict__ bsrc;
 };
 
 // ---------------------------------------------------------------- gathers
@@ -232,8 +237,13 @@ __device__ __forceinline__ void bwd_vertex_done(const BwdArgs& a, int v, uint64_
         const size_t idx = (size_t)v * kWave + lane;
         const double sg = a.sigma[idx];
         const double dl = sg * acc;
-        a.coef[idx] = (1.0 + dl) / sg;
-        contrib = compat ? dl * (double)a.npred[idx] : dl;
+        if (a.wgt) {
+            a.coef[idx] = ((double)a.wgt[v] + dl) / sg;
+            contrib = dl * (double)a.wgt[a.bsrc[lane]];
+        } else {
+            a.coef[idx] = (1.0 + dl) / sg;
+            contrib = compat ? dl * (double)a.npred[idx] : dl;
+        }
     }
     const double tot = wave_sum(contrib);
     if (lane == 0) a.bc[v] += tot;

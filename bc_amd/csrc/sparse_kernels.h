@@ -126,7 +126,7 @@ __global__ __launch_bounds__(kBlock) void k_sp_pull(SpArgs a) {
     uint64_t* Fn = a.Fd2 + (size_t)a.n * ((d + 1) & 1);
     const int nc = a.ctl[kCtlNCand];
     const int qbase = a.qoff[d + 1];
-    FwdArgs f;  // the shared per-vertex epilogue works on FwdArgs
+    FwdArgs f{};  // the shared per-vertex epilogue works on FwdArgs
     f.n = a.n; f.rp = a.rp; f.col = a.col; f.vis = a.vis; f.Fd = Fd; f.Fn = Fn; f.sigma = a.sigma;
     f.npred = a.npred; f.dist_sum = a.dist_sum; f.weight = a.weight; f.A = A; f.dnext = d + 1;
     FwdAccum r;
@@ -227,7 +227,7 @@ __global__ __launch_bounds__(kBlock) void k_sp_bwd(SpArgs a) {
     const int nw = gridDim.x * kWavesPerBlock;
     const uint64_t* Fn = a.Fd2 + (size_t)a.n * ((d + 1) & 1);
     uint64_t* Fo = a.Fd2 + (size_t)a.n * (d & 1);
-    BwdArgs b;
+    BwdArgs b{};
     b.sigma = a.sigma; b.coef = a.coef; b.npred = a.npred; b.bc = a.bc;
     const int beg = a.qoff[d], end = a.qoff[d + 1];
     for (int base = beg + gw * kWave; base < end; base += nw * kWave) {

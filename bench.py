@@ -76,6 +76,15 @@ def extra_blocktree(ctx, args):
     tplain = timed(lambda: ctx.cc(dg, None), 1)
     log(f"blocktree: persistent {tp * 1e3:.2f} ms, per-item {ti * 1e3:.1f} ms, "
         f"plain all-source MS-BFS {tplain * 1e3:.1f} ms")
+    # BC + closeness of every vertex through the reduction
+    bp, fp = ctx.bc_reduced(red, bc_amd.CENT_SCHED_PERSISTENT)
+    bi, fi = ctx.bc_reduced(red, bc_amd.CENT_SCHED_PER_ITEM)
+    assert (fp == far_p).all() and (fi == far_p).all()
+    import numpy as np
+    assert np.allclose(bp, bi, rtol=1e-9, atol=1e-9 * max(1.0, float(np.abs(bp).max())))
+    tbp = timed(lambda: ctx.bc_reduced(red, bc_amd.CENT_SCHED_PERSISTENT), 3)
+    tbi = timed(lambda: ctx.bc_reduced(red, bc_amd.CENT_SCHED_PER_ITEM), 1)
+    log(f"blocktree BC+CC: persistent {tbp * 1e3:.2f} ms, per-item {tbi * 1e3:.1f} ms")
     n, m = g.n, g.m
     res.update({
         "workload": "all-source closeness, block tree 2000 blocks (D1 + AP reduced), seed 3",
@@ -86,6 +95,9 @@ def extra_blocktree(ctx, args):
         "queries_per_s_persistent": round(1.0 / tp, 2), "queries_per_s_per_item": round(1.0 / ti, 2),
         "speedup_vs_per_item": round(ti / tp, 2),
         "effective_TEPS_persistent": round(n * m / tp, 1),
+        "bc_cc_persistent_ms": round(tbp * 1e3, 3), "bc_cc_per_item_ms": round(tbi * 1e3, 3),
+        "bc_cc_speedup_vs_per_item": round(tbi / tbp, 2),
+        "bc_cc_effective_TEPS_persistent": round(n * m / tbp, 1),
     })
     return res
 

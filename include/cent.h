@@ -254,6 +254,14 @@ void cent_reduced_drop_device(cent_reduced* r);
 int  cent_cc_reduced(cent_ctx* ctx, const cent_reduced* r, int64_t* farness, int32_t sched,
                      const cent_opts* opts);
 
+/* Betweenness of every vertex over all sources (textbook formula, ordered
+ * pairs; bc accumulated like computeBC) through the reduced form, plus the
+ * farness of every vertex (assigned, may be null) from the same pass.
+ * Equals cent_bc + cent_cc over all sources on every graph.  No reference
+ * counterpart (the reference reduces for closeness only). */
+int  cent_bc_reduced(cent_ctx* ctx, const cent_reduced* r, double* bc, int64_t* farness,
+                     int32_t sched, const cent_opts* opts);
+
 /* ---------------------------------------------------------------- serving */
 /*
  * Query server: BC / closeness requests on registered graphs, continuously

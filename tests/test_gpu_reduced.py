@@ -63,3 +63,42 @@ def test_reduced_power_law_and_grid(ctx):
     # giant block with hub rows + many components, trees and pendants
     check(ctx, bc_amd.gen_rmat(13, 8 << 13, seed=12, lcc=False), "rmat13-all-components")
     check(ctx, bc_amd.gen_grid(40, 40, 0.1, 4, seed=4), "grid40")
+
+
+def check_bc(ctx, g, what):
+    print(f"[gpu] reduced bc {what}: n={g.n} m={g.m}", flush=True)
+    red = bc_amd.Reduced(g)
+    want, wfar, _, _ = O.brandes(g, None, threads=min(16, O.max_threads()))
+    scale = max(1.0, float(np.abs(want).max()))
+    out = {}
+    for sched in SCHEDS:
+        bc, far = ctx.bc_reduced(red, sched)
+        np.testing.assert_allclose(bc, want, rtol=1e-9, atol=1e-9 * scale, err_msg=f"{what} sched {sched}")
+        assert np.array_equal(far, wfar), (what, sched)
+        out[sched] = bc
+    return red, out
+
+
+@pytest.mark.parametrize("name", ["karate.txt", "dolphins.txt", "football.txt", "polbooks.txt",
+                                  "testdata.txt", "testdata1.txt", "testdata2.txt",
+                                  "ia-fb-messages.mtx", "musae_git.txt"])
+def test_reduced_bc_fixtures(ctx, name):
+    check_bc(ctx, bc_amd.buildGraph(O.dataset_path(name)), name)
+
+
+def test_reduced_bc_shapes(ctx):
+    # long cycle: one block deeper than the persistent kernel's level limit
+    n = 600
+    check_bc(ctx, bc_amd.CSR.from_edges(n, np.arange(n), (np.arange(n) + 1) % n), "cycle600")
+    check_bc(ctx, bc_amd.gen_blocktree(300, 1.5, 8, 2000, 6.0, 0.3, 0.5, seed=3), "blocktree-300")
+    check_bc(ctx, bc_amd.gen_blocktree(6, 1.5, 2500, 6000, 6.0, 0.3, 0.5, seed=8), "blocktree-big")
+    check_bc(ctx, bc_amd.gen_rmat(12, 8 << 12, seed=12, lcc=False), "rmat12-all-components")
+    check_bc(ctx, bc_amd.gen_grid(30, 30, 0.1, 4, seed=4), "grid30")
+
+
+def test_reduced_bc_deterministic(ctx):
+    g = bc_amd.gen_blocktree(200, 1.5, 8, 3000, 6.0, 0.3, 0.5, seed=11)
+    red = bc_amd.Reduced(g)
+    a = ctx.bc_reduced(red)
+    b = ctx.bc_reduced(red)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
