@@ -9,7 +9,6 @@
 // context's stream, so the per-vertex BC sums are accumulated in a fixed
 // order and the result does not depend on timing.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
@@ -59,10 +58,6 @@ struct cent_ctx {
     DevBuf<int64_t> dist_sum, far, part_far;
     DevBuf<unsigned long long> part_cnt, cnt;
     DevBuf<int> src, flags;
-    // level-packed path counts / coefficients of the dense Brandes path
-    DevBuf<double> sigp, coefp;
-    DevBuf<int> poff, scan, lbase;
-    DevBuf<uint8_t> cub_tmp;
     // frontier-queue mode
     DevBuf<uint64_t> Fd2, QF, Adev;
     DevBuf<int> Q, qoff, cand, stamp, ctl;
@@ -297,8 +292,6 @@ extern "C" void cent_ctx_destroy(cent_ctx* ctx) {
     release(ctx->bc); release(ctx->hp_acc); release(ctx->npred); release(ctx->hp_cnt);
     release(ctx->dist_sum); release(ctx->far); release(ctx->part_far); release(ctx->part_cnt);
     release(ctx->cnt); release(ctx->src); release(ctx->flags);
-    release(ctx->sigp); release(ctx->coefp); release(ctx->poff); release(ctx->scan); release(ctx->lbase);
-    release(ctx->cub_tmp);
     release(ctx->Fd2); release(ctx->QF); release(ctx->Adev); release(ctx->Q); release(ctx->qoff);
     release(ctx->cand); release(ctx->stamp); release(ctx->ctl);
     release(ctx->red_D); release(ctx->red_head);
@@ -327,11 +320,10 @@ extern "C" int64_t cent_ctx_workspace_bytes(const cent_ctx* ctx) {
     return (int64_t)(8 * (ctx->vis.cap + ctx->F.cap + ctx->hp_or.cap + ctx->sigma.cap + ctx->coef.cap +
                           ctx->bc.cap + ctx->hp_acc.cap + ctx->dist_sum.cap + ctx->far.cap + ctx->part_far.cap +
                           ctx->part_cnt.cap + ctx->cnt.cap + ctx->Fd2.cap + ctx->QF.cap + ctx->Adev.cap +
-                          ctx->red_D.cap + ctx->sigp.cap + ctx->coefp.cap + ctx->red_part.cap + ctx->red_bcp.cap + ctx->it_sig.cap +
-                          ctx->it_coef.cap) + ctx->it_lvl.cap + ctx->cub_tmp.cap +
+                          ctx->red_D.cap + ctx->red_part.cap + ctx->red_bcp.cap + ctx->it_sig.cap +
+                          ctx->it_coef.cap) + ctx->it_lvl.cap +
                      4 * (ctx->npred.cap + ctx->hp_cnt.cap + ctx->src.cap + ctx->flags.cap + ctx->Q.cap +
                           ctx->qoff.cap + ctx->cand.cap + ctx->stamp.cap + ctx->ctl.cap + ctx->red_head.cap +
-                          ctx->poff.cap + ctx->scan.cap + ctx->lbase.cap +
                           ctx->red_deep.cap));
 }
 
@@ -343,8 +335,6 @@ extern "C" int cent_ctx_trim(cent_ctx* ctx) {
     release(ctx->bc); release(ctx->hp_acc); release(ctx->npred); release(ctx->hp_cnt);
     release(ctx->dist_sum); release(ctx->far); release(ctx->part_far); release(ctx->part_cnt);
     release(ctx->cnt); release(ctx->src); release(ctx->flags);
-    release(ctx->sigp); release(ctx->coefp); release(ctx->poff); release(ctx->scan); release(ctx->lbase);
-    release(ctx->cub_tmp);
     release(ctx->Fd2); release(ctx->QF); release(ctx->Adev); release(ctx->Q); release(ctx->qoff);
     release(ctx->cand); release(ctx->stamp); release(ctx->ctl);
     release(ctx->red_D); release(ctx->red_head);
@@ -545,13 +535,8 @@ static int ensure_workspace(cent_ctx* ctx, const DevCSR& d, const BatchSpec& sp,
     }
     ctx->F_levels = ctx->F.cap / N - 1;
     if (sp.with_bc) {
-        if (N * kWave >= (size_t)INT32_MAX) return CENT_E_UNSUPPORTED;  // packed offsets are int32
         TRY(ensure(ctx, ctx->sigma, N * kWave));
-        TRY(ensure(ctx, ctx->sigp, N * kWave));
-        TRY(ensure(ctx, ctx->coefp, N * kWave));
-        TRY(ensure(ctx, ctx->poff, N * (ctx->F_levels + 1)));
-        TRY(ensure(ctx, ctx->scan, N));
-        TRY(ensure(ctx, ctx->lbase, ctx->F_levels + 2));
+        TRY(ensure(ctx, ctx->coef, N * kWave));
         if (sp.compat) TRY(ensure(ctx, ctx->npred, N * kWave));
     }
     TRY(ensure(ctx, ctx->part_far, (size_t)G * kWave));
@@ -569,50 +554,14 @@ static int ensure_workspace(cent_ctx* ctx, const DevCSR& d, const BatchSpec& sp,
 static int grow_levels(cent_ctx* ctx, size_t N, int keep_levels) {
     const size_t ncap = ctx->F_levels * 2;
     uint64_t* nf = nullptr;
-    int* np = nullptr;
-    int* nl = nullptr;
     HIPTRY(hipMalloc((void**)&nf, sizeof(uint64_t) * N * (ncap + 1)));
-    HIPTRY(hipMalloc((void**)&np, sizeof(int) * N * (ncap + 1)));
-    HIPTRY(hipMalloc((void**)&nl, sizeof(int) * (ncap + 2)));
     HIPTRY(hipMemcpyAsync(nf, ctx->F.p, sizeof(uint64_t) * N * (size_t)keep_levels, hipMemcpyDeviceToDevice,
                           ctx->stream));
-    if (ctx->poff.p)
-        HIPTRY(hipMemcpyAsync(np, ctx->poff.p, sizeof(int) * N * (size_t)keep_levels, hipMemcpyDeviceToDevice,
-                              ctx->stream));
-    if (ctx->lbase.p)
-        HIPTRY(hipMemcpyAsync(nl, ctx->lbase.p, sizeof(int) * (size_t)(keep_levels + 1), hipMemcpyDeviceToDevice,
-                              ctx->stream));
     HIPTRY(hipStreamSynchronize(ctx->stream));
     HIPTRY(hipFree(ctx->F.p));
-    if (ctx->poff.p) HIPTRY(hipFree(ctx->poff.p));
-    if (ctx->lbase.p) HIPTRY(hipFree(ctx->lbase.p));
     ctx->F.p = nf;
     ctx->F.cap = N * (ncap + 1);
-    ctx->poff.p = np;
-    ctx->poff.cap = N * (ncap + 1);
-    ctx->lbase.p = nl;
-    ctx->lbase.cap = ncap + 2;
     ctx->F_levels = ncap;
-    return CENT_OK;
-}
-
-// Packs level d of the batch (masks F[d] over the view): exclusive scan of
-// the per-vertex lane counts, then absolute offsets and packed path counts.
-static int pack_level(cent_ctx* ctx, const GView& v, int d, size_t N) {
-    hipStream_t s = ctx->stream;
-    const int R = v.hi - v.lo;
-    if (R <= 0) return CENT_OK;
-    const uint64_t* F = ctx->F.p + N * (size_t)d;
-    hipcub::TransformInputIterator<int, PopcOp, const uint64_t*> it(F + v.lo, PopcOp());
-    size_t bytes = 0;
-    HIPTRY(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, it, ctx->scan.p, R, s));
-    TRY(ensure(ctx, ctx->cub_tmp, bytes + 1));
-    HIPTRY(hipcub::DeviceScan::ExclusiveSum(ctx->cub_tmp.p, bytes, it, ctx->scan.p, R, s));
-    const int G = grid_for(v.nchunk, nullptr);
-    k_pack<<<G, kBlock, 0, s>>>(v.nchunk, v.d->chunk_beg + v.c0, v.lo, F, ctx->scan.p, ctx->lbase.p, d,
-                                ctx->poff.p + N * (size_t)d, ctx->sigma.p, ctx->sigp.p);
-    k_pack_base<<<1, 1, 0, s>>>(F, ctx->scan.p, v.lo, v.hi, ctx->lbase.p, d);
-    HIPTRY(hipGetLastError());
     return CENT_OK;
 }
 
@@ -636,10 +585,6 @@ static int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const
     HIPTRY(hipMemsetAsync(ctx->flags.p, 0, sizeof(int) * 2, s));
     k_seed<<<1, kWave, 0, s>>>(d_src, kb, ctx->F.p, ctx->vis.p, sp.with_bc ? ctx->sigma.p : nullptr, sp.with_bc);
     HIPTRY(hipGetLastError());
-    if (sp.with_bc) {
-        HIPTRY(hipMemsetAsync(ctx->lbase.p, 0, sizeof(int), s));
-        TRY(pack_level(ctx, v, 0, N));
-    }
 
     HubArgs ha{};
     int Gseg = 1, Ghub = 1;
@@ -666,8 +611,6 @@ static int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const
         fa.npred = compat ? ctx->npred.p : nullptr;
         fa.dist_sum = sp.dist_sum;
         fa.weight = sp.weight;
-        fa.sigp = bc ? ctx->sigp.p : nullptr;
-        fa.poffd = bc ? ctx->poff.p + N * (size_t)dl : nullptr;
         fa.A = A; fa.dnext = dl + 1;
         fa.part_far = ctx->part_far.p; fa.part_cnt = ctx->part_cnt.p;
         fa.flag = ctx->flags.p; fa.overflow = ctx->flags.p + 1;
@@ -691,7 +634,6 @@ static int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const
             }
             HIPTRY(hipGetLastError());
         }
-        if (bc) TRY(pack_level(ctx, v, dl + 1, N));
         TRY(timer_end(ctx, tm, e0, true));
         ctx->stats.launches_fwd++;
         HIPTRY(hipMemcpyAsync(ctx->h_flags, ctx->flags.p, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
@@ -713,9 +655,6 @@ static int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const
             ba.bc = sp.bc;
             ba.wgt = sp.bwgt;
             ba.bsrc = d_src;
-            ba.coefp = ctx->coefp.p;
-            ba.poffd = ctx->poff.p + N * (size_t)lv;
-            ba.poffn = ctx->poff.p + N * (size_t)(lv + 1);
             hipEvent_t e0;
             TRY(timer_begin(ctx, tm, &e0));
             if (compat) k_bwd<true><<<G, kBlock, 0, s>>>(ba);

@@ -63,10 +63,6 @@ struct FwdArgs {
     uint32_t* __restrict__ npred;         // [n][64], reference-compat BC only
     int64_t* __restrict__ dist_sum;       // [n] or null
     const int64_t* __restrict__ weight;   // [n] vertex weights or null (== 1)
-    // level-packed path counts of level d (dense Brandes): the lanes of
-    // Fd[u] hold sigp[poffd[u] + rank of the lane within Fd[u]]
-    const double* __restrict__ sigp;
-    const int* __restrict__ poffd;
     uint64_t A;                           // active lanes of the batch
     int dnext;                            // d + 1
     int64_t* __restrict__ part_far;       // [grid][64] per-block lane partials
@@ -93,11 +89,6 @@ struct BwdArgs {
     // source's weight wgt[bsrc[s]].  Null: unweighted.
     const int64_t* __restrict__ wgt;
     const int* __restrict__ bsrc;
-    // level-packed coefficients (dense Brandes): level d's are written at
-    // poffd[v] + rank, level d+1's are read at poffn[w] + rank
-    double* __restrict__ coefp;
-    const int* __restrict__ poffd;
-    const int* __restrict__ poffn;
 };
 
 // ---------------------------------------------------------------- gathers
@@ -169,86 +160,6 @@ __device__ __forceinline__ double gather_bwd(const int* __restrict__ col, const 
                     const int wc = rl32(w, c);
                     const uint64_t gc = rl64(g, c);
                     if ((gc >> lane) & 1) x[t] = coef[(size_t)wc * kWave + lane];
-                }
-            }
-#pragma unroll
-            for (int t = 0; t < kUnroll; ++t) acc += x[t];
-        }
-    }
-    return acc;
-}
-
-// ---------------------------------------------------------------- packed gathers
-// Rank of `lane` among the set bits of mask m below it.
-__device__ __forceinline__ int lane_rank(uint64_t m, int lane) {
-    return __popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
-}
-
-// Forward scan over level-packed path counts: a neighbour u at level d keeps
-// only its level-d lanes, contiguously, so a gather touches popc(Fd[u]) x 8
-// bytes instead of a 512-byte row.
-__device__ __forceinline__ void gather_fwd_p(const int* __restrict__ col, const uint64_t* __restrict__ Fd,
-                                             const int* __restrict__ poffd, const double* __restrict__ sigp,
-                                             int beg, int end, uint64_t cand, int lane, double& acc,
-                                             uint32_t& cnt) {
-    for (int j0 = beg; j0 < end; j0 += kWave) {
-        const int j = j0 + lane;
-        int po = 0;
-        uint64_t fm = 0, f = 0;
-        if (j < end) {
-            const int u = col[j];
-            fm = Fd[u];
-            f = fm & cand;
-            if (f) po = poffd[u];
-        }
-        uint64_t ball = ballot(f != 0);
-        while (ball) {
-            double x[kUnroll];
-#pragma unroll
-            for (int t = 0; t < kUnroll; ++t) {
-                x[t] = 0.0;
-                if (ball) {
-                    const int c = __builtin_ctzll(ball);
-                    ball &= ball - 1;
-                    const uint64_t fmc = rl64(fm, c);
-                    if ((fmc & cand) >> lane & 1) {
-                        x[t] = sigp[rl32(po, c) + lane_rank(fmc, lane)];
-                        ++cnt;
-                    }
-                }
-            }
-#pragma unroll
-            for (int t = 0; t < kUnroll; ++t) acc += x[t];
-        }
-    }
-}
-
-// Backward scan over level-packed coefficients of level d+1.
-__device__ __forceinline__ double gather_bwd_p(const int* __restrict__ col, const uint64_t* __restrict__ Fn,
-                                               const int* __restrict__ poffn, const double* __restrict__ coefp,
-                                               int beg, int end, uint64_t m, int lane) {
-    double acc = 0.0;
-    for (int j0 = beg; j0 < end; j0 += kWave) {
-        const int j = j0 + lane;
-        int po = 0;
-        uint64_t fm = 0, g = 0;
-        if (j < end) {
-            const int w = col[j];
-            fm = Fn[w];
-            g = fm & m;
-            if (g) po = poffn[w];
-        }
-        uint64_t ball = ballot(g != 0);
-        while (ball) {
-            double x[kUnroll];
-#pragma unroll
-            for (int t = 0; t < kUnroll; ++t) {
-                x[t] = 0.0;
-                if (ball) {
-                    const int c = __builtin_ctzll(ball);
-                    ball &= ball - 1;
-                    const uint64_t fmc = rl64(fm, c);
-                    if ((fmc & m) >> lane & 1) x[t] = coefp[rl32(po, c) + lane_rank(fmc, lane)];
                 }
             }
 #pragma unroll
@@ -338,27 +249,6 @@ __device__ __forceinline__ void bwd_vertex_done(const BwdArgs& a, int v, uint64_
     if (lane == 0) a.bc[v] += tot;
 }
 
-// Packed form: own sigma from the full row, coefficient written level-packed.
-__device__ __forceinline__ void bwd_vertex_done_p(const BwdArgs& a, int v, uint64_t m, double acc, int lane,
-                                                  bool compat) {
-    double contrib = 0.0;
-    if ((m >> lane) & 1) {
-        const size_t idx = (size_t)v * kWave + lane;
-        const double sg = a.sigma[idx];
-        const double dl = sg * acc;
-        const int q = a.poffd[v] + lane_rank(m, lane);
-        if (a.wgt) {
-            a.coefp[q] = ((double)a.wgt[v] + dl) / sg;
-            contrib = dl * (double)a.wgt[a.bsrc[lane]];
-        } else {
-            a.coefp[q] = (1.0 + dl) / sg;
-            contrib = compat ? dl * (double)a.npred[idx] : dl;
-        }
-    }
-    const double tot = wave_sum(contrib);
-    if (lane == 0) a.bc[v] += tot;
-}
-
 // ---------------------------------------------------------------- forward
 template <bool SIGMA, bool COMPAT>
 __global__ __launch_bounds__(kBlock) void k_fwd(FwdArgs a) {
@@ -389,10 +279,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd(FwdArgs a) {
             double acc = 0.0;
             uint32_t cnt = 0;
             uint64_t orf = 0;
-            if constexpr (SIGMA)
-                gather_fwd_p(a.col, a.Fd, a.poffd, a.sigp, rl32(rbl, b), rl32(rel, b), A & ~old, lane, acc, cnt);
-            else
-                gather_fwd<false>(a.col, a.Fd, nullptr, rl32(rbl, b), rl32(rel, b), A & ~old, lane, acc, cnt, orf);
+            gather_fwd<SIGMA>(a.col, a.Fd, a.sigma, rl32(rbl, b), rl32(rel, b), A & ~old, lane, acc, cnt, orf);
             fwd_vertex_done<SIGMA, COMPAT>(a, v, old, acc, cnt, SIGMA ? 0 : wave_or(orf), lane, r);
         }
     }
@@ -411,10 +298,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_hubseg(FwdArgs a, HubArgs h) {
         double acc = 0.0;
         uint32_t cnt = 0;
         uint64_t orf = 0;
-        if constexpr (SIGMA)
-            gather_fwd_p(a.col, a.Fd, a.poffd, a.sigp, h.seg_beg[i], h.seg_end[i], cand, lane, acc, cnt);
-        else
-            gather_fwd<false>(a.col, a.Fd, nullptr, h.seg_beg[i], h.seg_end[i], cand, lane, acc, cnt, orf);
+        gather_fwd<SIGMA>(a.col, a.Fd, a.sigma, h.seg_beg[i], h.seg_end[i], cand, lane, acc, cnt, orf);
         if constexpr (SIGMA) {
             h.part_acc[(size_t)i * kWave + lane] = acc;
             h.part_cnt[(size_t)i * kWave + lane] = cnt;
@@ -475,8 +359,8 @@ __global__ __launch_bounds__(kBlock) void k_bwd(BwdArgs a) {
             const int b = __builtin_ctzll(work);
             work &= work - 1;
             const uint64_t m = rl64(fl, b);
-            const double acc = gather_bwd_p(a.col, a.Fn, a.poffn, a.coefp, rl32(rbl, b), rl32(rel, b), m, lane);
-            bwd_vertex_done_p(a, base + b, m, acc, lane, COMPAT);
+            const double acc = gather_bwd(a.col, a.Fn, a.coef, rl32(rbl, b), rl32(rel, b), m, lane);
+            bwd_vertex_done(a, base + b, m, acc, lane, COMPAT);
         }
     }
 }
@@ -488,8 +372,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd_hubseg(BwdArgs a, HubArgs h) {
     for (int i = gw; i < h.nseg; i += nw) {
         const uint64_t m = a.Fd[h.seg_v[i]];
         if (!m) continue;
-        h.part_acc[(size_t)i * kWave + lane] =
-            gather_bwd_p(a.col, a.Fn, a.poffn, a.coefp, h.seg_beg[i], h.seg_end[i], m, lane);
+        h.part_acc[(size_t)i * kWave + lane] = gather_bwd(a.col, a.Fn, a.coef, h.seg_beg[i], h.seg_end[i], m, lane);
     }
 }
 
@@ -505,55 +388,9 @@ __global__ __launch_bounds__(kBlock) void k_bwd_hubfin(BwdArgs a, HubArgs h) {
         double acc = 0.0;
         for (int i = h.hub_seg_off[k] - h.seg_base; i < h.hub_seg_off[k + 1] - h.seg_base; ++i)
             acc += h.part_acc[(size_t)i * kWave + lane];
-        bwd_vertex_done_p(a, v, m, acc, lane, COMPAT);
+        bwd_vertex_done(a, v, m, acc, lane, COMPAT);
     }
 }
-
-// ---------------------------------------------------------------- level packing
-// After level d's masks F (and full-row path counts) are final: with
-// scan[v - lo] = exclusive prefix sum of popc(F[v]) over the view, every
-// vertex of the level gets the absolute offset lbase[d] + scan, and its
-// level lanes are copied, in lane order, into sigp.
-__global__ __launch_bounds__(kBlock) void k_pack(int nchunk, const int* __restrict__ chunk_beg, int lo,
-                                                 const uint64_t* __restrict__ F, const int* __restrict__ scan,
-                                                 const int* __restrict__ lbase, int d, int* __restrict__ poff,
-                                                 const double* __restrict__ sigma, double* __restrict__ sigp) {
-    const int lane = lane_id();
-    const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-    const int nw = gridDim.x * kWavesPerBlock;
-    const int base_off = lbase[d];
-    for (int c = gw; c < nchunk; c += nw) {
-        const int base = chunk_beg[c];
-        const int v = base + lane;
-        uint64_t ml = 0;
-        int off = 0;
-        if (v < chunk_beg[c + 1]) {
-            ml = F[v];
-            if (ml) {
-                off = base_off + scan[v - lo];
-                poff[v] = off;
-            }
-        }
-        uint64_t work = ballot(ml != 0);
-        while (work) {
-            const int b = __builtin_ctzll(work);
-            work &= work - 1;
-            const uint64_t m = rl64(ml, b);
-            if ((m >> lane) & 1)
-                sigp[rl32(off, b) + lane_rank(m, lane)] = sigma[(size_t)(base + b) * kWave + lane];
-        }
-    }
-}
-
-// lbase[d + 1] = lbase[d] + total lanes of level d (last scan entry + its popcount).
-__global__ void k_pack_base(const uint64_t* __restrict__ F, const int* __restrict__ scan, int lo, int hi,
-                            int* __restrict__ lbase, int d) {
-    lbase[d + 1] = lbase[d] + scan[hi - 1 - lo] + __popcll(F[hi - 1]);
-}
-
-struct PopcOp {
-    __host__ __device__ int operator()(const uint64_t& x) const { return __builtin_popcountll(x); }
-};
 
 // ---------------------------------------------------------------- batch setup
 // Seeds a batch: lane s starts at src[s] (duplicates allowed, each its own lane).
