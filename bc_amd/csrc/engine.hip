@@ -58,7 +58,6 @@ struct cent_ctx {
     DevBuf<int64_t> dist_sum, far, part_far;
     DevBuf<unsigned long long> part_cnt, cnt;
     DevBuf<int> src, flags;
-    DevBuf<uint8_t> S;       // lane-group summaries of F, same level layout
     // frontier-queue mode
     DevBuf<uint64_t> Fd2, QF, Adev;
     DevBuf<int> Q, qoff, cand, stamp, ctl;
@@ -292,7 +291,7 @@ extern "C" void cent_ctx_destroy(cent_ctx* ctx) {
     release(ctx->vis); release(ctx->F); release(ctx->hp_or); release(ctx->sigma); release(ctx->coef);
     release(ctx->bc); release(ctx->hp_acc); release(ctx->npred); release(ctx->hp_cnt);
     release(ctx->dist_sum); release(ctx->far); release(ctx->part_far); release(ctx->part_cnt);
-    release(ctx->cnt); release(ctx->src); release(ctx->flags); release(ctx->S);
+    release(ctx->cnt); release(ctx->src); release(ctx->flags);
     release(ctx->Fd2); release(ctx->QF); release(ctx->Adev); release(ctx->Q); release(ctx->qoff);
     release(ctx->cand); release(ctx->stamp); release(ctx->ctl);
     release(ctx->red_D); release(ctx->red_head);
@@ -322,7 +321,7 @@ extern "C" int64_t cent_ctx_workspace_bytes(const cent_ctx* ctx) {
                           ctx->bc.cap + ctx->hp_acc.cap + ctx->dist_sum.cap + ctx->far.cap + ctx->part_far.cap +
                           ctx->part_cnt.cap + ctx->cnt.cap + ctx->Fd2.cap + ctx->QF.cap + ctx->Adev.cap +
                           ctx->red_D.cap + ctx->red_part.cap + ctx->red_bcp.cap + ctx->it_sig.cap +
-                          ctx->it_coef.cap) + ctx->it_lvl.cap + ctx->S.cap +
+                          ctx->it_coef.cap) + ctx->it_lvl.cap +
                      4 * (ctx->npred.cap + ctx->hp_cnt.cap + ctx->src.cap + ctx->flags.cap + ctx->Q.cap +
                           ctx->qoff.cap + ctx->cand.cap + ctx->stamp.cap + ctx->ctl.cap + ctx->red_head.cap +
                           ctx->red_deep.cap));
@@ -335,7 +334,7 @@ extern "C" int cent_ctx_trim(cent_ctx* ctx) {
     release(ctx->vis); release(ctx->F); release(ctx->hp_or); release(ctx->sigma); release(ctx->coef);
     release(ctx->bc); release(ctx->hp_acc); release(ctx->npred); release(ctx->hp_cnt);
     release(ctx->dist_sum); release(ctx->far); release(ctx->part_far); release(ctx->part_cnt);
-    release(ctx->cnt); release(ctx->src); release(ctx->flags); release(ctx->S);
+    release(ctx->cnt); release(ctx->src); release(ctx->flags);
     release(ctx->Fd2); release(ctx->QF); release(ctx->Adev); release(ctx->Q); release(ctx->qoff);
     release(ctx->cand); release(ctx->stamp); release(ctx->ctl);
     release(ctx->red_D); release(ctx->red_head);
@@ -535,7 +534,6 @@ static int ensure_workspace(cent_ctx* ctx, const DevCSR& d, const BatchSpec& sp,
         ctx->F_n = N;
     }
     ctx->F_levels = ctx->F.cap / N - 1;
-    TRY(ensure(ctx, ctx->S, N * (ctx->F_levels + 1)));
     if (sp.with_bc) {
         TRY(ensure(ctx, ctx->sigma, N * kWave));
         TRY(ensure(ctx, ctx->coef, N * kWave));
@@ -556,19 +554,13 @@ static int ensure_workspace(cent_ctx* ctx, const DevCSR& d, const BatchSpec& sp,
 static int grow_levels(cent_ctx* ctx, size_t N, int keep_levels) {
     const size_t ncap = ctx->F_levels * 2;
     uint64_t* nf = nullptr;
-    uint8_t* ns = nullptr;
     HIPTRY(hipMalloc((void**)&nf, sizeof(uint64_t) * N * (ncap + 1)));
-    HIPTRY(hipMalloc((void**)&ns, N * (ncap + 1)));
     HIPTRY(hipMemcpyAsync(nf, ctx->F.p, sizeof(uint64_t) * N * (size_t)keep_levels, hipMemcpyDeviceToDevice,
                           ctx->stream));
-    HIPTRY(hipMemcpyAsync(ns, ctx->S.p, N * (size_t)keep_levels, hipMemcpyDeviceToDevice, ctx->stream));
     HIPTRY(hipStreamSynchronize(ctx->stream));
     HIPTRY(hipFree(ctx->F.p));
-    HIPTRY(hipFree(ctx->S.p));
     ctx->F.p = nf;
     ctx->F.cap = N * (ncap + 1);
-    ctx->S.p = ns;
-    ctx->S.cap = N * (ncap + 1);
     ctx->F_levels = ncap;
     return CENT_OK;
 }
@@ -588,13 +580,10 @@ static int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const
     HIPTRY(hipMemsetAsync(ctx->vis.p + v.lo, 0, sizeof(uint64_t) * R, s));
     HIPTRY(hipMemsetAsync(ctx->F.p + v.lo, 0, sizeof(uint64_t) * R, s));
     HIPTRY(hipMemsetAsync(ctx->F.p + N + v.lo, 0, sizeof(uint64_t) * R, s));
-    HIPTRY(hipMemsetAsync(ctx->S.p + v.lo, 0, R, s));
-    HIPTRY(hipMemsetAsync(ctx->S.p + N + v.lo, 0, R, s));
     HIPTRY(hipMemsetAsync(ctx->part_far.p, 0, sizeof(int64_t) * (size_t)G * kWave, s));
     HIPTRY(hipMemsetAsync(ctx->part_cnt.p, 0, sizeof(unsigned long long) * (size_t)G * 2, s));
     HIPTRY(hipMemsetAsync(ctx->flags.p, 0, sizeof(int) * 2, s));
-    k_seed<<<1, kWave, 0, s>>>(d_src, kb, ctx->F.p, ctx->vis.p, sp.with_bc ? ctx->sigma.p : nullptr, sp.with_bc,
-                               ctx->S.p);
+    k_seed<<<1, kWave, 0, s>>>(d_src, kb, ctx->F.p, ctx->vis.p, sp.with_bc ? ctx->sigma.p : nullptr, sp.with_bc);
     HIPTRY(hipGetLastError());
 
     HubArgs ha{};
@@ -613,17 +602,11 @@ static int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const
         if (bc && (size_t)(dl + 2) > ctx->F_levels) TRY(grow_levels(ctx, N, dl + 1));
         const uint64_t* Fd = ctx->F.p + N * (size_t)(bc ? dl : (dl & 1));
         uint64_t* Fn = ctx->F.p + N * (size_t)(bc ? dl + 1 : ((dl + 1) & 1));
-        const uint8_t* Sd = ctx->S.p + N * (size_t)(bc ? dl : (dl & 1));
-        uint8_t* Sn = ctx->S.p + N * (size_t)(bc ? dl + 1 : ((dl + 1) & 1));
-        if (!bc || dl + 1 > 1) {
-            HIPTRY(hipMemsetAsync(Fn + v.lo, 0, sizeof(uint64_t) * R, s));
-            HIPTRY(hipMemsetAsync(Sn + v.lo, 0, R, s));
-        }
+        if (!bc || dl + 1 > 1) HIPTRY(hipMemsetAsync(Fn + v.lo, 0, sizeof(uint64_t) * R, s));
         HIPTRY(hipMemsetAsync(ctx->flags.p, 0, sizeof(int), s));
-        FwdArgs fa{};
+        FwdArgs fa;
         fa.n = d.n; fa.hub_deg = d.hub_deg; fa.nchunk = v.nchunk; fa.chunk_beg = d.chunk_beg + v.c0;
         fa.rp = d.rp; fa.col = d.col; fa.vis = ctx->vis.p; fa.Fd = Fd; fa.Fn = Fn;
-        fa.Sd = Sd; fa.Sn = Sn;
         fa.sigma = bc ? ctx->sigma.p : nullptr;
         fa.npred = compat ? ctx->npred.p : nullptr;
         fa.dist_sum = sp.dist_sum;
@@ -662,12 +645,11 @@ static int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const
     *levels += dl;  // deepest non-empty level
     if (bc) {
         for (int lv = dl; lv >= 1; --lv) {
-            BwdArgs ba{};
+            BwdArgs ba;
             ba.n = d.n; ba.hub_deg = d.hub_deg; ba.nchunk = v.nchunk; ba.chunk_beg = d.chunk_beg + v.c0;
             ba.rp = d.rp; ba.col = d.col;
             ba.Fd = ctx->F.p + N * (size_t)lv;
             ba.Fn = ctx->F.p + N * (size_t)(lv + 1);
-            ba.Sn = ctx->S.p + N * (size_t)(lv + 1);
             ba.sigma = ctx->sigma.p; ba.coef = ctx->coef.p;
             ba.npred = compat ? ctx->npred.p : nullptr;
             ba.bc = sp.bc;

@@ -49,16 +49,6 @@ struct HubArgs {
     uint64_t* __restrict__ part_or;       // [nseg]
 };
 
-// Lane-group summary of a mask: bit g is set when any of lanes 8g..8g+7 is.
-// One byte per vertex per level, so the per-edge frontier test reads a
-// 1-byte array that stays in L2 and loads the 8-byte mask only on a hit.
-__device__ __forceinline__ uint32_t group_summary(uint64_t m) {
-    uint32_t s = 0;
-#pragma unroll
-    for (int g = 0; g < 8; ++g) s |= (uint32_t)(((m >> (8 * g)) & 0xffull) != 0) << g;
-    return s;
-}
-
 struct FwdArgs {
     int n;
     int hub_deg;                          // rows longer than this are hub rows
@@ -69,8 +59,6 @@ struct FwdArgs {
     uint64_t* __restrict__ vis;
     const uint64_t* __restrict__ Fd;      // frontier at distance d
     uint64_t* __restrict__ Fn;            // frontier at distance d + 1 (zeroed)
-    const uint8_t* __restrict__ Sd;       // group summaries of Fd (or null: no filter)
-    uint8_t* __restrict__ Sn;             // group summaries of Fn (or null)
     double* __restrict__ sigma;           // [n][64]
     uint32_t* __restrict__ npred;         // [n][64], reference-compat BC only
     int64_t* __restrict__ dist_sum;       // [n] or null
@@ -92,7 +80,6 @@ struct BwdArgs {
     const int* __restrict__ col;
     const uint64_t* __restrict__ Fd;      // level d masks
     const uint64_t* __restrict__ Fn;      // level d + 1 masks
-    const uint8_t* __restrict__ Sn;       // group summaries of Fn (or null: no filter)
     const double* __restrict__ sigma;
     double* __restrict__ coef;
     const uint32_t* __restrict__ npred;
@@ -109,18 +96,16 @@ struct BwdArgs {
 // neighbours u with bit s in Fd[u] & cand, or ORs the masks (closeness only).
 template <bool SIGMA>
 __device__ __forceinline__ void gather_fwd(const int* __restrict__ col, const uint64_t* __restrict__ Fd,
-                                           const uint8_t* __restrict__ Sd,
                                            const double* __restrict__ sigma, int beg, int end,
                                            uint64_t cand, int lane, double& acc, uint32_t& cnt,
                                            uint64_t& orf) {
-    const uint32_t cs = group_summary(cand);
     for (int j0 = beg; j0 < end; j0 += kWave) {
         const int j = j0 + lane;
         int u = 0;
         uint64_t f = 0;
         if (j < end) {
             u = col[j];
-            if (!Sd || (Sd[u] & cs)) f = Fd[u] & cand;
+            f = Fd[u] & cand;
         }
         if constexpr (SIGMA) {
             uint64_t ball = ballot(f != 0);
@@ -152,18 +137,16 @@ __device__ __forceinline__ void gather_fwd(const int* __restrict__ col, const ui
 // Backward scan of edges [beg, end): lane s sums coef[w][s] over neighbours w
 // with bit s in Fn[w] & m.
 __device__ __forceinline__ double gather_bwd(const int* __restrict__ col, const uint64_t* __restrict__ Fn,
-                                             const uint8_t* __restrict__ Sn,
                                              const double* __restrict__ coef, int beg, int end,
                                              uint64_t m, int lane) {
     double acc = 0.0;
-    const uint32_t ms = group_summary(m);
     for (int j0 = beg; j0 < end; j0 += kWave) {
         const int j = j0 + lane;
         int w = 0;
         uint64_t g = 0;
         if (j < end) {
             w = col[j];
-            if (!Sn || (Sn[w] & ms)) g = Fn[w] & m;
+            g = Fn[w] & m;
         }
         uint64_t ball = ballot(g != 0);
         while (ball) {
@@ -211,7 +194,6 @@ __device__ __forceinline__ void fwd_vertex_done(const FwdArgs& a, int v, uint64_
     }
     if (lane == 0) {
         a.Fn[v] = nm;
-        if (a.Sn) a.Sn[v] = (uint8_t)group_summary(nm);
         a.vis[v] = old | nm;
         const int pc = __popcll(nm);
         if (a.dist_sum) a.dist_sum[v] += (int64_t)a.dnext * pc;
@@ -297,7 +279,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd(FwdArgs a) {
             double acc = 0.0;
             uint32_t cnt = 0;
             uint64_t orf = 0;
-            gather_fwd<SIGMA>(a.col, a.Fd, a.Sd, a.sigma, rl32(rbl, b), rl32(rel, b), A & ~old, lane, acc, cnt, orf);
+            gather_fwd<SIGMA>(a.col, a.Fd, a.sigma, rl32(rbl, b), rl32(rel, b), A & ~old, lane, acc, cnt, orf);
             fwd_vertex_done<SIGMA, COMPAT>(a, v, old, acc, cnt, SIGMA ? 0 : wave_or(orf), lane, r);
         }
     }
@@ -316,7 +298,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_hubseg(FwdArgs a, HubArgs h) {
         double acc = 0.0;
         uint32_t cnt = 0;
         uint64_t orf = 0;
-        gather_fwd<SIGMA>(a.col, a.Fd, a.Sd, a.sigma, h.seg_beg[i], h.seg_end[i], cand, lane, acc, cnt, orf);
+        gather_fwd<SIGMA>(a.col, a.Fd, a.sigma, h.seg_beg[i], h.seg_end[i], cand, lane, acc, cnt, orf);
         if constexpr (SIGMA) {
             h.part_acc[(size_t)i * kWave + lane] = acc;
             h.part_cnt[(size_t)i * kWave + lane] = cnt;
@@ -377,7 +359,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd(BwdArgs a) {
             const int b = __builtin_ctzll(work);
             work &= work - 1;
             const uint64_t m = rl64(fl, b);
-            const double acc = gather_bwd(a.col, a.Fn, a.Sn, a.coef, rl32(rbl, b), rl32(rel, b), m, lane);
+            const double acc = gather_bwd(a.col, a.Fn, a.coef, rl32(rbl, b), rl32(rel, b), m, lane);
             bwd_vertex_done(a, base + b, m, acc, lane, COMPAT);
         }
     }
@@ -390,7 +372,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd_hubseg(BwdArgs a, HubArgs h) {
     for (int i = gw; i < h.nseg; i += nw) {
         const uint64_t m = a.Fd[h.seg_v[i]];
         if (!m) continue;
-        h.part_acc[(size_t)i * kWave + lane] = gather_bwd(a.col, a.Fn, a.Sn, a.coef, h.seg_beg[i], h.seg_end[i], m, lane);
+        h.part_acc[(size_t)i * kWave + lane] = gather_bwd(a.col, a.Fn, a.coef, h.seg_beg[i], h.seg_end[i], m, lane);
     }
 }
 
@@ -413,17 +395,13 @@ __global__ __launch_bounds__(kBlock) void k_bwd_hubfin(BwdArgs a, HubArgs h) {
 // ---------------------------------------------------------------- batch setup
 // Seeds a batch: lane s starts at src[s] (duplicates allowed, each its own lane).
 __global__ void k_seed(const int* __restrict__ src, int k, uint64_t* __restrict__ F0,
-                       uint64_t* __restrict__ vis, double* __restrict__ sigma, bool with_sigma,
-                       uint8_t* __restrict__ S0) {
+                       uint64_t* __restrict__ vis, double* __restrict__ sigma, bool with_sigma) {
     const int lane = threadIdx.x;
-    const int s = (lane < k) ? src[lane] : 0;
-    if (lane < k) {
-        atomicOr((unsigned long long*)&F0[s], 1ull << lane);
-        atomicOr((unsigned long long*)&vis[s], 1ull << lane);
-        if (with_sigma) sigma[(size_t)s * kWave + lane] = 1.0;
-    }
-    __syncthreads();
-    if (lane < k && S0) S0[s] = (uint8_t)group_summary(F0[s]);
+    if (lane >= k) return;
+    const int s = src[lane];
+    atomicOr((unsigned long long*)&F0[s], 1ull << lane);
+    atomicOr((unsigned long long*)&vis[s], 1ull << lane);
+    if (with_sigma) sigma[(size_t)s * kWave + lane] = 1.0;
 }
 
 // Sums the per-block lane partials of a batch into far[0..k) and the counters.
