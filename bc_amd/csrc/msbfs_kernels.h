@@ -33,7 +33,7 @@ namespace cent {
 
 constexpr int kBlock = 256;               // 4 waves per workgroup
 constexpr int kWavesPerBlock = kBlock / kWave;
-constexpr int kUnroll = 4;                // row gathers in flight per wave
+constexpr int kUnroll = 8;                // row gathers in flight per wave
 
 struct HubArgs {
     int nseg;                       // number of hub edge segments
