@@ -319,6 +319,89 @@ _lib.computeBC_compat.argtypes = [_P_CSR, _F64P]
 _lib.computeCC.argtypes = [_P_CSR, _I64P]
 
 
+class TuneParams(C.Structure):
+    """Launch settings of the plain form (``cent_tune_params``)."""
+    _fields_ = [("hub_deg", C.c_int32), ("chunk_edges", C.c_int32), ("grid_waves", C.c_int32),
+                ("frontier", C.c_int32), ("ms", C.c_double)]
+
+    def as_dict(self) -> dict:
+        d = {f: getattr(self, f) for f, _ in self._fields_}
+        d["frontier"] = {v: k for k, v in FRONTIER.items()}.get(self.frontier, self.frontier)
+        return d
+
+
+class TuneSig(C.Structure):
+    """Graph signature the tuning table is keyed on (``cent_tune_sig``)."""
+    _fields_ = [("lg_n", C.c_int32), ("deg_x2", C.c_int32), ("skew", C.c_int32), ("lg_ecc", C.c_int32)]
+
+    def as_tuple(self) -> tuple:
+        return (self.lg_n, self.deg_x2, self.skew, self.lg_ecc)
+
+
+CENT_TUNE_BC = 1
+CENT_TUNE_CC = 2
+CENT_GRAPH_NO_TUNE = 2
+_lib.cent_graph_get_layout.argtypes = [C.c_void_p, C.POINTER(TuneParams)]
+_lib.cent_graph_set_layout.argtypes = [C.c_void_p, C.POINTER(TuneParams)]
+_lib.cent_graph_signature.argtypes = [C.c_void_p, C.POINTER(TuneSig)]
+_lib.cent_tune.argtypes = [C.c_void_p, C.c_void_p, _I32P, C.c_int32, C.c_int32, C.c_int32,
+                           C.POINTER(TuneParams)]
+_lib.cent_tune_signature.argtypes = [_I32P, C.c_int32, C.c_int64, C.c_int32, C.POINTER(TuneSig)]
+_lib.cent_tune_params_check.argtypes = [C.POINTER(TuneParams)]
+_lib.cent_tune_table_put.argtypes = [C.POINTER(TuneSig), C.POINTER(TuneParams)]
+_lib.cent_tune_table_get.argtypes = [C.POINTER(TuneSig), C.POINTER(TuneParams)]
+_lib.cent_tune_table_size.restype = C.c_int32
+_lib.cent_tune_table_clear.restype = None
+_lib.cent_tune_table_load.argtypes = [C.c_char_p]
+_lib.cent_tune_table_save.argtypes = [C.c_char_p]
+
+#: Tuning table measured on MI355X for the bundled workloads (loaded by
+#: ``tune_table_load()`` with no argument).
+TUNE_TABLE = os.path.join(_HERE, "tune_mi355x.txt")
+
+
+def tune_signature(csr: "CSR", ecc: int) -> tuple:
+    """Signature of a host CSR given its BFS-depth probe (host only)."""
+    sig = TuneSig()
+    _check(_lib.cent_tune_signature(_ptr(csr.row_ptr, C.c_int32), csr.n, csr.nnz, ecc, C.byref(sig)),
+           "cent_tune_signature")
+    return sig.as_tuple()
+
+
+def tune_params(hub_deg=256, chunk_edges=1024, grid_waves=0, frontier="auto", ms=0.0) -> TuneParams:
+    return TuneParams(hub_deg, chunk_edges, grid_waves, FRONTIER[frontier], ms)
+
+
+def tune_table_put(sig: tuple, p: TuneParams) -> None:
+    _check(_lib.cent_tune_table_put(C.byref(TuneSig(*sig)), C.byref(p)), "cent_tune_table_put")
+
+
+def tune_table_get(sig: tuple):
+    """Entry nearest to ``sig`` (exact, else within 2 buckets), or None."""
+    p = TuneParams()
+    st = _lib.cent_tune_table_get(C.byref(TuneSig(*sig)), C.byref(p))
+    if st == 8:  # CENT_E_NOTFOUND
+        return None
+    _check(st, "cent_tune_table_get")
+    return p
+
+
+def tune_table_size() -> int:
+    return int(_lib.cent_tune_table_size())
+
+
+def tune_table_clear() -> None:
+    _lib.cent_tune_table_clear()
+
+
+def tune_table_load(path: str = None) -> None:
+    _check(_lib.cent_tune_table_load((path or TUNE_TABLE).encode()), "cent_tune_table_load", path or TUNE_TABLE)
+
+
+def tune_table_save(path: str) -> None:
+    _check(_lib.cent_tune_table_save(path.encode()), "cent_tune_table_save", path)
+
+
 def _opts(mode=CENT_BC_TEXTBOOK, timing=False, grid_waves=0, frontier="auto") -> _Opts:
     o = _Opts()
     o.bc_mode = mode
@@ -329,18 +412,34 @@ def _opts(mode=CENT_BC_TEXTBOOK, timing=False, grid_waves=0, frontier="auto") ->
 
 
 class DeviceGraph:
-    def __init__(self, ctx: "Context", csr: CSR, keep_order: bool = False):
+    def __init__(self, ctx: "Context", csr: CSR, keep_order: bool = False, tuned: bool = True):
         self.ctx = ctx
         self.n = csr.n
         self.nnz = csr.nnz
         self._csr = csr
         h = C.c_void_p()
+        flags = (CENT_GRAPH_KEEP_ORDER if keep_order else 0) | (0 if tuned else CENT_GRAPH_NO_TUNE)
         _check(_lib.cent_graph_create(ctx._h, _ptr(csr.row_ptr, C.c_int32),
                                       _ptr(csr.col, C.c_int32) if csr.nnz else None,
-                                      csr.n, csr.nnz, CENT_GRAPH_KEEP_ORDER if keep_order else 0,
-                                      C.byref(h)),
+                                      csr.n, csr.nnz, flags, C.byref(h)),
                "cent_graph_create", ctx.last_error())
         self._h = h
+
+    @property
+    def layout(self) -> TuneParams:
+        p = TuneParams()
+        _check(_lib.cent_graph_get_layout(self._h, C.byref(p)), "cent_graph_get_layout")
+        return p
+
+    def set_layout(self, p: TuneParams) -> None:
+        _check(_lib.cent_graph_set_layout(self._h, C.byref(p)), "cent_graph_set_layout",
+               self.ctx.last_error())
+
+    @property
+    def signature(self) -> tuple:
+        sig = TuneSig()
+        _check(_lib.cent_graph_signature(self._h, C.byref(sig)), "cent_graph_signature")
+        return sig.as_tuple()
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -374,8 +473,17 @@ class Context:
                "cent_probe_copy_bandwidth", self.last_error())
         return out.value
 
-    def upload(self, csr: CSR, keep_order: bool = False) -> DeviceGraph:
-        return DeviceGraph(self, csr, keep_order)
+    def upload(self, csr: CSR, keep_order: bool = False, tuned: bool = True) -> DeviceGraph:
+        return DeviceGraph(self, csr, keep_order, tuned)
+
+    def tune(self, g: DeviceGraph, sources, what: int = CENT_TUNE_BC, max_trials: int = 0) -> TuneParams:
+        """Autotune g's launch settings on up to 256 sample sources; applies and
+        records the fastest (process-wide table) and returns it."""
+        s = np.ascontiguousarray(sources, dtype=np.int32)
+        p = TuneParams()
+        _check(_lib.cent_tune(self._h, g._h, _ptr(s, C.c_int32), len(s), what, max_trials, C.byref(p)),
+               "cent_tune", self.last_error())
+        return p
 
     def stats(self) -> dict:
         s = Stats()

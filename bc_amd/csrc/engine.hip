@@ -118,6 +118,7 @@ struct DevCSR {
     int* rp = nullptr;
     int* col = nullptr;
     int hub_deg = kHubDeg;
+    int chunk_edges = kChunkEdges;
     int nchunk = 0;
     int* chunk_beg = nullptr;
     int nseg = 0, nhub = 0;
@@ -141,13 +142,20 @@ static int upload(cent_ctx* ctx, T** dst, const T* src, size_t n) {
     return CENT_OK;
 }
 
-// bounds: optional sorted block starts (chunks never straddle a boundary)
-static int devcsr_build(cent_ctx* ctx, const int32_t* rp, const int32_t* col, int32_t n, int64_t nnz,
-                        const int32_t* bounds, int32_t nbounds, DevCSR& d) {
-    d.n = n;
-    d.nnz = nnz;
-    TRY(upload(ctx, &d.rp, rp, (size_t)n + 1));
-    TRY(upload(ctx, &d.col, col, (size_t)nnz));
+// Chunk and hub-segment tables of d for rows rp (host copy of the device
+// row offsets), replacing any previous ones.  bounds: optional sorted block
+// starts (chunks never straddle a boundary).
+static int devcsr_schedule(cent_ctx* ctx, const int32_t* rp, const int32_t* bounds, int32_t nbounds,
+                           int hub_deg, int chunk_edges, DevCSR& d) {
+    (void)ctx;
+    const int32_t n = d.n;
+    if (d.chunk_beg) (void)hipFree(d.chunk_beg);
+    if (d.hub_buf) (void)hipFree(d.hub_buf);
+    d.chunk_beg = nullptr;
+    d.hub_buf = nullptr;
+    d.seg_v = d.seg_beg = d.seg_end = d.hub_v = d.hub_seg_off = nullptr;
+    d.hub_deg = hub_deg;
+    d.chunk_edges = chunk_edges;
     // chunks
     std::vector<int>& cb = d.h_chunk_beg;
     cb.clear();
@@ -162,7 +170,7 @@ static int devcsr_build(cent_ctx* ctx, const int32_t* rp, const int32_t* col, in
         }
         int dg = rp[v + 1] - rp[v];
         if (dg > d.hub_deg) dg = 0;
-        if (cnt > 0 && (brk || cnt == kWave || edges + dg > kChunkEdges)) {
+        if (cnt > 0 && (brk || cnt == kWave || edges + dg > d.chunk_edges)) {
             cb.push_back(v);
             cnt = 0;
             edges = 0;
@@ -184,7 +192,7 @@ static int devcsr_build(cent_ctx* ctx, const int32_t* rp, const int32_t* col, in
         for (int x = b; x < e; x += d.hub_deg) {
             sv.push_back(v);
             sb.push_back(x);
-            se.push_back(x + d.hub_deg < e ? x + d.hub_deg : e);
+            se.push_back(e - x > d.hub_deg ? x + d.hub_deg : e);
         }
         d.h_hub_seg_off.push_back((int)sv.size());
     }
@@ -205,6 +213,15 @@ static int devcsr_build(cent_ctx* ctx, const int32_t* rp, const int32_t* col, in
         d.hub_seg_off = d.hub_v + d.nhub;
     }
     return CENT_OK;
+}
+
+static int devcsr_build(cent_ctx* ctx, const int32_t* rp, const int32_t* col, int32_t n, int64_t nnz,
+                        const int32_t* bounds, int32_t nbounds, DevCSR& d) {
+    d.n = n;
+    d.nnz = nnz;
+    TRY(upload(ctx, &d.rp, rp, (size_t)n + 1));
+    TRY(upload(ctx, &d.col, col, (size_t)nnz));
+    return devcsr_schedule(ctx, rp, bounds, nbounds, kHubDeg, kChunkEdges, d);
 }
 
 // A sub-range [lo, hi) of a DevCSR whose boundaries are chunk boundaries.
@@ -245,6 +262,10 @@ struct cent_graph {
     std::vector<int32_t> new_id;      // device id of each caller id (empty: identity)
     int32_t ecc_probe = 0;            // BFS depth from the highest-degree vertex
     int32_t width_probe = 0;          // widest level of that BFS
+    std::vector<int32_t> h_rp;        // device-order row offsets (host), for re-scheduling
+    cent_tune_sig sig{};              // autotuner signature
+    int32_t grid_waves = 0;           // launch defaults when the call's opts leave them open
+    int32_t frontier = CENT_FRONTIER_AUTO;
 };
 
 // Frontier mode for a plain call: the frontier-queue kernels win when BFS runs
@@ -449,7 +470,49 @@ extern "C" int cent_graph_create(cent_ctx* ctx, const int32_t* row_ptr, const in
         cent_graph_free(g);
         return st;
     }
+    g->h_rp.assign(urp, urp + (size_t)n + 1);
+    (void)cent_tune_signature(g->h_rp.data(), n, nnz, g->ecc_probe, &g->sig);
+    cent_tune_params tp;
+    if (!(flags & CENT_GRAPH_NO_TUNE) && cent_tune_table_get(&g->sig, &tp) == CENT_OK) {
+        tp.ms = 0.0;
+        st = cent_graph_set_layout(g, &tp);
+        if (st) {
+            cent_graph_free(g);
+            return st;
+        }
+    }
     *out = g;
+    return CENT_OK;
+}
+
+// ------------------------------------------------------------ tuning hooks
+extern "C" int cent_graph_get_layout(const cent_graph* g, cent_tune_params* out) {
+    if (!g || !out) return CENT_E_INVAL;
+    out->hub_deg = g->d.hub_deg;
+    out->chunk_edges = g->d.chunk_edges;
+    out->grid_waves = g->grid_waves;
+    out->frontier = g->frontier;
+    out->ms = 0.0;
+    return CENT_OK;
+}
+
+extern "C" int cent_graph_set_layout(cent_graph* g, const cent_tune_params* p) {
+    if (!g) return CENT_E_INVAL;
+    TRY(cent_tune_params_check(p));
+    if (p->hub_deg != g->d.hub_deg || p->chunk_edges != g->d.chunk_edges) {
+        cent_ctx* ctx = g->ctx;
+        TRY(check_device(ctx));
+        HIPTRY(hipDeviceSynchronize());  // no launch may still read the old tables
+        TRY(devcsr_schedule(ctx, g->h_rp.data(), nullptr, 0, p->hub_deg, p->chunk_edges, g->d));
+    }
+    g->grid_waves = p->grid_waves;
+    g->frontier = p->frontier;
+    return CENT_OK;
+}
+
+extern "C" int cent_graph_signature(const cent_graph* g, cent_tune_sig* out) {
+    if (!g || !out) return CENT_E_INVAL;
+    *out = g->sig;
     return CENT_OK;
 }
 
@@ -809,6 +872,12 @@ static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
     const int n = g->n;
     const size_t N = (size_t)(n > 0 ? n : 1);
     hipStream_t s = ctx->stream;
+    // the graph's tuned launch settings fill what the call leaves open
+    cent_opts eff{};
+    if (opts) eff = *opts;
+    if (eff.grid_waves == 0) eff.grid_waves = g->grid_waves;
+    if (eff.frontier == CENT_FRONTIER_AUTO) eff.frontier = g->frontier;
+    opts = &eff;
     BatchSpec sp;
     sp.with_bc = with_bc;
     sp.compat = with_bc && bc_mode == CENT_BC_REF_COMPAT;
@@ -928,6 +997,102 @@ extern "C" int cent_cc(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
     int st = run_batches(ctx, g, s, kk, false, 0, nullptr, farness, dist_sum, opts);
     ctx->stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return st;
+}
+
+// ------------------------------------------------------------ autotuner
+// Coordinate descent over the plain form's launch settings: frontier mode
+// first (the frontier-queue step ignores the other knobs, so a win there
+// ends the search), then up to two passes over hub_deg, chunk_edges and
+// grid_waves, each swept with the others held at their best so far.  A trial is the faster of two
+// runs of the sample (the first also warms the workspace).
+namespace {
+struct Tuner {
+    cent_ctx* ctx;
+    cent_graph* g;
+    const int32_t* src;
+    int32_t k;
+    int32_t what;
+    int32_t budget;  // remaining timed runs (< 0: unlimited)
+    std::vector<double> bc;
+    std::vector<int64_t> far;
+
+    int run_once(double* ms) {
+        std::fill(bc.begin(), bc.end(), 0.0);
+        const auto t0 = std::chrono::steady_clock::now();
+        if (what == CENT_TUNE_BC) TRY(cent_bc(ctx, g, src, k, bc.data(), far.data(), nullptr, nullptr));
+        else TRY(cent_cc(ctx, g, src, k, far.data(), nullptr, nullptr));
+        *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return CENT_OK;
+    }
+
+    // Time setting p; *ms = +inf when the budget is spent.
+    int trial(const cent_tune_params& p, double* ms) {
+        *ms = 1e300;
+        if (budget == 0) return CENT_OK;
+        TRY(cent_graph_set_layout(g, &p));
+        double a = 0.0, b = 0.0;
+        TRY(run_once(&a));
+        TRY(run_once(&b));
+        if (budget > 0) --budget;
+        *ms = a < b ? a : b;
+        return CENT_OK;
+    }
+};
+}  // namespace
+
+extern "C" int cent_tune(cent_ctx* ctx, cent_graph* g, const int32_t* src, int32_t k, int32_t what,
+                         int32_t max_trials, cent_tune_params* best) {
+    if (!ctx || !g || !best || g->ctx->device != ctx->device) return CENT_E_INVAL;
+    if (what != CENT_TUNE_BC && what != CENT_TUNE_CC) return CENT_E_INVAL;
+    if (k < 1 || k > 256 || !src || max_trials < 0) return CENT_E_INVAL;
+    for (int32_t i = 0; i < k; ++i)
+        if (src[i] < 0 || src[i] >= g->n) return CENT_E_INVAL;
+    const cent_stats saved = ctx->stats;
+    Tuner t{ctx, g, src, k, what, max_trials > 0 ? max_trials : -1, {}, {}};
+    t.bc.assign((size_t)(g->n > 0 ? g->n : 1), 0.0);
+    t.far.assign((size_t)k, 0);
+    cent_tune_params orig, cur;
+    TRY(cent_graph_get_layout(g, &orig));
+    cur = orig;
+    cur.frontier = CENT_FRONTIER_DENSE;
+    double best_ms = 1e300, ms = 0.0;
+    int st = CENT_OK;
+    auto sweep = [&](int32_t cent_tune_params::*field, std::initializer_list<int32_t> vals) {
+        cent_tune_params keep = cur;
+        for (int32_t v : vals) {
+            if (st) return;
+            cent_tune_params p = cur;
+            p.*field = v;
+            if (p.*field == keep.*field && best_ms < 1e300) continue;  // already timed
+            st = t.trial(p, &ms);
+            if (!st && ms < best_ms) {
+                best_ms = ms;
+                keep = p;
+            }
+        }
+        cur = keep;
+    };
+    sweep(&cent_tune_params::frontier, {CENT_FRONTIER_DENSE, CENT_FRONTIER_SPARSE});
+    for (int pass = 0; pass < 2 && !st && cur.frontier == CENT_FRONTIER_DENSE; ++pass) {
+        const cent_tune_params before = cur;
+        sweep(&cent_tune_params::hub_deg, {128, 256, 512, 1024, 2048, 4096});
+        sweep(&cent_tune_params::chunk_edges, {256, 512, 1024, 2048, 4096});
+        sweep(&cent_tune_params::grid_waves, {4096, 8192, 16384, 32768});
+        if (before.hub_deg == cur.hub_deg && before.chunk_edges == cur.chunk_edges &&
+            before.grid_waves == cur.grid_waves)
+            break;  // a pass that moves nothing is a fixed point
+    }
+    ctx->stats = saved;
+    if (!st && best_ms >= 1e300) st = CENT_E_INVAL;  // budget too small to time anything
+    if (st) {
+        (void)cent_graph_set_layout(g, &orig);  // leave the graph as it came
+        return st;
+    }
+    cur.ms = best_ms;
+    TRY(cent_graph_set_layout(g, &cur));
+    TRY(cent_tune_table_put(&g->sig, &cur));
+    *best = cur;
+    return CENT_OK;
 }
 
 // ------------------------------------------------------------ reduced closeness

@@ -257,6 +257,11 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=-1,
                     help="sources for the CPU baseline (default: 8 per core)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tune", action="store_true",
+                    help="autotune the launch settings on the first 64 sources before timing")
+    ap.add_argument("--tune-save", default="", help="with --tune: write the tuning table here")
+    ap.add_argument("--no-tune-table", action="store_true",
+                    help="ignore the bundled tuning table (engine defaults)")
     ap.add_argument("--keep-order", action="store_true",
                     help="keep the generator's vertex ids on the device (no degree relabel)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -288,8 +293,20 @@ def main() -> int:
     print(f"# graph: R-MAT scale {args.scale} ef {args.edge_factor}: n={g.n} m={g.m} "
           f"(generated in {time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
 
+    # launch settings: the bundled MI355X tuning table (applied at upload by
+    # graph signature), or a fresh tuning run on the first 64 sources
+    if not args.no_tune_table and os.path.exists(bc_amd.TUNE_TABLE):
+        bc_amd.tune_table_load()
     ctx = bc_amd.Context(0)
     dg = ctx.upload(g, keep_order=args.keep_order)
+    if args.tune:
+        t1 = time.perf_counter()
+        best = ctx.tune(dg, src[:64], bc_amd.CENT_TUNE_BC)
+        print(f"# tuned in {time.perf_counter() - t1:.1f}s: {best.as_dict()}", file=sys.stderr, flush=True)
+        if args.tune_save:
+            bc_amd.tune_table_save(args.tune_save)
+    layout = dg.layout.as_dict()
+    layout.pop("ms")
     bc = np.zeros(g.n)
     for i in range(args.warmup):
         ctx.bc(dg, src, bc=bc, dist_sum=True)
@@ -372,7 +389,8 @@ def main() -> int:
                                f"{args.edge_factor} (LCC), {k} sources",
                    "n": n, "m": m, "sources": k, "seed": 2, "global_batch": k,
                    "levels_per_batch": round(st["levels"] / max(1, st["batches"]), 2),
-                   "dag_edges_per_step": int(dag), "parallelism": "single GPU"},
+                   "dag_edges_per_step": int(dag), "parallelism": "single GPU",
+                   "launch": layout},
         "roofline": roofline, "cpu_baseline": cpu,
     }
     if workloads:

@@ -201,6 +201,62 @@ int  cent_bc(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k,
 int  cent_cc(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k,
              int64_t* farness, int64_t* dist_sum, const cent_opts* opts);
 
+/* ------------------------------------------------ autotuner */
+/*
+ * Launch settings of the plain form, per graph (no reference counterpart:
+ * the reference runs one fixed schedule).  The level kernels split rows
+ * longer than hub_deg into hub_deg-edge segments, group the other rows into
+ * chunks of at most 64 vertices and chunk_edges edges, and cap a dense level
+ * launch at grid_waves wavefronts; frontier picks the dense (pull over every
+ * vertex) or the frontier-queue (expand from the frontier) level step.
+ */
+typedef struct cent_tune_params {
+    int32_t hub_deg;      /* 32 .. 2^20                                      */
+    int32_t chunk_edges;  /* 64 .. 2^22                                      */
+    int32_t grid_waves;   /* 0 = engine default (2048 workgroups of 4 waves) */
+    int32_t frontier;     /* CENT_FRONTIER_*; AUTO = upload-time heuristic   */
+    double  ms;           /* measured trial time (cent_tune), else 0         */
+} cent_tune_params;
+
+/* Graph signature the tuning table is keyed on: log2 vertices, 2 x log2
+ * average degree, log2 (max degree / average degree), log2 (BFS depth + 1),
+ * each rounded. */
+typedef struct cent_tune_sig {
+    int32_t lg_n, deg_x2, skew, lg_ecc;
+} cent_tune_sig;
+
+#define CENT_TUNE_BC 1   /* trials time Brandes BC (+ farness)            */
+#define CENT_TUNE_CC 2   /* trials time closeness only                    */
+
+/* Current settings of an uploaded graph / new settings (the chunk and hub
+ * tables are rebuilt on the device when hub_deg or chunk_edges change). */
+int  cent_graph_get_layout(const cent_graph* g, cent_tune_params* out);
+int  cent_graph_set_layout(cent_graph* g, const cent_tune_params* p);
+int  cent_graph_signature(const cent_graph* g, cent_tune_sig* out);
+
+/* Times the plain call `what` on the sample sources src[0..k) (k <= 256) for
+ * candidate settings by coordinate descent (frontier mode, hub_deg,
+ * chunk_edges, grid_waves; at most max_trials timed runs, 0 = no limit),
+ * applies the fastest to g, records it in the process-wide table under g's
+ * signature and returns it in *best.  Results never depend on the settings
+ * beyond floating-point summation order. */
+int  cent_tune(cent_ctx* ctx, cent_graph* g, const int32_t* src, int32_t k, int32_t what,
+               int32_t max_trials, cent_tune_params* best);
+
+/* Process-wide table (thread-safe).  cent_graph_create applies the entry
+ * nearest to the new graph's signature (exact, else within 2 buckets)
+ * unless CENT_GRAPH_NO_TUNE is given.  load merges a text file written by
+ * save; a malformed file returns CENT_E_IO and changes nothing. */
+#define CENT_GRAPH_NO_TUNE 2
+int  cent_tune_signature(const int32_t* row_ptr, int32_t n, int64_t nnz, int32_t ecc, cent_tune_sig* out);
+int  cent_tune_params_check(const cent_tune_params* p);
+int  cent_tune_table_put(const cent_tune_sig* sig, const cent_tune_params* p);
+int  cent_tune_table_get(const cent_tune_sig* sig, cent_tune_params* out);
+int32_t cent_tune_table_size(void);
+void cent_tune_table_clear(void);
+int  cent_tune_table_load(const char* path);
+int  cent_tune_table_save(const char* path);
+
 /* ------------------------------------------------ reduced (D1 + AP) form */
 /*
  * Closeness through degree-1 folding and articulation-point splitting, the
