@@ -239,6 +239,61 @@ def extra_serve(ctx, args):
         res[label] = out
     res["speedup_queries_per_s"] = round(res["served"]["queries_per_s"] /
                                          max(1e-9, res["one_at_a_time"]["queries_per_s"]), 2)
+    if args.serve_plain > 0:
+        res["vs_plain"] = serve_vs_plain(ctx, pool, reqs[:args.serve_plain])
+    return res
+
+
+def serve_vs_plain(ctx, pool, reqs):
+    """Saturation throughput on the same requests: the served form (4-slot
+    server, device-resident graph and reduction cache, persistent item kernel,
+    all requests submitted at once) against the plain per-component-launch
+    form (one request at a time; each uploads its graph, BC through cent_bc,
+    all-vertex closeness through a fresh host D1/AP reduction and one launch
+    per (component, source-block) item).  Results must agree: farness exact,
+    BC within 1e-9 relative."""
+    import numpy as np
+    import bc_amd
+    out_plain = []
+    t0 = time.perf_counter()
+    for j, kind, src in reqs:
+        g = pool[j][1]
+        dg = ctx.upload(g)
+        if kind == bc_amd.CENT_Q_BC:
+            bc, far, _ = ctx.bc(dg, src)
+            out_plain.append((bc, far))
+        else:
+            red = bc_amd.Reduced(g)
+            out_plain.append((None, ctx.cc_reduced(red, sched=bc_amd.CENT_SCHED_PER_ITEM)))
+        del dg
+    t_plain = time.perf_counter() - t0
+    log(f"serve[plain]: {len(reqs)} requests in {t_plain:.2f} s")
+    srv = bc_amd.Server(slots=4, hbm_budget=200 << 30)
+    gids = [srv.add_graph(g) for _, g in pool]
+    for gid in gids:
+        srv.wait(srv.submit(gid, bc_amd.CENT_Q_BC, [0]))
+    t0 = time.perf_counter()
+    rids = [srv.submit(gids[j], kind, src) for j, kind, src in reqs]
+    got = []
+    for rid in rids:
+        srv.wait(rid)
+        st, bc, far, info = srv.result(rid)
+        if st != 0:
+            raise RuntimeError(f"served request failed: status {st}")
+        got.append((bc, far))
+    t_srv = time.perf_counter() - t0
+    srv.close()
+    far_equal, bc_rel = True, 0.0
+    for (pb, pf), (sb, sf) in zip(out_plain, got):
+        far_equal &= bool(np.array_equal(pf, sf))
+        if pb is not None:
+            bc_rel = max(bc_rel, float(np.max(np.abs(pb - sb)) / max(1.0, float(np.abs(pb).max()))))
+    res = {"requests": len(reqs), "plain_s": round(t_plain, 3), "served_s": round(t_srv, 3),
+           "plain_queries_per_s": round(len(reqs) / t_plain, 2),
+           "served_queries_per_s": round(len(reqs) / t_srv, 2),
+           "speedup": round(t_plain / t_srv, 2), "farness_equal": far_equal,
+           "bc_max_rel_diff": bc_rel}
+    log(f"serve[vs plain]: {res}")
     return res
 
 
@@ -274,6 +329,8 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-pool", type=int, default=12)
+    ap.add_argument("--serve-plain", type=int, default=60,
+                    help="requests of the stream timed in the plain form against the server")
     ap.add_argument("--serve-deadline", type=float, default=90.0,
                     help="seconds after the first submission; unfinished requests are cancelled")
     args = ap.parse_args()
