@@ -275,11 +275,13 @@ class Reduced:
 # ---------------------------------------------------------------- engine
 class _Opts(C.Structure):
     _fields_ = [("bc_mode", C.c_int32), ("flags", C.c_int32), ("block_size", C.c_int32),
-                ("grid_waves", C.c_int32), ("frontier", C.c_int32), ("reserved", C.c_int32)]
+                ("grid_waves", C.c_int32), ("frontier", C.c_int32), ("max_slots", C.c_int32)]
 
 
 CENT_FRONTIER_AUTO, CENT_FRONTIER_DENSE, CENT_FRONTIER_SPARSE = 0, 1, 2
-FRONTIER = {"auto": CENT_FRONTIER_AUTO, "dense": CENT_FRONTIER_DENSE, "sparse": CENT_FRONTIER_SPARSE}
+CENT_FRONTIER_SOURCE = 3
+FRONTIER = {"auto": CENT_FRONTIER_AUTO, "dense": CENT_FRONTIER_DENSE, "sparse": CENT_FRONTIER_SPARSE,
+            "source": CENT_FRONTIER_SOURCE}
 
 
 class Stats(C.Structure):
@@ -402,12 +404,13 @@ def tune_table_save(path: str) -> None:
     _check(_lib.cent_tune_table_save(path.encode()), "cent_tune_table_save", path)
 
 
-def _opts(mode=CENT_BC_TEXTBOOK, timing=False, grid_waves=0, frontier="auto") -> _Opts:
+def _opts(mode=CENT_BC_TEXTBOOK, timing=False, grid_waves=0, frontier="auto", max_slots=0) -> _Opts:
     o = _Opts()
     o.bc_mode = mode
     o.flags = CENT_OPT_TIMING if timing else 0
     o.grid_waves = grid_waves
     o.frontier = FRONTIER[frontier]
+    o.max_slots = max_slots
     return o
 
 
@@ -502,13 +505,13 @@ class Context:
 
     def bc(self, g: DeviceGraph, sources=None, mode: int = CENT_BC_TEXTBOOK, bc=None,
            farness: bool = True, dist_sum: bool = False, timing: bool = False,
-           grid_waves: int = 0, frontier: str = "auto"):
+           grid_waves: int = 0, frontier: str = "auto", max_slots: int = 0):
         """Brandes BC from ``sources`` (None: all).  Returns (bc, farness, dist_sum)."""
         s, k, kk = self._src(g, sources)
         out = np.zeros(g.n, np.float64) if bc is None else bc
         far = np.zeros(max(kk, 1), np.int64) if farness else None
         ds = np.zeros(max(g.n, 1), np.int64) if dist_sum else None
-        o = _opts(mode, timing, grid_waves, frontier)
+        o = _opts(mode, timing, grid_waves, frontier, max_slots)
         _check(_lib.cent_bc(self._h, g._h, _ptr(s, C.c_int32) if s is not None and k else None, k,
                             _ptr(out, C.c_double) if g.n else None,
                             _ptr(far, C.c_int64) if far is not None else None,
@@ -517,12 +520,12 @@ class Context:
         return out, (far[:kk] if far is not None else None), (ds[:g.n] if ds is not None else None)
 
     def cc(self, g: DeviceGraph, sources=None, dist_sum: bool = False, timing: bool = False,
-           grid_waves: int = 0, frontier: str = "auto"):
+           grid_waves: int = 0, frontier: str = "auto", max_slots: int = 0):
         """Closeness (farness) from ``sources`` (None: all).  Returns (farness, dist_sum)."""
         s, k, kk = self._src(g, sources)
         far = np.zeros(max(kk, 1), np.int64)
         ds = np.zeros(max(g.n, 1), np.int64) if dist_sum else None
-        o = _opts(0, timing, grid_waves, frontier)
+        o = _opts(0, timing, grid_waves, frontier, max_slots)
         _check(_lib.cent_cc(self._h, g._h, _ptr(s, C.c_int32) if s is not None and k else None, k,
                             _ptr(far, C.c_int64),
                             _ptr(ds, C.c_int64) if ds is not None else None, C.byref(o)),

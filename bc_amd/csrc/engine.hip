@@ -21,6 +21,7 @@
 #include "../Lib/reduce.h"
 #include "items_kernels.h"
 #include "msbfs_kernels.h"
+#include "persource_kernels.h"
 #include "sparse_kernels.h"
 
 using namespace cent;
@@ -67,6 +68,11 @@ struct cent_ctx {
     DevBuf<double> red_part, red_bcp, it_sig, it_coef;
     DevBuf<uint8_t> it_lvl;
     DevBuf<int> red_deep;
+    // source-parallel mode: per-slot arrays
+    DevBuf<PsRec> ps_rec;
+    DevBuf<int> ps_Q, ps_loff;
+    DevBuf<double> ps_bcp;
+    DevBuf<unsigned long long> ps_depth;
     size_t F_levels = 0;     // levels the F buffer holds for the current N
     size_t F_n = 0;
     int* h_flags = nullptr;  // pinned
@@ -92,6 +98,16 @@ static int ensure(cent_ctx* ctx, DevBuf<T>& b, size_t n) {
     size_t cap = n + n / 4 + 64;
     HIPTRY(hipMalloc((void**)&b.p, cap * sizeof(T)));
     b.cap = cap;
+    return CENT_OK;
+}
+
+// Exact-size variant for the large per-slot arrays (no growth slack).
+template <class T>
+static int ensure_exact(cent_ctx* ctx, DevBuf<T>& b, size_t n) {
+    if (n <= b.cap) return CENT_OK;
+    if (b.p) { HIPTRY(hipFree(b.p)); b.p = nullptr; b.cap = 0; }
+    HIPTRY(hipMalloc((void**)&b.p, (n > 0 ? n : 1) * sizeof(T)));
+    b.cap = n > 0 ? n : 1;
     return CENT_OK;
 }
 
@@ -268,17 +284,17 @@ struct cent_graph {
     int32_t frontier = CENT_FRONTIER_AUTO;
 };
 
-// Frontier mode for a plain call: the frontier-queue kernels win when BFS runs
-// many levels and the 64 lanes' frontiers together stay a small part of the
-// graph.  The probe is one BFS from the highest-degree vertex at upload:
-// its depth and its widest level.
-constexpr int kSparseDepth = 48;
+// Level step for a plain call.  Deep graphs (BFS depth above kDeepLevels
+// from the highest-degree vertex, probed at upload) go source-parallel: a
+// 64-lane batch there spreads its lanes over different levels and shares
+// no work.  Otherwise the dense 64-lane pull.  The frontier-queue step is
+// chosen only on request (or by the autotuner).
+constexpr int kDeepLevels = 48;
 
-static bool use_sparse(const cent_graph* g, const cent_opts* o) {
+static int level_mode(const cent_graph* g, const cent_opts* o) {
     const int m = o ? o->frontier : CENT_FRONTIER_AUTO;
-    if (m == CENT_FRONTIER_DENSE) return false;
-    if (m == CENT_FRONTIER_SPARSE) return true;
-    return g->ecc_probe > kSparseDepth && (int64_t)g->width_probe * kWave * 4 < (int64_t)g->n;
+    if (m != CENT_FRONTIER_AUTO) return m;
+    return g->ecc_probe > kDeepLevels ? CENT_FRONTIER_SOURCE : CENT_FRONTIER_DENSE;
 }
 
 extern "C" int cent_ctx_create(int32_t device, cent_ctx** out) {
@@ -318,6 +334,8 @@ extern "C" void cent_ctx_destroy(cent_ctx* ctx) {
     release(ctx->red_D); release(ctx->red_head);
     release(ctx->red_part); release(ctx->red_bcp); release(ctx->it_sig); release(ctx->it_coef);
     release(ctx->it_lvl); release(ctx->red_deep);
+    release(ctx->ps_rec); release(ctx->ps_Q); release(ctx->ps_loff); release(ctx->ps_bcp);
+    release(ctx->ps_depth);
     if (ctx->h_flags) (void)hipHostFree(ctx->h_flags);
     for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
@@ -342,10 +360,11 @@ extern "C" int64_t cent_ctx_workspace_bytes(const cent_ctx* ctx) {
                           ctx->bc.cap + ctx->hp_acc.cap + ctx->dist_sum.cap + ctx->far.cap + ctx->part_far.cap +
                           ctx->part_cnt.cap + ctx->cnt.cap + ctx->Fd2.cap + ctx->QF.cap + ctx->Adev.cap +
                           ctx->red_D.cap + ctx->red_part.cap + ctx->red_bcp.cap + ctx->it_sig.cap +
-                          ctx->it_coef.cap) + ctx->it_lvl.cap +
+                          ctx->it_coef.cap + 2 * ctx->ps_rec.cap + ctx->ps_bcp.cap + ctx->ps_depth.cap) +
+                     ctx->it_lvl.cap +
                      4 * (ctx->npred.cap + ctx->hp_cnt.cap + ctx->src.cap + ctx->flags.cap + ctx->Q.cap +
                           ctx->qoff.cap + ctx->cand.cap + ctx->stamp.cap + ctx->ctl.cap + ctx->red_head.cap +
-                          ctx->red_deep.cap));
+                          ctx->red_deep.cap + ctx->ps_Q.cap + ctx->ps_loff.cap));
 }
 
 extern "C" int cent_ctx_trim(cent_ctx* ctx) {
@@ -361,6 +380,8 @@ extern "C" int cent_ctx_trim(cent_ctx* ctx) {
     release(ctx->red_D); release(ctx->red_head);
     release(ctx->red_part); release(ctx->red_bcp); release(ctx->it_sig); release(ctx->it_coef);
     release(ctx->it_lvl); release(ctx->red_deep);
+    release(ctx->ps_rec); release(ctx->ps_Q); release(ctx->ps_loff); release(ctx->ps_bcp);
+    release(ctx->ps_depth);
     ctx->F_levels = 0;
     ctx->F_n = 0;
     return CENT_OK;
@@ -865,6 +886,78 @@ static int level_batch_sparse(cent_ctx* ctx, SpPlan& pl, const BatchSpec& sp, co
     return CENT_OK;
 }
 
+// ------------------------------------------------------------ source-parallel mode
+// All k sources of a plain call in one launch of k_ps: G workgroup slots,
+// each with its own per-vertex arrays, take sources s, s + G, ... in turn.
+// G is bounded by k, by the slot cap (opts->max_slots, else grid_waves / 8,
+// else 256: more concurrent traversals than that only contend for the
+// memory system) and by a memory budget (40% of free device memory, at most
+// 32 GiB).  Returns CENT_E_UNSUPPORTED, with nothing accumulated into the BC
+// output, if some source's BFS is deeper than the level table: the caller
+// then runs the batch form.
+constexpr int kPsDefaultSlots = 256;
+
+static int run_persource(cent_ctx* ctx, const cent_graph* g, const BatchSpec& sp, int32_t k, KernelTimer& tm,
+                         int64_t* levels) {
+    const DevCSR& d = g->d;
+    hipStream_t s = ctx->stream;
+    const size_t n = (size_t)(d.n > 0 ? d.n : 1);
+    const bool bc = sp.with_bc, compat = sp.compat;
+    if (sp.bwgt || k < 1) return CENT_E_UNSUPPORTED;
+    const int lcap = (int)std::min<size_t>(n + 2, 65536);
+    const size_t per_slot = n * (sizeof(PsRec) + 4 + (bc ? 8 : 0)) + (size_t)lcap * 4;
+    size_t cap = kPsDefaultSlots;
+    if (sp.opts && sp.opts->max_slots > 0) cap = (size_t)sp.opts->max_slots;
+    else if (sp.opts && sp.opts->grid_waves > 0) cap = std::max(1, sp.opts->grid_waves / kPsWaves);
+    size_t G = std::min((size_t)k, cap);
+    size_t freeb = 0, totalb = 0;
+    HIPTRY(hipMemGetInfo(&freeb, &totalb));
+    const size_t have = sizeof(PsRec) * ctx->ps_rec.cap + 4 * (ctx->ps_Q.cap + ctx->ps_loff.cap) +
+                        8 * ctx->ps_bcp.cap;
+    const size_t budget = std::min<size_t>((size_t)32 << 30, (freeb + have) / 5 * 2);
+    G = std::max<size_t>(1, std::min(G, budget / per_slot));
+    TRY(ensure_exact(ctx, ctx->ps_rec, n * G));
+    TRY(ensure_exact(ctx, ctx->ps_Q, n * G));
+    TRY(ensure_exact(ctx, ctx->ps_loff, (size_t)lcap * G));
+    if (bc) TRY(ensure_exact(ctx, ctx->ps_bcp, n * G));
+    TRY(ensure(ctx, ctx->ps_depth, 1));
+    TRY(ensure(ctx, ctx->flags, 2));
+    const size_t total = n * G;
+    const int ig = (int)std::min<size_t>((total + 255) / 256, 16384);
+    k_ps_init<<<ig, 256, 0, s>>>(ctx->ps_rec.p, bc ? ctx->ps_bcp.p : nullptr, total);
+    HIPTRY(hipGetLastError());
+    HIPTRY(hipMemsetAsync(ctx->ps_depth.p, 0, sizeof(unsigned long long), s));
+    HIPTRY(hipMemsetAsync(ctx->flags.p, 0, sizeof(int) * 2, s));
+    PsArgs a{};
+    a.n = d.n; a.rp = d.rp; a.col = d.col; a.src = ctx->src.p; a.k = k;
+    a.rec = ctx->ps_rec.p; a.Q = ctx->ps_Q.p;
+    a.loff = ctx->ps_loff.p; a.lcap = lcap;
+    a.bcp = bc ? ctx->ps_bcp.p : nullptr;
+    a.far = ctx->far.p; a.dist_sum = sp.dist_sum; a.weight = sp.weight;
+    a.cnt = ctx->cnt.p; a.depth_sum = ctx->ps_depth.p; a.overflow = ctx->flags.p;
+    hipEvent_t e0;
+    TRY(timer_begin(ctx, tm, &e0));
+    if (!bc) k_ps<false, false><<<(int)G, kPsBlock, 0, s>>>(a);
+    else if (compat) k_ps<true, true><<<(int)G, kPsBlock, 0, s>>>(a);
+    else k_ps<true, false><<<(int)G, kPsBlock, 0, s>>>(a);
+    HIPTRY(hipGetLastError());
+    TRY(timer_end(ctx, tm, e0, true));
+    ctx->stats.launches_fwd++;
+    unsigned long long depth = 0;
+    HIPTRY(hipMemcpyAsync(ctx->h_flags, ctx->flags.p, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+    HIPTRY(hipMemcpyAsync(&depth, ctx->ps_depth.p, sizeof(depth), hipMemcpyDeviceToHost, s));
+    HIPTRY(hipStreamSynchronize(s));
+    if (ctx->h_flags[0] & 2) return CENT_E_UNSUPPORTED;
+    if (ctx->h_flags[0] & 1) return CENT_E_OVERFLOW;
+    if (bc) {
+        k_ps_reduce<<<(int)((n + 255) / 256), 256, 0, s>>>(ctx->ps_bcp.p, (int)G, d.n, sp.bc);
+        HIPTRY(hipGetLastError());
+    }
+    *levels += (int64_t)depth;
+    ctx->stats.batches += k;
+    return CENT_OK;
+}
+
 // Runs all batches of one plain call.  with_bc: Brandes (sigma + backward);
 // otherwise closeness only (bitmask BFS).
 static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k, bool with_bc,
@@ -911,10 +1004,23 @@ static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
     ctx->ev_used = 0;
     const GView view = full_view(g->d);
     int64_t levels = 0;
-    const bool sparse = use_sparse(g, opts) && k > 0;
+    const int lmode = k > 0 ? level_mode(g, opts) : CENT_FRONTIER_DENSE;
+    bool batched = true;
+    if (lmode == CENT_FRONTIER_SOURCE) {
+        const int st = run_persource(ctx, g, sp, k, tm, &levels);
+        if (st == CENT_OK) {
+            batched = false;
+        } else if (st == CENT_E_UNSUPPORTED) {  // too deep for the level table: redo as batches
+            HIPTRY(hipMemsetAsync(ctx->cnt.p, 0, sizeof(unsigned long long) * 2, s));
+            if (sp.dist_sum) HIPTRY(hipMemsetAsync(sp.dist_sum, 0, sizeof(int64_t) * N, s));
+        } else {
+            return st;
+        }
+    }
+    const bool sparse = batched && lmode == CENT_FRONTIER_SPARSE;
     SpPlan plan;
     if (sparse) TRY(sp_prepare(ctx, g->d, sp, plan));
-    for (int32_t b0 = 0; b0 < k; b0 += kWave) {
+    for (int32_t b0 = 0; batched && b0 < k; b0 += kWave) {
         const int kb = (k - b0) < kWave ? (k - b0) : kWave;
         if (sparse) TRY(level_batch_sparse(ctx, plan, sp, ctx->src.p + b0, kb, ctx->far.p + b0, tm, &levels));
         else TRY(level_batch(ctx, view, sp, ctx->src.p + b0, kb, ctx->far.p + b0, tm, &levels));
@@ -1000,10 +1106,11 @@ extern "C" int cent_cc(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
 }
 
 // ------------------------------------------------------------ autotuner
-// Coordinate descent over the plain form's launch settings: frontier mode
-// first (the frontier-queue step ignores the other knobs, so a win there
-// ends the search), then up to two passes over hub_deg, chunk_edges and
-// grid_waves, each swept with the others held at their best so far.  A trial is the faster of two
+// Coordinate descent over the plain form's launch settings: level step
+// first; the source-parallel step then only sweeps its slot count
+// (grid_waves / 8) and the frontier-queue step nothing; the dense step gets
+// up to two passes over hub_deg, chunk_edges and grid_waves, each swept with
+// the others held at their best so far.  A trial is the faster of two
 // runs of the sample (the first also warms the workspace).
 namespace {
 struct Tuner {
@@ -1072,7 +1179,13 @@ extern "C" int cent_tune(cent_ctx* ctx, cent_graph* g, const int32_t* src, int32
         }
         cur = keep;
     };
-    sweep(&cent_tune_params::frontier, {CENT_FRONTIER_DENSE, CENT_FRONTIER_SPARSE});
+    sweep(&cent_tune_params::frontier, {CENT_FRONTIER_DENSE, CENT_FRONTIER_SPARSE, CENT_FRONTIER_SOURCE});
+    if (!st && cur.frontier == CENT_FRONTIER_SOURCE) {
+        // grid_waves / 8 = concurrent source slots; caps above the sample size
+        // time the same as the sample size, so only smaller ones are tried
+        for (int32_t gw : {kPsWaves * 64, kPsWaves * 128})
+            if (gw / kPsWaves <= k) sweep(&cent_tune_params::grid_waves, {gw});
+    }
     for (int pass = 0; pass < 2 && !st && cur.frontier == CENT_FRONTIER_DENSE; ++pass) {
         const cent_tune_params before = cur;
         sweep(&cent_tune_params::hub_deg, {128, 256, 512, 1024, 2048, 4096});

@@ -1,0 +1,418 @@
+// Source-parallel level kernels: one workgroup runs the whole Brandes (or
+// closeness) traversal of one source at a time, level by level with
+// workgroup barriers, so a call is a single launch with no host
+// synchronisation per level.  This is the form for high-diameter graphs
+// (grids, road-like meshes), where the 64 lanes of a bit-parallel batch sit
+// at different levels and share no work: the dense step would rescan every
+// vertex on each of hundreds of levels, while here a source touches each
+// reached vertex and edge a constant number of times.
+//
+// Per workgroup slot (stride n, reused source after source):
+//   R[v]      16-byte record, so a neighbour's level and path count share one
+//             cache line:
+//     .dist   int32   BFS level, -1 = unreached (restored after each source)
+//     .npred  uint32  predecessor count (per-predecessor formula only)
+//     .sig    double  path count; the backward overwrites it in place with
+//                     coef = (1 + delta) / sigma once v's delta is known
+//   Q[]       int32   BFS order; level l is Q[loff[l] .. loff[l+1])
+//   bcp[v]    double  this slot's BC partial over its sources
+//
+// Forward, level d: each vertex u of the level scans its row; a neighbour w
+// with dist -1 is claimed by compare-and-swap (dist := d + 1, appended to
+// Q), and every w at level d + 1 gets sig[w] += sig[u] by an FP64 atomic
+// add.  Path counts are integers, exact in FP64 below 2^53, so the sum does
+// not depend on the order of the adds.  This is the level step of the
+// reference's queue BFS (Seq_Brandes/Brandes.c:274-298) for one source.
+//
+// Backward, level d from the deepest up: v sums coef[w] over its neighbours
+// w at level d + 1 in row order (delta = sigma * sum, Brandes.c:317-327 in
+// textbook form), then coef[v] replaces sig[v] and bcp[v] += delta.  Slot s
+// handles sources s, s + G, s + 2G, ... in order and the host-side reduce
+// adds the slots in slot order, so BC is deterministic for a given G.
+//
+// Rows longer than kPsBigDeg are queued in LDS and scanned by a whole
+// wavefront (lane-strided edges, butterfly sum), so one hub row does not
+// serialise a level.
+#pragma once
+#include "wave.h"
+
+namespace cent {
+
+constexpr int kPsBlock = 512;   // 8 waves per workgroup
+constexpr int kPsWaves = kPsBlock / kWave;
+constexpr int kPsBigDeg = 64;   // longer rows are scanned by a wavefront
+constexpr int kPsBig = 1024;    // LDS capacity of the long-row list per level
+constexpr int kPsV = 4;         // rows a thread works on at once
+constexpr int kPsE = 4;         // edges per row issued together
+
+struct __align__(16) PsRec {
+    int dist;
+    uint32_t npred;
+    double sig;
+};
+
+struct PsArgs {
+    int n;
+    const int* __restrict__ rp;
+    const int* __restrict__ col;
+    const int* __restrict__ src;          // [k] device ids
+    int k;
+    PsRec* __restrict__ rec;              // [G][n]
+    int* __restrict__ Q;                  // [G][n]
+    int* __restrict__ loff;               // [G][lcap]
+    int lcap;
+    double* __restrict__ bcp;             // [G][n] or null (closeness only)
+    int64_t* __restrict__ far;            // [k]
+    int64_t* __restrict__ dist_sum;       // [n] or null
+    const int64_t* __restrict__ weight;   // [n] or null
+    unsigned long long* __restrict__ cnt; // [2]: reached pairs, DAG edges
+    unsigned long long* __restrict__ depth_sum;
+    int* __restrict__ overflow;           // path count left FP64, or depth > lcap
+};
+
+// Loads of words that other waves change by atomics: agent-scope relaxed
+// atomic loads read the L2 copy, never a stale L1 line.
+__device__ __forceinline__ int ld_dist(const int* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double ld_sig(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t ld_npred(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+struct PsLocal {
+    int64_t far = 0;
+    unsigned long long reached = 0, dag = 0;
+};
+
+// Visit edge u -> w at forward level d.
+template <bool BC, bool COMPAT>
+__device__ __forceinline__ void ps_visit(const PsArgs& a, PsRec* R, int* Q,
+                                         int w, double su, int d, int le, int* s_cnt, PsLocal& L) {
+    int lw = ld_dist(&R[w].dist);
+    if (lw < 0) {
+        const int old = atomicCAS(&R[w].dist, -1, d + 1);
+        if (old < 0) {
+            const int pos = le + atomicAdd(s_cnt, 1);
+            Q[pos] = w;
+            L.far += a.weight ? (int64_t)(d + 1) * a.weight[w] : (int64_t)(d + 1);
+            L.reached++;
+            if (a.dist_sum) atomicAdd((unsigned long long*)&a.dist_sum[w], (unsigned long long)(d + 1));
+            lw = d + 1;
+        } else {
+            lw = old;
+        }
+    }
+    if constexpr (BC) {
+        if (lw == d + 1) {
+            atomicAdd(&R[w].sig, su);
+            if constexpr (COMPAT) atomicAdd(&R[w].npred, 1u);
+            L.dag++;
+        }
+    }
+}
+
+// Forward edges of up to kPsV rows at once (edges past the first kPsE of a
+// row finish in a plain loop): all column loads, then all level loads, then
+// all claims, then the bookkeeping, so one thread keeps up to kPsV * kPsE
+// independent chains in flight instead of walking its rows edge by edge.
+constexpr int kNoEdge = -2;
+constexpr int kClaimed = -3;
+
+template <bool BC, bool COMPAT>
+__device__ __forceinline__ void ps_expand_rows(const PsArgs& a, PsRec* R, int* Q,
+                                               const int* rb, const int* dg, const double* su, int d, int le,
+                                               int* s_cnt, PsLocal& L) {
+    int w[kPsV][kPsE], lev[kPsV][kPsE], old[kPsV][kPsE];
+#pragma unroll
+    for (int m = 0; m < kPsV; ++m)
+#pragma unroll
+        for (int e = 0; e < kPsE; ++e) w[m][e] = e < dg[m] ? a.col[rb[m] + e] : -1;
+#pragma unroll
+    for (int m = 0; m < kPsV; ++m)
+#pragma unroll
+        for (int e = 0; e < kPsE; ++e) lev[m][e] = w[m][e] >= 0 ? ld_dist(&R[w[m][e]].dist) : kNoEdge;
+#pragma unroll
+    for (int m = 0; m < kPsV; ++m)
+#pragma unroll
+        for (int e = 0; e < kPsE; ++e) old[m][e] = lev[m][e] == -1 ? atomicCAS(&R[w[m][e]].dist, -1, d + 1) : 0;
+#pragma unroll
+    for (int m = 0; m < kPsV; ++m)
+#pragma unroll
+        for (int e = 0; e < kPsE; ++e)
+            if (lev[m][e] == -1) lev[m][e] = old[m][e] < 0 ? kClaimed : old[m][e];
+#pragma unroll
+    for (int m = 0; m < kPsV; ++m) {
+#pragma unroll
+        for (int e = 0; e < kPsE; ++e) {
+            const int x = w[m][e];
+            int lv = lev[m][e];
+            if (lv == kClaimed) {
+                Q[le + atomicAdd(s_cnt, 1)] = x;
+                L.far += a.weight ? (int64_t)(d + 1) * a.weight[x] : (int64_t)(d + 1);
+                L.reached++;
+                if (a.dist_sum) atomicAdd((unsigned long long*)&a.dist_sum[x], (unsigned long long)(d + 1));
+                lv = d + 1;
+            }
+            if constexpr (BC) {
+                if (lv == d + 1) {
+                    atomicAdd(&R[x].sig, su[m]);
+                    if constexpr (COMPAT) atomicAdd(&R[x].npred, 1u);
+                    L.dag++;
+                }
+            }
+        }
+        for (int e = kPsE; e < dg[m]; ++e)
+            ps_visit<BC, COMPAT>(a, R, Q, a.col[rb[m] + e], su[m], d, le, s_cnt, L);
+    }
+}
+
+// Backward sums of up to kPsV rows at once: acc[m] = sum over row m's
+// neighbours w at level `lvl` of coef[w] (held in sig), in row order.
+__device__ __forceinline__ void ps_pull_rows(const PsArgs& a, const PsRec* R, const int* rb,
+                                             const int* dg, int lvl, double* acc) {
+    int w[kPsV][kPsE], lev[kPsV][kPsE];
+    double x[kPsV][kPsE];
+#pragma unroll
+    for (int m = 0; m < kPsV; ++m)
+#pragma unroll
+        for (int e = 0; e < kPsE; ++e) w[m][e] = e < dg[m] ? a.col[rb[m] + e] : -1;
+#pragma unroll
+    for (int m = 0; m < kPsV; ++m)
+#pragma unroll
+        for (int e = 0; e < kPsE; ++e) lev[m][e] = w[m][e] >= 0 ? ld_dist(&R[w[m][e]].dist) : kNoEdge;
+#pragma unroll
+    for (int m = 0; m < kPsV; ++m)
+#pragma unroll
+        for (int e = 0; e < kPsE; ++e) x[m][e] = lev[m][e] == lvl ? ld_sig(&R[w[m][e]].sig) : 0.0;
+#pragma unroll
+    for (int m = 0; m < kPsV; ++m) {
+        double t = 0.0;
+#pragma unroll
+        for (int e = 0; e < kPsE; ++e) t += x[m][e];
+        for (int e = kPsE; e < dg[m]; ++e) {
+            const int y = a.col[rb[m] + e];
+            if (ld_dist(&R[y].dist) == lvl) t += ld_sig(&R[y].sig);
+        }
+        acc[m] = t;
+    }
+}
+
+template <bool BC, bool COMPAT>
+__global__ __launch_bounds__(kPsBlock) void k_ps(PsArgs a) {
+    __shared__ int s_cnt[3];     // appends of level d go to s_cnt[d % 3]
+    __shared__ int s_nbig[3];    // long rows of level d: s_big[..s_nbig[d % 3])
+    __shared__ int s_big[kPsBig];
+    __shared__ int64_t s_red64[kPsWaves][3];
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wid = tid >> 6;
+    const size_t so = (size_t)blockIdx.x * (size_t)a.n;
+    PsRec* R = a.rec + so;
+    int* Q = a.Q + so;
+    int* loff = a.loff + (size_t)blockIdx.x * (size_t)a.lcap;
+    double* bcp = BC ? a.bcp + so : nullptr;
+    for (int si = blockIdx.x; si < a.k; si += gridDim.x) {
+        const int s = a.src[si];
+        if (tid == 0) {
+            Q[0] = s;
+            R[s].dist = 0;
+            R[s].sig = 1.0;
+            loff[0] = 0;
+            loff[1] = 1;
+            s_cnt[0] = s_cnt[1] = s_cnt[2] = 0;
+            s_nbig[0] = s_nbig[1] = s_nbig[2] = 0;
+        }
+        __syncthreads();
+        PsLocal L;
+        int lb = 0, le = 1, d = 0;
+        bool deep = false;
+        // ---------------- forward
+        for (;;) {
+            const int c = d % 3;
+            if (tid == 0) {  // free the buffers level d + 1 will use (last read two levels ago)
+                s_cnt[(d + 1) % 3] = 0;
+                s_nbig[(d + 1) % 3] = 0;
+            }
+            for (int i0 = lb + tid; i0 < le; i0 += kPsBlock * kPsV) {
+                int u[kPsV], rb[kPsV], dg[kPsV];
+                double su[kPsV];
+#pragma unroll
+                for (int m = 0; m < kPsV; ++m) {
+                    const int i = i0 + m * kPsBlock;
+                    u[m] = i < le ? Q[i] : -1;
+                }
+#pragma unroll
+                for (int m = 0; m < kPsV; ++m) {
+                    rb[m] = 0;
+                    dg[m] = 0;
+                    su[m] = 0.0;
+                    if (u[m] >= 0) {
+                        rb[m] = a.rp[u[m]];
+                        dg[m] = a.rp[u[m] + 1] - rb[m];
+                        if (BC) su[m] = ld_sig(&R[u[m]].sig);
+                    }
+                }
+#pragma unroll
+                for (int m = 0; m < kPsV; ++m)
+                    if (dg[m] > kPsBigDeg) {
+                        const int slot = atomicAdd(&s_nbig[c], 1);
+                        if (slot < kPsBig) {
+                            s_big[slot] = u[m];
+                            dg[m] = 0;
+                        }
+                    }
+                ps_expand_rows<BC, COMPAT>(a, R, Q, rb, dg, su, d, le, &s_cnt[c], L);
+            }
+            __syncthreads();
+            const int nbig = min(s_nbig[c], kPsBig);
+            for (int t = wid; t < nbig; t += kPsWaves) {
+                const int u = s_big[t];
+                const double su = BC ? ld_sig(&R[u].sig) : 0.0;
+                for (int j = a.rp[u] + lane; j < a.rp[u + 1]; j += kWave)
+                    ps_visit<BC, COMPAT>(a, R, Q, a.col[j], su, d, le, &s_cnt[c], L);
+            }
+            __syncthreads();
+            const int nnew = s_cnt[c];
+            if (nnew == 0) break;
+            if (d + 3 > a.lcap) {  // level offsets would not fit: the host reruns another way
+                deep = true;
+                le += nnew;        // so the restore below also covers the last level
+                break;
+            }
+            lb = le;
+            le += nnew;
+            ++d;
+            if (tid == 0) loff[d + 1] = le;
+        }
+        const int depth = d;
+        const int tail = le;
+        if (deep && tid == 0) atomicOr(a.overflow, 2);
+        // ---------------- backward
+        if (BC && !deep) {
+            if (tid == 0) s_nbig[0] = s_nbig[1] = s_nbig[2] = 0;  // the forward's lists are spent
+            __syncthreads();
+            for (int dd = depth; dd >= 1; --dd) {
+                const int c = dd % 3;
+                if (tid == 0) s_nbig[(dd + 2) % 3] = 0;  // the list level dd - 1 will use
+                const int b0 = loff[dd], b1 = loff[dd + 1];
+                for (int i0 = b0 + tid; i0 < b1; i0 += kPsBlock * kPsV) {
+                    int v[kPsV], rb[kPsV], dg[kPsV];
+                    double sv[kPsV];
+#pragma unroll
+                    for (int m = 0; m < kPsV; ++m) {
+                        const int i = i0 + m * kPsBlock;
+                        v[m] = i < b1 ? Q[i] : -1;
+                    }
+#pragma unroll
+                    for (int m = 0; m < kPsV; ++m) {
+                        rb[m] = 0;
+                        dg[m] = 0;
+                        sv[m] = 1.0;
+                        if (v[m] >= 0) {
+                            rb[m] = a.rp[v[m]];
+                            dg[m] = a.rp[v[m] + 1] - rb[m];
+                            sv[m] = ld_sig(&R[v[m]].sig);
+                        }
+                    }
+                    bool big[kPsV];
+#pragma unroll
+                    for (int m = 0; m < kPsV; ++m) {
+                        big[m] = false;
+                        if (dg[m] > kPsBigDeg) {
+                            const int slot = atomicAdd(&s_nbig[c], 1);
+                            if (slot < kPsBig) {
+                                s_big[slot] = v[m];
+                                big[m] = true;
+                                dg[m] = 0;
+                            }
+                        }
+                    }
+                    double acc[kPsV];
+                    ps_pull_rows(a, R, rb, dg, dd + 1, acc);
+#pragma unroll
+                    for (int m = 0; m < kPsV; ++m) {
+                        if (v[m] < 0 || big[m]) continue;
+                        const double dl = sv[m] * acc[m];
+                        if (__builtin_isinf(sv[m])) atomicOr(a.overflow, 1);
+                        R[v[m]].sig = (1.0 + dl) / sv[m];
+                        bcp[v[m]] += COMPAT ? dl * (double)ld_npred(&R[v[m]].npred) : dl;
+                    }
+                }
+                __syncthreads();
+                const int nbig = min(s_nbig[c], kPsBig);
+                for (int t = wid; t < nbig; t += kPsWaves) {
+                    const int v = s_big[t];
+                    double acc = 0.0;
+                    for (int j = a.rp[v] + lane; j < a.rp[v + 1]; j += kWave) {
+                        const int w = a.col[j];
+                        if (ld_dist(&R[w].dist) == dd + 1) acc += ld_sig(&R[w].sig);
+                    }
+                    acc = wave_sum(acc);
+                    if (lane == 0) {
+                        const double sv = ld_sig(&R[v].sig);
+                        const double dl = sv * acc;
+                        if (__builtin_isinf(sv)) atomicOr(a.overflow, 1);
+                        R[v].sig = (1.0 + dl) / sv;
+                        bcp[v] += COMPAT ? dl * (double)ld_npred(&R[v].npred) : dl;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        // ---------------- per-source totals, then restore the slot
+        int64_t far = L.far;
+        int64_t rch = (int64_t)L.reached, dag = (int64_t)L.dag;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            far += (int64_t)__shfl_xor((long long)far, o);
+            rch += (int64_t)__shfl_xor((long long)rch, o);
+            dag += (int64_t)__shfl_xor((long long)dag, o);
+        }
+        if (lane == 0) {
+            s_red64[wid][0] = far;
+            s_red64[wid][1] = rch;
+            s_red64[wid][2] = dag;
+        }
+        for (int i = tid; i < tail; i += kPsBlock) {
+            R[Q[i]] = PsRec{-1, 0u, 0.0};
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int64_t f = 0, r = 0, g = 0;
+            for (int w = 0; w < kPsWaves; ++w) {
+                f += s_red64[w][0];
+                r += s_red64[w][1];
+                g += s_red64[w][2];
+            }
+            a.far[si] = f;
+            atomicAdd(&a.cnt[0], (unsigned long long)r);
+            atomicAdd(&a.cnt[1], (unsigned long long)g);
+            atomicAdd(a.depth_sum, (unsigned long long)depth);
+        }
+        __syncthreads();  // restores and totals done before the next source
+    }
+}
+
+// bc[v] = sum over slots g = 0 .. G-1 of bcp[g][v], in slot order.
+__global__ __launch_bounds__(256) void k_ps_reduce(const double* __restrict__ bcp, int G, int n,
+                                                   double* __restrict__ bc) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    double t = 0.0;
+    for (int g = 0; g < G; ++g) t += bcp[(size_t)g * (size_t)n + v];
+    bc[v] += t;
+}
+
+// Slot arrays to their idle state: every record {-1, 0, 0}, bcp 0.
+__global__ __launch_bounds__(256) void k_ps_init(PsRec* __restrict__ rec, double* __restrict__ bcp, size_t total) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        rec[i] = PsRec{-1, 0u, 0.0};
+        if (bcp) bcp[i] = 0.0;
+    }
+}
+
+}  // namespace cent

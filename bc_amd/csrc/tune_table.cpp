@@ -45,9 +45,7 @@ extern "C" int cent_tune_params_check(const cent_tune_params* p) {
     if (p->hub_deg < 32 || p->hub_deg > (1 << 20)) return CENT_E_INVAL;
     if (p->chunk_edges < 64 || p->chunk_edges > (1 << 22)) return CENT_E_INVAL;
     if (p->grid_waves < 0 || p->grid_waves > (1 << 22)) return CENT_E_INVAL;
-    if (p->frontier != CENT_FRONTIER_AUTO && p->frontier != CENT_FRONTIER_DENSE &&
-        p->frontier != CENT_FRONTIER_SPARSE)
-        return CENT_E_INVAL;
+    if (p->frontier < CENT_FRONTIER_AUTO || p->frontier > CENT_FRONTIER_SOURCE) return CENT_E_INVAL;
     return CENT_OK;
 }
 

@@ -31,7 +31,7 @@ def workloads(quick: bool):
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--out", required=True)
-    ap.add_argument("--sources", type=int, default=64)
+    ap.add_argument("--sources", type=int, default=256)
     ap.add_argument("--quick", action="store_true", help="skip the 4M-vertex grid")
     args = ap.parse_args()
     ctx = bc_amd.Context(0)

@@ -313,7 +313,7 @@ def main() -> int:
                     help="sources for the CPU baseline (default: 8 per core)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tune", action="store_true",
-                    help="autotune the launch settings on the first 64 sources before timing")
+                    help="autotune the launch settings on the first 256 sources before timing")
     ap.add_argument("--tune-save", default="", help="with --tune: write the tuning table here")
     ap.add_argument("--no-tune-table", action="store_true",
                     help="ignore the bundled tuning table (engine defaults)")
@@ -351,14 +351,14 @@ def main() -> int:
           f"(generated in {time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
 
     # launch settings: the bundled MI355X tuning table (applied at upload by
-    # graph signature), or a fresh tuning run on the first 64 sources
+    # graph signature), or a fresh tuning run on the first 256 sources
     if not args.no_tune_table and os.path.exists(bc_amd.TUNE_TABLE):
         bc_amd.tune_table_load()
     ctx = bc_amd.Context(0)
     dg = ctx.upload(g, keep_order=args.keep_order)
     if args.tune:
         t1 = time.perf_counter()
-        best = ctx.tune(dg, src[:64], bc_amd.CENT_TUNE_BC)
+        best = ctx.tune(dg, src[:256], bc_amd.CENT_TUNE_BC)
         print(f"# tuned in {time.perf_counter() - t1:.1f}s: {best.as_dict()}", file=sys.stderr, flush=True)
         if args.tune_save:
             bc_amd.tune_table_save(args.tune_save)

@@ -126,8 +126,9 @@ typedef struct cent_graph cent_graph;  /* device-resident CSR                  *
  * lists and replays hipGraph-captured runs of levels (best for hundreds of
  * levels with small frontiers), AUTO picks from the graph's BFS-depth probe. */
 #define CENT_FRONTIER_AUTO   0
-#define CENT_FRONTIER_DENSE  1
-#define CENT_FRONTIER_SPARSE 2
+#define CENT_FRONTIER_DENSE  1   /* 64-source bit-parallel pull over all vertices */
+#define CENT_FRONTIER_SPARSE 2   /* 64-source bit-parallel, frontier queues       */
+#define CENT_FRONTIER_SOURCE 3   /* one workgroup per source (deep graphs)         */
 
 typedef struct cent_opts {
     int32_t bc_mode;      /* CENT_BC_TEXTBOOK or CENT_BC_REF_COMPAT          */
@@ -135,7 +136,8 @@ typedef struct cent_opts {
     int32_t block_size;   /* reserved, 0                                     */
     int32_t grid_waves;   /* 0 = default cap on waves per dense level launch */
     int32_t frontier;     /* CENT_FRONTIER_*                                 */
-    int32_t reserved;
+    int32_t max_slots;    /* SOURCE mode: cap on concurrent per-source slots
+                             (0 = sized from free device memory)          */
 } cent_opts;
 
 typedef struct cent_stats {
@@ -213,7 +215,8 @@ int  cent_cc(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k,
 typedef struct cent_tune_params {
     int32_t hub_deg;      /* 32 .. 2^20                                      */
     int32_t chunk_edges;  /* 64 .. 2^22                                      */
-    int32_t grid_waves;   /* 0 = engine default (2048 workgroups of 4 waves) */
+    int32_t grid_waves;   /* 0 = engine default (2048 workgroups of 4 waves);
+                             source-parallel step: grid_waves / 8 slots   */
     int32_t frontier;     /* CENT_FRONTIER_*; AUTO = upload-time heuristic   */
     double  ms;           /* measured trial time (cent_tune), else 0         */
 } cent_tune_params;
@@ -234,12 +237,14 @@ int  cent_graph_get_layout(const cent_graph* g, cent_tune_params* out);
 int  cent_graph_set_layout(cent_graph* g, const cent_tune_params* p);
 int  cent_graph_signature(const cent_graph* g, cent_tune_sig* out);
 
-/* Times the plain call `what` on the sample sources src[0..k) (k <= 256) for
- * candidate settings by coordinate descent (frontier mode, hub_deg,
- * chunk_edges, grid_waves; at most max_trials timed runs, 0 = no limit),
- * applies the fastest to g, records it in the process-wide table under g's
- * signature and returns it in *best.  Results never depend on the settings
- * beyond floating-point summation order. */
+/* Times the plain call `what` on the sample sources src[0..k) (k <= 256;
+ * use 256 where the graph allows, so the level steps are compared at full
+ * width) for candidate settings by coordinate descent (level step, then
+ * hub_deg, chunk_edges and grid_waves for the dense step or the slot count
+ * for the source-parallel step; at most max_trials timed runs, 0 = no
+ * limit), applies the fastest to g, records it in the process-wide table
+ * under g's signature and returns it in *best.  Results never depend on the
+ * settings beyond floating-point summation order. */
 int  cent_tune(cent_ctx* ctx, cent_graph* g, const int32_t* src, int32_t k, int32_t what,
                int32_t max_trials, cent_tune_params* best);
 

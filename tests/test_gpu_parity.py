@@ -156,11 +156,12 @@ def test_device_order_does_not_change_results(ctx):
     assert np.array_equal(a[1], ofar) and np.array_equal(a[2], ods)
 
 
-@pytest.mark.parametrize("frontier", ["dense", "sparse"])
+@pytest.mark.parametrize("frontier", ["dense", "sparse", "source"])
 def test_frontier_modes_agree(ctx, frontier):
-    """Dense pull levels and frontier-queue (hipGraph-replayed) levels give the
-    oracle's results on a power-law graph, a high-diameter grid, a disconnected
-    graph with duplicate sources, and in the reference-compat formula."""
+    """Dense pull levels, frontier-queue (hipGraph-replayed) levels and
+    source-parallel workgroups give the oracle's results on a power-law graph,
+    a high-diameter grid, a disconnected graph with duplicate sources, and in
+    the reference-compat formula."""
     cases = [("rmat13", bc_amd.gen_rmat(13, 16 << 13, seed=3), 150),
              ("grid", bc_amd.gen_grid(64, 96, 0.05, 3, seed=4), 130),
              ("path", bc_amd.CSR.from_edges(300, np.arange(299), np.arange(1, 300)), 70)]
@@ -182,3 +183,44 @@ def test_frontier_modes_agree(ctx, frontier):
     obc, ofar, ods, _ = O.brandes(g, src)
     assert_bc_close(bc, obc)
     assert np.array_equal(far, ofar) and np.array_equal(ds, ods)
+
+
+@pytest.mark.parametrize("slots", [1, 3, 7])
+def test_source_mode_slot_reuse(ctx, slots):
+    """Source-parallel slots run many sources each (restoring their arrays in
+    between); rows longer than the wavefront threshold take the long-row path
+    (R-MAT hubs).  Results match the oracle for every slot count."""
+    for name, g, k in [("rmat12", bc_amd.gen_rmat(12, 16 << 12, seed=5), 90),
+                       ("grid", bc_amd.gen_grid(40, 50, 0.05, 5, seed=6), 75)]:
+        print(f"[gpu] source slots={slots} {name} n={g.n}", flush=True)
+        dg = ctx.upload(g)
+        src = bc_amd.sample_sources(g.n, k, 9)
+        for mode in (bc_amd.CENT_BC_TEXTBOOK, bc_amd.CENT_BC_REF_COMPAT):
+            bc, far, ds = ctx.bc(dg, src, mode=mode, dist_sum=True, frontier="source", max_slots=slots)
+            obc, ofar, ods, _ = O.brandes(g, src, mode=mode, threads=min(16, O.max_threads()))
+            assert_bc_close(bc, obc, f"{name} slots {slots} mode {mode}")
+            assert np.array_equal(far, ofar) and np.array_equal(ds, ods)
+        cfar, cds = ctx.cc(dg, src, dist_sum=True, frontier="source", max_slots=slots)
+        assert np.array_equal(cfar, ofar) and np.array_equal(cds, ods)
+
+
+def test_source_mode_deterministic(ctx):
+    """Same slot count, same bits: BC partials are reduced in slot order."""
+    g = bc_amd.gen_grid(48, 48, 0.05, 8, seed=2)
+    dg = ctx.upload(g)
+    src = bc_amd.sample_sources(g.n, 200, 3)
+    a = ctx.bc(dg, src, frontier="source", max_slots=16)[0]
+    b = ctx.bc(dg, src, frontier="source", max_slots=16)[0]
+    assert np.array_equal(a, b)
+
+
+def test_source_mode_deep_fallback(ctx):
+    """A BFS deeper than the source-parallel level table (path of 70000
+    vertices from an end) is redone in the batch form: farness n(n-1)/2."""
+    n = 70000
+    g = bc_amd.CSR.from_edges(n, np.arange(n - 1), np.arange(1, n))
+    dg = ctx.upload(g, keep_order=True)
+    print("[gpu] source-mode deep fallback, path 70000", flush=True)
+    far, ds = ctx.cc(dg, [0], dist_sum=True, frontier="source")
+    assert far[0] == n * (n - 1) // 2
+    assert np.array_equal(ds, np.arange(n, dtype=np.int64))
