@@ -321,6 +321,47 @@ _lib.computeBC_compat.argtypes = [_P_CSR, _F64P]
 _lib.computeCC.argtypes = [_P_CSR, _I64P]
 
 
+# ------------------------------------------------------- result files
+_lib.cent_write_bc_scores.argtypes = [C.c_char_p, _P_CSR, _F64P]
+_lib.cent_read_bc_scores.argtypes = [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _F64P, C.c_int32]
+_lib.cent_ans_path.argtypes = [C.c_char_p, C.c_char_p, C.c_size_t]
+_lib.cent_read_cc_answers.argtypes = [C.c_char_p, _P_CSR, _I64P]
+_lib.cent_write_cc_answers.argtypes = [C.c_char_p, _P_CSR, _I64P]
+
+
+def write_bc_scores(path: str, csr: "CSR", bc) -> None:
+    """BC score file in the reference's format ("start end", then "%f" per vertex)."""
+    b = np.ascontiguousarray(bc, dtype=np.float64)
+    _check(_lib.cent_write_bc_scores(path.encode(), csr._p, _ptr(b, C.c_double)), "cent_write_bc_scores", path)
+
+
+def read_bc_scores(path: str, cap: int = 1 << 26):
+    """(start, end, scores) with scores[start..end] from a reference score file."""
+    out = np.zeros(cap, np.float64)
+    s, e = C.c_int32(), C.c_int32()
+    _check(_lib.cent_read_bc_scores(path.encode(), C.byref(s), C.byref(e), _ptr(out, C.c_double), cap),
+           "cent_read_bc_scores", path)
+    return s.value, e.value, out[:e.value + 1].copy()
+
+
+def ans_path(dataset_path: str) -> str:
+    """The reference's answer-file name for a .mtx dataset."""
+    buf = C.create_string_buffer(4096)
+    _check(_lib.cent_ans_path(dataset_path.encode(), buf, 4096), "cent_ans_path", dataset_path)
+    return buf.value.decode()
+
+
+def read_cc_answers(path: str, csr: "CSR") -> np.ndarray:
+    ans = np.zeros(max(csr.n, 1), np.int64)
+    _check(_lib.cent_read_cc_answers(path.encode(), csr._p, _ptr(ans, C.c_int64)), "cent_read_cc_answers", path)
+    return ans[:csr.n]
+
+
+def write_cc_answers(path: str, csr: "CSR", cc) -> None:
+    c = np.ascontiguousarray(cc, dtype=np.int64)
+    _check(_lib.cent_write_cc_answers(path.encode(), csr._p, _ptr(c, C.c_int64)), "cent_write_cc_answers", path)
+
+
 class TuneParams(C.Structure):
     """Launch settings of the plain form (``cent_tune_params``)."""
     _fields_ = [("hub_deg", C.c_int32), ("chunk_edges", C.c_int32), ("grid_waves", C.c_int32),

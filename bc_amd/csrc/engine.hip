@@ -649,6 +649,18 @@ static int grow_levels(cent_ctx* ctx, size_t N, int keep_levels) {
     return CENT_OK;
 }
 
+// Batch totals from the G per-workgroup partials: far_out[0..kb) assigned,
+// ctx->cnt accumulated.  One reducing workgroup per 128 partial rows, at
+// most 32.
+static int reduce_batch(cent_ctx* ctx, int G, int kb, int64_t* far_out) {
+    hipStream_t s = ctx->stream;
+    HIPTRY(hipMemsetAsync(far_out, 0, sizeof(int64_t) * (size_t)(kb > 0 ? kb : 1), s));
+    const int rg = std::max(1, std::min(32, G / 128));
+    k_reduce_batch<<<rg, 1024, 0, s>>>(ctx->part_far.p, ctx->part_cnt.p, G, kb, far_out, ctx->cnt.p);
+    HIPTRY(hipGetLastError());
+    return CENT_OK;
+}
+
 // One batch of kb <= 64 sources (device ids d_src[0..kb)) whose BFS stays
 // inside the view's vertex range.  far_out (device, kb entries) gets the
 // weighted distance sums.  Adds the batch's depth to *levels.
@@ -754,8 +766,7 @@ static int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const
             ctx->stats.launches_bwd++;
         }
     }
-    k_reduce_batch<<<1, 1024, 0, s>>>(ctx->part_far.p, ctx->part_cnt.p, G, kb, far_out, ctx->cnt.p);
-    HIPTRY(hipGetLastError());
+    TRY(reduce_batch(ctx, G, kb, far_out));
     ctx->stats.batches++;
     return CENT_OK;
 }
@@ -880,8 +891,7 @@ static int level_batch_sparse(cent_ctx* ctx, SpPlan& pl, const BatchSpec& sp, co
         }
     }
     k_sp_clear<<<G, kBlock, 0, s>>>(pl.a);
-    k_reduce_batch<<<1, 1024, 0, s>>>(ctx->part_far.p, ctx->part_cnt.p, G, kb, far_out, ctx->cnt.p);
-    HIPTRY(hipGetLastError());
+    TRY(reduce_batch(ctx, G, kb, far_out));
     ctx->stats.batches++;
     return CENT_OK;
 }

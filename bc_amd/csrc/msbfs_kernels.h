@@ -404,8 +404,11 @@ __global__ void k_seed(const int* __restrict__ src, int k, uint64_t* __restrict_
     if (with_sigma) sigma[(size_t)s * kWave + lane] = 1.0;
 }
 
-// Sums the per-block lane partials of a batch into far[0..k) and the counters.
-// 1024 threads: thread t sums lane (t % 64) over blocks t/64, t/64 + 16, ...
+// Sums the per-block lane partials of a batch into far[0..k) and the
+// counters.  Several workgroups share the rows (workgroup g takes rows
+// g*16 + r, then every gridDim*16-th); each adds its lane totals to far /
+// cnt_out with 64-bit atomics.  The partials are integers, so the total does
+// not depend on the order.  far must be zeroed before the launch.
 __global__ __launch_bounds__(1024) void k_reduce_batch(const int64_t* __restrict__ part_far,
                                                        const unsigned long long* __restrict__ part_cnt,
                                                        int nblocks, int k, int64_t* __restrict__ far,
@@ -417,7 +420,7 @@ __global__ __launch_bounds__(1024) void k_reduce_batch(const int64_t* __restrict
     const int row = threadIdx.x >> 6;
     int64_t f = 0;
     unsigned long long c = 0;
-    for (int b = row; b < nblocks; b += kRows) {
+    for (int b = blockIdx.x * kRows + row; b < nblocks; b += gridDim.x * kRows) {
         f += part_far[(size_t)b * kWave + lane];
         if (lane < 2) c += part_cnt[(size_t)b * 2 + lane];
     }
@@ -427,11 +430,11 @@ __global__ __launch_bounds__(1024) void k_reduce_batch(const int64_t* __restrict
     if (row == 0) {
         int64_t t = 0;
         for (int r = 0; r < kRows; ++r) t += s_far[r][lane];
-        if (lane < k) far[lane] = t;
+        if (lane < k && t) atomicAdd((unsigned long long*)&far[lane], (unsigned long long)t);
         if (lane < 2) {
             unsigned long long u = 0;
             for (int r = 0; r < kRows; ++r) u += s_c[r][lane];
-            cnt_out[lane] += u;
+            if (u) atomicAdd(&cnt_out[lane], u);
         }
     }
 }

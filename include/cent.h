@@ -22,6 +22,7 @@
 #ifndef CENT_H
 #define CENT_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -202,6 +203,18 @@ int  cent_bc(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k,
  */
 int  cent_cc(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k,
              int64_t* farness, int64_t* dist_sum, const cent_opts* opts);
+
+/* ------------------------------------------------ result files */
+/* The reference's result formats.
+ * BC scores (Seq_Brandes/Brandes.c:2313-2322): "start end" then one "%f"
+ * line per vertex start..end.  read fills out[start..end] (cap >= end + 1).
+ * Closeness answers (Seq_CC/CC.cpp:2621-2684): "<dataset>_ans.txt" next to a
+ * ".mtx" dataset, one "id cc" line per vertex; read assigns ans[id]. */
+int  cent_write_bc_scores(const char* path, const struct CSR* csr, const double* bc);
+int  cent_read_bc_scores(const char* path, int32_t* start, int32_t* end, double* out, int32_t cap);
+int  cent_ans_path(const char* dataset_path, char* out, size_t cap);
+int  cent_read_cc_answers(const char* path, const struct CSR* csr, int64_t* ans);
+int  cent_write_cc_answers(const char* path, const struct CSR* csr, const int64_t* cc);
 
 /* ------------------------------------------------ autotuner */
 /*
