@@ -72,3 +72,24 @@ int cent_load_edgelist(const char* path, struct CSR** out) {
     *out = g;
     return CENT_OK;
 }
+
+/* Writes csr in the reference's edge-list format: "count m" (count =
+ * endNodeID - startNodeID + 1), then each undirected edge once as "u v" with
+ * u < v, in row order.  The reference's loader takes the id base from the
+ * first edge line (0-based iff it names vertex 0, Lib/AdjList/AdjList.c:39),
+ * so a 0-based graph whose vertex 0 has no edge cannot be written
+ * (CENT_E_UNSUPPORTED). */
+int cent_write_edgelist(const char* path, const struct CSR* csr) {
+    if (!path || !csr) return CENT_E_INVAL;
+    const int32_t n = csr->csrVSize - 1;
+    if (csr->startNodeID == 0 && csr->csrESize > 0 && n > 0 && csr->csrV[1] == csr->csrV[0])
+        return CENT_E_UNSUPPORTED;
+    FILE* f = fopen(path, "w");
+    if (!f) return CENT_E_IO;
+    int ok = fprintf(f, "%d %lld\n", csr->totalNodeNumber, (long long)(csr->csrESize / 2)) > 0;
+    for (int32_t u = 0; ok && u < n; ++u)
+        for (int32_t j = csr->csrV[u]; ok && j < csr->csrV[u + 1]; ++j)
+            if (csr->csrE[j] > u) ok = fprintf(f, "%d %d\n", u, csr->csrE[j]) > 0;
+    if (fclose(f) != 0) ok = 0;
+    return ok ? CENT_OK : CENT_E_IO;
+}

@@ -329,6 +329,14 @@ _lib.cent_read_cc_answers.argtypes = [C.c_char_p, _P_CSR, _I64P]
 _lib.cent_write_cc_answers.argtypes = [C.c_char_p, _P_CSR, _I64P]
 
 
+_lib.cent_write_edgelist.argtypes = [C.c_char_p, _P_CSR]
+
+
+def write_edgelist(path: str, csr: "CSR") -> None:
+    """csr as a reference edge-list file ("n m", then "u v" per undirected edge)."""
+    _check(_lib.cent_write_edgelist(path.encode(), csr._p), "cent_write_edgelist", path)
+
+
 def write_bc_scores(path: str, csr: "CSR", bc) -> None:
     """BC score file in the reference's format ("start end", then "%f" per vertex)."""
     b = np.ascontiguousarray(bc, dtype=np.float64)

@@ -139,6 +139,63 @@ def extra_grid(ctx, args):
             "fwd_ms": round(st["ms_fwd"], 2), "bwd_ms": round(st["ms_bwd"], 2)}
 
 
+def extra_rmat14(ctx, args):
+    """Exact all-source BC + closeness on R-MAT 2^14 (8 edge draws per vertex,
+    LCC, seed 1).  The engine's time beside the
+    reference's own computeBC and computeCC, compiled from its sources into
+    oracle/_ref (when that build exists), run single-threaded on this host on
+    the same graph written in the reference's edge-list format.  Results are
+    cross-checked: farness exact, per-predecessor BC within the reference's
+    FP32 rounding."""
+    import subprocess
+    import tempfile
+    import numpy as np
+    import bc_amd
+    g = bc_amd.gen_rmat(14, 8 << 14, seed=1)
+    dg = ctx.upload(g)
+    n, m = g.n, g.m
+    ctx.bc(dg, None)  # warm the workspace
+    t0 = time.perf_counter()
+    bc, far, _ = ctx.bc(dg, None)
+    t_gpu = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    bcc, _, _ = ctx.bc(dg, None, mode=bc_amd.CENT_BC_REF_COMPAT, farness=False)
+    t_gpu_compat = time.perf_counter() - t0
+    lay = dg.layout.as_dict()
+    lay.pop("ms")
+    res = {"workload": "exact all-source BC + closeness, R-MAT 2^14, 8 draws per vertex, LCC, seed 1",
+           "n": n, "m": m, "sources": n, "ms": round(t_gpu * 1e3, 2),
+           "teps": round(n * m / t_gpu, 1), "ms_ref_formula": round(t_gpu_compat * 1e3, 2), "launch": lay}
+    ref_bc = os.path.join(ROOT, "oracle", "_ref", "ref_bc")
+    ref_cc = os.path.join(ROOT, "oracle", "_ref", "ref_cc")
+    if os.path.exists(ref_bc) and os.path.exists(ref_cc):
+        with tempfile.TemporaryDirectory() as td:
+            path = os.path.join(td, "rmat14.txt")
+            bc_amd.write_edgelist(path, g)
+            t0 = time.perf_counter()
+            subprocess.run([ref_bc, path, os.path.join(td, "bc.txt")], check=True, timeout=900,
+                           stdout=subprocess.DEVNULL)
+            t_rbc = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            subprocess.run([ref_cc, "cc", path, os.path.join(td, "cc.txt")], check=True, timeout=900,
+                           stdout=subprocess.DEVNULL)
+            t_rcc = time.perf_counter() - t0
+            rbc = np.loadtxt(os.path.join(td, "bc.txt"))
+            rcc = np.loadtxt(os.path.join(td, "cc.txt"), dtype=np.int64)
+        scale = max(1.0, float(np.abs(bcc).max()))
+        res["cpu_baseline"] = {
+            "kind": "reference", "cores": 1, "unit": "TEPS",
+            "value": round(n * m / (t_rbc + t_rcc), 1),
+            "sample": f"whole workload: reference computeBC {t_rbc:.2f} s + computeCC {t_rcc:.2f} s, "
+                      f"CPU {cpu_model()}",
+            "speedup_vs_engine": round((t_rbc + t_rcc) / t_gpu, 1)}
+        res["parity_vs_reference"] = {
+            "farness_equal": bool(np.array_equal(rcc[:, 1], far[rcc[:, 0]])),
+            "bc_ref_formula_max_rel": float(np.max(np.abs(rbc - bcc[:len(rbc)])) / scale)}
+    log(f"rmat14: {res}")
+    return res
+
+
 def serve_pool(seed: int, size: int):
     """Seeded graph pool: vertex counts log-uniform in [1e3, 1e6]; even entries
     R-MAT (avg degree 16, LCC), odd entries block trees (about 31 vertices per
@@ -297,7 +354,7 @@ def serve_vs_plain(ctx, pool, reqs):
     return res
 
 
-EXTRA = {"blocktree": extra_blocktree, "blocktree-bc": extra_blocktree_bc, "grid": extra_grid,
+EXTRA = {"rmat14": extra_rmat14, "blocktree": extra_blocktree, "blocktree-bc": extra_blocktree_bc, "grid": extra_grid,
          "serve": extra_serve}
 
 

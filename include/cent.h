@@ -211,6 +211,9 @@ int  cent_cc(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k,
  * Closeness answers (Seq_CC/CC.cpp:2621-2684): "<dataset>_ans.txt" next to a
  * ".mtx" dataset, one "id cc" line per vertex; read assigns ans[id]. */
 int  cent_write_bc_scores(const char* path, const struct CSR* csr, const double* bc);
+/* The input side of the same formats: csr as a reference edge-list file
+ * ("n m", then "u v" once per undirected edge). */
+int  cent_write_edgelist(const char* path, const struct CSR* csr);
 int  cent_read_bc_scores(const char* path, int32_t* start, int32_t* end, double* out, int32_t cap);
 int  cent_ans_path(const char* dataset_path, char* out, size_t cap);
 int  cent_read_cc_answers(const char* path, const struct CSR* csr, int64_t* ans);

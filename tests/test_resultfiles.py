@@ -68,3 +68,18 @@ def test_answer_file_rejects_bad_input(tmp_path):
     bad.write_text("".join(f"{v} 1\n" for v in range(1, 34)) + "99 3\n")  # id out of range
     with pytest.raises(bc_amd.CentError):
         bc_amd.read_cc_answers(str(bad), g)
+
+
+def test_edgelist_writer_roundtrip(tmp_path):
+    for g in (bc_amd.gen_rmat(10, 8 << 10, seed=3), bc_amd.buildGraph(O.dataset_path("dolphins.txt"))):
+        p = str(tmp_path / "g.txt")
+        bc_amd.write_edgelist(p, g)
+        h = bc_amd.buildGraph(p)
+        assert h.n == g.n and h.nnz == g.nnz
+        assert np.array_equal(h.row_ptr, g.row_ptr) and np.array_equal(h.col, g.col)
+
+
+def test_edgelist_writer_refuses_ambiguous_base(tmp_path):
+    g = bc_amd.CSR.from_edges(4, np.array([1, 2]), np.array([2, 3]))  # 0-based, vertex 0 isolated
+    with pytest.raises(bc_amd.CentError):
+        bc_amd.write_edgelist(str(tmp_path / "x.txt"), g)
