@@ -36,6 +36,7 @@ CENT_OK, CENT_E_INVAL, CENT_E_OOM, CENT_E_DEVICE, CENT_E_OVERFLOW, CENT_E_IO = 0
 CENT_E_BUSY, CENT_E_CANCELLED, CENT_E_NOTFOUND, CENT_E_UNSUPPORTED = 6, 7, 8, 9
 CENT_BC_TEXTBOOK, CENT_BC_REF_COMPAT = 0, 1
 CENT_OPT_TIMING = 1
+CENT_OPT_SOURCE_ORDER = 2
 CENT_CSR_LCC = 1
 
 _lib.cent_strerror.restype = C.c_char_p
@@ -453,10 +454,11 @@ def tune_table_save(path: str) -> None:
     _check(_lib.cent_tune_table_save(path.encode()), "cent_tune_table_save", path)
 
 
-def _opts(mode=CENT_BC_TEXTBOOK, timing=False, grid_waves=0, frontier="auto", max_slots=0) -> _Opts:
+def _opts(mode=CENT_BC_TEXTBOOK, timing=False, grid_waves=0, frontier="auto", max_slots=0,
+          keep_source_order=False) -> _Opts:
     o = _Opts()
     o.bc_mode = mode
-    o.flags = CENT_OPT_TIMING if timing else 0
+    o.flags = (CENT_OPT_TIMING if timing else 0) | (CENT_OPT_SOURCE_ORDER if keep_source_order else 0)
     o.grid_waves = grid_waves
     o.frontier = FRONTIER[frontier]
     o.max_slots = max_slots
@@ -554,13 +556,14 @@ class Context:
 
     def bc(self, g: DeviceGraph, sources=None, mode: int = CENT_BC_TEXTBOOK, bc=None,
            farness: bool = True, dist_sum: bool = False, timing: bool = False,
-           grid_waves: int = 0, frontier: str = "auto", max_slots: int = 0):
+           grid_waves: int = 0, frontier: str = "auto", max_slots: int = 0,
+           keep_source_order: bool = False):
         """Brandes BC from ``sources`` (None: all).  Returns (bc, farness, dist_sum)."""
         s, k, kk = self._src(g, sources)
         out = np.zeros(g.n, np.float64) if bc is None else bc
         far = np.zeros(max(kk, 1), np.int64) if farness else None
         ds = np.zeros(max(g.n, 1), np.int64) if dist_sum else None
-        o = _opts(mode, timing, grid_waves, frontier, max_slots)
+        o = _opts(mode, timing, grid_waves, frontier, max_slots, keep_source_order)
         _check(_lib.cent_bc(self._h, g._h, _ptr(s, C.c_int32) if s is not None and k else None, k,
                             _ptr(out, C.c_double) if g.n else None,
                             _ptr(far, C.c_int64) if far is not None else None,
@@ -569,12 +572,13 @@ class Context:
         return out, (far[:kk] if far is not None else None), (ds[:g.n] if ds is not None else None)
 
     def cc(self, g: DeviceGraph, sources=None, dist_sum: bool = False, timing: bool = False,
-           grid_waves: int = 0, frontier: str = "auto", max_slots: int = 0):
+           grid_waves: int = 0, frontier: str = "auto", max_slots: int = 0,
+           keep_source_order: bool = False):
         """Closeness (farness) from ``sources`` (None: all).  Returns (farness, dist_sum)."""
         s, k, kk = self._src(g, sources)
         far = np.zeros(max(kk, 1), np.int64)
         ds = np.zeros(max(g.n, 1), np.int64) if dist_sum else None
-        o = _opts(0, timing, grid_waves, frontier, max_slots)
+        o = _opts(0, timing, grid_waves, frontier, max_slots, keep_source_order)
         _check(_lib.cent_cc(self._h, g._h, _ptr(s, C.c_int32) if s is not None and k else None, k,
                             _ptr(far, C.c_int64),
                             _ptr(ds, C.c_int64) if ds is not None else None, C.byref(o)),

@@ -279,6 +279,7 @@ struct cent_graph {
     int32_t ecc_probe = 0;            // BFS depth from the highest-degree vertex
     int32_t width_probe = 0;          // widest level of that BFS
     std::vector<int32_t> h_rp;        // device-order row offsets (host), for re-scheduling
+    std::vector<int32_t> probe_dist;  // BFS distance from the highest-degree vertex (caller ids, -1 unreached)
     cent_tune_sig sig{};              // autotuner signature
     int32_t grid_waves = 0;           // launch defaults when the call's opts leave them open
     int32_t frontier = CENT_FRONTIER_AUTO;
@@ -482,6 +483,7 @@ extern "C" int cent_graph_create(cent_ctx* ctx, const int32_t* row_ptr, const in
                 if (dist[(size_t)col[j]] < 0) { dist[(size_t)col[j]] = dist[(size_t)x] + 1; q[t++] = col[j]; }
         }
         g->ecc_probe = dist[(size_t)q[t - 1]];
+        g->probe_dist = dist;  // batch grouping key (caller ids)
         std::vector<int32_t> width((size_t)g->ecc_probe + 1, 0);
         for (size_t i = 0; i < t; ++i) width[(size_t)dist[(size_t)q[i]]]++;
         g->width_probe = *std::max_element(width.begin(), width.end());
@@ -999,11 +1001,37 @@ static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
     TRY(ensure(ctx, ctx->src, (size_t)(k > 0 ? k : 1)));
     TRY(ensure(ctx, ctx->cnt, 2));
     HIPTRY(hipMemsetAsync(ctx->cnt.p, 0, sizeof(unsigned long long) * 2, s));
+    // Batch order of the sources: the 64-lane steps group sources by BFS
+    // distance to the highest-degree vertex, then by degree, so the lanes
+    // of a batch tend to reach a vertex at the same levels (fewer
+    // (vertex, level) rows touched).  order[i] = caller index of slot i.
+    const int lm = k > 0 ? level_mode(g, opts) : CENT_FRONTIER_DENSE;
+    std::vector<int32_t> order;
+    if (k > kWave && lm != CENT_FRONTIER_SOURCE && !(opts->flags & CENT_OPT_SOURCE_ORDER) &&
+        !g->probe_dist.empty()) {
+        order.resize((size_t)k);
+        for (int32_t i = 0; i < k; ++i) order[(size_t)i] = i;
+        auto deg = [&](int32_t v) {
+            const int32_t x = g->new_id.empty() ? v : g->new_id[(size_t)v];
+            return g->h_rp[(size_t)x + 1] - g->h_rp[(size_t)x];
+        };
+        std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+            const int32_t sa = src[a], sb = src[b];
+            const int32_t da = g->probe_dist[(size_t)sa], db = g->probe_dist[(size_t)sb];
+            if (da != db) return da < db;
+            const int32_t ga = deg(sa), gb = deg(sb);
+            if (ga != gb) return ga > gb;
+            return sa < sb;
+        });
+    }
     std::vector<int32_t> dsrc;
     const int32_t* up_src = src;
-    if (!g->new_id.empty() && k > 0) {
+    if ((!g->new_id.empty() || !order.empty()) && k > 0) {
         dsrc.resize((size_t)k);
-        for (int32_t i = 0; i < k; ++i) dsrc[(size_t)i] = g->new_id[(size_t)src[i]];
+        for (int32_t i = 0; i < k; ++i) {
+            const int32_t s0 = order.empty() ? src[i] : src[order[(size_t)i]];
+            dsrc[(size_t)i] = g->new_id.empty() ? s0 : g->new_id[(size_t)s0];
+        }
         up_src = dsrc.data();
     }
     if (k > 0) HIPTRY(hipMemcpyAsync(ctx->src.p, up_src, sizeof(int) * (size_t)k, hipMemcpyHostToDevice, s));
@@ -1014,7 +1042,7 @@ static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
     ctx->ev_used = 0;
     const GView view = full_view(g->d);
     int64_t levels = 0;
-    const int lmode = k > 0 ? level_mode(g, opts) : CENT_FRONTIER_DENSE;
+    const int lmode = lm;
     bool batched = true;
     if (lmode == CENT_FRONTIER_SOURCE) {
         const int st = run_persource(ctx, g, sp, k, tm, &levels);
@@ -1051,10 +1079,19 @@ static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
         hd.resize(N);
         HIPTRY(hipMemcpyAsync(hd.data(), ctx->dist_sum.p, sizeof(int64_t) * N, hipMemcpyDeviceToHost, s));
     }
-    if (farness && k > 0)
-        HIPTRY(hipMemcpyAsync(farness, ctx->far.p, sizeof(int64_t) * (size_t)k, hipMemcpyDeviceToHost, s));
+    std::vector<int64_t> hf;
+    if (farness && k > 0) {
+        if (order.empty()) {
+            HIPTRY(hipMemcpyAsync(farness, ctx->far.p, sizeof(int64_t) * (size_t)k, hipMemcpyDeviceToHost, s));
+        } else {
+            hf.resize((size_t)k);
+            HIPTRY(hipMemcpyAsync(hf.data(), ctx->far.p, sizeof(int64_t) * (size_t)k, hipMemcpyDeviceToHost, s));
+        }
+    }
     HIPTRY(hipMemcpyAsync(hc, ctx->cnt.p, sizeof(hc), hipMemcpyDeviceToHost, s));
     HIPTRY(hipStreamSynchronize(s));
+    if (!hf.empty())
+        for (int32_t i = 0; i < k; ++i) farness[order[(size_t)i]] = hf[(size_t)i];
     const bool remap = !g->new_id.empty();
     if (with_bc && bc)
         for (int v = 0; v < n; ++v) bc[v] += hb[remap ? (size_t)g->new_id[(size_t)v] : (size_t)v];

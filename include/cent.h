@@ -121,6 +121,11 @@ typedef struct cent_graph cent_graph;  /* device-resident CSR                  *
 
 /* opts.flags */
 #define CENT_OPT_TIMING    1   /* record per-kernel HIP events (stats)         */
+#define CENT_OPT_SOURCE_ORDER 2 /* 64-lane steps: batch the sources in the given
+                                   order (default: grouped by BFS distance to
+                                   the highest-degree vertex, then degree, so a
+                                   batch's lanes share levels; results are the
+                                   same up to FP64 summation order)          */
 
 /* opts.frontier: how a level finds its work.  DENSE scans every vertex
  * (pull; best for low-diameter graphs), SPARSE keeps compacted frontier

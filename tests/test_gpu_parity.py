@@ -224,3 +224,52 @@ def test_source_mode_deep_fallback(ctx):
     far, ds = ctx.cc(dg, [0], dist_sum=True, frontier="source")
     assert far[0] == n * (n - 1) // 2
     assert np.array_equal(ds, np.arange(n, dtype=np.int64))
+
+
+def diamond_chain(k):
+    """k diamonds h_i - {a_i, b_i} - h_{i+1} in a row: 2^i shortest paths from
+    h_0 to h_i, so FP64 path counts leave their range after 1023 diamonds."""
+    u, v = [], []
+    for i in range(k):
+        h, a, b, h2 = 3 * i, 3 * i + 1, 3 * i + 2, 3 * i + 3
+        u += [h, h, a, b]
+        v += [a, b, h2, h2]
+    return bc_amd.CSR.from_edges(3 * k + 1, np.array(u), np.array(v))
+
+
+@pytest.mark.parametrize("frontier", ["dense", "sparse", "source"])
+def test_path_count_overflow_is_reported(ctx, frontier):
+    print(f"[gpu] overflow frontier={frontier}", flush=True)
+    g = diamond_chain(1100)
+    dg = ctx.upload(g, keep_order=True)
+    with pytest.raises(bc_amd.CentError) as e:
+        ctx.bc(dg, [0], frontier=frontier)
+    assert e.value.status == 4  # CENT_E_OVERFLOW
+    # closeness has no path counts: farness of h_0 in closed form
+    far, _ = ctx.cc(dg, [0], frontier=frontier)
+    k = 1100
+    assert far[0] == sum(2 * i for i in range(1, k + 1)) + sum(2 * (2 * i + 1) for i in range(k))
+    # below the limit the counts are exact powers of two and BC is finite
+    g2 = diamond_chain(1000)
+    dg2 = ctx.upload(g2, keep_order=True)
+    bc, far2, _ = ctx.bc(dg2, [0], frontier=frontier)
+    obc, ofar, _, _ = O.brandes(g2, [0])
+    assert_bc_close(bc, obc)
+    assert np.array_equal(far2, ofar)
+
+
+@pytest.mark.parametrize("frontier", ["dense", "sparse"])
+def test_batch_grouping_keeps_caller_order(ctx, frontier):
+    """The 64-lane steps regroup sources into batches; farness comes back in
+    the caller's order and every result matches the given-order run."""
+    g = bc_amd.gen_rmat(13, 16 << 13, seed=8)
+    dg = ctx.upload(g)
+    src = bc_amd.sample_sources(g.n, 300, 12)
+    a = ctx.bc(dg, src, dist_sum=True, frontier=frontier)
+    b = ctx.bc(dg, src, dist_sum=True, frontier=frontier, keep_source_order=True)
+    obc, ofar, ods, _ = O.brandes(g, src)
+    for bc, far, ds in (a, b):
+        assert_bc_close(bc, obc)
+        assert np.array_equal(far, ofar) and np.array_equal(ds, ods)
+    c = ctx.cc(dg, src, frontier=frontier)[0]
+    assert np.array_equal(c, ofar)
