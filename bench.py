@@ -469,6 +469,10 @@ def main() -> int:
         key = "bwd_pass" if kname.startswith("k_bwd") else "fwd_pass"
         roofline["traffic"] = tr[key]["traffic_bytes_per_launch"]
         roofline["traffic_unit"] = "bytes per launch of the level pass"
+        # counter-measured bytes over the live pass time: the HBM-side rate
+        tgbs = tr[key]["traffic_bytes_per_launch"] * launches / (ms * 1e-3) / 1e9
+        roofline["traffic_GBs"] = round(tgbs, 1)
+        roofline["traffic_frac"] = round(tgbs / HBM_PEAK_GBS, 4)
         roofline["algorithmic_bytes_per_launch"] = int(b / launches)
         roofline["traffic_source"] = tr["source"]
 
