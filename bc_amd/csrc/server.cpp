@@ -89,18 +89,32 @@ static double now_ms(const cent_server* s) {
     return std::chrono::duration<double, std::milli>(clk::now() - s->t0).count();
 }
 
-// Device bytes a request needs besides the graph (the context workspace the
-// engine sizes for it: per vertex 2 x 512 B of path counts / coefficients,
-// frontier masks, partials).
+// Source-parallel requests (deep graphs) run at most this many per-source
+// slots, so their workspace stays inside the estimate below.
+constexpr int32_t kSourceSlots = 64;
+
+// Device bytes a request needs besides the graph: the larger of the context
+// workspace of the 64-lane step (per vertex 2 x 512 B of path counts /
+// coefficients, frontier masks, partials) and of the source-parallel step
+// (kSourceSlots slots of 16-byte vertex records, BFS order and BC partials,
+// plus a level table of up to 64 Ki entries each).
 static int64_t workspace_estimate(const GraphEntry* g, int32_t kind, bool all, int64_t k) {
     const int64_t n = std::max<int64_t>(g->n, 1);
-    if (kind == CENT_Q_BC) return n * (1024 + 8 + 17 * 8 + 8 + 64) + k * 16 + (64 << 20);
+    const int64_t slots = std::min<int64_t>(std::max<int64_t>(k, 1), kSourceSlots);
+    const int64_t lcap = std::min<int64_t>(n + 2, 65536);
+    if (kind == CENT_Q_BC) {
+        const int64_t dense = n * (1024 + 8 + 17 * 8 + 8 + 64) + k * 16 + (64 << 20);
+        const int64_t source = slots * (n * (16 + 4 + 8) + lcap * 4) + n * 8 + k * 16 + (16 << 20);
+        return std::max(dense, source);
+    }
     if (all && g->red) {
         cent_reduced_view v;
         cent_reduced_get_view(g->red, &v);
         return (int64_t)v.np * (8 + 8 + 3 * 8 + 64) + (16 << 20);
     }
-    return n * (8 + 3 * 8 + 8 + 64) + k * 16 + (16 << 20);
+    const int64_t dense = n * (8 + 3 * 8 + 8 + 64) + k * 16 + (16 << 20);
+    const int64_t source = slots * (n * (16 + 4) + lcap * 4) + k * 16 + (16 << 20);
+    return std::max(dense, source);
 }
 
 static int64_t graph_estimate(const GraphEntry* g) {
@@ -142,6 +156,8 @@ static void run_request(cent_server* s, int slot, Request* r, GraphEntry* g) {
         if (!st && r->kind == CENT_Q_CC && r->all && !g->red) st = cent_reduce(g->host, &g->red);
     }
     const int64_t k = r->all ? g->n : (int64_t)r->src.size();
+    cent_opts opts{};
+    opts.max_slots = kSourceSlots;  // the admission estimate counts this many
     if (!st) {
         if (dry) {
             // modelled duration: sources x directed edges at 1e9 edge visits/s
@@ -151,13 +167,13 @@ static void run_request(cent_server* s, int slot, Request* r, GraphEntry* g) {
             r->bc.assign((size_t)g->n, 0.0);
             r->far.assign((size_t)std::max<int64_t>(k, 1), 0);
             st = cent_bc(ctx, g->dev, r->all ? nullptr : r->src.data(), r->all ? -1 : (int32_t)k, r->bc.data(),
-                         r->far.data(), nullptr, nullptr);
+                         r->far.data(), nullptr, &opts);
         } else if (r->all) {
             r->far.assign((size_t)std::max<int64_t>(g->n, 1), 0);
             st = cent_cc_reduced(ctx, g->red, r->far.data(), CENT_SCHED_PERSISTENT, nullptr);
         } else {
             r->far.assign((size_t)std::max<int64_t>(k, 1), 0);
-            st = cent_cc(ctx, g->dev, r->src.data(), (int32_t)k, r->far.data(), nullptr, nullptr);
+            st = cent_cc(ctx, g->dev, r->src.data(), (int32_t)k, r->far.data(), nullptr, &opts);
         }
     }
     // keep a slot's cached workspace within its share of the budget
