@@ -393,15 +393,15 @@ class TuneSig(C.Structure):
 CENT_TUNE_BC = 1
 CENT_TUNE_CC = 2
 CENT_GRAPH_NO_TUNE = 2
-_lib.cent_graph_get_layout.argtypes = [C.c_void_p, C.POINTER(TuneParams)]
-_lib.cent_graph_set_layout.argtypes = [C.c_void_p, C.POINTER(TuneParams)]
+_lib.cent_graph_get_layout.argtypes = [C.c_void_p, C.c_int32, C.POINTER(TuneParams)]
+_lib.cent_graph_set_layout.argtypes = [C.c_void_p, C.c_int32, C.POINTER(TuneParams)]
 _lib.cent_graph_signature.argtypes = [C.c_void_p, C.POINTER(TuneSig)]
 _lib.cent_tune.argtypes = [C.c_void_p, C.c_void_p, _I32P, C.c_int32, C.c_int32, C.c_int32,
                            C.POINTER(TuneParams)]
 _lib.cent_tune_signature.argtypes = [_I32P, C.c_int32, C.c_int64, C.c_int32, C.POINTER(TuneSig)]
 _lib.cent_tune_params_check.argtypes = [C.POINTER(TuneParams)]
-_lib.cent_tune_table_put.argtypes = [C.POINTER(TuneSig), C.POINTER(TuneParams)]
-_lib.cent_tune_table_get.argtypes = [C.POINTER(TuneSig), C.POINTER(TuneParams)]
+_lib.cent_tune_table_put.argtypes = [C.POINTER(TuneSig), C.c_int32, C.POINTER(TuneParams)]
+_lib.cent_tune_table_get.argtypes = [C.POINTER(TuneSig), C.c_int32, C.POINTER(TuneParams)]
 _lib.cent_tune_table_size.restype = C.c_int32
 _lib.cent_tune_table_clear.restype = None
 _lib.cent_tune_table_load.argtypes = [C.c_char_p]
@@ -424,14 +424,14 @@ def tune_params(hub_deg=256, chunk_edges=1024, grid_waves=0, frontier="auto", ms
     return TuneParams(hub_deg, chunk_edges, grid_waves, FRONTIER[frontier], ms)
 
 
-def tune_table_put(sig: tuple, p: TuneParams) -> None:
-    _check(_lib.cent_tune_table_put(C.byref(TuneSig(*sig)), C.byref(p)), "cent_tune_table_put")
+def tune_table_put(sig: tuple, p: TuneParams, what: int = CENT_TUNE_BC) -> None:
+    _check(_lib.cent_tune_table_put(C.byref(TuneSig(*sig)), what, C.byref(p)), "cent_tune_table_put")
 
 
-def tune_table_get(sig: tuple):
-    """Entry nearest to ``sig`` (exact, else within 2 buckets), or None."""
+def tune_table_get(sig: tuple, what: int = CENT_TUNE_BC):
+    """Entry for call type ``what`` nearest to ``sig`` (exact, else within 2 buckets), or None."""
     p = TuneParams()
-    st = _lib.cent_tune_table_get(C.byref(TuneSig(*sig)), C.byref(p))
+    st = _lib.cent_tune_table_get(C.byref(TuneSig(*sig)), what, C.byref(p))
     if st == 8:  # CENT_E_NOTFOUND
         return None
     _check(st, "cent_tune_table_get")
@@ -479,14 +479,18 @@ class DeviceGraph:
                "cent_graph_create", ctx.last_error())
         self._h = h
 
-    @property
-    def layout(self) -> TuneParams:
+    def layout_for(self, what: int) -> TuneParams:
+        """Launch settings used for betweenness (CENT_TUNE_BC) or closeness (CENT_TUNE_CC) calls."""
         p = TuneParams()
-        _check(_lib.cent_graph_get_layout(self._h, C.byref(p)), "cent_graph_get_layout")
+        _check(_lib.cent_graph_get_layout(self._h, what, C.byref(p)), "cent_graph_get_layout")
         return p
 
-    def set_layout(self, p: TuneParams) -> None:
-        _check(_lib.cent_graph_set_layout(self._h, C.byref(p)), "cent_graph_set_layout",
+    @property
+    def layout(self) -> TuneParams:
+        return self.layout_for(CENT_TUNE_BC)
+
+    def set_layout(self, p: TuneParams, what: int = CENT_TUNE_BC | CENT_TUNE_CC) -> None:
+        _check(_lib.cent_graph_set_layout(self._h, what, C.byref(p)), "cent_graph_set_layout",
                self.ctx.last_error())
 
     @property
@@ -608,6 +612,11 @@ class Context:
 CENT_SERVER_DRYRUN = 1
 CENT_Q_BC, CENT_Q_CC = 0, 1
 REQ_STATE = {0: "queued", 1: "running", 2: "done", 3: "failed", 4: "cancelled"}
+
+
+#: Source-parallel requests run by the server use at most this many per-source
+#: slots (bc_amd/csrc/server.cpp kSourceSlots; the admission estimate counts them).
+SERVER_SOURCE_SLOTS = 64
 
 
 class _ServerOpts(C.Structure):

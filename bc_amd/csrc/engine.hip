@@ -281,8 +281,10 @@ struct cent_graph {
     std::vector<int32_t> h_rp;        // device-order row offsets (host), for re-scheduling
     std::vector<int32_t> probe_dist;  // BFS distance from the highest-degree vertex (caller ids, -1 unreached)
     cent_tune_sig sig{};              // autotuner signature
-    int32_t grid_waves = 0;           // launch defaults when the call's opts leave them open
-    int32_t frontier = CENT_FRONTIER_AUTO;
+    // launch defaults when the call's opts leave them open: [0] betweenness,
+    // [1] closeness
+    int32_t grid_waves[2] = {0, 0};
+    int32_t frontier[2] = {CENT_FRONTIER_AUTO, CENT_FRONTIER_AUTO};
 };
 
 // Level step for a plain call.  Deep graphs (BFS depth above kDeepLevels
@@ -495,13 +497,17 @@ extern "C" int cent_graph_create(cent_ctx* ctx, const int32_t* row_ptr, const in
     }
     g->h_rp.assign(urp, urp + (size_t)n + 1);
     (void)cent_tune_signature(g->h_rp.data(), n, nnz, g->ecc_probe, &g->sig);
-    cent_tune_params tp;
-    if (!(flags & CENT_GRAPH_NO_TUNE) && cent_tune_table_get(&g->sig, &tp) == CENT_OK) {
-        tp.ms = 0.0;
-        st = cent_graph_set_layout(g, &tp);
-        if (st) {
-            cent_graph_free(g);
-            return st;
+    if (!(flags & CENT_GRAPH_NO_TUNE)) {
+        // closeness first, so the betweenness entry's chunk/hub layout wins
+        for (int32_t what : {CENT_TUNE_CC, CENT_TUNE_BC}) {
+            cent_tune_params tp;
+            if (cent_tune_table_get(&g->sig, what, &tp) != CENT_OK) continue;
+            tp.ms = 0.0;
+            st = cent_graph_set_layout(g, what, &tp);
+            if (st) {
+                cent_graph_free(g);
+                return st;
+            }
         }
     }
     *out = g;
@@ -509,18 +515,23 @@ extern "C" int cent_graph_create(cent_ctx* ctx, const int32_t* row_ptr, const in
 }
 
 // ------------------------------------------------------------ tuning hooks
-extern "C" int cent_graph_get_layout(const cent_graph* g, cent_tune_params* out) {
-    if (!g || !out) return CENT_E_INVAL;
+static bool what_ok(int32_t what, bool both_ok) {
+    return what == CENT_TUNE_BC || what == CENT_TUNE_CC || (both_ok && what == (CENT_TUNE_BC | CENT_TUNE_CC));
+}
+
+extern "C" int cent_graph_get_layout(const cent_graph* g, int32_t what, cent_tune_params* out) {
+    if (!g || !out || !what_ok(what, false)) return CENT_E_INVAL;
+    const int i = what == CENT_TUNE_BC ? 0 : 1;
     out->hub_deg = g->d.hub_deg;
     out->chunk_edges = g->d.chunk_edges;
-    out->grid_waves = g->grid_waves;
-    out->frontier = g->frontier;
+    out->grid_waves = g->grid_waves[i];
+    out->frontier = g->frontier[i];
     out->ms = 0.0;
     return CENT_OK;
 }
 
-extern "C" int cent_graph_set_layout(cent_graph* g, const cent_tune_params* p) {
-    if (!g) return CENT_E_INVAL;
+extern "C" int cent_graph_set_layout(cent_graph* g, int32_t what, const cent_tune_params* p) {
+    if (!g || !what_ok(what, true)) return CENT_E_INVAL;
     TRY(cent_tune_params_check(p));
     if (p->hub_deg != g->d.hub_deg || p->chunk_edges != g->d.chunk_edges) {
         cent_ctx* ctx = g->ctx;
@@ -528,8 +539,11 @@ extern "C" int cent_graph_set_layout(cent_graph* g, const cent_tune_params* p) {
         HIPTRY(hipDeviceSynchronize());  // no launch may still read the old tables
         TRY(devcsr_schedule(ctx, g->h_rp.data(), nullptr, 0, p->hub_deg, p->chunk_edges, g->d));
     }
-    g->grid_waves = p->grid_waves;
-    g->frontier = p->frontier;
+    for (int i = 0; i < 2; ++i)
+        if (what & (i == 0 ? CENT_TUNE_BC : CENT_TUNE_CC)) {
+            g->grid_waves[i] = p->grid_waves;
+            g->frontier[i] = p->frontier;
+        }
     return CENT_OK;
 }
 
@@ -611,12 +625,13 @@ struct BatchSpec {
 };
 
 // Workspace for a DevCSR with N vertices; the F store grows on demand.
-static int ensure_workspace(cent_ctx* ctx, const DevCSR& d, const BatchSpec& sp, int G) {
+static int ensure_workspace(cent_ctx* ctx, const DevCSR& d, const BatchSpec& sp, int G, int nb) {
     const size_t N = (size_t)(d.n > 0 ? d.n : 1);
-    TRY(ensure(ctx, ctx->vis, N));
-    const size_t want_levels = sp.with_bc ? 16 : 2;
-    if (ctx->F_n != N || ctx->F.cap < N * (want_levels + 1)) {
-        TRY(ensure(ctx, ctx->F, N * (want_levels + 1)));
+    TRY(ensure(ctx, ctx->vis, N * (size_t)nb));
+    // BC keeps one mask array per level; closeness a level-parity pair per batch
+    const size_t want = sp.with_bc ? N * 17 : N * (2 * (size_t)nb + 1);
+    if (ctx->F_n != N || ctx->F.cap < want) {
+        TRY(ensure(ctx, ctx->F, want));
         ctx->F_n = N;
     }
     ctx->F_levels = ctx->F.cap / N - 1;
@@ -625,14 +640,14 @@ static int ensure_workspace(cent_ctx* ctx, const DevCSR& d, const BatchSpec& sp,
         TRY(ensure(ctx, ctx->coef, N * kWave));
         if (sp.compat) TRY(ensure(ctx, ctx->npred, N * kWave));
     }
-    TRY(ensure(ctx, ctx->part_far, (size_t)G * kWave));
-    TRY(ensure(ctx, ctx->part_cnt, (size_t)G * 2));
+    TRY(ensure(ctx, ctx->part_far, (size_t)G * kWave * nb));
+    TRY(ensure(ctx, ctx->part_cnt, (size_t)G * 2 * nb));
     TRY(ensure(ctx, ctx->cnt, 2));
     TRY(ensure(ctx, ctx->flags, 2));
     if (d.nhub) {
         TRY(ensure(ctx, ctx->hp_acc, (size_t)d.nseg * kWave));
         TRY(ensure(ctx, ctx->hp_cnt, (size_t)d.nseg * kWave));
-        TRY(ensure(ctx, ctx->hp_or, (size_t)d.nseg));
+        TRY(ensure(ctx, ctx->hp_or, (size_t)d.nseg * nb));
     }
     return CENT_OK;
 }
@@ -651,37 +666,52 @@ static int grow_levels(cent_ctx* ctx, size_t N, int keep_levels) {
     return CENT_OK;
 }
 
-// Batch totals from the G per-workgroup partials: far_out[0..kb) assigned,
-// ctx->cnt accumulated.  One reducing workgroup per 128 partial rows, at
-// most 32.
-static int reduce_batch(cent_ctx* ctx, int G, int kb, int64_t* far_out) {
+// Totals of batch b from its G per-workgroup partials: far_out[0..kb)
+// assigned, ctx->cnt accumulated.  One reducing workgroup per 128 partial
+// rows, at most 32.
+static int reduce_batch(cent_ctx* ctx, int G, int kb, int64_t* far_out, int b = 0) {
     hipStream_t s = ctx->stream;
     HIPTRY(hipMemsetAsync(far_out, 0, sizeof(int64_t) * (size_t)(kb > 0 ? kb : 1), s));
     const int rg = std::max(1, std::min(32, G / 128));
-    k_reduce_batch<<<rg, 1024, 0, s>>>(ctx->part_far.p, ctx->part_cnt.p, G, kb, far_out, ctx->cnt.p);
+    k_reduce_batch<<<rg, 1024, 0, s>>>(ctx->part_far.p + (size_t)b * G * kWave, ctx->part_cnt.p + (size_t)b * G * 2,
+                                       G, kb, far_out, ctx->cnt.p);
     HIPTRY(hipGetLastError());
     return CENT_OK;
 }
 
-// One batch of kb <= 64 sources (device ids d_src[0..kb)) whose BFS stays
-// inside the view's vertex range.  far_out (device, kb entries) gets the
-// weighted distance sums.  Adds the batch's depth to *levels.
-static int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d_src, int kb,
+// Batches of the k sources d_src[0..k) (device ids) whose BFS stays inside
+// the view's vertex range: one 64-lane batch for BC (k <= 64), up to
+// kMultiBatch batches of 64 consecutive sources side by side for closeness
+// (each level is one launch over all of them, so launches and host
+// round trips are shared).  far_out (device, k entries) gets the weighted
+// distance sums.  Adds the depth to *levels once per batch.
+constexpr int kMultiBatch = 8;
+
+static int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d_src, int k,
                        int64_t* far_out, KernelTimer& tm, int64_t* levels) {
     const DevCSR& d = *v.d;
     hipStream_t s = ctx->stream;
     const size_t N = (size_t)(d.n > 0 ? d.n : 1);
     const size_t R = (size_t)(v.hi - v.lo);
+    const int nb = (k + kWave - 1) / kWave;
+    if (nb < 1 || nb > kMultiBatch || (sp.with_bc && nb != 1)) return CENT_E_INVAL;
     const int G = grid_for(v.nchunk, sp.opts);
-    TRY(ensure_workspace(ctx, d, sp, G));
-    const uint64_t A = (kb == 64) ? ~0ull : ((1ull << kb) - 1);
-    HIPTRY(hipMemsetAsync(ctx->vis.p + v.lo, 0, sizeof(uint64_t) * R, s));
-    HIPTRY(hipMemsetAsync(ctx->F.p + v.lo, 0, sizeof(uint64_t) * R, s));
-    HIPTRY(hipMemsetAsync(ctx->F.p + N + v.lo, 0, sizeof(uint64_t) * R, s));
-    HIPTRY(hipMemsetAsync(ctx->part_far.p, 0, sizeof(int64_t) * (size_t)G * kWave, s));
-    HIPTRY(hipMemsetAsync(ctx->part_cnt.p, 0, sizeof(unsigned long long) * (size_t)G * 2, s));
+    TRY(ensure_workspace(ctx, d, sp, G, nb));
+    const int kl = k - (nb - 1) * kWave;  // sources of the last batch
+    const uint64_t A_last = (kl == 64) ? ~0ull : ((1ull << kl) - 1);
+    const uint64_t A = nb == 1 ? A_last : ~0ull;
+    const size_t planes = sp.with_bc ? 1 : (size_t)nb;  // closeness: batch b's level-parity plane p at (p*nb + b)*N
+    for (int b = 0; b < nb; ++b) {
+        HIPTRY(hipMemsetAsync(ctx->vis.p + (size_t)b * N + v.lo, 0, sizeof(uint64_t) * R, s));
+        HIPTRY(hipMemsetAsync(ctx->F.p + (size_t)b * N + v.lo, 0, sizeof(uint64_t) * R, s));
+        HIPTRY(hipMemsetAsync(ctx->F.p + (planes + b) * N + v.lo, 0, sizeof(uint64_t) * R, s));
+    }
+    HIPTRY(hipMemsetAsync(ctx->part_far.p, 0, sizeof(int64_t) * (size_t)G * kWave * nb, s));
+    HIPTRY(hipMemsetAsync(ctx->part_cnt.p, 0, sizeof(unsigned long long) * (size_t)G * 2 * nb, s));
     HIPTRY(hipMemsetAsync(ctx->flags.p, 0, sizeof(int) * 2, s));
-    k_seed<<<1, kWave, 0, s>>>(d_src, kb, ctx->F.p, ctx->vis.p, sp.with_bc ? ctx->sigma.p : nullptr, sp.with_bc);
+    for (int b = 0; b < nb; ++b)
+        k_seed<<<1, kWave, 0, s>>>(d_src + b * kWave, b == nb - 1 ? kl : kWave, ctx->F.p + (size_t)b * N,
+                                   ctx->vis.p + (size_t)b * N, sp.with_bc ? ctx->sigma.p : nullptr, sp.with_bc);
     HIPTRY(hipGetLastError());
 
     HubArgs ha{};
@@ -698,11 +728,14 @@ static int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const
     int dl = 0;  // current frontier distance
     for (;;) {
         if (bc && (size_t)(dl + 2) > ctx->F_levels) TRY(grow_levels(ctx, N, dl + 1));
-        const uint64_t* Fd = ctx->F.p + N * (size_t)(bc ? dl : (dl & 1));
-        uint64_t* Fn = ctx->F.p + N * (size_t)(bc ? dl + 1 : ((dl + 1) & 1));
-        if (!bc || dl + 1 > 1) HIPTRY(hipMemsetAsync(Fn + v.lo, 0, sizeof(uint64_t) * R, s));
+        const uint64_t* Fd = ctx->F.p + N * (size_t)(bc ? dl : planes * (dl & 1));
+        uint64_t* Fn = ctx->F.p + N * (size_t)(bc ? dl + 1 : planes * ((dl + 1) & 1));
+        if (!bc || dl + 1 > 1)
+            for (int b = 0; b < nb; ++b)
+                HIPTRY(hipMemsetAsync(Fn + (size_t)b * N + v.lo, 0, sizeof(uint64_t) * R, s));
         HIPTRY(hipMemsetAsync(ctx->flags.p, 0, sizeof(int), s));
-        FwdArgs fa;
+        FwdArgs fa{};
+        fa.bstride = N; fa.pslots = G; fa.A_last = A_last; fa.dist_atomic = nb > 1;
         fa.n = d.n; fa.hub_deg = d.hub_deg; fa.nchunk = v.nchunk; fa.chunk_beg = d.chunk_beg + v.c0;
         fa.rp = d.rp; fa.col = d.col; fa.vis = ctx->vis.p; fa.Fd = Fd; fa.Fn = Fn;
         fa.sigma = bc ? ctx->sigma.p : nullptr;
@@ -718,7 +751,7 @@ static int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const
             if (compat) k_fwd<true, true><<<G, kBlock, 0, s>>>(fa);
             else k_fwd<true, false><<<G, kBlock, 0, s>>>(fa);
         } else {
-            k_fwd<false, false><<<G, kBlock, 0, s>>>(fa);
+            k_fwd<false, false><<<dim3(G, nb), kBlock, 0, s>>>(fa);
         }
         HIPTRY(hipGetLastError());
         if (v.nhub) {
@@ -727,8 +760,8 @@ static int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const
                 if (compat) k_fwd_hubfin<true, true><<<Ghub, kBlock, 0, s>>>(fa, ha);
                 else k_fwd_hubfin<true, false><<<Ghub, kBlock, 0, s>>>(fa, ha);
             } else {
-                k_fwd_hubseg<false><<<Gseg, kBlock, 0, s>>>(fa, ha);
-                k_fwd_hubfin<false, false><<<Ghub, kBlock, 0, s>>>(fa, ha);
+                k_fwd_hubseg<false><<<dim3(Gseg, nb), kBlock, 0, s>>>(fa, ha);
+                k_fwd_hubfin<false, false><<<dim3(Ghub, nb), kBlock, 0, s>>>(fa, ha);
             }
             HIPTRY(hipGetLastError());
         }
@@ -740,7 +773,7 @@ static int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const
         if (!ctx->h_flags[0]) break;
         ++dl;
     }
-    *levels += dl;  // deepest non-empty level
+    *levels += (int64_t)dl * nb;  // deepest non-empty level, per batch
     if (bc) {
         for (int lv = dl; lv >= 1; --lv) {
             BwdArgs ba;
@@ -768,8 +801,9 @@ static int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const
             ctx->stats.launches_bwd++;
         }
     }
-    TRY(reduce_batch(ctx, G, kb, far_out));
-    ctx->stats.batches++;
+    for (int b = 0; b < nb; ++b)
+        TRY(reduce_batch(ctx, G, b == nb - 1 ? kl : kWave, far_out + b * kWave, b));
+    ctx->stats.batches += nb;
     return CENT_OK;
 }
 
@@ -980,8 +1014,8 @@ static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
     // the graph's tuned launch settings fill what the call leaves open
     cent_opts eff{};
     if (opts) eff = *opts;
-    if (eff.grid_waves == 0) eff.grid_waves = g->grid_waves;
-    if (eff.frontier == CENT_FRONTIER_AUTO) eff.frontier = g->frontier;
+    if (eff.grid_waves == 0) eff.grid_waves = g->grid_waves[with_bc ? 0 : 1];
+    if (eff.frontier == CENT_FRONTIER_AUTO) eff.frontier = g->frontier[with_bc ? 0 : 1];
     opts = &eff;
     BatchSpec sp;
     sp.with_bc = with_bc;
@@ -1058,8 +1092,10 @@ static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
     const bool sparse = batched && lmode == CENT_FRONTIER_SPARSE;
     SpPlan plan;
     if (sparse) TRY(sp_prepare(ctx, g->d, sp, plan));
-    for (int32_t b0 = 0; batched && b0 < k; b0 += kWave) {
-        const int kb = (k - b0) < kWave ? (k - b0) : kWave;
+    // closeness on the dense step takes up to kMultiBatch batches per launch
+    const int32_t step = (!with_bc && !sparse) ? kWave * kMultiBatch : kWave;
+    for (int32_t b0 = 0; batched && b0 < k; b0 += step) {
+        const int kb = (k - b0) < step ? (k - b0) : step;
         if (sparse) TRY(level_batch_sparse(ctx, plan, sp, ctx->src.p + b0, kb, ctx->far.p + b0, tm, &levels));
         else TRY(level_batch(ctx, view, sp, ctx->src.p + b0, kb, ctx->far.p + b0, tm, &levels));
         if (tm.on && ctx->ev_used > 4096) {  // keep the event pool bounded
@@ -1183,7 +1219,7 @@ struct Tuner {
     int trial(const cent_tune_params& p, double* ms) {
         *ms = 1e300;
         if (budget == 0) return CENT_OK;
-        TRY(cent_graph_set_layout(g, &p));
+        TRY(cent_graph_set_layout(g, what, &p));
         double a = 0.0, b = 0.0;
         TRY(run_once(&a));
         TRY(run_once(&b));
@@ -1206,7 +1242,7 @@ extern "C" int cent_tune(cent_ctx* ctx, cent_graph* g, const int32_t* src, int32
     t.bc.assign((size_t)(g->n > 0 ? g->n : 1), 0.0);
     t.far.assign((size_t)k, 0);
     cent_tune_params orig, cur;
-    TRY(cent_graph_get_layout(g, &orig));
+    TRY(cent_graph_get_layout(g, what, &orig));
     cur = orig;
     cur.frontier = CENT_FRONTIER_DENSE;
     double best_ms = 1e300, ms = 0.0;
@@ -1245,12 +1281,12 @@ extern "C" int cent_tune(cent_ctx* ctx, cent_graph* g, const int32_t* src, int32
     ctx->stats = saved;
     if (!st && best_ms >= 1e300) st = CENT_E_INVAL;  // budget too small to time anything
     if (st) {
-        (void)cent_graph_set_layout(g, &orig);  // leave the graph as it came
+        (void)cent_graph_set_layout(g, what, &orig);  // leave the graph as it came
         return st;
     }
     cur.ms = best_ms;
-    TRY(cent_graph_set_layout(g, &cur));
-    TRY(cent_tune_table_put(&g->sig, &cur));
+    TRY(cent_graph_set_layout(g, what, &cur));
+    TRY(cent_tune_table_put(&g->sig, what, &cur));
     *best = cur;
     return CENT_OK;
 }
@@ -1378,7 +1414,23 @@ extern "C" int cent_cc_reduced(cent_ctx* ctx, const cent_reduced* rc, int64_t* f
             ctx->stats.launches_fwd++;
             ctx->stats.batches += rd->nsmall;
         }
-        for (int i : rd->big) TRY(run_item(i));
+        // large items: consecutive full items of one block (contiguous
+        // sources) go up to kMultiBatch at a time through one level loop
+        for (size_t j = 0; j < rd->big.size();) {
+            const cent_item& it = r->items[rd->big[j]];
+            int ns = it.ns;
+            size_t e = j + 1;
+            while (e < rd->big.size() && e - j < (size_t)kMultiBatch) {
+                const cent_item& pv = r->items[rd->big[e - 1]];
+                const cent_item& nx = r->items[rd->big[e]];
+                if (pv.ns != kWave || nx.lo != it.lo || nx.hi != it.hi || nx.s0 != pv.s0 + pv.ns) break;
+                ns += nx.ns;
+                ++e;
+            }
+            const GView v = sub_view(rd->d, it.lo, it.hi);
+            TRY(level_batch(ctx, v, sp, rd->iota + it.s0, ns, Dd + it.s0, tm, &levels));
+            j = e;
+        }
     }
     std::vector<int64_t> D(NP);
     HIPTRY(hipMemcpyAsync(D.data(), Dd, sizeof(int64_t) * NP, hipMemcpyDeviceToHost, s));

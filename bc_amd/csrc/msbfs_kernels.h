@@ -46,8 +46,15 @@ struct HubArgs {
     const int* __restrict__ hub_seg_off;  // [nhub + 1]
     double* __restrict__ part_acc;        // [nseg][64]
     uint32_t* __restrict__ part_cnt;      // [nseg][64]
-    uint64_t* __restrict__ part_or;       // [nseg]
+    uint64_t* __restrict__ part_or;       // [nseg] per batch
 };
+
+// Batch blockIdx.y's segment partials of the mask OR (closeness batches).
+__device__ __forceinline__ HubArgs hub_view(const HubArgs& h0) {
+    HubArgs h = h0;
+    h.part_or += (size_t)blockIdx.y * (size_t)h.nseg;
+    return h;
+}
 
 struct FwdArgs {
     int n;
@@ -69,7 +76,30 @@ struct FwdArgs {
     unsigned long long* __restrict__ part_cnt;  // [grid][2]: reached pairs, dag edges
     int* __restrict__ flag;               // set to 1 when anything was reached
     int* __restrict__ overflow;           // set to 1 when a path count is inf
+    // Several independent 64-source batches in one launch (closeness only):
+    // batch b = blockIdx.y uses vis / Fd / Fn + b * bstride, its own
+    // partial slots (pslots blocks each) and active lanes A (A_last for the
+    // last batch); dist_sum is then shared, so it is updated atomically.
+    size_t bstride;
+    int pslots;
+    uint64_t A_last;
+    bool dist_atomic;
 };
+
+// The arguments of batch blockIdx.y (the identity for a one-batch launch).
+__device__ __forceinline__ FwdArgs batch_view(const FwdArgs& a0) {
+    FwdArgs a = a0;
+    const int b = blockIdx.y;
+    if (b) {
+        a.vis += (size_t)b * a.bstride;
+        a.Fd += (size_t)b * a.bstride;
+        a.Fn += (size_t)b * a.bstride;
+        a.part_far += (size_t)b * a.pslots * kWave;
+        a.part_cnt += (size_t)b * a.pslots * 2;
+    }
+    if (b == (int)gridDim.y - 1) a.A = a.A_last;
+    return a;
+}
 
 struct BwdArgs {
     int n;
@@ -196,7 +226,10 @@ __device__ __forceinline__ void fwd_vertex_done(const FwdArgs& a, int v, uint64_
         a.Fn[v] = nm;
         a.vis[v] = old | nm;
         const int pc = __popcll(nm);
-        if (a.dist_sum) a.dist_sum[v] += (int64_t)a.dnext * pc;
+        if (a.dist_sum) {
+            if (a.dist_atomic) atomicAdd((unsigned long long*)&a.dist_sum[v], (unsigned long long)((int64_t)a.dnext * pc));
+            else a.dist_sum[v] += (int64_t)a.dnext * pc;
+        }
         r.nnew += (unsigned long long)pc;
     }
 }
@@ -251,7 +284,8 @@ __device__ __forceinline__ void bwd_vertex_done(const BwdArgs& a, int v, uint64_
 
 // ---------------------------------------------------------------- forward
 template <bool SIGMA, bool COMPAT>
-__global__ __launch_bounds__(kBlock) void k_fwd(FwdArgs a) {
+__global__ __launch_bounds__(kBlock) void k_fwd(FwdArgs a0) {
+    const FwdArgs a = batch_view(a0);
     const int lane = lane_id();
     const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const int nw = gridDim.x * kWavesPerBlock;
@@ -287,7 +321,9 @@ __global__ __launch_bounds__(kBlock) void k_fwd(FwdArgs a) {
 }
 
 template <bool SIGMA>
-__global__ __launch_bounds__(kBlock) void k_fwd_hubseg(FwdArgs a, HubArgs h) {
+__global__ __launch_bounds__(kBlock) void k_fwd_hubseg(FwdArgs a0, HubArgs h0) {
+    const FwdArgs a = batch_view(a0);
+    const HubArgs h = hub_view(h0);
     const int lane = lane_id();
     const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const int nw = gridDim.x * kWavesPerBlock;
@@ -310,7 +346,9 @@ __global__ __launch_bounds__(kBlock) void k_fwd_hubseg(FwdArgs a, HubArgs h) {
 }
 
 template <bool SIGMA, bool COMPAT>
-__global__ __launch_bounds__(kBlock) void k_fwd_hubfin(FwdArgs a, HubArgs h) {
+__global__ __launch_bounds__(kBlock) void k_fwd_hubfin(FwdArgs a0, HubArgs h0) {
+    const FwdArgs a = batch_view(a0);
+    const HubArgs h = hub_view(h0);
     const int lane = lane_id();
     const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const int nw = gridDim.x * kWavesPerBlock;

@@ -8,8 +8,13 @@
 // eccentricity probe + 1).  Lookups take the exact signature if present,
 // else the nearest entry within an L1 distance of 2 buckets.
 //
+// Entries are per call type (CENT_TUNE_BC or CENT_TUNE_CC): one graph can
+// want the source-parallel step for betweenness and the 512-source
+// bit-parallel step for closeness.
+//
 // File format, one entry per line, '#' starts a comment:
-//   lg_n deg_x2 skew lg_ecc  hub_deg chunk_edges grid_waves frontier  ms
+//   lg_n deg_x2 skew lg_ecc what  hub_deg chunk_edges grid_waves frontier  ms
+// (a line without the `what` field is a betweenness entry).
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -24,6 +29,7 @@ namespace {
 
 struct Entry {
     cent_tune_sig sig;
+    int32_t what;
     cent_tune_params p;
 };
 
@@ -65,24 +71,27 @@ extern "C" int cent_tune_signature(const int32_t* row_ptr, int32_t n, int64_t nn
     return CENT_OK;
 }
 
-extern "C" int cent_tune_table_put(const cent_tune_sig* sig, const cent_tune_params* p) {
-    if (!sig) return CENT_E_INVAL;
+static bool what_ok(int32_t what) { return what == CENT_TUNE_BC || what == CENT_TUNE_CC; }
+
+extern "C" int cent_tune_table_put(const cent_tune_sig* sig, int32_t what, const cent_tune_params* p) {
+    if (!sig || !what_ok(what)) return CENT_E_INVAL;
     if (int st = cent_tune_params_check(p)) return st;
     std::lock_guard<std::mutex> lk(g_mu);
     for (auto& e : g_table)
-        if (same_sig(e.sig, *sig)) {
+        if (e.what == what && same_sig(e.sig, *sig)) {
             e.p = *p;
             return CENT_OK;
         }
-    g_table.push_back(Entry{*sig, *p});
+    g_table.push_back(Entry{*sig, what, *p});
     return CENT_OK;
 }
 
-extern "C" int cent_tune_table_get(const cent_tune_sig* sig, cent_tune_params* out) {
-    if (!sig || !out) return CENT_E_INVAL;
+extern "C" int cent_tune_table_get(const cent_tune_sig* sig, int32_t what, cent_tune_params* out) {
+    if (!sig || !out || !what_ok(what)) return CENT_E_INVAL;
     std::lock_guard<std::mutex> lk(g_mu);
     int best = -1, bd = 3;
     for (size_t i = 0; i < g_table.size(); ++i) {
+        if (g_table[i].what != what) continue;
         const int d = sig_dist(g_table[i].sig, *sig);
         if (d < bd) {
             bd = d;
@@ -118,10 +127,33 @@ extern "C" int cent_tune_table_load(const char* path) {
         while (*p == ' ' || *p == '\t' || *p == '\r' || *p == '\n') ++p;
         if (!*p) continue;
         Entry e{};
-        const int got = sscanf(p, "%d %d %d %d %d %d %d %d %lf", &e.sig.lg_n, &e.sig.deg_x2, &e.sig.skew,
-                               &e.sig.lg_ecc, &e.p.hub_deg, &e.p.chunk_edges, &e.p.grid_waves, &e.p.frontier,
-                               &e.p.ms);
-        if (got != 9 || cent_tune_params_check(&e.p) != CENT_OK) {
+        int v[10] = {0};
+        double ms = 0.0;
+        int nf = 0;
+        {   // count the fields: 9 (betweenness entry) or 10 (with `what`)
+            char tmp[512];
+            strncpy(tmp, p, sizeof tmp - 1);
+            tmp[sizeof tmp - 1] = 0;
+            char* save = nullptr;
+            for (char* t = strtok_r(tmp, " \t\r\n", &save); t; t = strtok_r(nullptr, " \t\r\n", &save)) ++nf;
+        }
+        int got = -1;
+        if (nf == 10) {
+            got = sscanf(p, "%d %d %d %d %d %d %d %d %d %lf", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6],
+                         &v[7], &v[8], &ms);
+            e.sig = cent_tune_sig{v[0], v[1], v[2], v[3]};
+            e.what = v[4];
+            e.p = cent_tune_params{v[5], v[6], v[7], v[8], ms};
+            got = got == 10 ? 0 : -1;
+        } else if (nf == 9) {
+            got = sscanf(p, "%d %d %d %d %d %d %d %d %lf", &v[0], &v[1], &v[2], &v[3], &v[5], &v[6], &v[7], &v[8],
+                         &ms);
+            e.sig = cent_tune_sig{v[0], v[1], v[2], v[3]};
+            e.what = CENT_TUNE_BC;
+            e.p = cent_tune_params{v[5], v[6], v[7], v[8], ms};
+            got = got == 9 ? 0 : -1;
+        }
+        if (got != 0 || !what_ok(e.what) || cent_tune_params_check(&e.p) != CENT_OK) {
             st = CENT_E_IO;
             break;
         }
@@ -129,7 +161,7 @@ extern "C" int cent_tune_table_load(const char* path) {
     }
     fclose(f);
     if (st) return st;  // a malformed file changes nothing
-    for (const auto& e : add) cent_tune_table_put(&e.sig, &e.p);
+    for (const auto& e : add) cent_tune_table_put(&e.sig, e.what, &e.p);
     return CENT_OK;
 }
 
@@ -142,9 +174,9 @@ extern "C" int cent_tune_table_save(const char* path) {
     }
     FILE* f = fopen(path, "w");
     if (!f) return CENT_E_IO;
-    fprintf(f, "# cent autotuner table: lg_n deg_x2 skew lg_ecc  hub_deg chunk_edges grid_waves frontier  ms\n");
+    fprintf(f, "# cent autotuner table: lg_n deg_x2 skew lg_ecc what  hub_deg chunk_edges grid_waves frontier  ms\n");
     for (const auto& e : snap)
-        fprintf(f, "%d %d %d %d  %d %d %d %d  %.4f\n", e.sig.lg_n, e.sig.deg_x2, e.sig.skew, e.sig.lg_ecc,
+        fprintf(f, "%d %d %d %d %d  %d %d %d %d  %.4f\n", e.sig.lg_n, e.sig.deg_x2, e.sig.skew, e.sig.lg_ecc, e.what,
                 e.p.hub_deg, e.p.chunk_edges, e.p.grid_waves, e.p.frontier, e.p.ms);
     const bool ok = fflush(f) == 0;
     fclose(f);

@@ -252,10 +252,14 @@ typedef struct cent_tune_sig {
 #define CENT_TUNE_BC 1   /* trials time Brandes BC (+ farness)            */
 #define CENT_TUNE_CC 2   /* trials time closeness only                    */
 
-/* Current settings of an uploaded graph / new settings (the chunk and hub
- * tables are rebuilt on the device when hub_deg or chunk_edges change). */
-int  cent_graph_get_layout(const cent_graph* g, cent_tune_params* out);
-int  cent_graph_set_layout(cent_graph* g, const cent_tune_params* p);
+/* Current settings of an uploaded graph for the call type `what`
+ * (CENT_TUNE_BC or CENT_TUNE_CC) / new settings.  hub_deg and chunk_edges
+ * are shared by both call types (the chunk and hub tables are rebuilt on
+ * the device when they change); frontier and grid_waves are set for each
+ * call type named in `what` (CENT_TUNE_BC | CENT_TUNE_CC: both).  Not to be
+ * called while calls on g run. */
+int  cent_graph_get_layout(const cent_graph* g, int32_t what, cent_tune_params* out);
+int  cent_graph_set_layout(cent_graph* g, int32_t what, const cent_tune_params* p);
 int  cent_graph_signature(const cent_graph* g, cent_tune_sig* out);
 
 /* Times the plain call `what` on the sample sources src[0..k) (k <= 256;
@@ -263,21 +267,23 @@ int  cent_graph_signature(const cent_graph* g, cent_tune_sig* out);
  * width) for candidate settings by coordinate descent (level step, then
  * hub_deg, chunk_edges and grid_waves for the dense step or the slot count
  * for the source-parallel step; at most max_trials timed runs, 0 = no
- * limit), applies the fastest to g, records it in the process-wide table
- * under g's signature and returns it in *best.  Results never depend on the
+ * limit), applies the fastest to g for `what`, records it in the
+ * process-wide table under (g's signature, what) and returns it in *best.  Results never depend on the
  * settings beyond floating-point summation order. */
 int  cent_tune(cent_ctx* ctx, cent_graph* g, const int32_t* src, int32_t k, int32_t what,
                int32_t max_trials, cent_tune_params* best);
 
-/* Process-wide table (thread-safe).  cent_graph_create applies the entry
- * nearest to the new graph's signature (exact, else within 2 buckets)
- * unless CENT_GRAPH_NO_TUNE is given.  load merges a text file written by
- * save; a malformed file returns CENT_E_IO and changes nothing. */
+/* Process-wide table (thread-safe), one entry per (signature, call type).
+ * cent_graph_create applies, for each call type, the entry nearest to the
+ * new graph's signature (exact, else within 2 buckets; the chunk/hub layout
+ * comes from the betweenness entry when there is one) unless
+ * CENT_GRAPH_NO_TUNE is given.  load merges a text file written by save; a
+ * malformed file returns CENT_E_IO and changes nothing. */
 #define CENT_GRAPH_NO_TUNE 2
 int  cent_tune_signature(const int32_t* row_ptr, int32_t n, int64_t nnz, int32_t ecc, cent_tune_sig* out);
 int  cent_tune_params_check(const cent_tune_params* p);
-int  cent_tune_table_put(const cent_tune_sig* sig, const cent_tune_params* p);
-int  cent_tune_table_get(const cent_tune_sig* sig, cent_tune_params* out);
+int  cent_tune_table_put(const cent_tune_sig* sig, int32_t what, const cent_tune_params* p);
+int  cent_tune_table_get(const cent_tune_sig* sig, int32_t what, cent_tune_params* out);
 int32_t cent_tune_table_size(void);
 void cent_tune_table_clear(void);
 int  cent_tune_table_load(const char* path);

@@ -66,7 +66,8 @@ def test_tune_picks_applies_and_records(ctx, rmat):
     lay = dg.layout
     assert (lay.hub_deg, lay.chunk_edges, lay.grid_waves, lay.frontier) == \
         (best.hub_deg, best.chunk_edges, best.grid_waves, best.frontier)
-    assert bc_amd.tune_table_get(dg.signature).hub_deg == best.hub_deg
+    assert bc_amd.tune_table_get(dg.signature, bc_amd.CENT_TUNE_BC).hub_deg == best.hub_deg
+    assert bc_amd.tune_table_get(dg.signature, bc_amd.CENT_TUNE_CC) is None
     bc, far, _ = ctx.bc(dg, src)
     close(bc, obc)
     assert np.array_equal(far, ofar)
@@ -74,8 +75,15 @@ def test_tune_picks_applies_and_records(ctx, rmat):
     dg2 = ctx.upload(g)
     assert dg2.layout.hub_deg == best.hub_deg and dg2.layout.frontier == best.frontier
     # ... unless told not to
+    assert dg2.layout_for(bc_amd.CENT_TUNE_CC).frontier == bc_amd.FRONTIER["auto"]
     dg3 = ctx.upload(g, tuned=False)
     assert dg3.layout.hub_deg == 256 and dg3.layout.frontier == bc_amd.FRONTIER["auto"]
+    # closeness tuned separately: its level step is kept apart from betweenness'
+    cbest = ctx.tune(dg, src[:64], bc_amd.CENT_TUNE_CC, max_trials=4)
+    assert dg.layout_for(bc_amd.CENT_TUNE_CC).frontier == cbest.frontier
+    assert dg.layout_for(bc_amd.CENT_TUNE_BC).frontier == best.frontier
+    far, _ = ctx.cc(dg, src)
+    assert np.array_equal(far, ofar)
 
 
 def test_tune_rejects_bad_arguments(ctx, rmat):

@@ -57,7 +57,10 @@ def test_table_exact_nearest_and_miss():
 
 def test_table_save_load_roundtrip(tmp_path):
     p = bc_amd.tune_params(hub_deg=1024, chunk_edges=4096, grid_waves=32768, frontier="dense", ms=3.25)
-    bc_amd.tune_table_put((10, 6, 5, 2), p)
+    q = bc_amd.tune_params(hub_deg=512, chunk_edges=512, grid_waves=0, frontier="source", ms=1.5)
+    bc_amd.tune_table_put((10, 6, 5, 2), p, bc_amd.CENT_TUNE_BC)
+    bc_amd.tune_table_put((10, 6, 5, 2), q, bc_amd.CENT_TUNE_CC)
+    assert bc_amd.tune_table_size() == 2
     f = str(tmp_path / "t.txt")
     bc_amd.tune_table_save(f)
     bc_amd.tune_table_clear()
@@ -66,6 +69,20 @@ def test_table_save_load_roundtrip(tmp_path):
     got = bc_amd.tune_table_get((10, 6, 5, 2))
     assert got.as_dict() == {"hub_deg": 1024, "chunk_edges": 4096, "grid_waves": 32768,
                              "frontier": "dense", "ms": 3.25}
+    assert bc_amd.tune_table_get((10, 6, 5, 2), bc_amd.CENT_TUNE_CC).frontier == bc_amd.FRONTIER["source"]
+
+
+def test_table_entries_are_per_call_type(tmp_path):
+    bc_amd.tune_table_put((12, 8, 6, 2), bc_amd.tune_params(frontier="source"), bc_amd.CENT_TUNE_BC)
+    assert bc_amd.tune_table_get((12, 8, 6, 2), bc_amd.CENT_TUNE_CC) is None
+    # a line without the call-type field is a betweenness entry
+    f = tmp_path / "legacy.txt"
+    f.write_text("# old format\n7 7 7 7  256 1024 0 1  2.0\n")
+    bc_amd.tune_table_load(str(f))
+    assert bc_amd.tune_table_get((7, 7, 7, 7), bc_amd.CENT_TUNE_BC).frontier == 1
+    assert bc_amd.tune_table_get((7, 7, 7, 7), bc_amd.CENT_TUNE_CC) is None
+    with pytest.raises(bc_amd.CentError):
+        bc_amd.tune_table_put((1, 1, 1, 1), bc_amd.tune_params(), 3)
 
 
 def test_table_rejects_bad_input(tmp_path):
@@ -76,7 +93,7 @@ def test_table_rejects_bad_input(tmp_path):
     good = bc_amd.tune_params()
     bc_amd.tune_table_put((3, 3, 3, 3), good)
     f = tmp_path / "bad.txt"
-    f.write_text("# header\n5 5 5 5  256 1024 0 1  1.0\n5 5 5\n")
+    f.write_text("# header\n5 5 5 5 1  256 1024 0 1  1.0\n5 5 5\n")
     with pytest.raises(bc_amd.CentError):
         bc_amd.tune_table_load(str(f))
     assert bc_amd.tune_table_size() == 1        # nothing merged from the bad file
