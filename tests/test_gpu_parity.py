@@ -273,3 +273,39 @@ def test_batch_grouping_keeps_caller_order(ctx, frontier):
         assert np.array_equal(far, ofar) and np.array_equal(ds, ods)
     c = ctx.cc(dg, src, frontier=frontier)[0]
     assert np.array_equal(c, ofar)
+
+
+def test_grid2048_source_mode_properties(ctx):
+    """The high-diameter workload's graph (2048 x 2048 grid, 5% of edges removed,
+    1000 shortcuts) in the source-parallel step: the per-source sum identity
+    and distance-sum total at full size, and an 8-source subset against the
+    oracle and against the dense step."""
+    g = bc_amd.gen_grid(2048, 2048, 0.05, 1000, seed=4)
+    print(f"[gpu] grid2048 n={g.n} source mode", flush=True)
+    dg = ctx.upload(g)
+    src = bc_amd.sample_sources(g.n, 128, 4)
+    ctx.reset_stats()
+    bc, far, ds = ctx.bc(dg, src, dist_sum=True, frontier="source")
+    reached = ctx.stats()["reached"]
+    assert np.isclose(bc.sum(), far.sum() - (reached - len(src)), rtol=1e-9)
+    assert ds.sum() == far.sum()
+    sub = src[:8]
+    obc, ofar, ods, _ = O.brandes(g, sub, threads=min(16, O.max_threads()))
+    bc2, far2, ds2 = ctx.bc(dg, sub, dist_sum=True, frontier="source")
+    assert_bc_close(bc2, obc, "grid2048 subset")
+    assert np.array_equal(far2, ofar) and np.array_equal(ds2, ods)
+    bc3, far3, _ = ctx.bc(dg, sub, frontier="dense")
+    assert_bc_close(bc3, bc2, "grid2048 dense vs source")
+    assert np.array_equal(far3, far2)
+
+
+def test_grid512_source_mode_matches_oracle(ctx):
+    g = bc_amd.gen_grid(512, 512, 0.05, 50, seed=9)
+    print(f"[gpu] grid512 n={g.n} source mode", flush=True)
+    dg = ctx.upload(g)
+    src = bc_amd.sample_sources(g.n, 300, 9)
+    for mode in (bc_amd.CENT_BC_TEXTBOOK, bc_amd.CENT_BC_REF_COMPAT):
+        bc, far, ds = ctx.bc(dg, src, mode=mode, dist_sum=True, frontier="source")
+        obc, ofar, ods, _ = O.brandes(g, src, mode=mode, threads=min(16, O.max_threads()))
+        assert_bc_close(bc, obc, f"grid512 mode {mode}")
+        assert np.array_equal(far, ofar) and np.array_equal(ds, ods)
