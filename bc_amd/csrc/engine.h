@@ -1,0 +1,276 @@
+// Internal declarations shared by the engine's translation units
+// (engine.hip: contexts, graphs, probe and reference-shaped calls;
+// levels.hip: the level steps, plain calls and the autotuner; reduced.hip:
+// the D1/AP-reduced form).  Not installed; the C-ABI is include/cent.h.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/cent.h"
+#include "../Lib/reduce.h"
+#include "records.h"
+#include "wave.h"
+
+using namespace cent;
+
+// Engine functions shared between the translation units of libcent.so; not
+// part of the C-ABI, so not exported.
+#define CENT_INTERNAL __attribute__((visibility("hidden")))
+
+#define HIPTRY(expr)                                                           \
+    do {                                                                       \
+        hipError_t e_ = (expr);                                                \
+        if (e_ != hipSuccess) {                                                \
+            set_error(ctx, e_, #expr);                                         \
+            return (e_ == hipErrorOutOfMemory) ? CENT_E_OOM : CENT_E_DEVICE;   \
+        }                                                                      \
+    } while (0)
+
+#define TRY(expr)                  \
+    do {                           \
+        int st_ = (expr);          \
+        if (st_) return st_;       \
+    } while (0)
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t cap = 0;  // elements
+};
+
+struct cent_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    char last_error[256] = {0};
+    cent_stats stats{};
+    // workspace
+    DevBuf<uint64_t> vis, F, hp_or;
+    DevBuf<double> sigma, coef, bc, hp_acc;
+    DevBuf<uint32_t> npred, hp_cnt;
+    DevBuf<int64_t> dist_sum, far, part_far;
+    DevBuf<unsigned long long> part_cnt, cnt;
+    DevBuf<int> src, flags;
+    // frontier-queue mode
+    DevBuf<uint64_t> Fd2, QF, Adev;
+    DevBuf<int> Q, qoff, cand, stamp, ctl;
+    // reduced closeness / betweenness per-run buffers
+    DevBuf<int64_t> red_D;
+    DevBuf<int> red_head;
+    DevBuf<double> red_part, red_bcp, it_sig, it_coef;
+    DevBuf<uint8_t> it_lvl;
+    DevBuf<int> red_deep;
+    // source-parallel mode: per-slot arrays
+    DevBuf<PsRec> ps_rec;
+    DevBuf<int> ps_Q, ps_loff;
+    DevBuf<double> ps_bcp;
+    DevBuf<unsigned long long> ps_depth;
+    size_t F_levels = 0;     // levels the F buffer holds for the current N
+    size_t F_n = 0;
+    int* h_flags = nullptr;  // pinned
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+};
+
+// Rows longer than kHubDeg are scanned as kHubDeg-edge segments, one
+// wavefront each (see msbfs_kernels.h).
+constexpr int kHubDeg = 256;
+// Edge budget of one wavefront's vertex chunk in the main level kernels.
+constexpr int kChunkEdges = 1024;
+
+static inline void set_error(cent_ctx* ctx, hipError_t e, const char* what) {
+    if (!ctx) return;
+    snprintf(ctx->last_error, sizeof(ctx->last_error), "%s: %s", what, hipGetErrorString(e));
+}
+
+template <class T>
+static inline int ensure(cent_ctx* ctx, DevBuf<T>& b, size_t n) {
+    if (n <= b.cap) return CENT_OK;
+    if (b.p) { HIPTRY(hipFree(b.p)); b.p = nullptr; b.cap = 0; }
+    size_t cap = n + n / 4 + 64;
+    HIPTRY(hipMalloc((void**)&b.p, cap * sizeof(T)));
+    b.cap = cap;
+    return CENT_OK;
+}
+
+// Exact-size variant for the large per-slot arrays (no growth slack).
+template <class T>
+static inline int ensure_exact(cent_ctx* ctx, DevBuf<T>& b, size_t n) {
+    if (n <= b.cap) return CENT_OK;
+    if (b.p) { HIPTRY(hipFree(b.p)); b.p = nullptr; b.cap = 0; }
+    HIPTRY(hipMalloc((void**)&b.p, (n > 0 ? n : 1) * sizeof(T)));
+    b.cap = n > 0 ? n : 1;
+    return CENT_OK;
+}
+
+template <class T>
+static inline void release(DevBuf<T>& b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+static inline int check_device(cent_ctx* ctx) {
+    HIPTRY(hipSetDevice(ctx->device));
+    return CENT_OK;
+}
+
+// ------------------------------------------------------------ device CSR
+// Device copy of a CSR plus its scheduling metadata: vertex chunks (<= 64
+// vertices, <= kChunkEdges non-hub edges; optionally restarted at given
+// block boundaries) and hub segments.  Host copies of the chunk and hub
+// offsets allow sub-range views.
+struct DevCSR {
+    int n = 0;
+    int64_t nnz = 0;
+    int* rp = nullptr;
+    int* col = nullptr;
+    int hub_deg = kHubDeg;
+    int chunk_edges = kChunkEdges;
+    int nchunk = 0;
+    int* chunk_beg = nullptr;
+    int nseg = 0, nhub = 0;
+    int* hub_buf = nullptr;
+    int *seg_v = nullptr, *seg_beg = nullptr, *seg_end = nullptr, *hub_v = nullptr, *hub_seg_off = nullptr;
+    std::vector<int> h_chunk_beg, h_hub_v, h_hub_seg_off;
+};
+
+template <class T>
+static inline int upload(cent_ctx* ctx, T** dst, const T* src, size_t n) {
+    HIPTRY(hipMalloc((void**)dst, sizeof(T) * (n > 0 ? n : 1)));
+    if (n) HIPTRY(hipMemcpy(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice));
+    return CENT_OK;
+}
+
+// Chunk and hub tables, device copy (engine.hip).
+CENT_INTERNAL void devcsr_free(DevCSR& d);
+CENT_INTERNAL int devcsr_schedule(cent_ctx* ctx, const int32_t* rp, const int32_t* bounds, int32_t nbounds,
+                                  int hub_deg, int chunk_edges, DevCSR& d);
+CENT_INTERNAL int devcsr_build(cent_ctx* ctx, const int32_t* rp, const int32_t* col, int32_t n, int64_t nnz,
+                               const int32_t* bounds, int32_t nbounds, DevCSR& d);
+
+// A sub-range [lo, hi) of a DevCSR whose boundaries are chunk boundaries.
+struct GView {
+    const DevCSR* d;
+    int lo, hi;
+    int c0, nchunk;       // chunk range
+    int s0, nseg;         // hub segment range
+    int h0, nhub;         // hub range
+};
+
+static inline GView full_view(const DevCSR& d) {
+    return GView{&d, 0, d.n, 0, d.nchunk, 0, d.nseg, 0, d.nhub};
+}
+
+static inline GView sub_view(const DevCSR& d, int lo, int hi) {
+    GView v{&d, lo, hi, 0, 0, 0, 0, 0, 0};
+    const auto& cb = d.h_chunk_beg;
+    v.c0 = (int)(std::lower_bound(cb.begin(), cb.end(), lo) - cb.begin());
+    const int c1 = (int)(std::lower_bound(cb.begin(), cb.end(), hi) - cb.begin());
+    v.nchunk = c1 - v.c0;
+    const auto& hv = d.h_hub_v;
+    v.h0 = (int)(std::lower_bound(hv.begin(), hv.end(), lo) - hv.begin());
+    const int h1 = (int)(std::lower_bound(hv.begin(), hv.end(), hi) - hv.begin());
+    v.nhub = h1 - v.h0;
+    v.s0 = d.h_hub_seg_off[(size_t)v.h0];
+    v.nseg = d.h_hub_seg_off[(size_t)h1] - v.s0;
+    return v;
+}
+
+struct cent_graph {
+    cent_ctx* ctx = nullptr;
+    int32_t n = 0;
+    int64_t nnz = 0;
+    DevCSR d;
+    std::vector<int32_t> comp;        // component id per vertex (host, caller ids)
+    std::vector<int64_t> comp_nnz;    // directed edges per component
+    std::vector<int32_t> new_id;      // device id of each caller id (empty: identity)
+    int32_t ecc_probe = 0;            // BFS depth from the highest-degree vertex
+    int32_t width_probe = 0;          // widest level of that BFS
+    std::vector<int32_t> h_rp;        // device-order row offsets (host), for re-scheduling
+    std::vector<int32_t> probe_dist;  // BFS distance from the highest-degree vertex (caller ids, -1 unreached)
+    cent_tune_sig sig{};              // autotuner signature
+    // launch defaults when the call's opts leave them open: [0] betweenness,
+    // [1] closeness
+    int32_t grid_waves[2] = {0, 0};
+    int32_t frontier[2] = {CENT_FRONTIER_AUTO, CENT_FRONTIER_AUTO};
+};
+
+// Level step for a plain call.  Deep graphs (BFS depth above kDeepLevels
+// from the highest-degree vertex, probed at upload) go source-parallel: a
+// 64-lane batch there spreads its lanes over different levels and shares
+// no work.  Otherwise the dense 64-lane pull.  The frontier-queue step is
+// chosen only on request (or by the autotuner).
+constexpr int kDeepLevels = 48;
+
+static inline int level_mode(const cent_graph* g, const cent_opts* o) {
+    const int m = o ? o->frontier : CENT_FRONTIER_AUTO;
+    if (m != CENT_FRONTIER_AUTO) return m;
+    return g->ecc_probe > kDeepLevels ? CENT_FRONTIER_SOURCE : CENT_FRONTIER_DENSE;
+}
+
+// ------------------------------------------------------------ timing
+static inline hipEvent_t next_event(cent_ctx* ctx) {
+    if (ctx->ev_used == ctx->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        ctx->ev_pool.push_back(e);
+    }
+    return ctx->ev_pool[ctx->ev_used++];
+}
+
+struct KernelTimer {
+    bool on = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> fwd, bwd;
+};
+
+static inline int timer_begin(cent_ctx* ctx, KernelTimer& tm, hipEvent_t* e0) {
+    *e0 = nullptr;
+    if (!tm.on) return CENT_OK;
+    *e0 = next_event(ctx);
+    if (!*e0) return CENT_E_DEVICE;
+    HIPTRY(hipEventRecord(*e0, ctx->stream));
+    return CENT_OK;
+}
+
+static inline int timer_end(cent_ctx* ctx, KernelTimer& tm, hipEvent_t e0, bool fwd) {
+    if (!tm.on) return CENT_OK;
+    hipEvent_t e1 = next_event(ctx);
+    if (!e1) return CENT_E_DEVICE;
+    HIPTRY(hipEventRecord(e1, ctx->stream));
+    (fwd ? tm.fwd : tm.bwd).push_back({e0, e1});
+    return CENT_OK;
+}
+
+static inline void timer_collect(cent_ctx* ctx, KernelTimer& tm) {
+    float ms;
+    for (auto& p : tm.fwd) if (hipEventElapsedTime(&ms, p.first, p.second) == hipSuccess) ctx->stats.ms_fwd += ms;
+    for (auto& p : tm.bwd) if (hipEventElapsedTime(&ms, p.first, p.second) == hipSuccess) ctx->stats.ms_bwd += ms;
+    tm.fwd.clear();
+    tm.bwd.clear();
+    ctx->ev_used = 0;
+}
+
+// ------------------------------------------------------------ level loops (levels.hip)
+struct BatchSpec {
+    bool with_bc = false;
+    bool compat = false;
+    const int64_t* weight = nullptr;  // device, per vertex of the DevCSR, or null
+    int64_t* dist_sum = nullptr;      // device [n] or null
+    double* bc = nullptr;             // device [n] (with_bc)
+    const int64_t* bwgt = nullptr;    // weighted dependencies (reduced graphs), or null
+    const cent_opts* opts = nullptr;
+};
+
+// Closeness batches of 64 sources that one dense level launch carries.
+constexpr int kMultiBatch = 8;
+
+// Batches of the k sources d_src[0..k) (device ids) inside the view: one
+// 64-lane batch for BC (k <= 64), up to kMultiBatch for closeness.
+CENT_INTERNAL int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d_src, int k,
+                              int64_t* far_out, KernelTimer& tm, int64_t* levels);

@@ -1,0 +1,684 @@
+// Plain calls: the three level steps (dense 64-lane pull with hub
+// segments, frontier queues replayed from hipGraphs, source-parallel
+// workgroups), the batching of a call's sources, cent_bc / cent_cc and the
+// autotuner.  Shared types are in engine.h.
+#include "engine.h"
+#include "msbfs_kernels.h"
+#include "persource_kernels.h"
+#include "sparse_kernels.h"
+
+// ------------------------------------------------------------ level loop
+static int grid_for(int chunks, const cent_opts* o) {
+    int blocks = (chunks + kWavesPerBlock - 1) / kWavesPerBlock;
+    int cap = (o && o->grid_waves > 0) ? (o->grid_waves + kWavesPerBlock - 1) / kWavesPerBlock : 2048;
+    if (blocks > cap) blocks = cap;
+    if (blocks < 1) blocks = 1;
+    return blocks;
+}
+
+
+// Workspace for a DevCSR with N vertices; the F store grows on demand.
+static int ensure_workspace(cent_ctx* ctx, const DevCSR& d, const BatchSpec& sp, int G, int nb) {
+    const size_t N = (size_t)(d.n > 0 ? d.n : 1);
+    TRY(ensure(ctx, ctx->vis, N * (size_t)nb));
+    // BC keeps one mask array per level; closeness a level-parity pair per batch
+    const size_t want = sp.with_bc ? N * 17 : N * (2 * (size_t)nb + 1);
+    if (ctx->F_n != N || ctx->F.cap < want) {
+        TRY(ensure(ctx, ctx->F, want));
+        ctx->F_n = N;
+    }
+    ctx->F_levels = ctx->F.cap / N - 1;
+    if (sp.with_bc) {
+        TRY(ensure(ctx, ctx->sigma, N * kWave));
+        TRY(ensure(ctx, ctx->coef, N * kWave));
+        if (sp.compat) TRY(ensure(ctx, ctx->npred, N * kWave));
+    }
+    TRY(ensure(ctx, ctx->part_far, (size_t)G * kWave * nb));
+    TRY(ensure(ctx, ctx->part_cnt, (size_t)G * 2 * nb));
+    TRY(ensure(ctx, ctx->cnt, 2));
+    TRY(ensure(ctx, ctx->flags, 2));
+    if (d.nhub) {
+        TRY(ensure(ctx, ctx->hp_acc, (size_t)d.nseg * kWave));
+        TRY(ensure(ctx, ctx->hp_cnt, (size_t)d.nseg * kWave));
+        TRY(ensure(ctx, ctx->hp_or, (size_t)d.nseg * nb));
+    }
+    return CENT_OK;
+}
+
+static int grow_levels(cent_ctx* ctx, size_t N, int keep_levels) {
+    const size_t ncap = ctx->F_levels * 2;
+    uint64_t* nf = nullptr;
+    HIPTRY(hipMalloc((void**)&nf, sizeof(uint64_t) * N * (ncap + 1)));
+    HIPTRY(hipMemcpyAsync(nf, ctx->F.p, sizeof(uint64_t) * N * (size_t)keep_levels, hipMemcpyDeviceToDevice,
+                          ctx->stream));
+    HIPTRY(hipStreamSynchronize(ctx->stream));
+    HIPTRY(hipFree(ctx->F.p));
+    ctx->F.p = nf;
+    ctx->F.cap = N * (ncap + 1);
+    ctx->F_levels = ncap;
+    return CENT_OK;
+}
+
+// Totals of batch b from its G per-workgroup partials: far_out[0..kb)
+// assigned, ctx->cnt accumulated.  One reducing workgroup per 128 partial
+// rows, at most 32.
+static int reduce_batch(cent_ctx* ctx, int G, int kb, int64_t* far_out, int b = 0) {
+    hipStream_t s = ctx->stream;
+    HIPTRY(hipMemsetAsync(far_out, 0, sizeof(int64_t) * (size_t)(kb > 0 ? kb : 1), s));
+    const int rg = std::max(1, std::min(32, G / 128));
+    k_reduce_batch<<<rg, 1024, 0, s>>>(ctx->part_far.p + (size_t)b * G * kWave, ctx->part_cnt.p + (size_t)b * G * 2,
+                                       G, kb, far_out, ctx->cnt.p);
+    HIPTRY(hipGetLastError());
+    return CENT_OK;
+}
+
+
+// Batches of the k sources d_src[0..k) (device ids) whose BFS stays inside
+// the view's vertex range: one 64-lane batch for BC (k <= 64), up to
+// kMultiBatch batches of 64 consecutive sources side by side for closeness
+// (each level is one launch over all of them, so launches and host
+// round trips are shared).  far_out (device, k entries) gets the weighted
+// distance sums.  Adds the depth to *levels once per batch.
+int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d_src, int k,
+                       int64_t* far_out, KernelTimer& tm, int64_t* levels) {
+    const DevCSR& d = *v.d;
+    hipStream_t s = ctx->stream;
+    const size_t N = (size_t)(d.n > 0 ? d.n : 1);
+    const size_t R = (size_t)(v.hi - v.lo);
+    const int nb = (k + kWave - 1) / kWave;
+    if (nb < 1 || nb > kMultiBatch || (sp.with_bc && nb != 1)) return CENT_E_INVAL;
+    const int G = grid_for(v.nchunk, sp.opts);
+    TRY(ensure_workspace(ctx, d, sp, G, nb));
+    const int kl = k - (nb - 1) * kWave;  // sources of the last batch
+    const uint64_t A_last = (kl == 64) ? ~0ull : ((1ull << kl) - 1);
+    const uint64_t A = nb == 1 ? A_last : ~0ull;
+    const size_t planes = sp.with_bc ? 1 : (size_t)nb;  // closeness: batch b's level-parity plane p at (p*nb + b)*N
+    for (int b = 0; b < nb; ++b) {
+        HIPTRY(hipMemsetAsync(ctx->vis.p + (size_t)b * N + v.lo, 0, sizeof(uint64_t) * R, s));
+        HIPTRY(hipMemsetAsync(ctx->F.p + (size_t)b * N + v.lo, 0, sizeof(uint64_t) * R, s));
+        HIPTRY(hipMemsetAsync(ctx->F.p + (planes + b) * N + v.lo, 0, sizeof(uint64_t) * R, s));
+    }
+    HIPTRY(hipMemsetAsync(ctx->part_far.p, 0, sizeof(int64_t) * (size_t)G * kWave * nb, s));
+    HIPTRY(hipMemsetAsync(ctx->part_cnt.p, 0, sizeof(unsigned long long) * (size_t)G * 2 * nb, s));
+    HIPTRY(hipMemsetAsync(ctx->flags.p, 0, sizeof(int) * 2, s));
+    for (int b = 0; b < nb; ++b)
+        k_seed<<<1, kWave, 0, s>>>(d_src + b * kWave, b == nb - 1 ? kl : kWave, ctx->F.p + (size_t)b * N,
+                                   ctx->vis.p + (size_t)b * N, sp.with_bc ? ctx->sigma.p : nullptr, sp.with_bc);
+    HIPTRY(hipGetLastError());
+
+    HubArgs ha{};
+    int Gseg = 1, Ghub = 1;
+    if (v.nhub) {
+        ha.nseg = v.nseg; ha.seg_v = d.seg_v + v.s0; ha.seg_beg = d.seg_beg + v.s0; ha.seg_end = d.seg_end + v.s0;
+        ha.seg_base = v.s0;
+        ha.nhub = v.nhub; ha.hub_v = d.hub_v + v.h0; ha.hub_seg_off = d.hub_seg_off + v.h0;
+        ha.part_acc = ctx->hp_acc.p; ha.part_cnt = ctx->hp_cnt.p; ha.part_or = ctx->hp_or.p;
+        Gseg = std::min((v.nseg + kWavesPerBlock - 1) / kWavesPerBlock, 4096);
+        Ghub = std::min((v.nhub + kWavesPerBlock - 1) / kWavesPerBlock, G);  // hubfin flushes into G slots
+    }
+    const bool bc = sp.with_bc, compat = sp.compat;
+    int dl = 0;  // current frontier distance
+    for (;;) {
+        if (bc && (size_t)(dl + 2) > ctx->F_levels) TRY(grow_levels(ctx, N, dl + 1));
+        const uint64_t* Fd = ctx->F.p + N * (size_t)(bc ? dl : planes * (dl & 1));
+        uint64_t* Fn = ctx->F.p + N * (size_t)(bc ? dl + 1 : planes * ((dl + 1) & 1));
+        if (!bc || dl + 1 > 1)
+            for (int b = 0; b < nb; ++b)
+                HIPTRY(hipMemsetAsync(Fn + (size_t)b * N + v.lo, 0, sizeof(uint64_t) * R, s));
+        HIPTRY(hipMemsetAsync(ctx->flags.p, 0, sizeof(int), s));
+        FwdArgs fa{};
+        fa.bstride = N; fa.pslots = G; fa.A_last = A_last; fa.dist_atomic = nb > 1;
+        fa.n = d.n; fa.hub_deg = d.hub_deg; fa.nchunk = v.nchunk; fa.chunk_beg = d.chunk_beg + v.c0;
+        fa.rp = d.rp; fa.col = d.col; fa.vis = ctx->vis.p; fa.Fd = Fd; fa.Fn = Fn;
+        fa.sigma = bc ? ctx->sigma.p : nullptr;
+        fa.npred = compat ? ctx->npred.p : nullptr;
+        fa.dist_sum = sp.dist_sum;
+        fa.weight = sp.weight;
+        fa.A = A; fa.dnext = dl + 1;
+        fa.part_far = ctx->part_far.p; fa.part_cnt = ctx->part_cnt.p;
+        fa.flag = ctx->flags.p; fa.overflow = ctx->flags.p + 1;
+        hipEvent_t e0;
+        TRY(timer_begin(ctx, tm, &e0));
+        if (bc) {
+            if (compat) k_fwd<true, true><<<G, kBlock, 0, s>>>(fa);
+            else k_fwd<true, false><<<G, kBlock, 0, s>>>(fa);
+        } else {
+            k_fwd<false, false><<<dim3(G, nb), kBlock, 0, s>>>(fa);
+        }
+        HIPTRY(hipGetLastError());
+        if (v.nhub) {
+            if (bc) {
+                k_fwd_hubseg<true><<<Gseg, kBlock, 0, s>>>(fa, ha);
+                if (compat) k_fwd_hubfin<true, true><<<Ghub, kBlock, 0, s>>>(fa, ha);
+                else k_fwd_hubfin<true, false><<<Ghub, kBlock, 0, s>>>(fa, ha);
+            } else {
+                k_fwd_hubseg<false><<<dim3(Gseg, nb), kBlock, 0, s>>>(fa, ha);
+                k_fwd_hubfin<false, false><<<dim3(Ghub, nb), kBlock, 0, s>>>(fa, ha);
+            }
+            HIPTRY(hipGetLastError());
+        }
+        TRY(timer_end(ctx, tm, e0, true));
+        ctx->stats.launches_fwd++;
+        HIPTRY(hipMemcpyAsync(ctx->h_flags, ctx->flags.p, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+        HIPTRY(hipStreamSynchronize(s));
+        if (ctx->h_flags[1]) return CENT_E_OVERFLOW;
+        if (!ctx->h_flags[0]) break;
+        ++dl;
+    }
+    *levels += (int64_t)dl * nb;  // deepest non-empty level, per batch
+    if (bc) {
+        for (int lv = dl; lv >= 1; --lv) {
+            BwdArgs ba;
+            ba.n = d.n; ba.hub_deg = d.hub_deg; ba.nchunk = v.nchunk; ba.chunk_beg = d.chunk_beg + v.c0;
+            ba.rp = d.rp; ba.col = d.col;
+            ba.Fd = ctx->F.p + N * (size_t)lv;
+            ba.Fn = ctx->F.p + N * (size_t)(lv + 1);
+            ba.sigma = ctx->sigma.p; ba.coef = ctx->coef.p;
+            ba.npred = compat ? ctx->npred.p : nullptr;
+            ba.bc = sp.bc;
+            ba.wgt = sp.bwgt;
+            ba.bsrc = d_src;
+            hipEvent_t e0;
+            TRY(timer_begin(ctx, tm, &e0));
+            if (compat) k_bwd<true><<<G, kBlock, 0, s>>>(ba);
+            else k_bwd<false><<<G, kBlock, 0, s>>>(ba);
+            HIPTRY(hipGetLastError());
+            if (v.nhub) {
+                k_bwd_hubseg<<<Gseg, kBlock, 0, s>>>(ba, ha);
+                if (compat) k_bwd_hubfin<true><<<Ghub, kBlock, 0, s>>>(ba, ha);
+                else k_bwd_hubfin<false><<<Ghub, kBlock, 0, s>>>(ba, ha);
+                HIPTRY(hipGetLastError());
+            }
+            TRY(timer_end(ctx, tm, e0, false));
+            ctx->stats.launches_bwd++;
+        }
+    }
+    for (int b = 0; b < nb; ++b)
+        TRY(reduce_batch(ctx, G, b == nb - 1 ? kl : kWave, far_out + b * kWave, b));
+    ctx->stats.batches += nb;
+    return CENT_OK;
+}
+
+
+// ------------------------------------------------------------ frontier-queue mode
+constexpr int kSpReplay = 16;  // levels per hipGraph replay
+
+struct SpPlan {
+    SpArgs a{};
+    int G = 1;                 // blocks of the list kernels
+    hipGraph_t gf = nullptr, gb = nullptr;
+    hipGraphExec_t xf = nullptr, xb = nullptr;
+    ~SpPlan() {
+        if (xf) (void)hipGraphExecDestroy(xf);
+        if (xb) (void)hipGraphExecDestroy(xb);
+        if (gf) (void)hipGraphDestroy(gf);
+        if (gb) (void)hipGraphDestroy(gb);
+    }
+};
+
+static int sp_prepare(cent_ctx* ctx, const DevCSR& d, const BatchSpec& sp, SpPlan& pl) {
+    const size_t N = (size_t)(d.n > 0 ? d.n : 1);
+    pl.G = std::max(1, std::min((int)((N + kBlock - 1) / kBlock), 1024));
+    TRY(ensure(ctx, ctx->vis, N));
+    TRY(ensure(ctx, ctx->Fd2, 2 * N));
+    const size_t qcap = std::min<size_t>(N * kWave, (size_t)INT32_MAX);
+    TRY(ensure(ctx, ctx->Q, qcap));
+    TRY(ensure(ctx, ctx->QF, qcap));
+    TRY(ensure(ctx, ctx->qoff, N + 3));
+    TRY(ensure(ctx, ctx->cand, N));
+    TRY(ensure(ctx, ctx->stamp, N));
+    TRY(ensure(ctx, ctx->ctl, kCtlSlots));
+    TRY(ensure(ctx, ctx->Adev, 1));
+    TRY(ensure(ctx, ctx->part_far, (size_t)pl.G * kWave));
+    TRY(ensure(ctx, ctx->part_cnt, (size_t)pl.G * 2));
+    TRY(ensure(ctx, ctx->cnt, 2));
+    if (sp.with_bc) {
+        TRY(ensure(ctx, ctx->sigma, N * kWave));
+        TRY(ensure(ctx, ctx->coef, N * kWave));
+        if (sp.compat) TRY(ensure(ctx, ctx->npred, N * kWave));
+    }
+    hipStream_t s = ctx->stream;
+    // all-zero vis / masks between batches (batches clear what they touched)
+    HIPTRY(hipMemsetAsync(ctx->vis.p, 0, sizeof(uint64_t) * N, s));
+    HIPTRY(hipMemsetAsync(ctx->Fd2.p, 0, sizeof(uint64_t) * 2 * N, s));
+    // stamps restart with the call's level counter (ctl), so reset them too
+    HIPTRY(hipMemsetAsync(ctx->stamp.p, 0xff, sizeof(int) * N, s));
+    HIPTRY(hipMemsetAsync(ctx->ctl.p, 0, sizeof(int) * kCtlSlots, s));
+    SpArgs& a = pl.a;
+    a.n = d.n; a.rp = d.rp; a.col = d.col; a.vis = ctx->vis.p; a.Fd2 = ctx->Fd2.p;
+    a.sigma = sp.with_bc ? ctx->sigma.p : nullptr; a.coef = sp.with_bc ? ctx->coef.p : nullptr;
+    a.npred = sp.compat ? ctx->npred.p : nullptr; a.bc = sp.bc; a.dist_sum = sp.dist_sum;
+    a.weight = sp.weight; a.Q = ctx->Q.p; a.QF = ctx->QF.p; a.qoff = ctx->qoff.p;
+    a.qcap = (int)qcap; a.qoff_cap = (int)(N + 3); a.cand = ctx->cand.p; a.stamp = ctx->stamp.p;
+    a.ctl = ctx->ctl.p; a.Aptr = ctx->Adev.p; a.part_far = ctx->part_far.p; a.part_cnt = ctx->part_cnt.p;
+    // capture kSpReplay levels of the forward and of the backward once per call
+    const int G = pl.G;
+    HIPTRY(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+    for (int l = 0; l < kSpReplay; ++l) {
+        k_sp_expand<<<G, kBlock, 0, s>>>(a);
+        if (sp.with_bc) {
+            if (sp.compat) k_sp_pull<true, true><<<G, kBlock, 0, s>>>(a);
+            else k_sp_pull<true, false><<<G, kBlock, 0, s>>>(a);
+        } else {
+            k_sp_pull<false, false><<<G, kBlock, 0, s>>>(a);
+        }
+        k_sp_advance<<<1, 1, 0, s>>>(a);
+    }
+    HIPTRY(hipStreamEndCapture(s, &pl.gf));
+    HIPTRY(hipGraphInstantiate(&pl.xf, pl.gf, nullptr, nullptr, 0));
+    if (sp.with_bc) {
+        HIPTRY(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+        for (int l = 0; l < kSpReplay; ++l) {
+            if (sp.compat) k_sp_bwd<true><<<G, kBlock, 0, s>>>(a);
+            else k_sp_bwd<false><<<G, kBlock, 0, s>>>(a);
+            k_sp_bwd_next<<<G, kBlock, 0, s>>>(a);
+            k_sp_bwd_step<<<1, 1, 0, s>>>(a);
+        }
+        HIPTRY(hipStreamEndCapture(s, &pl.gb));
+        HIPTRY(hipGraphInstantiate(&pl.xb, pl.gb, nullptr, nullptr, 0));
+    }
+    return CENT_OK;
+}
+
+// One batch of kb <= 64 sources in frontier-queue mode (whole graph).
+static int level_batch_sparse(cent_ctx* ctx, SpPlan& pl, const BatchSpec& sp, const int* d_src, int kb,
+                              int64_t* far_out, KernelTimer& tm, int64_t* levels) {
+    hipStream_t s = ctx->stream;
+    const int G = pl.G;
+    const uint64_t A = (kb == 64) ? ~0ull : ((1ull << kb) - 1);
+    HIPTRY(hipMemsetAsync(ctx->part_far.p, 0, sizeof(int64_t) * (size_t)G * kWave, s));
+    HIPTRY(hipMemsetAsync(ctx->part_cnt.p, 0, sizeof(unsigned long long) * (size_t)G * 2, s));
+    k_sp_seed<<<1, kWave, 0, s>>>(pl.a, d_src, kb, sp.with_bc, A);
+    HIPTRY(hipGetLastError());
+    int deepest = 0;
+    for (;;) {
+        hipEvent_t e0;
+        TRY(timer_begin(ctx, tm, &e0));
+        HIPTRY(hipGraphLaunch(pl.xf, s));
+        TRY(timer_end(ctx, tm, e0, true));
+        ctx->stats.launches_fwd += kSpReplay;
+        HIPTRY(hipMemcpyAsync(ctx->h_flags, ctx->ctl.p + kCtlDone, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+        HIPTRY(hipStreamSynchronize(s));
+        if (ctx->h_flags[0]) break;
+    }
+    int ctl_h[kCtlSlots];
+    HIPTRY(hipMemcpyAsync(ctl_h, ctx->ctl.p, sizeof(ctl_h), hipMemcpyDeviceToHost, s));
+    HIPTRY(hipStreamSynchronize(s));
+    if (ctl_h[kCtlOverflow]) return CENT_E_OVERFLOW;
+    deepest = ctl_h[kCtlDeepest];
+    *levels += deepest;
+    if (sp.with_bc && deepest >= 1) {
+        k_sp_bwd_init<<<1, 1, 0, s>>>(pl.a);
+        const int replays = (deepest + kSpReplay - 1) / kSpReplay;
+        for (int r = 0; r < replays; ++r) {
+            hipEvent_t e0;
+            TRY(timer_begin(ctx, tm, &e0));
+            HIPTRY(hipGraphLaunch(pl.xb, s));
+            TRY(timer_end(ctx, tm, e0, false));
+            ctx->stats.launches_bwd += kSpReplay;
+        }
+    }
+    k_sp_clear<<<G, kBlock, 0, s>>>(pl.a);
+    TRY(reduce_batch(ctx, G, kb, far_out));
+    ctx->stats.batches++;
+    return CENT_OK;
+}
+
+// ------------------------------------------------------------ source-parallel mode
+// All k sources of a plain call in one launch of k_ps: G workgroup slots,
+// each with its own per-vertex arrays, take sources s, s + G, ... in turn.
+// G is bounded by k, by the slot cap (opts->max_slots, else grid_waves / 8,
+// else 256: more concurrent traversals than that only contend for the
+// memory system) and by a memory budget (40% of free device memory, at most
+// 32 GiB).  Returns CENT_E_UNSUPPORTED, with nothing accumulated into the BC
+// output, if some source's BFS is deeper than the level table: the caller
+// then runs the batch form.
+constexpr int kPsDefaultSlots = 256;
+
+static int run_persource(cent_ctx* ctx, const cent_graph* g, const BatchSpec& sp, int32_t k, KernelTimer& tm,
+                         int64_t* levels) {
+    const DevCSR& d = g->d;
+    hipStream_t s = ctx->stream;
+    const size_t n = (size_t)(d.n > 0 ? d.n : 1);
+    const bool bc = sp.with_bc, compat = sp.compat;
+    if (sp.bwgt || k < 1) return CENT_E_UNSUPPORTED;
+    const int lcap = (int)std::min<size_t>(n + 2, 65536);
+    const size_t per_slot = n * (sizeof(PsRec) + 4 + (bc ? 8 : 0)) + (size_t)lcap * 4;
+    size_t cap = kPsDefaultSlots;
+    if (sp.opts && sp.opts->max_slots > 0) cap = (size_t)sp.opts->max_slots;
+    else if (sp.opts && sp.opts->grid_waves > 0) cap = std::max(1, sp.opts->grid_waves / kPsWaves);
+    size_t G = std::min((size_t)k, cap);
+    size_t freeb = 0, totalb = 0;
+    HIPTRY(hipMemGetInfo(&freeb, &totalb));
+    const size_t have = sizeof(PsRec) * ctx->ps_rec.cap + 4 * (ctx->ps_Q.cap + ctx->ps_loff.cap) +
+                        8 * ctx->ps_bcp.cap;
+    const size_t budget = std::min<size_t>((size_t)32 << 30, (freeb + have) / 5 * 2);
+    G = std::max<size_t>(1, std::min(G, budget / per_slot));
+    TRY(ensure_exact(ctx, ctx->ps_rec, n * G));
+    TRY(ensure_exact(ctx, ctx->ps_Q, n * G));
+    TRY(ensure_exact(ctx, ctx->ps_loff, (size_t)lcap * G));
+    if (bc) TRY(ensure_exact(ctx, ctx->ps_bcp, n * G));
+    TRY(ensure(ctx, ctx->ps_depth, 1));
+    TRY(ensure(ctx, ctx->flags, 2));
+    const size_t total = n * G;
+    const int ig = (int)std::min<size_t>((total + 255) / 256, 16384);
+    k_ps_init<<<ig, 256, 0, s>>>(ctx->ps_rec.p, bc ? ctx->ps_bcp.p : nullptr, total);
+    HIPTRY(hipGetLastError());
+    HIPTRY(hipMemsetAsync(ctx->ps_depth.p, 0, sizeof(unsigned long long), s));
+    HIPTRY(hipMemsetAsync(ctx->flags.p, 0, sizeof(int) * 2, s));
+    PsArgs a{};
+    a.n = d.n; a.rp = d.rp; a.col = d.col; a.src = ctx->src.p; a.k = k;
+    a.rec = ctx->ps_rec.p; a.Q = ctx->ps_Q.p;
+    a.loff = ctx->ps_loff.p; a.lcap = lcap;
+    a.bcp = bc ? ctx->ps_bcp.p : nullptr;
+    a.far = ctx->far.p; a.dist_sum = sp.dist_sum; a.weight = sp.weight;
+    a.cnt = ctx->cnt.p; a.depth_sum = ctx->ps_depth.p; a.overflow = ctx->flags.p;
+    hipEvent_t e0;
+    TRY(timer_begin(ctx, tm, &e0));
+    if (!bc) k_ps<false, false><<<(int)G, kPsBlock, 0, s>>>(a);
+    else if (compat) k_ps<true, true><<<(int)G, kPsBlock, 0, s>>>(a);
+    else k_ps<true, false><<<(int)G, kPsBlock, 0, s>>>(a);
+    HIPTRY(hipGetLastError());
+    TRY(timer_end(ctx, tm, e0, true));
+    ctx->stats.launches_fwd++;
+    unsigned long long depth = 0;
+    HIPTRY(hipMemcpyAsync(ctx->h_flags, ctx->flags.p, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+    HIPTRY(hipMemcpyAsync(&depth, ctx->ps_depth.p, sizeof(depth), hipMemcpyDeviceToHost, s));
+    HIPTRY(hipStreamSynchronize(s));
+    if (ctx->h_flags[0] & 2) return CENT_E_UNSUPPORTED;
+    if (ctx->h_flags[0] & 1) return CENT_E_OVERFLOW;
+    if (bc) {
+        k_ps_reduce<<<(int)((n + 255) / 256), 256, 0, s>>>(ctx->ps_bcp.p, (int)G, d.n, sp.bc);
+        HIPTRY(hipGetLastError());
+    }
+    *levels += (int64_t)depth;
+    ctx->stats.batches += k;
+    return CENT_OK;
+}
+
+// Runs all batches of one plain call.  with_bc: Brandes (sigma + backward);
+// otherwise closeness only (bitmask BFS).
+static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k, bool with_bc,
+                       int bc_mode, double* bc, int64_t* farness, int64_t* dist_sum, const cent_opts* opts) {
+    const int n = g->n;
+    const size_t N = (size_t)(n > 0 ? n : 1);
+    hipStream_t s = ctx->stream;
+    // the graph's tuned launch settings fill what the call leaves open
+    cent_opts eff{};
+    if (opts) eff = *opts;
+    if (eff.grid_waves == 0) eff.grid_waves = g->grid_waves[with_bc ? 0 : 1];
+    if (eff.frontier == CENT_FRONTIER_AUTO) eff.frontier = g->frontier[with_bc ? 0 : 1];
+    opts = &eff;
+    BatchSpec sp;
+    sp.with_bc = with_bc;
+    sp.compat = with_bc && bc_mode == CENT_BC_REF_COMPAT;
+    sp.opts = opts;
+    if (with_bc) {
+        TRY(ensure(ctx, ctx->bc, N));
+        sp.bc = ctx->bc.p;
+        HIPTRY(hipMemsetAsync(ctx->bc.p, 0, sizeof(double) * N, s));
+    }
+    if (dist_sum) {
+        TRY(ensure(ctx, ctx->dist_sum, N));
+        sp.dist_sum = ctx->dist_sum.p;
+        HIPTRY(hipMemsetAsync(ctx->dist_sum.p, 0, sizeof(int64_t) * N, s));
+    }
+    TRY(ensure(ctx, ctx->far, (size_t)(k > 0 ? k : 1)));
+    TRY(ensure(ctx, ctx->src, (size_t)(k > 0 ? k : 1)));
+    TRY(ensure(ctx, ctx->cnt, 2));
+    HIPTRY(hipMemsetAsync(ctx->cnt.p, 0, sizeof(unsigned long long) * 2, s));
+    // Batch order of the sources: the 64-lane steps group sources by BFS
+    // distance to the highest-degree vertex, then by degree, so the lanes
+    // of a batch tend to reach a vertex at the same levels (fewer
+    // (vertex, level) rows touched).  order[i] = caller index of slot i.
+    const int lm = k > 0 ? level_mode(g, opts) : CENT_FRONTIER_DENSE;
+    std::vector<int32_t> order;
+    if (k > kWave && lm != CENT_FRONTIER_SOURCE && !(opts->flags & CENT_OPT_SOURCE_ORDER) &&
+        !g->probe_dist.empty()) {
+        order.resize((size_t)k);
+        for (int32_t i = 0; i < k; ++i) order[(size_t)i] = i;
+        auto deg = [&](int32_t v) {
+            const int32_t x = g->new_id.empty() ? v : g->new_id[(size_t)v];
+            return g->h_rp[(size_t)x + 1] - g->h_rp[(size_t)x];
+        };
+        std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+            const int32_t sa = src[a], sb = src[b];
+            const int32_t da = g->probe_dist[(size_t)sa], db = g->probe_dist[(size_t)sb];
+            if (da != db) return da < db;
+            const int32_t ga = deg(sa), gb = deg(sb);
+            if (ga != gb) return ga > gb;
+            return sa < sb;
+        });
+    }
+    std::vector<int32_t> dsrc;
+    const int32_t* up_src = src;
+    if ((!g->new_id.empty() || !order.empty()) && k > 0) {
+        dsrc.resize((size_t)k);
+        for (int32_t i = 0; i < k; ++i) {
+            const int32_t s0 = order.empty() ? src[i] : src[order[(size_t)i]];
+            dsrc[(size_t)i] = g->new_id.empty() ? s0 : g->new_id[(size_t)s0];
+        }
+        up_src = dsrc.data();
+    }
+    if (k > 0) HIPTRY(hipMemcpyAsync(ctx->src.p, up_src, sizeof(int) * (size_t)k, hipMemcpyHostToDevice, s));
+    HIPTRY(hipStreamSynchronize(s));  // dsrc is a pageable host buffer
+
+    KernelTimer tm;
+    tm.on = opts && (opts->flags & CENT_OPT_TIMING);
+    ctx->ev_used = 0;
+    const GView view = full_view(g->d);
+    int64_t levels = 0;
+    const int lmode = lm;
+    bool batched = true;
+    if (lmode == CENT_FRONTIER_SOURCE) {
+        const int st = run_persource(ctx, g, sp, k, tm, &levels);
+        if (st == CENT_OK) {
+            batched = false;
+        } else if (st == CENT_E_UNSUPPORTED) {  // too deep for the level table: redo as batches
+            HIPTRY(hipMemsetAsync(ctx->cnt.p, 0, sizeof(unsigned long long) * 2, s));
+            if (sp.dist_sum) HIPTRY(hipMemsetAsync(sp.dist_sum, 0, sizeof(int64_t) * N, s));
+        } else {
+            return st;
+        }
+    }
+    const bool sparse = batched && lmode == CENT_FRONTIER_SPARSE;
+    SpPlan plan;
+    if (sparse) TRY(sp_prepare(ctx, g->d, sp, plan));
+    // closeness on the dense step takes up to kMultiBatch batches per launch
+    const int32_t step = (!with_bc && !sparse) ? kWave * kMultiBatch : kWave;
+    for (int32_t b0 = 0; batched && b0 < k; b0 += step) {
+        const int kb = (k - b0) < step ? (k - b0) : step;
+        if (sparse) TRY(level_batch_sparse(ctx, plan, sp, ctx->src.p + b0, kb, ctx->far.p + b0, tm, &levels));
+        else TRY(level_batch(ctx, view, sp, ctx->src.p + b0, kb, ctx->far.p + b0, tm, &levels));
+        if (tm.on && ctx->ev_used > 4096) {  // keep the event pool bounded
+            HIPTRY(hipStreamSynchronize(s));
+            timer_collect(ctx, tm);
+        }
+    }
+    // results back to the host
+    std::vector<double> hb;
+    std::vector<int64_t> hd;
+    unsigned long long hc[2] = {0, 0};
+    if (with_bc && bc) {
+        hb.resize(N);
+        HIPTRY(hipMemcpyAsync(hb.data(), ctx->bc.p, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+    }
+    if (dist_sum) {
+        hd.resize(N);
+        HIPTRY(hipMemcpyAsync(hd.data(), ctx->dist_sum.p, sizeof(int64_t) * N, hipMemcpyDeviceToHost, s));
+    }
+    std::vector<int64_t> hf;
+    if (farness && k > 0) {
+        if (order.empty()) {
+            HIPTRY(hipMemcpyAsync(farness, ctx->far.p, sizeof(int64_t) * (size_t)k, hipMemcpyDeviceToHost, s));
+        } else {
+            hf.resize((size_t)k);
+            HIPTRY(hipMemcpyAsync(hf.data(), ctx->far.p, sizeof(int64_t) * (size_t)k, hipMemcpyDeviceToHost, s));
+        }
+    }
+    HIPTRY(hipMemcpyAsync(hc, ctx->cnt.p, sizeof(hc), hipMemcpyDeviceToHost, s));
+    HIPTRY(hipStreamSynchronize(s));
+    if (!hf.empty())
+        for (int32_t i = 0; i < k; ++i) farness[order[(size_t)i]] = hf[(size_t)i];
+    const bool remap = !g->new_id.empty();
+    if (with_bc && bc)
+        for (int v = 0; v < n; ++v) bc[v] += hb[remap ? (size_t)g->new_id[(size_t)v] : (size_t)v];
+    if (dist_sum)
+        for (int v = 0; v < n; ++v) dist_sum[v] += hd[remap ? (size_t)g->new_id[(size_t)v] : (size_t)v];
+    timer_collect(ctx, tm);
+    ctx->stats.levels += levels;
+    ctx->stats.reached += (int64_t)hc[0] + k;
+    ctx->stats.dag_edges += (int64_t)hc[1];
+    for (int32_t i = 0; i < k; ++i) ctx->stats.edges_reached += g->comp_nnz[(size_t)g->comp[(size_t)src[i]]];
+    return CENT_OK;
+}
+
+static int prepare_sources(const cent_graph* g, const int32_t* src, int32_t k, std::vector<int32_t>& all,
+                           const int32_t** out_src, int32_t* out_k) {
+    if (k == -1) {
+        all.resize((size_t)g->n);
+        for (int32_t v = 0; v < g->n; ++v) all[(size_t)v] = v;
+        *out_src = all.data();
+        *out_k = g->n;
+        return CENT_OK;
+    }
+    if (k < 0 || (k > 0 && !src)) return CENT_E_INVAL;
+    for (int32_t i = 0; i < k; ++i)
+        if (src[i] < 0 || src[i] >= g->n) return CENT_E_INVAL;
+    *out_src = src;
+    *out_k = k;
+    return CENT_OK;
+}
+
+extern "C" int cent_bc(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k, double* bc,
+                       int64_t* farness, int64_t* dist_sum, const cent_opts* opts) {
+    if (!ctx || !g || g->ctx->device != ctx->device) return CENT_E_INVAL;
+    int mode = opts ? opts->bc_mode : CENT_BC_TEXTBOOK;
+    if (mode != CENT_BC_TEXTBOOK && mode != CENT_BC_REF_COMPAT) return CENT_E_INVAL;
+    std::vector<int32_t> all;
+    const int32_t* s;
+    int32_t kk;
+    TRY(prepare_sources(g, src, k, all, &s, &kk));
+    TRY(check_device(ctx));
+    auto t0 = std::chrono::steady_clock::now();
+    int st = run_batches(ctx, g, s, kk, true, mode, bc, farness, dist_sum, opts);
+    ctx->stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return st;
+}
+
+extern "C" int cent_cc(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k, int64_t* farness,
+                       int64_t* dist_sum, const cent_opts* opts) {
+    if (!ctx || !g || g->ctx->device != ctx->device) return CENT_E_INVAL;
+    std::vector<int32_t> all;
+    const int32_t* s;
+    int32_t kk;
+    TRY(prepare_sources(g, src, k, all, &s, &kk));
+    TRY(check_device(ctx));
+    auto t0 = std::chrono::steady_clock::now();
+    int st = run_batches(ctx, g, s, kk, false, 0, nullptr, farness, dist_sum, opts);
+    ctx->stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return st;
+}
+
+// ------------------------------------------------------------ autotuner
+// Coordinate descent over the plain form's launch settings: level step
+// first; the source-parallel step then only sweeps its slot count
+// (grid_waves / 8) and the frontier-queue step nothing; the dense step gets
+// up to two passes over hub_deg, chunk_edges and grid_waves, each swept with
+// the others held at their best so far.  A trial is the faster of two
+// runs of the sample (the first also warms the workspace).
+namespace {
+struct Tuner {
+    cent_ctx* ctx;
+    cent_graph* g;
+    const int32_t* src;
+    int32_t k;
+    int32_t what;
+    int32_t budget;  // remaining timed runs (< 0: unlimited)
+    std::vector<double> bc;
+    std::vector<int64_t> far;
+
+    int run_once(double* ms) {
+        std::fill(bc.begin(), bc.end(), 0.0);
+        const auto t0 = std::chrono::steady_clock::now();
+        if (what == CENT_TUNE_BC) TRY(cent_bc(ctx, g, src, k, bc.data(), far.data(), nullptr, nullptr));
+        else TRY(cent_cc(ctx, g, src, k, far.data(), nullptr, nullptr));
+        *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return CENT_OK;
+    }
+
+    // Time setting p; *ms = +inf when the budget is spent.
+    int trial(const cent_tune_params& p, double* ms) {
+        *ms = 1e300;
+        if (budget == 0) return CENT_OK;
+        TRY(cent_graph_set_layout(g, what, &p));
+        double a = 0.0, b = 0.0;
+        TRY(run_once(&a));
+        TRY(run_once(&b));
+        if (budget > 0) --budget;
+        *ms = a < b ? a : b;
+        return CENT_OK;
+    }
+};
+}  // namespace
+
+extern "C" int cent_tune(cent_ctx* ctx, cent_graph* g, const int32_t* src, int32_t k, int32_t what,
+                         int32_t max_trials, cent_tune_params* best) {
+    if (!ctx || !g || !best || g->ctx->device != ctx->device) return CENT_E_INVAL;
+    if (what != CENT_TUNE_BC && what != CENT_TUNE_CC) return CENT_E_INVAL;
+    if (k < 1 || k > 256 || !src || max_trials < 0) return CENT_E_INVAL;
+    for (int32_t i = 0; i < k; ++i)
+        if (src[i] < 0 || src[i] >= g->n) return CENT_E_INVAL;
+    const cent_stats saved = ctx->stats;
+    Tuner t{ctx, g, src, k, what, max_trials > 0 ? max_trials : -1, {}, {}};
+    t.bc.assign((size_t)(g->n > 0 ? g->n : 1), 0.0);
+    t.far.assign((size_t)k, 0);
+    cent_tune_params orig, cur;
+    TRY(cent_graph_get_layout(g, what, &orig));
+    cur = orig;
+    cur.frontier = CENT_FRONTIER_DENSE;
+    double best_ms = 1e300, ms = 0.0;
+    int st = CENT_OK;
+    auto sweep = [&](int32_t cent_tune_params::*field, std::initializer_list<int32_t> vals) {
+        cent_tune_params keep = cur;
+        for (int32_t v : vals) {
+            if (st) return;
+            cent_tune_params p = cur;
+            p.*field = v;
+            if (p.*field == keep.*field && best_ms < 1e300) continue;  // already timed
+            st = t.trial(p, &ms);
+            if (!st && ms < best_ms) {
+                best_ms = ms;
+                keep = p;
+            }
+        }
+        cur = keep;
+    };
+    sweep(&cent_tune_params::frontier, {CENT_FRONTIER_DENSE, CENT_FRONTIER_SPARSE, CENT_FRONTIER_SOURCE});
+    if (!st && cur.frontier == CENT_FRONTIER_SOURCE) {
+        // grid_waves / 8 = concurrent source slots; caps above the sample size
+        // time the same as the sample size, so only smaller ones are tried
+        for (int32_t gw : {kPsWaves * 64, kPsWaves * 128})
+            if (gw / kPsWaves <= k) sweep(&cent_tune_params::grid_waves, {gw});
+    }
+    for (int pass = 0; pass < 2 && !st && cur.frontier == CENT_FRONTIER_DENSE; ++pass) {
+        const cent_tune_params before = cur;
+        sweep(&cent_tune_params::hub_deg, {128, 256, 512, 1024, 2048, 4096});
+        sweep(&cent_tune_params::chunk_edges, {256, 512, 1024, 2048, 4096});
+        sweep(&cent_tune_params::grid_waves, {4096, 8192, 16384, 32768});
+        if (before.hub_deg == cur.hub_deg && before.chunk_edges == cur.chunk_edges &&
+            before.grid_waves == cur.grid_waves)
+            break;  // a pass that moves nothing is a fixed point
+    }
+    ctx->stats = saved;
+    if (!st && best_ms >= 1e300) st = CENT_E_INVAL;  // budget too small to time anything
+    if (st) {
+        (void)cent_graph_set_layout(g, what, &orig);  // leave the graph as it came
+        return st;
+    }
+    cur.ms = best_ms;
+    TRY(cent_graph_set_layout(g, what, &cur));
+    TRY(cent_tune_table_put(&g->sig, what, &cur));
+    *best = cur;
+    return CENT_OK;
+}

@@ -34,6 +34,7 @@
 // wavefront (lane-strided edges, butterfly sum), so one hub row does not
 // serialise a level.
 #pragma once
+#include "records.h"
 #include "wave.h"
 
 namespace cent {
@@ -44,12 +45,6 @@ constexpr int kPsBigDeg = 64;   // longer rows are scanned by a wavefront
 constexpr int kPsBig = 1024;    // LDS capacity of the long-row list per level
 constexpr int kPsV = 4;         // rows a thread works on at once
 constexpr int kPsE = 4;         // edges per row issued together
-
-struct __align__(16) PsRec {
-    int dist;
-    uint32_t npred;
-    double sig;
-};
 
 struct PsArgs {
     int n;

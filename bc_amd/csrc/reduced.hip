@@ -1,0 +1,268 @@
+// The D1/AP-reduced form: device copies of packed block graphs, the
+// persistent work-queue kernels over (block, source block) items and the
+// level-synchronous path for large items.  Shared types are in engine.h.
+#include "engine.h"
+#include "items_kernels.h"
+
+// ------------------------------------------------------------ reduced closeness
+// Device copy of a packed block graph, cached on the cent_reduced object
+// (freed through its dev_free hook).
+struct RedDev {
+    cent_ctx* ctx = nullptr;
+    DevCSR d;
+    int64_t* w = nullptr;     // [np]
+    int* iota = nullptr;      // [np] packed ids 0..np-1 (batch sources)
+    int4* items = nullptr;    // [nitems]
+    int* order = nullptr;     // [nsmall] small items, largest range first
+    int nsmall = 0;
+    std::vector<int> big;     // items handled by the level-synchronous kernels
+    int64_t* poff = nullptr;  // [nitems] offsets of the items' BC partials (small items)
+    int64_t part_total = 0;
+    std::vector<int64_t> h_poff;
+};
+
+static void reddev_free(void* p) {
+    RedDev* rd = (RedDev*)p;
+    if (!rd) return;
+    if (rd->ctx) (void)hipSetDevice(rd->ctx->device);
+    devcsr_free(rd->d);
+    if (rd->w) (void)hipFree(rd->w);
+    if (rd->iota) (void)hipFree(rd->iota);
+    if (rd->items) (void)hipFree(rd->items);
+    if (rd->order) (void)hipFree(rd->order);
+    if (rd->poff) (void)hipFree(rd->poff);
+
+    delete rd;
+}
+
+static std::mutex g_reddev_mu;  // first-use creation of a reduced graph's device copy
+
+static int reddev_get(cent_ctx* ctx, cent_reduced* r, RedDev** out) {
+    std::lock_guard<std::mutex> lk(g_reddev_mu);
+    if (r->dev) {
+        RedDev* rd = (RedDev*)r->dev;
+        if (rd->ctx->device != ctx->device) return CENT_E_INVAL;
+        *out = rd;
+        return CENT_OK;
+    }
+    RedDev* rd = new (std::nothrow) RedDev();
+    if (!rd) return CENT_E_OOM;
+    rd->ctx = ctx;
+    r->dev = rd;
+    r->dev_free = reddev_free;
+    const size_t NP = (size_t)(r->np > 0 ? r->np : 1);
+    TRY(devcsr_build(ctx, r->prp, r->pcol, r->np, r->nnzp, r->boff, r->nblocks, rd->d));
+    TRY(upload(ctx, &rd->w, r->pw, (size_t)r->np));
+    std::vector<int> iota(NP);
+    for (size_t i = 0; i < NP; ++i) iota[i] = (int)i;
+    TRY(upload(ctx, &rd->iota, iota.data(), NP));
+    std::vector<int4> items((size_t)(r->nitems > 0 ? r->nitems : 1));
+    std::vector<int> small;
+    for (int i = 0; i < r->nitems; ++i) {
+        const cent_item& it = r->items[i];
+        items[(size_t)i] = make_int4(it.lo, it.hi, it.s0, it.ns);
+        if (it.hi - it.lo <= kItemLds) small.push_back(i);
+        else rd->big.push_back(i);
+    }
+    std::stable_sort(small.begin(), small.end(), [&](int a, int b) {
+        return (r->items[a].hi - r->items[a].lo) > (r->items[b].hi - r->items[b].lo);
+    });
+    rd->nsmall = (int)small.size();
+    TRY(upload(ctx, &rd->items, items.data(), (size_t)r->nitems));
+    TRY(upload(ctx, &rd->order, small.data(), small.size()));
+    rd->h_poff.assign((size_t)(r->nitems > 0 ? r->nitems : 1), -1);
+    for (int i : small) {
+        rd->h_poff[(size_t)i] = rd->part_total;
+        rd->part_total += r->items[i].hi - r->items[i].lo;
+    }
+    TRY(upload(ctx, &rd->poff, rd->h_poff.data(), rd->h_poff.size()));
+    *out = rd;
+    return CENT_OK;
+}
+
+extern "C" int cent_cc_reduced(cent_ctx* ctx, const cent_reduced* rc, int64_t* farness, int32_t sched,
+                               const cent_opts* opts) {
+    if (!ctx || !rc || !farness) return CENT_E_INVAL;
+    if (sched != CENT_SCHED_PERSISTENT && sched != CENT_SCHED_PER_ITEM) return CENT_E_INVAL;
+    cent_reduced* r = const_cast<cent_reduced*>(rc);  // the device cache lives on the object
+    TRY(check_device(ctx));
+    auto t0 = std::chrono::steady_clock::now();
+    RedDev* rd;
+    TRY(reddev_get(ctx, r, &rd));
+    hipStream_t s = ctx->stream;
+    const size_t NP = (size_t)(r->np > 0 ? r->np : 1);
+    // per-run buffers live in the context, so several contexts can share rd
+    TRY(ensure(ctx, ctx->red_D, NP));
+    TRY(ensure(ctx, ctx->red_head, 1));
+    int64_t* Dd = ctx->red_D.p;
+    HIPTRY(hipMemsetAsync(Dd, 0, sizeof(int64_t) * NP, s));
+    TRY(ensure(ctx, ctx->cnt, 2));
+    HIPTRY(hipMemsetAsync(ctx->cnt.p, 0, sizeof(unsigned long long) * 2, s));
+    KernelTimer tm;
+    tm.on = opts && (opts->flags & CENT_OPT_TIMING);
+    ctx->ev_used = 0;
+    BatchSpec sp;
+    sp.weight = rd->w;
+    sp.opts = opts;
+    int64_t levels = 0;
+    auto run_item = [&](int i) -> int {
+        const cent_item& it = r->items[i];
+        const GView v = sub_view(rd->d, it.lo, it.hi);
+        return level_batch(ctx, v, sp, rd->iota + it.s0, it.ns, Dd + it.s0, tm, &levels);
+    };
+    if (sched == CENT_SCHED_PER_ITEM) {
+        for (int i = 0; i < r->nitems; ++i) TRY(run_item(i));
+    } else {
+        if (rd->nsmall) {
+            HIPTRY(hipMemsetAsync(ctx->red_head.p, 0, sizeof(int), s));
+            ItemArgs ia;
+            ia.rp = rd->d.rp; ia.col = rd->d.col; ia.weight = rd->w; ia.items = rd->items; ia.order = rd->order;
+            ia.nitems = rd->nsmall; ia.head = ctx->red_head.p; ia.D = Dd;
+            const int grid = std::min(rd->nsmall, 2048);
+            hipEvent_t e0;
+            TRY(timer_begin(ctx, tm, &e0));
+            k_cc_items<<<grid, kItemBlock, 0, s>>>(ia);
+            HIPTRY(hipGetLastError());
+            TRY(timer_end(ctx, tm, e0, true));
+            ctx->stats.launches_fwd++;
+            ctx->stats.batches += rd->nsmall;
+        }
+        // large items: consecutive full items of one block (contiguous
+        // sources) go up to kMultiBatch at a time through one level loop
+        for (size_t j = 0; j < rd->big.size();) {
+            const cent_item& it = r->items[rd->big[j]];
+            int ns = it.ns;
+            size_t e = j + 1;
+            while (e < rd->big.size() && e - j < (size_t)kMultiBatch) {
+                const cent_item& pv = r->items[rd->big[e - 1]];
+                const cent_item& nx = r->items[rd->big[e]];
+                if (pv.ns != kWave || nx.lo != it.lo || nx.hi != it.hi || nx.s0 != pv.s0 + pv.ns) break;
+                ns += nx.ns;
+                ++e;
+            }
+            const GView v = sub_view(rd->d, it.lo, it.hi);
+            TRY(level_batch(ctx, v, sp, rd->iota + it.s0, ns, Dd + it.s0, tm, &levels));
+            j = e;
+        }
+    }
+    std::vector<int64_t> D(NP);
+    HIPTRY(hipMemcpyAsync(D.data(), Dd, sizeof(int64_t) * NP, hipMemcpyDeviceToHost, s));
+    HIPTRY(hipStreamSynchronize(s));
+    timer_collect(ctx, tm);
+    TRY(cent_reduced_finish(r, D.data(), farness));
+    ctx->stats.levels += levels;
+    ctx->stats.reached += (int64_t)r->n;  // every vertex is a source
+    ctx->stats.edges_reached += 2 * r->m;
+    ctx->stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return CENT_OK;
+}
+
+
+// Betweenness (+ closeness) through the reduction: weighted Brandes from every
+// packed vertex inside its block, then the host adds the piece-size terms.
+constexpr int kBcItemGrid = 512;
+
+extern "C" int cent_bc_reduced(cent_ctx* ctx, const cent_reduced* rc, double* bc, int64_t* farness,
+                               int32_t sched, const cent_opts* opts) {
+    if (!ctx || !rc || !bc) return CENT_E_INVAL;
+    if (sched != CENT_SCHED_PERSISTENT && sched != CENT_SCHED_PER_ITEM) return CENT_E_INVAL;
+    cent_reduced* r = const_cast<cent_reduced*>(rc);
+    TRY(check_device(ctx));
+    auto t0 = std::chrono::steady_clock::now();
+    RedDev* rd;
+    TRY(reddev_get(ctx, r, &rd));
+    hipStream_t s = ctx->stream;
+    const size_t NP = (size_t)(r->np > 0 ? r->np : 1);
+    TRY(ensure(ctx, ctx->red_D, NP));
+    TRY(ensure(ctx, ctx->red_head, 2));
+    TRY(ensure(ctx, ctx->red_bcp, NP));
+    TRY(ensure(ctx, ctx->red_deep, (size_t)(r->nitems > 0 ? r->nitems : 1)));
+    TRY(ensure(ctx, ctx->cnt, 2));
+    int64_t* Dd = ctx->red_D.p;
+    HIPTRY(hipMemsetAsync(Dd, 0, sizeof(int64_t) * NP, s));
+    HIPTRY(hipMemsetAsync(ctx->red_bcp.p, 0, sizeof(double) * NP, s));
+    HIPTRY(hipMemsetAsync(ctx->cnt.p, 0, sizeof(unsigned long long) * 2, s));
+    KernelTimer tm;
+    tm.on = opts && (opts->flags & CENT_OPT_TIMING);
+    ctx->ev_used = 0;
+    BatchSpec sp;
+    sp.with_bc = true;
+    sp.weight = rd->w;
+    sp.bwgt = rd->w;
+    sp.bc = ctx->red_bcp.p;
+    sp.opts = opts;
+    int64_t levels = 0;
+    auto run_item = [&](int i) -> int {
+        const cent_item& it = r->items[i];
+        const GView v = sub_view(rd->d, it.lo, it.hi);
+        return level_batch(ctx, v, sp, rd->iota + it.s0, it.ns, Dd + it.s0, tm, &levels);
+    };
+    std::vector<int> deep;
+    const bool persistent = (sched == CENT_SCHED_PERSISTENT) && rd->nsmall > 0;
+    if (persistent) {
+        const int grid = std::min(rd->nsmall, kBcItemGrid);
+        const size_t slot = (size_t)kItemLds * kWave;
+        TRY(ensure(ctx, ctx->it_sig, slot * grid));
+        TRY(ensure(ctx, ctx->it_coef, slot * grid));
+        TRY(ensure(ctx, ctx->it_lvl, slot * grid));
+        TRY(ensure(ctx, ctx->red_part, (size_t)(rd->part_total > 0 ? rd->part_total : 1)));
+        TRY(ensure(ctx, ctx->flags, 2));
+        HIPTRY(hipMemsetAsync(ctx->red_head.p, 0, sizeof(int) * 2, s));
+        HIPTRY(hipMemsetAsync(ctx->flags.p, 0, sizeof(int) * 2, s));
+        BcItemArgs ba;
+        ba.rp = rd->d.rp; ba.col = rd->d.col; ba.weight = rd->w; ba.items = rd->items; ba.order = rd->order;
+        ba.poff = rd->poff; ba.nitems = rd->nsmall; ba.head = ctx->red_head.p; ba.D = Dd;
+        ba.bcpart = ctx->red_part.p; ba.sig = ctx->it_sig.p; ba.coef = ctx->it_coef.p; ba.lvl = ctx->it_lvl.p;
+        ba.deep = ctx->red_deep.p; ba.ndeep = ctx->red_head.p + 1; ba.overflow = ctx->flags.p + 1;
+        hipEvent_t e0;
+        TRY(timer_begin(ctx, tm, &e0));
+        k_bc_items<<<grid, kItemBlock, 0, s>>>(ba);
+        HIPTRY(hipGetLastError());
+        TRY(timer_end(ctx, tm, e0, true));
+        ctx->stats.launches_fwd++;
+        ctx->stats.batches += rd->nsmall;
+        int hh[2];
+        HIPTRY(hipMemcpyAsync(hh, ctx->red_head.p, sizeof(hh), hipMemcpyDeviceToHost, s));
+        HIPTRY(hipMemcpyAsync(ctx->h_flags, ctx->flags.p, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+        HIPTRY(hipStreamSynchronize(s));
+        if (ctx->h_flags[1]) return CENT_E_OVERFLOW;
+        if (hh[1] > 0) {
+            deep.resize((size_t)hh[1]);
+            HIPTRY(hipMemcpy(deep.data(), ctx->red_deep.p, sizeof(int) * (size_t)hh[1], hipMemcpyDeviceToHost));
+            std::sort(deep.begin(), deep.end());
+        }
+        for (int i : rd->big) TRY(run_item(i));
+        for (int i : deep) TRY(run_item(i));
+    } else {
+        for (int i = 0; i < r->nitems; ++i) TRY(run_item(i));
+    }
+    std::vector<int64_t> D(NP);
+    std::vector<double> bcp(NP);
+    std::vector<double> part;
+    HIPTRY(hipMemcpyAsync(D.data(), Dd, sizeof(int64_t) * NP, hipMemcpyDeviceToHost, s));
+    HIPTRY(hipMemcpyAsync(bcp.data(), ctx->red_bcp.p, sizeof(double) * NP, hipMemcpyDeviceToHost, s));
+    if (persistent && rd->part_total > 0) {
+        part.resize((size_t)rd->part_total);
+        HIPTRY(hipMemcpyAsync(part.data(), ctx->red_part.p, sizeof(double) * part.size(), hipMemcpyDeviceToHost, s));
+    }
+    HIPTRY(hipStreamSynchronize(s));
+    timer_collect(ctx, tm);
+    if (persistent) {
+        // add the small items' partials in item order (deep items were redone above)
+        std::vector<char> is_deep((size_t)(r->nitems > 0 ? r->nitems : 1), 0);
+        for (int i : deep) is_deep[(size_t)i] = 1;
+        for (int i = 0; i < r->nitems; ++i) {
+            const int64_t o = rd->h_poff[(size_t)i];
+            if (o < 0 || is_deep[(size_t)i]) continue;
+            const cent_item& it = r->items[i];
+            for (int x = it.lo; x < it.hi; ++x) bcp[(size_t)x] += part[(size_t)(o + x - it.lo)];
+        }
+    }
+    TRY(cent_reduced_finish_bc(r, bcp.data(), bc));
+    if (farness) TRY(cent_reduced_finish(r, D.data(), farness));
+    ctx->stats.levels += levels;
+    ctx->stats.reached += (int64_t)r->n;
+    ctx->stats.edges_reached += 2 * r->m;
+    ctx->stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return CENT_OK;
+}
