@@ -72,10 +72,15 @@ struct cent_ctx {
     DevBuf<unsigned long long> ps_depth;
     size_t F_levels = 0;     // levels the F buffer holds for the current N
     size_t F_n = 0;
-    int* h_flags = nullptr;  // pinned
+    int* h_flags = nullptr;  // pinned, kHostFlags ints
+    DevBuf<int> lvf;         // per-level "non-empty" flags of an enqueued run of levels
+    int depth_hint = 0;      // depth of the last dense batch (length of the next run)
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
 };
+
+// Pinned host ints of a context (level flags of a run, status words).
+constexpr int kHostFlags = 64;
 
 // Rows longer than kHubDeg are scanned as kHubDeg-edge segments, one
 // wavefront each (see msbfs_kernels.h).

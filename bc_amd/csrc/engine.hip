@@ -116,7 +116,7 @@ extern "C" int cent_ctx_create(int32_t device, cent_ctx** out) {
         delete ctx;
         return CENT_E_DEVICE;
     }
-    if (hipHostMalloc((void**)&ctx->h_flags, sizeof(int) * 2, hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void**)&ctx->h_flags, sizeof(int) * kHostFlags, hipHostMallocDefault) != hipSuccess) {
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;
         return CENT_E_DEVICE;
@@ -139,7 +139,7 @@ extern "C" void cent_ctx_destroy(cent_ctx* ctx) {
     release(ctx->red_part); release(ctx->red_bcp); release(ctx->it_sig); release(ctx->it_coef);
     release(ctx->it_lvl); release(ctx->red_deep);
     release(ctx->ps_rec); release(ctx->ps_Q); release(ctx->ps_loff); release(ctx->ps_bcp);
-    release(ctx->ps_depth);
+    release(ctx->ps_depth); release(ctx->lvf);
     if (ctx->h_flags) (void)hipHostFree(ctx->h_flags);
     for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
@@ -185,7 +185,7 @@ extern "C" int cent_ctx_trim(cent_ctx* ctx) {
     release(ctx->red_part); release(ctx->red_bcp); release(ctx->it_sig); release(ctx->it_coef);
     release(ctx->it_lvl); release(ctx->red_deep);
     release(ctx->ps_rec); release(ctx->ps_Q); release(ctx->ps_loff); release(ctx->ps_bcp);
-    release(ctx->ps_depth);
+    release(ctx->ps_depth); release(ctx->lvf);
     ctx->F_levels = 0;
     ctx->F_n = 0;
     return CENT_OK;

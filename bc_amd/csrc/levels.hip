@@ -117,54 +117,74 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
         Ghub = std::min((v.nhub + kWavesPerBlock - 1) / kWavesPerBlock, G);  // hubfin flushes into G slots
     }
     const bool bc = sp.with_bc, compat = sp.compat;
-    int dl = 0;  // current frontier distance
+    // Forward levels go in runs enqueued without waiting: level d0 + i of a
+    // run reads lvf[i] ("level d0 + i is non-empty", lvf[0] = 1) and sets
+    // lvf[i + 1]; once the BFS has ended the remaining launches return at
+    // once.  The host checks the flags once per run.  A run is as long as
+    // the last batch's depth suggests.
+    TRY(ensure(ctx, ctx->lvf, (size_t)kHostFlags));
+    int dl = 0;  // first level of the current run; on exit, the deepest non-empty level
     for (;;) {
-        if (bc && (size_t)(dl + 2) > ctx->F_levels) TRY(grow_levels(ctx, N, dl + 1));
-        const uint64_t* Fd = ctx->F.p + N * (size_t)(bc ? dl : planes * (dl & 1));
-        uint64_t* Fn = ctx->F.p + N * (size_t)(bc ? dl + 1 : planes * ((dl + 1) & 1));
-        if (!bc || dl + 1 > 1)
-            for (int b = 0; b < nb; ++b)
-                HIPTRY(hipMemsetAsync(Fn + (size_t)b * N + v.lo, 0, sizeof(uint64_t) * R, s));
-        HIPTRY(hipMemsetAsync(ctx->flags.p, 0, sizeof(int), s));
-        FwdArgs fa{};
-        fa.bstride = N; fa.pslots = G; fa.A_last = A_last; fa.dist_atomic = nb > 1;
-        fa.n = d.n; fa.hub_deg = d.hub_deg; fa.nchunk = v.nchunk; fa.chunk_beg = d.chunk_beg + v.c0;
-        fa.rp = d.rp; fa.col = d.col; fa.vis = ctx->vis.p; fa.Fd = Fd; fa.Fn = Fn;
-        fa.sigma = bc ? ctx->sigma.p : nullptr;
-        fa.npred = compat ? ctx->npred.p : nullptr;
-        fa.dist_sum = sp.dist_sum;
-        fa.weight = sp.weight;
-        fa.A = A; fa.dnext = dl + 1;
-        fa.part_far = ctx->part_far.p; fa.part_cnt = ctx->part_cnt.p;
-        fa.flag = ctx->flags.p; fa.overflow = ctx->flags.p + 1;
-        hipEvent_t e0;
-        TRY(timer_begin(ctx, tm, &e0));
-        if (bc) {
-            if (compat) k_fwd<true, true><<<G, kBlock, 0, s>>>(fa);
-            else k_fwd<true, false><<<G, kBlock, 0, s>>>(fa);
-        } else {
-            k_fwd<false, false><<<dim3(G, nb), kBlock, 0, s>>>(fa);
-        }
-        HIPTRY(hipGetLastError());
-        if (v.nhub) {
+        int run = std::max(2, std::min(kHostFlags - 2, ctx->depth_hint + 2 - dl));
+        if (bc && (size_t)(dl + run + 1) > ctx->F_levels) TRY(grow_levels(ctx, N, dl + 1));
+        if (bc) run = std::min(run, (int)ctx->F_levels - 1 - dl);
+        HIPTRY(hipMemsetAsync(ctx->lvf.p, 0, sizeof(int) * (size_t)(run + 1), s));
+        HIPTRY(hipMemsetD32Async((hipDeviceptr_t)ctx->lvf.p, 1, 1, s));
+        for (int i = 0; i < run; ++i) {
+            const int lv = dl + i;
+            const uint64_t* Fd = ctx->F.p + N * (size_t)(bc ? lv : planes * (lv & 1));
+            uint64_t* Fn = ctx->F.p + N * (size_t)(bc ? lv + 1 : planes * ((lv + 1) & 1));
+            if (!bc || lv + 1 > 1)
+                for (int b = 0; b < nb; ++b)
+                    HIPTRY(hipMemsetAsync(Fn + (size_t)b * N + v.lo, 0, sizeof(uint64_t) * R, s));
+            FwdArgs fa{};
+            fa.bstride = N; fa.pslots = G; fa.A_last = A_last; fa.dist_atomic = nb > 1;
+            fa.n = d.n; fa.hub_deg = d.hub_deg; fa.nchunk = v.nchunk; fa.chunk_beg = d.chunk_beg + v.c0;
+            fa.rp = d.rp; fa.col = d.col; fa.vis = ctx->vis.p; fa.Fd = Fd; fa.Fn = Fn;
+            fa.sigma = bc ? ctx->sigma.p : nullptr;
+            fa.npred = compat ? ctx->npred.p : nullptr;
+            fa.dist_sum = sp.dist_sum;
+            fa.weight = sp.weight;
+            fa.A = A; fa.dnext = lv + 1;
+            fa.part_far = ctx->part_far.p; fa.part_cnt = ctx->part_cnt.p;
+            fa.flag = ctx->lvf.p + i + 1; fa.overflow = ctx->flags.p + 1;
+            fa.live = ctx->lvf.p + i;
+            hipEvent_t e0;
+            TRY(timer_begin(ctx, tm, &e0));
             if (bc) {
-                k_fwd_hubseg<true><<<Gseg, kBlock, 0, s>>>(fa, ha);
-                if (compat) k_fwd_hubfin<true, true><<<Ghub, kBlock, 0, s>>>(fa, ha);
-                else k_fwd_hubfin<true, false><<<Ghub, kBlock, 0, s>>>(fa, ha);
+                if (compat) k_fwd<true, true><<<G, kBlock, 0, s>>>(fa);
+                else k_fwd<true, false><<<G, kBlock, 0, s>>>(fa);
             } else {
-                k_fwd_hubseg<false><<<dim3(Gseg, nb), kBlock, 0, s>>>(fa, ha);
-                k_fwd_hubfin<false, false><<<dim3(Ghub, nb), kBlock, 0, s>>>(fa, ha);
+                k_fwd<false, false><<<dim3(G, nb), kBlock, 0, s>>>(fa);
             }
             HIPTRY(hipGetLastError());
+            if (v.nhub) {
+                if (bc) {
+                    k_fwd_hubseg<true><<<Gseg, kBlock, 0, s>>>(fa, ha);
+                    if (compat) k_fwd_hubfin<true, true><<<Ghub, kBlock, 0, s>>>(fa, ha);
+                    else k_fwd_hubfin<true, false><<<Ghub, kBlock, 0, s>>>(fa, ha);
+                } else {
+                    k_fwd_hubseg<false><<<dim3(Gseg, nb), kBlock, 0, s>>>(fa, ha);
+                    k_fwd_hubfin<false, false><<<dim3(Ghub, nb), kBlock, 0, s>>>(fa, ha);
+                }
+                HIPTRY(hipGetLastError());
+            }
+            TRY(timer_end(ctx, tm, e0, true));
         }
-        TRY(timer_end(ctx, tm, e0, true));
-        ctx->stats.launches_fwd++;
-        HIPTRY(hipMemcpyAsync(ctx->h_flags, ctx->flags.p, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+        HIPTRY(hipMemcpyAsync(ctx->h_flags, ctx->lvf.p, sizeof(int) * (size_t)(run + 1), hipMemcpyDeviceToHost, s));
+        HIPTRY(hipMemcpyAsync(ctx->h_flags + kHostFlags - 1, ctx->flags.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));
         HIPTRY(hipStreamSynchronize(s));
-        if (ctx->h_flags[1]) return CENT_E_OVERFLOW;
-        if (!ctx->h_flags[0]) break;
-        ++dl;
+        if (ctx->h_flags[kHostFlags - 1]) return CENT_E_OVERFLOW;
+        int i = 1;
+        while (i <= run && ctx->h_flags[i]) ++i;  // levels dl .. dl + i - 1 are non-empty
+        ctx->stats.launches_fwd += std::min(i, run);
+        if (i <= run) {
+            dl += i - 1;
+            break;
+        }
+        dl += run;
     }
+    ctx->depth_hint = dl;
     *levels += (int64_t)dl * nb;  // deepest non-empty level, per batch
     if (bc) {
         for (int lv = dl; lv >= 1; --lv) {

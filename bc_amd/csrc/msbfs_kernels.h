@@ -76,6 +76,9 @@ struct FwdArgs {
     unsigned long long* __restrict__ part_cnt;  // [grid][2]: reached pairs, dag edges
     int* __restrict__ flag;               // set to 1 when anything was reached
     int* __restrict__ overflow;           // set to 1 when a path count is inf
+    // Levels are enqueued ahead of the host's check: a launch whose level
+    // is empty (*live == 0, written by the previous level) returns at once.
+    const int* __restrict__ live;
     // Several independent 64-source batches in one launch (closeness only):
     // batch b = blockIdx.y uses vis / Fd / Fn + b * bstride, its own
     // partial slots (pslots blocks each) and active lanes A (A_last for the
@@ -286,6 +289,7 @@ __device__ __forceinline__ void bwd_vertex_done(const BwdArgs& a, int v, uint64_
 template <bool SIGMA, bool COMPAT>
 __global__ __launch_bounds__(kBlock) void k_fwd(FwdArgs a0) {
     const FwdArgs a = batch_view(a0);
+    if (a.live && *a.live == 0) return;  // uniform over the grid
     const int lane = lane_id();
     const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const int nw = gridDim.x * kWavesPerBlock;
@@ -323,6 +327,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd(FwdArgs a0) {
 template <bool SIGMA>
 __global__ __launch_bounds__(kBlock) void k_fwd_hubseg(FwdArgs a0, HubArgs h0) {
     const FwdArgs a = batch_view(a0);
+    if (a.live && *a.live == 0) return;  // uniform over the grid
     const HubArgs h = hub_view(h0);
     const int lane = lane_id();
     const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
@@ -348,6 +353,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_hubseg(FwdArgs a0, HubArgs h0) {
 template <bool SIGMA, bool COMPAT>
 __global__ __launch_bounds__(kBlock) void k_fwd_hubfin(FwdArgs a0, HubArgs h0) {
     const FwdArgs a = batch_view(a0);
+    if (a.live && *a.live == 0) return;  // uniform over the grid
     const HubArgs h = hub_view(h0);
     const int lane = lane_id();
     const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
