@@ -21,13 +21,15 @@ static int grid_for(int chunks, const cent_opts* o) {
 static int ensure_workspace(cent_ctx* ctx, const DevCSR& d, const BatchSpec& sp, int G, int nb) {
     const size_t N = (size_t)(d.n > 0 ? d.n : 1);
     TRY(ensure(ctx, ctx->vis, N * (size_t)nb));
-    // BC keeps one mask array per level; closeness a level-parity pair per batch
-    const size_t want = sp.with_bc ? N * 17 : N * (2 * (size_t)nb + 1);
+    // BC keeps one {mask, base} record plane per level (2 words a vertex);
+    // closeness a level-parity pair of mask planes per batch
+    const size_t fw = sp.with_bc ? 2 : 1;
+    const size_t want = sp.with_bc ? 2 * N * 17 : N * (2 * (size_t)nb + 1);
     if (ctx->F_n != N || ctx->F.cap < want) {
         TRY(ensure(ctx, ctx->F, want));
         ctx->F_n = N;
     }
-    ctx->F_levels = ctx->F.cap / N - 1;
+    ctx->F_levels = ctx->F.cap / (N * fw) - 1;
     if (sp.with_bc) {
         TRY(ensure(ctx, ctx->sigma, N * kWave));
         TRY(ensure(ctx, ctx->coef, N * kWave));
@@ -45,16 +47,17 @@ static int ensure_workspace(cent_ctx* ctx, const DevCSR& d, const BatchSpec& sp,
     return CENT_OK;
 }
 
-static int grow_levels(cent_ctx* ctx, size_t N, int keep_levels) {
+// Doubles the level planes of F (PL words each), keeping the first keep_levels.
+static int grow_levels(cent_ctx* ctx, size_t PL, int keep_levels) {
     const size_t ncap = ctx->F_levels * 2;
     uint64_t* nf = nullptr;
-    HIPTRY(hipMalloc((void**)&nf, sizeof(uint64_t) * N * (ncap + 1)));
-    HIPTRY(hipMemcpyAsync(nf, ctx->F.p, sizeof(uint64_t) * N * (size_t)keep_levels, hipMemcpyDeviceToDevice,
+    HIPTRY(hipMalloc((void**)&nf, sizeof(uint64_t) * PL * (ncap + 1)));
+    HIPTRY(hipMemcpyAsync(nf, ctx->F.p, sizeof(uint64_t) * PL * (size_t)keep_levels, hipMemcpyDeviceToDevice,
                           ctx->stream));
     HIPTRY(hipStreamSynchronize(ctx->stream));
     HIPTRY(hipFree(ctx->F.p));
     ctx->F.p = nf;
-    ctx->F.cap = N * (ncap + 1);
+    ctx->F.cap = PL * (ncap + 1);
     ctx->F_levels = ncap;
     return CENT_OK;
 }
@@ -92,17 +95,22 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
     const int kl = k - (nb - 1) * kWave;  // sources of the last batch
     const uint64_t A_last = (kl == 64) ? ~0ull : ((1ull << kl) - 1);
     const uint64_t A = nb == 1 ? A_last : ~0ull;
-    const size_t planes = sp.with_bc ? 1 : (size_t)nb;  // closeness: batch b's level-parity plane p at (p*nb + b)*N
+    // F planes: BC level lv at F + lv * PL ({mask, base} records, PL = 2N
+    // words); closeness batch b's level-parity plane p at F + (p * nb + b) * N
+    const size_t fw = sp.with_bc ? 2 : 1;
+    const size_t PL = N * fw;
+    const size_t planes = sp.with_bc ? 1 : (size_t)nb;
+    auto plane = [&](size_t p, size_t b) { return ctx->F.p + (p * planes + b) * PL; };
     for (int b = 0; b < nb; ++b) {
         HIPTRY(hipMemsetAsync(ctx->vis.p + (size_t)b * N + v.lo, 0, sizeof(uint64_t) * R, s));
-        HIPTRY(hipMemsetAsync(ctx->F.p + (size_t)b * N + v.lo, 0, sizeof(uint64_t) * R, s));
-        HIPTRY(hipMemsetAsync(ctx->F.p + (planes + b) * N + v.lo, 0, sizeof(uint64_t) * R, s));
+        HIPTRY(hipMemsetAsync(plane(0, b) + fw * v.lo, 0, sizeof(uint64_t) * fw * R, s));
+        HIPTRY(hipMemsetAsync(plane(1, b) + fw * v.lo, 0, sizeof(uint64_t) * fw * R, s));
     }
     HIPTRY(hipMemsetAsync(ctx->part_far.p, 0, sizeof(int64_t) * (size_t)G * kWave * nb, s));
     HIPTRY(hipMemsetAsync(ctx->part_cnt.p, 0, sizeof(unsigned long long) * (size_t)G * 2 * nb, s));
     HIPTRY(hipMemsetAsync(ctx->flags.p, 0, sizeof(int) * 2, s));
     for (int b = 0; b < nb; ++b)
-        k_seed<<<1, kWave, 0, s>>>(d_src + b * kWave, b == nb - 1 ? kl : kWave, ctx->F.p + (size_t)b * N,
+        k_seed<<<1, kWave, 0, s>>>(d_src + b * kWave, b == nb - 1 ? kl : kWave, plane(0, b),
                                    ctx->vis.p + (size_t)b * N, sp.with_bc ? ctx->sigma.p : nullptr, sp.with_bc);
     HIPTRY(hipGetLastError());
 
@@ -126,17 +134,17 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
     int dl = 0;  // first level of the current run; on exit, the deepest non-empty level
     for (;;) {
         int run = std::max(2, std::min(kHostFlags - 2, ctx->depth_hint + 2 - dl));
-        if (bc && (size_t)(dl + run + 1) > ctx->F_levels) TRY(grow_levels(ctx, N, dl + 1));
+        if (bc && (size_t)(dl + run + 1) > ctx->F_levels) TRY(grow_levels(ctx, PL, dl + 1));
         if (bc) run = std::min(run, (int)ctx->F_levels - 1 - dl);
         HIPTRY(hipMemsetAsync(ctx->lvf.p, 0, sizeof(int) * (size_t)(run + 1), s));
         HIPTRY(hipMemsetD32Async((hipDeviceptr_t)ctx->lvf.p, 1, 1, s));
         for (int i = 0; i < run; ++i) {
             const int lv = dl + i;
-            const uint64_t* Fd = ctx->F.p + N * (size_t)(bc ? lv : planes * (lv & 1));
-            uint64_t* Fn = ctx->F.p + N * (size_t)(bc ? lv + 1 : planes * ((lv + 1) & 1));
+            const uint64_t* Fd = bc ? ctx->F.p + PL * (size_t)lv : plane(lv & 1, 0);
+            uint64_t* Fn = bc ? ctx->F.p + PL * (size_t)(lv + 1) : plane((lv + 1) & 1, 0);
             if (!bc || lv + 1 > 1)
                 for (int b = 0; b < nb; ++b)
-                    HIPTRY(hipMemsetAsync(Fn + (size_t)b * N + v.lo, 0, sizeof(uint64_t) * R, s));
+                    HIPTRY(hipMemsetAsync(Fn + (size_t)b * PL + fw * v.lo, 0, sizeof(uint64_t) * fw * R, s));
             FwdArgs fa{};
             fa.bstride = N; fa.pslots = G; fa.A_last = A_last; fa.dist_atomic = nb > 1;
             fa.n = d.n; fa.hub_deg = d.hub_deg; fa.nchunk = v.nchunk; fa.chunk_beg = d.chunk_beg + v.c0;
@@ -191,8 +199,8 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
             BwdArgs ba;
             ba.n = d.n; ba.hub_deg = d.hub_deg; ba.nchunk = v.nchunk; ba.chunk_beg = d.chunk_beg + v.c0;
             ba.rp = d.rp; ba.col = d.col;
-            ba.Fd = ctx->F.p + N * (size_t)lv;
-            ba.Fn = ctx->F.p + N * (size_t)(lv + 1);
+            ba.Fd = ctx->F.p + PL * (size_t)lv;
+            ba.Fn = ctx->F.p + PL * (size_t)(lv + 1);
             ba.sigma = ctx->sigma.p; ba.coef = ctx->coef.p;
             ba.npred = compat ? ctx->npred.p : nullptr;
             ba.bc = sp.bc;

@@ -125,21 +125,53 @@ struct BwdArgs {
 };
 
 // ---------------------------------------------------------------- gathers
+// Level-sorted rows (LS, the dense betweenness step): a vertex's 64 sigma /
+// coef entries are kept in order of the level at which each lane reached
+// it, so one level's lanes form one contiguous slice of the 512-byte row
+// and a gather of that level touches one or two cache lines instead of four.
+// Lane s of v at level d sits at row position base_d(v) + rank of s in
+// F_d(v), base_d(v) = number of lanes that reached v earlier.  The frontier
+// entry of (d, v) is then a 16-byte record {F_d(v), base_d(v)}, read in the
+// same load as the mask.  Without LS (frontier-queue step): plain masks and
+// lane-indexed rows.
+struct FRec {
+    uint64_t mask;
+    uint64_t base;
+};
+
+// Number of set bits of m below the calling lane.
+__device__ __forceinline__ int lane_rank(uint64_t m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+template <bool LS>
+__device__ __forceinline__ void load_front(const uint64_t* __restrict__ F, int v, uint64_t& mask, int& base) {
+    if constexpr (LS) {
+        const FRec r = reinterpret_cast<const FRec*>(F)[v];
+        mask = r.mask;
+        base = (int)r.base;
+    } else {
+        mask = F[v];
+        base = 0;
+    }
+}
+
 // Forward scan of edges [beg, end): lane s sums sigma[u][s] (SIGMA) over the
 // neighbours u with bit s in Fd[u] & cand, or ORs the masks (closeness only).
-template <bool SIGMA>
+template <bool SIGMA, bool LS>
 __device__ __forceinline__ void gather_fwd(const int* __restrict__ col, const uint64_t* __restrict__ Fd,
                                            const double* __restrict__ sigma, int beg, int end,
                                            uint64_t cand, int lane, double& acc, uint32_t& cnt,
                                            uint64_t& orf) {
     for (int j0 = beg; j0 < end; j0 += kWave) {
         const int j = j0 + lane;
-        int u = 0;
-        uint64_t f = 0;
+        int u = 0, bu = 0;
+        uint64_t fm = 0;
         if (j < end) {
             u = col[j];
-            f = Fd[u] & cand;
+            load_front<LS>(Fd, u, fm, bu);
         }
+        const uint64_t f = fm & cand;
         if constexpr (SIGMA) {
             uint64_t ball = ballot(f != 0);
             while (ball) {
@@ -151,9 +183,10 @@ __device__ __forceinline__ void gather_fwd(const int* __restrict__ col, const ui
                         const int c = __builtin_ctzll(ball);
                         ball &= ball - 1;
                         const int uc = rl32(u, c);
-                        const uint64_t fc = rl64(f, c);
-                        if ((fc >> lane) & 1) {
-                            x[t] = sigma[(size_t)uc * kWave + lane];
+                        const uint64_t fmc = rl64(fm, c);
+                        if (((fmc & cand) >> lane) & 1) {
+                            const int pos = LS ? rl32(bu, c) + lane_rank(fmc) : lane;
+                            x[t] = sigma[(size_t)uc * kWave + pos];
                             ++cnt;
                         }
                     }
@@ -169,19 +202,20 @@ __device__ __forceinline__ void gather_fwd(const int* __restrict__ col, const ui
 
 // Backward scan of edges [beg, end): lane s sums coef[w][s] over neighbours w
 // with bit s in Fn[w] & m.
+template <bool LS>
 __device__ __forceinline__ double gather_bwd(const int* __restrict__ col, const uint64_t* __restrict__ Fn,
                                              const double* __restrict__ coef, int beg, int end,
                                              uint64_t m, int lane) {
     double acc = 0.0;
     for (int j0 = beg; j0 < end; j0 += kWave) {
         const int j = j0 + lane;
-        int w = 0;
-        uint64_t g = 0;
+        int w = 0, bw = 0;
+        uint64_t fm = 0;
         if (j < end) {
             w = col[j];
-            g = Fn[w] & m;
+            load_front<LS>(Fn, w, fm, bw);
         }
-        uint64_t ball = ballot(g != 0);
+        uint64_t ball = ballot((fm & m) != 0);
         while (ball) {
             double x[kUnroll];
 #pragma unroll
@@ -191,8 +225,11 @@ __device__ __forceinline__ double gather_bwd(const int* __restrict__ col, const 
                     const int c = __builtin_ctzll(ball);
                     ball &= ball - 1;
                     const int wc = rl32(w, c);
-                    const uint64_t gc = rl64(g, c);
-                    if ((gc >> lane) & 1) x[t] = coef[(size_t)wc * kWave + lane];
+                    const uint64_t fmc = rl64(fm, c);
+                    if (((fmc & m) >> lane) & 1) {
+                        const int pos = LS ? rl32(bw, c) + lane_rank(fmc) : lane;
+                        x[t] = coef[(size_t)wc * kWave + pos];
+                    }
                 }
             }
 #pragma unroll
@@ -209,24 +246,27 @@ struct FwdAccum {
     bool ovf = false;
 };
 
-template <bool SIGMA, bool COMPAT>
+template <bool SIGMA, bool COMPAT, bool LS>
 __device__ __forceinline__ void fwd_vertex_done(const FwdArgs& a, int v, uint64_t old, double acc,
                                                 uint32_t cnt, uint64_t orf_all, int lane, FwdAccum& r) {
     uint64_t nm;
     if constexpr (SIGMA) nm = ballot(cnt != 0);
     else nm = orf_all;
     if (!nm) return;
+    const int base = LS ? __popcll(old) : 0;  // lanes that reached v earlier
     if ((nm >> lane) & 1) {
+        const size_t idx = (size_t)v * kWave + (LS ? base + lane_rank(nm) : lane);
         if constexpr (SIGMA) {
-            a.sigma[(size_t)v * kWave + lane] = acc;
+            a.sigma[idx] = acc;
             if (__builtin_isinf(acc)) r.ovf = true;
         }
-        if constexpr (COMPAT) a.npred[(size_t)v * kWave + lane] = cnt;
+        if constexpr (COMPAT) a.npred[idx] = cnt;
         r.far += a.weight ? (int64_t)a.dnext * a.weight[v] : (int64_t)a.dnext;
         r.dag += cnt;
     }
     if (lane == 0) {
-        a.Fn[v] = nm;
+        if constexpr (LS) reinterpret_cast<FRec*>(a.Fn)[v] = FRec{nm, (uint64_t)base};
+        else a.Fn[v] = nm;
         a.vis[v] = old | nm;
         const int pc = __popcll(nm);
         if (a.dist_sum) {
@@ -266,11 +306,12 @@ __device__ __forceinline__ void fwd_block_flush(const FwdArgs& a, const FwdAccum
     }
 }
 
-__device__ __forceinline__ void bwd_vertex_done(const BwdArgs& a, int v, uint64_t m, double acc,
+template <bool LS>
+__device__ __forceinline__ void bwd_vertex_done(const BwdArgs& a, int v, uint64_t m, int base, double acc,
                                                 int lane, bool compat) {
     double contrib = 0.0;
     if ((m >> lane) & 1) {
-        const size_t idx = (size_t)v * kWave + lane;
+        const size_t idx = (size_t)v * kWave + (LS ? base + lane_rank(m) : lane);
         const double sg = a.sigma[idx];
         const double dl = sg * acc;
         if (a.wgt) {
@@ -317,8 +358,8 @@ __global__ __launch_bounds__(kBlock) void k_fwd(FwdArgs a0) {
             double acc = 0.0;
             uint32_t cnt = 0;
             uint64_t orf = 0;
-            gather_fwd<SIGMA>(a.col, a.Fd, a.sigma, rl32(rbl, b), rl32(rel, b), A & ~old, lane, acc, cnt, orf);
-            fwd_vertex_done<SIGMA, COMPAT>(a, v, old, acc, cnt, SIGMA ? 0 : wave_or(orf), lane, r);
+            gather_fwd<SIGMA, SIGMA>(a.col, a.Fd, a.sigma, rl32(rbl, b), rl32(rel, b), A & ~old, lane, acc, cnt, orf);
+            fwd_vertex_done<SIGMA, COMPAT, SIGMA>(a, v, old, acc, cnt, SIGMA ? 0 : wave_or(orf), lane, r);
         }
     }
     fwd_block_flush(a, r, blockIdx.x);
@@ -339,7 +380,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_hubseg(FwdArgs a0, HubArgs h0) {
         double acc = 0.0;
         uint32_t cnt = 0;
         uint64_t orf = 0;
-        gather_fwd<SIGMA>(a.col, a.Fd, a.sigma, h.seg_beg[i], h.seg_end[i], cand, lane, acc, cnt, orf);
+        gather_fwd<SIGMA, SIGMA>(a.col, a.Fd, a.sigma, h.seg_beg[i], h.seg_end[i], cand, lane, acc, cnt, orf);
         if constexpr (SIGMA) {
             h.part_acc[(size_t)i * kWave + lane] = acc;
             h.part_cnt[(size_t)i * kWave + lane] = cnt;
@@ -374,7 +415,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_hubfin(FwdArgs a0, HubArgs h0) {
                 orf |= h.part_or[i];
             }
         }
-        fwd_vertex_done<SIGMA, COMPAT>(a, v, old, acc, cnt, orf, lane, r);
+        fwd_vertex_done<SIGMA, COMPAT, SIGMA>(a, v, old, acc, cnt, orf, lane, r);
     }
     fwd_block_flush(a, r, blockIdx.x);
 }
@@ -389,9 +430,9 @@ __global__ __launch_bounds__(kBlock) void k_bwd(BwdArgs a) {
         const int base = a.chunk_beg[c];
         const int vl = base + lane;
         uint64_t fl = 0;
-        int rbl = 0, rel = 0;
+        int rbl = 0, rel = 0, bl = 0;
         if (vl < a.chunk_beg[c + 1]) {
-            fl = a.Fd[vl];
+            load_front<true>(a.Fd, vl, fl, bl);
             if (fl) {
                 rbl = a.rp[vl];
                 rel = a.rp[vl + 1];
@@ -403,8 +444,8 @@ __global__ __launch_bounds__(kBlock) void k_bwd(BwdArgs a) {
             const int b = __builtin_ctzll(work);
             work &= work - 1;
             const uint64_t m = rl64(fl, b);
-            const double acc = gather_bwd(a.col, a.Fn, a.coef, rl32(rbl, b), rl32(rel, b), m, lane);
-            bwd_vertex_done(a, base + b, m, acc, lane, COMPAT);
+            const double acc = gather_bwd<true>(a.col, a.Fn, a.coef, rl32(rbl, b), rl32(rel, b), m, lane);
+            bwd_vertex_done<true>(a, base + b, m, rl32(bl, b), acc, lane, COMPAT);
         }
     }
 }
@@ -414,9 +455,11 @@ __global__ __launch_bounds__(kBlock) void k_bwd_hubseg(BwdArgs a, HubArgs h) {
     const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const int nw = gridDim.x * kWavesPerBlock;
     for (int i = gw; i < h.nseg; i += nw) {
-        const uint64_t m = a.Fd[h.seg_v[i]];
+        uint64_t m;
+        int bm;
+        load_front<true>(a.Fd, h.seg_v[i], m, bm);
         if (!m) continue;
-        h.part_acc[(size_t)i * kWave + lane] = gather_bwd(a.col, a.Fn, a.coef, h.seg_beg[i], h.seg_end[i], m, lane);
+        h.part_acc[(size_t)i * kWave + lane] = gather_bwd<true>(a.col, a.Fn, a.coef, h.seg_beg[i], h.seg_end[i], m, lane);
     }
 }
 
@@ -427,25 +470,32 @@ __global__ __launch_bounds__(kBlock) void k_bwd_hubfin(BwdArgs a, HubArgs h) {
     const int nw = gridDim.x * kWavesPerBlock;
     for (int k = gw; k < h.nhub; k += nw) {
         const int v = h.hub_v[k];
-        const uint64_t m = a.Fd[v];
+        uint64_t m;
+        int bm;
+        load_front<true>(a.Fd, v, m, bm);
         if (!m) continue;
         double acc = 0.0;
         for (int i = h.hub_seg_off[k] - h.seg_base; i < h.hub_seg_off[k + 1] - h.seg_base; ++i)
             acc += h.part_acc[(size_t)i * kWave + lane];
-        bwd_vertex_done(a, v, m, acc, lane, COMPAT);
+        bwd_vertex_done<true>(a, v, m, bm, acc, lane, COMPAT);
     }
 }
 
 // ---------------------------------------------------------------- batch setup
-// Seeds a batch: lane s starts at src[s] (duplicates allowed, each its own lane).
+// Seeds a batch: lane s starts at src[s] (duplicates allowed, each its own
+// lane).  With path counts (betweenness) F0 holds {mask, base 0} records and
+// the sources' rows are level-sorted (base 0); the caller zeroes F0.
 __global__ void k_seed(const int* __restrict__ src, int k, uint64_t* __restrict__ F0,
                        uint64_t* __restrict__ vis, double* __restrict__ sigma, bool with_sigma) {
     const int lane = threadIdx.x;
-    if (lane >= k) return;
-    const int s = src[lane];
-    atomicOr((unsigned long long*)&F0[s], 1ull << lane);
-    atomicOr((unsigned long long*)&vis[s], 1ull << lane);
-    if (with_sigma) sigma[(size_t)s * kWave + lane] = 1.0;
+    const int s = (lane < k) ? src[lane] : 0;
+    uint64_t* m0 = with_sigma ? &reinterpret_cast<FRec*>(F0)[s].mask : &F0[s];
+    if (lane < k) {
+        atomicOr((unsigned long long*)m0, 1ull << lane);
+        atomicOr((unsigned long long*)&vis[s], 1ull << lane);
+    }
+    __syncthreads();
+    if (lane < k && with_sigma) sigma[(size_t)s * kWave + lane_rank(*m0)] = 1.0;
 }
 
 // Sums the per-block lane partials of a batch into far[0..k) and the

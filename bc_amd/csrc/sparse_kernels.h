@@ -151,11 +151,11 @@ __global__ __launch_bounds__(kBlock) void k_sp_pull(SpArgs a) {
             double acc = 0.0;
             uint32_t cnt = 0;
             uint64_t orf = 0;
-            gather_fwd<SIGMA>(a.col, Fd, a.sigma, rl32(rb, b), rl32(re, b), A & ~old, lane, acc, cnt, orf);
+            gather_fwd<SIGMA, false>(a.col, Fd, a.sigma, rl32(rb, b), rl32(re, b), A & ~old, lane, acc, cnt, orf);
             uint64_t nm;
             if constexpr (SIGMA) nm = ballot(cnt != 0);
             else nm = wave_or(orf);
-            fwd_vertex_done<SIGMA, COMPAT>(f, vb, old, acc, cnt, nm, lane, r);
+            fwd_vertex_done<SIGMA, COMPAT, false>(f, vb, old, acc, cnt, nm, lane, r);
             if (nm) {
                 newbits |= 1ull << b;
                 if (lane == b) my_nm = nm;
@@ -245,8 +245,8 @@ __global__ __launch_bounds__(kBlock) void k_sp_bwd(SpArgs a) {
             const int bb = __builtin_ctzll(work);
             work &= work - 1;
             const uint64_t m = rl64(fl, bb);
-            const double acc = gather_bwd(a.col, Fn, a.coef, rl32(rb, bb), rl32(re, bb), m, lane);
-            bwd_vertex_done(b, rl32(v, bb), m, acc, lane, COMPAT);
+            const double acc = gather_bwd<false>(a.col, Fn, a.coef, rl32(rb, bb), rl32(re, bb), m, lane);
+            bwd_vertex_done<false>(b, rl32(v, bb), m, 0, acc, lane, COMPAT);
         }
     }
     if (d + 2 <= a.ctl[kCtlDeepest] + 1) {
