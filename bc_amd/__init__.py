@@ -313,6 +313,15 @@ _lib.cent_bc.argtypes = [C.c_void_p, C.c_void_p, _I32P, C.c_int32, _F64P, _I64P,
                          C.POINTER(_Opts)]
 _lib.cent_cc.argtypes = [C.c_void_p, C.c_void_p, _I32P, C.c_int32, _I64P, _I64P,
                          C.POINTER(_Opts)]
+_lib.cent_bc_device.argtypes = [C.c_void_p, C.c_void_p, _I32P, C.c_int32, C.c_void_p, C.c_void_p,
+                                C.c_void_p, C.POINTER(_Opts)]
+_lib.cent_cc_device.argtypes = [C.c_void_p, C.c_void_p, _I32P, C.c_int32, C.c_void_p, C.c_void_p,
+                                C.POINTER(_Opts)]
+CENT_COPY_TO_DEVICE, CENT_COPY_TO_HOST, CENT_COPY_ON_DEVICE = 1, 2, 3
+_lib.cent_device_alloc.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_void_p)]
+_lib.cent_device_free.argtypes = [C.c_void_p, C.c_void_p]
+_lib.cent_device_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32]
+_lib.cent_device_fill.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int64]
 _lib.cent_probe_copy_bandwidth.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.POINTER(C.c_double)]
 _lib.cent_cc_reduced.argtypes = [C.c_void_p, C.c_void_p, _I64P, C.c_int32, C.POINTER(_Opts)]
 _lib.cent_bc_reduced.argtypes = [C.c_void_p, C.c_void_p, _F64P, _I64P, C.c_int32, C.POINTER(_Opts)]
@@ -465,6 +474,45 @@ def _opts(mode=CENT_BC_TEXTBOOK, timing=False, grid_waves=0, frontier="auto", ma
     return o
 
 
+class DeviceArray:
+    """n elements of a numpy dtype in device memory of a context's GPU
+    (cent_device_alloc), freed with the object."""
+
+    def __init__(self, ctx: "Context", n: int, dtype):
+        self.ctx = ctx
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = C.c_void_p()
+        _check(_lib.cent_device_alloc(ctx._h, self.nbytes, C.byref(p)), "cent_device_alloc",
+               ctx.last_error())
+        self.ptr = p.value
+
+    def zero(self) -> "DeviceArray":
+        _check(_lib.cent_device_fill(self.ctx._h, self.ptr, 0, self.nbytes), "cent_device_fill",
+               self.ctx.last_error())
+        return self
+
+    def copy_from(self, a) -> "DeviceArray":
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.size != self.n:
+            raise ValueError(f"copy_from: {a.size} elements into {self.n}")
+        _check(_lib.cent_device_copy(self.ctx._h, self.ptr, a.ctypes.data, self.nbytes,
+                                     CENT_COPY_TO_DEVICE), "cent_device_copy", self.ctx.last_error())
+        return self
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.n, self.dtype)
+        _check(_lib.cent_device_copy(self.ctx._h, out.ctypes.data, self.ptr, self.nbytes,
+                                     CENT_COPY_TO_HOST), "cent_device_copy", self.ctx.last_error())
+        return out
+
+    def __del__(self):
+        p, self.ptr = getattr(self, "ptr", None), None
+        if p and getattr(self.ctx, "_h", None):
+            _lib.cent_device_free(self.ctx._h, p)
+
+
 class DeviceGraph:
     def __init__(self, ctx: "Context", csr: CSR, keep_order: bool = False, tuned: bool = True):
         self.ctx = ctx
@@ -589,6 +637,51 @@ class Context:
                "cent_cc", self.last_error())
         return far[:kk], (ds[:g.n] if ds is not None else None)
 
+    def device_array(self, n: int, dtype, fill=None) -> "DeviceArray":
+        """A device array of n elements on this context's GPU."""
+        a = DeviceArray(self, n, dtype)
+        if fill is not None:
+            a.copy_from(np.full(n, fill, dtype=a.dtype))
+        return a
+
+    def _dev_arg(self, a, dtype, length: int, what: str):
+        """Device address of a DeviceArray of this context with at least
+        ``length`` elements of ``dtype`` (None passes through)."""
+        if a is None:
+            return None
+        if not isinstance(a, DeviceArray) or a.ctx is not self:
+            raise ValueError(f"{what}: need a DeviceArray of this context, got {type(a).__name__}")
+        if a.dtype != np.dtype(dtype):
+            raise ValueError(f"{what}: need {np.dtype(dtype)}, got {a.dtype}")
+        if a.n < length:
+            raise ValueError(f"{what}: {a.n} elements, need {length}")
+        return a.ptr
+
+    def bc_device(self, g: DeviceGraph, sources, bc, farness=None, dist_sum=None,
+                  mode: int = CENT_BC_TEXTBOOK, timing: bool = False, grid_waves: int = 0,
+                  frontier: str = "auto", max_slots: int = 0, keep_source_order: bool = False) -> None:
+        """Brandes BC with the results left in DeviceArrays of this context:
+        bc (float64, n) accumulated, farness (int64, k) assigned, dist_sum
+        (int64, n) accumulated; same meaning as bc()."""
+        s, k, kk = self._src(g, sources)
+        o = _opts(mode, timing, grid_waves, frontier, max_slots, keep_source_order)
+        _check(_lib.cent_bc_device(self._h, g._h, _ptr(s, C.c_int32) if s is not None and k else None, k,
+                                   self._dev_arg(bc, np.float64, g.n, "bc"),
+                                   self._dev_arg(farness, np.int64, kk, "farness"),
+                                   self._dev_arg(dist_sum, np.int64, g.n, "dist_sum"), C.byref(o)),
+               "cent_bc_device", self.last_error())
+
+    def cc_device(self, g: DeviceGraph, sources, farness, dist_sum=None, timing: bool = False,
+                  grid_waves: int = 0, frontier: str = "auto", max_slots: int = 0,
+                  keep_source_order: bool = False) -> None:
+        """Closeness with farness (int64, k) and dist_sum (int64, n) in
+        DeviceArrays of this context; same meaning as cc()."""
+        s, k, kk = self._src(g, sources)
+        o = _opts(0, timing, grid_waves, frontier, max_slots, keep_source_order)
+        _check(_lib.cent_cc_device(self._h, g._h, _ptr(s, C.c_int32) if s is not None and k else None, k,
+                                   self._dev_arg(farness, np.int64, kk, "farness"),
+                                   self._dev_arg(dist_sum, np.int64, g.n, "dist_sum"), C.byref(o)),
+               "cent_cc_device", self.last_error())
 
     def cc_reduced(self, red: Reduced, sched: int = CENT_SCHED_PERSISTENT, timing: bool = False):
         """Farness of every vertex through the D1/AP-reduced form."""

@@ -55,7 +55,7 @@ struct cent_ctx {
     DevBuf<uint32_t> npred, hp_cnt;
     DevBuf<int64_t> dist_sum, far, part_far;
     DevBuf<unsigned long long> part_cnt, cnt;
-    DevBuf<int> src, flags;
+    DevBuf<int> src, flags, ord;
     // frontier-queue mode
     DevBuf<uint64_t> Fd2, QF, Adev;
     DevBuf<int> Q, qoff, cand, stamp, ctl;
@@ -73,7 +73,10 @@ struct cent_ctx {
     size_t F_levels = 0;     // levels the F buffer holds for the current N
     size_t F_n = 0;
     int* h_flags = nullptr;  // pinned, kHostFlags ints
+    void* h_stage = nullptr; // pinned staging for a call's inputs and results
+    size_t h_stage_bytes = 0;
     DevBuf<int> lvf;         // per-level "non-empty" flags of an enqueued run of levels
+    DevBuf<unsigned long long> lvstat;  // bounded batches: [0] summed depth, [1] bound failures
     int depth_hint = 0;      // depth of the last dense batch (length of the next run)
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
@@ -100,6 +103,20 @@ static inline int ensure(cent_ctx* ctx, DevBuf<T>& b, size_t n) {
     size_t cap = n + n / 4 + 64;
     HIPTRY(hipMalloc((void**)&b.p, cap * sizeof(T)));
     b.cap = cap;
+    return CENT_OK;
+}
+
+// Pinned host staging of at least `bytes` bytes (kept and grown per context):
+// copies between it and device memory run at full PCIe rate, where pageable
+// buffers go through the runtime's small staging chunks.
+static inline int ensure_stage(cent_ctx* ctx, size_t bytes) {
+    if (bytes <= ctx->h_stage_bytes) return CENT_OK;
+    if (ctx->h_stage) HIPTRY(hipHostFree(ctx->h_stage));
+    ctx->h_stage = nullptr;
+    ctx->h_stage_bytes = 0;
+    const size_t cap = bytes + bytes / 4 + 4096;
+    HIPTRY(hipHostMalloc(&ctx->h_stage, cap, hipHostMallocDefault));
+    ctx->h_stage_bytes = cap;
     return CENT_OK;
 }
 
@@ -195,6 +212,7 @@ struct cent_graph {
     std::vector<int32_t> comp;        // component id per vertex (host, caller ids)
     std::vector<int64_t> comp_nnz;    // directed edges per component
     std::vector<int32_t> new_id;      // device id of each caller id (empty: identity)
+    int32_t* d_new_id = nullptr;      // the same table in device memory (null: identity)
     int32_t ecc_probe = 0;            // BFS depth from the highest-degree vertex
     int32_t width_probe = 0;          // widest level of that BFS
     std::vector<int32_t> h_rp;        // device-order row offsets (host), for re-scheduling
@@ -277,5 +295,9 @@ constexpr int kMultiBatch = 8;
 
 // Batches of the k sources d_src[0..k) (device ids) inside the view: one
 // 64-lane batch for BC (k <= 64), up to kMultiBatch for closeness.
+// bound > 0: every BFS of the batch is known to end within `bound` levels
+// (levels 0 .. bound - 1 carry frontiers), so the whole batch is enqueued
+// without a host check; its depth and a bound failure are added to
+// ctx->lvstat on the device.  bound == 0: levels in host-checked runs.
 CENT_INTERNAL int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d_src, int k,
-                              int64_t* far_out, KernelTimer& tm, int64_t* levels);
+                              int64_t* far_out, KernelTimer& tm, int64_t* levels, int bound = 0);

@@ -132,15 +132,16 @@ extern "C" void cent_ctx_destroy(cent_ctx* ctx) {
     release(ctx->vis); release(ctx->F); release(ctx->hp_or); release(ctx->sigma); release(ctx->coef);
     release(ctx->bc); release(ctx->hp_acc); release(ctx->npred); release(ctx->hp_cnt);
     release(ctx->dist_sum); release(ctx->far); release(ctx->part_far); release(ctx->part_cnt);
-    release(ctx->cnt); release(ctx->src); release(ctx->flags);
+    release(ctx->cnt); release(ctx->src); release(ctx->flags); release(ctx->ord);
     release(ctx->Fd2); release(ctx->QF); release(ctx->Adev); release(ctx->Q); release(ctx->qoff);
     release(ctx->cand); release(ctx->stamp); release(ctx->ctl);
     release(ctx->red_D); release(ctx->red_head);
     release(ctx->red_part); release(ctx->red_bcp); release(ctx->it_sig); release(ctx->it_coef);
     release(ctx->it_lvl); release(ctx->red_deep);
     release(ctx->ps_rec); release(ctx->ps_Q); release(ctx->ps_loff); release(ctx->ps_bcp);
-    release(ctx->ps_depth); release(ctx->lvf);
+    release(ctx->ps_depth); release(ctx->lvf); release(ctx->lvstat);
     if (ctx->h_flags) (void)hipHostFree(ctx->h_flags);
+    if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
     for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -166,7 +167,7 @@ extern "C" int64_t cent_ctx_workspace_bytes(const cent_ctx* ctx) {
                           ctx->red_D.cap + ctx->red_part.cap + ctx->red_bcp.cap + ctx->it_sig.cap +
                           ctx->it_coef.cap + 2 * ctx->ps_rec.cap + ctx->ps_bcp.cap + ctx->ps_depth.cap) +
                      ctx->it_lvl.cap +
-                     4 * (ctx->npred.cap + ctx->hp_cnt.cap + ctx->src.cap + ctx->flags.cap + ctx->Q.cap +
+                     4 * (ctx->npred.cap + ctx->hp_cnt.cap + ctx->src.cap + ctx->flags.cap + ctx->ord.cap + ctx->Q.cap +
                           ctx->qoff.cap + ctx->cand.cap + ctx->stamp.cap + ctx->ctl.cap + ctx->red_head.cap +
                           ctx->red_deep.cap + ctx->ps_Q.cap + ctx->ps_loff.cap));
 }
@@ -178,14 +179,17 @@ extern "C" int cent_ctx_trim(cent_ctx* ctx) {
     release(ctx->vis); release(ctx->F); release(ctx->hp_or); release(ctx->sigma); release(ctx->coef);
     release(ctx->bc); release(ctx->hp_acc); release(ctx->npred); release(ctx->hp_cnt);
     release(ctx->dist_sum); release(ctx->far); release(ctx->part_far); release(ctx->part_cnt);
-    release(ctx->cnt); release(ctx->src); release(ctx->flags);
+    release(ctx->cnt); release(ctx->src); release(ctx->flags); release(ctx->ord);
     release(ctx->Fd2); release(ctx->QF); release(ctx->Adev); release(ctx->Q); release(ctx->qoff);
     release(ctx->cand); release(ctx->stamp); release(ctx->ctl);
     release(ctx->red_D); release(ctx->red_head);
     release(ctx->red_part); release(ctx->red_bcp); release(ctx->it_sig); release(ctx->it_coef);
     release(ctx->it_lvl); release(ctx->red_deep);
     release(ctx->ps_rec); release(ctx->ps_Q); release(ctx->ps_loff); release(ctx->ps_bcp);
-    release(ctx->ps_depth); release(ctx->lvf);
+    release(ctx->ps_depth); release(ctx->lvf); release(ctx->lvstat);
+    if (ctx->h_stage) HIPTRY(hipHostFree(ctx->h_stage));
+    ctx->h_stage = nullptr;
+    ctx->h_stage_bytes = 0;
     ctx->F_levels = 0;
     ctx->F_n = 0;
     return CENT_OK;
@@ -210,6 +214,7 @@ extern "C" void cent_graph_free(cent_graph* g) {
     if (!g) return;
     if (g->ctx) (void)hipSetDevice(g->ctx->device);
     devcsr_free(g->d);
+    if (g->d_new_id) (void)hipFree(g->d_new_id);
     delete g;
 }
 
@@ -297,6 +302,16 @@ extern "C" int cent_graph_create(cent_ctx* ctx, const int32_t* row_ptr, const in
         return st;
     }
     g->h_rp.assign(urp, urp + (size_t)n + 1);
+    if (!g->new_id.empty()) {  // for results written to device memory in caller order
+        hipError_t e = hipMalloc((void**)&g->d_new_id, sizeof(int32_t) * (size_t)n);
+        if (e == hipSuccess)
+            e = hipMemcpy(g->d_new_id, g->new_id.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            set_error(ctx, e, "hipMalloc/hipMemcpy(d_new_id)");
+            cent_graph_free(g);
+            return e == hipErrorOutOfMemory ? CENT_E_OOM : CENT_E_DEVICE;
+        }
+    }
     (void)cent_tune_signature(g->h_rp.data(), n, nnz, g->ecc_probe, &g->sig);
     if (!(flags & CENT_GRAPH_NO_TUNE)) {
         // closeness first, so the betweenness entry's chunk/hub layout wins
@@ -398,6 +413,49 @@ extern "C" int cent_probe_copy_bandwidth(cent_ctx* ctx, int64_t bytes, int32_t i
     (void)hipFree(a);
     (void)hipFree(b);
     return rc;
+}
+
+// ------------------------------------------------------------ device arrays
+extern "C" int cent_device_alloc(cent_ctx* ctx, int64_t bytes, void** out) {
+    if (!ctx || !out || bytes < 0) return CENT_E_INVAL;
+    *out = nullptr;
+    TRY(check_device(ctx));
+    HIPTRY(hipMalloc(out, (size_t)(bytes > 0 ? bytes : 1)));
+    return CENT_OK;
+}
+
+extern "C" int cent_device_free(cent_ctx* ctx, void* p) {
+    if (!ctx) return CENT_E_INVAL;
+    if (!p) return CENT_OK;
+    TRY(check_device(ctx));
+    HIPTRY(hipStreamSynchronize(ctx->stream));
+    HIPTRY(hipFree(p));
+    return CENT_OK;
+}
+
+extern "C" int cent_device_copy(cent_ctx* ctx, void* dst, const void* src, int64_t bytes, int32_t kind) {
+    if (!ctx || bytes < 0 || (bytes > 0 && (!dst || !src))) return CENT_E_INVAL;
+    hipMemcpyKind k;
+    switch (kind) {
+        case CENT_COPY_TO_DEVICE: k = hipMemcpyHostToDevice; break;
+        case CENT_COPY_TO_HOST: k = hipMemcpyDeviceToHost; break;
+        case CENT_COPY_ON_DEVICE: k = hipMemcpyDeviceToDevice; break;
+        default: return CENT_E_INVAL;
+    }
+    if (bytes == 0) return CENT_OK;
+    TRY(check_device(ctx));
+    HIPTRY(hipMemcpyAsync(dst, src, (size_t)bytes, k, ctx->stream));
+    HIPTRY(hipStreamSynchronize(ctx->stream));
+    return CENT_OK;
+}
+
+extern "C" int cent_device_fill(cent_ctx* ctx, void* p, int32_t byte, int64_t bytes) {
+    if (!ctx || bytes < 0 || (bytes > 0 && !p)) return CENT_E_INVAL;
+    if (bytes == 0) return CENT_OK;
+    TRY(check_device(ctx));
+    HIPTRY(hipMemsetAsync(p, byte & 0xff, (size_t)bytes, ctx->stream));
+    HIPTRY(hipStreamSynchronize(ctx->stream));
+    return CENT_OK;
 }
 
 // ------------------------------------------------------------ reference shapes

@@ -83,7 +83,7 @@ static int reduce_batch(cent_ctx* ctx, int G, int kb, int64_t* far_out, int b = 
 // round trips are shared).  far_out (device, k entries) gets the weighted
 // distance sums.  Adds the depth to *levels once per batch.
 int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d_src, int k,
-                       int64_t* far_out, KernelTimer& tm, int64_t* levels) {
+                       int64_t* far_out, KernelTimer& tm, int64_t* levels, int bound) {
     const DevCSR& d = *v.d;
     hipStream_t s = ctx->stream;
     const size_t N = (size_t)(d.n > 0 ? d.n : 1);
@@ -92,6 +92,9 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
     if (nb < 1 || nb > kMultiBatch || (sp.with_bc && nb != 1)) return CENT_E_INVAL;
     const int G = grid_for(v.nchunk, sp.opts);
     TRY(ensure_workspace(ctx, d, sp, G, nb));
+    if (bound > kHostFlags - 2) bound = 0;  // the level flags of one run must fit
+    if (bound > 0 && sp.with_bc)
+        while ((size_t)(bound + 1) > ctx->F_levels) TRY(grow_levels(ctx, N * 2, 0));
     const int kl = k - (nb - 1) * kWave;  // sources of the last batch
     const uint64_t A_last = (kl == 64) ? ~0ull : ((1ull << kl) - 1);
     const uint64_t A = nb == 1 ? A_last : ~0ull;
@@ -133,7 +136,7 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
     TRY(ensure(ctx, ctx->lvf, (size_t)kHostFlags));
     int dl = 0;  // first level of the current run; on exit, the deepest non-empty level
     for (;;) {
-        int run = std::max(2, std::min(kHostFlags - 2, ctx->depth_hint + 2 - dl));
+        int run = bound > 0 ? bound : std::max(2, std::min(kHostFlags - 2, ctx->depth_hint + 2 - dl));
         if (bc && (size_t)(dl + run + 1) > ctx->F_levels) TRY(grow_levels(ctx, PL, dl + 1));
         if (bc) run = std::min(run, (int)ctx->F_levels - 1 - dl);
         HIPTRY(hipMemsetAsync(ctx->lvf.p, 0, sizeof(int) * (size_t)(run + 1), s));
@@ -179,6 +182,10 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
             }
             TRY(timer_end(ctx, tm, e0, true));
         }
+        if (bound > 0) {  // no host check: the backward follows at once, level by level gated
+            dl = run - 1;
+            break;
+        }
         HIPTRY(hipMemcpyAsync(ctx->h_flags, ctx->lvf.p, sizeof(int) * (size_t)(run + 1), hipMemcpyDeviceToHost, s));
         HIPTRY(hipMemcpyAsync(ctx->h_flags + kHostFlags - 1, ctx->flags.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));
         HIPTRY(hipStreamSynchronize(s));
@@ -192,11 +199,14 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
         }
         dl += run;
     }
-    ctx->depth_hint = dl;
-    *levels += (int64_t)dl * nb;  // deepest non-empty level, per batch
+    if (bound == 0) {
+        ctx->depth_hint = dl;
+        *levels += (int64_t)dl * nb;  // deepest non-empty level, per batch
+    }
     if (bc) {
         for (int lv = dl; lv >= 1; --lv) {
-            BwdArgs ba;
+            BwdArgs ba{};
+            ba.live = bound > 0 ? ctx->lvf.p + lv : nullptr;
             ba.n = d.n; ba.hub_deg = d.hub_deg; ba.nchunk = v.nchunk; ba.chunk_beg = d.chunk_beg + v.c0;
             ba.rp = d.rp; ba.col = d.col;
             ba.Fd = ctx->F.p + PL * (size_t)lv;
@@ -218,8 +228,12 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
                 HIPTRY(hipGetLastError());
             }
             TRY(timer_end(ctx, tm, e0, false));
-            ctx->stats.launches_bwd++;
+            if (bound == 0) ctx->stats.launches_bwd++;
         }
+    }
+    if (bound > 0) {
+        k_level_check<<<1, 1, 0, s>>>(ctx->lvf.p, bound, nb, bc, ctx->lvstat.p);
+        HIPTRY(hipGetLastError());
     }
     for (int b = 0; b < nb; ++b)
         TRY(reduce_batch(ctx, G, b == nb - 1 ? kl : kWave, far_out + b * kWave, b));
@@ -424,10 +438,52 @@ static int run_persource(cent_ctx* ctx, const cent_graph* g, const BatchSpec& sp
     return CENT_OK;
 }
 
+// Results into caller-order device arrays: out[v] += in[new_id[v]]
+// (identity when new_id is null).
+template <typename T>
+__global__ void k_out_accum(T* __restrict__ out, const T* __restrict__ in, const int32_t* __restrict__ new_id,
+                            int n) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+        out[v] += in[new_id ? new_id[v] : v];
+}
+
+// out[order[i]] = far[i] (order null: identity).
+__global__ void k_out_far(int64_t* __restrict__ out, const int64_t* __restrict__ far,
+                          const int32_t* __restrict__ order, int k) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < k; i += gridDim.x * blockDim.x)
+        out[order ? order[i] : i] = far[i];
+}
+
+// Checks that [p, p + bytes) lies inside one device allocation of the
+// context's GPU.
+static int check_dev_range(cent_ctx* ctx, const void* p, size_t bytes) {
+    if (!p) return CENT_OK;
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();  // an unregistered host pointer: not an error of the stream
+        return CENT_E_INVAL;
+    }
+    if ((a.type != hipMemoryTypeDevice && a.type != hipMemoryTypeManaged) || a.device != ctx->device)
+        return CENT_E_INVAL;
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p) != hipSuccess) {
+        (void)hipGetLastError();
+        return CENT_E_INVAL;
+    }
+    const char* b = static_cast<const char*>(base);
+    const char* q = static_cast<const char*>(p);
+    if (q < b || (size_t)(q - b) + bytes > size) return CENT_E_INVAL;
+    return CENT_OK;
+}
+
 // Runs all batches of one plain call.  with_bc: Brandes (sigma + backward);
-// otherwise closeness only (bitmask BFS).
+// otherwise closeness only (bitmask BFS).  dev_out: bc, farness and
+// dist_sum are device arrays of the context's GPU (same meaning and order
+// as the host ones), written by the device with no copy to the host.
 static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k, bool with_bc,
-                       int bc_mode, double* bc, int64_t* farness, int64_t* dist_sum, const cent_opts* opts) {
+                       int bc_mode, double* bc, int64_t* farness, int64_t* dist_sum, const cent_opts* opts,
+                       bool dev_out = false) {
     const int n = g->n;
     const size_t N = (size_t)(n > 0 ? n : 1);
     hipStream_t s = ctx->stream;
@@ -478,18 +534,26 @@ static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
             return sa < sb;
         });
     }
-    std::vector<int32_t> dsrc;
-    const int32_t* up_src = src;
-    if ((!g->new_id.empty() || !order.empty()) && k > 0) {
-        dsrc.resize((size_t)k);
+    // pinned staging: device-order sources in, then BC, distance sums and
+    // farness out.  A call that ended early on an error may have left copies
+    // of it in flight, so the stream is drained before the host writes it.
+    const size_t stage_words = (size_t)(k > 0 ? k : 1) + (with_bc ? N : 0) + (dist_sum ? N : 0);
+    HIPTRY(hipStreamSynchronize(s));
+    TRY(ensure_stage(ctx, sizeof(int64_t) * stage_words));
+    if (k > 0) {
+        int32_t* dsrc = static_cast<int32_t*>(ctx->h_stage);
         for (int32_t i = 0; i < k; ++i) {
             const int32_t s0 = order.empty() ? src[i] : src[order[(size_t)i]];
             dsrc[(size_t)i] = g->new_id.empty() ? s0 : g->new_id[(size_t)s0];
         }
-        up_src = dsrc.data();
+        HIPTRY(hipMemcpyAsync(ctx->src.p, dsrc, sizeof(int) * (size_t)k, hipMemcpyHostToDevice, s));
+        if (dev_out && farness && !order.empty()) {  // the batch order, for the farness scatter
+            TRY(ensure(ctx, ctx->ord, (size_t)k));
+            int32_t* dord = dsrc + k;
+            std::copy(order.begin(), order.end(), dord);
+            HIPTRY(hipMemcpyAsync(ctx->ord.p, dord, sizeof(int) * (size_t)k, hipMemcpyHostToDevice, s));
+        }
     }
-    if (k > 0) HIPTRY(hipMemcpyAsync(ctx->src.p, up_src, sizeof(int) * (size_t)k, hipMemcpyHostToDevice, s));
-    HIPTRY(hipStreamSynchronize(s));  // dsrc is a pageable host buffer
 
     KernelTimer tm;
     tm.on = opts && (opts->flags & CENT_OPT_TIMING);
@@ -514,45 +578,84 @@ static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
     if (sparse) TRY(sp_prepare(ctx, g->d, sp, plan));
     // closeness on the dense step takes up to kMultiBatch batches per launch
     const int32_t step = (!with_bc && !sparse) ? kWave * kMultiBatch : kWave;
-    for (int32_t b0 = 0; batched && b0 < k; b0 += step) {
-        const int kb = (k - b0) < step ? (k - b0) : step;
-        if (sparse) TRY(level_batch_sparse(ctx, plan, sp, ctx->src.p + b0, kb, ctx->far.p + b0, tm, &levels));
-        else TRY(level_batch(ctx, view, sp, ctx->src.p + b0, kb, ctx->far.p + b0, tm, &levels));
-        if (tm.on && ctx->ev_used > 4096) {  // keep the event pool bounded
-            HIPTRY(hipStreamSynchronize(s));
-            timer_collect(ctx, tm);
+    // Level bound of a launch group on the dense step: a BFS from s ends
+    // within d(s, hub) + ecc(hub) levels (hub = the highest-degree vertex of
+    // the upload probe, ecc its BFS depth), so a group whose sources all lie
+    // in the hub's component is enqueued whole, with no host check.  0: no
+    // bound known (host-checked runs).
+    auto group_bound = [&](int32_t b0, int kb) -> int {
+        if (sparse || g->probe_dist.empty()) return 0;
+        int mx = 0;
+        for (int i = 0; i < kb; ++i) {
+            const int32_t s0 = order.empty() ? src[b0 + i] : src[order[(size_t)(b0 + i)]];
+            const int32_t d0 = g->probe_dist[(size_t)s0];
+            if (d0 < 0) return 0;
+            mx = std::max(mx, d0);
         }
-    }
-    // results back to the host
-    std::vector<double> hb;
-    std::vector<int64_t> hd;
-    unsigned long long hc[2] = {0, 0};
-    if (with_bc && bc) {
-        hb.resize(N);
-        HIPTRY(hipMemcpyAsync(hb.data(), ctx->bc.p, sizeof(double) * N, hipMemcpyDeviceToHost, s));
-    }
-    if (dist_sum) {
-        hd.resize(N);
-        HIPTRY(hipMemcpyAsync(hd.data(), ctx->dist_sum.p, sizeof(int64_t) * N, hipMemcpyDeviceToHost, s));
-    }
-    std::vector<int64_t> hf;
-    if (farness && k > 0) {
-        if (order.empty()) {
-            HIPTRY(hipMemcpyAsync(farness, ctx->far.p, sizeof(int64_t) * (size_t)k, hipMemcpyDeviceToHost, s));
-        } else {
-            hf.resize((size_t)k);
-            HIPTRY(hipMemcpyAsync(hf.data(), ctx->far.p, sizeof(int64_t) * (size_t)k, hipMemcpyDeviceToHost, s));
+        return mx + g->ecc_probe + 1;
+    };
+    TRY(ensure(ctx, ctx->lvstat, 4));
+    unsigned long long lvst[4] = {0, 0, 0, 0};
+    for (int attempt = 0; batched && attempt < 2; ++attempt) {
+        // attempt 1 only runs if a bound failed, which a correct probe rules
+        // out: everything is redone with host-checked levels
+        if (attempt == 1) {
+            if (with_bc) HIPTRY(hipMemsetAsync(ctx->bc.p, 0, sizeof(double) * N, s));
+            if (sp.dist_sum) HIPTRY(hipMemsetAsync(sp.dist_sum, 0, sizeof(int64_t) * N, s));
+            HIPTRY(hipMemsetAsync(ctx->cnt.p, 0, sizeof(unsigned long long) * 2, s));
         }
+        HIPTRY(hipMemsetAsync(ctx->lvstat.p, 0, sizeof(unsigned long long) * 4, s));
+        for (int32_t b0 = 0; b0 < k; b0 += step) {
+            const int kb = (k - b0) < step ? (k - b0) : step;
+            if (sparse) {
+                TRY(level_batch_sparse(ctx, plan, sp, ctx->src.p + b0, kb, ctx->far.p + b0, tm, &levels));
+            } else {
+                const int bound = attempt == 0 ? group_bound(b0, kb) : 0;
+                TRY(level_batch(ctx, view, sp, ctx->src.p + b0, kb, ctx->far.p + b0, tm, &levels, bound));
+            }
+            if (tm.on && ctx->ev_used > 4096) {  // keep the event pool bounded
+                HIPTRY(hipStreamSynchronize(s));
+                timer_collect(ctx, tm);
+            }
+        }
+        HIPTRY(hipMemcpyAsync(lvst, ctx->lvstat.p, sizeof(lvst), hipMemcpyDeviceToHost, s));
+        HIPTRY(hipStreamSynchronize(s));
+        if (lvst[1] == 0) break;
     }
-    HIPTRY(hipMemcpyAsync(hc, ctx->cnt.p, sizeof(hc), hipMemcpyDeviceToHost, s));
-    HIPTRY(hipStreamSynchronize(s));
-    if (!hf.empty())
-        for (int32_t i = 0; i < k; ++i) farness[order[(size_t)i]] = hf[(size_t)i];
-    const bool remap = !g->new_id.empty();
-    if (with_bc && bc)
-        for (int v = 0; v < n; ++v) bc[v] += hb[remap ? (size_t)g->new_id[(size_t)v] : (size_t)v];
-    if (dist_sum)
-        for (int v = 0; v < n; ++v) dist_sum[v] += hd[remap ? (size_t)g->new_id[(size_t)v] : (size_t)v];
+    levels += (int64_t)lvst[0];
+    ctx->stats.launches_fwd += (int64_t)lvst[2];
+    ctx->stats.launches_bwd += (int64_t)lvst[3];
+    unsigned long long* hc = reinterpret_cast<unsigned long long*>(ctx->h_flags);
+    if (dev_out) {  // results stay in device memory, in caller order
+        const int32_t* nid = g->d_new_id;
+        const int rb = 256;
+        const int rgn = (int)std::min<size_t>((N + rb - 1) / rb, 8192);
+        if (with_bc && bc) k_out_accum<double><<<rgn, rb, 0, s>>>(bc, ctx->bc.p, nid, n);
+        if (dist_sum) k_out_accum<int64_t><<<rgn, rb, 0, s>>>(dist_sum, ctx->dist_sum.p, nid, n);
+        if (farness && k > 0) {
+            const int rgk = (int)std::min<int64_t>(((int64_t)k + rb - 1) / rb, 8192);
+            k_out_far<<<rgk, rb, 0, s>>>(farness, ctx->far.p, order.empty() ? nullptr : ctx->ord.p, k);
+        }
+        HIPTRY(hipGetLastError());
+        HIPTRY(hipMemcpyAsync(hc, ctx->cnt.p, sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost, s));
+        HIPTRY(hipStreamSynchronize(s));
+    } else {  // results back to the host, through the pinned staging
+        int64_t* hf = static_cast<int64_t*>(ctx->h_stage);  // [k]
+        double* hb = reinterpret_cast<double*>(hf + (k > 0 ? k : 1));  // [N] if BC
+        int64_t* hd = reinterpret_cast<int64_t*>(hb + (with_bc ? N : 0));  // [N] if distance sums
+        if (with_bc && bc) HIPTRY(hipMemcpyAsync(hb, ctx->bc.p, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+        if (dist_sum) HIPTRY(hipMemcpyAsync(hd, ctx->dist_sum.p, sizeof(int64_t) * N, hipMemcpyDeviceToHost, s));
+        if (farness && k > 0) HIPTRY(hipMemcpyAsync(hf, ctx->far.p, sizeof(int64_t) * (size_t)k, hipMemcpyDeviceToHost, s));
+        HIPTRY(hipMemcpyAsync(hc, ctx->cnt.p, sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost, s));
+        HIPTRY(hipStreamSynchronize(s));
+        if (farness && k > 0)
+            for (int32_t i = 0; i < k; ++i) farness[order.empty() ? i : order[(size_t)i]] = hf[(size_t)i];
+        const bool remap = !g->new_id.empty();
+        if (with_bc && bc)
+            for (int v = 0; v < n; ++v) bc[v] += hb[remap ? (size_t)g->new_id[(size_t)v] : (size_t)v];
+        if (dist_sum)
+            for (int v = 0; v < n; ++v) dist_sum[v] += hd[remap ? (size_t)g->new_id[(size_t)v] : (size_t)v];
+    }
     timer_collect(ctx, tm);
     ctx->stats.levels += levels;
     ctx->stats.reached += (int64_t)hc[0] + k;
@@ -590,6 +693,41 @@ extern "C" int cent_bc(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
     TRY(check_device(ctx));
     auto t0 = std::chrono::steady_clock::now();
     int st = run_batches(ctx, g, s, kk, true, mode, bc, farness, dist_sum, opts);
+    ctx->stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return st;
+}
+
+extern "C" int cent_bc_device(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k, double* d_bc,
+                              int64_t* d_farness, int64_t* d_dist_sum, const cent_opts* opts) {
+    if (!ctx || !g || g->ctx->device != ctx->device) return CENT_E_INVAL;
+    int mode = opts ? opts->bc_mode : CENT_BC_TEXTBOOK;
+    if (mode != CENT_BC_TEXTBOOK && mode != CENT_BC_REF_COMPAT) return CENT_E_INVAL;
+    std::vector<int32_t> all;
+    const int32_t* s;
+    int32_t kk;
+    TRY(prepare_sources(g, src, k, all, &s, &kk));
+    TRY(check_device(ctx));
+    TRY(check_dev_range(ctx, d_bc, sizeof(double) * (size_t)g->n));
+    TRY(check_dev_range(ctx, d_farness, sizeof(int64_t) * (size_t)kk));
+    TRY(check_dev_range(ctx, d_dist_sum, sizeof(int64_t) * (size_t)g->n));
+    auto t0 = std::chrono::steady_clock::now();
+    int st = run_batches(ctx, g, s, kk, true, mode, d_bc, d_farness, d_dist_sum, opts, true);
+    ctx->stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return st;
+}
+
+extern "C" int cent_cc_device(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k,
+                              int64_t* d_farness, int64_t* d_dist_sum, const cent_opts* opts) {
+    if (!ctx || !g || g->ctx->device != ctx->device) return CENT_E_INVAL;
+    std::vector<int32_t> all;
+    const int32_t* s;
+    int32_t kk;
+    TRY(prepare_sources(g, src, k, all, &s, &kk));
+    TRY(check_device(ctx));
+    TRY(check_dev_range(ctx, d_farness, sizeof(int64_t) * (size_t)kk));
+    TRY(check_dev_range(ctx, d_dist_sum, sizeof(int64_t) * (size_t)g->n));
+    auto t0 = std::chrono::steady_clock::now();
+    int st = run_batches(ctx, g, s, kk, false, 0, nullptr, d_farness, d_dist_sum, opts, true);
     ctx->stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return st;
 }

@@ -122,6 +122,9 @@ struct BwdArgs {
     // source's weight wgt[bsrc[s]].  Null: unweighted.
     const int64_t* __restrict__ wgt;
     const int* __restrict__ bsrc;
+    // Levels enqueued ahead of the forward's end: a launch for an empty
+    // level (*live == 0) returns at once.  Null: always run.
+    const int* __restrict__ live;
 };
 
 // ---------------------------------------------------------------- gathers
@@ -423,6 +426,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_hubfin(FwdArgs a0, HubArgs h0) {
 // ---------------------------------------------------------------- backward
 template <bool COMPAT>
 __global__ __launch_bounds__(kBlock) void k_bwd(BwdArgs a) {
+    if (a.live && *a.live == 0) return;  // uniform over the grid
     const int lane = lane_id();
     const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const int nw = gridDim.x * kWavesPerBlock;
@@ -451,6 +455,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd(BwdArgs a) {
 }
 
 __global__ __launch_bounds__(kBlock) void k_bwd_hubseg(BwdArgs a, HubArgs h) {
+    if (a.live && *a.live == 0) return;  // uniform over the grid
     const int lane = lane_id();
     const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const int nw = gridDim.x * kWavesPerBlock;
@@ -465,6 +470,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd_hubseg(BwdArgs a, HubArgs h) {
 
 template <bool COMPAT>
 __global__ __launch_bounds__(kBlock) void k_bwd_hubfin(BwdArgs a, HubArgs h) {
+    if (a.live && *a.live == 0) return;  // uniform over the grid
     const int lane = lane_id();
     const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const int nw = gridDim.x * kWavesPerBlock;
@@ -496,6 +502,22 @@ __global__ void k_seed(const int* __restrict__ src, int k, uint64_t* __restrict_
     }
     __syncthreads();
     if (lane < k && with_sigma) sigma[(size_t)s * kWave + lane_rank(*m0)] = 1.0;
+}
+
+// End of a launch group enqueued with a level bound `bound` (levels 0 ..
+// bound - 1 run forward): lvf[i] says level i is non-empty.  Adds the depth
+// times the group's batch count to out[0], counts a bound that did not hold
+// (level `bound` reached: the group is incomplete) in out[1], and the level
+// launches that did work in out[2] (forward) and out[3] (backward).
+__global__ void k_level_check(const int* __restrict__ lvf, int bound, int nbatch, bool backward,
+                              unsigned long long* __restrict__ out) {
+    int i = 1;
+    while (i <= bound && lvf[i]) ++i;
+    const unsigned long long depth = (unsigned long long)(i - 1);
+    atomicAdd(&out[0], depth * (unsigned long long)nbatch);
+    if (i > bound) atomicAdd(&out[1], 1ull);
+    atomicAdd(&out[2], depth + 1);
+    if (backward) atomicAdd(&out[3], depth);
 }
 
 // Sums the per-block lane partials of a batch into far[0..k) and the

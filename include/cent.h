@@ -209,6 +209,30 @@ int  cent_bc(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k,
 int  cent_cc(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k,
              int64_t* farness, int64_t* dist_sum, const cent_opts* opts);
 
+/*
+ * The same two calls with the outputs in device memory of the context's GPU
+ * (same meaning, vertex order and accumulate/assign rules as above; src
+ * stays a host array).  No counterpart in the reference, whose results are
+ * host arrays: for callers that keep the scores on the GPU, so a call costs
+ * no PCIe copy of n scores.  Each output must lie inside one device
+ * allocation made by this process's HIP runtime (hipMalloc, or
+ * cent_device_alloc below) on that GPU (checked: CENT_E_INVAL otherwise).
+ * Returns once the outputs are written.
+ */
+int  cent_bc_device(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k,
+                    double* d_bc, int64_t* d_farness, int64_t* d_dist_sum, const cent_opts* opts);
+int  cent_cc_device(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k,
+                    int64_t* d_farness, int64_t* d_dist_sum, const cent_opts* opts);
+
+/* Device arrays on the context's GPU, for callers with no HIP allocator of
+ * their own (FFI and Python callers; no reference counterpart).  Copies and
+ * fills run on the context's stream and return when done. */
+enum { CENT_COPY_TO_DEVICE = 1, CENT_COPY_TO_HOST = 2, CENT_COPY_ON_DEVICE = 3 };
+int  cent_device_alloc(cent_ctx* ctx, int64_t bytes, void** out);
+int  cent_device_free(cent_ctx* ctx, void* p);
+int  cent_device_copy(cent_ctx* ctx, void* dst, const void* src, int64_t bytes, int32_t kind);
+int  cent_device_fill(cent_ctx* ctx, void* p, int32_t byte, int64_t bytes);
+
 /* ------------------------------------------------ result files */
 /* The reference's result formats.
  * BC scores (Seq_Brandes/Brandes.c:2313-2322): "start end" then one "%f"
