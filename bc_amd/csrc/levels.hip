@@ -76,6 +76,10 @@ static int reduce_batch(cent_ctx* ctx, int G, int kb, int64_t* far_out, int b = 
 }
 
 
+// Workgroups of the level-1 push launches (their lane partials go to the
+// first kPushBlocks of the forward step's block slots).
+constexpr int kPushBlocks = 256;
+
 // Batches of the k sources d_src[0..k) (device ids) whose BFS stays inside
 // the view's vertex range: one 64-lane batch for BC (k <= 64), up to
 // kMultiBatch batches of 64 consecutive sources side by side for closeness
@@ -162,6 +166,20 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
             fa.live = ctx->lvf.p + i;
             hipEvent_t e0;
             TRY(timer_begin(ctx, tm, &e0));
+            if (lv == 0) {  // level 1: push from the sources' rows
+                const int Gp = std::min(G, kPushBlocks);
+                if (bc) {
+                    k_push1_or<true><<<Gp, kBlock, 0, s>>>(fa, d_src);
+                    if (compat) k_push1_fin<true, true><<<Gp, kBlock, 0, s>>>(fa, d_src);
+                    else k_push1_fin<true, false><<<Gp, kBlock, 0, s>>>(fa, d_src);
+                } else {
+                    k_push1_or<false><<<dim3(Gp, nb), kBlock, 0, s>>>(fa, d_src);
+                    k_push1_fin<false, false><<<dim3(Gp, nb), kBlock, 0, s>>>(fa, d_src);
+                }
+                HIPTRY(hipGetLastError());
+                TRY(timer_end(ctx, tm, e0, true));
+                continue;
+            }
             if (bc) {
                 if (compat) k_fwd<true, true><<<G, kBlock, 0, s>>>(fa);
                 else k_fwd<true, false><<<G, kBlock, 0, s>>>(fa);

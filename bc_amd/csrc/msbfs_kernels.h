@@ -423,6 +423,131 @@ __global__ __launch_bounds__(kBlock) void k_fwd_hubfin(FwdArgs a0, HubArgs h0) {
     fwd_block_flush(a, r, blockIdx.x);
 }
 
+// ---------------------------------------------------------------- level 1 push
+// Level 1 of a batch is its sources' neighbourhoods.  The pull step would scan
+// every row of the graph to find them; the push step scans only the batch's
+// source rows, flattened over the grid (edge e of the concatenated rows goes
+// to thread e mod grid size), so a hub source is spread over all waves.
+//   k_push1_or   ORs lane s into the level-1 mask of every neighbour of s;
+//   k_push1_fin  then finishes each level-1 vertex once, in the thread that
+//                holds the edge from the lowest lane of its final mask:
+//                record base, vis, path count 1 and predecessor count 1 per
+//                lane (the one shortest path is the edge), distance sums and
+//                the lane partials.
+// The results are those of the pull step: at level 1 every reached lane has
+// exactly one shortest path and one predecessor.
+
+// Wave 0 writes the prefix sums of the active lanes' source degrees
+// (s_pre[0..64]) and the row starts (s_beg); returns the edge total.
+__device__ __forceinline__ int push_rows(const FwdArgs& a, const int* __restrict__ src, int* s_pre,
+                                         int* s_beg) {
+    if (threadIdx.x < kWave) {
+        const int lane = threadIdx.x;
+        int beg = 0, deg = 0;
+        if ((a.A >> lane) & 1) {
+            const int sv = src[lane];
+            beg = a.rp[sv];
+            deg = a.rp[sv + 1] - beg;
+        }
+        int x = deg;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int y = __shfl_up(x, o);
+            if (lane >= o) x += y;
+        }
+        s_pre[lane + 1] = x;
+        if (lane == 0) s_pre[0] = 0;
+        s_beg[lane] = beg;
+    }
+    __syncthreads();
+    return s_pre[kWave];
+}
+
+// The lane whose row holds flattened edge e (s_pre[lane] <= e < s_pre[lane + 1]).
+__device__ __forceinline__ int push_lane(const int* s_pre, int e) {
+    int lo = 0, hi = kWave;
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+        const int mid = (lo + hi) >> 1;
+        if (s_pre[mid] <= e) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <bool LS>
+__device__ __forceinline__ uint64_t* front_mask(uint64_t* F, int v) {
+    if constexpr (LS) return &reinterpret_cast<FRec*>(F)[v].mask;
+    else return &F[v];
+}
+
+template <bool LS>
+__global__ __launch_bounds__(kBlock) void k_push1_or(FwdArgs a0, const int* __restrict__ src0) {
+    const FwdArgs a = batch_view(a0);
+    __shared__ int s_pre[kWave + 1];
+    __shared__ int s_beg[kWave];
+    const int* src = src0 + (size_t)blockIdx.y * kWave;
+    const int E = push_rows(a, src, s_pre, s_beg);
+    for (int e = blockIdx.x * kBlock + threadIdx.x; e < E; e += gridDim.x * kBlock) {
+        const int s = push_lane(s_pre, e);
+        const int u = a.col[s_beg[s] + e - s_pre[s]];
+        if ((a.vis[u] >> s) & 1) continue;  // a self-loop of lane s's source
+        atomicOr((unsigned long long*)front_mask<LS>(a.Fn, u), 1ull << s);
+    }
+}
+
+template <bool SIGMA, bool COMPAT>
+__global__ __launch_bounds__(kBlock) void k_push1_fin(FwdArgs a0, const int* __restrict__ src0) {
+    constexpr bool LS = SIGMA;
+    const FwdArgs a = batch_view(a0);
+    __shared__ int s_pre[kWave + 1];
+    __shared__ int s_beg[kWave];
+    __shared__ unsigned long long s_far[kWave];
+    __shared__ unsigned long long s_new;
+    const int* src = src0 + (size_t)blockIdx.y * kWave;
+    if (threadIdx.x < kWave) s_far[threadIdx.x] = 0;
+    if (threadIdx.x == 0) s_new = 0;
+    const int E = push_rows(a, src, s_pre, s_beg);  // synchronises the block
+    unsigned long long nn = 0;
+    for (int e = blockIdx.x * kBlock + threadIdx.x; e < E; e += gridDim.x * kBlock) {
+        const int s = push_lane(s_pre, e);
+        const int u = a.col[s_beg[s] + e - s_pre[s]];
+        const uint64_t nm = *front_mask<LS>(a.Fn, u);
+        if (!((nm >> s) & 1) || __builtin_ctzll(nm) != s) continue;  // finished by its lowest lane
+        const uint64_t old = a.vis[u];
+        const int base = LS ? __popcll(old) : 0;
+        if constexpr (LS) reinterpret_cast<FRec*>(a.Fn)[u].base = (uint64_t)base;
+        a.vis[u] = old | nm;
+        const unsigned long long w = a.weight ? (unsigned long long)a.weight[u] : 1ull;
+        int r = 0;
+        for (uint64_t m = nm; m; m &= m - 1, ++r) {
+            const int t = __builtin_ctzll(m);
+            if constexpr (SIGMA) {
+                const size_t idx = (size_t)u * kWave + (LS ? base + r : t);
+                a.sigma[idx] = 1.0;
+                if constexpr (COMPAT) a.npred[idx] = 1u;
+            }
+            atomicAdd(&s_far[t], w);
+        }
+        const int pc = __popcll(nm);
+        if (a.dist_sum) {
+            if (a.dist_atomic) atomicAdd((unsigned long long*)&a.dist_sum[u], (unsigned long long)pc);
+            else a.dist_sum[u] += (int64_t)pc;
+        }
+        nn += (unsigned long long)pc;
+    }
+    if (nn) atomicAdd(&s_new, nn);
+    __syncthreads();
+    const int slot = blockIdx.x;
+    if (threadIdx.x < kWave && s_far[threadIdx.x])
+        a.part_far[(size_t)slot * kWave + threadIdx.x] += (int64_t)s_far[threadIdx.x];
+    if (threadIdx.x == 0 && s_new) {
+        a.part_cnt[(size_t)slot * 2] += s_new;
+        a.part_cnt[(size_t)slot * 2 + 1] += s_new;  // one shortest-path DAG edge per reached pair
+        *a.flag = 1;
+    }
+}
+
 // ---------------------------------------------------------------- backward
 template <bool COMPAT>
 __global__ __launch_bounds__(kBlock) void k_bwd(BwdArgs a) {
