@@ -34,6 +34,7 @@ namespace cent {
 constexpr int kBlock = 256;               // 4 waves per workgroup
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kUnroll = 8;                // row gathers in flight per wave
+constexpr int kUnrollHub = 8;             // the same in the hub-segment kernels
 
 struct HubArgs {
     int nseg;                       // number of hub edge segments
@@ -161,7 +162,7 @@ __device__ __forceinline__ void load_front(const uint64_t* __restrict__ F, int v
 
 // Forward scan of edges [beg, end): lane s sums sigma[u][s] (SIGMA) over the
 // neighbours u with bit s in Fd[u] & cand, or ORs the masks (closeness only).
-template <bool SIGMA, bool LS>
+template <bool SIGMA, bool LS, int U = kUnroll>
 __device__ __forceinline__ void gather_fwd(const int* __restrict__ col, const uint64_t* __restrict__ Fd,
                                            const double* __restrict__ sigma, int beg, int end,
                                            uint64_t cand, int lane, double& acc, uint32_t& cnt,
@@ -178,9 +179,9 @@ __device__ __forceinline__ void gather_fwd(const int* __restrict__ col, const ui
         if constexpr (SIGMA) {
             uint64_t ball = ballot(f != 0);
             while (ball) {
-                double x[kUnroll];
+                double x[U];
 #pragma unroll
-                for (int t = 0; t < kUnroll; ++t) {
+                for (int t = 0; t < U; ++t) {
                     x[t] = 0.0;
                     if (ball) {
                         const int c = __builtin_ctzll(ball);
@@ -195,17 +196,22 @@ __device__ __forceinline__ void gather_fwd(const int* __restrict__ col, const ui
                     }
                 }
 #pragma unroll
-                for (int t = 0; t < kUnroll; ++t) acc += x[t];
+                for (int t = 0; t < U; ++t) acc += x[t];
             }
         } else {
+            // closeness needs only which lanes reach v, not how often: stop
+            // once every candidate lane is found (in the degree-ordered
+            // device layout a row lists its highest-degree neighbours, the
+            // earliest reached, first)
             orf |= f;
+            if (j0 + kWave < end && (wave_or(orf) & cand) == cand) break;
         }
     }
 }
 
 // Backward scan of edges [beg, end): lane s sums coef[w][s] over neighbours w
 // with bit s in Fn[w] & m.
-template <bool LS>
+template <bool LS, int U = kUnroll>
 __device__ __forceinline__ double gather_bwd(const int* __restrict__ col, const uint64_t* __restrict__ Fn,
                                              const double* __restrict__ coef, int beg, int end,
                                              uint64_t m, int lane) {
@@ -220,9 +226,9 @@ __device__ __forceinline__ double gather_bwd(const int* __restrict__ col, const 
         }
         uint64_t ball = ballot((fm & m) != 0);
         while (ball) {
-            double x[kUnroll];
+            double x[U];
 #pragma unroll
-            for (int t = 0; t < kUnroll; ++t) {
+            for (int t = 0; t < U; ++t) {
                 x[t] = 0.0;
                 if (ball) {
                     const int c = __builtin_ctzll(ball);
@@ -236,7 +242,7 @@ __device__ __forceinline__ double gather_bwd(const int* __restrict__ col, const 
                 }
             }
 #pragma unroll
-            for (int t = 0; t < kUnroll; ++t) acc += x[t];
+            for (int t = 0; t < U; ++t) acc += x[t];
         }
     }
     return acc;
@@ -383,7 +389,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_hubseg(FwdArgs a0, HubArgs h0) {
         double acc = 0.0;
         uint32_t cnt = 0;
         uint64_t orf = 0;
-        gather_fwd<SIGMA, SIGMA>(a.col, a.Fd, a.sigma, h.seg_beg[i], h.seg_end[i], cand, lane, acc, cnt, orf);
+        gather_fwd<SIGMA, SIGMA, kUnrollHub>(a.col, a.Fd, a.sigma, h.seg_beg[i], h.seg_end[i], cand, lane, acc, cnt, orf);
         if constexpr (SIGMA) {
             h.part_acc[(size_t)i * kWave + lane] = acc;
             h.part_cnt[(size_t)i * kWave + lane] = cnt;
@@ -589,7 +595,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd_hubseg(BwdArgs a, HubArgs h) {
         int bm;
         load_front<true>(a.Fd, h.seg_v[i], m, bm);
         if (!m) continue;
-        h.part_acc[(size_t)i * kWave + lane] = gather_bwd<true>(a.col, a.Fn, a.coef, h.seg_beg[i], h.seg_end[i], m, lane);
+        h.part_acc[(size_t)i * kWave + lane] = gather_bwd<true, kUnrollHub>(a.col, a.Fn, a.coef, h.seg_beg[i], h.seg_end[i], m, lane);
     }
 }
 
