@@ -37,6 +37,7 @@ CENT_E_BUSY, CENT_E_CANCELLED, CENT_E_NOTFOUND, CENT_E_UNSUPPORTED = 6, 7, 8, 9
 CENT_BC_TEXTBOOK, CENT_BC_REF_COMPAT = 0, 1
 CENT_OPT_TIMING = 1
 CENT_OPT_SOURCE_ORDER = 2
+CENT_OPT_ONE_STREAM = 4
 CENT_CSR_LCC = 1
 
 _lib.cent_strerror.restype = C.c_char_p
@@ -464,10 +465,11 @@ def tune_table_save(path: str) -> None:
 
 
 def _opts(mode=CENT_BC_TEXTBOOK, timing=False, grid_waves=0, frontier="auto", max_slots=0,
-          keep_source_order=False) -> _Opts:
+          keep_source_order=False, one_stream=False) -> _Opts:
     o = _Opts()
     o.bc_mode = mode
-    o.flags = (CENT_OPT_TIMING if timing else 0) | (CENT_OPT_SOURCE_ORDER if keep_source_order else 0)
+    o.flags = ((CENT_OPT_TIMING if timing else 0) | (CENT_OPT_SOURCE_ORDER if keep_source_order else 0) |
+               (CENT_OPT_ONE_STREAM if one_stream else 0))
     o.grid_waves = grid_waves
     o.frontier = FRONTIER[frontier]
     o.max_slots = max_slots
@@ -609,13 +611,13 @@ class Context:
     def bc(self, g: DeviceGraph, sources=None, mode: int = CENT_BC_TEXTBOOK, bc=None,
            farness: bool = True, dist_sum: bool = False, timing: bool = False,
            grid_waves: int = 0, frontier: str = "auto", max_slots: int = 0,
-           keep_source_order: bool = False):
+           keep_source_order: bool = False, one_stream: bool = False):
         """Brandes BC from ``sources`` (None: all).  Returns (bc, farness, dist_sum)."""
         s, k, kk = self._src(g, sources)
         out = np.zeros(g.n, np.float64) if bc is None else bc
         far = np.zeros(max(kk, 1), np.int64) if farness else None
         ds = np.zeros(max(g.n, 1), np.int64) if dist_sum else None
-        o = _opts(mode, timing, grid_waves, frontier, max_slots, keep_source_order)
+        o = _opts(mode, timing, grid_waves, frontier, max_slots, keep_source_order, one_stream)
         _check(_lib.cent_bc(self._h, g._h, _ptr(s, C.c_int32) if s is not None and k else None, k,
                             _ptr(out, C.c_double) if g.n else None,
                             _ptr(far, C.c_int64) if far is not None else None,
@@ -625,12 +627,12 @@ class Context:
 
     def cc(self, g: DeviceGraph, sources=None, dist_sum: bool = False, timing: bool = False,
            grid_waves: int = 0, frontier: str = "auto", max_slots: int = 0,
-           keep_source_order: bool = False):
+           keep_source_order: bool = False, one_stream: bool = False):
         """Closeness (farness) from ``sources`` (None: all).  Returns (farness, dist_sum)."""
         s, k, kk = self._src(g, sources)
         far = np.zeros(max(kk, 1), np.int64)
         ds = np.zeros(max(g.n, 1), np.int64) if dist_sum else None
-        o = _opts(0, timing, grid_waves, frontier, max_slots, keep_source_order)
+        o = _opts(0, timing, grid_waves, frontier, max_slots, keep_source_order, one_stream)
         _check(_lib.cent_cc(self._h, g._h, _ptr(s, C.c_int32) if s is not None and k else None, k,
                             _ptr(far, C.c_int64),
                             _ptr(ds, C.c_int64) if ds is not None else None, C.byref(o)),
@@ -659,12 +661,13 @@ class Context:
 
     def bc_device(self, g: DeviceGraph, sources, bc, farness=None, dist_sum=None,
                   mode: int = CENT_BC_TEXTBOOK, timing: bool = False, grid_waves: int = 0,
-                  frontier: str = "auto", max_slots: int = 0, keep_source_order: bool = False) -> None:
+                  frontier: str = "auto", max_slots: int = 0, keep_source_order: bool = False,
+                  one_stream: bool = False) -> None:
         """Brandes BC with the results left in DeviceArrays of this context:
         bc (float64, n) accumulated, farness (int64, k) assigned, dist_sum
         (int64, n) accumulated; same meaning as bc()."""
         s, k, kk = self._src(g, sources)
-        o = _opts(mode, timing, grid_waves, frontier, max_slots, keep_source_order)
+        o = _opts(mode, timing, grid_waves, frontier, max_slots, keep_source_order, one_stream)
         _check(_lib.cent_bc_device(self._h, g._h, _ptr(s, C.c_int32) if s is not None and k else None, k,
                                    self._dev_arg(bc, np.float64, g.n, "bc"),
                                    self._dev_arg(farness, np.int64, kk, "farness"),
@@ -673,11 +676,11 @@ class Context:
 
     def cc_device(self, g: DeviceGraph, sources, farness, dist_sum=None, timing: bool = False,
                   grid_waves: int = 0, frontier: str = "auto", max_slots: int = 0,
-                  keep_source_order: bool = False) -> None:
+                  keep_source_order: bool = False, one_stream: bool = False) -> None:
         """Closeness with farness (int64, k) and dist_sum (int64, n) in
         DeviceArrays of this context; same meaning as cc()."""
         s, k, kk = self._src(g, sources)
-        o = _opts(0, timing, grid_waves, frontier, max_slots, keep_source_order)
+        o = _opts(0, timing, grid_waves, frontier, max_slots, keep_source_order, one_stream)
         _check(_lib.cent_cc_device(self._h, g._h, _ptr(s, C.c_int32) if s is not None and k else None, k,
                                    self._dev_arg(farness, np.int64, kk, "farness"),
                                    self._dev_arg(dist_sum, np.int64, g.n, "dist_sum"), C.byref(o)),

@@ -80,6 +80,11 @@ struct cent_ctx {
     int depth_hint = 0;      // depth of the last dense batch (length of the next run)
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
+    // second stream of the dense step: a peer context (own stream and
+    // workspace, same GPU), created on first use; ev_join[0] orders the
+    // peer after this stream's set-up, ev_join[1] this stream after the peer
+    cent_ctx* peer = nullptr;
+    hipEvent_t ev_join[2] = {nullptr, nullptr};
 };
 
 // Pinned host ints of a context (level flags of a run, status words).
@@ -277,6 +282,7 @@ static inline void timer_collect(cent_ctx* ctx, KernelTimer& tm) {
     tm.fwd.clear();
     tm.bwd.clear();
     ctx->ev_used = 0;
+    if (ctx->peer) ctx->peer->ev_used = 0;
 }
 
 // ------------------------------------------------------------ level loops (levels.hip)
@@ -299,5 +305,8 @@ constexpr int kMultiBatch = 8;
 // (levels 0 .. bound - 1 carry frontiers), so the whole batch is enqueued
 // without a host check; its depth and a bound failure are added to
 // ctx->lvstat on the device.  bound == 0: levels in host-checked runs.
-CENT_INTERNAL int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d_src, int k,
+CENT_INTERNAL // Zeroes the context's sticky path-count overflow flag (flags[1]), which
+// every level step of a call sets and the call reads at its end.
+int level_flags_reset(cent_ctx* ctx);
+int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d_src, int k,
                               int64_t* far_out, KernelTimer& tm, int64_t* levels, int bound = 0);

@@ -127,8 +127,10 @@ extern "C" int cent_ctx_create(int32_t device, cent_ctx** out) {
 
 extern "C" void cent_ctx_destroy(cent_ctx* ctx) {
     if (!ctx) return;
+    if (ctx->peer) cent_ctx_destroy(ctx->peer);
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    for (auto e : ctx->ev_join) if (e) (void)hipEventDestroy(e);
     release(ctx->vis); release(ctx->F); release(ctx->hp_or); release(ctx->sigma); release(ctx->coef);
     release(ctx->bc); release(ctx->hp_acc); release(ctx->npred); release(ctx->hp_cnt);
     release(ctx->dist_sum); release(ctx->far); release(ctx->part_far); release(ctx->part_cnt);
@@ -161,7 +163,7 @@ extern "C" int cent_ctx_reset_stats(cent_ctx* ctx) {
 
 extern "C" int64_t cent_ctx_workspace_bytes(const cent_ctx* ctx) {
     if (!ctx) return 0;
-    return (int64_t)(8 * (ctx->vis.cap + ctx->F.cap + ctx->hp_or.cap + ctx->sigma.cap + ctx->coef.cap +
+    return cent_ctx_workspace_bytes(ctx->peer) + (int64_t)(8 * (ctx->vis.cap + ctx->F.cap + ctx->hp_or.cap + ctx->sigma.cap + ctx->coef.cap +
                           ctx->bc.cap + ctx->hp_acc.cap + ctx->dist_sum.cap + ctx->far.cap + ctx->part_far.cap +
                           ctx->part_cnt.cap + ctx->cnt.cap + ctx->Fd2.cap + ctx->QF.cap + ctx->Adev.cap +
                           ctx->red_D.cap + ctx->red_part.cap + ctx->red_bcp.cap + ctx->it_sig.cap +
@@ -174,6 +176,7 @@ extern "C" int64_t cent_ctx_workspace_bytes(const cent_ctx* ctx) {
 
 extern "C" int cent_ctx_trim(cent_ctx* ctx) {
     if (!ctx) return CENT_E_INVAL;
+    if (ctx->peer) TRY(cent_ctx_trim(ctx->peer));
     TRY(check_device(ctx));
     HIPTRY(hipStreamSynchronize(ctx->stream));
     release(ctx->vis); release(ctx->F); release(ctx->hp_or); release(ctx->sigma); release(ctx->coef);

@@ -76,6 +76,12 @@ static int reduce_batch(cent_ctx* ctx, int G, int kb, int64_t* far_out, int b = 
 }
 
 
+int level_flags_reset(cent_ctx* ctx) {
+    TRY(ensure(ctx, ctx->flags, 2));
+    HIPTRY(hipMemsetAsync(ctx->flags.p, 0, sizeof(int) * 2, ctx->stream));
+    return CENT_OK;
+}
+
 // Workgroups of the level-1 push launches (their lane partials go to the
 // first kPushBlocks of the forward step's block slots).
 constexpr int kPushBlocks = 256;
@@ -115,7 +121,6 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
     }
     HIPTRY(hipMemsetAsync(ctx->part_far.p, 0, sizeof(int64_t) * (size_t)G * kWave * nb, s));
     HIPTRY(hipMemsetAsync(ctx->part_cnt.p, 0, sizeof(unsigned long long) * (size_t)G * 2 * nb, s));
-    HIPTRY(hipMemsetAsync(ctx->flags.p, 0, sizeof(int) * 2, s));
     for (int b = 0; b < nb; ++b)
         k_seed<<<1, kWave, 0, s>>>(d_src + b * kWave, b == nb - 1 ? kl : kWave, plane(0, b),
                                    ctx->vis.p + (size_t)b * N, sp.with_bc ? ctx->sigma.p : nullptr, sp.with_bc);
@@ -456,6 +461,39 @@ static int run_persource(cent_ctx* ctx, const cent_graph* g, const BatchSpec& sp
     return CENT_OK;
 }
 
+// Adds the peer stream's partial results into this stream's: BC and
+// distance sums per vertex (null: not computed), the two counters, the level
+// statistics and the overflow flag.  One fixed order per vertex (this
+// stream's sum, then the peer's), so a call's BC does not depend on timing.
+__global__ void k_join(double* __restrict__ bc, const double* __restrict__ pbc, int64_t* __restrict__ ds,
+                       const int64_t* __restrict__ pds, int n, unsigned long long* __restrict__ cnt,
+                       const unsigned long long* __restrict__ pcnt, unsigned long long* __restrict__ lvst,
+                       const unsigned long long* __restrict__ plvst, int* __restrict__ flags,
+                       const int* __restrict__ pflags) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        if (bc) bc[v] += pbc[v];
+        if (ds) ds[v] += pds[v];
+    }
+    if (blockIdx.x == 0) {
+        if (threadIdx.x < 2) cnt[threadIdx.x] += pcnt[threadIdx.x];
+        if (threadIdx.x < 4) lvst[threadIdx.x] += plvst[threadIdx.x];
+        if (threadIdx.x == 0) flags[1] |= pflags[1];
+    }
+}
+
+// The context's peer (second stream and workspace on the same GPU).
+static int peer_get(cent_ctx* ctx, cent_ctx** out) {
+    if (!ctx->peer) {
+        cent_ctx* p = nullptr;
+        TRY(cent_ctx_create(ctx->device, &p));
+        ctx->peer = p;
+    }
+    for (auto& e : ctx->ev_join)
+        if (!e) HIPTRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    *out = ctx->peer;
+    return CENT_OK;
+}
+
 // Results into caller-order device arrays: out[v] += in[new_id[v]]
 // (identity when new_id is null).
 template <typename T>
@@ -613,32 +651,89 @@ static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
         return mx + g->ecc_probe + 1;
     };
     TRY(ensure(ctx, ctx->lvstat, 4));
-    unsigned long long lvst[4] = {0, 0, 0, 0};
+    // Dense step with two or more launch groups: the groups alternate between
+    // this context's stream and its peer's, so the small levels and the
+    // tails of one group's launches overlap the other's; the peer's BC,
+    // distance sums and counters are added at the end (k_join).
+    const bool dual = batched && !sparse && !(opts->flags & CENT_OPT_ONE_STREAM) && k > step;
+    cent_ctx* peer = nullptr;
+    BatchSpec spp = sp;
+    if (dual) {
+        TRY(peer_get(ctx, &peer));
+        TRY(check_device(peer));
+        if (with_bc) {
+            TRY(ensure(peer, peer->bc, N));
+            spp.bc = peer->bc.p;
+        }
+        if (sp.dist_sum) {
+            TRY(ensure(peer, peer->dist_sum, N));
+            spp.dist_sum = peer->dist_sum.p;
+        }
+        TRY(ensure(peer, peer->cnt, 2));
+        TRY(ensure(peer, peer->lvstat, 4));
+        peer->ev_used = 0;
+    }
+    const cent_stats pst0 = peer ? peer->stats : cent_stats{};
+    unsigned long long* lvst = reinterpret_cast<unsigned long long*>(ctx->h_flags);  // [4], then the overflow flag
+    for (int i = 0; i < 4; ++i) lvst[i] = 0;
     for (int attempt = 0; batched && attempt < 2; ++attempt) {
         // attempt 1 only runs if a bound failed, which a correct probe rules
-        // out: everything is redone with host-checked levels
+        // out: everything is redone with host-checked levels on one stream
+        const bool two = dual && attempt == 0;
         if (attempt == 1) {
             if (with_bc) HIPTRY(hipMemsetAsync(ctx->bc.p, 0, sizeof(double) * N, s));
             if (sp.dist_sum) HIPTRY(hipMemsetAsync(sp.dist_sum, 0, sizeof(int64_t) * N, s));
             HIPTRY(hipMemsetAsync(ctx->cnt.p, 0, sizeof(unsigned long long) * 2, s));
         }
         HIPTRY(hipMemsetAsync(ctx->lvstat.p, 0, sizeof(unsigned long long) * 4, s));
-        for (int32_t b0 = 0; b0 < k; b0 += step) {
+        TRY(level_flags_reset(ctx));
+        if (two) {
+            hipStream_t ps = peer->stream;
+            if (with_bc) HIPTRY(hipMemsetAsync(peer->bc.p, 0, sizeof(double) * N, ps));
+            if (sp.dist_sum) HIPTRY(hipMemsetAsync(peer->dist_sum.p, 0, sizeof(int64_t) * N, ps));
+            HIPTRY(hipMemsetAsync(peer->cnt.p, 0, sizeof(unsigned long long) * 2, ps));
+            HIPTRY(hipMemsetAsync(peer->lvstat.p, 0, sizeof(unsigned long long) * 4, ps));
+            TRY(level_flags_reset(peer));
+            // the peer's groups read the sources uploaded on this stream
+            HIPTRY(hipEventRecord(ctx->ev_join[0], s));
+            HIPTRY(hipStreamWaitEvent(ps, ctx->ev_join[0], 0));
+        }
+        int grp = 0;
+        for (int32_t b0 = 0; b0 < k; b0 += step, ++grp) {
             const int kb = (k - b0) < step ? (k - b0) : step;
             if (sparse) {
                 TRY(level_batch_sparse(ctx, plan, sp, ctx->src.p + b0, kb, ctx->far.p + b0, tm, &levels));
             } else {
                 const int bound = attempt == 0 ? group_bound(b0, kb) : 0;
-                TRY(level_batch(ctx, view, sp, ctx->src.p + b0, kb, ctx->far.p + b0, tm, &levels, bound));
+                cent_ctx* x = (two && (grp & 1)) ? peer : ctx;
+                TRY(level_batch(x, view, x == ctx ? sp : spp, ctx->src.p + b0, kb, ctx->far.p + b0, tm, &levels,
+                                bound));
             }
-            if (tm.on && ctx->ev_used > 4096) {  // keep the event pool bounded
+            if (tm.on && (ctx->ev_used > 4096 || (peer && peer->ev_used > 4096))) {  // keep the event pools bounded
                 HIPTRY(hipStreamSynchronize(s));
+                if (peer) HIPTRY(hipStreamSynchronize(peer->stream));
                 timer_collect(ctx, tm);
             }
         }
-        HIPTRY(hipMemcpyAsync(lvst, ctx->lvstat.p, sizeof(lvst), hipMemcpyDeviceToHost, s));
+        if (two) {
+            HIPTRY(hipEventRecord(ctx->ev_join[1], peer->stream));
+            HIPTRY(hipStreamWaitEvent(s, ctx->ev_join[1], 0));
+            const int jg = (int)std::min<size_t>((N + 255) / 256, 4096);
+            k_join<<<jg, 256, 0, s>>>(with_bc ? ctx->bc.p : nullptr, peer->bc.p, sp.dist_sum,
+                                      sp.dist_sum ? peer->dist_sum.p : nullptr, n, ctx->cnt.p, peer->cnt.p,
+                                      ctx->lvstat.p, peer->lvstat.p, ctx->flags.p, peer->flags.p);
+            HIPTRY(hipGetLastError());
+        }
+        HIPTRY(hipMemcpyAsync(lvst, ctx->lvstat.p, sizeof(unsigned long long) * 4, hipMemcpyDeviceToHost, s));
+        HIPTRY(hipMemcpyAsync(ctx->h_flags + 8, ctx->flags.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));
         HIPTRY(hipStreamSynchronize(s));
+        if (ctx->h_flags[8]) return CENT_E_OVERFLOW;
         if (lvst[1] == 0) break;
+    }
+    if (peer) {  // the peer's launch counts belong to this call
+        ctx->stats.batches += peer->stats.batches - pst0.batches;
+        ctx->stats.launches_fwd += peer->stats.launches_fwd - pst0.launches_fwd;
+        ctx->stats.launches_bwd += peer->stats.launches_bwd - pst0.launches_bwd;
     }
     levels += (int64_t)lvst[0];
     ctx->stats.launches_fwd += (int64_t)lvst[2];

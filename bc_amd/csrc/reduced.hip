@@ -98,6 +98,7 @@ extern "C" int cent_cc_reduced(cent_ctx* ctx, const cent_reduced* rc, int64_t* f
     HIPTRY(hipMemsetAsync(Dd, 0, sizeof(int64_t) * NP, s));
     TRY(ensure(ctx, ctx->cnt, 2));
     HIPTRY(hipMemsetAsync(ctx->cnt.p, 0, sizeof(unsigned long long) * 2, s));
+    TRY(level_flags_reset(ctx));
     KernelTimer tm;
     tm.on = opts && (opts->flags & CENT_OPT_TIMING);
     ctx->ev_used = 0;
@@ -182,6 +183,7 @@ extern "C" int cent_bc_reduced(cent_ctx* ctx, const cent_reduced* rc, double* bc
     HIPTRY(hipMemsetAsync(Dd, 0, sizeof(int64_t) * NP, s));
     HIPTRY(hipMemsetAsync(ctx->red_bcp.p, 0, sizeof(double) * NP, s));
     HIPTRY(hipMemsetAsync(ctx->cnt.p, 0, sizeof(unsigned long long) * 2, s));
+    TRY(level_flags_reset(ctx));
     KernelTimer tm;
     tm.on = opts && (opts->flags & CENT_OPT_TIMING);
     ctx->ev_used = 0;

@@ -158,6 +158,9 @@ static void run_request(cent_server* s, int slot, Request* r, GraphEntry* g) {
     const int64_t k = r->all ? g->n : (int64_t)r->src.size();
     cent_opts opts{};
     opts.max_slots = kSourceSlots;  // the admission estimate counts this many
+    // slots already run side by side; a second stream per slot would also
+    // double the workspace the admission estimate counts
+    opts.flags = CENT_OPT_ONE_STREAM;
     if (!st) {
         if (dry) {
             // modelled duration: sources x directed edges at 1e9 edge visits/s

@@ -126,6 +126,12 @@ typedef struct cent_graph cent_graph;  /* device-resident CSR                  *
                                    the highest-degree vertex, then degree, so a
                                    batch's lanes share levels; results are the
                                    same up to FP64 summation order)          */
+#define CENT_OPT_ONE_STREAM 4 /* dense 64-lane step: keep every launch group
+                                   on the context's stream (default: groups
+                                   alternate between it and a second stream
+                                   of a peer workspace on the same GPU, whose
+                                   BC / distance sums are added at the end;
+                                   results equal up to FP64 summation order) */
 
 /* opts.frontier: how a level finds its work.  DENSE scans every vertex
  * (pull; best for low-diameter graphs), SPARSE keeps compacted frontier

@@ -275,6 +275,32 @@ def test_batch_grouping_keeps_caller_order(ctx, frontier):
     assert np.array_equal(c, ofar)
 
 
+@pytest.mark.parametrize("what", ["bc", "cc"])
+def test_second_stream_matches_one_stream(ctx, what):
+    """Launch groups alternating over two streams (default) against one
+    stream: farness and distance sums exact, BC equal up to the summation
+    order of the two streams' partial sums, and the two-stream result the
+    same bits on every run."""
+    g = bc_amd.gen_rmat(13, 16 << 13, seed=31)
+    dg = ctx.upload(g)
+    src = bc_amd.sample_sources(g.n, 300 if what == "bc" else 1100, 9)
+    if what == "bc":
+        a = ctx.bc(dg, src, dist_sum=True)
+        b = ctx.bc(dg, src, dist_sum=True, one_stream=True)
+        c = ctx.bc(dg, src, dist_sum=True)
+        np.testing.assert_allclose(a[0], b[0], rtol=1e-12, atol=1e-12 * max(1.0, np.abs(b[0]).max()))
+        assert np.array_equal(a[0], c[0])
+        assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+        obc, ofar, ods, _ = O.brandes(g, src)
+        assert_bc_close(a[0], obc)
+        assert np.array_equal(a[1], ofar) and np.array_equal(a[2], ods)
+    else:
+        a = ctx.cc(dg, src, dist_sum=True)
+        b = ctx.cc(dg, src, dist_sum=True, one_stream=True)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+        assert np.array_equal(a[0], O.farness(g, src))
+
+
 def test_grid2048_source_mode_properties(ctx):
     """The high-diameter workload's graph (2048 x 2048 grid, 5% of edges removed,
     1000 shortcuts) in the source-parallel step: the per-source sum identity

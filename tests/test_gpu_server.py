@@ -39,9 +39,10 @@ def test_results_same_alone_and_in_mix(ctx):
         assert st == 0
         g = graphs[j]
         if kind == "bc":
-            # alone == in the mix, under the server's launch settings (it caps
-            # source-parallel requests at SERVER_SOURCE_SLOTS slots)
-            abc, afar, _ = ctx.bc(dgs[j], src, max_slots=bc_amd.SERVER_SOURCE_SLOTS)
+            # alone == in the mix, under the server's launch settings (one
+            # stream per slot; source-parallel requests capped at
+            # SERVER_SOURCE_SLOTS slots)
+            abc, afar, _ = ctx.bc(dgs[j], src, max_slots=bc_amd.SERVER_SOURCE_SLOTS, one_stream=True)
             assert np.array_equal(bc, abc) and np.array_equal(far, afar)
             obc, ofar, _, _ = O.brandes(g, src, threads=8)
             np.testing.assert_allclose(bc, obc, rtol=1e-9, atol=1e-9 * max(1.0, np.abs(obc).max()))
