@@ -378,6 +378,11 @@ def main() -> int:
                     help="ignore the bundled tuning table (engine defaults)")
     ap.add_argument("--keep-order", action="store_true",
                     help="keep the generator's vertex ids on the device (no degree relabel)")
+    ap.add_argument("--one-stream", action="store_true",
+                    help="timed steps with every launch group on one stream")
+    ap.add_argument("--timed-only", action="store_true",
+                    help="skip the host-output and one-stream steps run after the timed ones "
+                         "(for profiler runs that should cover exactly the timed steps)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also time a plain 1 GiB device copy (HBM ceiling, counter calibration)")
     ap.add_argument("--extra", default="",
@@ -428,7 +433,7 @@ def main() -> int:
     d_far = ctx.device_array(len(src), np.int64)
     d_ds = ctx.device_array(g.n, np.int64)
     for i in range(args.warmup):
-        ctx.bc_device(dg, src, d_bc.zero(), d_far, d_ds.zero())
+        ctx.bc_device(dg, src, d_bc.zero(), d_far, d_ds.zero(), one_stream=args.one_stream)
         print(f"# warmup {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
     ctx.reset_stats()
     times = []
@@ -436,29 +441,31 @@ def main() -> int:
         d_bc.zero()
         d_ds.zero()
         t1 = time.perf_counter()
-        ctx.bc_device(dg, src, d_bc, d_far, d_ds, timing=True)
+        ctx.bc_device(dg, src, d_bc, d_far, d_ds, timing=True, one_stream=args.one_stream)
         times.append(time.perf_counter() - t1)
         print(f"# step {i + 1}/{args.steps}: {times[-1] * 1e3:.1f} ms", file=sys.stderr, flush=True)
     st = ctx.stats()
     far = d_far.numpy()
-    # the same step through the host-array call (scores copied back over PCIe)
-    bc = np.zeros(g.n)
-    t1 = time.perf_counter()
-    _, far_h, _ = ctx.bc(dg, src, bc=bc, dist_sum=True)
-    t_host = time.perf_counter() - t1
-    if not (np.array_equal(far_h, far) and np.array_equal(bc, d_bc.numpy())):
-        print("# WARNING: host-output and device-output results differ", file=sys.stderr)
-    print(f"# host-output step: {t_host * 1e3:.1f} ms", file=sys.stderr, flush=True)
-    # one more step with every launch on one stream: the level passes' own
-    # rates without the second stream sharing the GPU (not part of value)
-    ctx.reset_stats()
-    d_bc.zero()
-    d_ds.zero()
-    t1 = time.perf_counter()
-    ctx.bc_device(dg, src, d_bc, d_far, d_ds, timing=True, one_stream=True)
-    t_one = time.perf_counter() - t1
-    st1 = ctx.stats()
-    print(f"# one-stream step: {t_one * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    t_host = t_one = st1 = None
+    if not args.timed_only:
+        # the same step through the host-array call (scores copied back over PCIe)
+        bc = np.zeros(g.n)
+        t1 = time.perf_counter()
+        _, far_h, _ = ctx.bc(dg, src, bc=bc, dist_sum=True, one_stream=args.one_stream)
+        t_host = time.perf_counter() - t1
+        if not (np.array_equal(far_h, far) and np.array_equal(bc, d_bc.numpy())):
+            print("# WARNING: host-output and device-output results differ", file=sys.stderr)
+        print(f"# host-output step: {t_host * 1e3:.1f} ms", file=sys.stderr, flush=True)
+        # one more step with every launch on one stream: the level passes' own
+        # rates without the second stream sharing the GPU (not part of value)
+        ctx.reset_stats()
+        d_bc.zero()
+        d_ds.zero()
+        t1 = time.perf_counter()
+        ctx.bc_device(dg, src, d_bc, d_far, d_ds, timing=True, one_stream=True)
+        t_one = time.perf_counter() - t1
+        st1 = ctx.stats()
+        print(f"# one-stream step: {t_one * 1e3:.1f} ms", file=sys.stderr, flush=True)
     K = args.steps
     t_step = sum(times) / K
     k, n, m = len(src), g.n, g.m
@@ -484,20 +491,21 @@ def main() -> int:
         # launch groups alternate over two streams, so a pass's launches
         # share the GPU with the other stream's: the per-launch durations
         # above overlap, and the whole step's rate is the aggregate
-        "streams": 2,
+        "streams": 1 if args.one_stream else 2,
         "aggregate_GBs": round((b_fwd + b_bwd) / t_step / 1e9, 1),
         "aggregate_frac": round((b_fwd + b_bwd) / t_step / 1e9 / HBM_PEAK_GBS, 4),
     }
-    f1, w1 = st1["ms_fwd"], st1["ms_bwd"]
-    b1 = b_bwd if w1 >= f1 else b_fwd
-    t1k = max(f1, w1)
-    roofline["one_stream"] = {
-        "ms_per_step": round(t_one * 1e3, 3), "fwd_ms": round(f1, 3), "bwd_ms": round(w1, 3),
-        "kernel": "k_bwd (Brandes backward level)" if w1 >= f1 else "k_fwd (Brandes forward level)",
-        "achieved": round(b1 / (t1k * 1e-3) / 1e9, 1) if t1k else None,
-        "frac": round(b1 / (t1k * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if t1k else None,
-        "launches_per_step": (st1["launches_bwd"] if w1 >= f1 else st1["launches_fwd"]),
-    }
+    if st1 is not None:
+        f1, w1 = st1["ms_fwd"], st1["ms_bwd"]
+        b1 = b_bwd if w1 >= f1 else b_fwd
+        t1k = max(f1, w1)
+        roofline["one_stream"] = {
+            "ms_per_step": round(t_one * 1e3, 3), "fwd_ms": round(f1, 3), "bwd_ms": round(w1, 3),
+            "kernel": "k_bwd (Brandes backward level)" if w1 >= f1 else "k_fwd (Brandes forward level)",
+            "achieved": round(b1 / (t1k * 1e-3) / 1e9, 1) if t1k else None,
+            "frac": round(b1 / (t1k * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if t1k else None,
+            "launches_per_step": (st1["launches_bwd"] if w1 >= f1 else st1["launches_fwd"]),
+        }
 
     if args.copy_probe:
         roofline["copy_probe_GBs"] = round(ctx.probe_copy_bandwidth(1 << 30, 10), 1)
@@ -510,11 +518,15 @@ def main() -> int:
         key = "bwd_pass" if kname.startswith("k_bwd") else "fwd_pass"
         roofline["traffic"] = tr[key]["traffic_bytes_per_launch"]
         roofline["traffic_unit"] = "bytes per launch of the level pass"
-        # counter-measured bytes over the live pass time: the HBM-side rate
-        tgbs = tr[key]["traffic_bytes_per_launch"] * launches / (ms * 1e-3) / 1e9
+        roofline["algorithmic_bytes_per_launch"] = int(b / launches)
+        # counter-measured bytes of both passes over the live step time: the
+        # fabric-side rate of the whole step (the passes of the two streams
+        # overlap, so a per-pass rate would count shared time twice)
+        tstep = sum(tr[p]["fetch_bytes_per_step"] + tr[p]["write_bytes_per_step"] for p in ("fwd_pass", "bwd_pass"))
+        tgbs = tstep / t_step / 1e9
+        roofline["traffic_bytes_per_step"] = int(tstep)
         roofline["traffic_GBs"] = round(tgbs, 1)
         roofline["traffic_frac"] = round(tgbs / HBM_PEAK_GBS, 4)
-        roofline["algorithmic_bytes_per_launch"] = int(b / launches)
         roofline["traffic_source"] = tr["source"]
 
     cpu = None
@@ -551,8 +563,9 @@ def main() -> int:
                    "dag_edges_per_step": int(dag), "parallelism": "single GPU",
                    "launch": layout},
         "roofline": roofline, "cpu_baseline": cpu,
-        "pcie_inclusive": {"ms_per_step": round(t_host * 1e3, 3), "value": round(k * m / t_host, 1),
-                           "unit": "TEPS", "note": "host-array outputs (cent_bc), one step"},
+        "pcie_inclusive": ({"ms_per_step": round(t_host * 1e3, 3), "value": round(k * m / t_host, 1),
+                            "unit": "TEPS", "note": "host-array outputs (cent_bc), one step"}
+                           if t_host is not None else None),
     }
     if workloads:
         out["workloads"] = workloads
