@@ -323,6 +323,9 @@ _lib.cent_device_alloc.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_void_p)]
 _lib.cent_device_free.argtypes = [C.c_void_p, C.c_void_p]
 _lib.cent_device_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32]
 _lib.cent_device_fill.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int64]
+_lib.cent_ctx_workspace_bytes.argtypes = [C.c_void_p]
+_lib.cent_ctx_workspace_bytes.restype = C.c_int64
+_lib.cent_ctx_trim.argtypes = [C.c_void_p]
 _lib.cent_probe_copy_bandwidth.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.POINTER(C.c_double)]
 _lib.cent_cc_reduced.argtypes = [C.c_void_p, C.c_void_p, _I64P, C.c_int32, C.POINTER(_Opts)]
 _lib.cent_bc_reduced.argtypes = [C.c_void_p, C.c_void_p, _F64P, _I64P, C.c_int32, C.POINTER(_Opts)]
@@ -592,6 +595,14 @@ class Context:
         _check(_lib.cent_tune(self._h, g._h, _ptr(s, C.c_int32), len(s), what, max_trials, C.byref(p)),
                "cent_tune", self.last_error())
         return p
+
+    def workspace_bytes(self) -> int:
+        """Device workspace the context holds (its second-stream peer included)."""
+        return int(_lib.cent_ctx_workspace_bytes(self._h))
+
+    def trim(self) -> None:
+        """Frees the context's device workspace (it grows again on demand)."""
+        _check(_lib.cent_ctx_trim(self._h), "cent_ctx_trim", self.last_error())
 
     def stats(self) -> dict:
         s = Stats()
