@@ -4,6 +4,7 @@
 // autotuner.  Shared types are in engine.h.
 #include "engine.h"
 #include "msbfs_kernels.h"
+#include "ccw_kernels.h"
 #include "persource_kernels.h"
 #include "sparse_kernels.h"
 
@@ -18,13 +19,18 @@ static int grid_for(int chunks, const cent_opts* o) {
 
 
 // Workspace for a DevCSR with N vertices; the F store grows on demand.
+// Batches side by side in the closeness step's interleaved masks: nb
+// rounded up to a power of two.
+static inline int ccw_width(int nb) { return nb <= 1 ? 1 : nb <= 2 ? 2 : nb <= 4 ? 4 : 8; }
+
 static int ensure_workspace(cent_ctx* ctx, const DevCSR& d, const BatchSpec& sp, int G, int nb) {
     const size_t N = (size_t)(d.n > 0 ? d.n : 1);
-    TRY(ensure(ctx, ctx->vis, N * (size_t)nb));
+    const size_t W = sp.with_bc ? 1 : (size_t)ccw_width(nb);
+    TRY(ensure(ctx, ctx->vis, N * W));
     // BC keeps one {mask, base} record plane per level (2 words a vertex);
-    // closeness a level-parity pair of mask planes per batch
+    // closeness a level-parity pair of planes of W interleaved masks
     const size_t fw = sp.with_bc ? 2 : 1;
-    const size_t want = sp.with_bc ? 2 * N * 17 : N * (2 * (size_t)nb + 1);
+    const size_t want = sp.with_bc ? 2 * N * 17 : N * (2 * W + 1);
     if (ctx->F_n != N || ctx->F.cap < want) {
         TRY(ensure(ctx, ctx->F, want));
         ctx->F_n = N;
@@ -42,7 +48,7 @@ static int ensure_workspace(cent_ctx* ctx, const DevCSR& d, const BatchSpec& sp,
     if (d.nhub) {
         TRY(ensure(ctx, ctx->hp_acc, (size_t)d.nseg * kWave));
         TRY(ensure(ctx, ctx->hp_cnt, (size_t)d.nseg * kWave));
-        TRY(ensure(ctx, ctx->hp_or, (size_t)d.nseg * nb));
+        TRY(ensure(ctx, ctx->hp_or, (size_t)d.nseg * W));
     }
     return CENT_OK;
 }
@@ -86,6 +92,24 @@ int level_flags_reset(cent_ctx* ctx) {
 // first kPushBlocks of the forward step's block slots).
 constexpr int kPushBlocks = 256;
 
+// One closeness level of a launch group on the interleaved step: level 1 by
+// push (grid y = batch), later levels by the pull over non-hub rows, then
+// the hub segments and their finish.
+template <int W>
+static void ccw_level(const CcwArgs& ca, int lv, int G, int Gp, int Gseg, int Ghub, bool hubs, int nb,
+                      const int* d_src, hipStream_t s) {
+    if (lv == 0) {
+        k_ccw_push_or<W><<<dim3(Gp, nb), kBlock, 0, s>>>(ca, d_src);
+        k_ccw_push_fin<W><<<dim3(Gp, nb), kBlock, 0, s>>>(ca, d_src);
+        return;
+    }
+    k_ccw<W><<<G, kBlock, 0, s>>>(ca);
+    if (hubs) {
+        k_ccw_hubseg<W><<<Gseg, kBlock, 0, s>>>(ca);
+        k_ccw_hubfin<W><<<Ghub, kBlock, 0, s>>>(ca);
+    }
+}
+
 // Batches of the k sources d_src[0..k) (device ids) whose BFS stays inside
 // the view's vertex range: one 64-lane batch for BC (k <= 64), up to
 // kMultiBatch batches of 64 consecutive sources side by side for closeness
@@ -109,21 +133,27 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
     const uint64_t A_last = (kl == 64) ? ~0ull : ((1ull << kl) - 1);
     const uint64_t A = nb == 1 ? A_last : ~0ull;
     // F planes: BC level lv at F + lv * PL ({mask, base} records, PL = 2N
-    // words); closeness batch b's level-parity plane p at F + (p * nb + b) * N
-    const size_t fw = sp.with_bc ? 2 : 1;
+    // words); closeness level parity p at F + p * PL, W interleaved masks a
+    // vertex (PL = W N words)
+    const int W = sp.with_bc ? 1 : ccw_width(nb);
+    const size_t fw = sp.with_bc ? 2 : (size_t)W;
     const size_t PL = N * fw;
-    const size_t planes = sp.with_bc ? 1 : (size_t)nb;
-    auto plane = [&](size_t p, size_t b) { return ctx->F.p + (p * planes + b) * PL; };
-    for (int b = 0; b < nb; ++b) {
-        HIPTRY(hipMemsetAsync(ctx->vis.p + (size_t)b * N + v.lo, 0, sizeof(uint64_t) * R, s));
-        HIPTRY(hipMemsetAsync(plane(0, b) + fw * v.lo, 0, sizeof(uint64_t) * fw * R, s));
-        HIPTRY(hipMemsetAsync(plane(1, b) + fw * v.lo, 0, sizeof(uint64_t) * fw * R, s));
-    }
+    auto plane = [&](size_t p) { return ctx->F.p + p * PL; };
+    HIPTRY(hipMemsetAsync(ctx->vis.p + (size_t)W * v.lo, 0, sizeof(uint64_t) * W * R, s));
+    HIPTRY(hipMemsetAsync(plane(0) + fw * v.lo, 0, sizeof(uint64_t) * fw * R, s));
+    HIPTRY(hipMemsetAsync(plane(1) + fw * v.lo, 0, sizeof(uint64_t) * fw * R, s));
     HIPTRY(hipMemsetAsync(ctx->part_far.p, 0, sizeof(int64_t) * (size_t)G * kWave * nb, s));
     HIPTRY(hipMemsetAsync(ctx->part_cnt.p, 0, sizeof(unsigned long long) * (size_t)G * 2 * nb, s));
-    for (int b = 0; b < nb; ++b)
-        k_seed<<<1, kWave, 0, s>>>(d_src + b * kWave, b == nb - 1 ? kl : kWave, plane(0, b),
-                                   ctx->vis.p + (size_t)b * N, sp.with_bc ? ctx->sigma.p : nullptr, sp.with_bc);
+    if (sp.with_bc) {
+        k_seed<<<1, kWave, 0, s>>>(d_src, kl, plane(0), ctx->vis.p, ctx->sigma.p, true);
+    } else {
+        switch (W) {
+            case 1: k_ccw_seed<1><<<nb, kWave, 0, s>>>(d_src, nb, kl, plane(0), ctx->vis.p); break;
+            case 2: k_ccw_seed<2><<<nb, kWave, 0, s>>>(d_src, nb, kl, plane(0), ctx->vis.p); break;
+            case 4: k_ccw_seed<4><<<nb, kWave, 0, s>>>(d_src, nb, kl, plane(0), ctx->vis.p); break;
+            default: k_ccw_seed<8><<<nb, kWave, 0, s>>>(d_src, nb, kl, plane(0), ctx->vis.p); break;
+        }
+    }
     HIPTRY(hipGetLastError());
 
     HubArgs ha{};
@@ -152,11 +182,33 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
         HIPTRY(hipMemsetD32Async((hipDeviceptr_t)ctx->lvf.p, 1, 1, s));
         for (int i = 0; i < run; ++i) {
             const int lv = dl + i;
-            const uint64_t* Fd = bc ? ctx->F.p + PL * (size_t)lv : plane(lv & 1, 0);
-            uint64_t* Fn = bc ? ctx->F.p + PL * (size_t)(lv + 1) : plane((lv + 1) & 1, 0);
-            if (!bc || lv + 1 > 1)
-                for (int b = 0; b < nb; ++b)
-                    HIPTRY(hipMemsetAsync(Fn + (size_t)b * PL + fw * v.lo, 0, sizeof(uint64_t) * fw * R, s));
+            const uint64_t* Fd = bc ? ctx->F.p + PL * (size_t)lv : plane(lv & 1);
+            uint64_t* Fn = bc ? ctx->F.p + PL * (size_t)(lv + 1) : plane((lv + 1) & 1);
+            if (lv + 1 > 1) HIPTRY(hipMemsetAsync(Fn + fw * v.lo, 0, sizeof(uint64_t) * fw * R, s));
+            if (!bc) {  // closeness: the interleaved step over the group's batches
+                CcwArgs ca{};
+                ca.hub_deg = d.hub_deg; ca.nchunk = v.nchunk; ca.chunk_beg = d.chunk_beg + v.c0;
+                ca.rp = d.rp; ca.col = d.col; ca.vis = ctx->vis.p; ca.Fd = Fd; ca.Fn = Fn;
+                ca.dist_sum = sp.dist_sum; ca.weight = sp.weight;
+                ca.nb = nb; ca.A_last = A_last; ca.dnext = lv + 1; ca.pslots = G;
+                ca.part_far = ctx->part_far.p; ca.part_cnt = ctx->part_cnt.p;
+                ca.flag = ctx->lvf.p + i + 1; ca.live = ctx->lvf.p + i;
+                ca.nseg = ha.nseg; ca.seg_v = ha.seg_v; ca.seg_beg = ha.seg_beg; ca.seg_end = ha.seg_end;
+                ca.seg_base = ha.seg_base; ca.nhub = ha.nhub; ca.hub_v = ha.hub_v; ca.hub_seg_off = ha.hub_seg_off;
+                ca.part_or = ctx->hp_or.p;
+                hipEvent_t e0;
+                TRY(timer_begin(ctx, tm, &e0));
+                const int Gp = std::min(G, kPushBlocks);
+                switch (W) {
+                    case 1: ccw_level<1>(ca, lv, G, Gp, Gseg, Ghub, v.nhub > 0, nb, d_src, s); break;
+                    case 2: ccw_level<2>(ca, lv, G, Gp, Gseg, Ghub, v.nhub > 0, nb, d_src, s); break;
+                    case 4: ccw_level<4>(ca, lv, G, Gp, Gseg, Ghub, v.nhub > 0, nb, d_src, s); break;
+                    default: ccw_level<8>(ca, lv, G, Gp, Gseg, Ghub, v.nhub > 0, nb, d_src, s); break;
+                }
+                HIPTRY(hipGetLastError());
+                TRY(timer_end(ctx, tm, e0, true));
+                continue;
+            }
             FwdArgs fa{};
             fa.bstride = N; fa.pslots = G; fa.A_last = A_last; fa.dist_atomic = nb > 1;
             fa.n = d.n; fa.hub_deg = d.hub_deg; fa.nchunk = v.nchunk; fa.chunk_beg = d.chunk_beg + v.c0;
@@ -173,34 +225,20 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
             TRY(timer_begin(ctx, tm, &e0));
             if (lv == 0) {  // level 1: push from the sources' rows
                 const int Gp = std::min(G, kPushBlocks);
-                if (bc) {
-                    k_push1_or<true><<<Gp, kBlock, 0, s>>>(fa, d_src);
-                    if (compat) k_push1_fin<true, true><<<Gp, kBlock, 0, s>>>(fa, d_src);
-                    else k_push1_fin<true, false><<<Gp, kBlock, 0, s>>>(fa, d_src);
-                } else {
-                    k_push1_or<false><<<dim3(Gp, nb), kBlock, 0, s>>>(fa, d_src);
-                    k_push1_fin<false, false><<<dim3(Gp, nb), kBlock, 0, s>>>(fa, d_src);
-                }
+                k_push1_or<true><<<Gp, kBlock, 0, s>>>(fa, d_src);
+                if (compat) k_push1_fin<true, true><<<Gp, kBlock, 0, s>>>(fa, d_src);
+                else k_push1_fin<true, false><<<Gp, kBlock, 0, s>>>(fa, d_src);
                 HIPTRY(hipGetLastError());
                 TRY(timer_end(ctx, tm, e0, true));
                 continue;
             }
-            if (bc) {
-                if (compat) k_fwd<true, true><<<G, kBlock, 0, s>>>(fa);
-                else k_fwd<true, false><<<G, kBlock, 0, s>>>(fa);
-            } else {
-                k_fwd<false, false><<<dim3(G, nb), kBlock, 0, s>>>(fa);
-            }
+            if (compat) k_fwd<true, true><<<G, kBlock, 0, s>>>(fa);
+            else k_fwd<true, false><<<G, kBlock, 0, s>>>(fa);
             HIPTRY(hipGetLastError());
             if (v.nhub) {
-                if (bc) {
-                    k_fwd_hubseg<true><<<Gseg, kBlock, 0, s>>>(fa, ha);
-                    if (compat) k_fwd_hubfin<true, true><<<Ghub, kBlock, 0, s>>>(fa, ha);
-                    else k_fwd_hubfin<true, false><<<Ghub, kBlock, 0, s>>>(fa, ha);
-                } else {
-                    k_fwd_hubseg<false><<<dim3(Gseg, nb), kBlock, 0, s>>>(fa, ha);
-                    k_fwd_hubfin<false, false><<<dim3(Ghub, nb), kBlock, 0, s>>>(fa, ha);
-                }
+                k_fwd_hubseg<true><<<Gseg, kBlock, 0, s>>>(fa, ha);
+                if (compat) k_fwd_hubfin<true, true><<<Ghub, kBlock, 0, s>>>(fa, ha);
+                else k_fwd_hubfin<true, false><<<Ghub, kBlock, 0, s>>>(fa, ha);
                 HIPTRY(hipGetLastError());
             }
             TRY(timer_end(ctx, tm, e0, true));

@@ -445,15 +445,15 @@ __global__ __launch_bounds__(kBlock) void k_fwd_hubfin(FwdArgs a0, HubArgs h0) {
 
 // Wave 0 writes the prefix sums of the active lanes' source degrees
 // (s_pre[0..64]) and the row starts (s_beg); returns the edge total.
-__device__ __forceinline__ int push_rows(const FwdArgs& a, const int* __restrict__ src, int* s_pre,
-                                         int* s_beg) {
+__device__ __forceinline__ int push_rows(uint64_t A, const int* __restrict__ rp, const int* __restrict__ src,
+                                         int* s_pre, int* s_beg) {
     if (threadIdx.x < kWave) {
         const int lane = threadIdx.x;
         int beg = 0, deg = 0;
-        if ((a.A >> lane) & 1) {
+        if ((A >> lane) & 1) {
             const int sv = src[lane];
-            beg = a.rp[sv];
-            deg = a.rp[sv + 1] - beg;
+            beg = rp[sv];
+            deg = rp[sv + 1] - beg;
         }
         int x = deg;
 #pragma unroll
@@ -493,7 +493,7 @@ __global__ __launch_bounds__(kBlock) void k_push1_or(FwdArgs a0, const int* __re
     __shared__ int s_pre[kWave + 1];
     __shared__ int s_beg[kWave];
     const int* src = src0 + (size_t)blockIdx.y * kWave;
-    const int E = push_rows(a, src, s_pre, s_beg);
+    const int E = push_rows(a.A, a.rp, src, s_pre, s_beg);
     for (int e = blockIdx.x * kBlock + threadIdx.x; e < E; e += gridDim.x * kBlock) {
         const int s = push_lane(s_pre, e);
         const int u = a.col[s_beg[s] + e - s_pre[s]];
@@ -513,7 +513,7 @@ __global__ __launch_bounds__(kBlock) void k_push1_fin(FwdArgs a0, const int* __r
     const int* src = src0 + (size_t)blockIdx.y * kWave;
     if (threadIdx.x < kWave) s_far[threadIdx.x] = 0;
     if (threadIdx.x == 0) s_new = 0;
-    const int E = push_rows(a, src, s_pre, s_beg);  // synchronises the block
+    const int E = push_rows(a.A, a.rp, src, s_pre, s_beg);  // synchronises the block
     unsigned long long nn = 0;
     for (int e = blockIdx.x * kBlock + threadIdx.x; e < E; e += gridDim.x * kBlock) {
         const int s = push_lane(s_pre, e);
