@@ -305,7 +305,10 @@ constexpr int kMultiBatch = 8;
 // (levels 0 .. bound - 1 carry frontiers), so the whole batch is enqueued
 // without a host check; its depth and a bound failure are added to
 // ctx->lvstat on the device.  bound == 0: levels in host-checked runs.
-CENT_INTERNAL // Zeroes the context's sticky path-count overflow flag (flags[1]), which
+CENT_INTERNAL // The context's peer (second stream and workspace on the same GPU), created
+// on first use, with the two join events.
+int peer_get(cent_ctx* ctx, cent_ctx** out);
+// Zeroes the context's sticky path-count overflow flag (flags[1]), which
 // every level step of a call sets and the call reads at its end.
 int level_flags_reset(cent_ctx* ctx);
 int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d_src, int k,

@@ -519,8 +519,7 @@ __global__ void k_join(double* __restrict__ bc, const double* __restrict__ pbc, 
     }
 }
 
-// The context's peer (second stream and workspace on the same GPU).
-static int peer_get(cent_ctx* ctx, cent_ctx** out) {
+int peer_get(cent_ctx* ctx, cent_ctx** out) {
     if (!ctx->peer) {
         cent_ctx* p = nullptr;
         TRY(cent_ctx_create(ctx->device, &p));

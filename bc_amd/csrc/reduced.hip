@@ -1,6 +1,8 @@
 // The D1/AP-reduced form: device copies of packed block graphs, the
 // persistent work-queue kernels over (block, source block) items and the
 // level-synchronous path for large items.  Shared types are in engine.h.
+#include <map>
+
 #include "engine.h"
 #include "items_kernels.h"
 
@@ -19,6 +21,11 @@ struct RedDev {
     int64_t* poff = nullptr;  // [nitems] offsets of the items' BC partials (small items)
     int64_t part_total = 0;
     std::vector<int64_t> h_poff;
+    // level bounds of the big items' blocks: BFS distance of every packed
+    // vertex of a big block from the block's highest-degree vertex (-1
+    // elsewhere), and that BFS's depth per block (keyed by the block's lo)
+    std::vector<int32_t> probe_d;
+    std::map<int, int> probe_ecc;
 };
 
 static void reddev_free(void* p) {
@@ -76,7 +83,108 @@ static int reddev_get(cent_ctx* ctx, cent_reduced* r, RedDev** out) {
         rd->part_total += r->items[i].hi - r->items[i].lo;
     }
     TRY(upload(ctx, &rd->poff, rd->h_poff.data(), rd->h_poff.size()));
+    // probe BFS per big block (host, linear in the block)
+    rd->probe_d.assign(NP, -1);
+    std::vector<int32_t> q;
+    for (int i : rd->big) {
+        const int lo = r->items[i].lo, hi = r->items[i].hi;
+        if (rd->probe_ecc.count(lo)) continue;
+        int h = lo;
+        for (int x = lo; x < hi; ++x)
+            if (r->prp[x + 1] - r->prp[x] > r->prp[h + 1] - r->prp[h]) h = x;
+        q.assign(1, h);
+        rd->probe_d[(size_t)h] = 0;
+        int depth = 0;
+        for (size_t qh = 0; qh < q.size(); ++qh) {
+            const int x = q[qh];
+            depth = rd->probe_d[(size_t)x];
+            for (int64_t j = r->prp[x]; j < r->prp[x + 1]; ++j) {
+                const int y = r->pcol[j];
+                if (rd->probe_d[(size_t)y] < 0) {
+                    rd->probe_d[(size_t)y] = depth + 1;
+                    q.push_back(y);
+                }
+            }
+        }
+        rd->probe_ecc[lo] = depth;
+    }
     *out = rd;
+    return CENT_OK;
+}
+
+// Level bound of a group of big-item sources [s0, s0 + ns) in the block
+// starting at lo: d(s, probe) + ecc(probe) + 1 over the group (0: none known).
+static int red_bound(const RedDev* rd, int lo, int s0, int ns) {
+    auto it = rd->probe_ecc.find(lo);
+    if (it == rd->probe_ecc.end()) return 0;
+    int mx = 0;
+    for (int i = s0; i < s0 + ns; ++i) {
+        const int d0 = rd->probe_d[(size_t)i];
+        if (d0 < 0) return 0;
+        mx = std::max(mx, d0);
+    }
+    return mx + it->second + 1;
+}
+
+// The big items' launch groups (lo, hi, s0, ns): each enqueued whole under
+// its block's level bound, the groups alternating between this context's
+// stream and its peer's (one stream with CENT_OPT_ONE_STREAM).  Every group
+// writes its own slice D[s0 .. s0 + ns), so the streams share no output.
+// If a bound ever failed (a correct probe rules it out), all groups are
+// redone with host-checked levels on one stream.
+static int run_big_groups(cent_ctx* ctx, const RedDev* rd, const BatchSpec& sp, const std::vector<int4>& groups,
+                          int64_t* D, const cent_opts* opts, KernelTimer& tm, int64_t* levels) {
+    if (groups.empty()) return CENT_OK;
+    hipStream_t s = ctx->stream;
+    const bool dual = groups.size() >= 2 && !(opts && (opts->flags & CENT_OPT_ONE_STREAM));
+    cent_ctx* peer = nullptr;
+    if (dual) {
+        TRY(peer_get(ctx, &peer));
+        TRY(check_device(peer));
+        TRY(ensure(peer, peer->lvstat, 4));
+        TRY(ensure(peer, peer->cnt, 2));
+        peer->ev_used = 0;
+    }
+    const cent_stats pst0 = peer ? peer->stats : cent_stats{};
+    TRY(ensure(ctx, ctx->lvstat, 4));
+    unsigned long long* lv = reinterpret_cast<unsigned long long*>(ctx->h_flags);  // [0..3] ctx, [4..7] peer
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        const bool two = dual && attempt == 0;
+        HIPTRY(hipMemsetAsync(ctx->lvstat.p, 0, sizeof(unsigned long long) * 4, s));
+        if (two) {
+            HIPTRY(hipMemsetAsync(peer->lvstat.p, 0, sizeof(unsigned long long) * 4, peer->stream));
+            HIPTRY(hipMemsetAsync(peer->cnt.p, 0, sizeof(unsigned long long) * 2, peer->stream));
+            TRY(level_flags_reset(peer));
+            // D was cleared on this stream
+            HIPTRY(hipEventRecord(ctx->ev_join[0], s));
+            HIPTRY(hipStreamWaitEvent(peer->stream, ctx->ev_join[0], 0));
+        }
+        for (size_t gi = 0; gi < groups.size(); ++gi) {
+            const int4 g = groups[gi];
+            cent_ctx* x = (two && (gi & 1)) ? peer : ctx;
+            const int bound = attempt == 0 ? red_bound(rd, g.x, g.z, g.w) : 0;
+            const GView v = sub_view(rd->d, g.x, g.y);
+            TRY(level_batch(x, v, sp, rd->iota + g.z, g.w, D + g.z, tm, levels, bound));
+            if (tm.on && (ctx->ev_used > 4096 || (peer && peer->ev_used > 4096))) {
+                HIPTRY(hipStreamSynchronize(s));
+                if (peer) HIPTRY(hipStreamSynchronize(peer->stream));
+                timer_collect(ctx, tm);
+            }
+        }
+        for (int i = 0; i < 8; ++i) lv[i] = 0;
+        if (two) {
+            HIPTRY(hipEventRecord(ctx->ev_join[1], peer->stream));
+            HIPTRY(hipStreamWaitEvent(s, ctx->ev_join[1], 0));
+            HIPTRY(hipMemcpyAsync(lv + 4, peer->lvstat.p, sizeof(unsigned long long) * 4, hipMemcpyDeviceToHost, s));
+        }
+        HIPTRY(hipMemcpyAsync(lv, ctx->lvstat.p, sizeof(unsigned long long) * 4, hipMemcpyDeviceToHost, s));
+        HIPTRY(hipStreamSynchronize(s));
+        *levels += (int64_t)(lv[0] + lv[4]);
+        ctx->stats.launches_fwd += (int64_t)(lv[2] + lv[6]);
+        ctx->stats.launches_bwd += (int64_t)(lv[3] + lv[7]);
+        if (lv[1] + lv[5] == 0) break;
+    }
+    if (peer) ctx->stats.batches += peer->stats.batches - pst0.batches;
     return CENT_OK;
 }
 
@@ -130,6 +238,7 @@ extern "C" int cent_cc_reduced(cent_ctx* ctx, const cent_reduced* rc, int64_t* f
         }
         // large items: consecutive full items of one block (contiguous
         // sources) go up to kMultiBatch at a time through one level loop
+        std::vector<int4> groups;  // (lo, hi, s0, ns)
         for (size_t j = 0; j < rd->big.size();) {
             const cent_item& it = r->items[rd->big[j]];
             int ns = it.ns;
@@ -141,10 +250,10 @@ extern "C" int cent_cc_reduced(cent_ctx* ctx, const cent_reduced* rc, int64_t* f
                 ns += nx.ns;
                 ++e;
             }
-            const GView v = sub_view(rd->d, it.lo, it.hi);
-            TRY(level_batch(ctx, v, sp, rd->iota + it.s0, ns, Dd + it.s0, tm, &levels));
+            groups.push_back(make_int4(it.lo, it.hi, it.s0, ns));
             j = e;
         }
+        TRY(run_big_groups(ctx, rd, sp, groups, Dd, opts, tm, &levels));
     }
     std::vector<int64_t> D(NP);
     HIPTRY(hipMemcpyAsync(D.data(), Dd, sizeof(int64_t) * NP, hipMemcpyDeviceToHost, s));

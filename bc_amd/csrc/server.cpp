@@ -173,7 +173,7 @@ static void run_request(cent_server* s, int slot, Request* r, GraphEntry* g) {
                          r->far.data(), nullptr, &opts);
         } else if (r->all) {
             r->far.assign((size_t)std::max<int64_t>(g->n, 1), 0);
-            st = cent_cc_reduced(ctx, g->red, r->far.data(), CENT_SCHED_PERSISTENT, nullptr);
+            st = cent_cc_reduced(ctx, g->red, r->far.data(), CENT_SCHED_PERSISTENT, &opts);
         } else {
             r->far.assign((size_t)std::max<int64_t>(k, 1), 0);
             st = cent_cc(ctx, g->dev, r->src.data(), (int32_t)k, r->far.data(), nullptr, &opts);
