@@ -588,8 +588,9 @@ class Context:
         return DeviceGraph(self, csr, keep_order, tuned)
 
     def tune(self, g: DeviceGraph, sources, what: int = CENT_TUNE_BC, max_trials: int = 0) -> TuneParams:
-        """Autotune g's launch settings on up to 256 sample sources; applies and
-        records the fastest (process-wide table) and returns it."""
+        """Autotune g's launch settings on the sample sources (at most 256 for
+        betweenness, 1024 for closeness); applies and records the fastest
+        (process-wide table) and returns it."""
         s = np.ascontiguousarray(sources, dtype=np.int32)
         p = TuneParams()
         _check(_lib.cent_tune(self._h, g._h, _ptr(s, C.c_int32), len(s), what, max_trials, C.byref(p)),

@@ -942,7 +942,7 @@ extern "C" int cent_tune(cent_ctx* ctx, cent_graph* g, const int32_t* src, int32
                          int32_t max_trials, cent_tune_params* best) {
     if (!ctx || !g || !best || g->ctx->device != ctx->device) return CENT_E_INVAL;
     if (what != CENT_TUNE_BC && what != CENT_TUNE_CC) return CENT_E_INVAL;
-    if (k < 1 || k > 256 || !src || max_trials < 0) return CENT_E_INVAL;
+    if (k < 1 || k > (what == CENT_TUNE_CC ? 1024 : 256) || !src || max_trials < 0) return CENT_E_INVAL;
     for (int32_t i = 0; i < k; ++i)
         if (src[i] < 0 || src[i] >= g->n) return CENT_E_INVAL;
     const cent_stats saved = ctx->stats;

@@ -31,7 +31,8 @@ def workloads(quick: bool):
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--out", required=True)
-    ap.add_argument("--sources", type=int, default=256)
+    ap.add_argument("--sources", type=int, default=256, help="betweenness sample size (at most 256)")
+    ap.add_argument("--cc-sources", type=int, default=1024, help="closeness sample size (at most 1024)")
     ap.add_argument("--quick", action="store_true", help="skip the 4M-vertex grid")
     args = ap.parse_args()
     ctx = bc_amd.Context(0)
@@ -39,9 +40,10 @@ def main() -> int:
     for name, make in workloads(args.quick):
         g = make()
         dg = ctx.upload(g, tuned=False)
-        src = bc_amd.sample_sources(g.n, min(args.sources, g.n), 11)
         # closeness first: the graph keeps the betweenness chunk/hub layout
         for what, label in ((bc_amd.CENT_TUNE_CC, "cc"), (bc_amd.CENT_TUNE_BC, "bc")):
+            ks = args.cc_sources if what == bc_amd.CENT_TUNE_CC else args.sources
+            src = bc_amd.sample_sources(g.n, min(ks, g.n), 11)
             t0 = time.perf_counter()
             best = ctx.tune(dg, src, what)
             print(f"{name} {label}: n={g.n} nnz={g.nnz} sig={dg.signature} best={best.as_dict()} "

@@ -292,9 +292,10 @@ int  cent_graph_get_layout(const cent_graph* g, int32_t what, cent_tune_params* 
 int  cent_graph_set_layout(cent_graph* g, int32_t what, const cent_tune_params* p);
 int  cent_graph_signature(const cent_graph* g, cent_tune_sig* out);
 
-/* Times the plain call `what` on the sample sources src[0..k) (k <= 256;
- * use 256 where the graph allows, so the level steps are compared at full
- * width) for candidate settings by coordinate descent (level step, then
+/* Times the plain call `what` on the sample sources src[0..k) (k <= 256
+ * for betweenness, k <= 1024 for closeness; use the maximum where the graph
+ * allows, so the level steps are compared at full width: four 64-lane
+ * batches, or two launch groups of eight for closeness) for candidate settings by coordinate descent (level step, then
  * hub_deg, chunk_edges and grid_waves for the dense step or the slot count
  * for the source-parallel step; at most max_trials timed runs, 0 = no
  * limit), applies the fastest to g for `what`, records it in the

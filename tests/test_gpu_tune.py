@@ -96,3 +96,19 @@ def test_tune_rejects_bad_arguments(ctx, rmat):
     with pytest.raises(bc_amd.CentError):
         dg.set_layout(bc_amd.tune_params(hub_deg=16))
     assert dg.layout.hub_deg == 256
+
+
+def test_tune_sample_limits(ctx, rmat):
+    """Betweenness tunes on at most 256 sample sources, closeness on at most
+    1024 (two launch groups of eight 64-lane batches)."""
+    g = rmat[0]
+    dg = ctx.upload(g, tuned=False)
+    src = bc_amd.sample_sources(g.n, 1025, 3)
+    with pytest.raises(bc_amd.CentError):
+        ctx.tune(dg, src[:257], bc_amd.CENT_TUNE_BC, max_trials=1)
+    with pytest.raises(bc_amd.CentError):
+        ctx.tune(dg, src[:1025], bc_amd.CENT_TUNE_CC, max_trials=1)
+    best = ctx.tune(dg, src[:1024], bc_amd.CENT_TUNE_CC, max_trials=2)
+    assert best.ms > 0
+    far, _ = ctx.cc(dg, src[:1024])
+    assert np.array_equal(far, O.farness(g, src[:1024]))
