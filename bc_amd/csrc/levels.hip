@@ -162,7 +162,7 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
         ha.nseg = v.nseg; ha.seg_v = d.seg_v + v.s0; ha.seg_beg = d.seg_beg + v.s0; ha.seg_end = d.seg_end + v.s0;
         ha.seg_base = v.s0;
         ha.nhub = v.nhub; ha.hub_v = d.hub_v + v.h0; ha.hub_seg_off = d.hub_seg_off + v.h0;
-        ha.part_acc = ctx->hp_acc.p; ha.part_cnt = ctx->hp_cnt.p; ha.part_or = ctx->hp_or.p;
+        ha.part_acc = ctx->hp_acc.p; ha.part_cnt = ctx->hp_cnt.p;
         Gseg = std::min((v.nseg + kWavesPerBlock - 1) / kWavesPerBlock, 4096);
         Ghub = std::min((v.nhub + kWavesPerBlock - 1) / kWavesPerBlock, G);  // hubfin flushes into G slots
     }
@@ -210,7 +210,6 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
                 continue;
             }
             FwdArgs fa{};
-            fa.bstride = N; fa.pslots = G; fa.A_last = A_last; fa.dist_atomic = nb > 1;
             fa.n = d.n; fa.hub_deg = d.hub_deg; fa.nchunk = v.nchunk; fa.chunk_beg = d.chunk_beg + v.c0;
             fa.rp = d.rp; fa.col = d.col; fa.vis = ctx->vis.p; fa.Fd = Fd; fa.Fn = Fn;
             fa.sigma = bc ? ctx->sigma.p : nullptr;
@@ -225,20 +224,20 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
             TRY(timer_begin(ctx, tm, &e0));
             if (lv == 0) {  // level 1: push from the sources' rows
                 const int Gp = std::min(G, kPushBlocks);
-                k_push1_or<true><<<Gp, kBlock, 0, s>>>(fa, d_src);
-                if (compat) k_push1_fin<true, true><<<Gp, kBlock, 0, s>>>(fa, d_src);
-                else k_push1_fin<true, false><<<Gp, kBlock, 0, s>>>(fa, d_src);
+                k_push1_or<<<Gp, kBlock, 0, s>>>(fa, d_src);
+                if (compat) k_push1_fin<true><<<Gp, kBlock, 0, s>>>(fa, d_src);
+                else k_push1_fin<false><<<Gp, kBlock, 0, s>>>(fa, d_src);
                 HIPTRY(hipGetLastError());
                 TRY(timer_end(ctx, tm, e0, true));
                 continue;
             }
-            if (compat) k_fwd<true, true><<<G, kBlock, 0, s>>>(fa);
-            else k_fwd<true, false><<<G, kBlock, 0, s>>>(fa);
+            if (compat) k_fwd<true><<<G, kBlock, 0, s>>>(fa);
+            else k_fwd<false><<<G, kBlock, 0, s>>>(fa);
             HIPTRY(hipGetLastError());
             if (v.nhub) {
-                k_fwd_hubseg<true><<<Gseg, kBlock, 0, s>>>(fa, ha);
-                if (compat) k_fwd_hubfin<true, true><<<Ghub, kBlock, 0, s>>>(fa, ha);
-                else k_fwd_hubfin<true, false><<<Ghub, kBlock, 0, s>>>(fa, ha);
+                k_fwd_hubseg<<<Gseg, kBlock, 0, s>>>(fa, ha);
+                if (compat) k_fwd_hubfin<true><<<Ghub, kBlock, 0, s>>>(fa, ha);
+                else k_fwd_hubfin<false><<<Ghub, kBlock, 0, s>>>(fa, ha);
                 HIPTRY(hipGetLastError());
             }
             TRY(timer_end(ctx, tm, e0, true));

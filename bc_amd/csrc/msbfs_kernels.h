@@ -1,11 +1,17 @@
-// Multi-source level kernels: one wavefront carries 64 BFS sources, lane s is
-// source s of the batch.  Per vertex v the engine keeps
+// Multi-source level kernels of the dense betweenness step: one wavefront
+// carries 64 BFS sources, lane s is source s of the batch.  Per vertex v the
+// engine keeps
 //   vis[v]        uint64  lanes that have reached v
-//   F[d][v]       uint64  lanes that reach v at distance d (one mask per level)
-//   sigma[v][64]  double  shortest-path counts, lane-contiguous (512 B rows)
+//   F[d][v]       {mask, base} record: lanes that reach v at distance d and
+//                 the number of lanes that reached it earlier (one plane per
+//                 level, kept for the backward pass)
+//   sigma[v][64]  double  shortest-path counts, level-sorted (below)
 //   coef[v][64]   double  (1 + delta) / sigma, written by the backward pass
-// so a neighbour's frontier test is one 8-byte load for all 64 sources and a
-// path-count gather is one coalesced 512-byte row read.
+// so a neighbour's frontier test is one 16-byte load for all 64 sources and a
+// path-count gather reads one contiguous slice of a 512-byte row.  The
+// closeness step (masks only, up to 8 batches interleaved) is in
+// ccw_kernels.h; the frontier-queue step reuses the gathers and the vertex
+// epilogue below with plain masks and lane-indexed rows.
 //
 // Forward (pull): every vertex with an unreached lane scans its neighbours u,
 // keeps f = F[d][u] & ~vis[v]; lane s sums sigma[u][s] over the neighbours
@@ -47,15 +53,7 @@ struct HubArgs {
     const int* __restrict__ hub_seg_off;  // [nhub + 1]
     double* __restrict__ part_acc;        // [nseg][64]
     uint32_t* __restrict__ part_cnt;      // [nseg][64]
-    uint64_t* __restrict__ part_or;       // [nseg] per batch
 };
-
-// Batch blockIdx.y's segment partials of the mask OR (closeness batches).
-__device__ __forceinline__ HubArgs hub_view(const HubArgs& h0) {
-    HubArgs h = h0;
-    h.part_or += (size_t)blockIdx.y * (size_t)h.nseg;
-    return h;
-}
 
 struct FwdArgs {
     int n;
@@ -80,30 +78,7 @@ struct FwdArgs {
     // Levels are enqueued ahead of the host's check: a launch whose level
     // is empty (*live == 0, written by the previous level) returns at once.
     const int* __restrict__ live;
-    // Several independent 64-source batches in one launch (closeness only):
-    // batch b = blockIdx.y uses vis / Fd / Fn + b * bstride, its own
-    // partial slots (pslots blocks each) and active lanes A (A_last for the
-    // last batch); dist_sum is then shared, so it is updated atomically.
-    size_t bstride;
-    int pslots;
-    uint64_t A_last;
-    bool dist_atomic;
 };
-
-// The arguments of batch blockIdx.y (the identity for a one-batch launch).
-__device__ __forceinline__ FwdArgs batch_view(const FwdArgs& a0) {
-    FwdArgs a = a0;
-    const int b = blockIdx.y;
-    if (b) {
-        a.vis += (size_t)b * a.bstride;
-        a.Fd += (size_t)b * a.bstride;
-        a.Fn += (size_t)b * a.bstride;
-        a.part_far += (size_t)b * a.pslots * kWave;
-        a.part_cnt += (size_t)b * a.pslots * 2;
-    }
-    if (b == (int)gridDim.y - 1) a.A = a.A_last;
-    return a;
-}
 
 struct BwdArgs {
     int n;
@@ -278,10 +253,7 @@ __device__ __forceinline__ void fwd_vertex_done(const FwdArgs& a, int v, uint64_
         else a.Fn[v] = nm;
         a.vis[v] = old | nm;
         const int pc = __popcll(nm);
-        if (a.dist_sum) {
-            if (a.dist_atomic) atomicAdd((unsigned long long*)&a.dist_sum[v], (unsigned long long)((int64_t)a.dnext * pc));
-            else a.dist_sum[v] += (int64_t)a.dnext * pc;
-        }
+        if (a.dist_sum) a.dist_sum[v] += (int64_t)a.dnext * pc;  // one wave per vertex and level
         r.nnew += (unsigned long long)pc;
     }
 }
@@ -336,9 +308,10 @@ __device__ __forceinline__ void bwd_vertex_done(const BwdArgs& a, int v, uint64_
 }
 
 // ---------------------------------------------------------------- forward
-template <bool SIGMA, bool COMPAT>
-__global__ __launch_bounds__(kBlock) void k_fwd(FwdArgs a0) {
-    const FwdArgs a = batch_view(a0);
+// Betweenness levels >= 2 (path counts, level-sorted rows; the closeness
+// step has its own interleaved kernels).
+template <bool COMPAT>
+__global__ __launch_bounds__(kBlock) void k_fwd(FwdArgs a) {
     if (a.live && *a.live == 0) return;  // uniform over the grid
     const int lane = lane_id();
     const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
@@ -367,18 +340,15 @@ __global__ __launch_bounds__(kBlock) void k_fwd(FwdArgs a0) {
             double acc = 0.0;
             uint32_t cnt = 0;
             uint64_t orf = 0;
-            gather_fwd<SIGMA, SIGMA>(a.col, a.Fd, a.sigma, rl32(rbl, b), rl32(rel, b), A & ~old, lane, acc, cnt, orf);
-            fwd_vertex_done<SIGMA, COMPAT, SIGMA>(a, v, old, acc, cnt, SIGMA ? 0 : wave_or(orf), lane, r);
+            gather_fwd<true, true>(a.col, a.Fd, a.sigma, rl32(rbl, b), rl32(rel, b), A & ~old, lane, acc, cnt, orf);
+            fwd_vertex_done<true, COMPAT, true>(a, v, old, acc, cnt, 0, lane, r);
         }
     }
     fwd_block_flush(a, r, blockIdx.x);
 }
 
-template <bool SIGMA>
-__global__ __launch_bounds__(kBlock) void k_fwd_hubseg(FwdArgs a0, HubArgs h0) {
-    const FwdArgs a = batch_view(a0);
+__global__ __launch_bounds__(kBlock) void k_fwd_hubseg(FwdArgs a, HubArgs h) {
     if (a.live && *a.live == 0) return;  // uniform over the grid
-    const HubArgs h = hub_view(h0);
     const int lane = lane_id();
     const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const int nw = gridDim.x * kWavesPerBlock;
@@ -389,22 +359,15 @@ __global__ __launch_bounds__(kBlock) void k_fwd_hubseg(FwdArgs a0, HubArgs h0) {
         double acc = 0.0;
         uint32_t cnt = 0;
         uint64_t orf = 0;
-        gather_fwd<SIGMA, SIGMA, kUnrollHub>(a.col, a.Fd, a.sigma, h.seg_beg[i], h.seg_end[i], cand, lane, acc, cnt, orf);
-        if constexpr (SIGMA) {
-            h.part_acc[(size_t)i * kWave + lane] = acc;
-            h.part_cnt[(size_t)i * kWave + lane] = cnt;
-        } else {
-            orf = wave_or(orf);
-            if (lane == 0) h.part_or[i] = orf;
-        }
+        gather_fwd<true, true, kUnrollHub>(a.col, a.Fd, a.sigma, h.seg_beg[i], h.seg_end[i], cand, lane, acc, cnt, orf);
+        h.part_acc[(size_t)i * kWave + lane] = acc;
+        h.part_cnt[(size_t)i * kWave + lane] = cnt;
     }
 }
 
-template <bool SIGMA, bool COMPAT>
-__global__ __launch_bounds__(kBlock) void k_fwd_hubfin(FwdArgs a0, HubArgs h0) {
-    const FwdArgs a = batch_view(a0);
+template <bool COMPAT>
+__global__ __launch_bounds__(kBlock) void k_fwd_hubfin(FwdArgs a, HubArgs h) {
     if (a.live && *a.live == 0) return;  // uniform over the grid
-    const HubArgs h = hub_view(h0);
     const int lane = lane_id();
     const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const int nw = gridDim.x * kWavesPerBlock;
@@ -415,16 +378,11 @@ __global__ __launch_bounds__(kBlock) void k_fwd_hubfin(FwdArgs a0, HubArgs h0) {
         if (!(a.A & ~old)) continue;
         double acc = 0.0;
         uint32_t cnt = 0;
-        uint64_t orf = 0;
         for (int i = h.hub_seg_off[k] - h.seg_base; i < h.hub_seg_off[k + 1] - h.seg_base; ++i) {
-            if constexpr (SIGMA) {
-                acc += h.part_acc[(size_t)i * kWave + lane];
-                cnt += h.part_cnt[(size_t)i * kWave + lane];
-            } else {
-                orf |= h.part_or[i];
-            }
+            acc += h.part_acc[(size_t)i * kWave + lane];
+            cnt += h.part_cnt[(size_t)i * kWave + lane];
         }
-        fwd_vertex_done<SIGMA, COMPAT, SIGMA>(a, v, old, acc, cnt, orf, lane, r);
+        fwd_vertex_done<true, COMPAT, true>(a, v, old, acc, cnt, 0, lane, r);
     }
     fwd_block_flush(a, r, blockIdx.x);
 }
@@ -481,65 +439,48 @@ __device__ __forceinline__ int push_lane(const int* s_pre, int e) {
     return lo;
 }
 
-template <bool LS>
-__device__ __forceinline__ uint64_t* front_mask(uint64_t* F, int v) {
-    if constexpr (LS) return &reinterpret_cast<FRec*>(F)[v].mask;
-    else return &F[v];
-}
-
-template <bool LS>
-__global__ __launch_bounds__(kBlock) void k_push1_or(FwdArgs a0, const int* __restrict__ src0) {
-    const FwdArgs a = batch_view(a0);
+__global__ __launch_bounds__(kBlock) void k_push1_or(FwdArgs a, const int* __restrict__ src) {
     __shared__ int s_pre[kWave + 1];
     __shared__ int s_beg[kWave];
-    const int* src = src0 + (size_t)blockIdx.y * kWave;
     const int E = push_rows(a.A, a.rp, src, s_pre, s_beg);
     for (int e = blockIdx.x * kBlock + threadIdx.x; e < E; e += gridDim.x * kBlock) {
         const int s = push_lane(s_pre, e);
         const int u = a.col[s_beg[s] + e - s_pre[s]];
         if ((a.vis[u] >> s) & 1) continue;  // a self-loop of lane s's source
-        atomicOr((unsigned long long*)front_mask<LS>(a.Fn, u), 1ull << s);
+        atomicOr((unsigned long long*)&reinterpret_cast<FRec*>(a.Fn)[u].mask, 1ull << s);
     }
 }
 
-template <bool SIGMA, bool COMPAT>
-__global__ __launch_bounds__(kBlock) void k_push1_fin(FwdArgs a0, const int* __restrict__ src0) {
-    constexpr bool LS = SIGMA;
-    const FwdArgs a = batch_view(a0);
+template <bool COMPAT>
+__global__ __launch_bounds__(kBlock) void k_push1_fin(FwdArgs a, const int* __restrict__ src) {
     __shared__ int s_pre[kWave + 1];
     __shared__ int s_beg[kWave];
     __shared__ unsigned long long s_far[kWave];
     __shared__ unsigned long long s_new;
-    const int* src = src0 + (size_t)blockIdx.y * kWave;
     if (threadIdx.x < kWave) s_far[threadIdx.x] = 0;
     if (threadIdx.x == 0) s_new = 0;
     const int E = push_rows(a.A, a.rp, src, s_pre, s_beg);  // synchronises the block
     unsigned long long nn = 0;
+    FRec* Fn = reinterpret_cast<FRec*>(a.Fn);
     for (int e = blockIdx.x * kBlock + threadIdx.x; e < E; e += gridDim.x * kBlock) {
         const int s = push_lane(s_pre, e);
         const int u = a.col[s_beg[s] + e - s_pre[s]];
-        const uint64_t nm = *front_mask<LS>(a.Fn, u);
+        const uint64_t nm = Fn[u].mask;
         if (!((nm >> s) & 1) || __builtin_ctzll(nm) != s) continue;  // finished by its lowest lane
         const uint64_t old = a.vis[u];
-        const int base = LS ? __popcll(old) : 0;
-        if constexpr (LS) reinterpret_cast<FRec*>(a.Fn)[u].base = (uint64_t)base;
+        const int base = __popcll(old);
+        Fn[u].base = (uint64_t)base;
         a.vis[u] = old | nm;
         const unsigned long long w = a.weight ? (unsigned long long)a.weight[u] : 1ull;
         int r = 0;
         for (uint64_t m = nm; m; m &= m - 1, ++r) {
-            const int t = __builtin_ctzll(m);
-            if constexpr (SIGMA) {
-                const size_t idx = (size_t)u * kWave + (LS ? base + r : t);
-                a.sigma[idx] = 1.0;
-                if constexpr (COMPAT) a.npred[idx] = 1u;
-            }
-            atomicAdd(&s_far[t], w);
+            const size_t idx = (size_t)u * kWave + base + r;
+            a.sigma[idx] = 1.0;
+            if constexpr (COMPAT) a.npred[idx] = 1u;
+            atomicAdd(&s_far[__builtin_ctzll(m)], w);
         }
         const int pc = __popcll(nm);
-        if (a.dist_sum) {
-            if (a.dist_atomic) atomicAdd((unsigned long long*)&a.dist_sum[u], (unsigned long long)pc);
-            else a.dist_sum[u] += (int64_t)pc;
-        }
+        if (a.dist_sum) a.dist_sum[u] += (int64_t)pc;  // u's finishing thread only
         nn += (unsigned long long)pc;
     }
     if (nn) atomicAdd(&s_new, nn);
