@@ -81,11 +81,12 @@ int cent_reduced_build(const struct CSR* g, struct cent_reduced** out) {
         st = CENT_E_OOM;
         goto fail;
     }
-    for (int32_t v = 0; v < n; ++v) { disc[v] = -1; stamp[v] = -1; pblock_of[v] = -1; }
+    /* disc: -2 folded (skipped), -1 not yet reached, else the DFS time */
+    for (int32_t v = 0; v < n; ++v) { disc[v] = ALIVE(v) ? -1 : -2; stamp[v] = -1; pblock_of[v] = -1; }
     int32_t timer = 0, nblk = 0;
     int64_t esp = 0;
     for (int32_t root = 0; root < n; ++root) {
-        if (!ALIVE(root) || r->d1.rdeg[root] == 0 || disc[root] >= 0) continue;
+        if (disc[root] != -1 || r->d1.rdeg[root] == 0) continue;
         int32_t fsp = 0;
         fstack[fsp++] = root;
         disc[root] = low[root] = timer++;
@@ -93,19 +94,28 @@ int cent_reduced_build(const struct CSR* g, struct cent_reduced** out) {
         it[root] = rp[root];
         while (fsp > 0) {
             const int32_t v = fstack[fsp - 1];
-            if (it[v] < rp[v + 1]) {
-                const int32_t x = col[it[v]++];
-                if (!ALIVE(x)) continue;
-                if (disc[x] < 0) {
-                    estack[esp].u = v; estack[esp].v = x; ++esp;
-                    dpar[x] = v;
-                    disc[x] = low[x] = timer++;
-                    it[x] = rp[x];
-                    fstack[fsp++] = x;
-                } else if (x != dpar[v] && disc[x] < disc[v]) {
-                    estack[esp].u = v; estack[esp].v = x; ++esp;
-                    if (disc[x] < low[v]) low[v] = disc[x];
+            /* scan v's remaining edges up to the next tree edge, with v's
+             * DFS state in registers */
+            const int32_t dv = disc[v], pv = dpar[v], je = rp[v + 1];
+            int32_t lv = low[v], j = it[v], x = -1;
+            for (; j < je; ++j) {
+                const int32_t y = col[j];
+                const int32_t dy = disc[y];
+                if (dy == -2) continue;
+                if (dy == -1) { x = y; ++j; break; }
+                if (y != pv && dy < dv) {  /* back edge to an ancestor */
+                    estack[esp].u = v; estack[esp].v = y; ++esp;
+                    if (dy < lv) lv = dy;
                 }
+            }
+            it[v] = j;
+            low[v] = lv;
+            if (x >= 0) {
+                estack[esp].u = v; estack[esp].v = x; ++esp;
+                dpar[x] = v;
+                disc[x] = low[x] = timer++;
+                it[x] = rp[x];
+                fstack[fsp++] = x;
             } else {
                 --fsp;
                 const int32_t p = dpar[v];
@@ -113,16 +123,16 @@ int cent_reduced_build(const struct CSR* g, struct cent_reduced** out) {
                 if (low[v] < low[p]) low[p] = low[v];
                 if (low[v] >= disc[p]) {
                     /* pop the block ending at tree edge (p, v); head = p */
-                    if ((st = cent_ivec_push(&bstart, (int32_t)bverts.n)) ||
-                        (st = cent_ivec_push(&bestart, (int32_t)bedges_u.n)) ||
-                        (st = cent_ivec_push(&bheadv, p)))
+                    if ((st = cent_ivec_put(&bstart, (int32_t)bverts.n)) ||
+                        (st = cent_ivec_put(&bestart, (int32_t)bedges_u.n)) ||
+                        (st = cent_ivec_put(&bheadv, p)))
                         goto fail;
                     for (;;) {
                         --esp;
                         const int32_t a = estack[esp].u, b = estack[esp].v;
-                        if ((st = cent_ivec_push(&bedges_u, a)) || (st = cent_ivec_push(&bedges_v, b))) goto fail;
-                        if (stamp[a] != nblk) { stamp[a] = nblk; if ((st = cent_ivec_push(&bverts, a))) goto fail; }
-                        if (stamp[b] != nblk) { stamp[b] = nblk; if ((st = cent_ivec_push(&bverts, b))) goto fail; }
+                        if ((st = cent_ivec_put(&bedges_u, a)) || (st = cent_ivec_put(&bedges_v, b))) goto fail;
+                        if (stamp[a] != nblk) { stamp[a] = nblk; if ((st = cent_ivec_put(&bverts, a))) goto fail; }
+                        if (stamp[b] != nblk) { stamp[b] = nblk; if ((st = cent_ivec_put(&bverts, b))) goto fail; }
                         if (dpar[b] == a) pblock_of[b] = nblk;  /* tree edge into b */
                         if (a == p && b == v) break;
                     }

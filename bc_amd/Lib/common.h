@@ -52,6 +52,11 @@ int64_t cent_sort_unique_u64(uint64_t* keys, int64_t n);
 /* ---- growable int32 vector */
 typedef struct { int32_t* a; int64_t n, cap; } cent_ivec;
 int  cent_ivec_push(cent_ivec* v, int32_t x);
+/* cent_ivec_push with the in-capacity case inlined (hot loops). */
+static inline int cent_ivec_put(cent_ivec* v, int32_t x) {
+    if (v->n < v->cap) { v->a[v->n++] = x; return CENT_OK; }
+    return cent_ivec_push(v, x);
+}
 void cent_ivec_free(cent_ivec* v);
 
 /* Connected-component labels (BFS), returns number of components; comp[v]
