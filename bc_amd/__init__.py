@@ -724,7 +724,7 @@ REQ_STATE = {0: "queued", 1: "running", 2: "done", 3: "failed", 4: "cancelled"}
 
 #: Source-parallel requests run by the server use at most this many per-source
 #: slots (bc_amd/csrc/server.cpp kSourceSlots; the admission estimate counts them).
-SERVER_SOURCE_SLOTS = 64
+SERVER_SOURCE_SLOTS = 256
 
 
 class _ServerOpts(C.Structure):

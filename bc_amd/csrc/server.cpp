@@ -90,8 +90,10 @@ static double now_ms(const cent_server* s) {
 }
 
 // Source-parallel requests (deep graphs) run at most this many per-source
-// slots, so their workspace stays inside the estimate below.
-constexpr int32_t kSourceSlots = 64;
+// slots, so their workspace stays inside the estimate below (the engine's
+// own default: fewer slots leave the GPU idle, e.g. 64 slots take 3x as long
+// on a 232k-vertex block tree).
+constexpr int32_t kSourceSlots = 256;
 
 // Device bytes a request needs besides the graph: the larger of the context
 // workspace of the 64-lane step (per vertex 2 x 512 B of path counts /
