@@ -141,6 +141,44 @@ def extra_grid(ctx, args):
             "fwd_ms": round(st["ms_fwd"], 2), "bwd_ms": round(st["ms_bwd"], 2)}
 
 
+def extra_closeness(ctx, args):
+    """Closeness alone: the main workload's 4096 sources on R-MAT 2^20 through
+    the interleaved 8-batch step, and all-vertex closeness on the serving
+    pool's 148k-vertex R-MAT graph, reduced (D1 + AP, persistent items plus
+    bounded two-stream big items) and plain.  Farness must agree."""
+    import numpy as np
+    import bc_amd
+
+    def best_of(fn, reps):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            out = fn()
+            ts.append(time.perf_counter() - t)
+        return min(ts), out
+
+    g = bc_amd.gen_rmat(args.scale, args.edge_factor << args.scale, 0.57, 0.19, 0.19, seed=2)
+    src = bc_amd.sample_sources(g.n, args.sources, 2)
+    dg = ctx.upload(g)
+    t20, (far20, _) = best_of(lambda: ctx.cc(dg, src), 3)
+    del dg
+    _, gp = serve_pool(5, 12)[0]
+    red = bc_amd.Reduced(gp)
+    dgp = ctx.upload(gp)
+    tr, far_r = best_of(lambda: ctx.cc_reduced(red), 3)
+    tp, (far_p, _) = best_of(lambda: ctx.cc(dgp, None), 2)
+    log(f"closeness: R-MAT 2^{args.scale} {len(src)} sources {t20 * 1e3:.1f} ms; 148k R-MAT all-vertex "
+        f"reduced {tr * 1e3:.1f} ms, plain {tp * 1e3:.1f} ms")
+    return {"workload": "closeness only: R-MAT 2^20 with the main workload's sources; all-vertex "
+                        "closeness on the serving pool's R-MAT graph 0 (seed 5000)",
+            "rmat20_sources": len(src), "rmat20_ms": round(t20 * 1e3, 2),
+            "rmat20_TEPS": round(len(src) * g.m / t20, 1),
+            "pool_rmat_n": gp.n, "pool_rmat_m": gp.m,
+            "reduced_ms": round(tr * 1e3, 2), "plain_ms": round(tp * 1e3, 2),
+            "farness_equal": bool(np.array_equal(far_r, far_p))}
+
+
 def extra_rmat14(ctx, args):
     """Exact all-source BC + closeness on R-MAT 2^14 (8 edge draws per vertex,
     LCC, seed 1).  The engine's time beside the
@@ -357,7 +395,7 @@ def serve_vs_plain(ctx, pool, reqs):
 
 
 EXTRA = {"rmat14": extra_rmat14, "blocktree": extra_blocktree, "blocktree-bc": extra_blocktree_bc, "grid": extra_grid,
-         "serve": extra_serve}
+         "closeness": extra_closeness, "serve": extra_serve}
 
 
 def main() -> int:
@@ -388,7 +426,8 @@ def main() -> int:
     ap.add_argument("--extra", default="",
                     help="comma list of extra workloads reported under 'workloads': "
                          "blocktree (reduced closeness, persistent vs per-item launches), "
-                         "blocktree-bc (plain BC+CC on the block tree), grid (sampled BC+CC on "
+                         "blocktree-bc (plain BC+CC on the block tree), closeness (closeness-only "
+                         "timings), grid (sampled BC+CC on "
                          "the 2048^2 grid)")
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--serve-rate", type=float, default=200.0)
@@ -445,6 +484,7 @@ def main() -> int:
         times.append(time.perf_counter() - t1)
         print(f"# step {i + 1}/{args.steps}: {times[-1] * 1e3:.1f} ms", file=sys.stderr, flush=True)
     st = ctx.stats()
+    ws_bytes = ctx.workspace_bytes()
     far = d_far.numpy()
     t_host = t_one = st1 = None
     if not args.timed_only:
@@ -561,6 +601,7 @@ def main() -> int:
                    "n": n, "m": m, "sources": k, "seed": 2, "global_batch": k,
                    "levels_per_batch": round(st["levels"] / max(1, st["batches"]), 2),
                    "dag_edges_per_step": int(dag), "parallelism": "single GPU",
+                   "device_workspace_bytes": ws_bytes,
                    "launch": layout},
         "roofline": roofline, "cpu_baseline": cpu,
         "pcie_inclusive": ({"ms_per_step": round(t_host * 1e3, 3), "value": round(k * m / t_host, 1),
