@@ -104,9 +104,9 @@ template <int W>
 __device__ __forceinline__ unsigned long long ccw_done(const CcwArgs& a, int v, uint64_t old, uint64_t nm,
                                                        long long (*s_far)[kWave], int lane) {
     const int b = lane & (W - 1), e = lane / W;
-    if (e == 0 && nm) {
-        a.Fn[(size_t)v * W + b] = nm;
-        a.vis[(size_t)v * W + b] = old | nm;
+    if (e == 0) {
+        a.Fn[(size_t)v * W + b] = nm;  // every entry of Fn is written each level (no clear)
+        if (nm) a.vis[(size_t)v * W + b] = old | nm;
     }
     if (ballot(nm != 0) == 0) return 0;
     ccw_count<W>(s_far, nm, a.weight ? (long long)a.weight[v] : 1ll, lane);
@@ -163,6 +163,13 @@ __global__ __launch_bounds__(kBlock) void k_ccw(CcwArgs a) {
                 const uint64_t* vv = a.vis + (size_t)vl * W;
 #pragma unroll
                 for (int b = 0; b < W; ++b) need |= (ccw_active(a, b) & ~vv[b]) != 0;
+            }
+            // Fn is not cleared between levels: vertices without a scan (and
+            // hub rows, which k_ccw_hubfin may overwrite) get zero masks here
+            if (!need) {
+                uint64_t* fn = a.Fn + (size_t)vl * W;
+#pragma unroll
+                for (int b = 0; b < W; ++b) fn[b] = 0;
             }
         }
         uint64_t work = ballot(need);
