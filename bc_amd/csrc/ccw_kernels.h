@@ -233,16 +233,13 @@ __global__ __launch_bounds__(kBlock) void k_ccw_hubfin(CcwArgs a) {
 }
 
 // Seeds the group: lane s of batch b = blockIdx.x starts at src[b * 64 + s].
+// Only vis is seeded: level 1 is pushed from the source list, so no kernel
+// reads the level-0 plane.
 template <int W>
-__global__ void k_ccw_seed(const int* __restrict__ src, int nb, int k_last, uint64_t* __restrict__ F0,
-                           uint64_t* __restrict__ vis) {
+__global__ void k_ccw_seed(const int* __restrict__ src, int nb, int k_last, uint64_t* __restrict__ vis) {
     const int b = blockIdx.x, s = threadIdx.x;
     const int kb = b == nb - 1 ? k_last : kWave;
-    if (s < kb) {
-        const int v = src[b * kWave + s];
-        atomicOr((unsigned long long*)&F0[(size_t)v * W + b], 1ull << s);
-        atomicOr((unsigned long long*)&vis[(size_t)v * W + b], 1ull << s);
-    }
+    if (s < kb) atomicOr((unsigned long long*)&vis[(size_t)src[b * kWave + s] * W + b], 1ull << s);
 }
 
 // Level 1 by push from the sources' rows (as k_push1_or / k_push1_fin for the

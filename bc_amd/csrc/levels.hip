@@ -68,15 +68,16 @@ static int grow_levels(cent_ctx* ctx, size_t PL, int keep_levels) {
     return CENT_OK;
 }
 
-// Totals of batch b from its G per-workgroup partials: far_out[0..kb)
-// assigned, ctx->cnt accumulated.  One reducing workgroup per 128 partial
-// rows, at most 32.
-static int reduce_batch(cent_ctx* ctx, int G, int kb, int64_t* far_out, int b = 0) {
+// Totals of the nb batches of a launch group from their G per-workgroup
+// partials each: far_out[0..k) assigned (k = 64 (nb - 1) + k_last),
+// ctx->cnt accumulated.  One launch, grid y = batch; one reducing workgroup
+// per 128 partial rows of a batch, at most 32.
+static int reduce_batch(cent_ctx* ctx, int G, int nb, int k_last, int64_t* far_out) {
     hipStream_t s = ctx->stream;
-    HIPTRY(hipMemsetAsync(far_out, 0, sizeof(int64_t) * (size_t)(kb > 0 ? kb : 1), s));
+    const size_t k = (size_t)(nb - 1) * kWave + (size_t)k_last;
+    HIPTRY(hipMemsetAsync(far_out, 0, sizeof(int64_t) * (k > 0 ? k : 1), s));
     const int rg = std::max(1, std::min(32, G / 128));
-    k_reduce_batch<<<rg, 1024, 0, s>>>(ctx->part_far.p + (size_t)b * G * kWave, ctx->part_cnt.p + (size_t)b * G * 2,
-                                       G, kb, far_out, ctx->cnt.p);
+    k_reduce_batch<<<dim3(rg, nb), 1024, 0, s>>>(ctx->part_far.p, ctx->part_cnt.p, G, k_last, far_out, ctx->cnt.p);
     HIPTRY(hipGetLastError());
     return CENT_OK;
 }
@@ -140,7 +141,7 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
     const size_t PL = N * fw;
     auto plane = [&](size_t p) { return ctx->F.p + p * PL; };
     HIPTRY(hipMemsetAsync(ctx->vis.p + (size_t)W * v.lo, 0, sizeof(uint64_t) * W * R, s));
-    HIPTRY(hipMemsetAsync(plane(0) + fw * v.lo, 0, sizeof(uint64_t) * fw * R, s));
+    if (sp.with_bc) HIPTRY(hipMemsetAsync(plane(0) + fw * v.lo, 0, sizeof(uint64_t) * fw * R, s));
     HIPTRY(hipMemsetAsync(plane(1) + fw * v.lo, 0, sizeof(uint64_t) * fw * R, s));
     HIPTRY(hipMemsetAsync(ctx->part_far.p, 0, sizeof(int64_t) * (size_t)G * kWave * nb, s));
     HIPTRY(hipMemsetAsync(ctx->part_cnt.p, 0, sizeof(unsigned long long) * (size_t)G * 2 * nb, s));
@@ -148,10 +149,10 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
         k_seed<<<1, kWave, 0, s>>>(d_src, kl, plane(0), ctx->vis.p, ctx->sigma.p, true);
     } else {
         switch (W) {
-            case 1: k_ccw_seed<1><<<nb, kWave, 0, s>>>(d_src, nb, kl, plane(0), ctx->vis.p); break;
-            case 2: k_ccw_seed<2><<<nb, kWave, 0, s>>>(d_src, nb, kl, plane(0), ctx->vis.p); break;
-            case 4: k_ccw_seed<4><<<nb, kWave, 0, s>>>(d_src, nb, kl, plane(0), ctx->vis.p); break;
-            default: k_ccw_seed<8><<<nb, kWave, 0, s>>>(d_src, nb, kl, plane(0), ctx->vis.p); break;
+            case 1: k_ccw_seed<1><<<nb, kWave, 0, s>>>(d_src, nb, kl, ctx->vis.p); break;
+            case 2: k_ccw_seed<2><<<nb, kWave, 0, s>>>(d_src, nb, kl, ctx->vis.p); break;
+            case 4: k_ccw_seed<4><<<nb, kWave, 0, s>>>(d_src, nb, kl, ctx->vis.p); break;
+            default: k_ccw_seed<8><<<nb, kWave, 0, s>>>(d_src, nb, kl, ctx->vis.p); break;
         }
     }
     HIPTRY(hipGetLastError());
@@ -296,8 +297,7 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
         k_level_check<<<1, 1, 0, s>>>(ctx->lvf.p, bound, nb, bc, ctx->lvstat.p);
         HIPTRY(hipGetLastError());
     }
-    for (int b = 0; b < nb; ++b)
-        TRY(reduce_batch(ctx, G, b == nb - 1 ? kl : kWave, far_out + b * kWave, b));
+    TRY(reduce_batch(ctx, G, nb, kl, far_out));
     ctx->stats.batches += nb;
     return CENT_OK;
 }
@@ -422,7 +422,7 @@ static int level_batch_sparse(cent_ctx* ctx, SpPlan& pl, const BatchSpec& sp, co
         }
     }
     k_sp_clear<<<G, kBlock, 0, s>>>(pl.a);
-    TRY(reduce_batch(ctx, G, kb, far_out));
+    TRY(reduce_batch(ctx, G, 1, kb, far_out));
     ctx->stats.batches++;
     return CENT_OK;
 }

@@ -592,20 +592,28 @@ __global__ void k_level_check(const int* __restrict__ lvf, int bound, int nbatch
     if (backward) atomicAdd(&out[3], depth);
 }
 
-// Sums the per-block lane partials of a batch into far[0..k) and the
-// counters.  Several workgroups share the rows (workgroup g takes rows
-// g*16 + r, then every gridDim*16-th); each adds its lane totals to far /
-// cnt_out with 64-bit atomics.  The partials are integers, so the total does
-// not depend on the order.  far must be zeroed before the launch.
+// Sums the per-block lane partials of the batches of a launch group into
+// far[0..k) and the counters: batch b = blockIdx.y has nblocks partial rows
+// from row b * nblocks on, 64 sources (k_last for the last batch) and its
+// farness at far + 64 b.  Several workgroups share a batch's rows
+// (workgroup g takes rows g*16 + r, then every gridDim.x*16-th); each adds
+// its lane totals to far / cnt_out with 64-bit atomics.  The partials are
+// integers, so the total does not depend on the order.  far must be zeroed
+// before the launch.
 __global__ __launch_bounds__(1024) void k_reduce_batch(const int64_t* __restrict__ part_far,
                                                        const unsigned long long* __restrict__ part_cnt,
-                                                       int nblocks, int k, int64_t* __restrict__ far,
+                                                       int nblocks, int k_last, int64_t* __restrict__ far,
                                                        unsigned long long* __restrict__ cnt_out) {
     constexpr int kRows = 1024 / kWave;
     __shared__ int64_t s_far[kRows][kWave];
     __shared__ unsigned long long s_c[kRows][2];
     const int lane = threadIdx.x & 63;
     const int row = threadIdx.x >> 6;
+    const int bb = blockIdx.y;
+    const int k = bb == (int)gridDim.y - 1 ? k_last : kWave;
+    part_far += (size_t)bb * nblocks * kWave;
+    part_cnt += (size_t)bb * nblocks * 2;
+    far += (size_t)bb * kWave;
     int64_t f = 0;
     unsigned long long c = 0;
     for (int b = blockIdx.x * kRows + row; b < nblocks; b += gridDim.x * kRows) {
