@@ -89,6 +89,23 @@ int level_flags_reset(cent_ctx* ctx) {
     return CENT_OK;
 }
 
+// Zeroes up to five ranges of 64-bit words in one launch (the clears that
+// start a launch group; one launch instead of a fill per buffer).
+struct ClearRanges {
+    uint64_t* p[5];
+    size_t n[5];
+    int count;
+};
+
+__global__ void k_clear(ClearRanges c) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (int r = 0; r < c.count; ++r) {
+        uint64_t* p = c.p[r];
+        const size_t n = c.n[r];
+        for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0;
+    }
+}
+
 // Workgroups of the level-1 push launches (their lane partials go to the
 // first kPushBlocks of the forward step's block slots).
 constexpr int kPushBlocks = 256;
@@ -140,11 +157,20 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
     const size_t fw = sp.with_bc ? 2 : (size_t)W;
     const size_t PL = N * fw;
     auto plane = [&](size_t p) { return ctx->F.p + p * PL; };
-    HIPTRY(hipMemsetAsync(ctx->vis.p + (size_t)W * v.lo, 0, sizeof(uint64_t) * W * R, s));
-    if (sp.with_bc) HIPTRY(hipMemsetAsync(plane(0) + fw * v.lo, 0, sizeof(uint64_t) * fw * R, s));
-    HIPTRY(hipMemsetAsync(plane(1) + fw * v.lo, 0, sizeof(uint64_t) * fw * R, s));
-    HIPTRY(hipMemsetAsync(ctx->part_far.p, 0, sizeof(int64_t) * (size_t)G * kWave * nb, s));
-    HIPTRY(hipMemsetAsync(ctx->part_cnt.p, 0, sizeof(unsigned long long) * (size_t)G * 2 * nb, s));
+    {  // vis, the level-1 plane (and the level-0 one for betweenness), the partials
+        ClearRanges cr{};
+        auto add = [&](void* p, size_t n) { cr.p[cr.count] = static_cast<uint64_t*>(p); cr.n[cr.count++] = n; };
+        add(ctx->vis.p + (size_t)W * v.lo, (size_t)W * R);
+        if (sp.with_bc) add(plane(0) + fw * v.lo, fw * R);
+        add(plane(1) + fw * v.lo, fw * R);
+        add(ctx->part_far.p, (size_t)G * kWave * nb);
+        add(ctx->part_cnt.p, (size_t)G * 2 * nb);
+        size_t tot = 0;
+        for (int i = 0; i < cr.count; ++i) tot = std::max(tot, cr.n[i]);
+        const int cg = (int)std::min<size_t>((tot + 255) / 256, 4096);
+        k_clear<<<std::max(cg, 1), 256, 0, s>>>(cr);
+        HIPTRY(hipGetLastError());
+    }
     if (sp.with_bc) {
         k_seed<<<1, kWave, 0, s>>>(d_src, kl, plane(0), ctx->vis.p, ctx->sigma.p, true);
     } else {
