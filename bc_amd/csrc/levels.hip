@@ -457,12 +457,19 @@ static int level_batch_sparse(cent_ctx* ctx, SpPlan& pl, const BatchSpec& sp, co
 // All k sources of a plain call in one launch of k_ps: G workgroup slots,
 // each with its own per-vertex arrays, take sources s, s + G, ... in turn.
 // G is bounded by k, by the slot cap (opts->max_slots, else grid_waves / 8,
-// else 256: more concurrent traversals than that only contend for the
-// memory system) and by a memory budget (40% of free device memory, at most
-// 32 GiB).  Returns CENT_E_UNSUPPORTED, with nothing accumulated into the BC
+// else 256 — more concurrent traversals than that only contend for the
+// memory system — doubled up to 1024 while all slots' arrays stay within
+// kPsSlotSetBytes) and by a memory budget (40% of free device memory, at
+// most 32 GiB).  Returns CENT_E_UNSUPPORTED, with nothing accumulated into the BC
 // output, if some source's BFS is deeper than the level table: the caller
 // then runs the batch form.
 constexpr int kPsDefaultSlots = 256;
+constexpr int kPsMaxCachedSlots = 1024;
+// Measured on MI355X: doubling the slots from 256 pays (by 8-30%) while the
+// slots' arrays stay within about 2 GB — R-MAT 2^10-2^14 and block trees of
+// 11k-113k vertices, all sources — and costs beyond (the 4.2M-vertex grid,
+// 117 MB per slot, is fastest at 256).
+constexpr size_t kPsSlotSetBytes = (size_t)2 << 30;
 
 static int run_persource(cent_ctx* ctx, const cent_graph* g, const BatchSpec& sp, int32_t k, KernelTimer& tm,
                          int64_t* levels) {
@@ -476,6 +483,8 @@ static int run_persource(cent_ctx* ctx, const cent_graph* g, const BatchSpec& sp
     size_t cap = kPsDefaultSlots;
     if (sp.opts && sp.opts->max_slots > 0) cap = (size_t)sp.opts->max_slots;
     else if (sp.opts && sp.opts->grid_waves > 0) cap = std::max(1, sp.opts->grid_waves / kPsWaves);
+    else
+        while (cap < (size_t)kPsMaxCachedSlots && cap * 2 * per_slot <= kPsSlotSetBytes) cap *= 2;
     size_t G = std::min((size_t)k, cap);
     size_t freeb = 0, totalb = 0;
     HIPTRY(hipMemGetInfo(&freeb, &totalb));
