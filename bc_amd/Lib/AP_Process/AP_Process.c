@@ -40,19 +40,7 @@ void cent_reduced_free(struct cent_reduced* r) {
 }
 
 typedef struct { int32_t u, v; } edge_t;
-typedef struct { int32_t deg, pos, x; } deg_pos_t;
 
-static int cmp_deg_pos(const void* a, const void* b) {
-    const deg_pos_t* p = (const deg_pos_t*)a;
-    const deg_pos_t* q = (const deg_pos_t*)b;
-    if (p->deg != q->deg) return p->deg > q->deg ? -1 : 1;
-    return (p->pos > q->pos) - (p->pos < q->pos);
-}
-
-static int cmp_i32(const void* a, const void* b) {
-    const int32_t x = *(const int32_t*)a, y = *(const int32_t*)b;
-    return (x > y) - (x < y);
-}
 
 int cent_reduced_build(const struct CSR* g, struct cent_reduced** out) {
     if (!g || !out) return CENT_E_INVAL;
@@ -206,24 +194,36 @@ int cent_reduced_build(const struct CSR* g, struct cent_reduced** out) {
      * in emission order), and each row sorted by id below: the highest-degree
      * vertices of a block share a small id range, and a row lists its
      * highest-degree neighbours — the earliest reached — first. */
-    {
+    {  /* counting sort per block: a block vertex's degree is below the block size */
         int32_t* bdeg = (int32_t*)calloc(N, sizeof(int32_t));
         int64_t maxbs = 1;
         for (int32_t b = 0; b < nblk; ++b) maxbs = bstart.a[b + 1] - bstart.a[b] > maxbs ? bstart.a[b + 1] - bstart.a[b] : maxbs;
-        deg_pos_t* tmp = (deg_pos_t*)malloc((size_t)maxbs * sizeof(deg_pos_t));
-        if (!bdeg || !tmp) { free(bdeg); free(tmp); free(sub_w); free(hang_w); free(local); free(fill); st = CENT_E_OOM; goto fail; }
+        int32_t* tmp = (int32_t*)malloc((size_t)maxbs * sizeof(int32_t));
+        int32_t* cnt = (int32_t*)calloc((size_t)maxbs + 1, sizeof(int32_t));
+        if (!bdeg || !tmp || !cnt) {
+            free(bdeg); free(tmp); free(cnt); free(sub_w); free(hang_w); free(local); free(fill);
+            st = CENT_E_OOM;
+            goto fail;
+        }
         for (int32_t b = 0; b < nblk; ++b) {
             for (int32_t e = bestart.a[b]; e < bestart.a[b + 1]; ++e) { bdeg[bedges_u.a[e]]++; bdeg[bedges_v.a[e]]++; }
             const int32_t b0 = bstart.a[b], bn = bstart.a[b + 1] - b0;
+            int32_t dmax = 0;
             for (int32_t i = 0; i < bn; ++i) {
-                const int32_t x = bverts.a[b0 + i];
-                tmp[i].deg = bdeg[x]; tmp[i].pos = i; tmp[i].x = x;
+                const int32_t d = bdeg[bverts.a[b0 + i]];
+                cnt[d]++;
+                if (d > dmax) dmax = d;
             }
-            qsort(tmp, (size_t)bn, sizeof(deg_pos_t), cmp_deg_pos);
-            for (int32_t i = 0; i < bn; ++i) { bverts.a[b0 + i] = tmp[i].x; bdeg[tmp[i].x] = 0; }
+            /* slots for descending degree: degree d starts after all larger ones */
+            int32_t at = 0;
+            for (int32_t d = dmax; d >= 0; --d) { const int32_t c = cnt[d]; cnt[d] = at; at += c; }
+            for (int32_t i = 0; i < bn; ++i) { const int32_t x = bverts.a[b0 + i]; tmp[cnt[bdeg[x]]++] = x; }
+            for (int32_t i = 0; i < bn; ++i) { const int32_t x = tmp[i]; bverts.a[b0 + i] = x; bdeg[x] = 0; }
+            for (int32_t d = 0; d <= dmax; ++d) cnt[d] = 0;
         }
         free(bdeg);
         free(tmp);
+        free(cnt);
     }
     for (int32_t b = 0; b < nblk; ++b) {
         const int32_t h = bheadv.a[b];
@@ -259,8 +259,18 @@ int cent_reduced_build(const struct CSR* g, struct cent_reduced** out) {
             r->pcol[fill[c]++] = a;
         }
     }
-    for (int32_t i = 0; i < r->np; ++i)
-        qsort(r->pcol + r->prp[i], (size_t)(r->prp[i + 1] - r->prp[i]), sizeof(int32_t), cmp_i32);
+    /* sorted rows by one transpose: appending a to each neighbour's row for
+     * a = 0, 1, ... writes every row in ascending order (the graph is
+     * symmetric, so the transpose has the same rows) */
+    {
+        int32_t* sorted = (int32_t*)malloc((size_t)(r->nnzp > 0 ? r->nnzp : 1) * sizeof(int32_t));
+        if (!sorted) { free(local); free(fill); free(sub_w); free(hang_w); st = CENT_E_OOM; goto fail; }
+        for (int32_t i = 0; i < r->np; ++i) fill[i] = r->prp[i];
+        for (int32_t a = 0; a < r->np; ++a)
+            for (int32_t j = r->prp[a]; j < r->prp[a + 1]; ++j) sorted[fill[r->pcol[j]]++] = a;
+        free(r->pcol);
+        r->pcol = sorted;
+    }
     free(local); free(fill); free(sub_w); free(hang_w);
 
     /* ---- articulation vertices: copies in more than one block */
