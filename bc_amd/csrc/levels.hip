@@ -211,8 +211,9 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
             const int lv = dl + i;
             const uint64_t* Fd = bc ? ctx->F.p + PL * (size_t)lv : plane(lv & 1);
             uint64_t* Fn = bc ? ctx->F.p + PL * (size_t)(lv + 1) : plane((lv + 1) & 1);
-            // the closeness kernels write every entry of a level >= 2 plane
-            if (bc && lv + 1 > 1) HIPTRY(hipMemsetAsync(Fn + fw * v.lo, 0, sizeof(uint64_t) * fw * R, s));
+            // no clear of the next plane: the level kernels (betweenness and
+            // closeness) write every entry of a level >= 2 plane, and level 1's
+            // plane is cleared when the batch starts
             if (!bc) {  // closeness: the interleaved step over the group's batches
                 CcwArgs ca{};
                 ca.hub_deg = d.hub_deg; ca.nchunk = v.nchunk; ca.chunk_beg = d.chunk_beg + v.c0;

@@ -236,7 +236,13 @@ __device__ __forceinline__ void fwd_vertex_done(const FwdArgs& a, int v, uint64_
     uint64_t nm;
     if constexpr (SIGMA) nm = ballot(cnt != 0);
     else nm = orf_all;
-    if (!nm) return;
+    if (!nm) {
+        // level-sorted planes are not cleared between levels: every record
+        // of the next plane is written, an empty one here
+        if constexpr (LS)
+            if (lane == 0) reinterpret_cast<FRec*>(a.Fn)[v] = FRec{0, 0};
+        return;
+    }
     const int base = LS ? __popcll(old) : 0;  // lanes that reached v earlier
     if ((nm >> lane) & 1) {
         const size_t idx = (size_t)v * kWave + (LS ? base + lane_rank(nm) : lane);
@@ -330,6 +336,10 @@ __global__ __launch_bounds__(kBlock) void k_fwd(FwdArgs a) {
                 rel = a.rp[vl + 1];
                 if (rel - rbl > a.hub_deg) visl = A;  // hub rows: k_fwd_hubseg
             }
+            // the next plane is not cleared between levels: vertices without
+            // a scan here (and hub rows, which k_fwd_hubfin may overwrite)
+            // get an empty record
+            if (visl == A) reinterpret_cast<FRec*>(a.Fn)[vl] = FRec{0, 0};
         }
         uint64_t work = ballot(visl != A);
         while (work) {
