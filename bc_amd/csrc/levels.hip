@@ -130,12 +130,15 @@ static void ccw_level(const CcwArgs& ca, int lv, int G, int Gp, int Gseg, int Gh
 
 // Batches of the k sources d_src[0..k) (device ids) whose BFS stays inside
 // the view's vertex range: one 64-lane batch for BC (k <= 64), up to
-// kMultiBatch batches of 64 consecutive sources side by side for closeness
-// (each level is one launch over all of them, so launches and host
-// round trips are shared).  far_out (device, k entries) gets the weighted
-// distance sums.  Adds the depth to *levels once per batch.
+// kMultiBatch batches of 64 consecutive sources for closeness, their masks
+// interleaved by vertex (one row scan per level serves all of them).
+// far_out (device, k entries) gets the weighted distance sums.  bound > 0:
+// the BFS of every source ends within `bound` levels, so all levels are
+// enqueued at once with no host check (0: host-checked runs).  Adds the
+// depth to *levels once per batch (host-checked runs only; bounded ones
+// report it through the context's level statistics).
 int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d_src, int k,
-                       int64_t* far_out, KernelTimer& tm, int64_t* levels, int bound) {
+                int64_t* far_out, KernelTimer& tm, int64_t* levels, int bound) {
     const DevCSR& d = *v.d;
     hipStream_t s = ctx->stream;
     const size_t N = (size_t)(d.n > 0 ? d.n : 1);
