@@ -90,3 +90,29 @@ def test_device_outputs_rejected(ctx):
     df = ctx.device_array(len(src), np.int64).zero()
     ctx.cc_device(dg, src, df)
     assert np.array_equal(df.numpy(), ctx.cc(dg, src)[0])
+
+
+def test_device_edge_cases(ctx):
+    """Empty source lists, a graph with isolated vertices (k = -1), and the
+    device-array helpers' argument checks."""
+    g = bc_amd.CSR.from_edges(6, [0, 1, 3], [1, 2, 4])  # vertex 5 isolated
+    dg = ctx.upload(g)
+    db = ctx.device_array(g.n, np.float64, 0.5)
+    ctx.bc_device(dg, [], db)                      # nothing to do: bc unchanged
+    assert np.array_equal(db.numpy(), np.full(g.n, 0.5))
+    df = ctx.device_array(g.n, np.int64, -1)
+    ctx.bc_device(dg, None, db, df)                # every vertex a source
+    hb, hf, _ = ctx.bc(dg, None, bc=np.full(g.n, 0.5))
+    assert np.array_equal(db.numpy(), hb) and np.array_equal(df.numpy(), hf)
+    assert df.numpy()[5] == 0
+    cf = ctx.device_array(g.n, np.int64).zero()
+    ctx.cc_device(dg, None, cf)
+    assert np.array_equal(cf.numpy(), hf)
+    # helpers: bad copy kind, negative sizes, zero-size operations
+    a = ctx.device_array(4, np.int64).zero()
+    host = np.arange(4, dtype=np.int64)
+    assert bc_amd._lib.cent_device_copy(ctx._h, a.ptr, host.ctypes.data, 32, 7) == bc_amd.CENT_E_INVAL
+    assert bc_amd._lib.cent_device_fill(ctx._h, a.ptr, 0, -1) == bc_amd.CENT_E_INVAL
+    assert bc_amd._lib.cent_device_copy(ctx._h, a.ptr, host.ctypes.data, 0, bc_amd.CENT_COPY_TO_DEVICE) == 0
+    a.copy_from(host)
+    assert np.array_equal(a.numpy(), host)
