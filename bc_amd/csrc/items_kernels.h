@@ -18,6 +18,7 @@ namespace cent {
 
 constexpr int kItemLds = 2048;  // largest item range kept in LDS
 constexpr int kItemBlock = 256;
+constexpr int kBcItemBlock = 1024;  // k_bc_items: more waves per item (its rows are walked one vertex a wave)
 
 struct ItemArgs {
     const int* __restrict__ rp;
@@ -145,17 +146,17 @@ struct BcItemArgs {
 
 constexpr int kItemMaxDepth = 254;
 
-__global__ __launch_bounds__(kItemBlock) void k_bc_items(BcItemArgs a) {
+__global__ __launch_bounds__(kBcItemBlock) void k_bc_items(BcItemArgs a) {
     __shared__ uint64_t s_vis[kItemLds];
     __shared__ uint64_t s_F[2][kItemLds];
-    __shared__ int64_t s_far[kItemBlock / kWave][kWave];
+    __shared__ int64_t s_far[kBcItemBlock / kWave][kWave];
     __shared__ int s_item;
     __shared__ int s_flag;
     __shared__ int s_deep;
     const int tid = threadIdx.x;
     const int lane = lane_id();
     const int wid = tid >> 6;
-    constexpr int kW = kItemBlock / kWave;
+    constexpr int kW = kBcItemBlock / kWave;
     double* __restrict__ sig = a.sig + (size_t)blockIdx.x * kItemLds * kWave;
     double* __restrict__ coef = a.coef + (size_t)blockIdx.x * kItemLds * kWave;
     uint8_t* __restrict__ lvl = a.lvl + (size_t)blockIdx.x * kItemLds * kWave;
@@ -169,8 +170,8 @@ __global__ __launch_bounds__(kItemBlock) void k_bc_items(BcItemArgs a) {
         const int lo = it.x, R = it.y - it.x, s0 = it.z, ns = it.w;
         const uint64_t A = (ns >= 64) ? ~0ull : ((1ull << ns) - 1);
         double* __restrict__ part = a.bcpart + a.poff[itx];
-        for (int i = tid; i < R; i += kItemBlock) { s_vis[i] = 0; s_F[0][i] = 0; s_F[1][i] = 0; part[i] = 0.0; }
-        for (int i = tid; i < R * (kWave / 8); i += kItemBlock) reinterpret_cast<uint64_t*>(lvl)[i] = ~0ull;
+        for (int i = tid; i < R; i += kBcItemBlock) { s_vis[i] = 0; s_F[0][i] = 0; s_F[1][i] = 0; part[i] = 0.0; }
+        for (int i = tid; i < R * (kWave / 8); i += kBcItemBlock) reinterpret_cast<uint64_t*>(lvl)[i] = ~0ull;
         if (tid == 0) s_deep = 0;
         __syncthreads();
         if (tid < ns) {
@@ -252,7 +253,7 @@ __global__ __launch_bounds__(kItemBlock) void k_bc_items(BcItemArgs a) {
                 __syncthreads();
                 break;
             }
-            for (int i = tid; i < R; i += kItemBlock) s_F[cur][i] = 0;
+            for (int i = tid; i < R; i += kBcItemBlock) s_F[cur][i] = 0;
             cur ^= 1;
             __syncthreads();
         }

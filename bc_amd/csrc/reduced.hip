@@ -327,7 +327,7 @@ extern "C" int cent_bc_reduced(cent_ctx* ctx, const cent_reduced* rc, double* bc
         ba.deep = ctx->red_deep.p; ba.ndeep = ctx->red_head.p + 1; ba.overflow = ctx->flags.p + 1;
         hipEvent_t e0;
         TRY(timer_begin(ctx, tm, &e0));
-        k_bc_items<<<grid, kItemBlock, 0, s>>>(ba);
+        k_bc_items<<<grid, kBcItemBlock, 0, s>>>(ba);
         HIPTRY(hipGetLastError());
         TRY(timer_end(ctx, tm, e0, true));
         ctx->stats.launches_fwd++;
