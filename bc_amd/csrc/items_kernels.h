@@ -17,8 +17,10 @@
 namespace cent {
 
 constexpr int kItemLds = 2048;  // largest item range kept in LDS
-constexpr int kItemBlock = 256;
-constexpr int kBcItemBlock = 1024;  // k_bc_items: more waves per item (its rows are walked one vertex a wave)
+// Threads per item workgroup: 16 waves, each walking the item's vertices one
+// at a time (measured: 4 waves took 2.2x (closeness) and 2.5x (BC) as long).
+constexpr int kBcItemBlock = 1024;
+constexpr int kCcItemBlock = 1024;
 
 struct ItemArgs {
     const int* __restrict__ rp;
@@ -31,16 +33,16 @@ struct ItemArgs {
     int64_t* __restrict__ D;             // [np]
 };
 
-__global__ __launch_bounds__(kItemBlock) void k_cc_items(ItemArgs a) {
+__global__ __launch_bounds__(kCcItemBlock) void k_cc_items(ItemArgs a) {
     __shared__ uint64_t s_vis[kItemLds];
     __shared__ uint64_t s_F[2][kItemLds];
-    __shared__ int64_t s_far[kItemBlock / kWave][kWave];
+    __shared__ int64_t s_far[kCcItemBlock / kWave][kWave];
     __shared__ int s_item;
     __shared__ int s_flag;
     const int tid = threadIdx.x;
     const int lane = lane_id();
     const int wid = tid >> 6;
-    constexpr int kW = kItemBlock / kWave;
+    constexpr int kW = kCcItemBlock / kWave;
     for (;;) {
         if (tid == 0) s_item = atomicAdd(a.head, 1);
         __syncthreads();
@@ -49,7 +51,7 @@ __global__ __launch_bounds__(kItemBlock) void k_cc_items(ItemArgs a) {
         const int4 it = a.items[a.order[q]];
         const int lo = it.x, R = it.y - it.x, s0 = it.z, ns = it.w;
         const uint64_t A = (ns >= 64) ? ~0ull : ((1ull << ns) - 1);
-        for (int i = tid; i < R; i += kItemBlock) { s_vis[i] = 0; s_F[0][i] = 0; s_F[1][i] = 0; }
+        for (int i = tid; i < R; i += kCcItemBlock) { s_vis[i] = 0; s_F[0][i] = 0; s_F[1][i] = 0; }
         __syncthreads();
         if (tid < ns) {
             s_F[0][s0 - lo + tid] = 1ull << tid;
@@ -95,7 +97,7 @@ __global__ __launch_bounds__(kItemBlock) void k_cc_items(ItemArgs a) {
             __syncthreads();
             if (!s_flag) break;
             // the buffer read this level is cleared for the level after next
-            for (int i = tid; i < R; i += kItemBlock) s_F[cur][i] = 0;
+            for (int i = tid; i < R; i += kCcItemBlock) s_F[cur][i] = 0;
             cur ^= 1;
             __syncthreads();
         }

@@ -230,7 +230,7 @@ extern "C" int cent_cc_reduced(cent_ctx* ctx, const cent_reduced* rc, int64_t* f
             const int grid = std::min(rd->nsmall, 2048);
             hipEvent_t e0;
             TRY(timer_begin(ctx, tm, &e0));
-            k_cc_items<<<grid, kItemBlock, 0, s>>>(ia);
+            k_cc_items<<<grid, kCcItemBlock, 0, s>>>(ia);
             HIPTRY(hipGetLastError());
             TRY(timer_end(ctx, tm, e0, true));
             ctx->stats.launches_fwd++;
