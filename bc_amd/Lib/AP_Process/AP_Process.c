@@ -42,6 +42,42 @@ void cent_reduced_free(struct cent_reduced* r) {
 typedef struct { int32_t u, v; } edge_t;
 
 
+/* Work items over the packed block graph: consecutive small blocks are
+ * grouped up to 64 sources; a block of more than 64 vertices gives one item
+ * per 64 consecutive sources over its whole range. */
+int cent_reduced_items(struct cent_reduced* r) {
+    const int32_t nblk = r->nblocks;
+    int64_t cap = r->np / 8 + nblk + 16;
+    free(r->items);
+    r->items = (struct cent_item*)malloc((size_t)cap * sizeof(struct cent_item));
+    if (!r->items) return CENT_E_OOM;
+    int32_t ni = 0;
+    int32_t b = 0;
+    while (b < nblk) {
+        const int32_t sz = r->boff[b + 1] - r->boff[b];
+        if (sz > 64) {
+            for (int32_t s0 = r->boff[b]; s0 < r->boff[b + 1]; s0 += 64) {
+                if (ni == cap) break;
+                const int32_t ns = (r->boff[b + 1] - s0) < 64 ? (r->boff[b + 1] - s0) : 64;
+                r->items[ni].lo = r->boff[b]; r->items[ni].hi = r->boff[b + 1];
+                r->items[ni].s0 = s0; r->items[ni].ns = ns;
+                ++ni;
+            }
+            ++b;
+        } else {
+            int32_t e = b, tot = 0;
+            while (e < nblk && tot + (r->boff[e + 1] - r->boff[e]) <= 64) { tot += r->boff[e + 1] - r->boff[e]; ++e; }
+            if (ni == cap) break;
+            r->items[ni].lo = r->boff[b]; r->items[ni].hi = r->boff[e];
+            r->items[ni].s0 = r->boff[b]; r->items[ni].ns = tot;
+            ++ni;
+            b = e;
+        }
+    }
+    r->nitems = ni;
+    return CENT_OK;
+}
+
 int cent_reduced_build(const struct CSR* g, struct cent_reduced** out) {
     if (!g || !out) return CENT_E_INVAL;
     *out = NULL;
@@ -283,36 +319,8 @@ int cent_reduced_build(const struct CSR* g, struct cent_reduced** out) {
         free(cnt);
     }
 
-    /* ---- work items: group consecutive small blocks up to 64 sources */
-    {
-        int64_t cap = r->np / 8 + nblk + 16;
-        r->items = (struct cent_item*)malloc((size_t)cap * sizeof(struct cent_item));
-        if (!r->items) { st = CENT_E_OOM; goto fail; }
-        int32_t ni = 0;
-        int32_t b = 0;
-        while (b < nblk) {
-            const int32_t sz = r->boff[b + 1] - r->boff[b];
-            if (sz > 64) {
-                for (int32_t s0 = r->boff[b]; s0 < r->boff[b + 1]; s0 += 64) {
-                    if (ni == cap) break;
-                    const int32_t ns = (r->boff[b + 1] - s0) < 64 ? (r->boff[b + 1] - s0) : 64;
-                    r->items[ni].lo = r->boff[b]; r->items[ni].hi = r->boff[b + 1];
-                    r->items[ni].s0 = s0; r->items[ni].ns = ns;
-                    ++ni;
-                }
-                ++b;
-            } else {
-                int32_t e = b, tot = 0;
-                while (e < nblk && tot + (r->boff[e + 1] - r->boff[e]) <= 64) { tot += r->boff[e + 1] - r->boff[e]; ++e; }
-                if (ni == cap) break;
-                r->items[ni].lo = r->boff[b]; r->items[ni].hi = r->boff[e];
-                r->items[ni].s0 = r->boff[b]; r->items[ni].ns = tot;
-                ++ni;
-                b = e;
-            }
-        }
-        r->nitems = ni;
-    }
+    /* ---- work items */
+    if ((st = cent_reduced_items(r))) goto fail;
     st = CENT_OK;
 fail:
     free(disc); free(low); free(dpar); free(it); free(fstack); free(estack); free(stamp); free(pblock_of);

@@ -74,6 +74,9 @@ struct cent_reduced {
 int  cent_reduced_build(const struct CSR* g, struct cent_reduced** out);
 void cent_reduced_free(struct cent_reduced* r);
 
+/* Fills r->items / r->nitems from r->boff (both builds share it). */
+int  cent_reduced_items(struct cent_reduced* r);
+
 /* Given D[x] = sum_y d_B(x, y) W(y) for every packed vertex x, fills the
  * farness of every input vertex (assigned, int64), including folded
  * vertices (recovered from their parents) and isolated / fully folded ones. */

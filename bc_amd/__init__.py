@@ -214,6 +214,7 @@ class _RView(C.Structure):
 
 
 _lib.cent_reduce.argtypes = [_P_CSR, C.POINTER(C.c_void_p)]
+_lib.cent_reduce_gpu.argtypes = [C.c_void_p, _P_CSR, C.POINTER(C.c_void_p)]
 _lib.cent_reduced_get_view.argtypes = [C.c_void_p, C.POINTER(_RView)]
 _lib.cent_reduced_farness.argtypes = [C.c_void_p, _I64P, _I64P]
 _lib.cent_reduced_bc.argtypes = [C.c_void_p, _F64P, _F64P]
@@ -223,17 +224,24 @@ CENT_SCHED_PERSISTENT, CENT_SCHED_PER_ITEM = 0, 1
 
 
 class Reduced:
-    """Degree-1 folded, biconnected-block packed form of a graph (host C).
+    """Degree-1 folded, biconnected-block packed form of a graph.
 
     Mirrors the reduction steps of compute_D1_AP_CC_shareBased_DegreeOrder
     (Seq_CC/CC.cpp:1742-1776: D1Folding, AP_detection + AP_Copy_And_Split,
-    rebuildGraph).  Arrays are zero-copy views of the library's buffers.
+    rebuildGraph).  Built by the host C library (cent_reduce), or, with
+    `ctx`, by GPU kernels on that context's device (cent_reduce_gpu); both
+    give the same blocks, weights and results.  Arrays are zero-copy views
+    of the library's buffers.
     """
 
-    def __init__(self, csr: CSR):
+    def __init__(self, csr: CSR, ctx: "Context" = None):
         self._csr = csr
+        self._ctx = ctx
         h = C.c_void_p()
-        _check(_lib.cent_reduce(csr._p, C.byref(h)), "cent_reduce")
+        if ctx is None:
+            _check(_lib.cent_reduce(csr._p, C.byref(h)), "cent_reduce")
+        else:
+            _check(_lib.cent_reduce_gpu(ctx._h, csr._p, C.byref(h)), "cent_reduce_gpu", ctx.last_error())
         self._h = h
         v = _RView()
         _check(_lib.cent_reduced_get_view(h, C.byref(v)), "cent_reduced_get_view")

@@ -351,6 +351,18 @@ typedef struct cent_reduced_view {
 } cent_reduced_view;
 
 int  cent_reduce(const struct CSR* csr, cent_reduced** out);
+/* The same reduction on the GPU of ctx: degree-1 folding by rounds, then
+ * the biconnected split by Tarjan-Vishkin over a BFS spanning forest, then
+ * the packing, all as device kernels; only the result comes back to host
+ * memory.  Replaces the same host preprocessing as cent_reduce (D1Folding,
+ * Lib/D1Process/D1Process.c:5-154; AP_detection / AP_Copy_And_Split /
+ * rebuildGraph, Lib/AP_Process/AP_Process.c:6-1065, called from
+ * Seq_CC/CC.cpp:1742-1776).  The result is a cent_reduced that every
+ * reduced entry point below takes.  Blocks, copies, weights, fold parents
+ * and all results equal cent_reduce's; the order of blocks inside the
+ * packed graph, of copies inside a block, and which vertex of a tree
+ * component survives folding may differ. */
+int  cent_reduce_gpu(cent_ctx* ctx, const struct CSR* csr, cent_reduced** out);
 int  cent_reduced_get_view(const cent_reduced* r, cent_reduced_view* out);
 /* farness[v] (assigned, length n) from D[x] = sum_y d(x, y) * pw[y] over each
  * packed vertex x's block. */
