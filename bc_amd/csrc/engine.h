@@ -65,6 +65,9 @@ struct cent_ctx {
     DevBuf<double> red_part, red_bcp, it_sig, it_coef;
     DevBuf<uint8_t> it_lvl;
     DevBuf<int> red_deep;
+    // device scratch of the graph reduction (cent_reduce_gpu), grown to the
+    // largest request and reused by later calls
+    DevBuf<char> red_arena;
     // source-parallel mode: per-slot arrays
     DevBuf<PsRec> ps_rec;
     DevBuf<int> ps_Q, ps_loff;

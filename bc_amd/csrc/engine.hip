@@ -141,7 +141,7 @@ extern "C" void cent_ctx_destroy(cent_ctx* ctx) {
     release(ctx->red_part); release(ctx->red_bcp); release(ctx->it_sig); release(ctx->it_coef);
     release(ctx->it_lvl); release(ctx->red_deep);
     release(ctx->ps_rec); release(ctx->ps_Q); release(ctx->ps_loff); release(ctx->ps_bcp);
-    release(ctx->ps_depth); release(ctx->lvf); release(ctx->lvstat);
+    release(ctx->ps_depth); release(ctx->lvf); release(ctx->lvstat); release(ctx->red_arena);
     if (ctx->h_flags) (void)hipHostFree(ctx->h_flags);
     if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
     for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
@@ -168,7 +168,7 @@ extern "C" int64_t cent_ctx_workspace_bytes(const cent_ctx* ctx) {
                           ctx->part_cnt.cap + ctx->cnt.cap + ctx->Fd2.cap + ctx->QF.cap + ctx->Adev.cap +
                           ctx->red_D.cap + ctx->red_part.cap + ctx->red_bcp.cap + ctx->it_sig.cap +
                           ctx->it_coef.cap + 2 * ctx->ps_rec.cap + ctx->ps_bcp.cap + ctx->ps_depth.cap) +
-                     ctx->it_lvl.cap +
+                     ctx->it_lvl.cap + ctx->red_arena.cap +
                      4 * (ctx->npred.cap + ctx->hp_cnt.cap + ctx->src.cap + ctx->flags.cap + ctx->ord.cap + ctx->Q.cap +
                           ctx->qoff.cap + ctx->cand.cap + ctx->stamp.cap + ctx->ctl.cap + ctx->red_head.cap +
                           ctx->red_deep.cap + ctx->ps_Q.cap + ctx->ps_loff.cap));
@@ -189,7 +189,7 @@ extern "C" int cent_ctx_trim(cent_ctx* ctx) {
     release(ctx->red_part); release(ctx->red_bcp); release(ctx->it_sig); release(ctx->it_coef);
     release(ctx->it_lvl); release(ctx->red_deep);
     release(ctx->ps_rec); release(ctx->ps_Q); release(ctx->ps_loff); release(ctx->ps_bcp);
-    release(ctx->ps_depth); release(ctx->lvf); release(ctx->lvstat);
+    release(ctx->ps_depth); release(ctx->lvf); release(ctx->lvstat); release(ctx->red_arena);
     if (ctx->h_stage) HIPTRY(hipHostFree(ctx->h_stage));
     ctx->h_stage = nullptr;
     ctx->h_stage_bytes = 0;

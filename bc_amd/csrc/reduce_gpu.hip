@@ -39,6 +39,9 @@
 // cent_reduced_finish_bc, the server) takes either.
 #include <hipcub/hipcub.hpp>
 
+#include <atomic>
+#include <thread>
+
 #include "engine.h"
 
 namespace {
@@ -48,6 +51,8 @@ constexpr int kG = 16;                 // lanes per vertex in the row-scanning k
 constexpr int kTB = 256;               // threads per workgroup
 constexpr int kLevelBlocks = 512;      // grid of the level kernels whose width the host does not know
 constexpr int kRunLevels = 16;         // BFS levels / fold rounds enqueued between host checks
+constexpr int kChunk = 256;            // row entries per work item of the row-scanning kernels
+constexpr int kHubRow = 1024;          // BFS: rows longer than this are scanned by a whole workgroup
 
 __device__ __forceinline__ int ld_rel(const int* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -85,22 +90,126 @@ __device__ void uf_union(int* p, int a, int b) {
 __device__ __forceinline__ int gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
 __device__ __forceinline__ int gstride() { return gridDim.x * blockDim.x; }
 
+// Per-key aggregation of atomics over a wavefront: the lanes with `ok` are
+// peeled one key at a time (the key of the lowest pending lane), and one
+// atomic per key carries the whole wave's contribution.  Keys repeat within
+// a wave almost always here (one giant component / block, a hub's children),
+// where per-lane atomics on one address serialise.  Every lane of the wave
+// must reach the call (ok = false where there is nothing to add).
+template <class T>
+__device__ __forceinline__ void wave_count(T* p, int key, bool ok) {
+    uint64_t pend = ballot(ok);
+    while (pend) {
+        const int lead = __builtin_ctzll(pend);
+        const int k0 = __shfl(key, lead);
+        const uint64_t m = ballot(ok && key == k0) & pend;
+        if (lane_id() == lead) {
+            if constexpr (sizeof(T) == 8) atomicAdd((unsigned long long*)(p + k0), (unsigned long long)__builtin_popcountll(m));
+            else atomicAdd(p + k0, (T)__builtin_popcountll(m));
+        }
+        pend &= ~m;
+    }
+}
+
+template <class T>
+__device__ __forceinline__ void wave_add(T* p, int key, T val, bool ok) {
+    uint64_t pend = ballot(ok);
+    while (pend) {
+        const int lead = __builtin_ctzll(pend);
+        const int k0 = __shfl(key, lead);
+        const bool mine = ok && key == k0 && ((pend >> lane_id()) & 1);
+        const uint64_t m = ballot(mine);
+        T sum = mine ? val : T(0);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+        if (lane_id() == lead) {
+            if constexpr (sizeof(T) == 8) atomicAdd((unsigned long long*)(p + k0), (unsigned long long)sum);
+            else atomicAdd(p + k0, sum);
+        }
+        pend &= ~m;
+    }
+}
+
+template <bool kMin>
+__device__ __forceinline__ void wave_minmax(int* p, int key, int val, bool ok) {
+    uint64_t pend = ballot(ok);
+    while (pend) {
+        const int lead = __builtin_ctzll(pend);
+        const int k0 = __shfl(key, lead);
+        const bool mine = ok && key == k0 && ((pend >> lane_id()) & 1);
+        const uint64_t m = ballot(mine);
+        int r = mine ? val : (kMin ? INT_MAX : INT_MIN);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) r = kMin ? min(r, __shfl_xor(r, o)) : max(r, __shfl_xor(r, o));
+        if (lane_id() == lead) {
+            if (kMin) atomicMin(p + k0, r);
+            else atomicMax(p + k0, r);
+        }
+        pend &= ~m;
+    }
+}
+
+// Appends val for every lane with ok at Q[base + (*cnt)++], one atomic per
+// wave.  Lanes that do not reach the call are simply not counted.
+__device__ __forceinline__ void wave_append(int* Q, int base, int* cnt, int val, bool ok) {
+    const uint64_t m = ballot(ok);
+    if (!m) return;
+    const int lead = __builtin_ctzll(m);
+    int pos = 0;
+    if (lane_id() == lead) pos = atomicAdd(cnt, (int)__builtin_popcountll(m));
+    pos = __shfl(pos, lead);
+    if (ok) Q[base + pos + (int)__builtin_popcountll(m & ((1ull << lane_id()) - 1ull))] = val;
+}
+
+// Rows split into work items of at most kChunk entries (every vertex has at
+// least one item), so a hub row is spread over many lane groups instead of
+// holding one group for its whole length.  ioff[i] = first item of list
+// entry i (list == nullptr: entry i is vertex i).
+__global__ void k_chunk_count(int cnt, const int* list, const int* rp, int* nck) {
+    const int i = gtid();
+    if (i > cnt) return;
+    if (i == cnt) { nck[i] = 0; return; }
+    const int v = list ? list[i] : i;
+    const int d = rp[v + 1] - rp[v];
+    nck[i] = d > 0 ? (d + kChunk - 1) / kChunk : 1;
+}
+
+struct RowItem {
+    int v, j0, j1;
+    bool first;  // the item that starts the row
+};
+
+__device__ __forceinline__ RowItem row_item(int g, const int* ioff, int cnt, const int* list, const int* rp) {
+    int lo = 0, hi = cnt;  // ioff[lo] <= g < ioff[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (ioff[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    RowItem it;
+    it.v = list ? list[lo] : lo;
+    const int k = g - ioff[lo];
+    it.j0 = rp[it.v] + k * kChunk;
+    it.j1 = min(rp[it.v + 1], it.j0 + kChunk);
+    it.first = (k == 0);
+    return it;
+}
+
 // ------------------------------------------------------------ folding
 __global__ void k_d1_init(const int* rp, int n, int* rdeg, int* fr, int* par, int64_t* w, int64_t* ff,
                           int* inq, int* Q, int* qc) {
     const int v = gtid();
-    if (v >= n) return;
-    const int d = rp[v + 1] - rp[v];
-    rdeg[v] = d;
-    fr[v] = kUnfolded;
-    par[v] = -1;
-    w[v] = 1;
-    ff[v] = 0;
-    inq[v] = -1;
-    if (d == 1) {
-        inq[v] = 0;
-        Q[atomicAdd(qc, 1)] = v;
+    int d = 0;
+    if (v < n) {
+        d = rp[v + 1] - rp[v];
+        rdeg[v] = d;
+        fr[v] = kUnfolded;
+        par[v] = -1;
+        w[v] = 1;
+        ff[v] = 0;
+        inq[v] = d == 1 ? 0 : -1;
     }
+    wave_append(Q, 0, qc, v, v < n && d == 1);
 }
 
 // Round r over the leaves Qin[0 .. qc[r % 3]); new leaves go to Qout with
@@ -146,13 +255,14 @@ __global__ void k_fill_i32(int n, int* a, int val) {
     if (v < n) a[v] = val;
 }
 
-// one kG-lane group per vertex; each undirected edge once (y > v)
-__global__ void k_cc_union(const int* rp, const int* col, int n, int* uf) {
+// one kG-lane group per row item; each undirected edge once (y > v)
+__global__ void k_cc_union(int ni, const int* ioff, const int* rp, const int* col, int n, int* uf) {
     const int g = gtid() / kG, lane = threadIdx.x & (kG - 1);
-    if (g >= n) return;
-    for (int j = rp[g] + lane; j < rp[g + 1]; j += kG) {
+    if (g >= ni) return;
+    const RowItem it = row_item(g, ioff, n, nullptr, rp);
+    for (int j = it.j0 + lane; j < it.j1; j += kG) {
         const int y = col[j];
-        if (y > g) uf_union(uf, g, y);
+        if (y > it.v) uf_union(uf, it.v, y);
     }
 }
 
@@ -170,33 +280,50 @@ __global__ void k_uf_flat(int n, int* uf, int* flag, int* root) {
 // vcomp holds each vertex's root on entry, its component number on exit
 __global__ void k_comp_assign(int n, const int* cid, int* vcomp, int64_t* comp_w) {
     const int v = gtid();
-    if (v >= n) return;
-    const int c = cid[vcomp[v]];
-    vcomp[v] = c;
-    atomicAdd((unsigned long long*)(comp_w + c), 1ull);
+    const bool ok = v < n;
+    int c = 0;
+    if (ok) {
+        c = cid[vcomp[v]];
+        vcomp[v] = c;
+    }
+    wave_count(comp_w, c, ok);
 }
 
 // ------------------------------------------------------------ BFS forest
+// smallest unfolded vertex of nonzero degree per component: a wave's lanes
+// hold increasing ids, so the lowest lane of a key holds its smallest id
 __global__ void k_croot(int n, const int* fr, const int* rdeg, const int* vcomp, int* croot) {
     const int v = gtid();
-    if (v >= n) return;
-    if (fr[v] == kUnfolded && rdeg[v] > 0) atomicMin(croot + vcomp[v], v);
+    const bool ok = v < n && fr[v] == kUnfolded && rdeg[v] > 0;
+    const int c = ok ? vcomp[v] : 0;
+    uint64_t pend = ballot(ok);
+    while (pend) {
+        const int lead = __builtin_ctzll(pend);
+        const int k0 = __shfl(c, lead);
+        const uint64_t m = ballot(ok && c == k0) & pend;
+        if (lane_id() == lead) atomicMin(croot + k0, v);
+        pend &= ~m;
+    }
 }
 
-__global__ void k_bfs_seed(int ncomp, const int* croot, int* depth, int* tpar, int* pre, int* Qb, int* lvcnt) {
+__global__ void k_bfs_seed(int ncomp, const int* croot, int* depth, int* tpar, int* pre, int* cursor, int* Qb,
+                           int* lvcnt) {
     const int c = gtid();
-    if (c >= ncomp) return;
-    const int r = croot[c];
-    if (r == kUnfolded) return;
-    depth[r] = 0;
-    tpar[r] = -1;
-    pre[r] = 0;
-    Qb[atomicAdd(lvcnt, 1)] = r;
+    const int r = c < ncomp ? croot[c] : kUnfolded;
+    const bool ok = r != kUnfolded;
+    if (ok) {
+        depth[r] = 0;
+        tpar[r] = -1;
+        pre[r] = 0;
+        cursor[r] = 1;
+    }
+    wave_append(Qb, 0, lvcnt, r, ok);
 }
 
 // Level L: the frontier is Qb[loff .. loff + lvcnt[L]), loff = sum of the
 // earlier levels' counts (kept in loff[L], written by level L - 1).  Level
-// L + 1 is appended after it.
+// L + 1 is appended after it.  Rows longer than kHubRow are left to
+// k_bfs_hubs.
 __global__ void k_bfs_level(int L, const int* rp, const int* col, const int* fr, int* Qb, int* loff, int* lvcnt,
                             int* depth, int* tpar) {
     const int lo = loff[L], cnt = lvcnt[L];
@@ -205,15 +332,42 @@ __global__ void k_bfs_level(int L, const int* rp, const int* col, const int* fr,
     const int lane = threadIdx.x & (kG - 1);
     for (int i = gtid() / kG; i < cnt; i += gstride() / kG) {
         const int u = Qb[lo + i];
-        for (int j = rp[u] + lane; j < rp[u + 1]; j += kG) {
+        const int end = rp[u + 1];
+        if (end - rp[u] > kHubRow) continue;
+        for (int j = rp[u] + lane; j < end; j += kG) {
             const int y = col[j];
-            if (fr[y] != kUnfolded || ld_rel(depth + y) >= 0) continue;
-            if (atomicCAS(depth + y, -1, L + 1) == -1) {
-                tpar[y] = u;
-                Qb[base + atomicAdd(lvcnt + L + 1, 1)] = y;
-            }
+            const bool won = fr[y] == kUnfolded && ld_rel(depth + y) < 0 && atomicCAS(depth + y, -1, L + 1) == -1;
+            if (won) tpar[y] = u;
+            wave_append(Qb, base, lvcnt + L + 1, y, won);
         }
     }
+}
+
+// the long rows of level L's frontier: one workgroup per hub of the graph,
+// idle unless the hub is on level L
+__global__ void k_bfs_hubs(int L, const int* hubs, const int* rp, const int* col, const int* fr, int* Qb,
+                           const int* loff, int* lvcnt, int* depth, int* tpar) {
+    const int u = hubs[blockIdx.x];
+    if (depth[u] != L) return;
+    const int base = loff[L] + lvcnt[L];
+    const int end = rp[u + 1];
+    for (int j = rp[u] + (int)threadIdx.x; j < end; j += (int)blockDim.x) {
+        const int y = col[j];
+        const bool won = fr[y] == kUnfolded && ld_rel(depth + y) < 0 && atomicCAS(depth + y, -1, L + 1) == -1;
+        if (won) tpar[y] = u;
+        wave_append(Qb, base, lvcnt + L + 1, y, won);
+    }
+}
+
+__global__ void k_hub_flag(int n, const int* rp, int* flag) {
+    const int v = gtid();
+    if (v < n) flag[v] = rp[v + 1] - rp[v] > kHubRow;
+    else if (v == n) flag[v] = 0;
+}
+
+__global__ void k_hub_list(int n, const int* flag, const int* off, int* hubs) {
+    const int v = gtid();
+    if (v < n && flag[v]) hubs[off[v]] = v;
 }
 
 // ------------------------------------------------------------ Tarjan–Vishkin
@@ -228,47 +382,75 @@ __global__ void k_tv_init(int total, const int* Qb, const int64_t* w, int* size,
 // children of level L + 1 are final: add them into their parents
 __global__ void k_up_size(int lo, int cnt, const int* Qb, const int* tpar, int* size, int64_t* wsub) {
     const int i = gtid();
-    if (i >= cnt) return;
-    const int v = Qb[lo + i], p = tpar[v];
-    atomicAdd(size + p, size[v]);
-    add64(wsub + p, wsub[v]);
+    const bool ok = i < cnt;
+    int p = 0, sz = 0;
+    int64_t ws = 0;
+    if (ok) {
+        const int v = Qb[lo + i];
+        p = tpar[v];
+        sz = size[v];
+        ws = wsub[v];
+    }
+    wave_add(size, p, sz, ok);
+    wave_add(wsub, p, ws, ok);
 }
 
-// preorder of level L's children: a child's number is its parent's + 1 + the
-// sizes of the children before it in the parent's row
-__global__ void k_pre_level(int lo, int cnt, const int* Qb, const int* rp, const int* col, const int* tpar,
-                            const int* size, int* pre) {
-    const int g = gtid() / kG, lane = threadIdx.x & (kG - 1);
-    if (g >= cnt) return;
-    const int v = Qb[lo + g];
-    int base = pre[v] + 1;
-    for (int j0 = rp[v]; j0 < rp[v + 1]; j0 += kG) {
-        const int j = j0 + lane;
-        int y = -1, s = 0;
-        if (j < rp[v + 1]) {
-            y = col[j];
-            if (tpar[y] == v) s = size[y];
-            else y = -1;
-        }
-        int incl = s;
+// preorder of level L (>= 1): each vertex takes the next free range of its
+// parent's, through the parent's cursor (one atomic per parent per wave), so
+// every subtree is one interval [pre, pre + size); children order is free
+__global__ void k_pre_level(int lo, int cnt, const int* Qb, const int* tpar, const int* size, int* pre,
+                            int* cursor) {
+    const int i = gtid();
+    const bool ok = i < cnt;
+    int v = 0, p = 0, sz = 0;
+    if (ok) {
+        v = Qb[lo + i];
+        p = tpar[v];
+        sz = size[v];
+    }
+    uint64_t pend = ballot(ok);
+    while (pend) {
+        const int lead = __builtin_ctzll(pend);
+        const int k0 = __shfl(p, lead);
+        const bool mine = ok && p == k0 && ((pend >> lane_id()) & 1);
+        const uint64_t m = ballot(mine);
+        const int val = mine ? sz : 0;
+        int incl = val;
 #pragma unroll
-        for (int o = 1; o < kG; o <<= 1) {
-            const int t = __shfl_up(incl, o, kG);
-            if (lane >= o) incl += t;
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int t = __shfl_up(incl, o);
+            if (lane_id() >= o) incl += t;
         }
-        if (y >= 0) pre[y] = base + incl - s;
-        base += __shfl(incl, kG - 1, kG);
+        const int tot = __shfl(incl, kWave - 1);
+        int base = 0;
+        if (lane_id() == lead) base = atomicAdd(cursor + k0, tot);
+        base = __shfl(base, lead);
+        if (mine) {
+            const int pv = base + incl - val;
+            pre[v] = pv;
+            cursor[v] = pv + 1;
+        }
+        pend &= ~m;
     }
 }
 
-// low/high of a vertex's own non-tree edges (and its own number)
-__global__ void k_lohi_local(int total, const int* Qb, const int* rp, const int* col, const int* depth,
-                             const int* tpar, const int* pre, int* lo, int* hi) {
+__global__ void k_lohi_init(int total, const int* Qb, const int* pre, int* lo, int* hi) {
+    const int i = gtid();
+    if (i >= total) return;
+    const int x = Qb[i];
+    lo[x] = pre[x];
+    hi[x] = pre[x];
+}
+
+// low/high of a vertex's own non-tree edges, one row item per group
+__global__ void k_lohi_local(int ni, const int* ioff, int total, const int* Qb, const int* rp, const int* col,
+                             const int* depth, const int* tpar, const int* pre, int* lo, int* hi) {
     const int g = gtid() / kG, lane = threadIdx.x & (kG - 1);
-    if (g >= total) return;
-    const int x = Qb[g], px = tpar[x];
+    if (g >= ni) return;
+    const RowItem it = row_item(g, ioff, total, Qb, rp);
+    const int x = it.v, px = tpar[x];
     int mn = pre[x], mx = pre[x];
-    for (int j = rp[x] + lane; j < rp[x + 1]; j += kG) {
+    for (int j = it.j0 + lane; j < it.j1; j += kG) {
         const int y = col[j];
         if (depth[y] < 0 || y == px || tpar[y] == x) continue;
         const int py = pre[y];
@@ -280,30 +462,42 @@ __global__ void k_lohi_local(int total, const int* Qb, const int* rp, const int*
         mn = min(mn, __shfl_xor(mn, o, kG));
         mx = max(mx, __shfl_xor(mx, o, kG));
     }
-    if (lane == 0) { lo[x] = mn; hi[x] = mx; }
+    if (lane == 0) {
+        if (mn < pre[x]) atomicMin(lo + x, mn);
+        if (mx > pre[x]) atomicMax(hi + x, mx);
+    }
 }
 
 __global__ void k_up_lohi(int lo0, int cnt, const int* Qb, const int* tpar, int* lo, int* hi) {
     const int i = gtid();
-    if (i >= cnt) return;
-    const int v = Qb[lo0 + i], p = tpar[v];
-    atomicMin(lo + p, lo[v]);
-    atomicMax(hi + p, hi[v]);
+    const bool ok = i < cnt;
+    int p = 0, l = 0, h = 0;
+    if (ok) {
+        const int v = Qb[lo0 + i];
+        p = tpar[v];
+        l = lo[v];
+        h = hi[v];
+    }
+    wave_minmax<true>(lo, p, l, ok);
+    wave_minmax<false>(hi, p, h, ok);
 }
 
-// union-find over tree edges (the edge into x is named x)
-__global__ void k_tv_union(int total, const int* Qb, const int* rp, const int* col, const int* depth,
-                           const int* tpar, const int* pre, const int* size, const int* lo, const int* hi, int* au) {
+// union-find over tree edges (the edge into x is named x), one row item per group
+__global__ void k_tv_union(int ni, const int* ioff, int total, const int* Qb, const int* rp, const int* col,
+                           const int* depth, const int* tpar, const int* pre, const int* size, const int* lo,
+                           const int* hi, int* au) {
     const int g = gtid() / kG, lane = threadIdx.x & (kG - 1);
-    if (g >= total) return;
-    const int x = Qb[g], px = tpar[x];
+    if (g >= ni) return;
+    const RowItem it = row_item(g, ioff, total, Qb, rp);
+    const int x = it.v, px = tpar[x];
     if (px < 0) return;  // a root: every edge of it is a tree edge
-    for (int j = rp[x] + lane; j < rp[x + 1]; j += kG) {
+    for (int j = it.j0 + lane; j < it.j1; j += kG) {
         const int y = col[j];
         if (y <= x || depth[y] < 0 || y == px || tpar[y] == x) continue;
         uf_union(au, x, y);
     }
-    if (lane == 0 && tpar[px] >= 0 && (lo[x] < pre[px] || hi[x] >= pre[px] + size[px])) uf_union(au, x, px);
+    if (it.first && lane == 0 && tpar[px] >= 0 && (lo[x] < pre[px] || hi[x] >= pre[px] + size[px]))
+        uf_union(au, x, px);
 }
 
 // blk[x] = block label of the tree edge into x; per block: member count,
@@ -312,18 +506,26 @@ __global__ void k_tv_union(int total, const int* Qb, const int* rp, const int* c
 __global__ void k_blk_info(int total, const int* Qb, const int* tpar, int* au, int* blk, int* memb,
                            int* head, const int64_t* wsub, int64_t* extra, int64_t* subw) {
     const int i = gtid();
-    if (i >= total) return;
-    const int x = Qb[i], p = tpar[x];
-    if (p < 0) return;
-    const int B = uf_find(au, x);
-    blk[x] = B;
-    atomicAdd(memb + B, 1);
-    const bool top = tpar[p] < 0 || uf_find(au, p) != B;
-    if (top) {
-        head[B] = p;
-        add64(extra + p, wsub[x]);
-        add64(subw + B, wsub[x]);
+    bool mem = false, top = false;
+    int p = 0, B = 0;
+    int64_t ws = 0;
+    if (i < total) {
+        const int x = Qb[i];
+        p = tpar[x];
+        if (p >= 0) {
+            mem = true;
+            B = uf_find(au, x);
+            blk[x] = B;
+            top = tpar[p] < 0 || uf_find(au, p) != B;
+            if (top) {
+                head[B] = p;
+                ws = wsub[x];
+            }
+        }
     }
+    wave_count(memb, B, mem);
+    wave_add(subw, B, ws, top);
+    wave_add(extra, p, ws, top);
 }
 
 __global__ void k_rootblk(int total, const int* Qb, const int* tpar, const int* blk, const int* head,
@@ -369,14 +571,15 @@ __device__ __forceinline__ int edge_block(int x, int y, const int* tpar, const i
 
 // block degree of every copy: x's member copy (bdm[x]) or the head copy of
 // block E (bdh[E]); lanes keep a running count per block to batch atomics
-__global__ void k_bdeg(int total, const int* Qb, const int* rp, const int* col, const int* depth, const int* tpar,
-                       const int* blk, int* bdm, int* bdh) {
+__global__ void k_bdeg(int ni, const int* ioff, int total, const int* Qb, const int* rp, const int* col,
+                       const int* depth, const int* tpar, const int* blk, int* bdm, int* bdh) {
     const int g = gtid() / kG, lane = threadIdx.x & (kG - 1);
-    if (g >= total) return;
-    const int x = Qb[g];
+    if (g >= ni) return;
+    const RowItem it = row_item(g, ioff, total, Qb, rp);
+    const int x = it.v;
     const int bx = tpar[x] >= 0 ? blk[x] : -1;
     int nm = 0, curE = -1, curc = 0;
-    for (int j = rp[x] + lane; j < rp[x + 1]; j += kG) {
+    for (int j = it.j0 + lane; j < it.j1; j += kG) {
         const int y = col[j];
         if (depth[y] < 0) continue;
         const int E = edge_block(x, y, tpar, blk);
@@ -391,7 +594,7 @@ __global__ void k_bdeg(int total, const int* Qb, const int* rp, const int* col, 
     if (curc) atomicAdd(bdh + curE, curc);
 #pragma unroll
     for (int o = kG / 2; o > 0; o >>= 1) nm += __shfl_xor(nm, o, kG);
-    if (lane == 0) bdm[x] = nm;
+    if (lane == 0 && nm) atomicAdd(bdm + x, nm);
 }
 
 // copies: members (value x < n) and head copies (value n + block rank);
@@ -439,22 +642,47 @@ __global__ void k_slots(int np, int n, const uint64_t* skeys, const int* svals, 
     }
 }
 
-// row entries (a << 32 | c) of every packed edge, a's row at prp[a] + k
-__global__ void k_fill_rows(int total, const int* Qb, const int* rp, const int* col, const int* depth,
-                            const int* tpar, const int* blk, const int* rank, const int* mslot, const int* hslot,
-                            const int* prp, int* cur, uint64_t* ent) {
+// row entries (a << 32 | c) of every packed edge, a's row at prp[a] + k.
+// Row a (a copy of x) gets entries from x's own row only, so only x's group
+// writes it; within the group, lanes with the same row take consecutive
+// places under one atomic.
+__global__ void k_fill_rows(int ni, const int* ioff, int total, const int* Qb, const int* rp, const int* col,
+                            const int* depth, const int* tpar, const int* blk, const int* rank, const int* mslot,
+                            const int* hslot, const int* prp, int* cur, uint64_t* ent) {
     const int g = gtid() / kG, lane = threadIdx.x & (kG - 1);
-    if (g >= total) return;
-    const int x = Qb[g];
+    if (g >= ni) return;  // whole groups
+    const int gbase = lane_id() & ~(kG - 1);
+    const RowItem it = row_item(g, ioff, total, Qb, rp);
+    const int x = it.v;
     const int bx = tpar[x] >= 0 ? blk[x] : -1;
-    for (int j = rp[x] + lane; j < rp[x + 1]; j += kG) {
-        const int y = col[j];
-        if (depth[y] < 0) continue;
-        const int E = edge_block(x, y, tpar, blk);
-        const int hs = hslot[rank[E]];
-        const int a = (E == bx) ? mslot[x] : hs;
-        const int c = (tpar[y] >= 0 && blk[y] == E) ? mslot[y] : hs;
-        ent[prp[a] + atomicAdd(cur + a, 1)] = ((uint64_t)(uint32_t)a << 32) | (uint32_t)c;
+    const int end = it.j1;
+    for (int j0 = it.j0; j0 < end; j0 += kG) {
+        const int j = j0 + lane;
+        bool ok = false;
+        int a = 0, c = 0;
+        if (j < end) {
+            const int y = col[j];
+            if (depth[y] >= 0) {
+                ok = true;
+                const int E = edge_block(x, y, tpar, blk);
+                const int hs = hslot[rank[E]];
+                a = (E == bx) ? mslot[x] : hs;
+                c = (tpar[y] >= 0 && blk[y] == E) ? mslot[y] : hs;
+            }
+        }
+        uint32_t pend = (uint32_t)(ballot(ok) >> gbase) & 0xffffu;
+        while (pend) {
+            const int lead = __builtin_ctz(pend);
+            const int a0 = __shfl(a, lead, kG);
+            const uint32_t m = (uint32_t)(ballot(ok && a == a0) >> gbase) & pend;
+            int base = 0;
+            if (lane == lead) base = atomicAdd(cur + a0, __builtin_popcount(m));
+            base = __shfl(base, lead, kG);
+            if ((m >> lane) & 1u)
+                ent[prp[a0] + base + __builtin_popcount(m & ((1u << lane) - 1u))] =
+                    ((uint64_t)(uint32_t)a0 << 32) | (uint32_t)c;
+            pend &= ~m;
+        }
     }
 }
 
@@ -497,21 +725,34 @@ inline unsigned blocks_for(int64_t threads) {
     return (unsigned)std::max<int64_t>(1, (threads + kTB - 1) / kTB);
 }
 
-// Device buffers of one reduction, freed together once the stream is idle.
+// Device buffers of one reduction: bump-allocated from the context's arena;
+// what does not fit is allocated on its own and freed at the end, and the
+// arena grows to the whole request for the next call.
 struct Scratch {
     cent_ctx* ctx;
-    std::vector<void*> ptrs;
+    size_t used = 0, want = 0;
+    std::vector<void*> spill;
     explicit Scratch(cent_ctx* c) : ctx(c) {}
     ~Scratch() {
         (void)hipStreamSynchronize(ctx->stream);
-        for (void* p : ptrs) (void)hipFree(p);
+        for (void* p : spill) (void)hipFree(p);
+        if (want > ctx->red_arena.cap) {
+            release(ctx->red_arena);
+            (void)ensure_exact(ctx, ctx->red_arena, want);
+        }
     }
     template <class T>
     int get(T** out, size_t n) {
-        *out = nullptr;
+        const size_t bytes = ((n > 0 ? n : 1) * sizeof(T) + 255) & ~size_t(255);
+        want += bytes;
+        if (used + bytes <= ctx->red_arena.cap) {
+            *out = (T*)(ctx->red_arena.p + used);
+            used += bytes;
+            return CENT_OK;
+        }
         void* p = nullptr;
-        HIPTRY(hipMalloc(&p, (n > 0 ? n : 1) * sizeof(T)));
-        ptrs.push_back(p);
+        HIPTRY(hipMalloc(&p, bytes));
+        spill.push_back(p);
         *out = (T*)p;
         return CENT_OK;
     }
@@ -520,15 +761,7 @@ struct Scratch {
     size_t tmp_bytes = 0;
     int tmp_at_least(size_t need) {
         if (need <= tmp_bytes) return CENT_OK;
-        HIPTRY(hipStreamSynchronize(ctx->stream));
-        if (tmp) {
-            for (auto& p : ptrs)
-                if (p == tmp) p = nullptr;
-            HIPTRY(hipFree(tmp));
-        }
-        tmp = nullptr;
-        tmp_bytes = 0;
-        TRY(get(&tmp, need));
+        TRY(get(&tmp, need));  // a larger block; the smaller one stays unused until the end
         tmp_bytes = need;
         return CENT_OK;
     }
@@ -555,13 +788,63 @@ struct Scratch {
     }
 };
 
-template <class T>
-int to_host(cent_ctx* ctx, T** dst, const T* src, size_t n) {
-    *dst = (T*)malloc((n > 0 ? n : 1) * sizeof(T));
-    if (!*dst) return CENT_E_OOM;
-    if (n) HIPTRY(hipMemcpyAsync(*dst, src, n * sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
-    return CENT_OK;
-}
+// Results to host memory: every array is copied into the context's pinned
+// staging buffer at full link rate, then into its malloc'd destination by
+// several host threads (a device-to-pageable copy runs through the
+// runtime's small staging chunks at a fraction of the link rate).
+struct CopyBack {
+    cent_ctx* ctx;
+    struct Item {
+        void* dst;
+        const void* src;
+        size_t bytes;
+    };
+    std::vector<Item> items;
+    template <class T>
+    int add(T** dst, const T* src, size_t n) {
+        *dst = (T*)malloc((n > 0 ? n : 1) * sizeof(T));
+        if (!*dst) return CENT_E_OOM;
+        if (n) items.push_back({*dst, src, n * sizeof(T)});
+        return CENT_OK;
+    }
+    int flush() {
+        size_t total = 0;
+        for (const Item& it : items) total += (it.bytes + 255) & ~size_t(255);
+        if (total == 0) return CENT_OK;
+        TRY(ensure_stage(ctx, total));
+        char* stage = (char*)ctx->h_stage;
+        size_t off = 0;
+        std::vector<size_t> at;
+        for (const Item& it : items) {
+            HIPTRY(hipMemcpyAsync(stage + off, it.src, it.bytes, hipMemcpyDeviceToHost, ctx->stream));
+            at.push_back(off);
+            off += (it.bytes + 255) & ~size_t(255);
+        }
+        HIPTRY(hipStreamSynchronize(ctx->stream));
+        // 8 MB pieces shared out to the threads
+        constexpr size_t kPiece = size_t(8) << 20;
+        struct Piece {
+            char* dst;
+            const char* src;
+            size_t bytes;
+        };
+        std::vector<Piece> pieces;
+        for (size_t i = 0; i < items.size(); ++i)
+            for (size_t o = 0; o < items[i].bytes; o += kPiece)
+                pieces.push_back({(char*)items[i].dst + o, stage + at[i] + o, std::min(kPiece, items[i].bytes - o)});
+        const unsigned nt = (unsigned)std::min<size_t>(pieces.size(), 8);
+        std::atomic<size_t> next{0};
+        auto work = [&]() {
+            for (size_t k = next++; k < pieces.size(); k = next++) memcpy(pieces[k].dst, pieces[k].src, pieces[k].bytes);
+        };
+        std::vector<std::thread> th;
+        for (unsigned t = 1; t < nt; ++t) th.emplace_back(work);
+        work();
+        for (auto& t : th) t.join();
+        items.clear();
+        return CENT_OK;
+    }
+};
 
 int read_int(cent_ctx* ctx, const int* p, int* out) {
     HIPTRY(hipMemcpyAsync(ctx->h_flags, p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -581,13 +864,13 @@ int reduce_gpu(cent_ctx* ctx, const struct CSR* g, cent_reduced* r) {
     const int n = g->csrVSize - 1;
     const int64_t nnz = g->csrESize;
     Scratch S(ctx);
+    CopyBack back{ctx, {}};
     int *rp, *col;
     TRY(S.get(&rp, (size_t)n + 1));
     TRY(S.get(&col, (size_t)nnz));
     HIPTRY(hipMemcpyAsync(rp, g->csrV, ((size_t)n + 1) * sizeof(int), hipMemcpyHostToDevice, s));
     if (nnz) HIPTRY(hipMemcpyAsync(col, g->csrE, (size_t)nnz * sizeof(int), hipMemcpyHostToDevice, s));
     const unsigned gn = blocks_for(n);
-    const unsigned gnG = blocks_for((int64_t)n * kG);
 
     // ---- 1. folding
     int *rdeg, *fr, *par, *inq, *Q0, *Q1, *qc, *order, *ordc;
@@ -618,8 +901,17 @@ int reduce_gpu(cent_ctx* ctx, const struct CSR* g, cent_reduced* r) {
     int64_t* comp_w;
     TRY(S.get(&uf, n)); TRY(S.get(&flag, (size_t)n + 1)); TRY(S.get(&cid, (size_t)n + 1)); TRY(S.get(&vcomp, n));
     TRY(S.get(&comp_w, n));
+    // row items over all vertices
+    int *nck, *ioff;
+    TRY(S.get(&nck, (size_t)n + 1));
+    TRY(S.get(&ioff, (size_t)n + 1));
+    k_chunk_count<<<blocks_for((int64_t)n + 1), kTB, 0, s>>>(n, nullptr, rp, nck);
+    TRY(S.scan(nck, ioff, n + 1));
+    int ni_all = 0;
+    TRY(read_int(ctx, ioff + n, &ni_all));
+    if ((int64_t)ni_all * kG > (int64_t)INT32_MAX) return CENT_E_INVAL;  // int thread ids
     k_iota<<<gn, kTB, 0, s>>>(n, uf);
-    k_cc_union<<<gnG, kTB, 0, s>>>(rp, col, n, uf);
+    k_cc_union<<<blocks_for((int64_t)ni_all * kG), kTB, 0, s>>>(ni_all, ioff, rp, col, n, uf);
     k_uf_flat<<<gn, kTB, 0, s>>>(n, uf, flag, vcomp);
     HIPTRY(hipMemsetAsync(flag + n, 0, sizeof(int), s));
     TRY(S.scan(flag, cid, n + 1));
@@ -629,9 +921,10 @@ int reduce_gpu(cent_ctx* ctx, const struct CSR* g, cent_reduced* r) {
     k_comp_assign<<<gn, kTB, 0, s>>>(n, cid, vcomp, comp_w);
 
     // ---- 3. BFS forest of the folded graph
-    int *croot, *depth, *tpar, *pre, *Qb, *loff, *lvcnt;
+    int *croot, *depth, *tpar, *pre, *cursor, *Qb, *loff, *lvcnt;
     const size_t nlv = (size_t)n + 2 + kRunLevels;
     TRY(S.get(&croot, (size_t)ncomp)); TRY(S.get(&depth, n)); TRY(S.get(&tpar, n)); TRY(S.get(&pre, n));
+    TRY(S.get(&cursor, n));
     TRY(S.get(&Qb, n)); TRY(S.get(&loff, nlv)); TRY(S.get(&lvcnt, nlv));
     k_fill_i32<<<blocks_for(ncomp), kTB, 0, s>>>(ncomp, croot, kUnfolded);
     k_fill_i32<<<gn, kTB, 0, s>>>(n, depth, -1);
@@ -639,14 +932,25 @@ int reduce_gpu(cent_ctx* ctx, const struct CSR* g, cent_reduced* r) {
     HIPTRY(hipMemsetAsync(loff, 0, nlv * sizeof(int), s));
     HIPTRY(hipMemsetAsync(lvcnt, 0, nlv * sizeof(int), s));
     k_croot<<<gn, kTB, 0, s>>>(n, fr, rdeg, vcomp, croot);
-    k_bfs_seed<<<blocks_for(ncomp), kTB, 0, s>>>(ncomp, croot, depth, tpar, pre, Qb, lvcnt);
+    k_bfs_seed<<<blocks_for(ncomp), kTB, 0, s>>>(ncomp, croot, depth, tpar, pre, cursor, Qb, lvcnt);
+    // hub rows of the BFS (nck / ioff reused: the all-vertex items are done)
+    k_hub_flag<<<blocks_for((int64_t)n + 1), kTB, 0, s>>>(n, rp, nck);
+    TRY(S.scan(nck, ioff, n + 1));
+    int nhub = 0;
+    TRY(read_int(ctx, ioff + n, &nhub));
+    int* hubs;
+    TRY(S.get(&hubs, (size_t)nhub));
+    k_hub_list<<<gn, kTB, 0, s>>>(n, nck, ioff, hubs);
     std::vector<int> h_cnt;  // vertices per level
     {
         std::vector<int> chunk(kRunLevels + 1);
         bool done = false;
         for (int L = 0; !done && L <= n; L += kRunLevels) {
-            for (int k = 0; k < kRunLevels; ++k)
+            for (int k = 0; k < kRunLevels; ++k) {
                 k_bfs_level<<<kLevelBlocks, kTB, 0, s>>>(L + k, rp, col, fr, Qb, loff, lvcnt, depth, tpar);
+                if (nhub > 0)
+                    k_bfs_hubs<<<nhub, kTB, 0, s>>>(L + k, hubs, rp, col, fr, Qb, loff, lvcnt, depth, tpar);
+            }
             HIPTRY(hipMemcpyAsync(chunk.data(), lvcnt + L, (kRunLevels + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
             HIPTRY(hipStreamSynchronize(s));
             // chunk[k] counts level L + k; chunk[kRunLevels] is read again as
@@ -668,16 +972,16 @@ int reduce_gpu(cent_ctx* ctx, const struct CSR* g, cent_reduced* r) {
     r->ncomp = ncomp;
     r->d1.n = n;
     r->d1.nfold = nfold;
-    TRY(to_host(ctx, &r->d1.order, order, (size_t)nfold));
-    TRY(to_host(ctx, &r->d1.parent, par, n));
-    TRY(to_host(ctx, &r->d1.rdeg, rdeg, n));
-    TRY(to_host(ctx, &r->d1.w, w, n));
-    TRY(to_host(ctx, &r->d1.ff, ff, n));
-    TRY(to_host(ctx, &r->vcomp, vcomp, n));
-    TRY(to_host(ctx, &r->comp_w, comp_w, (size_t)ncomp));
+    TRY(back.add(&r->d1.order, order, (size_t)nfold));
+    TRY(back.add(&r->d1.parent, par, n));
+    TRY(back.add(&r->d1.rdeg, rdeg, n));
+    TRY(back.add(&r->d1.w, w, n));
+    TRY(back.add(&r->d1.ff, ff, n));
+    TRY(back.add(&r->vcomp, vcomp, n));
+    TRY(back.add(&r->comp_w, comp_w, (size_t)ncomp));
 
     if (total == 0) {  // a forest: every component folded to one vertex
-        HIPTRY(hipStreamSynchronize(s));
+        TRY(back.flush());
         r->np = 0;
         r->nnzp = 0;
         r->nblocks = 0;
@@ -704,18 +1008,25 @@ int reduce_gpu(cent_ctx* ctx, const struct CSR* g, cent_reduced* r) {
     TRY(S.get(&size, n)); TRY(S.get(&lo, n)); TRY(S.get(&hi, n)); TRY(S.get(&au, n)); TRY(S.get(&blk, n));
     TRY(S.get(&memb, n)); TRY(S.get(&head, n)); TRY(S.get(&rootblk, (size_t)ncomp)); TRY(S.get(&nblkc, 1));
     TRY(S.get(&wsub, n)); TRY(S.get(&extra, n)); TRY(S.get(&subw, n));
-    const unsigned gt = blocks_for(total), gtG = blocks_for((int64_t)total * kG);
+    const unsigned gt = blocks_for(total);
+    // row items over the forest's vertices, in Qb order
+    k_chunk_count<<<blocks_for((int64_t)total + 1), kTB, 0, s>>>(total, Qb, rp, nck);
+    TRY(S.scan(nck, ioff, total + 1));
+    int ni = 0;
+    TRY(read_int(ctx, ioff + total, &ni));
+    if ((int64_t)ni * kG > (int64_t)INT32_MAX) return CENT_E_INVAL;
+    const unsigned gi = blocks_for((int64_t)ni * kG);
     k_tv_init<<<gt, kTB, 0, s>>>(total, Qb, w, size, wsub);
     for (int L = maxL; L >= 1; --L)
         k_up_size<<<blocks_for(h_cnt[L]), kTB, 0, s>>>(h_off[L], h_cnt[L], Qb, tpar, size, wsub);
-    for (int L = 0; L < maxL; ++L)
-        k_pre_level<<<blocks_for((int64_t)h_cnt[L] * kG), kTB, 0, s>>>(h_off[L], h_cnt[L], Qb, rp, col, tpar, size,
-                                                                       pre);
-    k_lohi_local<<<gtG, kTB, 0, s>>>(total, Qb, rp, col, depth, tpar, pre, lo, hi);
+    for (int L = 1; L <= maxL; ++L)
+        k_pre_level<<<blocks_for(h_cnt[L]), kTB, 0, s>>>(h_off[L], h_cnt[L], Qb, tpar, size, pre, cursor);
+    k_lohi_init<<<gt, kTB, 0, s>>>(total, Qb, pre, lo, hi);
+    k_lohi_local<<<gi, kTB, 0, s>>>(ni, ioff, total, Qb, rp, col, depth, tpar, pre, lo, hi);
     for (int L = maxL; L >= 1; --L)
         k_up_lohi<<<blocks_for(h_cnt[L]), kTB, 0, s>>>(h_off[L], h_cnt[L], Qb, tpar, lo, hi);
     k_iota<<<gn, kTB, 0, s>>>(n, au);
-    k_tv_union<<<gtG, kTB, 0, s>>>(total, Qb, rp, col, depth, tpar, pre, size, lo, hi, au);
+    k_tv_union<<<gi, kTB, 0, s>>>(ni, ioff, total, Qb, rp, col, depth, tpar, pre, size, lo, hi, au);
 
     // ---- 5. blocks, in children-first order
     HIPTRY(hipMemsetAsync(memb, 0, (size_t)n * sizeof(int), s));
@@ -745,7 +1056,8 @@ int reduce_gpu(cent_ctx* ctx, const struct CSR* g, cent_reduced* r) {
     TRY(S.get(&ck_in, (size_t)ncand)); TRY(S.get(&ck_out, (size_t)ncand));
     TRY(S.get(&cv_in, (size_t)ncand)); TRY(S.get(&cv_out, (size_t)ncand));
     HIPTRY(hipMemsetAsync(bdh, 0, (size_t)n * sizeof(int), s));
-    k_bdeg<<<gtG, kTB, 0, s>>>(total, Qb, rp, col, depth, tpar, blk, bdm, bdh);
+    HIPTRY(hipMemsetAsync(bdm, 0, (size_t)n * sizeof(int), s));
+    k_bdeg<<<gi, kTB, 0, s>>>(ni, ioff, total, Qb, rp, col, depth, tpar, blk, bdm, bdh);
     k_copy_keys<<<blocks_for(ncand), kTB, 0, s>>>(n, nblk, depth, blk, rank, bdm, blabel, bdh, ck_in, cv_in);
     TRY(S.sort_pairs(ck_in, ck_out, cv_in, cv_out, ncand));
     int *mslot, *hslot, *porig, *pblock, *rowlen, *prp;
@@ -765,7 +1077,8 @@ int reduce_gpu(cent_ctx* ctx, const struct CSR* g, cent_reduced* r) {
     TRY(S.get(&cur, (size_t)np)); TRY(S.get(&ent, (size_t)nnzp)); TRY(S.get(&ent_s, (size_t)nnzp));
     TRY(S.get(&pcol, (size_t)nnzp));
     HIPTRY(hipMemsetAsync(cur, 0, (size_t)np * sizeof(int), s));
-    k_fill_rows<<<gtG, kTB, 0, s>>>(total, Qb, rp, col, depth, tpar, blk, rank, mslot, hslot, prp, cur, ent);
+    k_fill_rows<<<gi, kTB, 0, s>>>(ni, ioff, total, Qb, rp, col, depth, tpar, blk, rank, mslot, hslot, prp, cur,
+                                   ent);
     if (nnzp > 0) {
         TRY(S.sort_keys(ent, ent_s, nnzp, 32 + bits_for(np)));
         k_low32<<<blocks_for(nnzp), kTB, 0, s>>>(nnzp, ent_s, pcol);
@@ -787,16 +1100,16 @@ int reduce_gpu(cent_ctx* ctx, const struct CSR* g, cent_reduced* r) {
     r->nnzp = nnzp;
     r->nblocks = nblk;
     r->n_ap = n_ap;
-    TRY(to_host(ctx, &r->prp, prp, (size_t)np + 1));
-    TRY(to_host(ctx, &r->pcol, pcol, (size_t)nnzp));
-    TRY(to_host(ctx, &r->pw, pw, (size_t)np));
-    TRY(to_host(ctx, &r->porig, porig, (size_t)np));
-    TRY(to_host(ctx, &r->pblock, pblock, (size_t)np));
-    TRY(to_host(ctx, &r->boff, boff, (size_t)nblk + 1));
-    TRY(to_host(ctx, &r->bhead, bhead, (size_t)nblk));
-    TRY(to_host(ctx, &r->bparent, bparent, (size_t)nblk));
-    TRY(to_host(ctx, &r->rep, rep, n));
-    HIPTRY(hipStreamSynchronize(s));
+    TRY(back.add(&r->prp, prp, (size_t)np + 1));
+    TRY(back.add(&r->pcol, pcol, (size_t)nnzp));
+    TRY(back.add(&r->pw, pw, (size_t)np));
+    TRY(back.add(&r->porig, porig, (size_t)np));
+    TRY(back.add(&r->pblock, pblock, (size_t)np));
+    TRY(back.add(&r->boff, boff, (size_t)nblk + 1));
+    TRY(back.add(&r->bhead, bhead, (size_t)nblk));
+    TRY(back.add(&r->bparent, bparent, (size_t)nblk));
+    TRY(back.add(&r->rep, rep, n));
+    TRY(back.flush());
     return CENT_OK;
 }
 

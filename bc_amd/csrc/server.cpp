@@ -155,7 +155,10 @@ static void run_request(cent_server* s, int slot, Request* r, GraphEntry* g) {
     {
         std::lock_guard<std::mutex> lk(g->build_mu);
         if (!dry && !g->dev) st = cent_graph_upload(ctx, g->host, &g->dev);
-        if (!st && r->kind == CENT_Q_CC && r->all && !g->red) st = cent_reduce(g->host, &g->red);
+        // the reduction runs on this slot's GPU (its scratch is transient,
+        // freed before the query runs); the dry run reduces on the host
+        if (!st && r->kind == CENT_Q_CC && r->all && !g->red)
+            st = dry ? cent_reduce(g->host, &g->red) : cent_reduce_gpu(ctx, g->host, &g->red);
     }
     const int64_t k = r->all ? g->n : (int64_t)r->src.size();
     cent_opts opts{};

@@ -104,6 +104,34 @@ def extra_blocktree(ctx, args):
     return res
 
 
+def extra_reduce(ctx, args):
+    """Graph reduction (degree-1 folding + biconnected split + packing), the
+    one-time step per graph before reduced closeness / BC: host C
+    (cent_reduce, Hopcroft-Tarjan) against GPU kernels (cent_reduce_gpu,
+    Tarjan-Vishkin over a BFS forest), input CSR in host memory and result
+    back in host memory for both.  Each GPU result is checked against the
+    host one (same block / copy / edge / fold counts)."""
+    import bc_amd
+    out = {}
+    graphs = (("blocktree2000", lambda: bc_amd.gen_blocktree(2000, 1.5, 8, 50000, 6.0, 0.3, 0.5, seed=3)),
+              ("blocktree30000", lambda: bc_amd.gen_blocktree(30000, 1.5, 8, 50000, 6.0, 0.3, 0.5, seed=5)),
+              ("rmat20", lambda: bc_amd.gen_rmat(20, 16 << 20, seed=2)),
+              ("grid2048", lambda: bc_amd.gen_grid(2048, 2048, 0.05, 1000, seed=4)))
+    for name, make in graphs:
+        g = make()
+        host = bc_amd.Reduced(g)
+        th = timed(lambda: bc_amd.Reduced(g), 2)
+        td = timed(lambda: bc_amd.Reduced(g, ctx=ctx), 3)
+        dev = bc_amd.Reduced(g, ctx=ctx)
+        for f in ("np", "nnzp", "nblocks", "nfold", "n_ap", "ncomp"):
+            assert getattr(dev, f) == getattr(host, f), (name, f)
+        out[name] = {"n": g.n, "m": g.m, "blocks": host.nblocks, "articulation": host.n_ap, "folded": host.nfold,
+                     "host_ms": round(th * 1e3, 2), "gpu_ms": round(td * 1e3, 2), "speedup": round(th / td, 2)}
+        log(f"reduce {name}: n={g.n} m={g.m} blocks={host.nblocks}: host {th * 1e3:.1f} ms, GPU {td * 1e3:.1f} ms")
+        del g, host, dev
+    return out
+
+
 def extra_blocktree_bc(ctx, args):
     """All-source Brandes BC + closeness on the block-tree graph (unreduced)."""
     import bc_amd
@@ -395,7 +423,7 @@ def serve_vs_plain(ctx, pool, reqs):
 
 
 EXTRA = {"rmat14": extra_rmat14, "blocktree": extra_blocktree, "blocktree-bc": extra_blocktree_bc, "grid": extra_grid,
-         "closeness": extra_closeness, "serve": extra_serve}
+         "closeness": extra_closeness, "serve": extra_serve, "reduce": extra_reduce}
 
 
 def main() -> int:
@@ -428,7 +456,8 @@ def main() -> int:
                          "blocktree (reduced closeness, persistent vs per-item launches), "
                          "blocktree-bc (plain BC+CC on the block tree), closeness (closeness-only "
                          "timings), grid (sampled BC+CC on "
-                         "the 2048^2 grid)")
+                         "the 2048^2 grid), serve (request server), reduce (graph reduction, "
+                         "host vs GPU)")
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-pool", type=int, default=12)
