@@ -220,7 +220,7 @@ _lib.cent_reduced_farness.argtypes = [C.c_void_p, _I64P, _I64P]
 _lib.cent_reduced_bc.argtypes = [C.c_void_p, _F64P, _F64P]
 _lib.cent_reduced_free.argtypes = [C.c_void_p]
 _lib.cent_reduced_free.restype = None
-CENT_SCHED_PERSISTENT, CENT_SCHED_PER_ITEM = 0, 1
+CENT_SCHED_PERSISTENT, CENT_SCHED_PER_ITEM, CENT_SCHED_SHARED = 0, 1, 2  # see include/cent.h
 
 
 class Reduced:

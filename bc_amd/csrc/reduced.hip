@@ -5,6 +5,7 @@
 
 #include "engine.h"
 #include "items_kernels.h"
+#include "share_kernels.h"
 
 // ------------------------------------------------------------ reduced closeness
 // Device copy of a packed block graph, cached on the cent_reduced object
@@ -26,6 +27,13 @@ struct RedDev {
     // elsewhere), and that BFS's depth per block (keyed by the block's lo)
     std::vector<int32_t> probe_d;
     std::map<int, int> probe_ecc;
+    // neighbour-sharing units over the blocks of at most kShareLds vertices
+    // (built on the first CENT_SCHED_SHARED call)
+    bool share_built = false;
+    int nunits = 0;
+    int4* units = nullptr;  // (center, lo, hi, first neighbour)
+    int* ucnt = nullptr;
+    int* unbr = nullptr;
 };
 
 static void reddev_free(void* p) {
@@ -38,6 +46,9 @@ static void reddev_free(void* p) {
     if (rd->items) (void)hipFree(rd->items);
     if (rd->order) (void)hipFree(rd->order);
     if (rd->poff) (void)hipFree(rd->poff);
+    if (rd->units) (void)hipFree(rd->units);
+    if (rd->ucnt) (void)hipFree(rd->ucnt);
+    if (rd->unbr) (void)hipFree(rd->unbr);
 
     delete rd;
 }
@@ -109,6 +120,54 @@ static int reddev_get(cent_ctx* ctx, cent_reduced* r, RedDev** out) {
         rd->probe_ecc[lo] = depth;
     }
     *out = rd;
+    return CENT_OK;
+}
+
+// Units of the neighbour-sharing schedule, in the reference's order
+// (CC.cpp:1779-1866): inside each block, sources by decreasing degree (the
+// packed order); a source not yet answered with at least 3 unanswered
+// neighbours answers up to 32 of them too (first in row order), otherwise
+// it is a unit alone.  Blocks larger than kShareLds stay with the 64-lane
+// big-item path.  Largest blocks first, for the work queue.
+static int reddev_share(cent_ctx* ctx, const cent_reduced* r, RedDev* rd) {
+    std::lock_guard<std::mutex> lk(g_reddev_mu);
+    if (rd->share_built) return CENT_OK;
+    std::vector<int> blocks;
+    for (int b = 0; b < r->nblocks; ++b)
+        if (r->boff[b + 1] - r->boff[b] <= kShareLds) blocks.push_back(b);
+    std::stable_sort(blocks.begin(), blocks.end(), [&](int x, int y) {
+        return r->boff[x + 1] - r->boff[x] > r->boff[y + 1] - r->boff[y];
+    });
+    std::vector<int4> units;
+    std::vector<int> cnt, nbr;
+    std::vector<uint8_t> done((size_t)(r->np > 0 ? r->np : 1), 0);
+    for (int b : blocks) {
+        const int lo = r->boff[b], hi = r->boff[b + 1];
+        for (int v = lo; v < hi; ++v) {
+            if (done[(size_t)v]) continue;
+            done[(size_t)v] = 1;
+            int undone = 0;
+            for (int j = r->prp[v]; j < r->prp[v + 1]; ++j) undone += !done[(size_t)r->pcol[j]];
+            const int off = (int)nbr.size();
+            int k = 0;
+            if (undone >= 3) {
+                for (int j = r->prp[v]; j < r->prp[v + 1] && k < kShareMax; ++j) {
+                    const int y = r->pcol[j];
+                    if (done[(size_t)y]) continue;
+                    done[(size_t)y] = 1;
+                    nbr.push_back(y);
+                    ++k;
+                }
+            }
+            units.push_back(make_int4(v, lo, hi, off));
+            cnt.push_back(k);
+        }
+    }
+    rd->nunits = (int)units.size();
+    TRY(upload(ctx, &rd->units, units.data(), units.size()));
+    TRY(upload(ctx, &rd->ucnt, cnt.data(), cnt.size()));
+    TRY(upload(ctx, &rd->unbr, nbr.data(), nbr.size()));
+    rd->share_built = true;
     return CENT_OK;
 }
 
@@ -191,12 +250,14 @@ static int run_big_groups(cent_ctx* ctx, const RedDev* rd, const BatchSpec& sp, 
 extern "C" int cent_cc_reduced(cent_ctx* ctx, const cent_reduced* rc, int64_t* farness, int32_t sched,
                                const cent_opts* opts) {
     if (!ctx || !rc || !farness) return CENT_E_INVAL;
-    if (sched != CENT_SCHED_PERSISTENT && sched != CENT_SCHED_PER_ITEM) return CENT_E_INVAL;
+    if (sched != CENT_SCHED_PERSISTENT && sched != CENT_SCHED_PER_ITEM && sched != CENT_SCHED_SHARED)
+        return CENT_E_INVAL;
     cent_reduced* r = const_cast<cent_reduced*>(rc);  // the device cache lives on the object
     TRY(check_device(ctx));
     auto t0 = std::chrono::steady_clock::now();
     RedDev* rd;
     TRY(reddev_get(ctx, r, &rd));
+    if (sched == CENT_SCHED_SHARED) TRY(reddev_share(ctx, r, rd));
     hipStream_t s = ctx->stream;
     const size_t NP = (size_t)(r->np > 0 ? r->np : 1);
     // per-run buffers live in the context, so several contexts can share rd
@@ -222,7 +283,22 @@ extern "C" int cent_cc_reduced(cent_ctx* ctx, const cent_reduced* rc, int64_t* f
     if (sched == CENT_SCHED_PER_ITEM) {
         for (int i = 0; i < r->nitems; ++i) TRY(run_item(i));
     } else {
-        if (rd->nsmall) {
+        if (sched == CENT_SCHED_SHARED) {
+            if (rd->nunits) {
+                HIPTRY(hipMemsetAsync(ctx->red_head.p, 0, sizeof(int), s));
+                ShareArgs sa;
+                sa.rp = rd->d.rp; sa.col = rd->d.col; sa.weight = rd->w; sa.units = rd->units; sa.ucnt = rd->ucnt;
+                sa.nbr = rd->unbr; sa.nunits = rd->nunits; sa.head = ctx->red_head.p; sa.D = Dd;
+                const int grid = std::min(rd->nunits, 1024);
+                hipEvent_t e0;
+                TRY(timer_begin(ctx, tm, &e0));
+                k_cc_share<<<grid, kShareBlock, 0, s>>>(sa);
+                HIPTRY(hipGetLastError());
+                TRY(timer_end(ctx, tm, e0, true));
+                ctx->stats.launches_fwd++;
+                ctx->stats.batches += rd->nunits;
+            }
+        } else if (rd->nsmall) {
             HIPTRY(hipMemsetAsync(ctx->red_head.p, 0, sizeof(int), s));
             ItemArgs ia;
             ia.rp = rd->d.rp; ia.col = rd->d.col; ia.weight = rd->w; ia.items = rd->items; ia.order = rd->order;

@@ -72,12 +72,15 @@ def extra_blocktree(ctx, args):
     far_p = ctx.cc_reduced(red, bc_amd.CENT_SCHED_PERSISTENT)
     far_i = ctx.cc_reduced(red, bc_amd.CENT_SCHED_PER_ITEM)
     assert (far_p == far_i).all()
+    far_s = ctx.cc_reduced(red, bc_amd.CENT_SCHED_SHARED)
+    assert (far_s == far_p).all()
     tp = timed(lambda: ctx.cc_reduced(red, bc_amd.CENT_SCHED_PERSISTENT), 5)
     ti = timed(lambda: ctx.cc_reduced(red, bc_amd.CENT_SCHED_PER_ITEM), 2)
+    tsh = timed(lambda: ctx.cc_reduced(red, bc_amd.CENT_SCHED_SHARED), 5)
     dg = ctx.upload(g)
     tplain = timed(lambda: ctx.cc(dg, None), 1)
     log(f"blocktree: persistent {tp * 1e3:.2f} ms, per-item {ti * 1e3:.1f} ms, "
-        f"plain all-source MS-BFS {tplain * 1e3:.1f} ms")
+        f"neighbour-sharing {tsh * 1e3:.2f} ms, plain all-source MS-BFS {tplain * 1e3:.1f} ms")
     # BC + closeness of every vertex through the reduction
     bp, fp = ctx.bc_reduced(red, bc_amd.CENT_SCHED_PERSISTENT)
     bi, fi = ctx.bc_reduced(red, bc_amd.CENT_SCHED_PER_ITEM)
@@ -93,6 +96,7 @@ def extra_blocktree(ctx, args):
         "n": n, "m": m, "blocks": red.nblocks, "articulation": red.n_ap, "folded": red.nfold,
         "packed_vertices": red.np, "items": red.nitems,
         "persistent_ms": round(tp * 1e3, 3), "per_item_ms": round(ti * 1e3, 3),
+        "neighbour_sharing_ms": round(tsh * 1e3, 3),
         "plain_unreduced_ms": round(tplain * 1e3, 3),
         "queries_per_s_persistent": round(1.0 / tp, 2), "queries_per_s_per_item": round(1.0 / ti, 2),
         "speedup_vs_per_item": round(ti / tp, 2),

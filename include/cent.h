@@ -379,6 +379,12 @@ void cent_reduced_drop_device(cent_reduced* r);
 /* Schedules for cent_cc_reduced. */
 #define CENT_SCHED_PERSISTENT 0  /* one persistent launch over the work-item queue */
 #define CENT_SCHED_PER_ITEM   1  /* level-synchronous launches per work item       */
+/* neighbour sharing (closeness only): one BFS from a source also answers up
+ * to 32 of its neighbours by the SI/R rule of
+ * compute_D1_AP_CC_shareBased_DegreeOrder (Seq_CC/CC.cpp:1818-2022), one
+ * workgroup per source in a persistent queue over the blocks of at most
+ * 2048 vertices; larger blocks as in CENT_SCHED_PERSISTENT */
+#define CENT_SCHED_SHARED     2
 
 /* farness (length n, assigned) of every input vertex through the reduced
  * form; equals cent_cc over all sources on every graph. */

@@ -45,7 +45,7 @@ def check_same(ctx, g, what, bc=False):
         assert np.all(np.diff(row) > 0), (what, "row not sorted", a)
     assert canon(dev) == canon(host), (what, "blocks differ")
     want = O.weighted_farness(g.n, g.row_ptr, g.col, np.ones(g.n, np.int64))
-    for sched in (bc_amd.CENT_SCHED_PERSISTENT, bc_amd.CENT_SCHED_PER_ITEM):
+    for sched in (bc_amd.CENT_SCHED_PERSISTENT, bc_amd.CENT_SCHED_PER_ITEM, bc_amd.CENT_SCHED_SHARED):
         got = ctx.cc_reduced(dev, sched)
         bad = np.nonzero(got != want)[0]
         assert len(bad) == 0, (what, sched, bad[:10], got[bad[:5]], want[bad[:5]])

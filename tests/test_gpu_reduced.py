@@ -15,6 +15,8 @@ from conftest import GOLDEN
 pytestmark = pytest.mark.gpu
 
 SCHEDS = (bc_amd.CENT_SCHED_PERSISTENT, bc_amd.CENT_SCHED_PER_ITEM)
+# closeness also through the neighbour-sharing schedule (the SI/R rule)
+CC_SCHEDS = SCHEDS + (bc_amd.CENT_SCHED_SHARED,)
 
 
 def plain_farness(g):
@@ -25,7 +27,7 @@ def check(ctx, g, what):
     print(f"[gpu] reduced {what}: n={g.n} m={g.m}", flush=True)
     red = bc_amd.Reduced(g)
     want = plain_farness(g)
-    for sched in SCHEDS:
+    for sched in CC_SCHEDS:
         got = ctx.cc_reduced(red, sched)
         bad = np.nonzero(got != want)[0]
         assert len(bad) == 0, (what, sched, bad[:10], got[bad[:5]], want[bad[:5]])
@@ -102,3 +104,19 @@ def test_reduced_bc_deterministic(ctx):
     a = ctx.bc_reduced(red)
     b = ctx.bc_reduced(red)
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_shared_schedule_shapes(ctx):
+    # centers with more than 32 unanswered neighbours (a star inside a block),
+    # with exactly 3, and blocks where every source stays alone (a cycle);
+    # BC through the shared schedule is refused
+    n = 80
+    hub = [(0, i) for i in range(1, n)] + [(i, i + 1) for i in range(1, n - 1)]  # a fan: one block
+    e = np.array(hub)
+    check(ctx, bc_amd.CSR.from_edges(n, e[:, 0], e[:, 1]), "fan80")
+    check(ctx, bc_amd.CSR.from_edges(9, np.arange(9), (np.arange(9) + 1) % 9), "cycle9")
+    k4 = np.array([(a, b) for a in range(4) for b in range(a + 1, 4)])
+    check(ctx, bc_amd.CSR.from_edges(4, k4[:, 0], k4[:, 1]), "K4")
+    red = bc_amd.Reduced(bc_amd.CSR.from_edges(4, k4[:, 0], k4[:, 1]))
+    with pytest.raises(bc_amd.CentError):
+        ctx.bc_reduced(red, bc_amd.CENT_SCHED_SHARED)
