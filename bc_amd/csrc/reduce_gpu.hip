@@ -16,7 +16,7 @@
 //     so a component's root is its smallest vertex and components are
 //     numbered by smallest vertex, as the host numbers them).
 //  3. A BFS spanning forest of the folded graph, one root per component
-//     (its smallest unfolded vertex of nonzero degree), level by level.
+//     (its unfolded vertex of highest remaining degree), level by level.
 //  4. Tarjan–Vishkin over that forest: subtree sizes and weights bottom-up,
 //     preorder numbers top-down, low/high (extreme preorder reached by a
 //     non-tree edge from the subtree) bottom-up, then a union-find over the
@@ -290,27 +290,39 @@ __global__ void k_comp_assign(int n, const int* cid, int* vcomp, int64_t* comp_w
 }
 
 // ------------------------------------------------------------ BFS forest
-// smallest unfolded vertex of nonzero degree per component: a wave's lanes
-// hold increasing ids, so the lowest lane of a key holds its smallest id
-__global__ void k_croot(int n, const int* fr, const int* rdeg, const int* vcomp, int* croot) {
+// BFS root per component: its unfolded vertex of highest remaining degree
+// (smallest id among equals), as key (rdeg << 32 | ~id) under atomicMax.  A
+// hub sits near the middle of a power-law graph and a shortcut end near the
+// middle of a grid, so the forest is shallow and the per-level passes few;
+// the blocks do not depend on the root.
+__global__ void k_croot(int n, const int* fr, const int* rdeg, const int* vcomp, unsigned long long* croot) {
     const int v = gtid();
     const bool ok = v < n && fr[v] == kUnfolded && rdeg[v] > 0;
     const int c = ok ? vcomp[v] : 0;
+    const unsigned long long key = ok ? ((unsigned long long)(uint32_t)rdeg[v] << 32) | (0xffffffffu - (uint32_t)v) : 0ull;
     uint64_t pend = ballot(ok);
     while (pend) {
         const int lead = __builtin_ctzll(pend);
         const int k0 = __shfl(c, lead);
-        const uint64_t m = ballot(ok && c == k0) & pend;
-        if (lane_id() == lead) atomicMin(croot + k0, v);
+        const bool mine = ok && c == k0 && ((pend >> lane_id()) & 1);
+        const uint64_t m = ballot(mine);
+        unsigned long long mx = mine ? key : 0ull;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const unsigned long long t = __shfl_xor(mx, o);
+            mx = t > mx ? t : mx;
+        }
+        if (lane_id() == lead) atomicMax(croot + k0, mx);
         pend &= ~m;
     }
 }
 
-__global__ void k_bfs_seed(int ncomp, const int* croot, int* depth, int* tpar, int* pre, int* cursor, int* Qb,
-                           int* lvcnt) {
+__global__ void k_bfs_seed(int ncomp, const unsigned long long* croot, int* depth, int* tpar, int* pre, int* cursor,
+                           int* Qb, int* lvcnt) {
     const int c = gtid();
-    const int r = c < ncomp ? croot[c] : kUnfolded;
-    const bool ok = r != kUnfolded;
+    const unsigned long long key = c < ncomp ? croot[c] : 0ull;
+    const bool ok = key != 0ull;  // 0: no unfolded vertex of nonzero degree
+    const int r = ok ? (int)(0xffffffffu - (uint32_t)key) : 0;
     if (ok) {
         depth[r] = 0;
         tpar[r] = -1;
@@ -921,12 +933,13 @@ int reduce_gpu(cent_ctx* ctx, const struct CSR* g, cent_reduced* r) {
     k_comp_assign<<<gn, kTB, 0, s>>>(n, cid, vcomp, comp_w);
 
     // ---- 3. BFS forest of the folded graph
-    int *croot, *depth, *tpar, *pre, *cursor, *Qb, *loff, *lvcnt;
+    unsigned long long* croot;
+    int *depth, *tpar, *pre, *cursor, *Qb, *loff, *lvcnt;
     const size_t nlv = (size_t)n + 2 + kRunLevels;
     TRY(S.get(&croot, (size_t)ncomp)); TRY(S.get(&depth, n)); TRY(S.get(&tpar, n)); TRY(S.get(&pre, n));
     TRY(S.get(&cursor, n));
     TRY(S.get(&Qb, n)); TRY(S.get(&loff, nlv)); TRY(S.get(&lvcnt, nlv));
-    k_fill_i32<<<blocks_for(ncomp), kTB, 0, s>>>(ncomp, croot, kUnfolded);
+    HIPTRY(hipMemsetAsync(croot, 0, (size_t)ncomp * sizeof(unsigned long long), s));
     k_fill_i32<<<gn, kTB, 0, s>>>(n, depth, -1);
     k_fill_i32<<<gn, kTB, 0, s>>>(n, tpar, -1);
     HIPTRY(hipMemsetAsync(loff, 0, nlv * sizeof(int), s));
