@@ -316,6 +316,10 @@ _lib.cent_graph_from_csr.argtypes = [C.c_void_p, _I32P, _I32P, C.c_int32, C.c_in
 _lib.cent_graph_create.argtypes = [C.c_void_p, _I32P, _I32P, C.c_int32, C.c_int64, C.c_int32,
                                    C.POINTER(C.c_void_p)]
 CENT_GRAPH_KEEP_ORDER = 1
+CENT_GRAPH_DEGREE_ORDER = 4
+CENT_GRAPH_BFS_ORDER = 8
+_ORDER_FLAGS = {"auto": 0, "keep": CENT_GRAPH_KEEP_ORDER, "degree": CENT_GRAPH_DEGREE_ORDER,
+                "bfs": CENT_GRAPH_BFS_ORDER}
 _lib.cent_graph_free.argtypes = [C.c_void_p]
 _lib.cent_graph_free.restype = None
 _lib.cent_bc.argtypes = [C.c_void_p, C.c_void_p, _I32P, C.c_int32, _F64P, _I64P, _I64P,
@@ -527,13 +531,20 @@ class DeviceArray:
 
 
 class DeviceGraph:
-    def __init__(self, ctx: "Context", csr: CSR, keep_order: bool = False, tuned: bool = True):
+    def __init__(self, ctx: "Context", csr: CSR, keep_order: bool = False, tuned: bool = True,
+                 order: str = "auto"):
+        """order: device relabel, "auto" (degree order, or BFS order for graphs
+        deeper than 48 levels from the highest-degree vertex), "degree", "bfs"
+        or "keep" (the caller's ids; the same as keep_order=True)."""
         self.ctx = ctx
         self.n = csr.n
         self.nnz = csr.nnz
         self._csr = csr
         h = C.c_void_p()
-        flags = (CENT_GRAPH_KEEP_ORDER if keep_order else 0) | (0 if tuned else CENT_GRAPH_NO_TUNE)
+        if order not in _ORDER_FLAGS:
+            raise ValueError(f"order must be one of {sorted(_ORDER_FLAGS)}")
+        flags = ((CENT_GRAPH_KEEP_ORDER if keep_order else _ORDER_FLAGS[order])
+                 | (0 if tuned else CENT_GRAPH_NO_TUNE))
         _check(_lib.cent_graph_create(ctx._h, _ptr(csr.row_ptr, C.c_int32),
                                       _ptr(csr.col, C.c_int32) if csr.nnz else None,
                                       csr.n, csr.nnz, flags, C.byref(h)),
@@ -592,8 +603,8 @@ class Context:
                "cent_probe_copy_bandwidth", self.last_error())
         return out.value
 
-    def upload(self, csr: CSR, keep_order: bool = False, tuned: bool = True) -> DeviceGraph:
-        return DeviceGraph(self, csr, keep_order, tuned)
+    def upload(self, csr: CSR, keep_order: bool = False, tuned: bool = True, order: str = "auto") -> DeviceGraph:
+        return DeviceGraph(self, csr, keep_order, tuned, order)
 
     def tune(self, g: DeviceGraph, sources, what: int = CENT_TUNE_BC, max_trials: int = 0) -> TuneParams:
         """Autotune g's launch settings on the sample sources (at most 256 for

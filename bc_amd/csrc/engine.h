@@ -70,6 +70,7 @@ struct cent_ctx {
     DevBuf<char> red_arena;
     // source-parallel mode: per-slot arrays
     DevBuf<PsRec> ps_rec;
+    DevBuf<uint32_t> ps_bits;
     DevBuf<int> ps_Q, ps_loff;
     DevBuf<double> ps_bcp;
     DevBuf<unsigned long long> ps_depth;
@@ -225,6 +226,8 @@ struct cent_graph {
     int32_t width_probe = 0;          // widest level of that BFS
     std::vector<int32_t> h_rp;        // device-order row offsets (host), for re-scheduling
     std::vector<int32_t> probe_dist;  // BFS distance from the highest-degree vertex (caller ids, -1 unreached)
+    std::vector<int32_t> probe_order; // that BFS's visit order (caller ids; dropped after the relabel)
+    bool bfs_order = false;           // device ids are a BFS order (else degree order or the caller's)
     cent_tune_sig sig{};              // autotuner signature
     // launch defaults when the call's opts leave them open: [0] betweenness,
     // [1] closeness

@@ -458,21 +458,23 @@ static int level_batch_sparse(cent_ctx* ctx, SpPlan& pl, const BatchSpec& sp, co
 }
 
 // ------------------------------------------------------------ source-parallel mode
-// All k sources of a plain call in one launch of k_ps: G workgroup slots,
-// each with its own per-vertex arrays, take sources s, s + G, ... in turn.
+// All k sources of a plain call in one launch of k_ps_bc (betweenness) or
+// k_ps_cc (closeness only): G workgroup slots, each with its own per-vertex
+// arrays, take sources s, s + G, ... in turn.
 // G is bounded by k, by the slot cap (opts->max_slots, else grid_waves / 8,
 // else 256 — more concurrent traversals than that only contend for the
 // memory system — doubled up to 1024 while all slots' arrays stay within
 // kPsSlotSetBytes) and by a memory budget (40% of free device memory, at
-// most 32 GiB).  Returns CENT_E_UNSUPPORTED, with nothing accumulated into the BC
-// output, if some source's BFS is deeper than the level table: the caller
-// then runs the batch form.
+// most 32 GiB).  Betweenness returns CENT_E_UNSUPPORTED, with nothing
+// accumulated into the BC output, if some source's BFS is deeper than the
+// level table: the caller then runs the batch form.  Closeness keeps no
+// level table and has no depth limit.
 constexpr int kPsDefaultSlots = 256;
 constexpr int kPsMaxCachedSlots = 1024;
 // Measured on MI355X: doubling the slots from 256 pays (by 8-30%) while the
 // slots' arrays stay within about 2 GB — R-MAT 2^10-2^14 and block trees of
 // 11k-113k vertices, all sources — and costs beyond (the 4.2M-vertex grid,
-// 117 MB per slot, is fastest at 256).
+// 117 MB per slot for BC, is fastest at 256).
 constexpr size_t kPsSlotSetBytes = (size_t)2 << 30;
 
 static int run_persource(cent_ctx* ctx, const cent_graph* g, const BatchSpec& sp, int32_t k, KernelTimer& tm,
@@ -483,7 +485,8 @@ static int run_persource(cent_ctx* ctx, const cent_graph* g, const BatchSpec& sp
     const bool bc = sp.with_bc, compat = sp.compat;
     if (sp.bwgt || k < 1) return CENT_E_UNSUPPORTED;
     const int lcap = (int)std::min<size_t>(n + 2, 65536);
-    const size_t per_slot = n * (sizeof(PsRec) + 4 + (bc ? 8 : 0)) + (size_t)lcap * 4;
+    const size_t nw = ((n + 31) / 32 + 3) / 4 * 4;  // closeness map words, padded for 16-byte clears
+    const size_t per_slot = bc ? n * (sizeof(PsRec) + 4 + 8) + (size_t)lcap * 4 : n * 4 + nw * 4;
     size_t cap = kPsDefaultSlots;
     if (sp.opts && sp.opts->max_slots > 0) cap = (size_t)sp.opts->max_slots;
     else if (sp.opts && sp.opts->grid_waves > 0) cap = std::max(1, sp.opts->grid_waves / kPsWaves);
@@ -492,34 +495,46 @@ static int run_persource(cent_ctx* ctx, const cent_graph* g, const BatchSpec& sp
     size_t G = std::min((size_t)k, cap);
     size_t freeb = 0, totalb = 0;
     HIPTRY(hipMemGetInfo(&freeb, &totalb));
-    const size_t have = sizeof(PsRec) * ctx->ps_rec.cap + 4 * (ctx->ps_Q.cap + ctx->ps_loff.cap) +
+    const size_t have = sizeof(PsRec) * ctx->ps_rec.cap + 4 * (ctx->ps_Q.cap + ctx->ps_loff.cap + ctx->ps_bits.cap) +
                         8 * ctx->ps_bcp.cap;
     const size_t budget = std::min<size_t>((size_t)32 << 30, (freeb + have) / 5 * 2);
     G = std::max<size_t>(1, std::min(G, budget / per_slot));
-    TRY(ensure_exact(ctx, ctx->ps_rec, n * G));
     TRY(ensure_exact(ctx, ctx->ps_Q, n * G));
-    TRY(ensure_exact(ctx, ctx->ps_loff, (size_t)lcap * G));
-    if (bc) TRY(ensure_exact(ctx, ctx->ps_bcp, n * G));
+    if (bc) {
+        TRY(ensure_exact(ctx, ctx->ps_rec, n * G));
+        TRY(ensure_exact(ctx, ctx->ps_loff, (size_t)lcap * G));
+        TRY(ensure_exact(ctx, ctx->ps_bcp, n * G));
+    } else {
+        TRY(ensure_exact(ctx, ctx->ps_bits, nw * G));
+    }
     TRY(ensure(ctx, ctx->ps_depth, 1));
     TRY(ensure(ctx, ctx->flags, 2));
-    const size_t total = n * G;
-    const int ig = (int)std::min<size_t>((total + 255) / 256, 16384);
-    k_ps_init<<<ig, 256, 0, s>>>(ctx->ps_rec.p, bc ? ctx->ps_bcp.p : nullptr, total);
-    HIPTRY(hipGetLastError());
+    if (bc) {
+        const size_t total = n * G;
+        const int ig = (int)std::min<size_t>((total + 255) / 256, 16384);
+        k_ps_init<<<ig, 256, 0, s>>>(ctx->ps_rec.p, ctx->ps_bcp.p, total);
+        HIPTRY(hipGetLastError());
+    }
     HIPTRY(hipMemsetAsync(ctx->ps_depth.p, 0, sizeof(unsigned long long), s));
     HIPTRY(hipMemsetAsync(ctx->flags.p, 0, sizeof(int) * 2, s));
     PsArgs a{};
-    a.n = d.n; a.rp = d.rp; a.col = d.col; a.src = ctx->src.p; a.k = k;
-    a.rec = ctx->ps_rec.p; a.Q = ctx->ps_Q.p;
-    a.loff = ctx->ps_loff.p; a.lcap = lcap;
-    a.bcp = bc ? ctx->ps_bcp.p : nullptr;
+    a.n = d.n; a.nw = (int)nw; a.rp = d.rp; a.col = d.col; a.src = ctx->src.p; a.k = k;
+    a.Q = ctx->ps_Q.p;
+    if (bc) {
+        a.rec = ctx->ps_rec.p;
+        a.loff = ctx->ps_loff.p;
+        a.lcap = lcap;
+        a.bcp = ctx->ps_bcp.p;
+    } else {
+        a.bits = ctx->ps_bits.p;
+    }
     a.far = ctx->far.p; a.dist_sum = sp.dist_sum; a.weight = sp.weight;
     a.cnt = ctx->cnt.p; a.depth_sum = ctx->ps_depth.p; a.overflow = ctx->flags.p;
     hipEvent_t e0;
     TRY(timer_begin(ctx, tm, &e0));
-    if (!bc) k_ps<false, false><<<(int)G, kPsBlock, 0, s>>>(a);
-    else if (compat) k_ps<true, true><<<(int)G, kPsBlock, 0, s>>>(a);
-    else k_ps<true, false><<<(int)G, kPsBlock, 0, s>>>(a);
+    if (!bc) k_ps_cc<<<(int)G, kPsBlock, 0, s>>>(a);
+    else if (compat) k_ps_bc<true><<<(int)G, kPsBlock, 0, s>>>(a);
+    else k_ps_bc<false><<<(int)G, kPsBlock, 0, s>>>(a);
     HIPTRY(hipGetLastError());
     TRY(timer_end(ctx, tm, e0, true));
     ctx->stats.launches_fwd++;

@@ -1,13 +1,14 @@
-// Source-parallel level kernels: one workgroup runs the whole Brandes (or
-// closeness) traversal of one source at a time, level by level with
-// workgroup barriers, so a call is a single launch with no host
-// synchronisation per level.  This is the form for high-diameter graphs
-// (grids, road-like meshes), where the 64 lanes of a bit-parallel batch sit
-// at different levels and share no work: the dense step would rescan every
-// vertex on each of hundreds of levels, while here a source touches each
-// reached vertex and edge a constant number of times.
+// Source-parallel level kernels: one workgroup runs the whole traversal of
+// one source at a time, level by level with workgroup barriers, so a call is
+// a single launch with no host synchronisation per level.  This is the form
+// for high-diameter graphs (grids, road-like meshes), where the 64 lanes of
+// a bit-parallel batch sit at different levels and share no work: the dense
+// step would rescan every vertex on each of hundreds of levels, while here a
+// source touches each reached vertex and edge a constant number of times.
+// Each workgroup slot keeps its own per-vertex arrays and takes sources s,
+// s + G, s + 2G, ... in turn.
 //
-// Per workgroup slot (stride n, reused source after source):
+// Betweenness (k_ps_bc), per slot:
 //   R[v]      16-byte record, so a neighbour's level and path count share one
 //             cache line:
 //     .dist   int32   BFS level, -1 = unreached (restored after each source)
@@ -16,19 +17,28 @@
 //                     coef = (1 + delta) / sigma once v's delta is known
 //   Q[]       int32   BFS order; level l is Q[loff[l] .. loff[l+1])
 //   bcp[v]    double  this slot's BC partial over its sources
-//
 // Forward, level d: each vertex u of the level scans its row; a neighbour w
 // with dist -1 is claimed by compare-and-swap (dist := d + 1, appended to
 // Q), and every w at level d + 1 gets sig[w] += sig[u] by an FP64 atomic
 // add.  Path counts are integers, exact in FP64 below 2^53, so the sum does
 // not depend on the order of the adds.  This is the level step of the
 // reference's queue BFS (Seq_Brandes/Brandes.c:274-298) for one source.
-//
 // Backward, level d from the deepest up: v sums coef[w] over its neighbours
 // w at level d + 1 in row order (delta = sigma * sum, Brandes.c:317-327 in
-// textbook form), then coef[v] replaces sig[v] and bcp[v] += delta.  Slot s
-// handles sources s, s + G, s + 2G, ... in order and the host-side reduce
-// adds the slots in slot order, so BC is deterministic for a given G.
+// textbook form), then coef[v] replaces sig[v] and bcp[v] += delta.  The
+// host-side reduce adds the slots in slot order, so BC is deterministic for
+// a given G.
+//
+// Closeness only (k_ps_cc), per slot:
+//   V         bitmap, n bits: vertices reached so far.  Claims are atomic ORs
+//             on its 32-bit words, and the test "already reached?" on an
+//             edge reads one map word: one bit per vertex instead of a
+//             16-byte record (2048^2 grid, 1024 sources: 495 -> 304 ms).
+//   Q[]       BFS order of the current and the next level.
+// Farness is the level of each claimed vertex, summed by its claimer.  The
+// same bit maps do not pay for betweenness: a pull of path counts over
+// level maps rescans every row once more and scatters over three more maps
+// (1338 -> 1547 ms on the same grid), so BC keeps the records above.
 //
 // Rows longer than kPsBigDeg are queued in LDS and scanned by a whole
 // wavefront (lane-strided edges, butterfly sum), so one hub row does not
@@ -39,7 +49,10 @@
 
 namespace cent {
 
-constexpr int kPsBlock = 512;   // 8 waves per workgroup
+#ifndef CENT_PS_BLOCK
+#define CENT_PS_BLOCK 512
+#endif
+constexpr int kPsBlock = CENT_PS_BLOCK;  // 8 waves per workgroup
 constexpr int kPsWaves = kPsBlock / kWave;
 constexpr int kPsBigDeg = 64;   // longer rows are scanned by a wavefront
 constexpr int kPsBig = 1024;    // LDS capacity of the long-row list per level
@@ -48,15 +61,17 @@ constexpr int kPsE = 4;         // edges per row issued together
 
 struct PsArgs {
     int n;
+    int nw;                               // closeness: 32-bit words per slot map, a multiple of 4
     const int* __restrict__ rp;
     const int* __restrict__ col;
     const int* __restrict__ src;          // [k] device ids
     int k;
-    PsRec* __restrict__ rec;              // [G][n]
+    PsRec* __restrict__ rec;              // BC: [G][n]
+    uint32_t* __restrict__ bits;          // closeness: [G][nw]
     int* __restrict__ Q;                  // [G][n]
-    int* __restrict__ loff;               // [G][lcap]
+    int* __restrict__ loff;               // BC: [G][lcap]
     int lcap;
-    double* __restrict__ bcp;             // [G][n] or null (closeness only)
+    double* __restrict__ bcp;             // BC: [G][n]
     int64_t* __restrict__ far;            // [k]
     int64_t* __restrict__ dist_sum;       // [n] or null
     const int64_t* __restrict__ weight;   // [n] or null
@@ -82,8 +97,9 @@ struct PsLocal {
     unsigned long long reached = 0, dag = 0;
 };
 
+// ============================================================ betweenness
 // Visit edge u -> w at forward level d.
-template <bool BC, bool COMPAT>
+template <bool COMPAT>
 __device__ __forceinline__ void ps_visit(const PsArgs& a, PsRec* R, int* Q,
                                          int w, double su, int d, int le, int* s_cnt, PsLocal& L) {
     int lw = ld_dist(&R[w].dist);
@@ -100,12 +116,10 @@ __device__ __forceinline__ void ps_visit(const PsArgs& a, PsRec* R, int* Q,
             lw = old;
         }
     }
-    if constexpr (BC) {
-        if (lw == d + 1) {
-            atomicAdd(&R[w].sig, su);
-            if constexpr (COMPAT) atomicAdd(&R[w].npred, 1u);
-            L.dag++;
-        }
+    if (lw == d + 1) {
+        atomicAdd(&R[w].sig, su);
+        if constexpr (COMPAT) atomicAdd(&R[w].npred, 1u);
+        L.dag++;
     }
 }
 
@@ -116,8 +130,8 @@ __device__ __forceinline__ void ps_visit(const PsArgs& a, PsRec* R, int* Q,
 constexpr int kNoEdge = -2;
 constexpr int kClaimed = -3;
 
-template <bool BC, bool COMPAT>
-__device__ __forceinline__ void ps_expand_rows(const PsArgs& a, PsRec* R, int* Q,
+template <bool COMPAT>
+__device__ __forceinline__ void ps_expand_rec(const PsArgs& a, PsRec* R, int* Q,
                                                const int* rb, const int* dg, const double* su, int d, int le,
                                                int* s_cnt, PsLocal& L) {
     int w[kPsV][kPsE], lev[kPsV][kPsE], old[kPsV][kPsE];
@@ -151,22 +165,20 @@ __device__ __forceinline__ void ps_expand_rows(const PsArgs& a, PsRec* R, int* Q
                 if (a.dist_sum) atomicAdd((unsigned long long*)&a.dist_sum[x], (unsigned long long)(d + 1));
                 lv = d + 1;
             }
-            if constexpr (BC) {
-                if (lv == d + 1) {
-                    atomicAdd(&R[x].sig, su[m]);
-                    if constexpr (COMPAT) atomicAdd(&R[x].npred, 1u);
-                    L.dag++;
-                }
+            if (lv == d + 1) {
+                atomicAdd(&R[x].sig, su[m]);
+                if constexpr (COMPAT) atomicAdd(&R[x].npred, 1u);
+                L.dag++;
             }
         }
         for (int e = kPsE; e < dg[m]; ++e)
-            ps_visit<BC, COMPAT>(a, R, Q, a.col[rb[m] + e], su[m], d, le, s_cnt, L);
+            ps_visit<COMPAT>(a, R, Q, a.col[rb[m] + e], su[m], d, le, s_cnt, L);
     }
 }
 
 // Backward sums of up to kPsV rows at once: acc[m] = sum over row m's
 // neighbours w at level `lvl` of coef[w] (held in sig), in row order.
-__device__ __forceinline__ void ps_pull_rows(const PsArgs& a, const PsRec* R, const int* rb,
+__device__ __forceinline__ void ps_pull_rec(const PsArgs& a, const PsRec* R, const int* rb,
                                              const int* dg, int lvl, double* acc) {
     int w[kPsV][kPsE], lev[kPsV][kPsE];
     double x[kPsV][kPsE];
@@ -195,8 +207,8 @@ __device__ __forceinline__ void ps_pull_rows(const PsArgs& a, const PsRec* R, co
     }
 }
 
-template <bool BC, bool COMPAT>
-__global__ __launch_bounds__(kPsBlock) void k_ps(PsArgs a) {
+template <bool COMPAT>
+__global__ __launch_bounds__(kPsBlock) void k_ps_bc(PsArgs a) {
     __shared__ int s_cnt[3];     // appends of level d go to s_cnt[d % 3]
     __shared__ int s_nbig[3];    // long rows of level d: s_big[..s_nbig[d % 3])
     __shared__ int s_big[kPsBig];
@@ -208,7 +220,7 @@ __global__ __launch_bounds__(kPsBlock) void k_ps(PsArgs a) {
     PsRec* R = a.rec + so;
     int* Q = a.Q + so;
     int* loff = a.loff + (size_t)blockIdx.x * (size_t)a.lcap;
-    double* bcp = BC ? a.bcp + so : nullptr;
+    double* bcp = a.bcp + so;
     for (int si = blockIdx.x; si < a.k; si += gridDim.x) {
         const int s = a.src[si];
         if (tid == 0) {
@@ -247,7 +259,7 @@ __global__ __launch_bounds__(kPsBlock) void k_ps(PsArgs a) {
                     if (u[m] >= 0) {
                         rb[m] = a.rp[u[m]];
                         dg[m] = a.rp[u[m] + 1] - rb[m];
-                        if (BC) su[m] = ld_sig(&R[u[m]].sig);
+                        su[m] = ld_sig(&R[u[m]].sig);
                     }
                 }
 #pragma unroll
@@ -259,15 +271,15 @@ __global__ __launch_bounds__(kPsBlock) void k_ps(PsArgs a) {
                             dg[m] = 0;
                         }
                     }
-                ps_expand_rows<BC, COMPAT>(a, R, Q, rb, dg, su, d, le, &s_cnt[c], L);
+                ps_expand_rec<COMPAT>(a, R, Q, rb, dg, su, d, le, &s_cnt[c], L);
             }
             __syncthreads();
             const int nbig = min(s_nbig[c], kPsBig);
             for (int t = wid; t < nbig; t += kPsWaves) {
                 const int u = s_big[t];
-                const double su = BC ? ld_sig(&R[u].sig) : 0.0;
+                const double su = ld_sig(&R[u].sig);
                 for (int j = a.rp[u] + lane; j < a.rp[u + 1]; j += kWave)
-                    ps_visit<BC, COMPAT>(a, R, Q, a.col[j], su, d, le, &s_cnt[c], L);
+                    ps_visit<COMPAT>(a, R, Q, a.col[j], su, d, le, &s_cnt[c], L);
             }
             __syncthreads();
             const int nnew = s_cnt[c];
@@ -286,7 +298,7 @@ __global__ __launch_bounds__(kPsBlock) void k_ps(PsArgs a) {
         const int tail = le;
         if (deep && tid == 0) atomicOr(a.overflow, 2);
         // ---------------- backward
-        if (BC && !deep) {
+        if (!deep) {
             if (tid == 0) s_nbig[0] = s_nbig[1] = s_nbig[2] = 0;  // the forward's lists are spent
             __syncthreads();
             for (int dd = depth; dd >= 1; --dd) {
@@ -326,7 +338,7 @@ __global__ __launch_bounds__(kPsBlock) void k_ps(PsArgs a) {
                         }
                     }
                     double acc[kPsV];
-                    ps_pull_rows(a, R, rb, dg, dd + 1, acc);
+                    ps_pull_rec(a, R, rb, dg, dd + 1, acc);
 #pragma unroll
                     for (int m = 0; m < kPsV; ++m) {
                         if (v[m] < 0 || big[m]) continue;
@@ -391,6 +403,146 @@ __global__ __launch_bounds__(kPsBlock) void k_ps(PsArgs a) {
     }
 }
 
+
+// ============================================================ closeness only
+__device__ __forceinline__ bool bit_in(const uint32_t* M, int v) {
+    return (ld_npred(M + (v >> 5)) >> (v & 31)) & 1u;
+}
+// True for the one caller that flips v's bit.
+__device__ __forceinline__ bool bit_claim(uint32_t* M, int v) {
+    const uint32_t m = 1u << (v & 31);
+    return (atomicOr(M + (v >> 5), m) & m) == 0u;
+}
+
+// w has just been claimed for level d + 1.
+__device__ __forceinline__ void ps_claimed(const PsArgs& a, int* Q, int w, int d, int le, int* s_cnt, PsLocal& L) {
+    Q[le + atomicAdd(s_cnt, 1)] = w;
+    L.far += a.weight ? (int64_t)(d + 1) * a.weight[w] : (int64_t)(d + 1);
+    L.reached++;
+    if (a.dist_sum) atomicAdd((unsigned long long*)&a.dist_sum[w], (unsigned long long)(d + 1));
+}
+
+// Expands up to kPsV rows at once, as ps_expand_rec: all column loads, then
+// all map words, then all claims, then the bookkeeping.
+__device__ __forceinline__ void ps_expand_bits(const PsArgs& a, uint32_t* V, int* Q, const int* rb, const int* dg,
+                                               int d, int le, int* s_cnt, PsLocal& L) {
+    int w[kPsV][kPsE];
+    bool seen[kPsV][kPsE], got[kPsV][kPsE];
+#pragma unroll
+    for (int m = 0; m < kPsV; ++m)
+#pragma unroll
+        for (int e = 0; e < kPsE; ++e) w[m][e] = e < dg[m] ? a.col[rb[m] + e] : -1;
+#pragma unroll
+    for (int m = 0; m < kPsV; ++m)
+#pragma unroll
+        for (int e = 0; e < kPsE; ++e) seen[m][e] = w[m][e] < 0 || bit_in(V, w[m][e]);
+#pragma unroll
+    for (int m = 0; m < kPsV; ++m)
+#pragma unroll
+        for (int e = 0; e < kPsE; ++e) got[m][e] = !seen[m][e] && bit_claim(V, w[m][e]);
+#pragma unroll
+    for (int m = 0; m < kPsV; ++m) {
+#pragma unroll
+        for (int e = 0; e < kPsE; ++e)
+            if (got[m][e]) ps_claimed(a, Q, w[m][e], d, le, s_cnt, L);
+        for (int e = kPsE; e < dg[m]; ++e) {
+            const int x = a.col[rb[m] + e];
+            if (!bit_in(V, x) && bit_claim(V, x)) ps_claimed(a, Q, x, d, le, s_cnt, L);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kPsBlock) void k_ps_cc(PsArgs a) {
+    __shared__ int s_cnt[3];     // appends of level d go to s_cnt[d % 3]
+    __shared__ int s_nbig[3];    // long rows of level d: s_big[..s_nbig[d % 3])
+    __shared__ int s_big[kPsBig];
+    __shared__ int64_t s_red64[kPsWaves][2];
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wid = tid >> 6;
+    uint32_t* V = a.bits + (size_t)blockIdx.x * (size_t)a.nw;
+    int* Q = a.Q + (size_t)blockIdx.x * (size_t)a.n;
+    for (int si = blockIdx.x; si < a.k; si += gridDim.x) {
+        const int s = a.src[si];
+        {   // the slot's map empty (16-byte aligned: nw is a multiple of 4)
+            uint4* w4 = (uint4*)V;
+            for (int i = tid; i < a.nw / 4; i += kPsBlock) w4[i] = make_uint4(0u, 0u, 0u, 0u);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            Q[0] = s;
+            atomicOr(V + (s >> 5), 1u << (s & 31));
+            s_cnt[0] = s_cnt[1] = s_cnt[2] = 0;
+            s_nbig[0] = s_nbig[1] = s_nbig[2] = 0;
+        }
+        __syncthreads();
+        PsLocal L;
+        int lb = 0, le = 1, d = 0;
+        for (;;) {
+            const int c = d % 3;
+            if (tid == 0) {  // free the counters level d + 1 will use (last read two levels ago)
+                s_cnt[(d + 1) % 3] = 0;
+                s_nbig[(d + 1) % 3] = 0;
+            }
+            for (int i0 = lb + tid; i0 < le; i0 += kPsBlock * kPsV) {
+                int rb[kPsV], dg[kPsV];
+#pragma unroll
+                for (int m = 0; m < kPsV; ++m) {
+                    const int i = i0 + m * kPsBlock;
+                    const int u = i < le ? Q[i] : -1;
+                    rb[m] = u >= 0 ? a.rp[u] : 0;
+                    dg[m] = u >= 0 ? a.rp[u + 1] - rb[m] : 0;
+                    if (dg[m] > kPsBigDeg) {
+                        const int slot = atomicAdd(&s_nbig[c], 1);
+                        if (slot < kPsBig) {
+                            s_big[slot] = u;
+                            dg[m] = 0;
+                        }
+                    }
+                }
+                ps_expand_bits(a, V, Q, rb, dg, d, le, &s_cnt[c], L);
+            }
+            __syncthreads();
+            const int nbig = min(s_nbig[c], kPsBig);
+            for (int t = wid; t < nbig; t += kPsWaves) {
+                const int u = s_big[t];
+                for (int j = a.rp[u] + lane; j < a.rp[u + 1]; j += kWave) {
+                    const int w = a.col[j];
+                    if (!bit_in(V, w) && bit_claim(V, w)) ps_claimed(a, Q, w, d, le, &s_cnt[c], L);
+                }
+            }
+            __syncthreads();
+            const int nnew = s_cnt[c];
+            if (nnew == 0) break;
+            lb = le;
+            le += nnew;
+            ++d;
+        }
+        int64_t far = L.far, rch = (int64_t)L.reached;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            far += (int64_t)__shfl_xor((long long)far, o);
+            rch += (int64_t)__shfl_xor((long long)rch, o);
+        }
+        if (lane == 0) {
+            s_red64[wid][0] = far;
+            s_red64[wid][1] = rch;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int64_t f = 0, r = 0;
+            for (int w = 0; w < kPsWaves; ++w) {
+                f += s_red64[w][0];
+                r += s_red64[w][1];
+            }
+            a.far[si] = f;
+            atomicAdd(&a.cnt[0], (unsigned long long)r);
+            atomicAdd(a.depth_sum, (unsigned long long)d);
+        }
+        __syncthreads();  // totals read before the next source's clears
+    }
+}
+
 // bc[v] = sum over slots g = 0 .. G-1 of bcp[g][v], in slot order.
 __global__ __launch_bounds__(256) void k_ps_reduce(const double* __restrict__ bcp, int G, int n,
                                                    double* __restrict__ bc) {
@@ -401,12 +553,12 @@ __global__ __launch_bounds__(256) void k_ps_reduce(const double* __restrict__ bc
     bc[v] += t;
 }
 
-// Slot arrays to their idle state: every record {-1, 0, 0}, bcp 0.
+// Betweenness slot arrays to their idle state: every record {-1, 0, 0}, bcp 0.
 __global__ __launch_bounds__(256) void k_ps_init(PsRec* __restrict__ rec, double* __restrict__ bcp, size_t total) {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
         rec[i] = PsRec{-1, 0u, 0.0};
-        if (bcp) bcp[i] = 0.0;
+        bcp[i] = 0.0;
     }
 }
 

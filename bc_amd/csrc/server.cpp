@@ -98,8 +98,9 @@ constexpr int32_t kSourceSlots = 256;
 // Device bytes a request needs besides the graph: the larger of the context
 // workspace of the 64-lane step (per vertex 2 x 512 B of path counts /
 // coefficients, frontier masks, partials) and of the source-parallel step
-// (kSourceSlots slots of 16-byte vertex records, BFS order and BC partials,
-// plus a level table of up to 64 Ki entries each).
+// (kSourceSlots slots, each with per vertex a 16-byte record, a BFS-order
+// entry and a BC partial plus a level table of up to 64 Ki entries for BC,
+// and a BFS-order entry and one map bit for closeness).
 static int64_t workspace_estimate(const GraphEntry* g, int32_t kind, bool all, int64_t k) {
     const int64_t n = std::max<int64_t>(g->n, 1);
     const int64_t slots = std::min<int64_t>(std::max<int64_t>(k, 1), kSourceSlots);
@@ -115,7 +116,7 @@ static int64_t workspace_estimate(const GraphEntry* g, int32_t kind, bool all, i
         return (int64_t)v.np * (8 + 8 + 3 * 8 + 64) + (16 << 20);
     }
     const int64_t dense = n * (8 + 3 * 8 + 8 + 64) + k * 16 + (16 << 20);
-    const int64_t source = slots * (n * (16 + 4) + lcap * 4) + k * 16 + (16 << 20);
+    const int64_t source = slots * (n * (4 + 1) + lcap * 4) + k * 16 + (16 << 20);
     return std::max(dense, source);
 }
 

@@ -182,10 +182,16 @@ const char* cent_ctx_last_error(const cent_ctx* ctx);
 int  cent_probe_copy_bandwidth(cent_ctx* ctx, int64_t bytes, int32_t iters, double* gbs);
 
 /* Uploads a CSR (validated; ids in range, offsets monotone).  By default the
- * device copy is relabelled by descending degree for cache locality; every
- * input and output of the engine stays in the caller's ids either way.
- * flags: CENT_GRAPH_KEEP_ORDER keeps the caller's ids on the device too. */
+ * device copy is relabelled for cache locality: by descending degree when
+ * the BFS from the highest-degree vertex is at most 48 levels deep, else in
+ * BFS order from that vertex (each further component in BFS order from its
+ * smallest id).  Every input and output of the engine stays in the caller's
+ * ids either way.
+ * flags: CENT_GRAPH_KEEP_ORDER keeps the caller's ids on the device too;
+ * CENT_GRAPH_DEGREE_ORDER / CENT_GRAPH_BFS_ORDER force either relabel. */
 #define CENT_GRAPH_KEEP_ORDER 1
+#define CENT_GRAPH_DEGREE_ORDER 4
+#define CENT_GRAPH_BFS_ORDER 8
 int  cent_graph_create(cent_ctx* ctx, const int32_t* row_ptr, const int32_t* col,
                        int32_t n, int64_t nnz, int32_t flags, cent_graph** out);
 int  cent_graph_upload(cent_ctx* ctx, const struct CSR* csr, cent_graph** out);

@@ -3,6 +3,7 @@
 Bar: farness / distance sums / reached counts bit-exact (int64); BC within
 1e-9 relative (FP64, summation order differs from the oracle's).
 """
+import ctypes as C
 import json
 import os
 
@@ -143,17 +144,40 @@ def test_large_rmat_properties(ctx):
     assert np.array_equal(far2, ofar) and np.array_equal(ds2, ods)
 
 
-def test_device_order_does_not_change_results(ctx):
-    """Degree-relabelled device layout (default) vs the caller's order."""
-    g = bc_amd.gen_rmat(15, 16 << 15, seed=4)
+@pytest.mark.parametrize("graph", ["rmat", "grid"])
+@pytest.mark.parametrize("order", ["keep", "degree", "bfs"])
+def test_device_order_does_not_change_results(ctx, graph, order):
+    """Default device layout (degree order for the shallow R-MAT, BFS order for
+    the deep grid) vs each forced layout, on the step the graph runs by
+    default (dense 64-lane for R-MAT, source-parallel for the grid), BC and
+    closeness, against the oracle."""
+    print(f"[gpu] device order {order} on {graph}", flush=True)
+    if graph == "rmat":
+        g = bc_amd.gen_rmat(15, 16 << 15, seed=4)
+    else:
+        g = bc_amd.gen_grid(256, 256, 0.05, 20, seed=6)
     src = bc_amd.sample_sources(g.n, 100, 4)
     a = ctx.bc(ctx.upload(g), src, dist_sum=True)
-    b = ctx.bc(ctx.upload(g, keep_order=True), src, dist_sum=True)
+    dg = ctx.upload(g, order=order)
+    b = ctx.bc(dg, src, dist_sum=True)
     assert_bc_close(a[0], b[0])
     assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+    cf, cd = ctx.cc(dg, src, dist_sum=True)
+    assert np.array_equal(cf, a[1]) and np.array_equal(cd, a[2])
     obc, ofar, ods, _ = O.brandes(g, src)
     assert_bc_close(a[0], obc)
     assert np.array_equal(a[1], ofar) and np.array_equal(a[2], ods)
+
+
+def test_device_order_flags_exclusive(ctx):
+    g = bc_amd.gen_rmat(8, 8 << 8, seed=1)
+    h = C.c_void_p()
+    flags = bc_amd.CENT_GRAPH_DEGREE_ORDER | bc_amd.CENT_GRAPH_BFS_ORDER
+    rc = bc_amd._lib.cent_graph_create(ctx._h, bc_amd._ptr(g.row_ptr, C.c_int32), bc_amd._ptr(g.col, C.c_int32),
+                                       g.n, g.nnz, flags, C.byref(h))
+    assert rc == bc_amd.CENT_E_INVAL and not h.value
+    with pytest.raises(ValueError):
+        ctx.upload(g, order="random")
 
 
 @pytest.mark.parametrize("frontier", ["dense", "sparse", "source"])
@@ -214,13 +238,14 @@ def test_source_mode_deterministic(ctx):
     assert np.array_equal(a, b)
 
 
-def test_source_mode_deep_fallback(ctx):
-    """A BFS deeper than the source-parallel level table (path of 70000
-    vertices from an end) is redone in the batch form: farness n(n-1)/2."""
+def test_source_mode_deep_closeness(ctx):
+    """A BFS deeper than the betweenness step's level table (path of 70000
+    vertices from an end): the closeness step keeps no level table and
+    finishes it itself: farness n(n-1)/2, distance sums 0 .. n-1."""
     n = 70000
     g = bc_amd.CSR.from_edges(n, np.arange(n - 1), np.arange(1, n))
     dg = ctx.upload(g, keep_order=True)
-    print("[gpu] source-mode deep fallback, path 70000", flush=True)
+    print("[gpu] source-mode deep closeness, path 70000", flush=True)
     far, ds = ctx.cc(dg, [0], dist_sum=True, frontier="source")
     assert far[0] == n * (n - 1) // 2
     assert np.array_equal(ds, np.arange(n, dtype=np.int64))
