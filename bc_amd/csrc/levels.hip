@@ -37,8 +37,7 @@ static int ensure_workspace(cent_ctx* ctx, const DevCSR& d, const BatchSpec& sp,
     }
     ctx->F_levels = ctx->F.cap / (N * fw) - 1;
     if (sp.with_bc) {
-        TRY(ensure(ctx, ctx->sigma, N * kWave));
-        TRY(ensure(ctx, ctx->coef, N * kWave));
+        TRY(ensure(ctx, ctx->sigma, N * kWave));  // coef is written over it in the backward
         if (sp.compat) TRY(ensure(ctx, ctx->npred, N * kWave));
     }
     TRY(ensure(ctx, ctx->part_far, (size_t)G * kWave * nb));
@@ -303,7 +302,7 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
             ba.rp = d.rp; ba.col = d.col;
             ba.Fd = ctx->F.p + PL * (size_t)lv;
             ba.Fn = ctx->F.p + PL * (size_t)(lv + 1);
-            ba.sigma = ctx->sigma.p; ba.coef = ctx->coef.p;
+            ba.sigma = ctx->sigma.p; ba.coef = ctx->sigma.p;  // coef over sigma, in place
             ba.npred = compat ? ctx->npred.p : nullptr;
             ba.bc = sp.bc;
             ba.wgt = sp.bwgt;

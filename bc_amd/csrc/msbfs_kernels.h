@@ -5,8 +5,9 @@
 //   F[d][v]       {mask, base} record: lanes that reach v at distance d and
 //                 the number of lanes that reached it earlier (one plane per
 //                 level, kept for the backward pass)
-//   sigma[v][64]  double  shortest-path counts, level-sorted (below)
-//   coef[v][64]   double  (1 + delta) / sigma, written by the backward pass
+//   sigma[v][64]  double  shortest-path counts, level-sorted (below); the
+//                         backward writes coef = (1 + delta) / sigma over
+//                         each entry in place
 // so a neighbour's frontier test is one 16-byte load for all 64 sources and a
 // path-count gather reads one contiguous slice of a 512-byte row.  The
 // closeness step (masks only, up to 8 batches interleaved) is in
@@ -89,8 +90,12 @@ struct BwdArgs {
     const int* __restrict__ col;
     const uint64_t* __restrict__ Fd;      // level d masks
     const uint64_t* __restrict__ Fn;      // level d + 1 masks
-    const double* __restrict__ sigma;
-    double* __restrict__ coef;
+    // coef may be sigma itself (the dense step writes coef over sigma in
+    // place: a lane's entry of v is read as sigma and rewritten as coef at
+    // v's level for that lane, while the level's gathers read the entries
+    // of lanes one level deeper), so neither is restrict.
+    const double* sigma;
+    double* coef;
     const uint32_t* __restrict__ npred;
     double* __restrict__ bc;
     // weighted dependencies (reduced graphs): target weight wgt[w] replaces the

@@ -96,8 +96,8 @@ static double now_ms(const cent_server* s) {
 constexpr int32_t kSourceSlots = 256;
 
 // Device bytes a request needs besides the graph: the larger of the context
-// workspace of the 64-lane step (per vertex 2 x 512 B of path counts /
-// coefficients, frontier masks, partials) and of the source-parallel step
+// workspace of the 64-lane step (per vertex 512 B of path counts, which the
+// backward overwrites with coefficients, frontier masks, partials) and of the source-parallel step
 // (kSourceSlots slots, each with per vertex a 16-byte record, a BFS-order
 // entry and a BC partial plus a level table of up to 64 Ki entries for BC,
 // and a BFS-order entry and one map bit for closeness).
@@ -106,7 +106,7 @@ static int64_t workspace_estimate(const GraphEntry* g, int32_t kind, bool all, i
     const int64_t slots = std::min<int64_t>(std::max<int64_t>(k, 1), kSourceSlots);
     const int64_t lcap = std::min<int64_t>(n + 2, 65536);
     if (kind == CENT_Q_BC) {
-        const int64_t dense = n * (1024 + 8 + 17 * 8 + 8 + 64) + k * 16 + (64 << 20);
+        const int64_t dense = n * (512 + 8 + 17 * 8 + 8 + 64) + k * 16 + (64 << 20);
         const int64_t source = slots * (n * (16 + 4 + 8) + lcap * 4) + n * 8 + k * 16 + (16 << 20);
         return std::max(dense, source);
     }
