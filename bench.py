@@ -164,13 +164,32 @@ def extra_grid(ctx, args):
     ctx.bc(dg, src, timing=True)
     t = time.perf_counter() - t1
     st = ctx.stats()
+    # bytes model per source (the graph is connected): Brandes 32m + 16 D_s + 60n,
+    # plain BFS farness 8m + 12n; the source-parallel step is one launch
+    k = len(src)
+    b_bc = k * (32 * g.m + 60 * g.n) + 16 * st["dag_edges"]
+    kms = st["ms_fwd"]
     log(f"grid: {t * 1e3:.1f} ms, levels/batch {st['levels'] / max(1, st['batches']):.1f}, "
-        f"fwd {st['ms_fwd']:.1f} ms bwd {st['ms_bwd']:.1f} ms")
+        f"kernel {kms:.1f} ms, {b_bc / (kms * 1e-3) / 1e9:.0f} GB/s by the bytes model")
+    ctx.reset_stats()
+    t2 = time.perf_counter()
+    ctx.cc(dg, src, timing=True)
+    tc = time.perf_counter() - t2
+    kc = ctx.stats()["ms_fwd"]
+    b_cc = k * (8 * g.m + 12 * g.n)
+    log(f"grid closeness only: {tc * 1e3:.1f} ms, kernel {kc:.1f} ms, "
+        f"{b_cc / (kc * 1e-3) / 1e9:.0f} GB/s by the bytes model")
     return {"workload": "sampled BC + closeness, 2048^2 grid (5% removed, LCC, +1000 shortcuts), "
                         "1024 sources", "n": g.n, "m": g.m, "ms": round(t * 1e3, 2),
-            "TEPS": round(1024 * g.m / t, 1),
+            "TEPS": round(k * g.m / t, 1),
             "levels_per_batch": round(st["levels"] / max(1, st["batches"]), 2),
-            "fwd_ms": round(st["ms_fwd"], 2), "bwd_ms": round(st["ms_bwd"], 2)}
+            "kernel": "k_ps_bc", "kernel_ms": round(kms, 2),
+            "algorithmic_GBs": round(b_bc / (kms * 1e-3) / 1e9, 1),
+            "frac": round(b_bc / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "closeness_only": {"kernel": "k_ps_cc", "ms": round(tc * 1e3, 2), "kernel_ms": round(kc, 2),
+                               "TEPS": round(k * g.m / tc, 1),
+                               "algorithmic_GBs": round(b_cc / (kc * 1e-3) / 1e9, 1),
+                               "frac": round(b_cc / (kc * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
 
 
 def extra_closeness(ctx, args):
