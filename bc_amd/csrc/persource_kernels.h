@@ -58,6 +58,7 @@ constexpr int kPsBigDeg = 64;   // longer rows are scanned by a wavefront
 constexpr int kPsBig = 1024;    // LDS capacity of the long-row list per level
 constexpr int kPsV = 4;         // rows a thread works on at once
 constexpr int kPsE = 4;         // edges per row issued together
+constexpr int kPsHubE = 8;      // edges per lane issued together on a long row
 
 struct PsArgs {
     int n;
@@ -176,6 +177,69 @@ __device__ __forceinline__ void ps_expand_rec(const PsArgs& a, PsRec* R, int* Q,
     }
 }
 
+// Forward edges of a long row [rb, re) for one lane of a wavefront: edges
+// j, j + 64, ... taken kPsHubE at a time, staged like ps_expand_rec (column
+// loads, level loads, claims, bookkeeping), so a lane keeps kPsHubE
+// independent chains in flight instead of one.
+template <bool COMPAT>
+__device__ __forceinline__ void ps_expand_long(const PsArgs& a, PsRec* R, int* Q, int rb, int re, int lane,
+                                               double su, int d, int le, int* s_cnt, PsLocal& L) {
+    for (int j0 = rb + lane; j0 < re; j0 += kWave * kPsHubE) {
+        int w[kPsHubE], lev[kPsHubE], old[kPsHubE];
+#pragma unroll
+        for (int e = 0; e < kPsHubE; ++e) {
+            const int j = j0 + e * kWave;
+            w[e] = j < re ? a.col[j] : -1;
+        }
+#pragma unroll
+        for (int e = 0; e < kPsHubE; ++e) lev[e] = w[e] >= 0 ? ld_dist(&R[w[e]].dist) : kNoEdge;
+#pragma unroll
+        for (int e = 0; e < kPsHubE; ++e) old[e] = lev[e] == -1 ? atomicCAS(&R[w[e]].dist, -1, d + 1) : 0;
+#pragma unroll
+        for (int e = 0; e < kPsHubE; ++e) {
+            const int x = w[e];
+            int lv = lev[e];
+            if (lv == -1) lv = old[e] < 0 ? kClaimed : old[e];
+            if (lv == kClaimed) {
+                Q[le + atomicAdd(s_cnt, 1)] = x;
+                L.far += a.weight ? (int64_t)(d + 1) * a.weight[x] : (int64_t)(d + 1);
+                L.reached++;
+                if (a.dist_sum) atomicAdd((unsigned long long*)&a.dist_sum[x], (unsigned long long)(d + 1));
+                lv = d + 1;
+            }
+            if (lv == d + 1) {
+                atomicAdd(&R[x].sig, su);
+                if constexpr (COMPAT) atomicAdd(&R[x].npred, 1u);
+                L.dag++;
+            }
+        }
+    }
+}
+
+// Backward partial sum of a long row [rb, re) for one lane: coef[w] over the
+// lane's edges j, j + 64, ... to neighbours at level `lvl`, kPsHubE edges'
+// loads in flight at a time; fixed order, so the wavefront sum that follows
+// is deterministic.
+__device__ __forceinline__ double ps_pull_long(const PsArgs& a, const PsRec* R, int rb, int re, int lane, int lvl) {
+    double acc = 0.0;
+    for (int j0 = rb + lane; j0 < re; j0 += kWave * kPsHubE) {
+        int w[kPsHubE], lev[kPsHubE];
+        double x[kPsHubE];
+#pragma unroll
+        for (int e = 0; e < kPsHubE; ++e) {
+            const int j = j0 + e * kWave;
+            w[e] = j < re ? a.col[j] : -1;
+        }
+#pragma unroll
+        for (int e = 0; e < kPsHubE; ++e) lev[e] = w[e] >= 0 ? ld_dist(&R[w[e]].dist) : kNoEdge;
+#pragma unroll
+        for (int e = 0; e < kPsHubE; ++e) x[e] = lev[e] == lvl ? ld_sig(&R[w[e]].sig) : 0.0;
+#pragma unroll
+        for (int e = 0; e < kPsHubE; ++e) acc += x[e];
+    }
+    return acc;
+}
+
 // Backward sums of up to kPsV rows at once: acc[m] = sum over row m's
 // neighbours w at level `lvl` of coef[w] (held in sig), in row order.
 __device__ __forceinline__ void ps_pull_rec(const PsArgs& a, const PsRec* R, const int* rb,
@@ -277,9 +341,7 @@ __global__ __launch_bounds__(kPsBlock) void k_ps_bc(PsArgs a) {
             const int nbig = min(s_nbig[c], kPsBig);
             for (int t = wid; t < nbig; t += kPsWaves) {
                 const int u = s_big[t];
-                const double su = ld_sig(&R[u].sig);
-                for (int j = a.rp[u] + lane; j < a.rp[u + 1]; j += kWave)
-                    ps_visit<COMPAT>(a, R, Q, a.col[j], su, d, le, &s_cnt[c], L);
+                ps_expand_long<COMPAT>(a, R, Q, a.rp[u], a.rp[u + 1], lane, ld_sig(&R[u].sig), d, le, &s_cnt[c], L);
             }
             __syncthreads();
             const int nnew = s_cnt[c];
@@ -352,12 +414,7 @@ __global__ __launch_bounds__(kPsBlock) void k_ps_bc(PsArgs a) {
                 const int nbig = min(s_nbig[c], kPsBig);
                 for (int t = wid; t < nbig; t += kPsWaves) {
                     const int v = s_big[t];
-                    double acc = 0.0;
-                    for (int j = a.rp[v] + lane; j < a.rp[v + 1]; j += kWave) {
-                        const int w = a.col[j];
-                        if (ld_dist(&R[w].dist) == dd + 1) acc += ld_sig(&R[w].sig);
-                    }
-                    acc = wave_sum(acc);
+                    const double acc = wave_sum(ps_pull_long(a, R, a.rp[v], a.rp[v + 1], lane, dd + 1));
                     if (lane == 0) {
                         const double sv = ld_sig(&R[v].sig);
                         const double dl = sv * acc;
@@ -506,9 +563,20 @@ __global__ __launch_bounds__(kPsBlock) void k_ps_cc(PsArgs a) {
             const int nbig = min(s_nbig[c], kPsBig);
             for (int t = wid; t < nbig; t += kPsWaves) {
                 const int u = s_big[t];
-                for (int j = a.rp[u] + lane; j < a.rp[u + 1]; j += kWave) {
-                    const int w = a.col[j];
-                    if (!bit_in(V, w) && bit_claim(V, w)) ps_claimed(a, Q, w, d, le, &s_cnt[c], L);
+                const int re = a.rp[u + 1];
+                for (int j0 = a.rp[u] + lane; j0 < re; j0 += kWave * kPsHubE) {  // staged as ps_expand_bits
+                    int w[kPsHubE];
+                    bool seen[kPsHubE];
+#pragma unroll
+                    for (int e = 0; e < kPsHubE; ++e) {
+                        const int j = j0 + e * kWave;
+                        w[e] = j < re ? a.col[j] : -1;
+                    }
+#pragma unroll
+                    for (int e = 0; e < kPsHubE; ++e) seen[e] = w[e] < 0 || bit_in(V, w[e]);
+#pragma unroll
+                    for (int e = 0; e < kPsHubE; ++e)
+                        if (!seen[e] && bit_claim(V, w[e])) ps_claimed(a, Q, w[e], d, le, &s_cnt[c], L);
                 }
             }
             __syncthreads();

@@ -228,6 +228,28 @@ def test_source_mode_slot_reuse(ctx, slots):
         assert np.array_equal(cfar, ofar) and np.array_equal(cds, ods)
 
 
+def test_source_mode_closeness_all_sources(ctx):
+    """Closeness-only source-parallel step at its widest: every vertex of a
+    small graph is a source, so the slot rule runs 1024 slots with their bit
+    maps side by side (map words shared by up to 32 vertices); isolated
+    vertices and a second component included.  Farness and distance sums
+    bit-exact against the oracle, and against the 64-lane step."""
+    g0 = bc_amd.gen_rmat(12, 16 << 12, seed=8, lcc=False)
+    n = g0.n + 40  # 40 more vertices: a path of 37 (its own component) and 3 isolated
+    u = np.concatenate([np.repeat(np.arange(g0.n), np.diff(g0.row_ptr)), np.arange(g0.n, g0.n + 36)])
+    v = np.concatenate([g0.col, np.arange(g0.n + 1, g0.n + 37)])
+    keep = u < v
+    g = bc_amd.CSR.from_edges(n, u[keep], v[keep])
+    print(f"[gpu] source-mode closeness, all {g.n} sources", flush=True)
+    dg = ctx.upload(g)
+    far, ds = ctx.cc(dg, None, dist_sum=True, frontier="source")
+    ofar = O.farness(g, np.arange(g.n, dtype=np.int32))
+    assert np.array_equal(far, ofar)
+    assert ds.sum() == far.sum()
+    f2, d2 = ctx.cc(dg, None, dist_sum=True, frontier="dense")
+    assert np.array_equal(f2, far) and np.array_equal(d2, ds)
+
+
 def test_source_mode_deterministic(ctx):
     """Same slot count, same bits: BC partials are reduced in slot order."""
     g = bc_amd.gen_grid(48, 48, 0.05, 8, seed=2)
