@@ -1,11 +1,11 @@
-// Closeness level step over W side-by-side 64-source batches (W = 1, 2, 4
-// or 8; the batches of a launch group are numbered b = 0 .. nb - 1 <= W - 1).
-// The masks are interleaved by vertex:
+// Closeness level step over W side-by-side 64-source batches (W = 1, 2, 4,
+// 8, 16 or 32; the batches of a launch group are numbered b = 0 .. nb - 1 <=
+// W - 1).  The masks are interleaved by vertex:
 //   vis[v * W + b]  lanes of batch b that have reached v
 //   F[v * W + b]    lanes of batch b at the current distance
 // so one scan of v's row serves every batch, and a neighbour's masks for all
-// batches are one contiguous W x 8-byte read (one 64-byte segment at W = 8)
-// instead of W reads from W separate planes.
+// batches are one contiguous W x 8-byte read (one 128-byte cache line at
+// W = 16, two at W = 32) instead of W reads from W separate planes.
 //
 // Lane layout inside a wave: b = lane % W is the batch, e = lane / W the edge
 // slot, so a row is read E = 64 / W edges at a time and each lane ORs the
@@ -94,7 +94,7 @@ __device__ __forceinline__ uint64_t ccw_scan(const CcwArgs& a, int beg, int end,
 template <int W>
 __device__ __forceinline__ void ccw_count(long long (*s_far)[kWave], uint64_t nm, long long w, int lane) {
     const int b = lane & (W - 1), e = lane / W;
-    const uint32_t bits = (uint32_t)(nm >> (e * W)) & ((1u << W) - 1u);
+    const uint32_t bits = (uint32_t)((nm >> (e * W)) & (W == 32 ? 0xffffffffull : ((1ull << W) - 1ull)));
     for (uint32_t m = bits; m; m &= m - 1) atomicAdd((unsigned long long*)&s_far[b][e * W + __builtin_ctz(m)], (unsigned long long)w);
 }
 

@@ -21,7 +21,7 @@ static int grid_for(int chunks, const cent_opts* o) {
 // Workspace for a DevCSR with N vertices; the F store grows on demand.
 // Batches side by side in the closeness step's interleaved masks: nb
 // rounded up to a power of two.
-static inline int ccw_width(int nb) { return nb <= 1 ? 1 : nb <= 2 ? 2 : nb <= 4 ? 4 : 8; }
+static inline int ccw_width(int nb) { return nb <= 1 ? 1 : nb <= 2 ? 2 : nb <= 4 ? 4 : nb <= 8 ? 8 : nb <= 16 ? 16 : 32; }
 
 static int ensure_workspace(cent_ctx* ctx, const DevCSR& d, const BatchSpec& sp, int G, int nb) {
     const size_t N = (size_t)(d.n > 0 ? d.n : 1);
@@ -180,7 +180,9 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
             case 1: k_ccw_seed<1><<<nb, kWave, 0, s>>>(d_src, nb, kl, ctx->vis.p); break;
             case 2: k_ccw_seed<2><<<nb, kWave, 0, s>>>(d_src, nb, kl, ctx->vis.p); break;
             case 4: k_ccw_seed<4><<<nb, kWave, 0, s>>>(d_src, nb, kl, ctx->vis.p); break;
-            default: k_ccw_seed<8><<<nb, kWave, 0, s>>>(d_src, nb, kl, ctx->vis.p); break;
+            case 8: k_ccw_seed<8><<<nb, kWave, 0, s>>>(d_src, nb, kl, ctx->vis.p); break;
+            case 16: k_ccw_seed<16><<<nb, kWave, 0, s>>>(d_src, nb, kl, ctx->vis.p); break;
+            default: k_ccw_seed<32><<<nb, kWave, 0, s>>>(d_src, nb, kl, ctx->vis.p); break;
         }
     }
     HIPTRY(hipGetLastError());
@@ -234,7 +236,9 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
                     case 1: ccw_level<1>(ca, lv, G, Gp, Gseg, Ghub, v.nhub > 0, nb, d_src, s); break;
                     case 2: ccw_level<2>(ca, lv, G, Gp, Gseg, Ghub, v.nhub > 0, nb, d_src, s); break;
                     case 4: ccw_level<4>(ca, lv, G, Gp, Gseg, Ghub, v.nhub > 0, nb, d_src, s); break;
-                    default: ccw_level<8>(ca, lv, G, Gp, Gseg, Ghub, v.nhub > 0, nb, d_src, s); break;
+                    case 8: ccw_level<8>(ca, lv, G, Gp, Gseg, Ghub, v.nhub > 0, nb, d_src, s); break;
+                    case 16: ccw_level<16>(ca, lv, G, Gp, Gseg, Ghub, v.nhub > 0, nb, d_src, s); break;
+                    default: ccw_level<32>(ca, lv, G, Gp, Gseg, Ghub, v.nhub > 0, nb, d_src, s); break;
                 }
                 HIPTRY(hipGetLastError());
                 TRY(timer_end(ctx, tm, e0, true));
@@ -723,7 +727,16 @@ static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
     SpPlan plan;
     if (sparse) TRY(sp_prepare(ctx, g->d, sp, plan));
     // closeness on the dense step takes up to kMultiBatch batches per launch
-    const int32_t step = (!with_bc && !sparse) ? kWave * kMultiBatch : kWave;
+    // group, halved while that leaves fewer than four groups (two per stream)
+    // and a group keeps eight or more (R-MAT 2^20, 4096 sources: 16 batches a
+    // group 9.7 ms, 32 10.2 ms; all-vertex on a 148k-vertex R-MAT: 32 a group
+    // 68.7 ms, 16 85.3 ms, 8 108 ms)
+    int gb = kMultiBatch;
+    {
+        const int nbt = (int)((k + kWave - 1) / kWave);
+        while (gb > 8 && (nbt + gb - 1) / gb < 4) gb /= 2;
+    }
+    const int32_t step = (!with_bc && !sparse) ? kWave * gb : kWave;
     // Level bound of a launch group on the dense step: a BFS from s ends
     // within d(s, hub) + ecc(hub) levels (hub = the highest-degree vertex of
     // the upload probe, ecc its BFS depth), so a group whose sources all lie
