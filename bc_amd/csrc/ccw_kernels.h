@@ -94,8 +94,8 @@ __device__ __forceinline__ uint64_t ccw_scan(const CcwArgs& a, int beg, int end,
 template <int W>
 __device__ __forceinline__ void ccw_count(long long (*s_far)[kWave], uint64_t nm, long long w, int lane) {
     const int b = lane & (W - 1), e = lane / W;
-    const uint32_t bits = (uint32_t)((nm >> (e * W)) & (W == 32 ? 0xffffffffull : ((1ull << W) - 1ull)));
-    for (uint32_t m = bits; m; m &= m - 1) atomicAdd((unsigned long long*)&s_far[b][e * W + __builtin_ctz(m)], (unsigned long long)w);
+    const uint64_t bits = W == 64 ? nm : (nm >> (e * W)) & ((1ull << (W & 63)) - 1ull);
+    for (uint64_t m = bits; m; m &= m - 1) atomicAdd((unsigned long long*)&s_far[b][e * W + __builtin_ctzll(m)], (unsigned long long)w);
 }
 
 // Finishes vertex v: writes the new masks, vis, the lane partials and the

@@ -305,7 +305,7 @@ struct BatchSpec {
 // Closeness batches of 64 sources that one dense level launch carries at
 // most (32: a vertex's masks for all of them fill two 128-byte cache lines,
 // and one row scan serves 2048 sources).
-constexpr int kMultiBatch = 32;
+constexpr int kMultiBatch = 64;
 
 // Batches of the k sources d_src[0..k) (device ids) inside the view: one
 // 64-lane batch for BC (k <= 64), up to kMultiBatch for closeness.

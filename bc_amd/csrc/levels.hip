@@ -21,7 +21,7 @@ static int grid_for(int chunks, const cent_opts* o) {
 // Workspace for a DevCSR with N vertices; the F store grows on demand.
 // Batches side by side in the closeness step's interleaved masks: nb
 // rounded up to a power of two.
-static inline int ccw_width(int nb) { return nb <= 1 ? 1 : nb <= 2 ? 2 : nb <= 4 ? 4 : nb <= 8 ? 8 : nb <= 16 ? 16 : 32; }
+static inline int ccw_width(int nb) { return nb <= 1 ? 1 : nb <= 2 ? 2 : nb <= 4 ? 4 : nb <= 8 ? 8 : nb <= 16 ? 16 : nb <= 32 ? 32 : 64; }
 
 static int ensure_workspace(cent_ctx* ctx, const DevCSR& d, const BatchSpec& sp, int G, int nb) {
     const size_t N = (size_t)(d.n > 0 ? d.n : 1);
@@ -182,7 +182,8 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
             case 4: k_ccw_seed<4><<<nb, kWave, 0, s>>>(d_src, nb, kl, ctx->vis.p); break;
             case 8: k_ccw_seed<8><<<nb, kWave, 0, s>>>(d_src, nb, kl, ctx->vis.p); break;
             case 16: k_ccw_seed<16><<<nb, kWave, 0, s>>>(d_src, nb, kl, ctx->vis.p); break;
-            default: k_ccw_seed<32><<<nb, kWave, 0, s>>>(d_src, nb, kl, ctx->vis.p); break;
+            case 32: k_ccw_seed<32><<<nb, kWave, 0, s>>>(d_src, nb, kl, ctx->vis.p); break;
+            default: k_ccw_seed<64><<<nb, kWave, 0, s>>>(d_src, nb, kl, ctx->vis.p); break;
         }
     }
     HIPTRY(hipGetLastError());
@@ -238,7 +239,8 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
                     case 4: ccw_level<4>(ca, lv, G, Gp, Gseg, Ghub, v.nhub > 0, nb, d_src, s); break;
                     case 8: ccw_level<8>(ca, lv, G, Gp, Gseg, Ghub, v.nhub > 0, nb, d_src, s); break;
                     case 16: ccw_level<16>(ca, lv, G, Gp, Gseg, Ghub, v.nhub > 0, nb, d_src, s); break;
-                    default: ccw_level<32>(ca, lv, G, Gp, Gseg, Ghub, v.nhub > 0, nb, d_src, s); break;
+                    case 32: ccw_level<32>(ca, lv, G, Gp, Gseg, Ghub, v.nhub > 0, nb, d_src, s); break;
+                    default: ccw_level<64>(ca, lv, G, Gp, Gseg, Ghub, v.nhub > 0, nb, d_src, s); break;
                 }
                 HIPTRY(hipGetLastError());
                 TRY(timer_end(ctx, tm, e0, true));
