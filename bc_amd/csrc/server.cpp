@@ -110,12 +110,15 @@ static int64_t workspace_estimate(const GraphEntry* g, int32_t kind, bool all, i
         const int64_t source = slots * (n * (16 + 4 + 8) + lcap * 4) + n * 8 + k * 16 + (16 << 20);
         return std::max(dense, source);
     }
+    // closeness: a launch group of up to 64 interleaved batches keeps, per
+    // vertex, 64 reached-masks and two planes of 64 frontier masks (1.5 KB)
+    constexpr int64_t kCcVertex = 8 * (3 * 64 + 1) + 8 + 64;
     if (all && g->red) {
         cent_reduced_view v;
         cent_reduced_get_view(g->red, &v);
-        return (int64_t)v.np * (8 + 8 + 3 * 8 + 64) + (16 << 20);
+        return (int64_t)v.np * kCcVertex + (16 << 20);
     }
-    const int64_t dense = n * (8 + 3 * 8 + 8 + 64) + k * 16 + (16 << 20);
+    const int64_t dense = n * kCcVertex + k * 16 + (16 << 20);
     const int64_t source = slots * (n * (4 + 1) + lcap * 4) + k * 16 + (16 << 20);
     return std::max(dense, source);
 }
