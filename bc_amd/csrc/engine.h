@@ -235,17 +235,24 @@ struct cent_graph {
     int32_t frontier[2] = {CENT_FRONTIER_AUTO, CENT_FRONTIER_AUTO};
 };
 
-// Level step for a plain call.  Deep graphs (BFS depth above kDeepLevels
-// from the highest-degree vertex, probed at upload) go source-parallel: a
-// 64-lane batch there spreads its lanes over different levels and shares
-// no work.  Otherwise the dense 64-lane pull.  The frontier-queue step is
-// chosen only on request (or by the autotuner).
+// Level step for a plain call that neither the call nor the graph's tuned
+// settings choose.  Deep graphs (BFS depth above kDeepLevels from the
+// highest-degree vertex, probed at upload) go source-parallel: a 64-lane
+// batch there spreads its lanes over different levels and shares no work.
+// So does betweenness on graphs of at most kSmallBcVertices vertices, where
+// the 64-lane step's per-level launches outweigh the traversals (the tuned
+// table picks the same for R-MAT up to 2^13 vertices and for block trees up
+// to 113k; a 1554-vertex R-MAT with no table entry: 1024-source BC 12.9 ms
+// on the 64-lane step).  Otherwise the dense 64-lane pull.  The
+// frontier-queue step is chosen only on request (or by the autotuner).
 constexpr int kDeepLevels = 48;
+constexpr int kSmallBcVertices = 16384;
 
-static inline int level_mode(const cent_graph* g, const cent_opts* o) {
+static inline int level_mode(const cent_graph* g, const cent_opts* o, bool with_bc) {
     const int m = o ? o->frontier : CENT_FRONTIER_AUTO;
     if (m != CENT_FRONTIER_AUTO) return m;
-    return g->ecc_probe > kDeepLevels ? CENT_FRONTIER_SOURCE : CENT_FRONTIER_DENSE;
+    if (g->ecc_probe > kDeepLevels || (with_bc && g->n <= kSmallBcVertices)) return CENT_FRONTIER_SOURCE;
+    return CENT_FRONTIER_DENSE;
 }
 
 // ------------------------------------------------------------ timing

@@ -667,7 +667,7 @@ static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
     // distance to the highest-degree vertex, then by degree, so the lanes
     // of a batch tend to reach a vertex at the same levels (fewer
     // (vertex, level) rows touched).  order[i] = caller index of slot i.
-    const int lm = k > 0 ? level_mode(g, opts) : CENT_FRONTIER_DENSE;
+    const int lm = k > 0 ? level_mode(g, opts, with_bc) : CENT_FRONTIER_DENSE;
     std::vector<int32_t> order;
     if (k > kWave && lm != CENT_FRONTIER_SOURCE && !(opts->flags & CENT_OPT_SOURCE_ORDER) &&
         !g->probe_dist.empty()) {

@@ -386,17 +386,17 @@ __global__ __launch_bounds__(kPsBlock) void k_ps_bc(PsArgs a) {
                             sv[m] = ld_sig(&R[v[m]].sig);
                         }
                     }
+                    // long rows always go to the wavefronts (listed while the
+                    // list has room), so each row's sum has one order whoever
+                    // takes it and BC has the same bits on every run
                     bool big[kPsV];
 #pragma unroll
                     for (int m = 0; m < kPsV; ++m) {
-                        big[m] = false;
-                        if (dg[m] > kPsBigDeg) {
+                        big[m] = dg[m] > kPsBigDeg;
+                        if (big[m]) {
                             const int slot = atomicAdd(&s_nbig[c], 1);
-                            if (slot < kPsBig) {
-                                s_big[slot] = v[m];
-                                big[m] = true;
-                                dg[m] = 0;
-                            }
+                            if (slot < kPsBig) s_big[slot] = v[m];
+                            dg[m] = 0;
                         }
                     }
                     double acc[kPsV];
@@ -411,9 +411,7 @@ __global__ __launch_bounds__(kPsBlock) void k_ps_bc(PsArgs a) {
                     }
                 }
                 __syncthreads();
-                const int nbig = min(s_nbig[c], kPsBig);
-                for (int t = wid; t < nbig; t += kPsWaves) {
-                    const int v = s_big[t];
+                auto long_row = [&](int v) {
                     const double acc = wave_sum(ps_pull_long(a, R, a.rp[v], a.rp[v + 1], lane, dd + 1));
                     if (lane == 0) {
                         const double sv = ld_sig(&R[v].sig);
@@ -421,6 +419,15 @@ __global__ __launch_bounds__(kPsBlock) void k_ps_bc(PsArgs a) {
                         if (__builtin_isinf(sv)) atomicOr(a.overflow, 1);
                         R[v].sig = (1.0 + dl) / sv;
                         bcp[v] += COMPAT ? dl * (double)ld_npred(&R[v].npred) : dl;
+                    }
+                };
+                const int nbig = s_nbig[c];
+                if (nbig <= kPsBig) {
+                    for (int t = wid; t < nbig; t += kPsWaves) long_row(s_big[t]);
+                } else {  // the list overflowed: the wavefronts take the level's long rows from Q
+                    for (int i = b0 + wid; i < b1; i += kPsWaves) {
+                        const int v = Q[i];
+                        if (a.rp[v + 1] - a.rp[v] > kPsBigDeg) long_row(v);
                     }
                 }
                 __syncthreads();
