@@ -10,7 +10,7 @@
 //                         each entry in place
 // so a neighbour's frontier test is one 16-byte load for all 64 sources and a
 // path-count gather reads one contiguous slice of a 512-byte row.  The
-// closeness step (masks only, up to 8 batches interleaved) is in
+// closeness step (masks only, up to 64 batches interleaved) is in
 // ccw_kernels.h; the frontier-queue step reuses the gathers and the vertex
 // epilogue below with plain masks and lane-indexed rows.
 //
