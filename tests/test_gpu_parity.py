@@ -250,6 +250,26 @@ def test_source_mode_closeness_all_sources(ctx):
     assert np.array_equal(f2, far) and np.array_equal(d2, ds)
 
 
+def test_closeness_wide_groups(ctx):
+    """The interleaved closeness step at its widest launch groups: all
+    sources of an R-MAT 2^14 graph (groups of 32 batches) and 16437 sources
+    with repeats (groups of 64 batches and a ragged last one), farness and
+    distance sums bit-exact against the source-parallel step, and a subset
+    against the oracle."""
+    g = bc_amd.gen_rmat(14, 16 << 14, seed=12)
+    dg = ctx.upload(g)
+    rng = np.random.default_rng(3)
+    for src in (None, rng.integers(0, g.n, 16437).astype(np.int32)):
+        k = g.n if src is None else len(src)
+        print(f"[gpu] closeness wide groups n={g.n} k={k}", flush=True)
+        far, ds = ctx.cc(dg, src, dist_sum=True, frontier="dense")
+        far2, ds2 = ctx.cc(dg, src, dist_sum=True, frontier="source")
+        assert np.array_equal(far, far2) and np.array_equal(ds, ds2)
+        sub = np.arange(g.n, dtype=np.int32)[:97] if src is None else src[-97:]
+        want = O.farness(g, sub)
+        assert np.array_equal(far[:97] if src is None else far[-97:], want)
+
+
 def test_source_mode_deterministic(ctx):
     """Same slot count, same bits: BC partials are reduced in slot order, and
     every row longer than the wavefront threshold is summed by a wavefront,
