@@ -735,6 +735,15 @@ class Context:
         return bc[:red.n], far[:red.n]
 
 
+_lib.cent_debug_fail_upload.argtypes = [C.c_int32]
+
+
+def debug_fail_upload(nth: int) -> None:
+    """Test hook: the nth device upload of a graph structure from now on fails
+    with CENT_E_OOM (0 switches the injection off)."""
+    _check(_lib.cent_debug_fail_upload(nth), "cent_debug_fail_upload")
+
+
 # ---------------------------------------------------------------- serving
 CENT_SERVER_DRYRUN = 1
 CENT_Q_BC, CENT_Q_CC = 0, 1

@@ -5,6 +5,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -171,8 +172,14 @@ struct DevCSR {
     std::vector<int> h_chunk_beg, h_hub_v, h_hub_seg_off;
 };
 
+// Failure injection for tests of the error paths (cent_debug_fail_upload):
+// while positive, counts down at every upload() below, and the upload that
+// brings it to zero fails with CENT_E_OOM before allocating anything.
+CENT_INTERNAL extern std::atomic<int> g_fail_upload;
+
 template <class T>
 static inline int upload(cent_ctx* ctx, T** dst, const T* src, size_t n) {
+    if (g_fail_upload.load(std::memory_order_relaxed) > 0 && g_fail_upload.fetch_sub(1) == 1) return CENT_E_OOM;
     HIPTRY(hipMalloc((void**)dst, sizeof(T) * (n > 0 ? n : 1)));
     if (n) HIPTRY(hipMemcpy(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice));
     return CENT_OK;

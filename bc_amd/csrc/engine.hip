@@ -10,6 +10,14 @@
 // order and the result does not depend on timing.
 #include "engine.h"
 
+std::atomic<int> g_fail_upload{0};
+
+extern "C" int cent_debug_fail_upload(int32_t nth) {
+    if (nth < 0) return CENT_E_INVAL;
+    g_fail_upload.store(nth);
+    return CENT_OK;
+}
+
 void devcsr_free(DevCSR& d) {
     if (d.rp) (void)hipFree(d.rp);
     if (d.col) (void)hipFree(d.col);

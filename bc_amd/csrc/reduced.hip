@@ -55,6 +55,8 @@ static void reddev_free(void* p) {
 
 static std::mutex g_reddev_mu;  // first-use creation of a reduced graph's device copy
 
+static int reddev_build(cent_ctx* ctx, const cent_reduced* r, RedDev* rd);
+
 static int reddev_get(cent_ctx* ctx, cent_reduced* r, RedDev** out) {
     std::lock_guard<std::mutex> lk(g_reddev_mu);
     if (r->dev) {
@@ -66,8 +68,20 @@ static int reddev_get(cent_ctx* ctx, cent_reduced* r, RedDev** out) {
     RedDev* rd = new (std::nothrow) RedDev();
     if (!rd) return CENT_E_OOM;
     rd->ctx = ctx;
+    // built privately and published on r only once every upload succeeded:
+    // a half-built copy (no items uploaded) must never be found by a later call
+    const int st = reddev_build(ctx, r, rd);
+    if (st != CENT_OK) {
+        reddev_free(rd);
+        return st;
+    }
     r->dev = rd;
     r->dev_free = reddev_free;
+    *out = rd;
+    return CENT_OK;
+}
+
+static int reddev_build(cent_ctx* ctx, const cent_reduced* r, RedDev* rd) {
     const size_t NP = (size_t)(r->np > 0 ? r->np : 1);
     TRY(devcsr_build(ctx, r->prp, r->pcol, r->np, r->nnzp, r->boff, r->nblocks, rd->d));
     TRY(upload(ctx, &rd->w, r->pw, (size_t)r->np));
@@ -119,7 +133,6 @@ static int reddev_get(cent_ctx* ctx, cent_reduced* r, RedDev** out) {
         }
         rd->probe_ecc[lo] = depth;
     }
-    *out = rd;
     return CENT_OK;
 }
 
@@ -163,10 +176,17 @@ static int reddev_share(cent_ctx* ctx, const cent_reduced* r, RedDev* rd) {
             cnt.push_back(k);
         }
     }
+    int st = upload(ctx, &rd->units, units.data(), units.size());
+    if (!st) st = upload(ctx, &rd->ucnt, cnt.data(), cnt.size());
+    if (!st) st = upload(ctx, &rd->unbr, nbr.data(), nbr.size());
+    if (st) {  // leave no half-built plan behind: the next call rebuilds it
+        for (int** p : {(int**)&rd->units, &rd->ucnt, &rd->unbr}) {
+            if (*p) (void)hipFree(*p);
+            *p = nullptr;
+        }
+        return st;
+    }
     rd->nunits = (int)units.size();
-    TRY(upload(ctx, &rd->units, units.data(), units.size()));
-    TRY(upload(ctx, &rd->ucnt, cnt.data(), cnt.size()));
-    TRY(upload(ctx, &rd->unbr, nbr.data(), nbr.size()));
     rd->share_built = true;
     return CENT_OK;
 }

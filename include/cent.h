@@ -470,6 +470,14 @@ int  cent_server_result(cent_server* s, int64_t rid, double* bc, int64_t* farnes
 int  cent_server_cancel(cent_server* s, int64_t rid);
 int  cent_server_get_stats(cent_server* s, cent_server_stats* out);
 
+/* ------------------------------------------------------------ test hooks */
+/* Failure injection for tests of the engine's error paths (no reference
+ * counterpart): the nth host-to-device upload of a graph structure from now
+ * on (device CSR, reduced-graph arrays, schedule tables) fails with
+ * CENT_E_OOM before it allocates; 0 switches injection off.  Process-wide;
+ * not for production use. */
+int  cent_debug_fail_upload(int32_t nth);
+
 /* ------------------------------------------- reference-shaped entry points */
 /* Drop-in shapes of the reference functions over the own struct CSR, running
  * on a process-wide default context (device 0).  Ids startNodeID..endNodeID

@@ -58,3 +58,54 @@ def test_results_same_alone_and_in_mix(ctx):
     print(f"[gpu] server stats {st}", flush=True)
     assert st["completed"] == 24 and st["max_running"] >= 2
     srv.close()
+
+
+@pytest.mark.timeout(600)
+def test_mix_with_pool_sized_graphs(ctx):
+    """Requests on graphs of serving-pool size (>= 1e5 vertices: an R-MAT with
+    16 draws per vertex and a 4000-block tree) mixed with small ones.  Every
+    result equals the same call alone; BC and source farness equal the
+    oracle's; all-vertex farness equals the plain all-source engine call and,
+    on a sample of 1024 vertices, the oracle's BFS."""
+    graphs = [bc_amd.gen_rmat(18, 16 << 18, seed=31),
+              bc_amd.gen_blocktree(4000, 1.5, 8, 50000, 6.0, 0.3, 0.5, seed=32),
+              bc_amd.gen_rmat(12, 16 << 12, seed=33)]
+    assert graphs[0].n >= 100_000 and graphs[1].n >= 100_000, [g.n for g in graphs]
+    print(f"[gpu] server pool-size mix: n = {[g.n for g in graphs]}", flush=True)
+    srv = bc_amd.Server(slots=4, hbm_budget=64 << 30)
+    gids = [srv.add_graph(g) for g in graphs]
+    plan = [(0, "bc", 512), (1, "cc_all", 0), (0, "cc_all", 0), (2, "bc", 300), (1, "bc", 256),
+            (0, "bc", 64), (2, "cc_all", 0), (1, "cc", 700), (0, "cc", 1024), (1, "cc_all", 0)]
+    reqs = []
+    for i, (j, kind, k) in enumerate(plan):
+        g = graphs[j]
+        src = bc_amd.sample_sources(g.n, k, 300 + i) if k else None
+        qk = bc_amd.CENT_Q_BC if kind == "bc" else bc_amd.CENT_Q_CC
+        reqs.append((srv.submit(gids[j], qk, src), j, kind, src))
+    dgs = [ctx.upload(g) for g in graphs]
+    all_far = {}
+    for rid, j, kind, src in reqs:
+        info = srv.wait(rid)
+        assert info["state"] == "done", info
+        st, bc, far, info = srv.result(rid)
+        assert st == 0
+        g = graphs[j]
+        if kind == "bc":
+            abc, afar, _ = ctx.bc(dgs[j], src, max_slots=bc_amd.SERVER_SOURCE_SLOTS, one_stream=True)
+            assert np.array_equal(bc, abc) and np.array_equal(far, afar)
+            obc, ofar, _, _ = O.brandes(g, src, threads=16)
+            np.testing.assert_allclose(bc, obc, rtol=1e-9, atol=1e-9 * max(1.0, np.abs(obc).max()))
+            assert np.array_equal(far, ofar)
+        elif kind == "cc":
+            assert np.array_equal(far, O.weighted_farness(g.n, g.row_ptr, g.col, np.ones(g.n, np.int64), src))
+        else:
+            if j not in all_far:
+                all_far[j] = ctx.cc(dgs[j], None)[0]
+                sample = bc_amd.sample_sources(g.n, 1024, 77)
+                want = O.weighted_farness(g.n, g.row_ptr, g.col, np.ones(g.n, np.int64), sample)
+                assert np.array_equal(all_far[j][sample], want)
+            assert np.array_equal(far, all_far[j])
+        print(f"[gpu] server pool-size mix: {kind} on graph {j} ok", flush=True)
+    st = srv.stats()
+    assert st["completed"] == len(plan) and st["max_running"] >= 2, st
+    srv.close()
