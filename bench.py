@@ -12,12 +12,16 @@ resident in HBM before the timed region and the step's outputs stay there
 copied back over PCIe) is reported beside it as pcie_inclusive.
 
 Prints one JSON line.  value = TEPS = sources x m / seconds per step, with m the
-undirected edges of the (connected) graph.  roofline = the dominant kernel's
-algorithmic bytes / its summed HIP-event duration:
-    forward  per source: 16m + 20n bytes
-    backward per source: 16m + 40n bytes, plus 16 bytes per shortest-path DAG edge
+undirected edges of the (connected) graph.  roofline = the dominant level
+pass's HBM-side bytes per launch (rocprofv3 counters of this workload,
+profiles/rmat20_traffic.json, used only while its key matches the tree) over
+its average launch duration in a one-stream step (HIP events); the per-source
+bytes model (forward 16m + 20n, backward 16m + 40n + 16 per shortest-path
+DAG edge) is reported beside it as an effective rate only.
 cpu_baseline = the CPU oracle (a C restatement of the same Brandes pass)
-timed on a bounded source sample on this host's cores.
+timed on a bounded source sample on this host's cores, 16 threads and one.
+serving = the request-server workload (Poisson stream, p50/p99, served vs
+plain).  Any result that differs from its check makes the exit status 1.
 """
 import argparse
 import json
@@ -288,9 +292,11 @@ def extra_rmat14(ctx, args):
 
 
 def serve_pool(seed: int, size: int):
-    """Seeded graph pool: vertex counts log-uniform in [1e3, 1e6]; even entries
-    R-MAT (avg degree 16, LCC), odd entries block trees (about 31 vertices per
-    block with the 30% pendant chains)."""
+    """Seeded graph pool of the serving workload: vertex counts log-uniform in
+    [1e3, 1e6]; even entries R-MAT with 16 edge draws per vertex (average
+    degree 16, the edge factor of the R-MAT workloads; largest component
+    kept), odd entries block trees as in the block-tree workload (about 31
+    vertices per block with the 30% pendant chains)."""
     import numpy as np
     import bc_amd
     rng = np.random.default_rng(seed)
@@ -299,7 +305,7 @@ def serve_pool(seed: int, size: int):
         n = float(np.exp(rng.uniform(np.log(1e3), np.log(1e6))))
         if i % 2 == 0:
             scale = max(10, int(round(np.log2(n))))
-            g = bc_amd.gen_rmat(scale, 8 << scale, seed=seed * 1000 + i)
+            g = bc_amd.gen_rmat(scale, 16 << scale, seed=seed * 1000 + i)
             pool.append(("rmat", g))
         else:
             nb = max(8, int(n / 31))
@@ -308,49 +314,15 @@ def serve_pool(seed: int, size: int):
     return pool
 
 
-def run_stream(srv, gids, pool, reqs, rate, deadline_s):
-    """Open-loop Poisson submission; returns per-request (kind, latency_ms) and
-    the wall time from the first submission to the last completion."""
+def serve_requests(pool, count: int, seed: int = 5):
+    """count requests (graph index, kind, sources): each picks a pool graph
+    uniformly; half sampled BC with k log-uniform in [64, 1024], half
+    all-vertex closeness."""
     import numpy as np
     import bc_amd
-    rng = np.random.default_rng(55)
-    gaps = rng.exponential(1.0 / rate, len(reqs))
-    t0 = time.perf_counter()
-    t_next = t0
-    rids = []
-    for (j, kind, src), gap in zip(reqs, gaps):
-        t_next += gap
-        d = t_next - time.perf_counter()
-        if d > 0:
-            time.sleep(d)
-        rids.append((srv.submit(gids[j], kind, src), kind))
-    lat, done_t = [], []
-    end = t0 + deadline_s
-    for rid, kind in rids:
-        left = int(max(0.0, end - time.perf_counter()) * 1000)
-        info = srv.wait(rid, left)
-        if info["state"] not in ("done", "failed", "cancelled"):
-            srv.cancel(rid)
-            info = srv.wait(rid)
-        st, _, _, info = srv.result(rid)
-        if info["state"] == "done":
-            lat.append((kind, info["t_end_ms"] - info["t_submit_ms"]))
-            done_t.append(info["t_end_ms"])
-    return lat, time.perf_counter() - t0
-
-
-def extra_serve(ctx, args):
-    """Serving workload: Poisson arrivals at 200 requests/s; each request picks a
-    graph from the pool; half sampled BC (k log-uniform in [64, 1024]), half
-    all-vertex closeness (through the D1/AP-reduced path); seed 5.  Reports
-    p50/p99 latency and completed queries/s for the continuous-batching server
-    (4 slots) and for a single-slot server (one request at a time)."""
-    import numpy as np
-    import bc_amd
-    pool = serve_pool(5, args.serve_pool)
-    rng = np.random.default_rng(5)
+    rng = np.random.default_rng(seed)
     reqs = []
-    for i in range(args.serve_requests):
+    for i in range(count):
         j = int(rng.integers(0, len(pool)))
         g = pool[j][1]
         if rng.random() < 0.5:
@@ -358,68 +330,72 @@ def extra_serve(ctx, args):
             reqs.append((j, bc_amd.CENT_Q_BC, bc_amd.sample_sources(g.n, k, 1000 + i)))
         else:
             reqs.append((j, bc_amd.CENT_Q_CC, None))
-    log(f"serve: pool {[(kind, g.n) for kind, g in pool]}, {len(reqs)} requests")
-    res = {"workload": f"Poisson {args.serve_rate:g} req/s, {len(reqs)} requests over a pool of "
-                       f"{len(pool)} graphs (n log-uniform in [1e3, 1e6], half R-MAT deg 16, half "
-                       "block tree), 50% sampled BC k in [64,1024], 50% all-vertex closeness",
-           "pool_n": [g.n for _, g in pool]}
-    for label, slots in (("served", 4), ("one_at_a_time", 1)):
-        srv = bc_amd.Server(slots=slots, hbm_budget=200 << 30)
-        gids = [srv.add_graph(g) for _, g in pool]
-        # warm the graph cache and reductions
-        for gid in gids:
-            srv.wait(srv.submit(gid, bc_amd.CENT_Q_BC, [0]))
-        lat, wall = run_stream(srv, gids, pool, reqs, args.serve_rate, args.serve_deadline)
-        st = srv.stats()
-        srv.close()
-        ms = np.array([x for _, x in lat]) if lat else np.zeros(1)
-        out = {"completed": len(lat), "of": len(reqs), "wall_s": round(wall, 3),
-               "queries_per_s": round(len(lat) / wall, 2),
-               "p50_ms": round(float(np.percentile(ms, 50)), 2),
-               "p99_ms": round(float(np.percentile(ms, 99)), 2),
-               "max_running": st["max_running"], "max_queued": st["max_queued"]}
-        for kind, name in ((bc_amd.CENT_Q_BC, "bc"), (bc_amd.CENT_Q_CC, "cc")):
-            x = np.array([v for kd, v in lat if kd == kind])
-            if len(x):
-                out[f"{name}_p50_ms"] = round(float(np.percentile(x, 50)), 2)
-                out[f"{name}_p99_ms"] = round(float(np.percentile(x, 99)), 2)
-        log(f"serve[{label}]: {out}")
-        res[label] = out
-    res["speedup_queries_per_s"] = round(res["served"]["queries_per_s"] /
-                                         max(1e-9, res["one_at_a_time"]["queries_per_s"]), 2)
-    if args.serve_plain > 0:
-        res["vs_plain"] = serve_vs_plain(ctx, pool, reqs[:args.serve_plain])
-    return res
+    return reqs
 
 
-def serve_vs_plain(ctx, pool, reqs):
-    """Saturation throughput on the same requests: the served form (4-slot
-    server, device-resident graph and reduction cache, persistent item kernel,
-    all requests submitted at once) against the plain per-component-launch
-    form (one request at a time; each uploads its graph, BC through cent_bc,
-    all-vertex closeness through a fresh host D1/AP reduction and one launch
-    per (component, source-block) item).  Results must agree: farness exact,
-    BC within 1e-9 relative."""
+def run_stream(srv, gids, reqs, rate, drain_s):
+    """Open-loop Poisson submission at `rate` requests/s.  A collector thread
+    takes the results in submission order as they finish (so finished
+    results do not pile up in the server).  Requests still unfinished
+    drain_s seconds after the last submission are cancelled.  Returns the
+    per-request (kind, latency ms), the wall time from the first submission
+    to the last completion, and the number cancelled."""
+    import queue
+    import threading
     import numpy as np
-    import bc_amd
-    out_plain = []
+    rng = np.random.default_rng(55)
+    gaps = rng.exponential(1.0 / rate, len(reqs))
+    pending = queue.Queue()
+    lat, last_end = [], [0.0]
+    cancelled = [0]
+    deadline = [None]
+
+    def collect():
+        while True:
+            item = pending.get()
+            if item is None:
+                return
+            rid, kind = item
+            while True:
+                dl = deadline[0]
+                left = 1000 if dl is None else int(max(0.0, dl - time.perf_counter()) * 1000)
+                info = srv.wait(rid, min(left, 1000))
+                if info["state"] in ("done", "failed", "cancelled"):
+                    break
+                if dl is not None and time.perf_counter() >= dl:
+                    srv.cancel(rid)
+                    cancelled[0] += 1
+                    info = srv.wait(rid)
+                    break
+            st, _, _, info = srv.result(rid)
+            if info["state"] == "done":
+                lat.append((kind, info["t_end_ms"] - info["t_submit_ms"]))
+                last_end[0] = max(last_end[0], time.perf_counter())
+
+    th = threading.Thread(target=collect, daemon=True)
+    th.start()
     t0 = time.perf_counter()
-    for j, kind, src in reqs:
-        g = pool[j][1]
-        dg = ctx.upload(g)
-        if kind == bc_amd.CENT_Q_BC:
-            bc, far, _ = ctx.bc(dg, src)
-            out_plain.append((bc, far))
-        else:
-            red = bc_amd.Reduced(g)
-            out_plain.append((None, ctx.cc_reduced(red, sched=bc_amd.CENT_SCHED_PER_ITEM)))
-        del dg
-    t_plain = time.perf_counter() - t0
-    log(f"serve[plain]: {len(reqs)} requests in {t_plain:.2f} s")
-    srv = bc_amd.Server(slots=4, hbm_budget=200 << 30)
-    gids = [srv.add_graph(g) for _, g in pool]
-    for gid in gids:
-        srv.wait(srv.submit(gid, bc_amd.CENT_Q_BC, [0]))
+    t_next = t0
+    for i, ((j, kind, src), gap) in enumerate(zip(reqs, gaps)):
+        t_next += gap
+        d = t_next - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        pending.put((srv.submit(gids[j], kind, src), kind))
+        if (i + 1) % 2000 == 0:
+            log(f"serve stream: {i + 1}/{len(reqs)} submitted, {len(lat)} done "
+                f"({time.perf_counter() - t0:.1f} s)")
+    deadline[0] = time.perf_counter() + drain_s
+    pending.put(None)
+    while th.is_alive():
+        th.join(10.0)
+        if th.is_alive():
+            log(f"serve stream: draining, {len(lat)} done ({time.perf_counter() - t0:.1f} s)")
+    return lat, (last_end[0] or time.perf_counter()) - t0, cancelled[0]
+
+
+def _served_pass(srv, gids, reqs):
+    """All requests submitted at once (saturation); returns (seconds, results)."""
     t0 = time.perf_counter()
     rids = [srv.submit(gids[j], kind, src) for j, kind, src in reqs]
     got = []
@@ -429,24 +405,211 @@ def serve_vs_plain(ctx, pool, reqs):
         if st != 0:
             raise RuntimeError(f"served request failed: status {st}")
         got.append((bc, far))
-    t_srv = time.perf_counter() - t0
+    return time.perf_counter() - t0, got
+
+
+def _plain_pass(ctx, pool, reqs, resident):
+    """The plain form, one request at a time: BC through cent_bc, all-vertex
+    closeness through the D1/AP reduction with one level-synchronous launch
+    sequence per (component, source block) item (CENT_SCHED_PER_ITEM, the
+    launch structure of parallel_CC/para_CC.cu:175-364).  resident=None:
+    every request uploads its graph and reduces it on the host; otherwise
+    resident[j] = (device graph, reduction) made before the timed pass."""
+    import bc_amd
+    out = []
+    t0 = time.perf_counter()
+    for j, kind, src in reqs:
+        g = pool[j][1]
+        if resident is None:
+            dg = ctx.upload(g)
+            red = bc_amd.Reduced(g) if kind == bc_amd.CENT_Q_CC else None
+        else:
+            dg, red = resident[j]
+        if kind == bc_amd.CENT_Q_BC:
+            bc, far, _ = ctx.bc(dg, src)
+            out.append((bc, far))
+        else:
+            out.append((None, ctx.cc_reduced(red, sched=bc_amd.CENT_SCHED_PER_ITEM)))
+        if resident is None:
+            del dg, red
+    return time.perf_counter() - t0, out
+
+
+def serving(ctx, args, failures):
+    """Serving workload (continuous batching): a seeded pool of graphs, a
+    Poisson stream of requests at --serve-rate into a 4-slot server that
+    starts with nothing cached, p50/p99 latency and completed queries/s;
+    then, on the stream's first --serve-plain requests at saturation, the
+    served form against the plain form, each cold (graphs uploaded and
+    reduced on demand) and resident (graphs already on the device), and 4
+    slots against 1.  Every served result is checked against the plain
+    form's: farness equal, BC within 1e-9 relative."""
+    import numpy as np
+    import bc_amd
+    t0 = time.time()
+    pool = serve_pool(5, args.serve_pool)
+    reqs = serve_requests(pool, args.serve_requests)
+    log(f"serve: pool n={[g.n for _, g in pool]} built in {time.time() - t0:.1f}s, {len(reqs)} requests")
+    budget = 200 << 30
+    res = {"workload": f"Poisson {args.serve_rate:g} req/s, {len(reqs)} requests over a pool of {len(pool)} "
+                       "graphs (n log-uniform in [1e3, 1e6], half R-MAT avg degree 16, half block tree), "
+                       "50% sampled BC k log-uniform in [64, 1024], 50% all-vertex closeness; seed 5",
+           "pool_n": [g.n for _, g in pool], "pool_m": [g.m for _, g in pool]}
+    srv = bc_amd.Server(slots=4, hbm_budget=budget)
+    gids = [srv.add_graph(g) for _, g in pool]
+    lat, wall, ncancel = run_stream(srv, gids, reqs, args.serve_rate, args.serve_drain)
+    st = srv.stats()
     srv.close()
+    ms = np.array([x for _, x in lat]) if lat else np.zeros(1)
+    stream = {"slots": 4, "offered_per_s": args.serve_rate, "completed": len(lat), "of": len(reqs),
+              "cancelled": ncancel, "wall_s": round(wall, 3), "queries_per_s": round(len(lat) / wall, 2),
+              "p50_ms": round(float(np.percentile(ms, 50)), 2), "p99_ms": round(float(np.percentile(ms, 99)), 2),
+              "max_running": st["max_running"], "max_queued": st["max_queued"],
+              "graph_uploads": st["graph_uploads"]}
+    for kind, name in ((bc_amd.CENT_Q_BC, "bc"), (bc_amd.CENT_Q_CC, "cc")):
+        x = np.array([v for kd, v in lat if kd == kind])
+        if len(x):
+            stream[f"{name}_p50_ms"] = round(float(np.percentile(x, 50)), 2)
+            stream[f"{name}_p99_ms"] = round(float(np.percentile(x, 99)), 2)
+    log(f"serve stream: {stream}")
+    res["stream"] = stream
+
+    sub = reqs[:args.serve_plain]
+    if not sub:
+        return res
+    t_pc, ref = _plain_pass(ctx, pool, sub, None)
+    log(f"serve plain cold: {len(sub)} requests {t_pc:.2f} s")
+    resident = {}
+    for j in sorted({j for j, _, _ in sub}):
+        g = pool[j][1]
+        resident[j] = (ctx.upload(g), bc_amd.Reduced(g))
+    _plain_pass(ctx, pool, sub[:2], resident)  # device copies of the reductions
+    t_pr, got_pr = _plain_pass(ctx, pool, sub, resident)
+    log(f"serve plain resident: {t_pr:.2f} s")
+    del resident
+    srv = bc_amd.Server(slots=4, hbm_budget=budget)
+    gids = [srv.add_graph(g) for _, g in pool]
+    t_sc, got_sc = _served_pass(srv, gids, sub)
+    t_sw, got_sw = _served_pass(srv, gids, sub)
+    srv.close()
+    srv = bc_amd.Server(slots=1, hbm_budget=budget)
+    gids = [srv.add_graph(g) for _, g in pool]
+    _served_pass(srv, gids, sub)
+    t_s1, got_s1 = _served_pass(srv, gids, sub)
+    srv.close()
+    log(f"serve served: cold {t_sc:.2f} s, warm {t_sw:.2f} s, one slot warm {t_s1:.2f} s")
     far_equal, bc_rel = True, 0.0
-    for (pb, pf), (sb, sf) in zip(out_plain, got):
-        far_equal &= bool(np.array_equal(pf, sf))
-        if pb is not None:
-            bc_rel = max(bc_rel, float(np.max(np.abs(pb - sb)) / max(1.0, float(np.abs(pb).max()))))
-    res = {"requests": len(reqs), "plain_s": round(t_plain, 3), "served_s": round(t_srv, 3),
-           "plain_queries_per_s": round(len(reqs) / t_plain, 2),
-           "served_queries_per_s": round(len(reqs) / t_srv, 2),
-           "speedup": round(t_plain / t_srv, 2), "farness_equal": far_equal,
-           "bc_max_rel_diff": bc_rel}
-    log(f"serve[vs plain]: {res}")
+    for got in (got_pr, got_sc, got_sw, got_s1):
+        for (pb, pf), (sb, sf) in zip(ref, got):
+            far_equal &= bool(np.array_equal(pf, sf))
+            if pb is not None:
+                bc_rel = max(bc_rel, float(np.max(np.abs(pb - sb)) / max(1.0, float(np.abs(pb).max()))))
+    if not far_equal:
+        failures.append("serving: a served farness differs from the plain form's")
+    if not bc_rel <= 1e-9:
+        failures.append(f"serving: served BC differs from the plain form's (max rel {bc_rel:.2e})")
+    q = len(sub)
+    res["saturation"] = {
+        "requests": q,
+        "plain_cold_qps": round(q / t_pc, 2), "plain_resident_qps": round(q / t_pr, 2),
+        "served_cold_qps": round(q / t_sc, 2), "served_warm_qps": round(q / t_sw, 2),
+        "one_slot_warm_qps": round(q / t_s1, 2),
+        "served_cold_vs_plain_cold": round(t_pc / t_sc, 2),
+        "served_warm_vs_plain_resident": round(t_pr / t_sw, 2),
+        "four_slots_vs_one": round(t_s1 / t_sw, 2),
+        "farness_equal": far_equal, "bc_max_rel_diff": bc_rel}
+    log(f"serve saturation: {res['saturation']}")
     return res
 
 
 EXTRA = {"rmat14": extra_rmat14, "blocktree": extra_blocktree, "blocktree-bc": extra_blocktree_bc, "grid": extra_grid,
-         "closeness": extra_closeness, "serve": extra_serve, "reduce": extra_reduce}
+         "closeness": extra_closeness, "reduce": extra_reduce}
+
+# Sources whose text decides the dense step's HBM traffic per launch: the
+# counter figures in profiles/rmat20_traffic.json are used only while their
+# key (these files' hash, the launch layout, the workload) matches the tree.
+TRAFFIC_SOURCES = ("bc_amd/csrc/msbfs_kernels.h", "bc_amd/csrc/wave.h", "bc_amd/csrc/records.h",
+                   "bc_amd/csrc/levels.hip", "bc_amd/csrc/engine.h", "bc_amd/Makefile")
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "rmat20_traffic.json")
+
+
+def kernel_sha16() -> str:
+    import hashlib
+    h = hashlib.sha256()
+    for f in TRAFFIC_SOURCES:
+        h.update(f.encode())
+        h.update(open(os.path.join(ROOT, f), "rb").read())
+    return h.hexdigest()[:16]
+
+
+def traffic_key(args, layout) -> dict:
+    return {"scale": args.scale, "edge_factor": args.edge_factor, "sources": args.sources,
+            "launch": {k: layout[k] for k in ("hub_deg", "chunk_edges", "grid_waves", "frontier")},
+            "kernel_sha16": kernel_sha16()}
+
+
+def _read_counters(d: str):
+    """Per-kernel (dispatches, summed counter value) from a rocprofv3 --pmc
+    output directory (its *counter_collection.csv)."""
+    import csv
+    import glob
+    files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise SystemExit(f"no counter_collection.csv under {d}")
+    out = {}
+    for f in files:
+        for row in csv.DictReader(open(f)):
+            name = row["Kernel_Name"]
+            disp, tot = out.get(name, (set(), 0.0))
+            disp.add(row.get("Dispatch_Id", len(disp)))
+            out[name] = (disp, tot + float(row["Counter_Value"]))
+    return {k: (len(v[0]), v[1]) for k, v in out.items()}
+
+
+def write_traffic(args) -> int:
+    """--write-traffic FETCH_DIR WRITE_DIR: turns two rocprofv3 --pmc runs of
+    this bench (one for FETCH_SIZE, one for WRITE_SIZE, each over
+    `--steps 1 --warmup 0 --no-cpu-baseline --no-serve --timed-only
+    --copy-probe`) into profiles/rmat20_traffic.json.  Readings are in KiB
+    and are scaled so that the copy probe's reads equal its known bytes."""
+    import bc_amd
+    fdir, wdir = args.write_traffic
+    fetch, write = _read_counters(fdir), _read_counters(wdir)
+    probe = [k for k in fetch if "k_copy_probe" in k]
+    if not probe:
+        raise SystemExit("the counter runs need --copy-probe (calibration kernel)")
+    nprobe, fprobe = fetch[probe[0]]
+    _, wprobe = write[probe[0]]
+    known_kib = (1 << 30) / 1024.0  # 1 GiB read and 1 GiB written per probe launch
+    fs = known_kib / (fprobe / nprobe)
+    ws = known_kib / (wprobe / nprobe)
+    passes = {"fwd_pass": ("k_fwd", "k_push1"), "bwd_pass": ("k_bwd",)}
+    if not args.no_tune_table and os.path.exists(bc_amd.TUNE_TABLE):
+        bc_amd.tune_table_load()
+    g = bc_amd.gen_rmat(args.scale, args.edge_factor << args.scale, 0.57, 0.19, 0.19, seed=2)
+    ctx = bc_amd.Context(0)
+    layout = ctx.upload(g).layout.as_dict()
+    out = {"workload": f"sampled BC + closeness, R-MAT 2^{args.scale} avg degree {args.edge_factor} (LCC), "
+                       f"{args.sources} sources",
+           "key": traffic_key(args, layout), "fetch_scale": round(fs, 4), "write_scale": round(ws, 4),
+           "source": "profiles/rmat20_sampled_bc_hbm_counters.md"}
+    for pname, prefixes in passes.items():
+        kern = {}
+        for k, (nd, v) in fetch.items():
+            short = k.split("(")[0].replace("void ", "").replace("cent::", "")
+            if any(short.startswith(p) for p in prefixes):
+                kern[short] = {"dispatches": nd, "fetch_bytes": int(v * fs * 1024),
+                               "write_bytes": int(write.get(k, (0, 0.0))[1] * ws * 1024)}
+        # one level launch of the pass = one dispatch of its main kernel
+        main = "k_bwd<false>" if pname == "bwd_pass" else "k_fwd<false>"
+        nl = kern[main]["dispatches"] + (kern.get("k_push1_or", {}).get("dispatches", 0) if pname == "fwd_pass" else 0)
+        fb = sum(x["fetch_bytes"] for x in kern.values())
+        wb = sum(x["write_bytes"] for x in kern.values())
+        out[pname] = {"launches_per_step": nl, "fetch_bytes_per_step": fb, "write_bytes_per_step": wb,
+                      "traffic_bytes_per_launch": int((fb + wb) / nl), "kernels": kern}
+    json.dump(out, open(TRAFFIC_JSON, "w"), indent=1)
+    print(json.dumps(out, indent=1))
+    return 0
 
 
 def main() -> int:
@@ -458,7 +621,9 @@ def main() -> int:
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--sources", type=int, default=4096)
     ap.add_argument("--cpu-sample", type=int, default=-1,
-                    help="sources for the CPU baseline (default: 8 per core)")
+                    help="sources of the all-thread CPU baseline (default: 8 per thread)")
+    ap.add_argument("--cpu-sample-1t", type=int, default=16,
+                    help="sources of the single-thread CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tune", action="store_true",
                     help="autotune the launch settings on the first 256 sources before timing")
@@ -478,16 +643,19 @@ def main() -> int:
                     help="comma list of extra workloads reported under 'workloads': "
                          "blocktree (reduced closeness, persistent vs per-item launches), "
                          "blocktree-bc (plain BC+CC on the block tree), closeness (closeness-only "
-                         "timings), grid (sampled BC+CC on "
-                         "the 2048^2 grid), serve (request server), reduce (graph reduction, "
-                         "host vs GPU)")
-    ap.add_argument("--serve-requests", type=int, default=1000)
+                         "timings), grid (sampled BC+CC on the 2048^2 grid), reduce (graph "
+                         "reduction, host vs GPU), rmat14 (exact all-source BC+CC beside the "
+                         "reference's own code)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
+    ap.add_argument("--serve-requests", type=int, default=20000)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-pool", type=int, default=12)
     ap.add_argument("--serve-plain", type=int, default=60,
-                    help="requests of the stream timed in the plain form against the server")
-    ap.add_argument("--serve-deadline", type=float, default=90.0,
-                    help="seconds after the first submission; unfinished requests are cancelled")
+                    help="requests of the stream timed at saturation in the plain and served forms")
+    ap.add_argument("--serve-drain", type=float, default=240.0,
+                    help="seconds after the last submission; unfinished requests are cancelled")
+    ap.add_argument("--write-traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="write profiles/rmat20_traffic.json from two rocprofv3 --pmc runs and exit")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -495,15 +663,18 @@ def main() -> int:
               "build (sources shard across GPUs with one all-reduce of the BC vector — "
               "design only).", file=sys.stderr)
         return 2
+    if args.write_traffic:
+        return write_traffic(args)
 
     import numpy as np
     import bc_amd
 
+    failures = []
     t0 = time.time()
     g = bc_amd.gen_rmat(args.scale, args.edge_factor << args.scale, 0.57, 0.19, 0.19, seed=2)
     src = bc_amd.sample_sources(g.n, args.sources, 2)
-    print(f"# graph: R-MAT scale {args.scale} ef {args.edge_factor}: n={g.n} m={g.m} "
-          f"(generated in {time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    log(f"graph: R-MAT scale {args.scale} ef {args.edge_factor}: n={g.n} m={g.m} "
+        f"(generated in {time.time() - t0:.1f}s)")
 
     # launch settings: the bundled MI355X tuning table (applied at upload by
     # graph signature), or a fresh tuning run on the first 256 sources
@@ -514,7 +685,7 @@ def main() -> int:
     if args.tune:
         t1 = time.perf_counter()
         best = ctx.tune(dg, src[:256], bc_amd.CENT_TUNE_BC)
-        print(f"# tuned in {time.perf_counter() - t1:.1f}s: {best.as_dict()}", file=sys.stderr, flush=True)
+        log(f"tuned in {time.perf_counter() - t1:.1f}s: {best.as_dict()}")
         if args.tune_save:
             bc_amd.tune_table_save(args.tune_save)
     layout = dg.layout.as_dict()
@@ -525,7 +696,7 @@ def main() -> int:
     d_ds = ctx.device_array(g.n, np.int64)
     for i in range(args.warmup):
         ctx.bc_device(dg, src, d_bc.zero(), d_far, d_ds.zero(), one_stream=args.one_stream)
-        print(f"# warmup {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+        log(f"warmup {i + 1}/{args.warmup} done")
     ctx.reset_stats()
     times = []
     for i in range(args.steps):
@@ -534,22 +705,22 @@ def main() -> int:
         t1 = time.perf_counter()
         ctx.bc_device(dg, src, d_bc, d_far, d_ds, timing=True, one_stream=args.one_stream)
         times.append(time.perf_counter() - t1)
-        print(f"# step {i + 1}/{args.steps}: {times[-1] * 1e3:.1f} ms", file=sys.stderr, flush=True)
+        log(f"step {i + 1}/{args.steps}: {times[-1] * 1e3:.1f} ms")
     st = ctx.stats()
     ws_bytes = ctx.workspace_bytes()
-    far = d_far.numpy()
+    far, bc_dev, ds_dev = d_far.numpy(), d_bc.numpy(), d_ds.numpy()
     t_host = t_one = st1 = None
     if not args.timed_only:
         # the same step through the host-array call (scores copied back over PCIe)
         bc = np.zeros(g.n)
         t1 = time.perf_counter()
-        _, far_h, _ = ctx.bc(dg, src, bc=bc, dist_sum=True, one_stream=args.one_stream)
+        _, far_h, ds_h = ctx.bc(dg, src, bc=bc, dist_sum=True, one_stream=args.one_stream)
         t_host = time.perf_counter() - t1
-        if not (np.array_equal(far_h, far) and np.array_equal(bc, d_bc.numpy())):
-            print("# WARNING: host-output and device-output results differ", file=sys.stderr)
-        print(f"# host-output step: {t_host * 1e3:.1f} ms", file=sys.stderr, flush=True)
+        if not (np.array_equal(far_h, far) and np.array_equal(ds_h, ds_dev) and np.array_equal(bc, bc_dev)):
+            failures.append("host-output and device-output results differ")
+        log(f"host-output step: {t_host * 1e3:.1f} ms")
         # one more step with every launch on one stream: the level passes' own
-        # rates without the second stream sharing the GPU (not part of value)
+        # per-launch durations without a second stream sharing the GPU
         ctx.reset_stats()
         d_bc.zero()
         d_ds.zero()
@@ -557,91 +728,110 @@ def main() -> int:
         ctx.bc_device(dg, src, d_bc, d_far, d_ds, timing=True, one_stream=True)
         t_one = time.perf_counter() - t1
         st1 = ctx.stats()
-        print(f"# one-stream step: {t_one * 1e3:.1f} ms", file=sys.stderr, flush=True)
+        b1 = d_bc.numpy()
+        if not (np.array_equal(d_far.numpy(), far) and np.array_equal(d_ds.numpy(), ds_dev)):
+            failures.append("one-stream farness / distance sums differ from the two-stream step's")
+        rel1 = float(np.max(np.abs(b1 - bc_dev))) / max(1.0, float(np.abs(bc_dev).max()))
+        if not rel1 <= 1e-12:
+            failures.append(f"one-stream BC differs from the two-stream step's (max rel {rel1:.2e})")
+        log(f"one-stream step: {t_one * 1e3:.1f} ms")
     K = args.steps
     t_step = sum(times) / K
     k, n, m = len(src), g.n, g.m
     teps = k * m / t_step
     dag = st["dag_edges"] / K
+    pairs = st["reached"] / K
     b_fwd = k * (16 * m + 20 * n)
     b_bwd = k * (16 * m + 40 * n) + 16 * dag
-    ms_fwd = st["ms_fwd"] / K
-    ms_bwd = st["ms_bwd"] / K
-    if ms_bwd >= ms_fwd:
-        kname, b, ms, launches = "k_bwd (Brandes backward level)", b_bwd, ms_bwd, st["launches_bwd"] / K
-    else:
-        kname, b, ms, launches = "k_fwd (Brandes forward level)", b_fwd, ms_fwd, st["launches_fwd"] / K
-    achieved = b / (ms * 1e-3) / 1e9
-    roofline = {
-        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kname,
-        "launches_per_step": launches, "kernel_ms_per_step": round(ms, 3),
-        "algorithmic_bytes_per_step": int(b),
-        "fwd_ms": round(ms_fwd, 3), "bwd_ms": round(ms_bwd, 3),
-        "fwd_GBs": round(b_fwd / (ms_fwd * 1e-3) / 1e9, 1) if ms_fwd else None,
-        "bwd_GBs": round(b_bwd / (ms_bwd * 1e-3) / 1e9, 1) if ms_bwd else None,
-        # launch groups alternate over two streams, so a pass's launches
-        # share the GPU with the other stream's: the per-launch durations
-        # above overlap, and the whole step's rate is the aggregate
-        "streams": 1 if args.one_stream else 2,
-        "aggregate_GBs": round((b_fwd + b_bwd) / t_step / 1e9, 1),
-        "aggregate_frac": round((b_fwd + b_bwd) / t_step / 1e9 / HBM_PEAK_GBS, 4),
-    }
-    if st1 is not None:
-        f1, w1 = st1["ms_fwd"], st1["ms_bwd"]
-        b1 = b_bwd if w1 >= f1 else b_fwd
-        t1k = max(f1, w1)
-        roofline["one_stream"] = {
-            "ms_per_step": round(t_one * 1e3, 3), "fwd_ms": round(f1, 3), "bwd_ms": round(w1, 3),
-            "kernel": "k_bwd (Brandes backward level)" if w1 >= f1 else "k_fwd (Brandes forward level)",
-            "achieved": round(b1 / (t1k * 1e-3) / 1e9, 1) if t1k else None,
-            "frac": round(b1 / (t1k * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if t1k else None,
-            "launches_per_step": (st1["launches_bwd"] if w1 >= f1 else st1["launches_fwd"]),
-        }
 
+    # Roofline of the dominant level pass, from the one-stream step (on two
+    # streams every launch shares the GPU with the other stream's, so its
+    # duration is not the kernel's own).  achieved = HBM-side bytes per
+    # launch (rocprofv3 FETCH_SIZE + WRITE_SIZE of the same pass, calibrated
+    # against a copy kernel; profiles/rmat20_traffic.json, used only while
+    # its key matches this tree and workload) over the pass's average
+    # one-stream launch duration (HIP events on the launching stream).
+    roofline = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "streams_timed": 1 if args.one_stream else 2}
+    stats_for = st1 if st1 is not None else (st if args.one_stream else None)
+    if stats_for is not None:
+        fms, bms = stats_for["ms_fwd"], stats_for["ms_bwd"]
+        per = K if stats_for is st else 1
+        fl, bl = stats_for["launches_fwd"] / per, stats_for["launches_bwd"] / per
+        dom = "bwd_pass" if bms >= fms else "fwd_pass"
+        ms_launch = (bms / per / bl) if dom == "bwd_pass" else (fms / per / fl)
+        launches = bl if dom == "bwd_pass" else fl
+        b_model = b_bwd if dom == "bwd_pass" else b_fwd
+        roofline.update({
+            "kernel": ("k_bwd + k_bwd_hubseg + k_bwd_hubfin (Brandes backward level)" if dom == "bwd_pass"
+                       else "k_fwd + k_fwd_hubseg + k_fwd_hubfin (Brandes forward level)"),
+            "launch_ms": round(ms_launch, 4), "launches_per_step": launches,
+            "one_stream_fwd_ms": round(fms / per, 3), "one_stream_bwd_ms": round(bms / per, 3),
+            # the per-source bytes model (every edge charged once per source);
+            # a 64-lane launch reads an edge once for 64 sources, so this
+            # effective rate exceeds any physical bandwidth
+            "model_effective_GBs": round(b_model / launches / (ms_launch * 1e-3) / 1e9, 1)})
+        tr = json.load(open(TRAFFIC_JSON)) if os.path.exists(TRAFFIC_JSON) else {}
+        key = traffic_key(args, layout)
+        if tr.get("key") == key and not args.keep_order:
+            tb = tr[dom]["traffic_bytes_per_launch"]
+            ach = tb / (ms_launch * 1e-3) / 1e9
+            tstep = sum(tr[p]["fetch_bytes_per_step"] + tr[p]["write_bytes_per_step"] for p in ("fwd_pass", "bwd_pass"))
+            roofline.update({"achieved": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": tb,
+                             "basis": "counter bytes per launch / one-stream launch duration",
+                             "traffic_bytes_per_step": int(tstep),
+                             "step_traffic_GBs": round(tstep / t_step / 1e9, 1),
+                             "step_traffic_frac": round(tstep / t_step / 1e9 / HBM_PEAK_GBS, 4),
+                             "traffic_source": tr["source"]})
+        else:
+            # no counters for this tree: a lower bound of the bytes a backward
+            # (forward) level must move — every vertex's frontier record once,
+            # and per reached (vertex, source) pair its path count read and
+            # its coefficient written and gathered once
+            lb = (16 * n + 24 * pairs / launches)
+            ach = lb / (ms_launch * 1e-3) / 1e9
+            roofline.update({"achieved": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                             "basis": "lower-bound bytes model per launch (no counters for this tree)"})
     if args.copy_probe:
         roofline["copy_probe_GBs"] = round(ctx.probe_copy_bandwidth(1 << 30, 10), 1)
-    # HBM-side bytes per launch come from the rocprofv3 counter passes of this
-    # same workload (profiles/rmat20_traffic.json, FETCH_SIZE / WRITE_SIZE
-    # scaled by the copy-kernel calibration); they are not measured live.
-    tpath = os.path.join(ROOT, "profiles", "rmat20_traffic.json")
-    if (args.scale, args.edge_factor, args.sources) == (20, 16, 4096) and os.path.exists(tpath):
-        tr = json.load(open(tpath))
-        key = "bwd_pass" if kname.startswith("k_bwd") else "fwd_pass"
-        roofline["traffic"] = tr[key]["traffic_bytes_per_launch"]
-        roofline["traffic_unit"] = "bytes per launch of the level pass"
-        roofline["algorithmic_bytes_per_launch"] = int(b / launches)
-        # counter-measured bytes of both passes over the live step time: the
-        # fabric-side rate of the whole step (the passes of the two streams
-        # overlap, so a per-pass rate would count shared time twice)
-        tstep = sum(tr[p]["fetch_bytes_per_step"] + tr[p]["write_bytes_per_step"] for p in ("fwd_pass", "bwd_pass"))
-        tgbs = tstep / t_step / 1e9
-        roofline["traffic_bytes_per_step"] = int(tstep)
-        roofline["traffic_GBs"] = round(tgbs, 1)
-        roofline["traffic_frac"] = round(tgbs / HBM_PEAK_GBS, 4)
-        roofline["traffic_source"] = tr["source"]
 
     cpu = None
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import _oracle as O
+        # the box's CPU share per GPU is 16 threads, whatever nproc shows
         threads = max(1, min(16, os.cpu_count() or 1, O.max_threads()))
         ks = args.cpu_sample if args.cpu_sample > 0 else 8 * threads
         sub = src[:ks]
-        print(f"# cpu baseline: oracle Brandes, {ks} sources, {threads} threads", file=sys.stderr,
-              flush=True)
+        log(f"cpu baseline: oracle Brandes, {ks} sources, {threads} threads")
         t1 = time.perf_counter()
         obc, ofar, ods, _ = O.brandes(g, sub, threads=threads)
         tc = time.perf_counter() - t1
-        if not np.array_equal(ofar, far[:ks]):
-            print("# WARNING: CPU and GPU farness differ", file=sys.stderr)
+        k1 = min(args.cpu_sample_1t, ks)
+        t1 = time.perf_counter()
+        O.brandes(g, src[:k1], threads=1)
+        tc1 = time.perf_counter() - t1
+        # parity of the sample: the timed step's farness of these sources, and
+        # a GPU call on the sample itself for BC and distance sums
+        gbc, gfar, gds = ctx.bc(dg, sub, dist_sum=True)
+        if not (np.array_equal(ofar, far[:ks]) and np.array_equal(ofar, gfar)):
+            failures.append("CPU and GPU farness differ")
+        if not np.array_equal(ods, gds):
+            failures.append("CPU and GPU distance sums differ")
+        brel = float(np.max(np.abs(gbc - obc))) / max(1.0, float(np.abs(obc).max()))
+        if not brel <= 1e-9:
+            failures.append(f"CPU and GPU BC differ (max rel {brel:.2e})")
         cpu = {"value": round(ks * m / tc, 1), "unit": "TEPS", "cores": threads, "kind": "port",
-               "sample": f"first {ks} of the {k} sources (same graph), {tc:.2f} s, "
-                         f"CPU {cpu_model()}"}
+               "sample": f"first {ks} of the {k} sources (same graph), {tc:.2f} s on {threads} threads; "
+                         f"CPU {cpu_model()}, nproc {os.cpu_count()}",
+               "single_thread": {"value": round(k1 * m / tc1, 1), "sources": k1, "s": round(tc1, 3)},
+               "parity": {"farness_equal": bool(np.array_equal(ofar, gfar)), "bc_max_rel": brel}}
 
     workloads = {}
     for name in [x for x in args.extra.split(",") if x]:
+        if name == "serve":
+            continue  # the serving workload runs by default (--no-serve skips it)
         workloads[name] = EXTRA[name](ctx, args)
+    serve = None if args.no_serve else serving(ctx, args, failures)
 
     out = {
         "metric": "TEPS (sources×edges/s) BC+CC, batched/served vs plain; p50/p99 query latency",
@@ -659,10 +849,16 @@ def main() -> int:
         "pcie_inclusive": ({"ms_per_step": round(t_host * 1e3, 3), "value": round(k * m / t_host, 1),
                             "unit": "TEPS", "note": "host-array outputs (cent_bc), one step"}
                            if t_host is not None else None),
+        "serving": serve,
+        "parity": {"ok": not failures, "failures": failures},
     }
     if workloads:
         out["workloads"] = workloads
     print(json.dumps(out), flush=True)
+    if failures:
+        for f in failures:
+            print(f"bench.py: PARITY FAILURE: {f}", file=sys.stderr, flush=True)
+        return 1
     return 0
 
 
