@@ -80,6 +80,7 @@ struct cent_ctx {
     int* h_flags = nullptr;  // pinned, kHostFlags ints
     void* h_stage = nullptr; // pinned staging for a call's inputs and results
     size_t h_stage_bytes = 0;
+    DevBuf<int> lexp;        // [levels][64] path-count exponents of a betweenness batch (scaling)
     DevBuf<int> lvf;         // per-level "non-empty" flags of an enqueued run of levels
     DevBuf<unsigned long long> lvstat;  // bounded batches: [0] summed depth, [1] bound failures
     int depth_hint = 0;      // depth of the last dense batch (length of the next run)

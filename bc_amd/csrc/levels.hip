@@ -18,6 +18,24 @@ static int grid_for(int chunks, const cent_opts* o) {
 }
 
 
+// Path-count exponent rows (64 ints a level) for levels 0 .. levels + 1,
+// keeping the first keep_rows rows when the table grows.
+static int ensure_lexp(cent_ctx* ctx, size_t levels, size_t keep_rows) {
+    const size_t want = (levels + 2) * kWave;
+    if (want <= ctx->lexp.cap) return CENT_OK;
+    int* nl = nullptr;
+    HIPTRY(hipMalloc((void**)&nl, sizeof(int) * want));
+    HIPTRY(hipMemsetAsync(nl, 0, sizeof(int) * want, ctx->stream));
+    if (keep_rows && ctx->lexp.p)
+        HIPTRY(hipMemcpyAsync(nl, ctx->lexp.p, sizeof(int) * kWave * std::min(keep_rows, ctx->lexp.cap / kWave),
+                              hipMemcpyDeviceToDevice, ctx->stream));
+    HIPTRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->lexp.p) HIPTRY(hipFree(ctx->lexp.p));
+    ctx->lexp.p = nl;
+    ctx->lexp.cap = want;
+    return CENT_OK;
+}
+
 // Workspace for a DevCSR with N vertices; the F store grows on demand.
 // Batches side by side in the closeness step's interleaved masks: nb
 // rounded up to a power of two.
@@ -37,6 +55,7 @@ static int ensure_workspace(cent_ctx* ctx, const DevCSR& d, const BatchSpec& sp,
     }
     ctx->F_levels = ctx->F.cap / (N * fw) - 1;
     if (sp.with_bc) {
+        TRY(ensure_lexp(ctx, ctx->F_levels, 0));
         TRY(ensure(ctx, ctx->sigma, N * kWave));  // coef is written over it in the backward
         if (sp.compat) TRY(ensure(ctx, ctx->npred, N * kWave));
     }
@@ -64,6 +83,7 @@ static int grow_levels(cent_ctx* ctx, size_t PL, int keep_levels) {
     ctx->F.p = nf;
     ctx->F.cap = PL * (ncap + 1);
     ctx->F_levels = ncap;
+    if (ctx->lexp.p) TRY(ensure_lexp(ctx, ncap, (size_t)keep_levels + 1));
     return CENT_OK;
 }
 
@@ -91,8 +111,8 @@ int level_flags_reset(cent_ctx* ctx) {
 // Zeroes up to five ranges of 64-bit words in one launch (the clears that
 // start a launch group; one launch instead of a fill per buffer).
 struct ClearRanges {
-    uint64_t* p[5];
-    size_t n[5];
+    uint64_t* p[6];
+    size_t n[6];
     int count;
 };
 
@@ -167,6 +187,7 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
         add(plane(1) + fw * v.lo, fw * R);
         add(ctx->part_far.p, (size_t)G * kWave * nb);
         add(ctx->part_cnt.p, (size_t)G * 2 * nb);
+        if (sp.with_bc) add(ctx->lexp.p, ctx->lexp.cap / 2);  // 64 ints a level: 32 words
         size_t tot = 0;
         for (int i = 0; i < cr.count; ++i) tot = std::max(tot, cr.n[i]);
         const int cg = (int)std::min<size_t>((tot + 255) / 256, 4096);
@@ -257,6 +278,7 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
             fa.part_far = ctx->part_far.p; fa.part_cnt = ctx->part_cnt.p;
             fa.flag = ctx->lvf.p + i + 1; fa.overflow = ctx->flags.p + 1;
             fa.live = ctx->lvf.p + i;
+            fa.lexp = ctx->lexp.p;
             hipEvent_t e0;
             TRY(timer_begin(ctx, tm, &e0));
             if (lv == 0) {  // level 1: push from the sources' rows
@@ -313,6 +335,8 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
             ba.bc = sp.bc;
             ba.wgt = sp.bwgt;
             ba.bsrc = d_src;
+            ba.lexp = ctx->lexp.p;
+            ba.d = lv;
             hipEvent_t e0;
             TRY(timer_begin(ctx, tm, &e0));
             if (compat) k_bwd<true><<<G, kBlock, 0, s>>>(ba);
@@ -340,6 +364,7 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
 
 // ------------------------------------------------------------ frontier-queue mode
 constexpr int kSpReplay = 16;  // levels per hipGraph replay
+constexpr size_t kSpLevels = 65536;  // deepest BFS of a betweenness batch on this step
 
 struct SpPlan {
     SpArgs a{};
@@ -370,12 +395,17 @@ static int sp_prepare(cent_ctx* ctx, const DevCSR& d, const BatchSpec& sp, SpPla
     TRY(ensure(ctx, ctx->part_far, (size_t)pl.G * kWave));
     TRY(ensure(ctx, ctx->part_cnt, (size_t)pl.G * 2));
     TRY(ensure(ctx, ctx->cnt, 2));
+    // path-count exponents of up to kSpLevels levels (betweenness); a
+    // deeper BFS makes the call fall back to the dense step
+    const int lexp_levels = (int)std::min<size_t>(N + 3, kSpLevels);
     if (sp.with_bc) {
         TRY(ensure(ctx, ctx->sigma, N * kWave));
         TRY(ensure(ctx, ctx->coef, N * kWave));
         if (sp.compat) TRY(ensure(ctx, ctx->npred, N * kWave));
+        TRY(ensure_lexp(ctx, (size_t)lexp_levels, 0));
     }
     hipStream_t s = ctx->stream;
+    if (sp.with_bc) HIPTRY(hipMemsetAsync(ctx->lexp.p, 0, sizeof(int) * ctx->lexp.cap, s));
     // all-zero vis / masks between batches (batches clear what they touched)
     HIPTRY(hipMemsetAsync(ctx->vis.p, 0, sizeof(uint64_t) * N, s));
     HIPTRY(hipMemsetAsync(ctx->Fd2.p, 0, sizeof(uint64_t) * 2 * N, s));
@@ -389,6 +419,8 @@ static int sp_prepare(cent_ctx* ctx, const DevCSR& d, const BatchSpec& sp, SpPla
     a.weight = sp.weight; a.Q = ctx->Q.p; a.QF = ctx->QF.p; a.qoff = ctx->qoff.p;
     a.qcap = (int)qcap; a.qoff_cap = (int)(N + 3); a.cand = ctx->cand.p; a.stamp = ctx->stamp.p;
     a.ctl = ctx->ctl.p; a.Aptr = ctx->Adev.p; a.part_far = ctx->part_far.p; a.part_cnt = ctx->part_cnt.p;
+    a.lexp = sp.with_bc ? ctx->lexp.p : nullptr;
+    a.lexp_levels = lexp_levels;
     // capture kSpReplay levels of the forward and of the backward once per call
     const int G = pl.G;
     HIPTRY(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
@@ -443,6 +475,7 @@ static int level_batch_sparse(cent_ctx* ctx, SpPlan& pl, const BatchSpec& sp, co
     HIPTRY(hipMemcpyAsync(ctl_h, ctx->ctl.p, sizeof(ctl_h), hipMemcpyDeviceToHost, s));
     HIPTRY(hipStreamSynchronize(s));
     if (ctl_h[kCtlOverflow]) return CENT_E_OVERFLOW;
+    if (ctl_h[kCtlTooDeep]) return CENT_E_UNSUPPORTED;  // the caller reruns on the dense step
     deepest = ctl_h[kCtlDeepest];
     *levels += deepest;
     if (sp.with_bc && deepest >= 1) {
@@ -491,7 +524,7 @@ static int run_persource(cent_ctx* ctx, const cent_graph* g, const BatchSpec& sp
     if (sp.bwgt || k < 1) return CENT_E_UNSUPPORTED;
     const int lcap = (int)std::min<size_t>(n + 2, 65536);
     const size_t nw = ((n + 31) / 32 + 3) / 4 * 4;  // closeness map words, padded for 16-byte clears
-    const size_t per_slot = bc ? n * (sizeof(PsRec) + 4 + 8) + (size_t)lcap * 4 : n * 4 + nw * 4;
+    const size_t per_slot = bc ? n * (sizeof(PsRec) + 4 + 8) + (size_t)lcap * 8 : n * 4 + nw * 4;
     size_t cap = kPsDefaultSlots;
     if (sp.opts && sp.opts->max_slots > 0) cap = (size_t)sp.opts->max_slots;
     else if (sp.opts && sp.opts->grid_waves > 0) cap = std::max(1, sp.opts->grid_waves / kPsWaves);
@@ -507,7 +540,7 @@ static int run_persource(cent_ctx* ctx, const cent_graph* g, const BatchSpec& sp
     TRY(ensure_exact(ctx, ctx->ps_Q, n * G));
     if (bc) {
         TRY(ensure_exact(ctx, ctx->ps_rec, n * G));
-        TRY(ensure_exact(ctx, ctx->ps_loff, (size_t)lcap * G));
+        TRY(ensure_exact(ctx, ctx->ps_loff, (size_t)lcap * 2 * G));  // level offsets + count shifts
         TRY(ensure_exact(ctx, ctx->ps_bcp, n * G));
     } else {
         TRY(ensure_exact(ctx, ctx->ps_bits, nw * G));
@@ -725,7 +758,7 @@ static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
             return st;
         }
     }
-    const bool sparse = batched && lmode == CENT_FRONTIER_SPARSE;
+    bool sparse = batched && lmode == CENT_FRONTIER_SPARSE;
     SpPlan plan;
     if (sparse) TRY(sp_prepare(ctx, g->d, sp, plan));
     // closeness on the dense step takes up to kMultiBatch batches per launch
@@ -783,7 +816,9 @@ static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
     for (int i = 0; i < 4; ++i) lvst[i] = 0;
     for (int attempt = 0; batched && attempt < 2; ++attempt) {
         // attempt 1 only runs if a bound failed, which a correct probe rules
-        // out: everything is redone with host-checked levels on one stream
+        // out, or if a frontier-queue batch was deeper than its exponent
+        // table: everything is redone with host-checked dense levels on one
+        // stream
         const bool two = dual && attempt == 0;
         if (attempt == 1) {
             if (with_bc) HIPTRY(hipMemsetAsync(ctx->bc.p, 0, sizeof(double) * N, s));
@@ -807,7 +842,13 @@ static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
         for (int32_t b0 = 0; b0 < k; b0 += step, ++grp) {
             const int kb = (k - b0) < step ? (k - b0) : step;
             if (sparse) {
-                TRY(level_batch_sparse(ctx, plan, sp, ctx->src.p + b0, kb, ctx->far.p + b0, tm, &levels));
+                const int st = level_batch_sparse(ctx, plan, sp, ctx->src.p + b0, kb, ctx->far.p + b0, tm, &levels);
+                if (st == CENT_E_UNSUPPORTED) {  // too deep for the exponent table
+                    sparse = false;
+                    lvst[1] = 1;  // forces attempt 1 below
+                    break;
+                }
+                TRY(st);
             } else {
                 const int bound = attempt == 0 ? group_bound(b0, kb) : 0;
                 cent_ctx* x = (two && (grp & 1)) ? peer : ctx;
@@ -828,6 +869,10 @@ static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
                                       sp.dist_sum ? peer->dist_sum.p : nullptr, n, ctx->cnt.p, peer->cnt.p,
                                       ctx->lvstat.p, peer->lvstat.p, ctx->flags.p, peer->flags.p);
             HIPTRY(hipGetLastError());
+        }
+        if (!sparse && lmode == CENT_FRONTIER_SPARSE && attempt == 0) {
+            levels = 0;
+            continue;  // the sparse batches gave up: attempt 1 runs them dense
         }
         HIPTRY(hipMemcpyAsync(lvst, ctx->lvstat.p, sizeof(unsigned long long) * 4, hipMemcpyDeviceToHost, s));
         HIPTRY(hipMemcpyAsync(ctx->h_flags + 8, ctx->flags.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));

@@ -27,6 +27,19 @@
 // the lane sum of delta (or of delta * #predecessors in the reference's
 // per-predecessor formula), one deterministic butterfly reduction per level.
 //
+// Path-count scaling: counts grow exponentially with depth on some graphs
+// (a chain of k diamonds has 2^k shortest paths) and would leave the FP64
+// range.  Each level's stored counts carry a per-(level, lane) power-of-two
+// scale: when a new count reaches 2^512 its exponent goes into lexp[d][lane]
+// (atomic max); the next level multiplies its gathered sums by
+// 2^(256 - lexp[d][lane]) and the backward multiplies the level-d
+// dependencies by the same factor, because
+//   sigma~_{d+1} = sum sigma~_d * 2^-k_d   and   delta_d = sigma~_d * 2^-k_d * sum coef~_{d+1}
+// with coef~ = (1 + delta) / sigma~ computed from the stored counts.
+// Multiplying by a power of two is exact, and the factor is 1.0 unless a
+// level reached 2^512, so results are bit-identical to unscaled runs
+// wherever those stay finite.
+//
 // Load balance: a wavefront takes one vertex chunk at a time — at most 64
 // consecutive vertices whose rows hold at most a fixed edge budget — and
 // rows longer than the hub threshold are cut into fixed edge
@@ -79,6 +92,9 @@ struct FwdArgs {
     // Levels are enqueued ahead of the host's check: a launch whose level
     // is empty (*live == 0, written by the previous level) returns at once.
     const int* __restrict__ live;
+    // [levels][64] largest exponent of a stored path count per level and
+    // lane, 0 below 2^512 (null: no scaling, closeness steps)
+    int* __restrict__ lexp;
 };
 
 struct BwdArgs {
@@ -106,7 +122,19 @@ struct BwdArgs {
     // Levels enqueued ahead of the forward's end: a launch for an empty
     // level (*live == 0) returns at once.  Null: always run.
     const int* __restrict__ live;
+    // path-count exponents of the forward (see FwdArgs) and the level d of
+    // the vertices this launch finishes
+    const int* __restrict__ lexp;
+    int d;
 };
+
+// Factor that brings level d's stored counts of this lane to about 2^kSigTo
+// (1.0 when the level stayed below 2^kSigBig or lexp is null).
+__device__ __forceinline__ double level_scale(const int* __restrict__ lexp, int d, int lane) {
+    if (!lexp) return 1.0;
+    const int e = lexp[(size_t)d * kWave + lane];
+    return e >= kSigBig ? __longlong_as_double((long long)(1023 + kSigTo - e) << 52) : 1.0;
+}
 
 // ---------------------------------------------------------------- gathers
 // Level-sorted rows (LS, the dense betweenness step): a vertex's 64 sigma /
@@ -237,7 +265,8 @@ struct FwdAccum {
 
 template <bool SIGMA, bool COMPAT, bool LS>
 __device__ __forceinline__ void fwd_vertex_done(const FwdArgs& a, int v, uint64_t old, double acc,
-                                                uint32_t cnt, uint64_t orf_all, int lane, FwdAccum& r) {
+                                                uint32_t cnt, uint64_t orf_all, int lane, FwdAccum& r,
+                                                double sc = 1.0) {
     uint64_t nm;
     if constexpr (SIGMA) nm = ballot(cnt != 0);
     else nm = orf_all;
@@ -252,8 +281,10 @@ __device__ __forceinline__ void fwd_vertex_done(const FwdArgs& a, int v, uint64_
     if ((nm >> lane) & 1) {
         const size_t idx = (size_t)v * kWave + (LS ? base + lane_rank(nm) : lane);
         if constexpr (SIGMA) {
+            acc *= sc;  // level d's scale (exact: a power of two)
             a.sigma[idx] = acc;
-            if (__builtin_isinf(acc)) r.ovf = true;
+            if (acc >= 0x1p512 && a.lexp) atomicMax(&a.lexp[(size_t)a.dnext * kWave + lane], dexp(acc));
+            if (!(acc <= 0x1.fffffffffffffp1023)) r.ovf = true;  // inf or NaN
         }
         if constexpr (COMPAT) a.npred[idx] = cnt;
         r.far += a.weight ? (int64_t)a.dnext * a.weight[v] : (int64_t)a.dnext;
@@ -300,12 +331,12 @@ __device__ __forceinline__ void fwd_block_flush(const FwdArgs& a, const FwdAccum
 
 template <bool LS>
 __device__ __forceinline__ void bwd_vertex_done(const BwdArgs& a, int v, uint64_t m, int base, double acc,
-                                                int lane, bool compat) {
+                                                int lane, bool compat, double sc = 1.0) {
     double contrib = 0.0;
     if ((m >> lane) & 1) {
         const size_t idx = (size_t)v * kWave + (LS ? base + lane_rank(m) : lane);
         const double sg = a.sigma[idx];
-        const double dl = sg * acc;
+        const double dl = (sg * acc) * sc;  // level d's scale, as in the forward
         if (a.wgt) {
             a.coef[idx] = ((double)a.wgt[v] + dl) / sg;
             contrib = dl * (double)a.wgt[a.bsrc[lane]];
@@ -328,6 +359,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd(FwdArgs a) {
     const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const int nw = gridDim.x * kWavesPerBlock;
     const uint64_t A = a.A;
+    const double sc = level_scale(a.lexp, a.dnext - 1, lane);
     FwdAccum r;
     for (int c = gw; c < a.nchunk; c += nw) {
         const int base = a.chunk_beg[c];
@@ -356,7 +388,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd(FwdArgs a) {
             uint32_t cnt = 0;
             uint64_t orf = 0;
             gather_fwd<true, true>(a.col, a.Fd, a.sigma, rl32(rbl, b), rl32(rel, b), A & ~old, lane, acc, cnt, orf);
-            fwd_vertex_done<true, COMPAT, true>(a, v, old, acc, cnt, 0, lane, r);
+            fwd_vertex_done<true, COMPAT, true>(a, v, old, acc, cnt, 0, lane, r, sc);
         }
     }
     fwd_block_flush(a, r, blockIdx.x);
@@ -386,6 +418,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_hubfin(FwdArgs a, HubArgs h) {
     const int lane = lane_id();
     const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const int nw = gridDim.x * kWavesPerBlock;
+    const double sc = level_scale(a.lexp, a.dnext - 1, lane);
     FwdAccum r;
     for (int k = gw; k < h.nhub; k += nw) {
         const int v = h.hub_v[k];
@@ -397,7 +430,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_hubfin(FwdArgs a, HubArgs h) {
             acc += h.part_acc[(size_t)i * kWave + lane];
             cnt += h.part_cnt[(size_t)i * kWave + lane];
         }
-        fwd_vertex_done<true, COMPAT, true>(a, v, old, acc, cnt, 0, lane, r);
+        fwd_vertex_done<true, COMPAT, true>(a, v, old, acc, cnt, 0, lane, r, sc);
     }
     fwd_block_flush(a, r, blockIdx.x);
 }
@@ -517,6 +550,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd(BwdArgs a) {
     const int lane = lane_id();
     const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const int nw = gridDim.x * kWavesPerBlock;
+    const double sc = level_scale(a.lexp, a.d, lane);
     for (int c = gw; c < a.nchunk; c += nw) {
         const int base = a.chunk_beg[c];
         const int vl = base + lane;
@@ -536,7 +570,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd(BwdArgs a) {
             work &= work - 1;
             const uint64_t m = rl64(fl, b);
             const double acc = gather_bwd<true>(a.col, a.Fn, a.coef, rl32(rbl, b), rl32(rel, b), m, lane);
-            bwd_vertex_done<true>(a, base + b, m, rl32(bl, b), acc, lane, COMPAT);
+            bwd_vertex_done<true>(a, base + b, m, rl32(bl, b), acc, lane, COMPAT, sc);
         }
     }
 }
@@ -561,6 +595,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd_hubfin(BwdArgs a, HubArgs h) {
     const int lane = lane_id();
     const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const int nw = gridDim.x * kWavesPerBlock;
+    const double sc = level_scale(a.lexp, a.d, lane);
     for (int k = gw; k < h.nhub; k += nw) {
         const int v = h.hub_v[k];
         uint64_t m;
@@ -570,7 +605,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd_hubfin(BwdArgs a, HubArgs h) {
         double acc = 0.0;
         for (int i = h.hub_seg_off[k] - h.seg_base; i < h.hub_seg_off[k + 1] - h.seg_base; ++i)
             acc += h.part_acc[(size_t)i * kWave + lane];
-        bwd_vertex_done<true>(a, v, m, bm, acc, lane, COMPAT);
+        bwd_vertex_done<true>(a, v, m, bm, acc, lane, COMPAT, sc);
     }
 }
 

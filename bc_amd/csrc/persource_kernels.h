@@ -43,6 +43,18 @@
 // Rows longer than kPsBigDeg are queued in LDS and scanned by a whole
 // wavefront (lane-strided edges, butterfly sum), so one hub row does not
 // serialise a level.
+//
+// Path counts that grow past 2^512 (deep graphs with many parallel
+// shortest paths, e.g. chains of diamonds) would leave FP64 a few levels
+// later.  A source whose forward reads such a count is undone (its
+// records restored and its distance sums taken back) and run again in
+// scaled form: before each level is expanded, the level's largest count
+// exponent E is found and, when E >= 2^512's, every count read from the
+// level is multiplied by 2^(256 - E) (exact, a power of two), the shift
+// being kept per level in the slot's table; the backward multiplies that
+// level's dependencies by the same factor (delta_d = sigma~_d * 2^-k_d *
+// sum coef~_{d+1}, with coef~ = (1 + delta) / sigma~ of the stored counts).
+// Sources whose counts stay below 2^512 run as before, bit for bit.
 #pragma once
 #include "records.h"
 #include "wave.h"
@@ -70,7 +82,7 @@ struct PsArgs {
     PsRec* __restrict__ rec;              // BC: [G][n]
     uint32_t* __restrict__ bits;          // closeness: [G][nw]
     int* __restrict__ Q;                  // [G][n]
-    int* __restrict__ loff;               // BC: [G][lcap]
+    int* __restrict__ loff;               // BC: [G][2 lcap]: level offsets, then per-level count shifts
     int lcap;
     double* __restrict__ bcp;             // BC: [G][n]
     int64_t* __restrict__ far;            // [k]
@@ -277,17 +289,23 @@ __global__ __launch_bounds__(kPsBlock) void k_ps_bc(PsArgs a) {
     __shared__ int s_nbig[3];    // long rows of level d: s_big[..s_nbig[d % 3])
     __shared__ int s_big[kPsBig];
     __shared__ int64_t s_red64[kPsWaves][3];
+    __shared__ int s_large;      // a count read by the forward reached 2^512
+    __shared__ int s_emax;       // scaled form: largest count exponent of the level
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
     const int wid = tid >> 6;
     const size_t so = (size_t)blockIdx.x * (size_t)a.n;
     PsRec* R = a.rec + so;
     int* Q = a.Q + so;
-    int* loff = a.loff + (size_t)blockIdx.x * (size_t)a.lcap;
+    int* loff = a.loff + (size_t)blockIdx.x * (size_t)a.lcap * 2;
+    int* lsh = loff + a.lcap;    // scaled form: shift k_d of level d's counts
     double* bcp = a.bcp + so;
+    int scaled_si = -1;          // the source now being redone in scaled form
     for (int si = blockIdx.x; si < a.k; si += gridDim.x) {
         const int s = a.src[si];
+        const bool scaled = si == scaled_si;
         if (tid == 0) {
+            s_large = 0;
             Q[0] = s;
             R[s].dist = 0;
             R[s].sig = 1.0;
@@ -307,6 +325,18 @@ __global__ __launch_bounds__(kPsBlock) void k_ps_bc(PsArgs a) {
                 s_cnt[(d + 1) % 3] = 0;
                 s_nbig[(d + 1) % 3] = 0;
             }
+            double sc = 1.0;  // scale of level d's counts
+            if (scaled) {
+                if (tid == 0) s_emax = 0;
+                __syncthreads();
+                int e = 0;
+                for (int i = lb + tid; i < le; i += kPsBlock) e = max(e, dexp(ld_sig(&R[Q[i]].sig)));
+                if (e >= kSigBig) atomicMax(&s_emax, e);
+                __syncthreads();
+                const int kd = s_emax >= kSigBig ? s_emax - kSigTo : 0;
+                if (tid == 0) lsh[d] = kd;
+                sc = __longlong_as_double((long long)(1023 - kd) << 52);
+            }
             for (int i0 = lb + tid; i0 < le; i0 += kPsBlock * kPsV) {
                 int u[kPsV], rb[kPsV], dg[kPsV];
                 double su[kPsV];
@@ -323,7 +353,8 @@ __global__ __launch_bounds__(kPsBlock) void k_ps_bc(PsArgs a) {
                     if (u[m] >= 0) {
                         rb[m] = a.rp[u[m]];
                         dg[m] = a.rp[u[m] + 1] - rb[m];
-                        su[m] = ld_sig(&R[u[m]].sig);
+                        su[m] = ld_sig(&R[u[m]].sig) * sc;
+                        if (su[m] >= 0x1p512) s_large = 1;  // unscaled form only (scaled counts stay near 2^256)
                     }
                 }
 #pragma unroll
@@ -341,10 +372,16 @@ __global__ __launch_bounds__(kPsBlock) void k_ps_bc(PsArgs a) {
             const int nbig = min(s_nbig[c], kPsBig);
             for (int t = wid; t < nbig; t += kPsWaves) {
                 const int u = s_big[t];
-                ps_expand_long<COMPAT>(a, R, Q, a.rp[u], a.rp[u + 1], lane, ld_sig(&R[u].sig), d, le, &s_cnt[c], L);
+                const double su = ld_sig(&R[u].sig) * sc;
+                if (su >= 0x1p512) s_large = 1;
+                ps_expand_long<COMPAT>(a, R, Q, a.rp[u], a.rp[u + 1], lane, su, d, le, &s_cnt[c], L);
             }
             __syncthreads();
             const int nnew = s_cnt[c];
+            if (s_large) {  // undo this pass of the source; it runs again in scaled form
+                le += nnew;
+                break;
+            }
             if (nnew == 0) break;
             if (d + 3 > a.lcap) {  // level offsets would not fit: the host reruns another way
                 deep = true;
@@ -358,6 +395,20 @@ __global__ __launch_bounds__(kPsBlock) void k_ps_bc(PsArgs a) {
         }
         const int depth = d;
         const int tail = le;
+        if (s_large) {
+            // records back to idle and this pass's distance sums taken back
+            // (integers, so the redo leaves them exactly as one pass would)
+            for (int i = tid; i < tail; i += kPsBlock) {
+                const int v = Q[i];
+                const int dv = R[v].dist;
+                if (a.dist_sum && dv > 0) atomicAdd((unsigned long long*)&a.dist_sum[v], (unsigned long long)(-(int64_t)dv));
+                R[v] = PsRec{-1, 0u, 0.0};
+            }
+            __syncthreads();
+            scaled_si = si;
+            si -= gridDim.x;  // the same source again
+            continue;
+        }
         if (deep && tid == 0) atomicOr(a.overflow, 2);
         // ---------------- backward
         if (!deep) {
@@ -367,6 +418,7 @@ __global__ __launch_bounds__(kPsBlock) void k_ps_bc(PsArgs a) {
                 const int c = dd % 3;
                 if (tid == 0) s_nbig[(dd + 2) % 3] = 0;  // the list level dd - 1 will use
                 const int b0 = loff[dd], b1 = loff[dd + 1];
+                const double sc = scaled ? __longlong_as_double((long long)(1023 - lsh[dd]) << 52) : 1.0;
                 for (int i0 = b0 + tid; i0 < b1; i0 += kPsBlock * kPsV) {
                     int v[kPsV], rb[kPsV], dg[kPsV];
                     double sv[kPsV];
@@ -404,7 +456,7 @@ __global__ __launch_bounds__(kPsBlock) void k_ps_bc(PsArgs a) {
 #pragma unroll
                     for (int m = 0; m < kPsV; ++m) {
                         if (v[m] < 0 || big[m]) continue;
-                        const double dl = sv[m] * acc[m];
+                        const double dl = (sv[m] * acc[m]) * sc;
                         if (__builtin_isinf(sv[m])) atomicOr(a.overflow, 1);
                         R[v[m]].sig = (1.0 + dl) / sv[m];
                         bcp[v[m]] += COMPAT ? dl * (double)ld_npred(&R[v[m]].npred) : dl;
@@ -415,7 +467,7 @@ __global__ __launch_bounds__(kPsBlock) void k_ps_bc(PsArgs a) {
                     const double acc = wave_sum(ps_pull_long(a, R, a.rp[v], a.rp[v + 1], lane, dd + 1));
                     if (lane == 0) {
                         const double sv = ld_sig(&R[v].sig);
-                        const double dl = sv * acc;
+                        const double dl = (sv * acc) * sc;
                         if (__builtin_isinf(sv)) atomicOr(a.overflow, 1);
                         R[v].sig = (1.0 + dl) / sv;
                         bcp[v] += COMPAT ? dl * (double)ld_npred(&R[v].npred) : dl;

@@ -29,7 +29,8 @@ enum {
     kCtlOverflow = 5, // path count became inf, or a list ran out of room
     kCtlDeepest = 6,  // deepest non-empty level (set when the forward ends)
     kCtlBwdLevel = 7, // backward: level being processed
-    kCtlSlots = 8
+    kCtlTooDeep = 8,  // the BFS needs more levels than the path-count exponent table
+    kCtlSlots = 9
 };
 
 struct SpArgs {
@@ -55,6 +56,8 @@ struct SpArgs {
     const uint64_t* __restrict__ Aptr;    // active lanes of the batch
     int64_t* __restrict__ part_far;       // [grid][64]
     unsigned long long* __restrict__ part_cnt;  // [grid][2]
+    int* __restrict__ lexp;               // [lexp_levels][64] path-count exponents (betweenness)
+    int lexp_levels;
 };
 
 __device__ __forceinline__ uint64_t lanemask_lt(int lane) { return (lane == 0) ? 0ull : (~0ull >> (64 - lane)); }
@@ -129,6 +132,8 @@ __global__ __launch_bounds__(kBlock) void k_sp_pull(SpArgs a) {
     FwdArgs f{};  // the shared per-vertex epilogue works on FwdArgs
     f.n = a.n; f.rp = a.rp; f.col = a.col; f.vis = a.vis; f.Fd = Fd; f.Fn = Fn; f.sigma = a.sigma;
     f.npred = a.npred; f.dist_sum = a.dist_sum; f.weight = a.weight; f.A = A; f.dnext = d + 1;
+    f.lexp = SIGMA ? a.lexp : nullptr;
+    const double sc = SIGMA ? level_scale(a.lexp, d, lane) : 1.0;
     FwdAccum r;
     for (int base = gw * kWave; base < nc; base += nw * kWave) {
         const int i = base + lane;
@@ -155,7 +160,7 @@ __global__ __launch_bounds__(kBlock) void k_sp_pull(SpArgs a) {
             uint64_t nm;
             if constexpr (SIGMA) nm = ballot(cnt != 0);
             else nm = wave_or(orf);
-            fwd_vertex_done<SIGMA, COMPAT, false>(f, vb, old, acc, cnt, nm, lane, r);
+            fwd_vertex_done<SIGMA, COMPAT, false>(f, vb, old, acc, cnt, nm, lane, r, sc);
             if (nm) {
                 newbits |= 1ull << b;
                 if (lane == b) my_nm = nm;
@@ -204,8 +209,9 @@ __global__ void k_sp_advance(SpArgs a) {
     a.ctl[kCtlNCand] = 0;
     a.ctl[kCtlNNext] = 0;
     a.ctl[kCtlStamp] += 1;
-    if (cnt == 0 || a.ctl[kCtlOverflow] || d + 3 >= a.qoff_cap) {
-        if (cnt != 0) a.ctl[kCtlOverflow] = 1;
+    const bool deep = a.lexp && d + 2 >= a.lexp_levels;
+    if (cnt == 0 || a.ctl[kCtlOverflow] || d + 3 >= a.qoff_cap || deep) {
+        if (cnt != 0) a.ctl[deep ? kCtlTooDeep : kCtlOverflow] = 1;
         a.ctl[kCtlDone] = 1;
         a.ctl[kCtlDeepest] = d;
         a.qoff[d + 2] = a.qoff[d + 1];  // empty level d+1 (qoff_cap >= d + 3 always holds here)
@@ -229,6 +235,7 @@ __global__ __launch_bounds__(kBlock) void k_sp_bwd(SpArgs a) {
     uint64_t* Fo = a.Fd2 + (size_t)a.n * (d & 1);
     BwdArgs b{};
     b.sigma = a.sigma; b.coef = a.coef; b.npred = a.npred; b.bc = a.bc;
+    const double sc = level_scale(a.lexp, d, lane);
     const int beg = a.qoff[d], end = a.qoff[d + 1];
     for (int base = beg + gw * kWave; base < end; base += nw * kWave) {
         const int i = base + lane;
@@ -246,7 +253,7 @@ __global__ __launch_bounds__(kBlock) void k_sp_bwd(SpArgs a) {
             work &= work - 1;
             const uint64_t m = rl64(fl, bb);
             const double acc = gather_bwd<false>(a.col, Fn, a.coef, rl32(rb, bb), rl32(re, bb), m, lane);
-            bwd_vertex_done<false>(b, rl32(v, bb), m, 0, acc, lane, COMPAT);
+            bwd_vertex_done<false>(b, rl32(v, bb), m, 0, acc, lane, COMPAT, sc);
         }
     }
     if (d + 2 <= a.ctl[kCtlDeepest] + 1) {
@@ -310,15 +317,21 @@ __global__ void k_sp_seed(SpArgs a, const int* __restrict__ src, int k, bool wit
         a.ctl[kCtlOverflow] = 0;
         a.ctl[kCtlDeepest] = 0;
         a.ctl[kCtlBwdLevel] = 0;
+        a.ctl[kCtlTooDeep] = 0;
     }
 }
 
 // End of batch: clear vis and both dense mask arrays for every vertex the
-// batch reached (all of them are on some frontier list).
+// batch reached (all of them are on some frontier list), and the path-count
+// exponents of its levels.
 __global__ __launch_bounds__(kBlock) void k_sp_clear(SpArgs a) {
     const int end = a.qoff[a.ctl[kCtlDeepest] + 1];
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int nthr = gridDim.x * blockDim.x;
+    if (a.lexp) {
+        const int ne = min(a.ctl[kCtlDeepest] + 2, a.lexp_levels) * kWave;
+        for (int i = gtid; i < ne; i += nthr) a.lexp[i] = 0;
+    }
     for (int i = gtid; i < end; i += nthr) {
         const int v = a.Q[i];
         a.vis[v] = 0;

@@ -39,4 +39,16 @@ __device__ __forceinline__ int64_t wave_sum_i64(int64_t x) {
     return x;
 }
 
+// Path-count scaling (betweenness steps): counts at or above 2^kSigBig
+// mark their level, whose counts the next level brings back to about
+// 2^kSigTo by a power-of-two factor.  A level's sums then stay below
+// 2^(kSigBig + 31) (at most 2^31 terms), well inside FP64.
+constexpr int kSigBig = 512;
+constexpr int kSigTo = 256;
+
+// Unbiased exponent of a positive normal double.
+__device__ __forceinline__ int dexp(double x) {
+    return (int)((__double_as_longlong(x) >> 52) & 0x7ff) - 1023;
+}
+
 }  // namespace cent
