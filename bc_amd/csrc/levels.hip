@@ -187,7 +187,8 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
         add(plane(1) + fw * v.lo, fw * R);
         add(ctx->part_far.p, (size_t)G * kWave * nb);
         add(ctx->part_cnt.p, (size_t)G * 2 * nb);
-        if (sp.with_bc) add(ctx->lexp.p, ctx->lexp.cap / 2);  // 64 ints a level: 32 words
+        if (sp.with_bc)  // the exponent rows of every level this batch can reach: 64 ints, 32 words each
+            add(ctx->lexp.p, std::min(ctx->lexp.cap, (ctx->F_levels + 2) * (size_t)kWave) / 2);
         size_t tot = 0;
         for (int i = 0; i < cr.count; ++i) tot = std::max(tot, cr.n[i]);
         const int cg = (int)std::min<size_t>((tot + 255) / 256, 4096);

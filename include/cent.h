@@ -35,7 +35,8 @@ enum {
     CENT_E_INVAL     = 1,   /* bad argument (null pointer, id out of range, ...) */
     CENT_E_OOM       = 2,   /* host or device allocation failed / over budget   */
     CENT_E_DEVICE    = 3,   /* HIP runtime error, or no usable gfx950 device     */
-    CENT_E_OVERFLOW  = 4,   /* shortest-path count left the FP64 range          */
+    CENT_E_OVERFLOW  = 4,   /* shortest-path counts of one BFS level span more
+                               than FP64's range even after per-level scaling  */
     CENT_E_IO        = 5,   /* file could not be opened or parsed               */
     CENT_E_BUSY      = 6,   /* server queue full                                */
     CENT_E_CANCELLED = 7,   /* request cancelled                                */
@@ -207,7 +208,11 @@ void cent_graph_free(cent_graph* g);
  *   farness[i]  = sum of BFS distances from src[i] over its component
  *                 (assigned, int64)                 — Seq_CC/CC.cpp:94
  *   dist_sum[v] += sum over the sources of d(src, v) (int64, push form)
- * Returns CENT_E_OVERFLOW if a path count leaves the FP64 range.
+ * Path counts are kept in FP64 with a power-of-two scale per BFS level and
+ * source, so counts beyond 2^1024 (the reference's FP32 counts overflow
+ * silently past 2^128, Seq_Brandes/Brandes.c:226-227) still give Brandes'
+ * values; CENT_E_OVERFLOW remains only for counts within one level that
+ * differ by more than a factor 2^1278.
  */
 int  cent_bc(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k,
              double* bc, int64_t* farness, int64_t* dist_sum, const cent_opts* opts);

@@ -309,18 +309,29 @@ def diamond_chain(k):
 
 
 @pytest.mark.parametrize("frontier", ["dense", "sparse", "source"])
-def test_path_count_overflow_is_reported(ctx, frontier):
-    print(f"[gpu] overflow frontier={frontier}", flush=True)
-    g = diamond_chain(1100)
-    dg = ctx.upload(g, keep_order=True)
-    with pytest.raises(bc_amd.CentError) as e:
-        ctx.bc(dg, [0], frontier=frontier)
-    assert e.value.status == 4  # CENT_E_OVERFLOW
-    # closeness has no path counts: farness of h_0 in closed form
-    far, _ = ctx.cc(dg, [0], frontier=frontier)
+def test_path_counts_beyond_fp64_are_scaled(ctx, frontier):
+    """2^1100 shortest paths from h_0 to h_1100: the counts are scaled per
+    level, not refused.  From h_0 alone the dependencies have a closed form:
+    delta(h_i) = 3 (k - i) vertices lie beyond h_i, and each middle vertex of
+    diamond i carries half of the paths into h_{i+1}:
+    delta(a_i) = (1 + 3 (k - i - 1)) / 2."""
+    print(f"[gpu] scaled counts frontier={frontier}", flush=True)
     k = 1100
+    g = diamond_chain(k)
+    dg = ctx.upload(g, keep_order=True)
+    bc, far, _ = ctx.bc(dg, [0], frontier=frontier)
+    want = np.zeros(g.n)
+    for i in range(k):
+        want[3 * i + 1] = want[3 * i + 2] = (1 + 3 * (k - i - 1)) / 2
+        if i > 0:
+            want[3 * i] = 3 * (k - i)
+    assert np.all(np.isfinite(bc))
+    np.testing.assert_allclose(bc, want, rtol=1e-12, atol=0)
+    # farness in closed form: h_i at 2i, the middle vertices of diamond i at 2i + 1
     assert far[0] == sum(2 * i for i in range(1, k + 1)) + sum(2 * (2 * i + 1) for i in range(k))
-    # below the limit the counts are exact powers of two and BC is finite
+    cfar, _ = ctx.cc(dg, [0], frontier=frontier)
+    assert cfar[0] == far[0]
+    # below 2^1024 the counts are exact powers of two and BC equals the oracle's
     g2 = diamond_chain(1000)
     dg2 = ctx.upload(g2, keep_order=True)
     bc, far2, _ = ctx.bc(dg2, [0], frontier=frontier)
