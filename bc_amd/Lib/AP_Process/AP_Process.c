@@ -396,7 +396,20 @@ int cent_reduced_finish(const struct cent_reduced* r, const int64_t* D, int64_t*
 }
 
 /* ---- C-ABI wrappers (include/cent.h) */
-int cent_reduce(const struct CSR* csr, cent_reduced** out) { return cent_reduced_build(csr, out); }
+int cent_reduce(const struct CSR* csr, cent_reduced** out) {
+    if (!csr || !out) return CENT_E_INVAL;
+    const int multi = cent_csr_is_multigraph(csr);
+    if (multi < 0) return CENT_E_OOM;
+    if (!multi) return cent_reduced_build(csr, out);
+    /* distances (hence farness) ignore parallel edges and self-loops */
+    struct CSR* simple = NULL;
+    int st = cent_csr_simple(csr, &simple);
+    if (st) return st;
+    st = cent_reduced_build(simple, out);
+    cent_csr_free(simple);
+    if (!st) (*out)->multi = 1;
+    return st;
+}
 
 int cent_reduced_get_view(const cent_reduced* r, cent_reduced_view* v) {
     if (!r || !v) return CENT_E_INVAL;
@@ -464,4 +477,7 @@ int cent_reduced_finish_bc(const struct cent_reduced* r, const double* bcp, doub
     return CENT_OK;
 }
 
-int cent_reduced_bc(const cent_reduced* r, const double* bcp, double* bc) { return cent_reduced_finish_bc(r, bcp, bc); }
+int cent_reduced_bc(const cent_reduced* r, const double* bcp, double* bc) {
+    if (r && r->multi) return CENT_E_UNSUPPORTED;
+    return cent_reduced_finish_bc(r, bcp, bc);
+}

@@ -29,8 +29,10 @@ static int parse2(const char* s, long long* a, long long* b) {
     return 1;
 }
 
-int cent_load_edgelist(const char* path, struct CSR** out) {
-    if (!path || !out) return CENT_E_INVAL;
+int cent_load_edgelist(const char* path, struct CSR** out) { return cent_load_edgelist_ex(path, 0, out); }
+
+int cent_load_edgelist_ex(const char* path, int32_t flags, struct CSR** out) {
+    if (!path || !out || (flags & ~CENT_CSR_RAW)) return CENT_E_INVAL;
     FILE* f = fopen(path, "r");
     if (!f) return CENT_E_IO;
     char buf[4096];
@@ -62,7 +64,7 @@ int cent_load_edgelist(const char* path, struct CSR** out) {
     int32_t n = zero ? (int32_t)nn : (int32_t)nn + 1;
     if (mx >= n) { free(u); free(v); return CENT_E_IO; }
     struct CSR* g = NULL;
-    int st = cent_csr_from_edges(n, u, v, m, 0, &g, NULL);
+    int st = cent_csr_from_edges(n, u, v, m, flags, &g, NULL);
     free(u); free(v);
     if (st) return st;
     g->startAtZero = zero;
@@ -87,9 +89,12 @@ int cent_write_edgelist(const char* path, const struct CSR* csr) {
     FILE* f = fopen(path, "w");
     if (!f) return CENT_E_IO;
     int ok = fprintf(f, "%d %lld\n", csr->totalNodeNumber, (long long)(csr->csrESize / 2)) > 0;
-    for (int32_t u = 0; ok && u < n; ++u)
+    for (int32_t u = 0; ok && u < n; ++u) {
+        int self = 0;  /* a self-loop is listed twice in its row: written once */
         for (int32_t j = csr->csrV[u]; ok && j < csr->csrV[u + 1]; ++j)
-            if (csr->csrE[j] > u) ok = fprintf(f, "%d %d\n", u, csr->csrE[j]) > 0;
+            if (csr->csrE[j] > u || (csr->csrE[j] == u && (self++ & 1) == 0))
+                ok = fprintf(f, "%d %d\n", u, csr->csrE[j]) > 0;
+    }
     if (fclose(f) != 0) ok = 0;
     return ok ? CENT_OK : CENT_E_IO;
 }

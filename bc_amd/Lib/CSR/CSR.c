@@ -3,7 +3,8 @@
  * adjacency build of buildGraph, Lib/AdjList/AdjList.c:76-92).
  *
  * Differences from the reference, on purpose:
- *  - self-loops and duplicate edges are removed (the reference keeps them);
+ *  - self-loops and duplicate edges are removed (the reference keeps them),
+ *    unless CENT_CSR_RAW asks for the reference's multigraph as it is;
  *  - rows are sorted by neighbour id, not by neighbour degree (AdjList.c:105-131);
  *    this changes traversal order only, never a BFS distance, path count or sum;
  *  - no spare capacity for articulation-point clones: the reduction code
@@ -42,7 +43,8 @@ int cent_ivec_push(cent_ivec* v, int32_t x) {
 }
 void cent_ivec_free(cent_ivec* v) { free(v->a); v->a = NULL; v->n = v->cap = 0; }
 
-int64_t cent_sort_unique_u64(uint64_t* keys, int64_t n) {
+/* Radix sort of keys (ascending); returns n, or -1 when out of memory. */
+int64_t cent_sort_u64(uint64_t* keys, int64_t n) {
     if (n <= 1) return n;
     uint64_t* tmp = (uint64_t*)malloc((size_t)n * sizeof(uint64_t));
     if (!tmp) return -1;
@@ -62,6 +64,13 @@ int64_t cent_sort_unique_u64(uint64_t* keys, int64_t n) {
     }
     if (src != keys) memcpy(keys, src, (size_t)n * sizeof(uint64_t));
     free(tmp);
+    return n;
+}
+
+/* Sorts keys and drops repeats; returns the number kept, or -1 (OOM). */
+int64_t cent_sort_unique_u64(uint64_t* keys, int64_t n) {
+    if (cent_sort_u64(keys, n) < 0) return -1;
+    if (n <= 1) return n;
     int64_t w = 1;
     for (int64_t i = 1; i < n; ++i) if (keys[i] != keys[w - 1]) keys[w++] = keys[i];
     return w;
@@ -189,18 +198,23 @@ static int keep_lcc(struct CSR* g, struct CSR** out, int32_t** old_id) {
 int cent_csr_from_edges(int32_t n, const int32_t* u, const int32_t* v, int64_t m,
                         int32_t flags, struct CSR** out, int32_t** old_id) {
     if (!out || n < 0 || m < 0 || (m > 0 && (!u || !v))) return CENT_E_INVAL;
+    if ((flags & CENT_CSR_RAW) && (flags & CENT_CSR_LCC)) return CENT_E_INVAL;
     if (old_id) *old_id = NULL;
+    const int raw = (flags & CENT_CSR_RAW) != 0;
     uint64_t* keys = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)(2 * m + 1));
     if (!keys) return CENT_E_OOM;
     int64_t nk = 0;
     for (int64_t i = 0; i < m; ++i) {
         int32_t a = u[i], b = v[i];
         if (a < 0 || b < 0 || a >= n || b >= n) { free(keys); return CENT_E_INVAL; }
-        if (a == b) continue;
+        if (a == b && !raw) continue;
+        /* raw: every line gives both directions, so a self-loop lists its
+         * vertex twice in its own row, as the reference's adjacency build
+         * (Lib/AdjList/AdjList.c:76-92) does */
         keys[nk++] = ((uint64_t)(uint32_t)a << 32) | (uint32_t)b;
         keys[nk++] = ((uint64_t)(uint32_t)b << 32) | (uint32_t)a;
     }
-    nk = cent_sort_unique_u64(keys, nk);
+    nk = raw ? cent_sort_u64(keys, nk) : cent_sort_unique_u64(keys, nk);
     if (nk < 0) { free(keys); return CENT_E_OOM; }
     struct CSR* g = NULL;
     int st = cent_csr_from_sorted_keys(n, keys, nk, &g);
@@ -229,5 +243,47 @@ int cent_csr_from_arrays(int32_t n, const int32_t* row_ptr, const int32_t* col, 
     if (nnz) memcpy(g->csrE, col, sizeof(int32_t) * (size_t)nnz);
     csr_finish(g);
     *out = g;
+    return CENT_OK;
+}
+
+/* 1 if some row lists a vertex twice or lists its own vertex. */
+int cent_csr_is_multigraph(const struct CSR* g) {
+    const int32_t n = g->csrVSize - 1;
+    int32_t* stamp = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    if (!stamp) return -1;
+    for (int32_t v = 0; v < n; ++v) stamp[v] = -1;
+    int multi = 0;
+    for (int32_t v = 0; v < n && !multi; ++v)
+        for (int32_t j = g->csrV[v]; j < g->csrV[v + 1]; ++j) {
+            const int32_t y = g->csrE[j];
+            if (y == v || stamp[y] == v) { multi = 1; break; }
+            stamp[y] = v;
+        }
+    free(stamp);
+    return multi;
+}
+
+/* The simple graph under g: self-loops dropped, each neighbour once per row
+ * (rows sorted by id). */
+int cent_csr_simple(const struct CSR* g, struct CSR** out) {
+    const int32_t n = g->csrVSize - 1;
+    const int64_t nnz = g->csrV[n];
+    uint64_t* keys = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)(nnz > 0 ? nnz : 1));
+    if (!keys) return CENT_E_OOM;
+    int64_t nk = 0;
+    for (int32_t v = 0; v < n; ++v)
+        for (int32_t j = g->csrV[v]; j < g->csrV[v + 1]; ++j)
+            if (g->csrE[j] != v) keys[nk++] = ((uint64_t)(uint32_t)v << 32) | (uint32_t)g->csrE[j];
+    nk = cent_sort_unique_u64(keys, nk);
+    if (nk < 0) { free(keys); return CENT_E_OOM; }
+    struct CSR* h = NULL;
+    int st = cent_csr_from_sorted_keys(n, keys, nk, &h);
+    free(keys);
+    if (st) return st;
+    h->startNodeID = g->startNodeID;
+    h->endNodeID = g->endNodeID;
+    h->totalNodeNumber = g->totalNodeNumber;
+    h->startAtZero = g->startAtZero;
+    *out = h;
     return CENT_OK;
 }

@@ -48,6 +48,8 @@ static inline uint64_t cent_rng_below(cent_rng* r, uint64_t bound) {
 /* ---- sort + unique of 64-bit keys (LSD radix, 8-bit digits, passes whose
  *      digit is constant are skipped).  Returns the number of unique keys. */
 int64_t cent_sort_unique_u64(uint64_t* keys, int64_t n);
+/* The same sort keeping repeats; returns n (-1: out of memory). */
+int64_t cent_sort_u64(uint64_t* keys, int64_t n);
 
 /* ---- growable int32 vector */
 typedef struct { int32_t* a; int64_t n, cap; } cent_ivec;

@@ -33,6 +33,14 @@ struct cent_d1 {
 };
 
 int  cent_d1_fold(const struct CSR* g, struct cent_d1* out);
+
+/* 1 if a row lists a neighbour twice or lists its own vertex (a graph
+ * loaded with CENT_CSR_RAW), 0 if not, -1 when out of memory.  Both
+ * reductions work on the simple graph of such an input (Lib/CSR/CSR.c). */
+int cent_csr_is_multigraph(const struct CSR* g);
+/* The simple graph under g (self-loops dropped, repeated neighbours once),
+ * same ids and id base. */
+int cent_csr_simple(const struct CSR* g, struct CSR** out);
 void cent_d1_free(struct cent_d1* d);
 
 /* One schedulable unit of GPU work: sources s0 .. s0+ns-1 (packed ids, ns <= 64)
@@ -62,6 +70,10 @@ struct cent_reduced {
     int64_t* comp_w;        /* [ncomp] vertices in the component               */
     int32_t  ncomp;
     int32_t  n_ap;          /* articulation vertices of the folded graph       */
+    int32_t  multi;         /* built from a multigraph (CENT_CSR_RAW input): the
+                               reduction is of its simple graph, which has the
+                               same distances, so closeness holds and
+                               betweenness (path counts) does not            */
     /* work items over the packed graph */
     int32_t  nitems;
     struct cent_item* items;

@@ -39,6 +39,7 @@ CENT_OPT_TIMING = 1
 CENT_OPT_SOURCE_ORDER = 2
 CENT_OPT_ONE_STREAM = 4
 CENT_CSR_LCC = 1
+CENT_CSR_RAW = 2
 
 _lib.cent_strerror.restype = C.c_char_p
 _lib.cent_strerror.argtypes = [C.c_int]
@@ -87,6 +88,7 @@ _lib.cent_csr_free.restype = None
 _lib.cent_free.argtypes = [C.c_void_p]
 _lib.cent_free.restype = None
 _lib.cent_load_edgelist.argtypes = [C.c_char_p, C.POINTER(_P_CSR)]
+_lib.cent_load_edgelist_ex.argtypes = [C.c_char_p, C.c_int32, C.POINTER(_P_CSR)]
 _lib.cent_gen_rmat.argtypes = [C.c_int32, C.c_int64, C.c_double, C.c_double, C.c_double,
                                C.c_uint64, C.c_int32, C.POINTER(_P_CSR)]
 _lib.cent_gen_grid.argtypes = [C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_uint64,
@@ -136,7 +138,8 @@ class CSR:
             self._p = None
 
     @classmethod
-    def from_edges(cls, n: int, u, v, lcc: bool = False):
+    def from_edges(cls, n: int, u, v, lcc: bool = False, raw: bool = False):
+        """raw=True keeps repeated edges and self-loops (CENT_CSR_RAW)."""
         u = np.ascontiguousarray(u, dtype=np.int32)
         v = np.ascontiguousarray(v, dtype=np.int32)
         if u.shape != v.shape:
@@ -144,7 +147,7 @@ class CSR:
         out = _P_CSR()
         old = _I32P()
         _check(_lib.cent_csr_from_edges(n, _ptr(u, C.c_int32), _ptr(v, C.c_int32), len(u),
-                                        CENT_CSR_LCC if lcc else 0, C.byref(out),
+                                        (CENT_CSR_LCC if lcc else 0) | (CENT_CSR_RAW if raw else 0), C.byref(out),
                                         C.byref(old) if lcc else None), "cent_csr_from_edges")
         g = cls(out)
         if lcc and old:
@@ -166,10 +169,13 @@ class CSR:
         return np.arange(self.startNodeID, self.endNodeID + 1, dtype=np.int32)
 
 
-def buildGraph(path: str) -> CSR:
-    """Load a reference-format edge list (Lib/AdjList/AdjList.c:13-150)."""
+def buildGraph(path: str, raw: bool = False) -> CSR:
+    """Load a reference-format edge list (Lib/AdjList/AdjList.c:13-150).
+    raw=True keeps repeated edges and self-loops as the reference's loader
+    does (CENT_CSR_RAW); by default they are removed."""
     out = _P_CSR()
-    _check(_lib.cent_load_edgelist(os.fsencode(path), C.byref(out)), "buildGraph", path)
+    _check(_lib.cent_load_edgelist_ex(os.fsencode(path), CENT_CSR_RAW if raw else 0, C.byref(out)),
+           "buildGraph", path)
     return CSR(out)
 
 

@@ -277,7 +277,9 @@ __global__ __launch_bounds__(kBlock) void k_ccw_push_fin(CcwArgs a, const int* _
         const int u = a.col[s_beg[s] + e - s_pre[s]];
         const uint64_t nm = a.Fn[(size_t)u * W + b];
         if (!((nm >> s) & 1) || __builtin_ctzll(nm) != s) continue;  // finished by its lowest lane
-        a.vis[(size_t)u * W + b] |= nm;
+        // ... once: a repeated edge gives that lane two threads for u, and the
+        // first to set the new lanes in vis finishes it (vis held none of them)
+        if (atomicOr((unsigned long long*)&a.vis[(size_t)u * W + b], (unsigned long long)nm) & nm) continue;
         const unsigned long long w = a.weight ? (unsigned long long)a.weight[u] : 1ull;
         for (uint64_t m = nm; m; m &= m - 1) atomicAdd(&s_far[__builtin_ctzll(m)], w);
         const int pc = __popcll(nm);

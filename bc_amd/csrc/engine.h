@@ -171,6 +171,7 @@ struct DevCSR {
     int* hub_buf = nullptr;
     int *seg_v = nullptr, *seg_beg = nullptr, *seg_end = nullptr, *hub_v = nullptr, *hub_seg_off = nullptr;
     std::vector<int> h_chunk_beg, h_hub_v, h_hub_seg_off;
+    bool multi = false;  // rows may repeat a neighbour or list their own vertex
 };
 
 // Failure injection for tests of the error paths (cent_debug_fail_upload):
@@ -236,6 +237,7 @@ struct cent_graph {
     std::vector<int32_t> probe_dist;  // BFS distance from the highest-degree vertex (caller ids, -1 unreached)
     std::vector<int32_t> probe_order; // that BFS's visit order (caller ids; dropped after the relabel)
     bool bfs_order = false;           // device ids are a BFS order (else degree order or the caller's)
+    bool multi = false;               // parallel edges or self-loops (CENT_CSR_RAW input)
     cent_tune_sig sig{};              // autotuner signature
     // launch defaults when the call's opts leave them open: [0] betweenness,
     // [1] closeness

@@ -260,6 +260,21 @@ extern "C" int cent_graph_create(cent_ctx* ctx, const int32_t* row_ptr, const in
         g->comp_nnz.push_back(e);
         ++nc;
     }
+    // a multigraph (a CENT_CSR_RAW input: repeated neighbours or self-loops)
+    // gets its level-1 path counts from the pull, which counts every
+    // parallel edge, instead of the push, which takes one path per neighbour
+    {
+        std::vector<int32_t> stamp((size_t)(n > 0 ? n : 1), -1);
+        for (int32_t v = 0; v < n && !g->multi; ++v)
+            for (int32_t j = row_ptr[v]; j < row_ptr[v + 1]; ++j) {
+                const int32_t y = col[j];
+                if (y == v || stamp[(size_t)y] == v) {
+                    g->multi = true;
+                    break;
+                }
+                stamp[(size_t)y] = v;
+            }
+    }
     // eccentricity probe from the highest-degree vertex (host BFS, O(n + m))
     if (n > 0) {
         int32_t s0 = 0;
@@ -345,6 +360,7 @@ extern "C" int cent_graph_create(cent_ctx* ctx, const int32_t* row_ptr, const in
         cent_graph_free(g);
         return st;
     }
+    g->d.multi = g->multi;
     g->h_rp.assign(urp, urp + (size_t)n + 1);
     if (!g->new_id.empty()) {  // for results written to device memory in caller order
         hipError_t e = hipMalloc((void**)&g->d_new_id, sizeof(int32_t) * (size_t)n);

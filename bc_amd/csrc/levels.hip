@@ -282,7 +282,7 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
             fa.lexp = ctx->lexp.p;
             hipEvent_t e0;
             TRY(timer_begin(ctx, tm, &e0));
-            if (lv == 0) {  // level 1: push from the sources' rows
+            if (lv == 0 && !d.multi) {  // level 1: push from the sources' rows (one path per neighbour)
                 const int Gp = std::min(G, kPushBlocks);
                 k_push1_or<<<Gp, kBlock, 0, s>>>(fa, d_src);
                 if (compat) k_push1_fin<true><<<Gp, kBlock, 0, s>>>(fa, d_src);

@@ -1135,14 +1135,23 @@ extern "C" int cent_reduce_gpu(cent_ctx* ctx, const struct CSR* csr, cent_reduce
         csr->csrESize > (int64_t)INT32_MAX || (int64_t)csr->csrVSize * kG > (int64_t)INT32_MAX)
         return CENT_E_INVAL;  // the row-scanning kernels index kG-lane groups with int thread ids
     TRY(check_device(ctx));
+    const int multi = cent_csr_is_multigraph(csr);
+    if (multi < 0) return CENT_E_OOM;
+    struct CSR* simple = nullptr;  // distances (hence farness) ignore parallel edges and self-loops
+    if (multi) TRY(cent_csr_simple(csr, &simple));
     cent_reduced* r = (cent_reduced*)calloc(1, sizeof(cent_reduced));
-    if (!r) return CENT_E_OOM;
-    int st = reduce_gpu(ctx, csr, r);
+    if (!r) {
+        cent_csr_free(simple);
+        return CENT_E_OOM;
+    }
+    int st = reduce_gpu(ctx, simple ? simple : csr, r);
+    cent_csr_free(simple);
     if (!st) st = cent_reduced_items(r);
     if (st) {
         cent_reduced_free(r);
         return st;
     }
+    r->multi = multi;
     *out = r;
     return CENT_OK;
 }

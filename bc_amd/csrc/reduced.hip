@@ -372,6 +372,7 @@ extern "C" int cent_bc_reduced(cent_ctx* ctx, const cent_reduced* rc, double* bc
                                int32_t sched, const cent_opts* opts) {
     if (!ctx || !rc || !bc) return CENT_E_INVAL;
     if (sched != CENT_SCHED_PERSISTENT && sched != CENT_SCHED_PER_ITEM) return CENT_E_INVAL;
+    if (rc->multi) return CENT_E_UNSUPPORTED;  // parallel edges multiply path counts: use cent_bc
     cent_reduced* r = const_cast<cent_reduced*>(rc);
     TRY(check_device(ctx));
     auto t0 = std::chrono::steady_clock::now();

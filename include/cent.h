@@ -51,7 +51,8 @@ const char* cent_strerror(int status);
  * Own CSR.  Field names follow the reference's struct CSR (Lib/CSR/CSR.h:34-95)
  * where they carry the same meaning; the layout is 0-based and immutable:
  * vertex v's neighbours are csrE[csrV[v] .. csrV[v+1]).  Self-loops and
- * duplicate edges are removed on construction, both directions are stored.
+ * duplicate edges are removed on construction (unless CENT_CSR_RAW keeps
+ * them), both directions are stored.
  * For a 1-based input file vertex 0 exists with degree 0 (startNodeID = 1),
  * so outputs stay indexed by the file's own ids.
  */
@@ -75,6 +76,15 @@ struct CSR {
  * call (allocated by the library, free with cent_free).  Replaces createCSR
  * (Lib/CSR/CSR.c:9-108), which also did no cleaning. */
 #define CENT_CSR_LCC 1
+/* flags: CENT_CSR_RAW keeps the input as the reference's loader does
+ * (Lib/AdjList/AdjList.c:76-92, no cleaning): a repeated edge stays a
+ * parallel edge (it adds a shortest path wherever it lies on one, like the
+ * reference's per-edge path-count sums), and a self-loop u-u lists u twice
+ * in its own row.  Rows stay sorted by neighbour id.  Not with CENT_CSR_LCC.
+ * Every engine call accepts such a CSR; the reduced (D1/AP) form works on
+ * its simple graph, which has the same distances, so reduced closeness is
+ * unchanged and reduced betweenness returns CENT_E_UNSUPPORTED for it. */
+#define CENT_CSR_RAW 2
 int  cent_csr_from_edges(int32_t n, const int32_t* u, const int32_t* v, int64_t m,
                          int32_t flags, struct CSR** out, int32_t** old_id);
 /* Copy of existing CSR arrays (validated: offsets monotone, ids in range). */
@@ -87,6 +97,10 @@ void cent_free(void* p);
  * lines "u v"; 0- or 1-based ids detected as in Lib/AdjList/AdjList.c:39-86).
  * Replaces buildGraph + createCSR. */
 int  cent_load_edgelist(const char* path, struct CSR** out);
+/* The same with CSR flags: CENT_CSR_RAW keeps repeated edges and self-loops
+ * exactly as the file lists them (the reference's buildGraph does no
+ * cleaning). */
+int  cent_load_edgelist_ex(const char* path, int32_t flags, struct CSR** out);
 
 /* Seeded synthetic generators (host C, deterministic for a given seed). */
 /* R-MAT, 2^scale ids, n_samples edge draws with quadrant probabilities

@@ -9,7 +9,13 @@ ref_golden.json  outputs of the reference's OWN computeBC (FP32, per-predecessor
                  (`make -C oracle ref`); skipped when those are not built.
 
 Vectors are indexed by the file's own ids startNodeID..endNodeID.
+
+RAW_GRAPHS are multigraph fixtures (repeated edges, self-loops) that the
+reference loads as they are (Lib/AdjList/AdjList.c:76-92); they get
+reference vectors only (networkx graphs are simple).  `python make_golden.py
+ref` rewrites ref_golden.json alone.
 """
+import sys
 import json
 import os
 import subprocess
@@ -20,6 +26,7 @@ DS = os.path.join(HERE, "datasets")
 ROOT = os.path.dirname(os.path.dirname(HERE))
 GRAPHS = ["karate.txt", "dolphins.txt", "football.txt", "polbooks.txt", "testdata.txt",
           "testdata1.txt", "testdata2.txt", "ia-fb-messages.mtx"]
+RAW_GRAPHS = ["multi.txt"]
 
 
 def read_edges(path):
@@ -58,9 +65,9 @@ def ref_golden():
         return
     out = {}
     with tempfile.TemporaryDirectory() as td:
-        for name in GRAPHS:
+        for name in GRAPHS + RAW_GRAPHS:
             path = os.path.join(DS, name)
-            entry = {}
+            entry = {"raw": True} if name in RAW_GRAPHS else {}
             for mode in ("cc", "d1ap"):
                 o = os.path.join(td, mode)
                 subprocess.run([ref_cc, mode, path, o], check=True, stdout=subprocess.DEVNULL,
@@ -77,5 +84,6 @@ def ref_golden():
 
 
 if __name__ == "__main__":
-    nx_golden()
+    if sys.argv[1:] != ["ref"]:
+        nx_golden()
     ref_golden()

@@ -153,6 +153,8 @@ def test_oracle_matches_reference_build_outputs():
     by oracle/Makefile, written by tests/golden/make_golden.py)."""
     ref = json.load(open(os.path.join(GOLDEN, "ref_golden.json")))
     for name, e in ref.items():
+        if e.get("raw"):
+            continue  # multigraph fixtures: tests/test_raw_input.py
         g = load(name)
         src = g.sources()
         s = e["start"]
