@@ -347,6 +347,7 @@ def run_stream(srv, gids, reqs, rate, drain_s):
     gaps = rng.exponential(1.0 / rate, len(reqs))
     pending = queue.Queue()
     lat, last_end = [], [0.0]
+    service = {}  # (graph index, kind) -> [count, summed ms on a slot]
     cancelled = [0]
     deadline = [None]
 
@@ -355,7 +356,7 @@ def run_stream(srv, gids, reqs, rate, drain_s):
             item = pending.get()
             if item is None:
                 return
-            rid, kind = item
+            rid, kind, j = item
             while True:
                 dl = deadline[0]
                 left = 1000 if dl is None else int(max(0.0, dl - time.perf_counter()) * 1000)
@@ -371,6 +372,9 @@ def run_stream(srv, gids, reqs, rate, drain_s):
             if info["state"] == "done":
                 lat.append((kind, info["t_end_ms"] - info["t_submit_ms"]))
                 last_end[0] = max(last_end[0], time.perf_counter())
+                sv = service.setdefault((j, kind), [0, 0.0])
+                sv[0] += 1
+                sv[1] += info["t_end_ms"] - info["t_start_ms"]
 
     th = threading.Thread(target=collect, daemon=True)
     th.start()
@@ -381,7 +385,7 @@ def run_stream(srv, gids, reqs, rate, drain_s):
         d = t_next - time.perf_counter()
         if d > 0:
             time.sleep(d)
-        pending.put((srv.submit(gids[j], kind, src), kind))
+        pending.put((srv.submit(gids[j], kind, src), kind, j))
         if (i + 1) % 2000 == 0:
             log(f"serve stream: {i + 1}/{len(reqs)} submitted, {len(lat)} done "
                 f"({time.perf_counter() - t0:.1f} s)")
@@ -391,7 +395,7 @@ def run_stream(srv, gids, reqs, rate, drain_s):
         th.join(10.0)
         if th.is_alive():
             log(f"serve stream: draining, {len(lat)} done ({time.perf_counter() - t0:.1f} s)")
-    return lat, (last_end[0] or time.perf_counter()) - t0, cancelled[0]
+    return lat, (last_end[0] or time.perf_counter()) - t0, cancelled[0], service
 
 
 def _served_pass(srv, gids, reqs):
@@ -457,7 +461,7 @@ def serving(ctx, args, failures):
            "pool_n": [g.n for _, g in pool], "pool_m": [g.m for _, g in pool]}
     srv = bc_amd.Server(slots=4, hbm_budget=budget)
     gids = [srv.add_graph(g) for _, g in pool]
-    lat, wall, ncancel = run_stream(srv, gids, reqs, args.serve_rate, args.serve_drain)
+    lat, wall, ncancel, service = run_stream(srv, gids, reqs, args.serve_rate, args.serve_drain)
     st = srv.stats()
     srv.close()
     ms = np.array([x for _, x in lat]) if lat else np.zeros(1)
@@ -471,6 +475,11 @@ def serving(ctx, args, failures):
         if len(x):
             stream[f"{name}_p50_ms"] = round(float(np.percentile(x, 50)), 2)
             stream[f"{name}_p99_ms"] = round(float(np.percentile(x, 99)), 2)
+    # slot time per request by graph and kind: where the capacity goes
+    names = {bc_amd.CENT_Q_BC: "bc", bc_amd.CENT_Q_CC: "cc"}
+    stream["slot_ms_by_graph"] = {f"{j}:{names[kd]}": [c, round(t / c, 2)]
+                                  for (j, kd), (c, t) in sorted(service.items())}
+    stream["slot_s_total"] = round(sum(t for _, t in service.values()) / 1e3, 2)
     log(f"serve stream: {stream}")
     res["stream"] = stream
 
