@@ -337,6 +337,7 @@ _lib.cent_bc_device.argtypes = [C.c_void_p, C.c_void_p, _I32P, C.c_int32, C.c_vo
 _lib.cent_cc_device.argtypes = [C.c_void_p, C.c_void_p, _I32P, C.c_int32, C.c_void_p, C.c_void_p,
                                 C.POINTER(_Opts)]
 CENT_COPY_TO_DEVICE, CENT_COPY_TO_HOST, CENT_COPY_ON_DEVICE = 1, 2, 3
+_lib.cent_cc_device_noalloc.argtypes = [C.c_void_p, C.c_void_p, _I32P, C.c_int32, C.c_void_p, C.c_void_p]
 _lib.cent_device_alloc.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_void_p)]
 _lib.cent_device_free.argtypes = [C.c_void_p, C.c_void_p]
 _lib.cent_device_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32]
@@ -722,6 +723,15 @@ class Context:
                                    self._dev_arg(farness, np.int64, kk, "farness"),
                                    self._dev_arg(dist_sum, np.int64, g.n, "dist_sum"), C.byref(o)),
                "cent_cc_device", self.last_error())
+
+    def cc_device_noalloc(self, g: DeviceGraph, sources, farness=None, dist_sum=None) -> None:
+        """cent_cc_device_noalloc: cc_device with every device buffer of the
+        call in the library's fixed per-GPU scratch (nothing allocated)."""
+        s, k, kk = self._src(g, sources)
+        _check(_lib.cent_cc_device_noalloc(self._h, g._h, _ptr(s, C.c_int32) if s is not None and k else None, k,
+                                           self._dev_arg(farness, np.int64, kk, "farness"),
+                                           self._dev_arg(dist_sum, np.int64, g.n, "dist_sum")),
+               "cent_cc_device_noalloc", self.last_error())
 
     def cc_reduced(self, red: Reduced, sched: int = CENT_SCHED_PERSISTENT, timing: bool = False):
         """Farness of every vertex through the D1/AP-reduced form."""

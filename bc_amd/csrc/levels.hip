@@ -1010,6 +1010,117 @@ extern "C" int cent_cc(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
     return st;
 }
 
+// ------------------------------------------------------------ allocation-free closeness
+// cent_cc_device_noalloc: cent_cc_device on the source-parallel closeness
+// kernel (k_ps_cc) with every device buffer of the call — the source list,
+// the counters, the per-slot visited maps and BFS queues, the device-order
+// distance sums — carved out of one file-scope device array instead of the
+// context's workspace.  Nothing is allocated per call, on the device or the
+// host.  The array is per GPU (each device that runs the call holds its
+// own copy until the process ends), and every call on a GPU uses that GPU's
+// copy: calls that overlap on one GPU race.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kNoAllocMaxDevices = 64;
+constexpr int kNoAllocMaxSlots = 256;
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+// Device address of this GPU's scratch (the host-side name is only a
+// stand-in): looked up once per device, with that device current.
+static int noalloc_scratch(cent_ctx* ctx, unsigned char** out) {
+    if (ctx->device < 0 || ctx->device >= kNoAllocMaxDevices) return CENT_E_INVAL;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[ctx->device]) {
+        void* p = nullptr;
+        HIPTRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return CENT_E_DEVICE;
+        g_scratch_ptr[ctx->device] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch_ptr[ctx->device];
+    return CENT_OK;
+}
+
+// Zeroes n 64-bit words (a kernel rather than a memset node, so the call
+// stays free of runtime-internal work).
+__global__ void k_zero_words(unsigned long long* __restrict__ p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0ull;
+}
+
+// Caller ids to device ids in place (new_id: caller id -> device id).
+__global__ void k_remap_ids(int32_t* __restrict__ ids, int k, const int32_t* __restrict__ new_id) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < k; i += gridDim.x * blockDim.x) ids[i] = new_id[ids[i]];
+}
+
+static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// ids[i] = i (all-vertex calls: every vertex is a source, in caller order).
+__global__ void k_iota(int32_t* __restrict__ ids, int k) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < k; i += gridDim.x * blockDim.x) ids[i] = i;
+}
+
+extern "C" int cent_cc_device_noalloc(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k,
+                                      int64_t* d_farness, int64_t* d_dist_sum) {
+    if (!ctx || !g || g->ctx->device != ctx->device || k < -1 || (k > 0 && !src)) return CENT_E_INVAL;
+    const int32_t n = g->n;
+    const bool all = k == -1;  // every vertex, farness indexed by vertex
+    if (all) k = n;
+    for (int32_t i = 0; !all && i < k; ++i)
+        if (src[i] < 0 || src[i] >= n) return CENT_E_INVAL;
+    TRY(check_device(ctx));
+    TRY(check_dev_range(ctx, d_farness, sizeof(int64_t) * (size_t)k));
+    TRY(check_dev_range(ctx, d_dist_sum, sizeof(int64_t) * (size_t)n));
+    if (k == 0) return CENT_OK;
+    // layout: counters | sources | farness (when the caller passes none) |
+    // device-order distance sums (relabelled graphs) | per-slot queue + map
+    const size_t N = (size_t)(n > 0 ? n : 1);
+    const size_t nw = ((N + 31) / 32 + 3) / 4 * 4;
+    const bool ds_remap = d_dist_sum && g->d_new_id;
+    const size_t o_src = 256;
+    const size_t o_far = o_src + align256(sizeof(int32_t) * (size_t)k);
+    const size_t o_ds = o_far + (d_farness ? 0 : align256(sizeof(int64_t) * (size_t)k));
+    const size_t o_slots = o_ds + (ds_remap ? align256(sizeof(int64_t) * N) : 0);
+    const size_t per_slot = align256(sizeof(int32_t) * N) + align256(sizeof(uint32_t) * nw);
+    if (o_slots + per_slot > kNoAllocScratchBytes) {
+        snprintf(ctx->last_error, sizeof(ctx->last_error),
+                 "cent_cc_device_noalloc: %d vertices and %d sources need %zu bytes of scratch, the maximum is %zu",
+                 n, k, o_slots + per_slot, kNoAllocScratchBytes);
+        return CENT_E_INVAL;
+    }
+    const size_t G = std::min<size_t>({(size_t)k, (size_t)kNoAllocMaxSlots, (kNoAllocScratchBytes - o_slots) / per_slot});
+    unsigned char* base = nullptr;
+    TRY(noalloc_scratch(ctx, &base));
+    hipStream_t s = ctx->stream;
+    unsigned long long* ctr = reinterpret_cast<unsigned long long*>(base);  // [0..1] counts, [2] depth sum
+    int32_t* dsrc = reinterpret_cast<int32_t*>(base + o_src);
+    int64_t* far = d_farness ? d_farness : reinterpret_cast<int64_t*>(base + o_far);
+    int64_t* ds = ds_remap ? reinterpret_cast<int64_t*>(base + o_ds) : d_dist_sum;
+    int* Q = reinterpret_cast<int*>(base + o_slots);
+    uint32_t* bits = reinterpret_cast<uint32_t*>(base + o_slots + G * align256(sizeof(int32_t) * N));
+    k_zero_words<<<1, 64, 0, s>>>(ctr, 4);
+    if (ds_remap) k_zero_words<<<(int)std::min<size_t>((N + 255) / 256, 4096), 256, 0, s>>>(
+        reinterpret_cast<unsigned long long*>(ds), N);
+    if (all) k_iota<<<(int)std::min<int64_t>(((int64_t)k + 255) / 256, 4096), 256, 0, s>>>(dsrc, k);
+    else HIPTRY(hipMemcpyAsync(dsrc, src, sizeof(int32_t) * (size_t)k, hipMemcpyHostToDevice, s));
+    if (g->d_new_id) k_remap_ids<<<(int)std::min<int64_t>(((int64_t)k + 255) / 256, 4096), 256, 0, s>>>(dsrc, k, g->d_new_id);
+    HIPTRY(hipGetLastError());
+    PsArgs a{};
+    a.n = n; a.nw = (int)nw; a.rp = g->d.rp; a.col = g->d.col; a.src = dsrc; a.k = k;
+    a.Q = Q; a.bits = bits; a.far = far; a.dist_sum = ds; a.weight = nullptr;
+    a.cnt = ctr; a.depth_sum = ctr + 2; a.overflow = nullptr;  // closeness sets no flag
+    // slot g's queue is Q + g n (the kernel's stride) and its map bits + g nw
+    k_ps_cc<<<(int)G, kPsBlock, 0, s>>>(a);
+    HIPTRY(hipGetLastError());
+    if (ds_remap) {
+        const int rgn = (int)std::min<size_t>((N + 255) / 256, 8192);
+        k_out_accum<int64_t><<<rgn, 256, 0, s>>>(d_dist_sum, ds, g->d_new_id, n);
+        HIPTRY(hipGetLastError());
+    }
+    HIPTRY(hipStreamSynchronize(s));
+    return CENT_OK;
+}
+
 // ------------------------------------------------------------ autotuner
 // Coordinate descent over the plain form's launch settings: level step
 // first; the source-parallel step then only sweeps its slot count

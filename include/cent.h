@@ -255,6 +255,22 @@ int  cent_bc_device(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int3
 int  cent_cc_device(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k,
                     int64_t* d_farness, int64_t* d_dist_sum, const cent_opts* opts);
 
+/* Allocation-free closeness (no reference counterpart).  For callers that
+ * cannot allocate device or pinned memory during the call — code under HIP
+ * graph capture, tight serving loops, harnesses that forbid allocation.
+ * Same arguments, meaning and output rules as cent_cc_device (k == -1:
+ * every vertex), run on the
+ * source-parallel closeness step; all its device buffers live in one fixed
+ * device array of 64 MiB per GPU (held from the first call until the
+ * process ends), so a call allocates nothing.  A graph whose n vertices and
+ * k sources do not fit (about 4.1 n bytes per traversal slot, 12 n with
+ * distance sums on a relabelled graph, plus 12 k) returns CENT_E_INVAL with
+ * the sizes in cent_ctx_last_error.  All calls on one GPU share its array:
+ * calls that overlap on one GPU race, so serialise them.  The context's
+ * statistics are not updated.  Returns once the outputs are written. */
+int  cent_cc_device_noalloc(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k,
+                            int64_t* d_farness, int64_t* d_dist_sum);
+
 /* Device arrays on the context's GPU, for callers with no HIP allocator of
  * their own (FFI and Python callers; no reference counterpart).  Copies and
  * fills run on the context's stream and return when done. */
