@@ -343,14 +343,15 @@ fail:
  * Blocks are stored children-first, so one forward pass gives down(), one
  * backward pass gives K.
  */
-int cent_reduced_finish(const struct cent_reduced* r, const int64_t* D, int64_t* farness) {
-    if (!r || (!D && r->np > 0) || !farness) return CENT_E_INVAL;
+/* Block constants K_B from the packed distance sums D, which need to be
+ * valid only at each non-root block's head copy and at the head vertex's
+ * copy in the parent block. */
+static int reduced_block_constants(const struct cent_reduced* r, const int64_t* D, int64_t* K) {
     const int32_t n = r->n, nb = r->nblocks;
     const size_t N = (size_t)(n > 0 ? n : 1);
     int64_t* down = (int64_t*)calloc((size_t)nb + 1, sizeof(int64_t));
-    int64_t* K = (int64_t*)calloc((size_t)nb + 1, sizeof(int64_t));
     int64_t* hang_ff = (int64_t*)malloc(N * sizeof(int64_t)); /* ff[v] + sum down(C) of child blocks at v */
-    if (!down || !K || !hang_ff) { free(down); free(K); free(hang_ff); return CENT_E_OOM; }
+    if (!down || !hang_ff) { free(down); free(hang_ff); return CENT_E_OOM; }
     for (int32_t v = 0; v < n; ++v) hang_ff[v] = r->d1.ff[v];
     /* bottom-up */
     for (int32_t b = 0; b < nb; ++b) {
@@ -374,11 +375,22 @@ int cent_reduced_finish(const struct cent_reduced* r, const int64_t* D, int64_t*
             /* farness of the head vertex from its copy in the parent block */
             const int32_t P = r->bparent[b];
             const int32_t xp = r->rep[r->porig[h]];  /* the head vertex's copy in P */
-            if (xp < 0 || r->pblock[xp] != P) { free(down); free(K); free(hang_ff); return CENT_E_INVAL; }
+            if (xp < 0 || r->pblock[xp] != P) { free(down); free(hang_ff); return CENT_E_INVAL; }
             const int64_t far_h = D[xp] + K[P];
             K[b] = (far_h - down[b]) + s;
         }
     }
+    free(down); free(hang_ff);
+    return CENT_OK;
+}
+
+int cent_reduced_finish(const struct cent_reduced* r, const int64_t* D, int64_t* farness) {
+    if (!r || (!D && r->np > 0) || !farness) return CENT_E_INVAL;
+    const int32_t n = r->n;
+    int64_t* K = (int64_t*)calloc((size_t)r->nblocks + 1, sizeof(int64_t));
+    if (!K) return CENT_E_OOM;
+    const int st = reduced_block_constants(r, D, K);
+    if (st) { free(K); return st; }
     /* unfolded vertices */
     for (int32_t v = 0; v < n; ++v) {
         if (r->d1.parent[v] >= 0) continue;
@@ -391,7 +403,84 @@ int cent_reduced_finish(const struct cent_reduced* r, const int64_t* D, int64_t*
         const int32_t p = r->d1.parent[d];
         farness[d] = farness[p] + r->comp_w[r->vcomp[d]] - 2 * r->d1.w[d];
     }
-    free(down); free(K); free(hang_ff);
+    free(K);
+    return CENT_OK;
+}
+
+/* farness of the given source vertices only (assigned, one per source).  D
+ * needs to be valid at every copy reduced_block_constants reads and at the
+ * copy answering for each source's unfolded ancestor (a folded vertex is
+ * recovered from its parent: farness(d) = farness(p) + N - 2 w[d]). */
+int cent_reduced_farness_sources(const cent_reduced* r, const int64_t* D, const int32_t* src, int32_t k,
+                                 int64_t* far) {
+    if (!r || k < 0 || (k > 0 && (!src || !far)) || (!D && r->np > 0)) return CENT_E_INVAL;
+    for (int32_t i = 0; i < k; ++i)
+        if (src[i] < 0 || src[i] >= r->n) return CENT_E_INVAL;
+    int64_t* K = (int64_t*)calloc((size_t)r->nblocks + 1, sizeof(int64_t));
+    if (!K) return CENT_E_OOM;
+    const int st = reduced_block_constants(r, D, K);
+    if (st) { free(K); return st; }
+    for (int32_t i = 0; i < k; ++i) {
+        int32_t v = src[i];
+        int64_t add = 0;
+        while (r->d1.parent[v] >= 0) {  /* up the fold tree */
+            add += r->comp_w[r->vcomp[v]] - 2 * r->d1.w[v];
+            v = r->d1.parent[v];
+        }
+        const int32_t x = r->rep[v];
+        far[i] = add + ((x >= 0) ? D[x] + K[r->pblock[x]] : r->d1.ff[v]);
+    }
+    free(K);
+    return CENT_OK;
+}
+
+/* Source-count weights of a source multiset S (sampled betweenness through
+ * the reduction): the counterpart of the vertex weights W with every vertex
+ * counted by its multiplicity in S instead of 1.  wsf[v] (optional, [n]) =
+ * sources among v and the vertices folded into it; WS[x] ([np]) = sources
+ * hanging off copy x on its side of x's block (all of the component's
+ * sources minus those below the block for a block's head copy); Scomp
+ * (optional, [ncomp]) = sources per component.  With S = every vertex once,
+ * WS equals the copy weights pw. */
+static int source_weights(const cent_reduced* r, const int32_t* src, int32_t k, int64_t* WS, int64_t* wsf_out,
+                          int64_t* Scomp_out) {
+    if (!r || k < 0 || (k > 0 && !src) || (!WS && r->np > 0)) return CENT_E_INVAL;
+    const int32_t n = r->n, nb = r->nblocks;
+    const size_t N = (size_t)(n > 0 ? n : 1);
+    int64_t* ws = (int64_t*)calloc(N, sizeof(int64_t));
+    int64_t* down = (int64_t*)calloc((size_t)nb + 1, sizeof(int64_t));
+    int64_t* Sc = (int64_t*)calloc((size_t)r->ncomp + 1, sizeof(int64_t));
+    if (!ws || !down || !Sc) { free(ws); free(down); free(Sc); return CENT_E_OOM; }
+    for (int32_t i = 0; i < k; ++i) {
+        if (src[i] < 0 || src[i] >= n) { free(ws); free(down); free(Sc); return CENT_E_INVAL; }
+        ws[src[i]]++;
+        Sc[r->vcomp[src[i]]]++;
+    }
+    /* fold subtrees (children fold before their parents) */
+    for (int32_t i = 0; i < r->d1.nfold; ++i) {
+        const int32_t d = r->d1.order[i];
+        ws[r->d1.parent[d]] += ws[d];
+    }
+    if (wsf_out) memcpy(wsf_out, ws, N * sizeof(int64_t));
+    /* bottom-up over the block-cut tree: ws[v] gathers the child blocks headed at v */
+    for (int32_t b = 0; b < nb; ++b) {
+        if (r->bparent[b] < 0) continue;
+        const int32_t h = r->bhead[b];
+        int64_t t = 0;
+        for (int32_t i = r->boff[b]; i < r->boff[b + 1]; ++i)
+            if (i != h) t += ws[r->porig[i]];
+        down[b] = t;
+        ws[r->porig[h]] += t;
+    }
+    for (int32_t b = 0; b < nb; ++b) {
+        const int32_t h = r->bparent[b] >= 0 ? r->bhead[b] : -1;
+        for (int32_t i = r->boff[b]; i < r->boff[b + 1]; ++i) {
+            const int32_t v = r->porig[i];
+            WS[i] = (i == h) ? Sc[r->vcomp[v]] - down[b] : ws[v];
+        }
+    }
+    if (Scomp_out) memcpy(Scomp_out, Sc, (size_t)(r->ncomp > 0 ? r->ncomp : 1) * sizeof(int64_t));
+    free(ws); free(down); free(Sc);
     return CENT_OK;
 }
 
@@ -475,6 +564,52 @@ int cent_reduced_finish_bc(const struct cent_reduced* r, const double* bcp, doub
     free(sumsq);
     free(inblock);
     return CENT_OK;
+}
+
+int cent_reduced_source_weights(const cent_reduced* r, const int32_t* src, int32_t k, int64_t* WS) {
+    return source_weights(r, src, k, WS, NULL, NULL);
+}
+
+/* Betweenness from a source multiset S through the reduction (sampled
+ * sources; the all-source case is cent_reduced_finish_bc).  Counting the
+ * ordered pairs (s, t), s in S, t any other vertex, whose shortest paths run
+ * through v:
+ *   BC_S(v) = sum over the pieces P of v of S(P) (N - 1 - |P|)
+ *           + sum over the blocks B holding v of the in-block term
+ *             sum_{a != b, both != v} WS(a) W(b) sigma_ab(v) / sigma_ab,
+ * the first part being the pairs whose endpoints lie in different pieces
+ * (S(P) = sources in P: wsf of a fold child's subtree, S_comp - wsf[c] for a
+ * folded vertex's rest, S_comp - WS(x) behind the block of copy x), the
+ * second what the GPU accumulates at v's copies (weighted Brandes from
+ * every copy with WS > 0, source weight WS, target weight W).  With every
+ * vertex once in S this is cent_reduced_finish_bc's formula. */
+int cent_reduced_bc_sources(const cent_reduced* r, const int32_t* src, int32_t k, const double* bcp, double* bc) {
+    if (!r || (!bcp && r->np > 0) || !bc || k < 0 || (k > 0 && !src)) return CENT_E_INVAL;
+    if (r->multi) return CENT_E_UNSUPPORTED;
+    const int32_t n = r->n;
+    const size_t N = (size_t)(n > 0 ? n : 1);
+    int64_t* WS = (int64_t*)malloc((size_t)(r->np > 0 ? r->np : 1) * sizeof(int64_t));
+    int64_t* wsf = (int64_t*)malloc(N * sizeof(int64_t));
+    int64_t* Sc = (int64_t*)malloc((size_t)(r->ncomp > 0 ? r->ncomp : 1) * sizeof(int64_t));
+    double* acc = (double*)calloc(N, sizeof(double));
+    if (!WS || !wsf || !Sc || !acc) { free(WS); free(wsf); free(Sc); free(acc); return CENT_E_OOM; }
+    int st = source_weights(r, src, k, WS, wsf, Sc);
+    if (!st) {
+        for (int32_t i = 0; i < r->d1.nfold; ++i) {
+            const int32_t c = r->d1.order[i];
+            const int32_t cc = r->vcomp[c];
+            const double Nc = (double)r->comp_w[cc], wc = (double)r->d1.w[c];
+            acc[r->d1.parent[c]] += (double)wsf[c] * (Nc - 1.0 - wc);      /* c's subtree, a piece of its parent */
+            acc[c] += (double)(Sc[cc] - wsf[c]) * (wc - 1.0);             /* the rest, a piece of c */
+        }
+        for (int32_t x = 0; x < r->np; ++x) {
+            const int32_t v = r->porig[x];
+            acc[v] += (double)(Sc[r->vcomp[v]] - WS[x]) * ((double)r->pw[x] - 1.0) + bcp[x];
+        }
+        for (int32_t v = 0; v < n; ++v) bc[v] += acc[v];
+    }
+    free(WS); free(wsf); free(Sc); free(acc);
+    return st;
 }
 
 int cent_reduced_bc(const cent_reduced* r, const double* bcp, double* bc) {

@@ -225,6 +225,9 @@ _lib.cent_reduced_get_view.argtypes = [C.c_void_p, C.POINTER(_RView)]
 _lib.cent_reduced_farness.argtypes = [C.c_void_p, _I64P, _I64P]
 _lib.cent_reduced_bc.argtypes = [C.c_void_p, _F64P, _F64P]
 _lib.cent_reduced_free.argtypes = [C.c_void_p]
+_lib.cent_reduced_source_weights.argtypes = [C.c_void_p, _I32P, C.c_int32, _I64P]
+_lib.cent_reduced_bc_sources.argtypes = [C.c_void_p, _I32P, C.c_int32, _F64P, _F64P]
+_lib.cent_reduced_farness_sources.argtypes = [C.c_void_p, _I64P, _I32P, C.c_int32, _I64P]
 _lib.cent_reduced_free.restype = None
 CENT_SCHED_PERSISTENT, CENT_SCHED_PER_ITEM, CENT_SCHED_SHARED = 0, 1, 2  # see include/cent.h
 
@@ -270,6 +273,34 @@ class Reduced:
         _check(_lib.cent_reduced_farness(self._h, _ptr(D, C.c_int64) if self.np else None,
                                          _ptr(out, C.c_int64)), "cent_reduced_farness")
         return out[:self.n]
+
+    def source_weights(self, sources) -> np.ndarray:
+        """WS[x]: sources (with repeats) hanging off packed copy x (pw for all vertices once)."""
+        src = np.ascontiguousarray(sources, dtype=np.int32)
+        out = np.zeros(max(self.np, 1), np.int64)
+        _check(_lib.cent_reduced_source_weights(self._h, _ptr(src, C.c_int32) if len(src) else None, len(src),
+                                                _ptr(out, C.c_int64)), "cent_reduced_source_weights")
+        return out[:self.np]
+
+    def bc_from_sources(self, sources, bcp) -> np.ndarray:
+        """betweenness from the source multiset, given the packed dependencies bcp
+        (weighted Brandes from every copy with source weights WS)."""
+        src = np.ascontiguousarray(sources, dtype=np.int32)
+        bcp = np.ascontiguousarray(bcp, dtype=np.float64)
+        out = np.zeros(max(self.n, 1))
+        _check(_lib.cent_reduced_bc_sources(self._h, _ptr(src, C.c_int32) if len(src) else None, len(src),
+                                            _ptr(bcp, C.c_double) if self.np else None, _ptr(out, C.c_double)),
+               "cent_reduced_bc_sources")
+        return out[:self.n]
+
+    def farness_of_sources(self, sources, D) -> np.ndarray:
+        src = np.ascontiguousarray(sources, dtype=np.int32)
+        D = np.ascontiguousarray(D, dtype=np.int64)
+        out = np.zeros(max(len(src), 1), np.int64)
+        _check(_lib.cent_reduced_farness_sources(self._h, _ptr(D, C.c_int64) if self.np else None,
+                                                 _ptr(src, C.c_int32) if len(src) else None, len(src),
+                                                 _ptr(out, C.c_int64)), "cent_reduced_farness_sources")
+        return out[:len(src)]
 
     def bc_from(self, bcp) -> np.ndarray:
         """betweenness of every input vertex from the packed weighted dependencies bcp."""
@@ -348,6 +379,8 @@ _lib.cent_ctx_trim.argtypes = [C.c_void_p]
 _lib.cent_probe_copy_bandwidth.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.POINTER(C.c_double)]
 _lib.cent_cc_reduced.argtypes = [C.c_void_p, C.c_void_p, _I64P, C.c_int32, C.POINTER(_Opts)]
 _lib.cent_bc_reduced.argtypes = [C.c_void_p, C.c_void_p, _F64P, _I64P, C.c_int32, C.POINTER(_Opts)]
+_lib.cent_bc_reduced_sources.argtypes = [C.c_void_p, C.c_void_p, _I32P, C.c_int32, _F64P, _I64P, C.c_int32,
+                                         C.POINTER(_Opts)]
 _lib.compute_D1_AP_CC_shareBased_DegreeOrder.argtypes = [_P_CSR, _I64P]
 _lib.computeBC.argtypes = [_P_CSR, _F64P]
 _lib.computeBC_compat.argtypes = [_P_CSR, _F64P]
@@ -741,6 +774,18 @@ class Context:
                "cent_cc_reduced", self.last_error())
         return out[:red.n]
 
+    def bc_reduced_sources(self, red: Reduced, sources, sched: int = CENT_SCHED_PERSISTENT, timing: bool = False):
+        """(bc, farness of each source): betweenness from the source multiset
+        through the reduced form (cent_bc_reduced_sources)."""
+        src = np.ascontiguousarray(sources, dtype=np.int32)
+        bc = np.zeros(max(red.n, 1))
+        far = np.zeros(max(len(src), 1), np.int64)
+        o = _opts(0, timing)
+        _check(_lib.cent_bc_reduced_sources(self._h, red._h, _ptr(src, C.c_int32) if len(src) else None, len(src),
+                                            _ptr(bc, C.c_double), _ptr(far, C.c_int64), sched, C.byref(o)),
+               "cent_bc_reduced_sources", self.last_error())
+        return bc[:red.n], far[:len(src)]
+
     def bc_reduced(self, red: Reduced, sched: int = CENT_SCHED_PERSISTENT, timing: bool = False):
         """(bc, farness) of every vertex, all sources, through the reduced form."""
         bc = np.zeros(max(red.n, 1))
@@ -758,6 +803,8 @@ def debug_fail_upload(nth: int) -> None:
     """Test hook: the nth device upload of a graph structure from now on fails
     with CENT_E_OOM (0 switches the injection off)."""
     _check(_lib.cent_debug_fail_upload(nth), "cent_debug_fail_upload")
+
+
 
 
 # ---------------------------------------------------------------- serving

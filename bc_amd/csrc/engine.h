@@ -66,6 +66,11 @@ struct cent_ctx {
     DevBuf<double> red_part, red_bcp, it_sig, it_coef;
     DevBuf<uint8_t> it_lvl;
     DevBuf<int> red_deep;
+    // sampled sources through the reduction: listed copies, their source
+    // weights, and the call's items (range, list offset, count)
+    DevBuf<int> red_src, red_order;
+    DevBuf<int64_t> red_swgt, red_poff;
+    DevBuf<int4> red_items;
     // device scratch of the graph reduction (cent_reduce_gpu), grown to the
     // largest request and reused by later calls
     DevBuf<char> red_arena;
@@ -316,6 +321,7 @@ struct BatchSpec {
     int64_t* dist_sum = nullptr;      // device [n] or null
     double* bc = nullptr;             // device [n] (with_bc)
     const int64_t* bwgt = nullptr;    // weighted dependencies (reduced graphs), or null
+    const int64_t* swgt = nullptr;    // source weights if not bwgt (sampled sources), or null
     const cent_opts* opts = nullptr;
 };
 

@@ -150,6 +150,8 @@ extern "C" void cent_ctx_destroy(cent_ctx* ctx) {
     release(ctx->it_lvl); release(ctx->red_deep);
     release(ctx->ps_rec); release(ctx->ps_bits); release(ctx->ps_Q); release(ctx->ps_loff); release(ctx->ps_bcp);
     release(ctx->ps_depth); release(ctx->lvf); release(ctx->lvstat); release(ctx->red_arena);
+    release(ctx->lexp); release(ctx->red_src); release(ctx->red_order); release(ctx->red_swgt);
+    release(ctx->red_poff); release(ctx->red_items);
     if (ctx->h_flags) (void)hipHostFree(ctx->h_flags);
     if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
     for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
@@ -175,11 +177,13 @@ extern "C" int64_t cent_ctx_workspace_bytes(const cent_ctx* ctx) {
                           ctx->bc.cap + ctx->hp_acc.cap + ctx->dist_sum.cap + ctx->far.cap + ctx->part_far.cap +
                           ctx->part_cnt.cap + ctx->cnt.cap + ctx->Fd2.cap + ctx->QF.cap + ctx->Adev.cap +
                           ctx->red_D.cap + ctx->red_part.cap + ctx->red_bcp.cap + ctx->it_sig.cap +
-                          ctx->it_coef.cap + 2 * ctx->ps_rec.cap + ctx->ps_bcp.cap + ctx->ps_depth.cap) +
+                          ctx->it_coef.cap + 2 * ctx->ps_rec.cap + ctx->ps_bcp.cap + ctx->ps_depth.cap +
+                          ctx->red_swgt.cap + ctx->red_poff.cap + 2 * ctx->red_items.cap) +
                      ctx->it_lvl.cap + ctx->red_arena.cap +
                      4 * (ctx->npred.cap + ctx->hp_cnt.cap + ctx->src.cap + ctx->flags.cap + ctx->ord.cap + ctx->Q.cap +
                           ctx->qoff.cap + ctx->cand.cap + ctx->stamp.cap + ctx->ctl.cap + ctx->red_head.cap +
-                          ctx->red_deep.cap + ctx->ps_Q.cap + ctx->ps_loff.cap + ctx->ps_bits.cap));
+                          ctx->red_deep.cap + ctx->ps_Q.cap + ctx->ps_loff.cap + ctx->ps_bits.cap +
+                          ctx->lexp.cap + ctx->red_src.cap + ctx->red_order.cap));
 }
 
 extern "C" int cent_ctx_trim(cent_ctx* ctx) {
@@ -198,6 +202,8 @@ extern "C" int cent_ctx_trim(cent_ctx* ctx) {
     release(ctx->it_lvl); release(ctx->red_deep);
     release(ctx->ps_rec); release(ctx->ps_bits); release(ctx->ps_Q); release(ctx->ps_loff); release(ctx->ps_bcp);
     release(ctx->ps_depth); release(ctx->lvf); release(ctx->lvstat); release(ctx->red_arena);
+    release(ctx->lexp); release(ctx->red_src); release(ctx->red_order); release(ctx->red_swgt);
+    release(ctx->red_poff); release(ctx->red_items);
     if (ctx->h_stage) HIPTRY(hipHostFree(ctx->h_stage));
     ctx->h_stage = nullptr;
     ctx->h_stage_bytes = 0;

@@ -144,6 +144,11 @@ struct BcItemArgs {
     int* __restrict__ deep;               // [nitems] item indices
     int* __restrict__ ndeep;
     int* __restrict__ overflow;           // set when a path count is inf
+    // sampled sources: item sources are srcs[s0 .. s0 + ns) (packed ids)
+    // instead of s0 .. s0 + ns - 1, each with source weight swgt[packed id]
+    // (null: weight); D and the partials stay indexed by list position
+    const int* __restrict__ srcs;
+    const int64_t* __restrict__ swgt;
 };
 
 constexpr int kItemMaxDepth = 254;
@@ -176,10 +181,13 @@ __global__ __launch_bounds__(kBcItemBlock) void k_bc_items(BcItemArgs a) {
         for (int i = tid; i < R * (kWave / 8); i += kBcItemBlock) reinterpret_cast<uint64_t*>(lvl)[i] = ~0ull;
         if (tid == 0) s_deep = 0;
         __syncthreads();
+        // this lane's source (packed id) and its weight as a source
+        const int my_src = lane < ns ? (a.srcs ? a.srcs[s0 + lane] : s0 + lane) : lo;
+        const double my_sw = lane < ns ? (double)(a.swgt ? a.swgt[my_src] : a.weight[my_src]) : 0.0;
         if (tid < ns) {
-            const int i = s0 - lo + tid;
-            s_F[0][i] = 1ull << tid;
-            s_vis[i] = 1ull << tid;
+            const int i = my_src - lo;
+            atomicOr((unsigned long long*)&s_F[0][i], 1ull << tid);  // repeated sources share a vertex
+            atomicOr((unsigned long long*)&s_vis[i], 1ull << tid);
             sig[(size_t)i * kWave + tid] = 1.0;
             lvl[(size_t)i * kWave + tid] = 0;
         }
@@ -308,7 +316,7 @@ __global__ __launch_bounds__(kBcItemBlock) void k_bc_items(BcItemArgs a) {
                         const double sg = sig[idx];
                         const double dl = sg * acc;
                         coef[idx] = ((double)a.weight[lo + vi] + dl) / sg;
-                        contrib = dl * (double)a.weight[s0 + lane];
+                        contrib = dl * my_sw;
                     }
                     const double tot = wave_sum(contrib);
                     if (lane == 0) part[vi] += tot;

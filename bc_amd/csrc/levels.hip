@@ -335,6 +335,7 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
             ba.npred = compat ? ctx->npred.p : nullptr;
             ba.bc = sp.bc;
             ba.wgt = sp.bwgt;
+            ba.swgt = sp.swgt;
             ba.bsrc = d_src;
             ba.lexp = ctx->lexp.p;
             ba.d = lv;

@@ -119,6 +119,9 @@ struct BwdArgs {
     // source's weight wgt[bsrc[s]].  Null: unweighted.
     const int64_t* __restrict__ wgt;
     const int* __restrict__ bsrc;
+    // source weights when they differ from the target weights (sampled
+    // sources through the reduction), indexed like wgt; null: wgt
+    const int64_t* __restrict__ swgt;
     // Levels enqueued ahead of the forward's end: a launch for an empty
     // level (*live == 0) returns at once.  Null: always run.
     const int* __restrict__ live;
@@ -339,7 +342,7 @@ __device__ __forceinline__ void bwd_vertex_done(const BwdArgs& a, int v, uint64_
         const double dl = (sg * acc) * sc;  // level d's scale, as in the forward
         if (a.wgt) {
             a.coef[idx] = ((double)a.wgt[v] + dl) / sg;
-            contrib = dl * (double)a.wgt[a.bsrc[lane]];
+            contrib = dl * (double)(a.swgt ? a.swgt : a.wgt)[a.bsrc[lane]];
         } else {
             a.coef[idx] = (1.0 + dl) / sg;
             contrib = compat ? dl * (double)a.npred[idx] : dl;

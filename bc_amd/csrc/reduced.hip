@@ -417,7 +417,7 @@ extern "C" int cent_bc_reduced(cent_ctx* ctx, const cent_reduced* rc, double* bc
         TRY(ensure(ctx, ctx->flags, 2));
         HIPTRY(hipMemsetAsync(ctx->red_head.p, 0, sizeof(int) * 2, s));
         HIPTRY(hipMemsetAsync(ctx->flags.p, 0, sizeof(int) * 2, s));
-        BcItemArgs ba;
+        BcItemArgs ba{};
         ba.rp = rd->d.rp; ba.col = rd->d.col; ba.weight = rd->w; ba.items = rd->items; ba.order = rd->order;
         ba.poff = rd->poff; ba.nitems = rd->nsmall; ba.head = ctx->red_head.p; ba.D = Dd;
         ba.bcpart = ctx->red_part.p; ba.sig = ctx->it_sig.p; ba.coef = ctx->it_coef.p; ba.lvl = ctx->it_lvl.p;
@@ -471,6 +471,183 @@ extern "C" int cent_bc_reduced(cent_ctx* ctx, const cent_reduced* rc, double* bc
     ctx->stats.levels += levels;
     ctx->stats.reached += (int64_t)r->n;
     ctx->stats.edges_reached += 2 * r->m;
+    ctx->stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return CENT_OK;
+}
+
+
+// Betweenness from a sampled source multiset through the reduction: weighted
+// Brandes inside each block from the copies with sources behind them only
+// (source weight WS, target weight W), then the host adds the piece terms
+// (cent_reduced_bc_sources).  On a graph of many blocks most blocks need one
+// or two traversals instead of one per source.
+extern "C" int cent_bc_reduced_sources(cent_ctx* ctx, const cent_reduced* rc, const int32_t* src, int32_t k,
+                                       double* bc, int64_t* farness, int32_t sched, const cent_opts* opts) {
+    if (!ctx || !rc || !bc || k < 0 || (k > 0 && !src)) return CENT_E_INVAL;
+    if (sched != CENT_SCHED_PERSISTENT && sched != CENT_SCHED_PER_ITEM) return CENT_E_INVAL;
+    for (int32_t i = 0; i < k; ++i)
+        if (src[i] < 0 || src[i] >= rc->n) return CENT_E_INVAL;
+    if (rc->multi) return CENT_E_UNSUPPORTED;  // parallel edges multiply path counts: use cent_bc
+    cent_reduced* r = const_cast<cent_reduced*>(rc);
+    TRY(check_device(ctx));
+    auto t0 = std::chrono::steady_clock::now();
+    RedDev* rd;
+    TRY(reddev_get(ctx, r, &rd));
+    hipStream_t s = ctx->stream;
+    const size_t NP = (size_t)(r->np > 0 ? r->np : 1);
+    // traversals: copies with sources behind them, plus (for farness) each
+    // non-root block's head copy and the head vertex's copy in the parent
+    std::vector<int64_t> WS(NP, 0);
+    TRY(cent_reduced_source_weights(r, src, k, WS.data()));
+    std::vector<char> need(NP, 0);
+    for (int32_t x = 0; x < r->np; ++x) need[(size_t)x] = WS[(size_t)x] > 0;
+    if (farness)
+        for (int32_t b = 0; b < r->nblocks; ++b)
+            if (r->bparent[b] >= 0) {
+                need[(size_t)r->bhead[b]] = 1;
+                need[(size_t)r->rep[r->porig[r->bhead[b]]]] = 1;
+            }
+    // items over contiguous block ranges, at most 64 listed copies each;
+    // blocks beyond the LDS window go through the level kernels
+    std::vector<int> list;
+    std::vector<int4> items;       // (lo, hi, offset into list, count)
+    std::vector<int4> big;
+    for (int32_t b = 0; b < r->nblocks; ++b) {
+        const int lo = r->boff[b], hi = r->boff[b + 1];
+        const int first = (int)list.size();
+        for (int x = lo; x < hi; ++x)
+            if (need[(size_t)x]) list.push_back(x);
+        const int cnt = (int)list.size() - first;
+        if (!cnt) continue;
+        for (int o = 0; o < cnt; o += kWave) {
+            const int ns = std::min(kWave, cnt - o);
+            if (hi - lo > kItemLds) {
+                big.push_back(make_int4(lo, hi, first + o, ns));
+                continue;
+            }
+            if (o == 0 && !items.empty()) {  // join the previous item when adjacent and small enough
+                int4& p = items.back();
+                if (p.y == lo && p.z + p.w == first && p.w + ns <= kWave && hi - p.x <= kItemLds) {
+                    p.y = hi;
+                    p.w += ns;
+                    continue;
+                }
+            }
+            items.push_back(make_int4(lo, hi, first + o, ns));
+        }
+    }
+    const size_t L = list.size() > 0 ? list.size() : 1;
+    std::vector<int64_t> poff(items.size() + 1, 0);
+    for (size_t i = 0; i < items.size(); ++i) poff[i + 1] = poff[i] + (items[i].y - items[i].x);
+    std::vector<int> order(items.size());
+    for (size_t i = 0; i < items.size(); ++i) order[i] = (int)i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return (items[(size_t)a].y - items[(size_t)a].x) > (items[(size_t)b].y - items[(size_t)b].x);
+    });
+    TRY(ensure(ctx, ctx->red_D, L));
+    TRY(ensure(ctx, ctx->red_bcp, NP));
+    TRY(ensure(ctx, ctx->red_head, 2));
+    TRY(ensure(ctx, ctx->red_src, L));
+    TRY(ensure(ctx, ctx->red_swgt, NP));
+    TRY(ensure(ctx, ctx->red_items, items.size() + 1));
+    TRY(ensure(ctx, ctx->red_order, items.size() + 1));
+    TRY(ensure(ctx, ctx->red_poff, items.size() + 1));
+    TRY(ensure(ctx, ctx->red_deep, items.size() + 1));
+    TRY(ensure(ctx, ctx->cnt, 2));
+    TRY(ensure(ctx, ctx->flags, 2));
+    int64_t* Dd = ctx->red_D.p;
+    HIPTRY(hipMemsetAsync(Dd, 0, sizeof(int64_t) * L, s));
+    HIPTRY(hipMemsetAsync(ctx->red_bcp.p, 0, sizeof(double) * NP, s));
+    HIPTRY(hipMemsetAsync(ctx->cnt.p, 0, sizeof(unsigned long long) * 2, s));
+    HIPTRY(hipMemsetAsync(ctx->red_head.p, 0, sizeof(int) * 2, s));
+    HIPTRY(hipMemsetAsync(ctx->flags.p, 0, sizeof(int) * 2, s));
+    if (!list.empty()) HIPTRY(hipMemcpyAsync(ctx->red_src.p, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice, s));
+    HIPTRY(hipMemcpyAsync(ctx->red_swgt.p, WS.data(), sizeof(int64_t) * NP, hipMemcpyHostToDevice, s));
+    if (!items.empty()) {
+        HIPTRY(hipMemcpyAsync(ctx->red_items.p, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice, s));
+        HIPTRY(hipMemcpyAsync(ctx->red_order.p, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice, s));
+        HIPTRY(hipMemcpyAsync(ctx->red_poff.p, poff.data(), sizeof(int64_t) * items.size(), hipMemcpyHostToDevice, s));
+    }
+    TRY(level_flags_reset(ctx));
+    KernelTimer tm;
+    tm.on = opts && (opts->flags & CENT_OPT_TIMING);
+    ctx->ev_used = 0;
+    BatchSpec sp;
+    sp.with_bc = true;
+    sp.weight = rd->w;
+    sp.bwgt = rd->w;
+    sp.swgt = ctx->red_swgt.p;
+    sp.bc = ctx->red_bcp.p;
+    sp.opts = opts;
+    int64_t levels = 0;
+    auto run_range = [&](const int4& it) -> int {
+        const GView v = sub_view(rd->d, it.x, it.y);
+        return level_batch(ctx, v, sp, ctx->red_src.p + it.z, it.w, Dd + it.z, tm, &levels);
+    };
+    std::vector<int> deep;
+    const bool persistent = sched == CENT_SCHED_PERSISTENT && !items.empty();
+    if (persistent) {
+        const int grid = std::min((int)items.size(), kBcItemGrid);
+        const size_t slot = (size_t)kItemLds * kWave;
+        TRY(ensure(ctx, ctx->it_sig, slot * grid));
+        TRY(ensure(ctx, ctx->it_coef, slot * grid));
+        TRY(ensure(ctx, ctx->it_lvl, slot * grid));
+        TRY(ensure(ctx, ctx->red_part, (size_t)(poff.back() > 0 ? poff.back() : 1)));
+        BcItemArgs ba{};
+        ba.rp = rd->d.rp; ba.col = rd->d.col; ba.weight = rd->w; ba.items = ctx->red_items.p;
+        ba.order = ctx->red_order.p; ba.poff = ctx->red_poff.p; ba.nitems = (int)items.size();
+        ba.head = ctx->red_head.p; ba.D = Dd; ba.bcpart = ctx->red_part.p; ba.sig = ctx->it_sig.p;
+        ba.coef = ctx->it_coef.p; ba.lvl = ctx->it_lvl.p; ba.deep = ctx->red_deep.p; ba.ndeep = ctx->red_head.p + 1;
+        ba.overflow = ctx->flags.p + 1; ba.srcs = ctx->red_src.p; ba.swgt = ctx->red_swgt.p;
+        hipEvent_t e0;
+        TRY(timer_begin(ctx, tm, &e0));
+        k_bc_items<<<grid, kBcItemBlock, 0, s>>>(ba);
+        HIPTRY(hipGetLastError());
+        TRY(timer_end(ctx, tm, e0, true));
+        ctx->stats.launches_fwd++;
+        ctx->stats.batches += (int64_t)items.size();
+        int hh[2];
+        HIPTRY(hipMemcpyAsync(hh, ctx->red_head.p, sizeof(hh), hipMemcpyDeviceToHost, s));
+        HIPTRY(hipMemcpyAsync(ctx->h_flags, ctx->flags.p, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+        HIPTRY(hipStreamSynchronize(s));
+        if (ctx->h_flags[1]) return CENT_E_OVERFLOW;
+        if (hh[1] > 0) {
+            deep.resize((size_t)hh[1]);
+            HIPTRY(hipMemcpy(deep.data(), ctx->red_deep.p, sizeof(int) * (size_t)hh[1], hipMemcpyDeviceToHost));
+            std::sort(deep.begin(), deep.end());
+        }
+        for (int i : deep) TRY(run_range(items[(size_t)i]));
+    } else {
+        for (const int4& it : items) TRY(run_range(it));
+    }
+    for (const int4& it : big) TRY(run_range(it));
+    std::vector<int64_t> Dl(L);
+    std::vector<double> bcp(NP), part;
+    HIPTRY(hipMemcpyAsync(Dl.data(), Dd, sizeof(int64_t) * L, hipMemcpyDeviceToHost, s));
+    HIPTRY(hipMemcpyAsync(bcp.data(), ctx->red_bcp.p, sizeof(double) * NP, hipMemcpyDeviceToHost, s));
+    if (persistent && poff.back() > 0) {
+        part.resize((size_t)poff.back());
+        HIPTRY(hipMemcpyAsync(part.data(), ctx->red_part.p, sizeof(double) * part.size(), hipMemcpyDeviceToHost, s));
+    }
+    HIPTRY(hipStreamSynchronize(s));
+    timer_collect(ctx, tm);
+    if (persistent) {  // the items' partials in item order (deep items were redone above)
+        std::vector<char> is_deep(items.size() + 1, 0);
+        for (int i : deep) is_deep[(size_t)i] = 1;
+        for (size_t i = 0; i < items.size(); ++i) {
+            if (is_deep[i]) continue;
+            const int4& it = items[i];
+            for (int x = it.x; x < it.y; ++x) bcp[(size_t)x] += part[(size_t)(poff[i] + x - it.x)];
+        }
+    }
+    TRY(cent_reduced_bc_sources(r, src, k, bcp.data(), bc));
+    if (farness && k > 0) {
+        std::vector<int64_t> D(NP, 0);
+        for (size_t i = 0; i < list.size(); ++i) D[(size_t)list[i]] = Dl[i];
+        TRY(cent_reduced_farness_sources(r, D.data(), src, k, farness));
+    }
+    ctx->stats.levels += levels;
+    ctx->stats.reached += k;
     ctx->stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return CENT_OK;
 }

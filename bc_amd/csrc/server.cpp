@@ -152,6 +152,32 @@ static bool make_room(cent_server* s, int64_t need, const GraphEntry* keep) {
     return true;
 }
 
+// Betweenness through the reduction traverses, inside each block, only the
+// copies with sources behind them; the plain call traverses the whole graph
+// once per source.  The reduced form is taken when its edge count is under
+// half the plain one (block trees: a few percent; a graph with one giant
+// block: about the same, so plain).
+static bool reduced_bc_pays(const GraphEntry* g, const Request* r) {
+    cent_reduced_view v;
+    if (cent_reduced_get_view(g->red, &v) != CENT_OK || v.np <= 0) return false;
+    const int64_t k = r->all ? g->n : (int64_t)r->src.size();
+    const double plain = (double)k * (double)std::max<int64_t>(g->nnz, 1);
+    double red = 0.0;
+    if (r->all) {
+        for (int32_t b = 0; b < v.nblocks; ++b)
+            red += (double)(v.boff[b + 1] - v.boff[b]) * (double)(v.prp[v.boff[b + 1]] - v.prp[v.boff[b]]);
+    } else {
+        std::vector<int64_t> ws((size_t)v.np);
+        if (cent_reduced_source_weights(g->red, r->src.data(), (int32_t)k, ws.data()) != CENT_OK) return false;
+        for (int32_t b = 0; b < v.nblocks; ++b) {
+            const double eb = (double)(v.prp[v.boff[b + 1]] - v.prp[v.boff[b]]);
+            for (int32_t x = v.boff[b]; x < v.boff[b + 1]; ++x)
+                if (ws[(size_t)x] > 0) red += eb;
+        }
+    }
+    return red < 0.5 * plain;
+}
+
 static void run_request(cent_server* s, int slot, Request* r, GraphEntry* g) {
     int st = CENT_OK;
     const bool dry = (s->o.flags & CENT_SERVER_DRYRUN) != 0;
@@ -160,8 +186,10 @@ static void run_request(cent_server* s, int slot, Request* r, GraphEntry* g) {
         std::lock_guard<std::mutex> lk(g->build_mu);
         if (!dry && !g->dev) st = cent_graph_upload(ctx, g->host, &g->dev);
         // the reduction runs on this slot's GPU (its scratch is transient,
-        // freed before the query runs); the dry run reduces on the host
-        if (!st && r->kind == CENT_Q_CC && r->all && !g->red)
+        // freed before the query runs); the dry run reduces on the host.
+        // All-vertex closeness always goes through it, betweenness when it
+        // is cheaper there (below)
+        if (!st && ((r->kind == CENT_Q_CC && r->all) || r->kind == CENT_Q_BC) && !g->red)
             st = dry ? cent_reduce(g->host, &g->red) : cent_reduce_gpu(ctx, g->host, &g->red);
     }
     const int64_t k = r->all ? g->n : (int64_t)r->src.size();
@@ -178,8 +206,14 @@ static void run_request(cent_server* s, int slot, Request* r, GraphEntry* g) {
         } else if (r->kind == CENT_Q_BC) {
             r->bc.assign((size_t)g->n, 0.0);
             r->far.assign((size_t)std::max<int64_t>(k, 1), 0);
-            st = cent_bc(ctx, g->dev, r->all ? nullptr : r->src.data(), r->all ? -1 : (int32_t)k, r->bc.data(),
-                         r->far.data(), nullptr, &opts);
+            st = CENT_E_UNSUPPORTED;
+            if (g->red && reduced_bc_pays(g, r))  // (unsupported for a multigraph's reduction)
+                st = r->all ? cent_bc_reduced(ctx, g->red, r->bc.data(), r->far.data(), CENT_SCHED_PERSISTENT, &opts)
+                            : cent_bc_reduced_sources(ctx, g->red, r->src.data(), (int32_t)k, r->bc.data(),
+                                                      r->far.data(), CENT_SCHED_PERSISTENT, &opts);
+            if (st == CENT_E_UNSUPPORTED)
+                st = cent_bc(ctx, g->dev, r->all ? nullptr : r->src.data(), r->all ? -1 : (int32_t)k, r->bc.data(),
+                             r->far.data(), nullptr, &opts);
         } else if (r->all) {
             r->far.assign((size_t)std::max<int64_t>(g->n, 1), 0);
             st = cent_cc_reduced(ctx, g->red, r->far.data(), CENT_SCHED_PERSISTENT, &opts);
@@ -191,7 +225,7 @@ static void run_request(cent_server* s, int slot, Request* r, GraphEntry* g) {
     // keep a slot's cached workspace within its share of the budget
     if (!dry && cent_ctx_workspace_bytes(ctx) > s->o.hbm_budget / std::max(1, s->o.slots) / 2) cent_ctx_trim(ctx);
     std::lock_guard<std::mutex> lk(s->mu);
-    if (!dry && g->red && r->kind == CENT_Q_CC && r->all) {
+    if (!dry && g->red && (r->kind == CENT_Q_BC || r->all)) {
         // the reduced device copy now exists; account for it once
         const int64_t rb = reduced_estimate(g->red);
         if (g->dev_bytes < graph_estimate(g) + rb) {

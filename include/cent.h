@@ -414,6 +414,21 @@ void cent_reduced_free(cent_reduced* r);
  * vertex x (source and target weights pw).  No reference counterpart: the
  * reference reduces the graph for closeness only. */
 int  cent_reduced_bc(const cent_reduced* r, const double* bcp, double* bc);
+/* Sampled sources through the reduction (no reference counterpart): for a
+ * source multiset src[0..k) (repeats count), WS[x] (length np) = sources
+ * hanging off packed copy x on its side of x's block; with every vertex
+ * once it equals pw.  cent_reduced_bc_sources adds into bc[v] the
+ * betweenness from those sources (the sum over them of Brandes'
+ * dependencies, like computeBC over a source subset) given bcp[x] = weighted
+ * Brandes dependencies at x from every copy a with source weight WS[a] and
+ * target weights pw.  cent_reduced_farness_sources assigns far[i] = farness
+ * of src[i] given D[x] = sum_y d(x, y) pw[y] at the copies that answer for
+ * the sources (and at every non-root block's head copy and its parent-side
+ * copy).  Multigraph reductions: CENT_E_UNSUPPORTED. */
+int  cent_reduced_source_weights(const cent_reduced* r, const int32_t* src, int32_t k, int64_t* WS);
+int  cent_reduced_bc_sources(const cent_reduced* r, const int32_t* src, int32_t k, const double* bcp, double* bc);
+int  cent_reduced_farness_sources(const cent_reduced* r, const int64_t* D, const int32_t* src, int32_t k,
+                                  int64_t* far);
 /* Frees the device copy the engine caches on r (re-created on next use). */
 void cent_reduced_drop_device(cent_reduced* r);
 
@@ -439,6 +454,17 @@ int  cent_cc_reduced(cent_ctx* ctx, const cent_reduced* r, int64_t* farness, int
  * counterpart (the reference reduces for closeness only). */
 int  cent_bc_reduced(cent_ctx* ctx, const cent_reduced* r, double* bc, int64_t* farness,
                      int32_t sched, const cent_opts* opts);
+
+/* Betweenness from the sources src[0..k) (repeats count; textbook formula,
+ * ordered pairs; bc accumulated like computeBC) through the reduced form,
+ * plus farness[i] of src[i] (assigned, may be null).  Equals cent_bc on the
+ * same sources.  Inside each block only the copies with sources behind
+ * them are traversed, so on graphs of many blocks (block trees, road-like
+ * graphs) a sampled call costs a few traversals per block; on a graph with
+ * one giant block it costs about what cent_bc does.  No reference
+ * counterpart.  Multigraph reductions: CENT_E_UNSUPPORTED. */
+int  cent_bc_reduced_sources(cent_ctx* ctx, const cent_reduced* r, const int32_t* src, int32_t k, double* bc,
+                             int64_t* farness, int32_t sched, const cent_opts* opts);
 
 /* ---------------------------------------------------------------- serving */
 /*

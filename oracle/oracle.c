@@ -16,6 +16,12 @@
  *                       delta[w] once per predecessor of w (:319-326).
  *  oracle_brandes_f32   mode-1 formula in FP32 exactly as the reference types
  *                       it (Brandes.c:226-227), to check the karate score file.
+ *  oracle_weighted_brandes / oracle_weighted_brandes_src
+ *                       the same traversal with vertex target weights and
+ *                       source weights (per vertex, or per listed source),
+ *                       the checker of the reduced (D1/AP) betweenness; with
+ *                       unit weights they are oracle_brandes mode 0, which
+ *                       the tests check.
  *
  * Pinning (tests/test_oracle.py): closed forms on paths, stars, cycles,
  * complete graphs and full grids; brute-force shortest-path enumeration on
@@ -272,6 +278,47 @@ int oracle_weighted_farness(int32_t n, const int32_t* rp, const int32_t* col, co
         free(dist); free(queue);
     }
     return err ? 2 : 0;
+}
+
+/* Weighted Brandes from listed sources src[i] with their own source weights
+ * sw[i] and target weights w[v] (checker for sampled betweenness through the
+ * reduction): bc[v] += sw[i] * delta_s(v), delta as below; far[i] (may be
+ * NULL) = sum over reached v of d(s, v) w[v].  Serial, bc accumulated. */
+int oracle_weighted_brandes_src(int32_t n, const int32_t* rp, const int32_t* col, const int64_t* w,
+                                const int32_t* src, const int64_t* sw, int32_t k, double* bc, int64_t* far) {
+    ws_t ws;
+    if (!ws_alloc(&ws, n)) { ws_free(&ws); return 2; }
+    for (int32_t v = 0; v < n; ++v) ws.dist[v] = -1;
+    for (int32_t i = 0; i < k; ++i) {
+        const int32_t s = src[i];
+        int64_t h = 0, t = 0, f = 0;
+        ws.queue[t++] = s; ws.dist[s] = 0; ws.sigma[s] = 1.0;
+        while (h < t) {
+            int32_t x = ws.queue[h++];
+            for (int32_t j = rp[x]; j < rp[x + 1]; ++j) {
+                int32_t y = col[j];
+                if (ws.dist[y] < 0) {
+                    ws.dist[y] = ws.dist[x] + 1; ws.sigma[y] = 0.0; ws.queue[t++] = y;
+                    f += (int64_t)ws.dist[y] * w[y];
+                }
+                if (ws.dist[y] == ws.dist[x] + 1) ws.sigma[y] += ws.sigma[x];
+            }
+        }
+        if (far) far[i] = f;
+        for (int64_t q = t - 1; q >= 0; --q) {
+            int32_t x = ws.queue[q];
+            double acc = 0.0;
+            for (int32_t j = rp[x]; j < rp[x + 1]; ++j) {
+                int32_t y = col[j];
+                if (ws.dist[y] == ws.dist[x] + 1) acc += ((double)w[y] + ws.delta[y]) / ws.sigma[y];
+            }
+            ws.delta[x] = ws.sigma[x] * acc;
+            if (x != s) bc[x] += (double)sw[i] * ws.delta[x];
+        }
+        for (int64_t q = 0; q < t; ++q) ws.dist[ws.queue[q]] = -1;
+    }
+    ws_free(&ws);
+    return 0;
 }
 
 /* Weighted Brandes (checker for the reduced BC path): vertex weights w act as

@@ -32,6 +32,7 @@ def _load():
     lib.oracle_max_threads.restype = C.c_int
     lib.oracle_weighted_farness.argtypes = [C.c_int32, I32P, I32P, I64P, I32P, C.c_int32, I64P]
     lib.oracle_weighted_brandes.argtypes = [C.c_int32, I32P, I32P, I64P, I32P, C.c_int32, F64P, I64P]
+    lib.oracle_weighted_brandes_src.argtypes = [C.c_int32, I32P, I32P, I64P, I32P, I64P, C.c_int32, F64P, I64P]
     return lib
 
 
@@ -106,6 +107,21 @@ def weighted_brandes(n, rp, col, w, sources=None):
                                         _p(src, C.c_int32), len(src), _p(bc, C.c_double),
                                         _p(far, C.c_int64)) == 0
     return bc[:n], far[:len(src)]
+
+
+def weighted_brandes_src(n, rp, col, w, sources, source_weights):
+    """(bc, weighted farness) from listed sources with their own weights."""
+    rp = np.ascontiguousarray(rp, np.int32)
+    col = np.ascontiguousarray(col, np.int32) if len(col) else np.zeros(1, np.int32)
+    w = np.ascontiguousarray(w, np.int64) if len(w) else np.zeros(1, np.int64)
+    src = np.ascontiguousarray(sources, np.int32) if len(sources) else np.zeros(1, np.int32)
+    sw = np.ascontiguousarray(source_weights, np.int64) if len(sources) else np.zeros(1, np.int64)
+    bc = np.zeros(max(n, 1))
+    far = np.zeros(max(len(sources), 1), np.int64)
+    assert _lib.oracle_weighted_brandes_src(n, _p(rp, C.c_int32), _p(col, C.c_int32), _p(w, C.c_int64),
+                                            _p(src, C.c_int32), _p(sw, C.c_int64), len(sources),
+                                            _p(bc, C.c_double), _p(far, C.c_int64)) == 0
+    return bc[:n], far[:len(sources)]
 
 
 def max_threads():

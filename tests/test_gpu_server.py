@@ -1,6 +1,7 @@
-"""Serving on the GPU: every request's result equals the engine run alone
-(bit-identical — each request's kernels run in order on its own stream and
-workspace) and the oracle, with several requests overlapping on the device."""
+"""Serving on the GPU: every request's result equals the same request served
+alone (bit-identical — each request's kernels run in order on its own stream
+and workspace, whatever runs beside it) and the oracle, with several
+requests overlapping on the device."""
 import numpy as np
 import pytest
 
@@ -8,6 +9,18 @@ import _oracle as O
 import bc_amd
 
 pytestmark = pytest.mark.gpu
+
+
+def served_alone(graphs, j, kind, src):
+    """The same request on a fresh one-slot server, nothing else running."""
+    srv = bc_amd.Server(slots=1, hbm_budget=64 << 30)
+    gids = [srv.add_graph(g) for g in graphs]
+    rid = srv.submit(gids[j], kind, src)
+    srv.wait(rid)
+    st, bc, far, _ = srv.result(rid)
+    srv.close()
+    assert st == 0
+    return bc, far
 
 
 def test_results_same_alone_and_in_mix(ctx):
@@ -39,10 +52,7 @@ def test_results_same_alone_and_in_mix(ctx):
         assert st == 0
         g = graphs[j]
         if kind == "bc":
-            # alone == in the mix, under the server's launch settings (one
-            # stream per slot; source-parallel requests capped at
-            # SERVER_SOURCE_SLOTS slots)
-            abc, afar, _ = ctx.bc(dgs[j], src, max_slots=bc_amd.SERVER_SOURCE_SLOTS, one_stream=True)
+            abc, afar = served_alone(graphs, j, bc_amd.CENT_Q_BC, src)
             assert np.array_equal(bc, abc) and np.array_equal(far, afar)
             obc, ofar, _, _ = O.brandes(g, src, threads=8)
             np.testing.assert_allclose(bc, obc, rtol=1e-9, atol=1e-9 * max(1.0, np.abs(obc).max()))
@@ -91,7 +101,7 @@ def test_mix_with_pool_sized_graphs(ctx):
         assert st == 0
         g = graphs[j]
         if kind == "bc":
-            abc, afar, _ = ctx.bc(dgs[j], src, max_slots=bc_amd.SERVER_SOURCE_SLOTS, one_stream=True)
+            abc, afar = served_alone(graphs, j, bc_amd.CENT_Q_BC, src)
             assert np.array_equal(bc, abc) and np.array_equal(far, afar)
             obc, ofar, _, _ = O.brandes(g, src, threads=16)
             np.testing.assert_allclose(bc, obc, rtol=1e-9, atol=1e-9 * max(1.0, np.abs(obc).max()))

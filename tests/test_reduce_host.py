@@ -108,6 +108,16 @@ def test_weighted_oracle_reduces_to_plain():
     want, wfar, _, _ = O.brandes(g, None)
     np.testing.assert_allclose(bc, want, rtol=1e-12)
     assert np.array_equal(far, wfar)
+    # listed sources with their own weights: unit weights give plain Brandes
+    # over the list, and a source listed with weight 2 counts twice
+    src = np.array([3, 9, 9, 20], np.int32)
+    bc2, far2 = O.weighted_brandes_src(g.n, g.row_ptr, g.col, np.ones(g.n, np.int64), src, np.ones(4, np.int64))
+    want2, wfar2, _, _ = O.brandes(g, src)
+    np.testing.assert_allclose(bc2, want2, rtol=1e-12)
+    assert np.array_equal(far2, wfar2)
+    bc3, _ = O.weighted_brandes_src(g.n, g.row_ptr, g.col, np.ones(g.n, np.int64), src[[0, 1, 3]],
+                                    np.array([1, 2, 1], np.int64))
+    np.testing.assert_allclose(bc3, want2, rtol=1e-12)
 
 
 @pytest.mark.parametrize("name", ["karate.txt", "dolphins.txt", "football.txt", "polbooks.txt",
@@ -134,3 +144,48 @@ def test_reduced_bc_shapes_and_random():
         check_bc(bc_amd.CSR.from_edges(n, rng.integers(0, n, m), rng.integers(0, n, m)))
     check_bc(bc_amd.gen_blocktree(40, 1.5, 8, 200, 6.0, 0.3, 0.5, seed=3))
     check_bc(bc_amd.gen_rmat(9, 8 << 9, seed=7, lcc=False))
+
+
+# ---- sampled sources through the reduction (the host side of cent_bc_reduced_sources)
+def _sampled_check(g, src):
+    r = bc_amd.Reduced(g)
+    WS = r.source_weights(src)
+    T = np.nonzero(WS > 0)[0].astype(np.int32)
+    bcp, _ = O.weighted_brandes_src(r.np, r.prp, r.pcol, r.pw, T, WS[T])
+    got = r.bc_from_sources(src, bcp)
+    want, wfar, _, _ = O.brandes(g, src)
+    np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-9 * max(1.0, np.abs(want).max()))
+    D = O.weighted_farness(r.np, r.prp, r.pcol, r.pw)
+    assert np.array_equal(r.farness_of_sources(src, D), wfar)
+    return r, WS, T
+
+
+def test_source_weights_of_all_vertices_are_copy_weights():
+    for g in (bc_amd.gen_blocktree(300, 1.5, 8, 2000, 6.0, 0.3, 0.5, seed=3), bc_amd.gen_rmat(11, 16 << 11, seed=4),
+              bc_amd.buildGraph(O.dataset_path("testdata2.txt"))):
+        r = bc_amd.Reduced(g)
+        assert np.array_equal(r.source_weights(np.arange(g.n)), r.pw)
+
+
+@pytest.mark.parametrize("name", ["karate.txt", "dolphins.txt", "testdata2.txt", "football.txt"])
+def test_sampled_reduced_bc_fixtures(name):
+    g = bc_amd.buildGraph(O.dataset_path(name))
+    rng = np.random.default_rng(len(name))
+    for k in (1, 5, 17):
+        src = rng.integers(g.startNodeID, g.n, k).astype(np.int32)  # repeats allowed
+        _sampled_check(g, src)
+    _sampled_check(g, g.sources())  # all sources: the all-source formula
+
+
+def test_sampled_reduced_bc_block_trees_and_forests():
+    g = bc_amd.gen_blocktree(400, 1.5, 8, 3000, 6.0, 0.3, 0.5, seed=11)
+    for k, seed in ((64, 1), (300, 2), (7, 3)):
+        src = bc_amd.sample_sources(g.n, k, seed)
+        r, WS, T = _sampled_check(g, src)
+        assert len(T) < r.np // 2  # far fewer traversals than all-source
+    # a forest of pendant trees and paths (everything folds), a cycle, isolated vertices
+    u = [0, 1, 1, 3, 5, 6, 7, 8, 9, 11]
+    v = [1, 2, 3, 4, 6, 7, 8, 5, 10, 12]
+    g = bc_amd.CSR.from_edges(15, u, v)
+    for src in ([0], [2, 2, 4], [5, 9, 13], list(range(15))):
+        _sampled_check(g, np.array(src, np.int32))
