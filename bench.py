@@ -557,33 +557,50 @@ def traffic_key(args, layout) -> dict:
             "kernel_sha16": kernel_sha16()}
 
 
-def _read_counters(d: str):
-    """Per-kernel (dispatches, summed counter value) from a rocprofv3 --pmc
-    output directory (its *counter_collection.csv)."""
+def _read_counters(d: str, counter: str = None):
+    """Per-kernel (dispatches, summed value) of one counter from a rocprofv3
+    --pmc output directory (its SQLite *_results.db, or a
+    *counter_collection.csv).  counter None: the only counter of the run."""
     import csv
     import glob
-    files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
-    if not files:
-        raise SystemExit(f"no counter_collection.csv under {d}")
+    import sqlite3
+    rows = []
+    for f in glob.glob(os.path.join(d, "**", "*_results.db"), recursive=True):
+        con = sqlite3.connect(f)
+        rows += con.execute("select kernel_name, dispatch_id, counter_name, value from counters_collection").fetchall()
+        con.close()
+    if not rows:
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            rows += [(r["Kernel_Name"], r.get("Dispatch_Id", ""), r.get("Counter_Name", ""), float(r["Counter_Value"]))
+                     for r in csv.DictReader(open(f))]
+    if not rows:
+        raise SystemExit(f"no counter results under {d}")
     out = {}
-    for f in files:
-        for row in csv.DictReader(open(f)):
-            name = row["Kernel_Name"]
-            disp, tot = out.get(name, (set(), 0.0))
-            disp.add(row.get("Dispatch_Id", len(disp)))
-            out[name] = (disp, tot + float(row["Counter_Value"]))
+    for name, disp, cname, val in rows:
+        if counter and cname != counter:
+            continue
+        ds, tot = out.get(name, (set(), 0.0))
+        ds.add(disp)
+        out[name] = (ds, tot + float(val))
     return {k: (len(v[0]), v[1]) for k, v in out.items()}
 
 
 def write_traffic(args) -> int:
     """--write-traffic FETCH_DIR WRITE_DIR: turns two rocprofv3 --pmc runs of
-    this bench (one for FETCH_SIZE, one for WRITE_SIZE, each over
-    `--steps 1 --warmup 0 --no-cpu-baseline --no-serve --timed-only
-    --copy-probe`) into profiles/rmat20_traffic.json.  Readings are in KiB
-    and are scaled so that the copy probe's reads equal its known bytes."""
+    this bench (one for FETCH_SIZE and TCC_EA0_RDREQ_DRAM_32B_sum, one for
+    WRITE_SIZE, each over `--steps 1 --warmup 0 --no-cpu-baseline --no-serve
+    --timed-only --copy-probe`) into profiles/rmat20_traffic.json.  Readings
+    are scaled so that the copy probe's reads and writes equal its known
+    bytes.  FETCH_SIZE counts every L2 miss sent to the fabric (Infinity-
+    Cache hits included); the DRAM counter counts the read requests that
+    the fabric routes to the memory controllers."""
     import bc_amd
     fdir, wdir = args.write_traffic
-    fetch, write = _read_counters(fdir), _read_counters(wdir)
+    fetch, write = _read_counters(fdir, "FETCH_SIZE"), _read_counters(wdir, "WRITE_SIZE")
+    try:
+        dram = _read_counters(fdir, "TCC_EA0_RDREQ_DRAM_32B_sum")
+    except SystemExit:
+        dram = {}
     probe = [k for k in fetch if "k_copy_probe" in k]
     if not probe:
         raise SystemExit("the counter runs need --copy-probe (calibration kernel)")
@@ -592,6 +609,9 @@ def write_traffic(args) -> int:
     known_kib = (1 << 30) / 1024.0  # 1 GiB read and 1 GiB written per probe launch
     fs = known_kib / (fprobe / nprobe)
     ws = known_kib / (wprobe / nprobe)
+    ds = None  # DRAM counter: 32-byte units, calibrated like the others
+    if dram.get(probe[0], (0, 0.0))[1] > 0:
+        ds = (1 << 30) / (dram[probe[0]][1] / dram[probe[0]][0])
     passes = {"fwd_pass": ("k_fwd", "k_push1"), "bwd_pass": ("k_bwd",)}
     if not args.no_tune_table and os.path.exists(bc_amd.TUNE_TABLE):
         bc_amd.tune_table_load()
@@ -601,6 +621,7 @@ def write_traffic(args) -> int:
     out = {"workload": f"sampled BC + closeness, R-MAT 2^{args.scale} avg degree {args.edge_factor} (LCC), "
                        f"{args.sources} sources",
            "key": traffic_key(args, layout), "fetch_scale": round(fs, 4), "write_scale": round(ws, 4),
+           "dram_scale_bytes_per_unit": round(ds, 4) if ds else None,
            "source": "profiles/rmat20_sampled_bc_hbm_counters.md"}
     for pname, prefixes in passes.items():
         kern = {}
@@ -609,6 +630,8 @@ def write_traffic(args) -> int:
             if any(short.startswith(p) for p in prefixes):
                 kern[short] = {"dispatches": nd, "fetch_bytes": int(v * fs * 1024),
                                "write_bytes": int(write.get(k, (0, 0.0))[1] * ws * 1024)}
+                if ds:
+                    kern[short]["dram_read_bytes"] = int(dram.get(k, (0, 0.0))[1] * ds)
         # one level launch of the pass = one dispatch of its main kernel
         main = "k_bwd<false>" if pname == "bwd_pass" else "k_fwd<false>"
         nl = kern[main]["dispatches"] + (kern.get("k_push1_or", {}).get("dispatches", 0) if pname == "fwd_pass" else 0)
@@ -616,7 +639,11 @@ def write_traffic(args) -> int:
         wb = sum(x["write_bytes"] for x in kern.values())
         out[pname] = {"launches_per_step": nl, "fetch_bytes_per_step": fb, "write_bytes_per_step": wb,
                       "traffic_bytes_per_launch": int((fb + wb) / nl), "kernels": kern}
-    json.dump(out, open(TRAFFIC_JSON, "w"), indent=1)
+        if ds:
+            db = sum(x.get("dram_read_bytes", 0) for x in kern.values())
+            out[pname]["dram_read_bytes_per_step"] = db
+            out[pname]["dram_bytes_per_launch"] = int((db + wb) / nl)
+    json.dump(out, open(args.traffic_out, "w"), indent=1)
     print(json.dumps(out, indent=1))
     return 0
 
@@ -664,7 +691,9 @@ def main() -> int:
     ap.add_argument("--serve-drain", type=float, default=240.0,
                     help="seconds after the last submission; unfinished requests are cancelled")
     ap.add_argument("--write-traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
-                    help="write profiles/rmat20_traffic.json from two rocprofv3 --pmc runs and exit")
+                    help="write the traffic file from two rocprofv3 --pmc runs and exit")
+    ap.add_argument("--traffic-out", default=TRAFFIC_JSON,
+                    help="where --write-traffic writes (default profiles/rmat20_traffic.json)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -785,6 +814,10 @@ def main() -> int:
             tb = tr[dom]["traffic_bytes_per_launch"]
             ach = tb / (ms_launch * 1e-3) / 1e9
             tstep = sum(tr[p]["fetch_bytes_per_step"] + tr[p]["write_bytes_per_step"] for p in ("fwd_pass", "bwd_pass"))
+            if tr[dom].get("dram_bytes_per_launch"):
+                dgbs = tr[dom]["dram_bytes_per_launch"] / (ms_launch * 1e-3) / 1e9
+                roofline.update({"dram_GBs": round(dgbs, 1), "dram_frac": round(dgbs / HBM_PEAK_GBS, 4),
+                                 "dram_bytes_per_launch": tr[dom]["dram_bytes_per_launch"]})
             roofline.update({"achieved": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": tb,
                              "basis": "counter bytes per launch / one-stream launch duration",
                              "traffic_bytes_per_step": int(tstep),
