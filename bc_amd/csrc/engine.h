@@ -177,6 +177,7 @@ struct DevCSR {
     int *seg_v = nullptr, *seg_beg = nullptr, *seg_end = nullptr, *hub_v = nullptr, *hub_seg_off = nullptr;
     std::vector<int> h_chunk_beg, h_hub_v, h_hub_seg_off;
     bool multi = false;  // rows may repeat a neighbour or list their own vertex
+    int4* ell = nullptr; // [n] inline rows of the source-parallel step (see persource_kernels.h), or null
 };
 
 // Failure injection for tests of the error paths (cent_debug_fail_upload):

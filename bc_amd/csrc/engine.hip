@@ -23,6 +23,7 @@ void devcsr_free(DevCSR& d) {
     if (d.col) (void)hipFree(d.col);
     if (d.chunk_beg) (void)hipFree(d.chunk_beg);
     if (d.hub_buf) (void)hipFree(d.hub_buf);
+    if (d.ell) (void)hipFree(d.ell);
     d = DevCSR();
 }
 
@@ -367,6 +368,23 @@ extern "C" int cent_graph_create(cent_ctx* ctx, const int32_t* row_ptr, const in
         return st;
     }
     g->d.multi = g->multi;
+    {   // inline rows for the source-parallel step: a vertex of degree <= 4 holds its neighbours in one
+        // 16-byte row (-1 padded); a longer row is marked and read from the CSR
+        std::vector<int4> ell((size_t)(n > 0 ? n : 1));
+        for (int32_t v = 0; v < n; ++v) {
+            const int32_t b = urp[v], dgv = urp[v + 1] - b;
+            int e[4] = {-1, -1, -1, -1};
+            if (dgv > 4) e[0] = INT32_MIN;
+            else
+                for (int32_t j = 0; j < dgv; ++j) e[j] = ucol[b + j];
+            ell[(size_t)v] = make_int4(e[0], e[1], e[2], e[3]);
+        }
+        st = upload(ctx, &g->d.ell, ell.data(), (size_t)n);
+        if (st) {
+            cent_graph_free(g);
+            return st;
+        }
+    }
     g->h_rp.assign(urp, urp + (size_t)n + 1);
     if (!g->new_id.empty()) {  // for results written to device memory in caller order
         hipError_t e = hipMalloc((void**)&g->d_new_id, sizeof(int32_t) * (size_t)n);

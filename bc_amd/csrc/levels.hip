@@ -558,7 +558,7 @@ static int run_persource(cent_ctx* ctx, const cent_graph* g, const BatchSpec& sp
     HIPTRY(hipMemsetAsync(ctx->ps_depth.p, 0, sizeof(unsigned long long), s));
     HIPTRY(hipMemsetAsync(ctx->flags.p, 0, sizeof(int) * 2, s));
     PsArgs a{};
-    a.n = d.n; a.nw = (int)nw; a.rp = d.rp; a.col = d.col; a.src = ctx->src.p; a.k = k;
+    a.n = d.n; a.nw = (int)nw; a.rp = d.rp; a.col = d.col; a.ell = d.ell; a.src = ctx->src.p; a.k = k;
     a.Q = ctx->ps_Q.p;
     if (bc) {
         a.rec = ctx->ps_rec.p;
@@ -1107,7 +1107,7 @@ extern "C" int cent_cc_device_noalloc(cent_ctx* ctx, const cent_graph* g, const 
     if (g->d_new_id) k_remap_ids<<<(int)std::min<int64_t>(((int64_t)k + 255) / 256, 4096), 256, 0, s>>>(dsrc, k, g->d_new_id);
     HIPTRY(hipGetLastError());
     PsArgs a{};
-    a.n = n; a.nw = (int)nw; a.rp = g->d.rp; a.col = g->d.col; a.src = dsrc; a.k = k;
+    a.n = n; a.nw = (int)nw; a.rp = g->d.rp; a.col = g->d.col; a.ell = g->d.ell; a.src = dsrc; a.k = k;
     a.Q = Q; a.bits = bits; a.far = far; a.dist_sum = ds; a.weight = nullptr;
     a.cnt = ctr; a.depth_sum = ctr + 2; a.overflow = nullptr;  // closeness sets no flag
     // slot g's queue is Q + g n (the kernel's stride) and its map bits + g nw

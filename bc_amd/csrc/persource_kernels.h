@@ -44,6 +44,11 @@
 // wavefront (lane-strided edges, butterfly sum), so one hub row does not
 // serialise a level.
 //
+// Rows of at most kPsE edges (nearly every vertex of a grid or mesh) are
+// read from a 16-byte inline row per vertex, one load that holds the whole
+// row, instead of a row-offset line and a column-id line: on the 2048^2
+// grid that is about a third of the lines a traversal misses in L2.
+//
 // Path counts that grow past 2^512 (deep graphs with many parallel
 // shortest paths, e.g. chains of diamonds) would leave FP64 a few levels
 // later.  A source whose forward reads such a count is undone (its
@@ -77,6 +82,10 @@ struct PsArgs {
     int nw;                               // closeness: 32-bit words per slot map, a multiple of 4
     const int* __restrict__ rp;
     const int* __restrict__ col;
+    // [n] inline rows: the neighbours of a vertex of degree <= kPsE (-1
+    // padded), kEllLong in .x for a longer row (read from rp / col); null:
+    // every row from rp / col
+    const int4* __restrict__ ell;
     const int* __restrict__ src;          // [k] device ids
     int k;
     PsRec* __restrict__ rec;              // BC: [G][n]
@@ -142,16 +151,45 @@ __device__ __forceinline__ void ps_visit(const PsArgs& a, PsRec* R, int* Q,
 // independent chains in flight instead of walking its rows edge by edge.
 constexpr int kNoEdge = -2;
 constexpr int kClaimed = -3;
+constexpr int kEllLong = INT32_MIN;  // inline-row marker: the row is read from rp / col
+static_assert(kPsE == 4, "inline rows hold kPsE = 4 neighbours");
+
+// Row of u for the staged expansions: wi = its neighbours from the inline
+// row (rb = dg = 0, no tail), or wi[0] = kEllLong and rb / dg its CSR row.
+// u < 0: an empty row.
+__device__ __forceinline__ void ps_row(const PsArgs& a, int u, int* wi, int& rb, int& dg) {
+    rb = 0;
+    dg = 0;
+    if (u < 0) {
+        wi[0] = wi[1] = wi[2] = wi[3] = -1;
+        return;
+    }
+    if (a.ell) {
+        const int4 e4 = a.ell[u];
+        if (e4.x != kEllLong) {
+            wi[0] = e4.x; wi[1] = e4.y; wi[2] = e4.z; wi[3] = e4.w;
+            return;
+        }
+    }
+    wi[0] = kEllLong;
+    rb = a.rp[u];
+    dg = a.rp[u + 1] - rb;
+}
+
+// Neighbour e (< kPsE) of a row staged by ps_row, -1 past its end.
+__device__ __forceinline__ int ps_nbr(const PsArgs& a, const int* wi, int rb, int dg, int e) {
+    return wi[0] != kEllLong ? wi[e] : (e < dg ? a.col[rb + e] : -1);
+}
 
 template <bool COMPAT>
-__device__ __forceinline__ void ps_expand_rec(const PsArgs& a, PsRec* R, int* Q,
+__device__ __forceinline__ void ps_expand_rec(const PsArgs& a, PsRec* R, int* Q, const int (*wi)[kPsE],
                                                const int* rb, const int* dg, const double* su, int d, int le,
                                                int* s_cnt, PsLocal& L) {
     int w[kPsV][kPsE], lev[kPsV][kPsE], old[kPsV][kPsE];
 #pragma unroll
     for (int m = 0; m < kPsV; ++m)
 #pragma unroll
-        for (int e = 0; e < kPsE; ++e) w[m][e] = e < dg[m] ? a.col[rb[m] + e] : -1;
+        for (int e = 0; e < kPsE; ++e) w[m][e] = ps_nbr(a, wi[m], rb[m], dg[m], e);
 #pragma unroll
     for (int m = 0; m < kPsV; ++m)
 #pragma unroll
@@ -254,14 +292,14 @@ __device__ __forceinline__ double ps_pull_long(const PsArgs& a, const PsRec* R, 
 
 // Backward sums of up to kPsV rows at once: acc[m] = sum over row m's
 // neighbours w at level `lvl` of coef[w] (held in sig), in row order.
-__device__ __forceinline__ void ps_pull_rec(const PsArgs& a, const PsRec* R, const int* rb,
+__device__ __forceinline__ void ps_pull_rec(const PsArgs& a, const PsRec* R, const int (*wi)[kPsE], const int* rb,
                                              const int* dg, int lvl, double* acc) {
     int w[kPsV][kPsE], lev[kPsV][kPsE];
     double x[kPsV][kPsE];
 #pragma unroll
     for (int m = 0; m < kPsV; ++m)
 #pragma unroll
-        for (int e = 0; e < kPsE; ++e) w[m][e] = e < dg[m] ? a.col[rb[m] + e] : -1;
+        for (int e = 0; e < kPsE; ++e) w[m][e] = ps_nbr(a, wi[m], rb[m], dg[m], e);
 #pragma unroll
     for (int m = 0; m < kPsV; ++m)
 #pragma unroll
@@ -338,7 +376,7 @@ __global__ __launch_bounds__(kPsBlock) void k_ps_bc(PsArgs a) {
                 sc = __longlong_as_double((long long)(1023 - kd) << 52);
             }
             for (int i0 = lb + tid; i0 < le; i0 += kPsBlock * kPsV) {
-                int u[kPsV], rb[kPsV], dg[kPsV];
+                int u[kPsV], rb[kPsV], dg[kPsV], wi[kPsV][kPsE];
                 double su[kPsV];
 #pragma unroll
                 for (int m = 0; m < kPsV; ++m) {
@@ -347,12 +385,9 @@ __global__ __launch_bounds__(kPsBlock) void k_ps_bc(PsArgs a) {
                 }
 #pragma unroll
                 for (int m = 0; m < kPsV; ++m) {
-                    rb[m] = 0;
-                    dg[m] = 0;
+                    ps_row(a, u[m], wi[m], rb[m], dg[m]);
                     su[m] = 0.0;
                     if (u[m] >= 0) {
-                        rb[m] = a.rp[u[m]];
-                        dg[m] = a.rp[u[m] + 1] - rb[m];
                         su[m] = ld_sig(&R[u[m]].sig) * sc;
                         if (su[m] >= 0x1p512) s_large = 1;  // unscaled form only (scaled counts stay near 2^256)
                     }
@@ -366,7 +401,7 @@ __global__ __launch_bounds__(kPsBlock) void k_ps_bc(PsArgs a) {
                             dg[m] = 0;
                         }
                     }
-                ps_expand_rec<COMPAT>(a, R, Q, rb, dg, su, d, le, &s_cnt[c], L);
+                ps_expand_rec<COMPAT>(a, R, Q, wi, rb, dg, su, d, le, &s_cnt[c], L);
             }
             __syncthreads();
             const int nbig = min(s_nbig[c], kPsBig);
@@ -427,16 +462,11 @@ __global__ __launch_bounds__(kPsBlock) void k_ps_bc(PsArgs a) {
                         const int i = i0 + m * kPsBlock;
                         v[m] = i < b1 ? Q[i] : -1;
                     }
+                    int wi[kPsV][kPsE];
 #pragma unroll
                     for (int m = 0; m < kPsV; ++m) {
-                        rb[m] = 0;
-                        dg[m] = 0;
-                        sv[m] = 1.0;
-                        if (v[m] >= 0) {
-                            rb[m] = a.rp[v[m]];
-                            dg[m] = a.rp[v[m] + 1] - rb[m];
-                            sv[m] = ld_sig(&R[v[m]].sig);
-                        }
+                        ps_row(a, v[m], wi[m], rb[m], dg[m]);
+                        sv[m] = v[m] >= 0 ? ld_sig(&R[v[m]].sig) : 1.0;
                     }
                     // long rows always go to the wavefronts (listed while the
                     // list has room), so each row's sum has one order whoever
@@ -452,7 +482,7 @@ __global__ __launch_bounds__(kPsBlock) void k_ps_bc(PsArgs a) {
                         }
                     }
                     double acc[kPsV];
-                    ps_pull_rec(a, R, rb, dg, dd + 1, acc);
+                    ps_pull_rec(a, R, wi, rb, dg, dd + 1, acc);
 #pragma unroll
                     for (int m = 0; m < kPsV; ++m) {
                         if (v[m] < 0 || big[m]) continue;
@@ -540,14 +570,14 @@ __device__ __forceinline__ void ps_claimed(const PsArgs& a, int* Q, int w, int d
 
 // Expands up to kPsV rows at once, as ps_expand_rec: all column loads, then
 // all map words, then all claims, then the bookkeeping.
-__device__ __forceinline__ void ps_expand_bits(const PsArgs& a, uint32_t* V, int* Q, const int* rb, const int* dg,
-                                               int d, int le, int* s_cnt, PsLocal& L) {
+__device__ __forceinline__ void ps_expand_bits(const PsArgs& a, uint32_t* V, int* Q, const int (*wi)[kPsE],
+                                               const int* rb, const int* dg, int d, int le, int* s_cnt, PsLocal& L) {
     int w[kPsV][kPsE];
     bool seen[kPsV][kPsE], got[kPsV][kPsE];
 #pragma unroll
     for (int m = 0; m < kPsV; ++m)
 #pragma unroll
-        for (int e = 0; e < kPsE; ++e) w[m][e] = e < dg[m] ? a.col[rb[m] + e] : -1;
+        for (int e = 0; e < kPsE; ++e) w[m][e] = ps_nbr(a, wi[m], rb[m], dg[m], e);
 #pragma unroll
     for (int m = 0; m < kPsV; ++m)
 #pragma unroll
@@ -601,13 +631,12 @@ __global__ __launch_bounds__(kPsBlock) void k_ps_cc(PsArgs a) {
                 s_nbig[(d + 1) % 3] = 0;
             }
             for (int i0 = lb + tid; i0 < le; i0 += kPsBlock * kPsV) {
-                int rb[kPsV], dg[kPsV];
+                int rb[kPsV], dg[kPsV], wi[kPsV][kPsE];
 #pragma unroll
                 for (int m = 0; m < kPsV; ++m) {
                     const int i = i0 + m * kPsBlock;
                     const int u = i < le ? Q[i] : -1;
-                    rb[m] = u >= 0 ? a.rp[u] : 0;
-                    dg[m] = u >= 0 ? a.rp[u + 1] - rb[m] : 0;
+                    ps_row(a, u, wi[m], rb[m], dg[m]);
                     if (dg[m] > kPsBigDeg) {
                         const int slot = atomicAdd(&s_nbig[c], 1);
                         if (slot < kPsBig) {
@@ -616,7 +645,7 @@ __global__ __launch_bounds__(kPsBlock) void k_ps_cc(PsArgs a) {
                         }
                     }
                 }
-                ps_expand_bits(a, V, Q, rb, dg, d, le, &s_cnt[c], L);
+                ps_expand_bits(a, V, Q, wi, rb, dg, d, le, &s_cnt[c], L);
             }
             __syncthreads();
             const int nbig = min(s_nbig[c], kPsBig);

@@ -124,7 +124,8 @@ static int64_t workspace_estimate(const GraphEntry* g, int32_t kind, bool all, i
 }
 
 static int64_t graph_estimate(const GraphEntry* g) {
-    return 4 * (g->n + 1) + 4 * g->nnz + g->n / 4 + g->nnz / 16 + (1 << 20);
+    // CSR, the 16-byte inline rows, the id map and the chunk / hub tables
+    return 4 * (g->n + 1) + 4 * g->nnz + 16 * g->n + 4 * g->n + g->n / 4 + g->nnz / 16 + (1 << 20);
 }
 
 static int64_t reduced_estimate(const cent_reduced* r) {
