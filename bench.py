@@ -66,6 +66,7 @@ def extra_blocktree(ctx, args):
     +30% pendant chains (Geometric(0.5) lengths), seed 3; all-source closeness
     through D1 folding + block split.  Compares one persistent launch over the
     work-item queue with level-synchronous launches per work item."""
+    import numpy as np
     import bc_amd
     t0 = time.time()
     g = bc_amd.gen_blocktree(2000, 1.5, 8, 50000, 6.0, 0.3, 0.5, seed=3)
@@ -95,6 +96,28 @@ def extra_blocktree(ctx, args):
     tbi = timed(lambda: ctx.bc_reduced(red, bc_amd.CENT_SCHED_PER_ITEM), 1)
     log(f"blocktree BC+CC: persistent {tbp * 1e3:.2f} ms, per-item {tbi * 1e3:.1f} ms")
     n, m = g.n, g.m
+    # the reference's own compute_D1_AP_CC_shareBased_DegreeOrder (Seq_CC/CC.cpp:1728-2052),
+    # built from its sources into oracle/_ref, on the same graph, one host thread
+    ref_cc = os.path.join(ROOT, "oracle", "_ref", "ref_cc")
+    if os.path.exists(ref_cc):
+        import subprocess
+        import tempfile
+        with tempfile.TemporaryDirectory() as td:
+            path = os.path.join(td, "blocktree.txt")
+            bc_amd.write_edgelist(path, g)
+            t0 = time.perf_counter()
+            subprocess.run([ref_cc, "d1ap", path, os.path.join(td, "d1ap.txt")], check=True, timeout=900,
+                           stdout=subprocess.DEVNULL)
+            t_ref = time.perf_counter() - t0
+            rcc = np.loadtxt(os.path.join(td, "d1ap.txt"), dtype=np.int64)
+        res["cpu_baseline"] = {
+            "kind": "reference", "cores": 1, "s": round(t_ref, 3),
+            "sample": "whole workload: the reference's compute_D1_AP_CC_shareBased_DegreeOrder "
+                      f"(built from its sources), CPU {cpu_model()}",
+            "farness_equal": bool(np.array_equal(rcc[:, 1], far_p[rcc[:, 0]])),
+            "speedup_persistent": round(t_ref / tp, 1)}
+        log(f"blocktree: reference D1/AP closeness {t_ref:.2f} s on one core, "
+            f"farness equal: {res['cpu_baseline']['farness_equal']}")
     res.update({
         "workload": "all-source closeness, block tree 2000 blocks (D1 + AP reduced), seed 3",
         "n": n, "m": m, "blocks": red.nblocks, "articulation": red.n_ap, "folded": red.nfold,
