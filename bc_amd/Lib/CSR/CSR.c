@@ -112,10 +112,15 @@ int cent_csr_from_sorted_keys(int32_t n, const uint64_t* keys, int64_t nk, struc
     struct CSR* g = cent_csr_alloc(n, nk);
     if (!g) return CENT_E_OOM;
     memset(g->csrV, 0, sizeof(int32_t) * ((size_t)n + 1));
-    for (int64_t i = 0; i < nk; ++i) g->csrV[(keys[i] >> 32) + 1]++;
+    int multi = 0;  /* sorted keys: a repeat is adjacent, a self-loop has equal halves */
+    for (int64_t i = 0; i < nk; ++i) {
+        g->csrV[(keys[i] >> 32) + 1]++;
+        if ((i > 0 && keys[i] == keys[i - 1]) || (keys[i] >> 32) == (keys[i] & 0xffffffffu)) multi = 1;
+    }
     for (int32_t v = 0; v < n; ++v) g->csrV[v + 1] += g->csrV[v];
     for (int64_t i = 0; i < nk; ++i) g->csrE[i] = (int32_t)(keys[i] & 0xffffffffu);
     csr_finish(g);
+    g->kind = multi ? CENT_CSR_KIND_MULTI : CENT_CSR_KIND_SIMPLE;
     *out = g;
     return CENT_OK;
 }
@@ -169,6 +174,7 @@ int cent_csr_induced(const struct CSR* g, const uint8_t* keep, struct CSR** out,
         h->csrV[nid[v] + 1] = (int32_t)e;
     }
     csr_finish(h);
+    h->kind = g->kind == CENT_CSR_KIND_SIMPLE ? CENT_CSR_KIND_SIMPLE : CENT_CSR_KIND_UNKNOWN;
     if (old_id) {
         int32_t* o = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n2 > 0 ? n2 : 1));
         if (!o) { free(nid); cent_csr_free(h); return CENT_E_OOM; }
@@ -242,12 +248,18 @@ int cent_csr_from_arrays(int32_t n, const int32_t* row_ptr, const int32_t* col, 
     memcpy(g->csrV, row_ptr, sizeof(int32_t) * ((size_t)n + 1));
     if (nnz) memcpy(g->csrE, col, sizeof(int32_t) * (size_t)nnz);
     csr_finish(g);
+    g->kind = CENT_CSR_KIND_UNKNOWN;
+    const int multi = cent_csr_is_multigraph(g);
+    if (multi >= 0) g->kind = multi ? CENT_CSR_KIND_MULTI : CENT_CSR_KIND_SIMPLE;
     *out = g;
     return CENT_OK;
 }
 
-/* 1 if some row lists a vertex twice or lists its own vertex. */
+/* 1 if some row lists a vertex twice or lists its own vertex (the
+ * constructors record it in g->kind; a caller-built CSR is scanned). */
 int cent_csr_is_multigraph(const struct CSR* g) {
+    if (g->kind == CENT_CSR_KIND_SIMPLE) return 0;
+    if (g->kind == CENT_CSR_KIND_MULTI) return 1;
     const int32_t n = g->csrVSize - 1;
     int32_t* stamp = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
     if (!stamp) return -1;

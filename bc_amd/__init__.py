@@ -40,6 +40,7 @@ CENT_OPT_SOURCE_ORDER = 2
 CENT_OPT_ONE_STREAM = 4
 CENT_CSR_LCC = 1
 CENT_CSR_RAW = 2
+CENT_CSR_KIND_UNKNOWN, CENT_CSR_KIND_SIMPLE, CENT_CSR_KIND_MULTI = 0, 1, 2
 
 _lib.cent_strerror.restype = C.c_char_p
 _lib.cent_strerror.argtypes = [C.c_int]
@@ -72,6 +73,7 @@ class _CSR(C.Structure):
         ("totalNodeNumber", C.c_int32),
         ("maxDegree", C.c_int32),
         ("startAtZero", C.c_int32),
+        ("kind", C.c_int32),
     ]
 
 
@@ -120,6 +122,7 @@ class CSR:
         self.totalNodeNumber = int(s.totalNodeNumber)
         self.maxDegree = int(s.maxDegree)
         self.startAtZero = int(s.startAtZero)
+        self.kind = int(s.kind)  # CENT_CSR_KIND_*: simple graph, multigraph or not known
         self.row_ptr = np.ctypeslib.as_array(s.csrV, shape=(self.n + 1,))
         self.col = (np.ctypeslib.as_array(s.csrE, shape=(self.nnz,)) if self.nnz
                     else np.zeros(0, np.int32))

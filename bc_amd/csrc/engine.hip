@@ -267,21 +267,6 @@ extern "C" int cent_graph_create(cent_ctx* ctx, const int32_t* row_ptr, const in
         g->comp_nnz.push_back(e);
         ++nc;
     }
-    // a multigraph (a CENT_CSR_RAW input: repeated neighbours or self-loops)
-    // gets its level-1 path counts from the pull, which counts every
-    // parallel edge, instead of the push, which takes one path per neighbour
-    {
-        std::vector<int32_t> stamp((size_t)(n > 0 ? n : 1), -1);
-        for (int32_t v = 0; v < n && !g->multi; ++v)
-            for (int32_t j = row_ptr[v]; j < row_ptr[v + 1]; ++j) {
-                const int32_t y = col[j];
-                if (y == v || stamp[(size_t)y] == v) {
-                    g->multi = true;
-                    break;
-                }
-                stamp[(size_t)y] = v;
-            }
-    }
     // eccentricity probe from the highest-degree vertex (host BFS, O(n + m))
     if (n > 0) {
         int32_t s0 = 0;
@@ -355,10 +340,28 @@ extern "C" int cent_graph_create(cent_ctx* ctx, const int32_t* row_ptr, const in
             int32_t o = drp[(size_t)i];
             for (int32_t j = row_ptr[v]; j < row_ptr[v + 1]; ++j) dcol[(size_t)o++] = g->new_id[(size_t)col[j]];
             std::sort(dcol.begin() + drp[(size_t)i], dcol.begin() + o);
+            // sorted: a repeated neighbour is adjacent
+            for (int32_t j = drp[(size_t)i]; j < o; ++j)
+                if (dcol[(size_t)j] == i || (j > drp[(size_t)i] && dcol[(size_t)j] == dcol[(size_t)j - 1])) g->multi = true;
             drp[(size_t)i + 1] = o;
         }
         urp = drp.data();
         ucol = dcol.data();
+    } else {
+        // a multigraph (a CENT_CSR_RAW input: repeated neighbours or
+        // self-loops) gets its level-1 path counts from the pull, which
+        // counts every parallel edge, instead of the push, which takes one
+        // path per neighbour
+        std::vector<int32_t> stamp((size_t)(n > 0 ? n : 1), -1);
+        for (int32_t v = 0; v < n && !g->multi; ++v)
+            for (int32_t j = row_ptr[v]; j < row_ptr[v + 1]; ++j) {
+                const int32_t y = col[j];
+                if (y == v || stamp[(size_t)y] == v) {
+                    g->multi = true;
+                    break;
+                }
+                stamp[(size_t)y] = v;
+            }
     }
     g->probe_order.clear();
     g->probe_order.shrink_to_fit();

@@ -557,11 +557,14 @@ def serving(ctx, args, failures):
 EXTRA = {"rmat14": extra_rmat14, "blocktree": extra_blocktree, "blocktree-bc": extra_blocktree_bc, "grid": extra_grid,
          "closeness": extra_closeness, "reduce": extra_reduce}
 
-# Sources whose text decides the dense step's HBM traffic per launch: the
-# counter figures in profiles/rmat20_traffic.json are used only while their
-# key (these files' hash, the launch layout, the workload) matches the tree.
-TRAFFIC_SOURCES = ("bc_amd/csrc/msbfs_kernels.h", "bc_amd/csrc/wave.h", "bc_amd/csrc/records.h",
-                   "bc_amd/csrc/levels.hip", "bc_amd/csrc/engine.h", "bc_amd/Makefile")
+# Sources whose text decides the dense step's HBM traffic per launch: its
+# kernels, the level-loop section of levels.hip that launches them, and the
+# build flags.  The counter figures in profiles/rmat20_traffic.json are used
+# only while their key (this hash, the launch layout, the workload) matches
+# the tree.
+TRAFFIC_SOURCES = ("bc_amd/csrc/msbfs_kernels.h", "bc_amd/csrc/wave.h", "bc_amd/Makefile")
+TRAFFIC_SECTION = ("bc_amd/csrc/levels.hip", "// ------------------------------------------------------------ level loop",
+                   "// ------------------------------------------------------------ frontier-queue mode")
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "rmat20_traffic.json")
 
 
@@ -571,6 +574,9 @@ def kernel_sha16() -> str:
     for f in TRAFFIC_SOURCES:
         h.update(f.encode())
         h.update(open(os.path.join(ROOT, f), "rb").read())
+    f, begin, end = TRAFFIC_SECTION
+    text = open(os.path.join(ROOT, f)).read()
+    h.update(text[text.index(begin):text.index(end)].encode())
     return h.hexdigest()[:16]
 
 

@@ -67,7 +67,14 @@ struct CSR {
     int32_t  totalNodeNumber; /* endNodeID - startNodeID + 1                   */
     int32_t  maxDegree;
     int32_t  startAtZero;     /* 1 if the input ids start at 0                 */
+    int32_t  kind;            /* CENT_CSR_KIND_*, set by the constructors      */
 };
+/* struct CSR.kind: whether rows may repeat a neighbour or list their own
+ * vertex (a CENT_CSR_RAW input).  UNKNOWN (0, a CSR built by the caller)
+ * makes the engine look. */
+#define CENT_CSR_KIND_UNKNOWN 0
+#define CENT_CSR_KIND_SIMPLE  1
+#define CENT_CSR_KIND_MULTI   2
 
 /* Build from an undirected edge list (u[i], v[i]), ids in [0, n).
  * flags: CENT_CSR_LCC keeps only the largest connected component and

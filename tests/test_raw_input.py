@@ -34,6 +34,21 @@ def test_raw_loader_keeps_parallel_edges_and_self_loops():
     assert h.nnz < g.nnz and all(h.col[h.row_ptr[v]:h.row_ptr[v + 1]].tolist().count(v) == 0 for v in range(h.n))
 
 
+def test_constructors_record_the_graph_kind():
+    raw = bc_amd.buildGraph(O.dataset_path(RAW), raw=True)
+    assert raw.kind == bc_amd.CENT_CSR_KIND_MULTI
+    assert bc_amd.buildGraph(O.dataset_path(RAW)).kind == bc_amd.CENT_CSR_KIND_SIMPLE
+    # a self-loop alone, and a repeated edge alone, each make a multigraph
+    assert bc_amd.CSR.from_edges(3, [0, 1], [1, 1], raw=True).kind == bc_amd.CENT_CSR_KIND_MULTI
+    assert bc_amd.CSR.from_edges(3, [0, 1, 1], [1, 2, 0], raw=True).kind == bc_amd.CENT_CSR_KIND_MULTI
+    assert bc_amd.CSR.from_edges(3, [0, 1], [1, 2], raw=True).kind == bc_amd.CENT_CSR_KIND_SIMPLE
+    # rows handed over as arrays are scanned once
+    arr = bc_amd.CSR.from_arrays(raw.row_ptr, raw.col)
+    assert arr.kind == bc_amd.CENT_CSR_KIND_MULTI
+    assert bc_amd.CSR.from_arrays(bc_amd.gen_grid(4, 4, 0.0, 0, seed=1).row_ptr,
+                                  bc_amd.gen_grid(4, 4, 0.0, 0, seed=1).col).kind == bc_amd.CENT_CSR_KIND_SIMPLE
+
+
 def test_raw_flag_rejects_lcc():
     with pytest.raises(bc_amd.CentError):
         bc_amd.CSR.from_edges(3, [0, 1], [1, 2], lcc=True, raw=True)
