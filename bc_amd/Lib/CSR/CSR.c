@@ -12,6 +12,7 @@
  */
 #include <stdio.h>
 #include "../common.h"
+#include "../reduce.h"
 
 const char* cent_strerror(int s) {
     switch (s) {

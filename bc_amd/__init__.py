@@ -277,6 +277,19 @@ class Reduced:
                                          _ptr(out, C.c_int64)), "cent_reduced_farness")
         return out[:self.n]
 
+    def workspace_estimate(self, what: int = None, k: int = -1, sched: int = CENT_SCHED_PERSISTENT,
+                           one_stream: bool = False) -> int:
+        """cent_reduced_workspace_estimate: device bytes a context allocates
+        at most for cent_cc_reduced (CENT_TUNE_CC, the default),
+        cent_bc_reduced (CENT_TUNE_BC, k = -1) or cent_bc_reduced_sources
+        (k sources)."""
+        what = CENT_TUNE_CC if what is None else what
+        out = C.c_int64()
+        o = _opts(one_stream=one_stream)
+        _check(_lib.cent_reduced_workspace_estimate(self._h, what, k, sched, C.addressof(o), C.byref(out)),
+               "cent_reduced_workspace_estimate")
+        return int(out.value)
+
     def source_weights(self, sources) -> np.ndarray:
         """WS[x]: sources (with repeats) hanging off packed copy x (pw for all vertices once)."""
         src = np.ascontiguousarray(sources, dtype=np.int32)
@@ -534,6 +547,39 @@ def _opts(mode=CENT_BC_TEXTBOOK, timing=False, grid_waves=0, frontier="auto", ma
     return o
 
 
+_lib.cent_workspace_estimate.argtypes = [C.c_void_p, C.c_int32, C.c_int64, C.c_int32, C.c_int32, C.c_int32,
+                                         C.c_int32, C.c_void_p, C.POINTER(C.c_int64)]
+_lib.cent_graph_workspace_estimate.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.POINTER(C.c_int64)]
+_lib.cent_reduced_workspace_estimate.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
+                                                 C.POINTER(C.c_int64)]
+
+
+def workspace_estimate(csr: CSR, what: int = CENT_TUNE_BC, k: int = -1, depth: int = -1, tuned: bool = True,
+                       mode: int = CENT_BC_TEXTBOOK, grid_waves: int = 0, frontier: str = "auto",
+                       max_slots: int = 0, one_stream: bool = False) -> int:
+    """Device bytes a context allocates at most for one plain call (cent_bc
+    for CENT_TUNE_BC, cent_cc for CENT_TUNE_CC) with k sources (-1: all) on
+    csr before it is uploaded (cent_workspace_estimate).  depth: BFS depth
+    from the highest-degree vertex, -1 if not known."""
+    out = C.c_int64()
+    o = _opts(mode, False, grid_waves, frontier, max_slots, False, one_stream)
+    _check(_lib.cent_workspace_estimate(csr.row_ptr.ctypes.data, csr.n, csr.nnz, depth,
+                                        0 if tuned else CENT_GRAPH_NO_TUNE, what, k, C.addressof(o),
+                                        C.byref(out)), "cent_workspace_estimate")
+    return int(out.value)
+
+
+_lib.cent_reduce_gpu_workspace_estimate.argtypes = [C.c_int32, C.c_int64, C.POINTER(C.c_int64)]
+
+
+def reduce_gpu_workspace_estimate(csr: CSR) -> int:
+    """Device scratch cent_reduce_gpu takes for csr (cent_reduce_gpu_workspace_estimate)."""
+    out = C.c_int64()
+    _check(_lib.cent_reduce_gpu_workspace_estimate(csr.n, csr.nnz, C.byref(out)),
+           "cent_reduce_gpu_workspace_estimate")
+    return int(out.value)
+
+
 class DeviceArray:
     """n elements of a numpy dtype in device memory of a context's GPU
     (cent_device_alloc), freed with the object."""
@@ -607,6 +653,17 @@ class DeviceGraph:
     def set_layout(self, p: TuneParams, what: int = CENT_TUNE_BC | CENT_TUNE_CC) -> None:
         _check(_lib.cent_graph_set_layout(self._h, what, C.byref(p)), "cent_graph_set_layout",
                self.ctx.last_error())
+
+    def workspace_estimate(self, what: int = CENT_TUNE_BC, k: int = -1, mode: int = CENT_BC_TEXTBOOK,
+                           grid_waves: int = 0, frontier: str = "auto", max_slots: int = 0,
+                           one_stream: bool = False) -> int:
+        """cent_graph_workspace_estimate: device bytes a context allocates at
+        most for one plain call on this graph."""
+        out = C.c_int64()
+        o = _opts(mode, False, grid_waves, frontier, max_slots, False, one_stream)
+        _check(_lib.cent_graph_workspace_estimate(self._h, what, k, C.addressof(o), C.byref(out)),
+               "cent_graph_workspace_estimate")
+        return int(out.value)
 
     @property
     def signature(self) -> tuple:

@@ -112,11 +112,15 @@ static inline void set_error(cent_ctx* ctx, hipError_t e, const char* what) {
     snprintf(ctx->last_error, sizeof(ctx->last_error), "%s: %s", what, hipGetErrorString(e));
 }
 
+// Capacity ensure() gives a buffer asked for n elements (headroom, so a
+// slightly larger next call does not re-allocate).
+static inline size_t ensure_cap(size_t n) { return n + n / 4 + 64; }
+
 template <class T>
 static inline int ensure(cent_ctx* ctx, DevBuf<T>& b, size_t n) {
     if (n <= b.cap) return CENT_OK;
     if (b.p) { HIPTRY(hipFree(b.p)); b.p = nullptr; b.cap = 0; }
-    size_t cap = n + n / 4 + 64;
+    size_t cap = ensure_cap(n);
     HIPTRY(hipMalloc((void**)&b.p, cap * sizeof(T)));
     b.cap = cap;
     return CENT_OK;
@@ -229,7 +233,7 @@ static inline GView sub_view(const DevCSR& d, int lo, int hi) {
 }
 
 struct cent_graph {
-    cent_ctx* ctx = nullptr;
+    int device = 0;                   // GPU holding the device copy (the graph does not keep its context)
     int32_t n = 0;
     int64_t nnz = 0;
     DevCSR d;
@@ -331,17 +335,39 @@ struct BatchSpec {
 // and one row scan serves 2048 sources).
 constexpr int kMultiBatch = 64;
 
+// The context's peer (second stream and workspace on the same GPU), created
+// on first use, with the two join events.
+CENT_INTERNAL int peer_get(cent_ctx* ctx, cent_ctx** out);
+// Zeroes the context's sticky path-count overflow flag (flags[1]), which
+// every level step of a call sets and the call reads at its end.
+CENT_INTERNAL int level_flags_reset(cent_ctx* ctx);
 // Batches of the k sources d_src[0..k) (device ids) inside the view: one
 // 64-lane batch for BC (k <= 64), up to kMultiBatch for closeness.
 // bound > 0: every BFS of the batch is known to end within `bound` levels
 // (levels 0 .. bound - 1 carry frontiers), so the whole batch is enqueued
 // without a host check; its depth and a bound failure are added to
 // ctx->lvstat on the device.  bound == 0: levels in host-checked runs.
-CENT_INTERNAL // The context's peer (second stream and workspace on the same GPU), created
-// on first use, with the two join events.
-int peer_get(cent_ctx* ctx, cent_ctx** out);
-// Zeroes the context's sticky path-count overflow flag (flags[1]), which
-// every level step of a call sets and the call reads at its end.
-int level_flags_reset(cent_ctx* ctx);
-int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d_src, int k,
+CENT_INTERNAL int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d_src, int k,
                               int64_t* far_out, KernelTimer& tm, int64_t* levels, int bound = 0);
+
+// ------------------------------------------------------------ workspace estimate
+// Device bytes of a context's buffers, added in the order a call allocates
+// them (a buffer that grows is re-allocated before the old one is freed):
+// `cur` is what the context holds after the call, `peak` the most at once.
+struct WsEst {
+    size_t cur = 0, peak = 0;
+    void add(size_t b) { cur += b; peak = std::max(peak, cur); }
+    void sub(size_t b) { cur -= b; }
+};
+// Dense 64-lane step on a DevCSR of N vertices with nseg hub segments
+// (ensure_workspace, then grow_levels until the level store holds `levels`
+// levels for betweenness): nb batches a launch group, G workgroups a launch.
+CENT_INTERNAL void ws_dense(WsEst& w, size_t N, size_t nseg, bool with_bc, bool compat, int nb, int G,
+                            size_t levels);
+// Workgroups of a dense level launch over nchunk vertex chunks (grid_waves:
+// the call's or the graph's cap, 0 = default).
+CENT_INTERNAL int ws_grid(size_t nchunk, int grid_waves);
+// Level store a dense betweenness call may need on a graph whose probe BFS
+// (from the highest-degree vertex) is `depth` levels deep (-1: unknown, the
+// depth up to which the dense step is chosen by default).
+CENT_INTERNAL size_t ws_levels(int32_t depth);

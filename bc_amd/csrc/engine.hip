@@ -230,7 +230,7 @@ static int validate_csr(const int32_t* row_ptr, const int32_t* col, int32_t n, i
 
 extern "C" void cent_graph_free(cent_graph* g) {
     if (!g) return;
-    if (g->ctx) (void)hipSetDevice(g->ctx->device);
+    (void)hipSetDevice(g->device);
     devcsr_free(g->d);
     if (g->d_new_id) (void)hipFree(g->d_new_id);
     delete g;
@@ -245,7 +245,7 @@ extern "C" int cent_graph_create(cent_ctx* ctx, const int32_t* row_ptr, const in
     TRY(check_device(ctx));
     cent_graph* g = new (std::nothrow) cent_graph();
     if (!g) return CENT_E_OOM;
-    g->ctx = ctx;
+    g->device = ctx->device;
     g->n = n;
     g->nnz = nnz;
     // component ids + per-component directed edge counts, for TEPS accounting
@@ -437,8 +437,8 @@ extern "C" int cent_graph_set_layout(cent_graph* g, int32_t what, const cent_tun
     if (!g || !what_ok(what, true)) return CENT_E_INVAL;
     TRY(cent_tune_params_check(p));
     if (p->hub_deg != g->d.hub_deg || p->chunk_edges != g->d.chunk_edges) {
-        cent_ctx* ctx = g->ctx;
-        TRY(check_device(ctx));
+        cent_ctx* ctx = nullptr;  // errors have no context to record them in
+        HIPTRY(hipSetDevice(g->device));
         HIPTRY(hipDeviceSynchronize());  // no launch may still read the old tables
         TRY(devcsr_schedule(ctx, g->h_rp.data(), nullptr, 0, p->hub_deg, p->chunk_edges, g->d));
     }

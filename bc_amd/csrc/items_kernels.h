@@ -120,7 +120,7 @@ namespace cent {
 // Persistent weighted Brandes over the same work items (reduced graphs).
 // Frontier masks stay in LDS as in k_cc_items; the per-(vertex, lane) path
 // counts, coefficients and levels of the item's range live in a per-
-// workgroup slice of HBM scratch (kItemLds x 64 entries each).  Outputs:
+// workgroup slice of HBM scratch (64 entries a vertex of the largest item).  Outputs:
 //   D[s]                      weighted farness of every source (assigned)
 //   bcpart[poff[item] + i]    the item's weighted dependencies at vertex lo+i
 //                             (added across the item's levels; the host sums
@@ -138,9 +138,10 @@ struct BcItemArgs {
     int* __restrict__ head;
     int64_t* __restrict__ D;
     double* __restrict__ bcpart;
-    double* __restrict__ sig;             // [grid][kItemLds * 64]
-    double* __restrict__ coef;            // [grid][kItemLds * 64]
-    uint8_t* __restrict__ lvl;            // [grid][kItemLds * 64], 255 = unreached
+    double* __restrict__ sig;             // [grid][stride]
+    double* __restrict__ coef;            // [grid][stride]
+    uint8_t* __restrict__ lvl;            // [grid][stride], 255 = unreached
+    size_t stride;                        // entries a workgroup: 64 x the largest item range
     int* __restrict__ deep;               // [nitems] item indices
     int* __restrict__ ndeep;
     int* __restrict__ overflow;           // set when a path count is inf
@@ -164,9 +165,9 @@ __global__ __launch_bounds__(kBcItemBlock) void k_bc_items(BcItemArgs a) {
     const int lane = lane_id();
     const int wid = tid >> 6;
     constexpr int kW = kBcItemBlock / kWave;
-    double* __restrict__ sig = a.sig + (size_t)blockIdx.x * kItemLds * kWave;
-    double* __restrict__ coef = a.coef + (size_t)blockIdx.x * kItemLds * kWave;
-    uint8_t* __restrict__ lvl = a.lvl + (size_t)blockIdx.x * kItemLds * kWave;
+    double* __restrict__ sig = a.sig + blockIdx.x * a.stride;
+    double* __restrict__ coef = a.coef + blockIdx.x * a.stride;
+    uint8_t* __restrict__ lvl = a.lvl + blockIdx.x * a.stride;
     for (;;) {
         if (tid == 0) s_item = atomicAdd(a.head, 1);
         __syncthreads();

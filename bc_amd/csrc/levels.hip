@@ -948,7 +948,7 @@ static int prepare_sources(const cent_graph* g, const int32_t* src, int32_t k, s
 
 extern "C" int cent_bc(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k, double* bc,
                        int64_t* farness, int64_t* dist_sum, const cent_opts* opts) {
-    if (!ctx || !g || g->ctx->device != ctx->device) return CENT_E_INVAL;
+    if (!ctx || !g || g->device != ctx->device) return CENT_E_INVAL;
     int mode = opts ? opts->bc_mode : CENT_BC_TEXTBOOK;
     if (mode != CENT_BC_TEXTBOOK && mode != CENT_BC_REF_COMPAT) return CENT_E_INVAL;
     std::vector<int32_t> all;
@@ -964,7 +964,7 @@ extern "C" int cent_bc(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
 
 extern "C" int cent_bc_device(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k, double* d_bc,
                               int64_t* d_farness, int64_t* d_dist_sum, const cent_opts* opts) {
-    if (!ctx || !g || g->ctx->device != ctx->device) return CENT_E_INVAL;
+    if (!ctx || !g || g->device != ctx->device) return CENT_E_INVAL;
     int mode = opts ? opts->bc_mode : CENT_BC_TEXTBOOK;
     if (mode != CENT_BC_TEXTBOOK && mode != CENT_BC_REF_COMPAT) return CENT_E_INVAL;
     std::vector<int32_t> all;
@@ -983,7 +983,7 @@ extern "C" int cent_bc_device(cent_ctx* ctx, const cent_graph* g, const int32_t*
 
 extern "C" int cent_cc_device(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k,
                               int64_t* d_farness, int64_t* d_dist_sum, const cent_opts* opts) {
-    if (!ctx || !g || g->ctx->device != ctx->device) return CENT_E_INVAL;
+    if (!ctx || !g || g->device != ctx->device) return CENT_E_INVAL;
     std::vector<int32_t> all;
     const int32_t* s;
     int32_t kk;
@@ -999,7 +999,7 @@ extern "C" int cent_cc_device(cent_ctx* ctx, const cent_graph* g, const int32_t*
 
 extern "C" int cent_cc(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k, int64_t* farness,
                        int64_t* dist_sum, const cent_opts* opts) {
-    if (!ctx || !g || g->ctx->device != ctx->device) return CENT_E_INVAL;
+    if (!ctx || !g || g->device != ctx->device) return CENT_E_INVAL;
     std::vector<int32_t> all;
     const int32_t* s;
     int32_t kk;
@@ -1063,7 +1063,7 @@ __global__ void k_iota(int32_t* __restrict__ ids, int k) {
 
 extern "C" int cent_cc_device_noalloc(cent_ctx* ctx, const cent_graph* g, const int32_t* src, int32_t k,
                                       int64_t* d_farness, int64_t* d_dist_sum) {
-    if (!ctx || !g || g->ctx->device != ctx->device || k < -1 || (k > 0 && !src)) return CENT_E_INVAL;
+    if (!ctx || !g || g->device != ctx->device || k < -1 || (k > 0 && !src)) return CENT_E_INVAL;
     const int32_t n = g->n;
     const bool all = k == -1;  // every vertex, farness indexed by vertex
     if (all) k = n;
@@ -1166,7 +1166,7 @@ struct Tuner {
 
 extern "C" int cent_tune(cent_ctx* ctx, cent_graph* g, const int32_t* src, int32_t k, int32_t what,
                          int32_t max_trials, cent_tune_params* best) {
-    if (!ctx || !g || !best || g->ctx->device != ctx->device) return CENT_E_INVAL;
+    if (!ctx || !g || !best || g->device != ctx->device) return CENT_E_INVAL;
     if (what != CENT_TUNE_BC && what != CENT_TUNE_CC) return CENT_E_INVAL;
     if (k < 1 || k > (what == CENT_TUNE_CC ? 1024 : 256) || !src || max_trials < 0) return CENT_E_INVAL;
     for (int32_t i = 0; i < k; ++i)
@@ -1222,5 +1222,172 @@ extern "C" int cent_tune(cent_ctx* ctx, cent_graph* g, const int32_t* src, int32
     TRY(cent_graph_set_layout(g, what, &cur));
     TRY(cent_tune_table_put(&g->sig, what, &cur));
     *best = cur;
+    return CENT_OK;
+}
+
+// ------------------------------------------------------------ workspace estimate
+// The allocations of run_batches and the level steps, replayed on a WsEst.
+void ws_dense(WsEst& w, size_t N, size_t nseg, bool bc, bool compat, int nb, int G, size_t levels) {
+    const size_t W = bc ? 1 : (size_t)ccw_width(nb);
+    w.add(8 * ensure_cap(N * W));                                   // vis
+    size_t fcap = ensure_cap(bc ? 2 * N * 17 : N * (2 * W + 1));     // F
+    w.add(8 * fcap);
+    size_t fl = fcap / (N * (bc ? 2 : 1)) - 1;
+    if (bc) {
+        w.add(4 * (fl + 2) * kWave);                                 // lexp
+        w.add(8 * ensure_cap(N * kWave));                            // sigma (coef written over it)
+        if (compat) w.add(4 * ensure_cap(N * kWave));                // npred
+    }
+    w.add(8 * ensure_cap((size_t)G * kWave * nb) + 8 * ensure_cap((size_t)G * 2 * nb));  // part_far, part_cnt
+    w.add(8 * ensure_cap(2) + 4 * ensure_cap(2));                    // cnt, flags
+    if (nseg) w.add(8 * ensure_cap(nseg * kWave) + 4 * ensure_cap(nseg * kWave) + 8 * ensure_cap(nseg * W));
+    while (bc && fl < levels) {                                      // grow_levels
+        const size_t ncap = fl * 2, nf = 2 * N * (ncap + 1);
+        w.add(8 * nf);
+        w.sub(8 * fcap);
+        w.add(4 * (ncap + 2) * kWave);
+        w.sub(4 * (fl + 2) * kWave);
+        fcap = nf;
+        fl = ncap;
+    }
+}
+
+int ws_grid(size_t nchunk, int grid_waves) {
+    cent_opts o{};
+    o.grid_waves = grid_waves;
+    return grid_for((int)std::min<size_t>(nchunk, INT32_MAX), &o);
+}
+
+size_t ws_levels(int32_t depth) {
+    // a bounded launch group needs its bound + 1 levels (bound <= 2 depth + 1
+    // in the probe's component); host-checked runs overshoot the deepest
+    // level by up to a run of kHostFlags - 2 levels
+    const size_t d = depth >= 0 ? (size_t)depth : (size_t)kDeepLevels;
+    return std::max(2 * d + 2, d + kHostFlags);
+}
+
+// Peak workspace of one plain call (cent_bc / cent_cc and the _device forms)
+// on a graph of n vertices with nseg hub segments and nchunk vertex chunks,
+// probe depth `depth` (-1: unknown), tuned grid_waves / frontier for the
+// call type (0 / AUTO: none).
+static size_t ws_plain(int32_t n, size_t nseg, size_t nchunk, int32_t depth, int32_t gw_tuned,
+                       int32_t fr_tuned, bool bc, int32_t k, const cent_opts* opts) {
+    const size_t N = (size_t)(n > 0 ? n : 1);
+    if (k < 0) k = n;
+    const size_t K = (size_t)(k > 0 ? k : 1);
+    cent_opts eff{};
+    if (opts) eff = *opts;
+    if (eff.grid_waves == 0) eff.grid_waves = gw_tuned;
+    if (eff.frontier == CENT_FRONTIER_AUTO) eff.frontier = fr_tuned;
+    const bool compat = bc && eff.bc_mode == CENT_BC_REF_COMPAT;
+    WsEst base;
+    if (bc) base.add(8 * ensure_cap(N));                             // bc
+    base.add(8 * ensure_cap(N));                                     // dist_sum
+    base.add(8 * ensure_cap(K) + 4 * ensure_cap(K) + 4 * ensure_cap(K));  // far, src, ord
+    base.add(8 * ensure_cap(2) + 8 * ensure_cap(4) + 4 * ensure_cap(kHostFlags));  // cnt, lvstat, lvf
+    std::vector<int> modes;
+    if (eff.frontier != CENT_FRONTIER_AUTO) modes.push_back(eff.frontier);
+    else if (depth >= 0)
+        modes.push_back(depth > kDeepLevels || (bc && n <= kSmallBcVertices) ? CENT_FRONTIER_SOURCE
+                                                                           : CENT_FRONTIER_DENSE);
+    else
+        modes = {CENT_FRONTIER_SOURCE, CENT_FRONTIER_DENSE};
+    const size_t levels = ws_levels(depth);
+    const int G = ws_grid(nchunk, eff.grid_waves);
+    // dense step of the call (with the peer context's copy on two streams)
+    auto dense = [&](WsEst& w, bool may_dual) {
+        int gb = kMultiBatch;
+        const int nbt = (int)((K + kWave - 1) / kWave);
+        while (gb > 8 && (nbt + gb - 1) / gb < 4) gb /= 2;
+        const int nb = bc ? 1 : std::min(gb, nbt);
+        ws_dense(w, N, nseg, bc, compat, nb, G, levels);
+        const size_t step = bc ? (size_t)kWave : (size_t)kWave * gb;
+        if (may_dual && !(eff.flags & CENT_OPT_ONE_STREAM) && K > step) {
+            WsEst p;
+            if (bc) p.add(8 * ensure_cap(N));
+            p.add(8 * ensure_cap(N) + 8 * ensure_cap(2) + 8 * ensure_cap(4));
+            ws_dense(p, N, nseg, bc, compat, nb, G, levels);
+            w.add(p.peak);
+        }
+    };
+    size_t best = 0;
+    for (int m : modes) {
+        WsEst w = base;
+        if (m == CENT_FRONTIER_SOURCE) {
+            const size_t lcap = std::min<size_t>(N + 2, 65536);
+            const size_t nw = ((N + 31) / 32 + 3) / 4 * 4;
+            const size_t per_slot = bc ? N * (sizeof(PsRec) + 4 + 8) + lcap * 8 : N * 4 + nw * 4;
+            size_t cap = kPsDefaultSlots;
+            if (eff.max_slots > 0) cap = (size_t)eff.max_slots;
+            else if (eff.grid_waves > 0) cap = (size_t)std::max(1, eff.grid_waves / kPsWaves);
+            else
+                while (cap < (size_t)kPsMaxCachedSlots && cap * 2 * per_slot <= kPsSlotSetBytes) cap *= 2;
+            const size_t G2 = std::max<size_t>(1, std::min(K, cap));  // the free-memory cap only lowers it
+            w.add(per_slot * G2 + 8 * ensure_cap(1) + 4 * ensure_cap(2));
+            // a betweenness BFS deeper than the level table reruns as batches
+            if (bc && (depth < 0 ? false : 2 * (size_t)depth + 2 >= lcap)) dense(w, true);
+        } else if (m == CENT_FRONTIER_SPARSE) {
+            const size_t G2 = (size_t)std::max(1, std::min((int)((N + kBlock - 1) / kBlock), 1024));
+            const size_t qcap = std::min<size_t>(N * kWave, (size_t)INT32_MAX);
+            w.add(8 * ensure_cap(N) + 8 * ensure_cap(2 * N) + 4 * ensure_cap(qcap) + 8 * ensure_cap(qcap));
+            w.add(4 * ensure_cap(N + 3) + 8 * ensure_cap(N) + 4 * ensure_cap(kCtlSlots) + 8 * ensure_cap(1));
+            w.add(8 * ensure_cap(G2 * kWave) + 8 * ensure_cap(G2 * 2));
+            if (bc) {
+                w.add(16 * ensure_cap(N * kWave) + (compat ? 4 * ensure_cap(N * kWave) : 0));
+                w.add(4 * (std::min<size_t>(N + 3, kSpLevels) + 2) * kWave);
+            }
+            // a batch deeper than the exponent table reruns on the dense step
+            if (bc && (depth < 0 || 2 * (size_t)depth + 2 >= kSpLevels)) dense(w, false);
+        } else {
+            dense(w, true);
+        }
+        best = std::max(best, w.peak);
+    }
+    return best;
+}
+
+static size_t ws_nseg(const int32_t* rp, int32_t n, int hub_deg, size_t* nchunk) {
+    size_t nseg = 0;
+    for (int32_t v = 0; v < n; ++v) {
+        const size_t d = (size_t)(rp[v + 1] - rp[v]);
+        if (d > (size_t)hub_deg) nseg += (d + hub_deg - 1) / hub_deg;
+    }
+    *nchunk = (size_t)(n > 0 ? n : 1);  // at least one vertex a chunk
+    return nseg;
+}
+
+extern "C" int cent_workspace_estimate(const int32_t* row_ptr, int32_t n, int64_t nnz, int32_t depth,
+                                       int32_t gflags, int32_t what, int32_t k, const cent_opts* opts,
+                                       int64_t* bytes) {
+    if (!bytes || n < 0 || nnz < 0 || (n > 0 && !row_ptr) || (what != CENT_TUNE_BC && what != CENT_TUNE_CC) ||
+        k < -1)
+        return CENT_E_INVAL;
+    // the layout cent_graph_create gives the graph: table entries for the
+    // signature (the betweenness entry's hub threshold wins), else defaults
+    int hub_deg = kHubDeg;
+    cent_tune_params mine{};
+    mine.frontier = CENT_FRONTIER_AUTO;
+    if (!(gflags & CENT_GRAPH_NO_TUNE) && depth >= 0) {
+        cent_tune_sig sig;
+        TRY(cent_tune_signature(row_ptr, n, nnz, depth, &sig));
+        for (int32_t w : {CENT_TUNE_CC, CENT_TUNE_BC}) {
+            cent_tune_params tp;
+            if (cent_tune_table_get(&sig, w, &tp) != CENT_OK) continue;
+            hub_deg = tp.hub_deg;
+            if (w == what) mine = tp;
+        }
+    }
+    size_t nchunk = 0;
+    const size_t nseg = ws_nseg(row_ptr, n, hub_deg, &nchunk);
+    *bytes = (int64_t)ws_plain(n, nseg, nchunk, depth, mine.grid_waves, mine.frontier, what == CENT_TUNE_BC, k, opts);
+    return CENT_OK;
+}
+
+extern "C" int cent_graph_workspace_estimate(const cent_graph* g, int32_t what, int32_t k, const cent_opts* opts,
+                                             int64_t* bytes) {
+    if (!g || !bytes || (what != CENT_TUNE_BC && what != CENT_TUNE_CC) || k < -1) return CENT_E_INVAL;
+    const int i = what == CENT_TUNE_BC ? 0 : 1;
+    *bytes = (int64_t)ws_plain(g->n, g->d.nhub ? (size_t)g->d.nseg : 0, (size_t)std::max(1, g->d.nchunk),
+                               g->ecc_probe, g->grid_waves[i], g->frontier[i], what == CENT_TUNE_BC, k, opts);
     return CENT_OK;
 }

@@ -1155,3 +1155,13 @@ extern "C" int cent_reduce_gpu(cent_ctx* ctx, const struct CSR* csr, cent_reduce
     *out = r;
     return CENT_OK;
 }
+
+// The reduction's scratch (the context's arena afterwards): about 45
+// per-vertex arrays of at most 8 bytes, the copies' and candidates' arrays
+// (at most 2n entries each), the packed rows' sort keys (two 8-byte keys an
+// edge), the rows themselves and the sorts' temporary storage.
+extern "C" int cent_reduce_gpu_workspace_estimate(int32_t n, int64_t nnz, int64_t* bytes) {
+    if (!bytes || n < 0 || nnz < 0) return CENT_E_INVAL;
+    *bytes = 640 * (int64_t)n + 48 * nnz + ((int64_t)32 << 20);
+    return CENT_OK;
+}

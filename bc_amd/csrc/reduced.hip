@@ -11,13 +11,14 @@
 // Device copy of a packed block graph, cached on the cent_reduced object
 // (freed through its dev_free hook).
 struct RedDev {
-    cent_ctx* ctx = nullptr;
+    int device = 0;           // GPU holding it (no context is kept: it may be gone by the next call)
     DevCSR d;
     int64_t* w = nullptr;     // [np]
     int* iota = nullptr;      // [np] packed ids 0..np-1 (batch sources)
     int4* items = nullptr;    // [nitems]
     int* order = nullptr;     // [nsmall] small items, largest range first
     int nsmall = 0;
+    int max_small = 1;        // largest range of a small item
     std::vector<int> big;     // items handled by the level-synchronous kernels
     int64_t* poff = nullptr;  // [nitems] offsets of the items' BC partials (small items)
     int64_t part_total = 0;
@@ -39,7 +40,7 @@ struct RedDev {
 static void reddev_free(void* p) {
     RedDev* rd = (RedDev*)p;
     if (!rd) return;
-    if (rd->ctx) (void)hipSetDevice(rd->ctx->device);
+    (void)hipSetDevice(rd->device);
     devcsr_free(rd->d);
     if (rd->w) (void)hipFree(rd->w);
     if (rd->iota) (void)hipFree(rd->iota);
@@ -61,13 +62,13 @@ static int reddev_get(cent_ctx* ctx, cent_reduced* r, RedDev** out) {
     std::lock_guard<std::mutex> lk(g_reddev_mu);
     if (r->dev) {
         RedDev* rd = (RedDev*)r->dev;
-        if (rd->ctx->device != ctx->device) return CENT_E_INVAL;
+        if (rd->device != ctx->device) return CENT_E_INVAL;
         *out = rd;
         return CENT_OK;
     }
     RedDev* rd = new (std::nothrow) RedDev();
     if (!rd) return CENT_E_OOM;
-    rd->ctx = ctx;
+    rd->device = ctx->device;
     // built privately and published on r only once every upload succeeded:
     // a half-built copy (no items uploaded) must never be found by a later call
     const int st = reddev_build(ctx, r, rd);
@@ -100,6 +101,7 @@ static int reddev_build(cent_ctx* ctx, const cent_reduced* r, RedDev* rd) {
         return (r->items[a].hi - r->items[a].lo) > (r->items[b].hi - r->items[b].lo);
     });
     rd->nsmall = (int)small.size();
+    if (!small.empty()) rd->max_small = std::max(1, r->items[small[0]].hi - r->items[small[0]].lo);
     TRY(upload(ctx, &rd->items, items.data(), (size_t)r->nitems));
     TRY(upload(ctx, &rd->order, small.data(), small.size()));
     rd->h_poff.assign((size_t)(r->nitems > 0 ? r->nitems : 1), -1);
@@ -409,7 +411,7 @@ extern "C" int cent_bc_reduced(cent_ctx* ctx, const cent_reduced* rc, double* bc
     const bool persistent = (sched == CENT_SCHED_PERSISTENT) && rd->nsmall > 0;
     if (persistent) {
         const int grid = std::min(rd->nsmall, kBcItemGrid);
-        const size_t slot = (size_t)kItemLds * kWave;
+        const size_t slot = (size_t)rd->max_small * kWave;
         TRY(ensure(ctx, ctx->it_sig, slot * grid));
         TRY(ensure(ctx, ctx->it_coef, slot * grid));
         TRY(ensure(ctx, ctx->it_lvl, slot * grid));
@@ -419,7 +421,7 @@ extern "C" int cent_bc_reduced(cent_ctx* ctx, const cent_reduced* rc, double* bc
         HIPTRY(hipMemsetAsync(ctx->flags.p, 0, sizeof(int) * 2, s));
         BcItemArgs ba{};
         ba.rp = rd->d.rp; ba.col = rd->d.col; ba.weight = rd->w; ba.items = rd->items; ba.order = rd->order;
-        ba.poff = rd->poff; ba.nitems = rd->nsmall; ba.head = ctx->red_head.p; ba.D = Dd;
+        ba.poff = rd->poff; ba.nitems = rd->nsmall; ba.head = ctx->red_head.p; ba.D = Dd; ba.stride = slot;
         ba.bcpart = ctx->red_part.p; ba.sig = ctx->it_sig.p; ba.coef = ctx->it_coef.p; ba.lvl = ctx->it_lvl.p;
         ba.deep = ctx->red_deep.p; ba.ndeep = ctx->red_head.p + 1; ba.overflow = ctx->flags.p + 1;
         hipEvent_t e0;
@@ -588,7 +590,9 @@ extern "C" int cent_bc_reduced_sources(cent_ctx* ctx, const cent_reduced* rc, co
     const bool persistent = sched == CENT_SCHED_PERSISTENT && !items.empty();
     if (persistent) {
         const int grid = std::min((int)items.size(), kBcItemGrid);
-        const size_t slot = (size_t)kItemLds * kWave;
+        int max_range = 1;
+        for (const int4& it : items) max_range = std::max(max_range, it.y - it.x);
+        const size_t slot = (size_t)max_range * kWave;
         TRY(ensure(ctx, ctx->it_sig, slot * grid));
         TRY(ensure(ctx, ctx->it_coef, slot * grid));
         TRY(ensure(ctx, ctx->it_lvl, slot * grid));
@@ -596,7 +600,7 @@ extern "C" int cent_bc_reduced_sources(cent_ctx* ctx, const cent_reduced* rc, co
         BcItemArgs ba{};
         ba.rp = rd->d.rp; ba.col = rd->d.col; ba.weight = rd->w; ba.items = ctx->red_items.p;
         ba.order = ctx->red_order.p; ba.poff = ctx->red_poff.p; ba.nitems = (int)items.size();
-        ba.head = ctx->red_head.p; ba.D = Dd; ba.bcpart = ctx->red_part.p; ba.sig = ctx->it_sig.p;
+        ba.head = ctx->red_head.p; ba.D = Dd; ba.bcpart = ctx->red_part.p; ba.sig = ctx->it_sig.p; ba.stride = slot;
         ba.coef = ctx->it_coef.p; ba.lvl = ctx->it_lvl.p; ba.deep = ctx->red_deep.p; ba.ndeep = ctx->red_head.p + 1;
         ba.overflow = ctx->flags.p + 1; ba.srcs = ctx->red_src.p; ba.swgt = ctx->red_swgt.p;
         hipEvent_t e0;
@@ -649,5 +653,76 @@ extern "C" int cent_bc_reduced_sources(cent_ctx* ctx, const cent_reduced* rc, co
     ctx->stats.levels += levels;
     ctx->stats.reached += k;
     ctx->stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return CENT_OK;
+}
+
+// ------------------------------------------------------------ workspace estimate
+// The allocations of cent_cc_reduced / cent_bc_reduced /
+// cent_bc_reduced_sources replayed on a WsEst (levels.hip); the reduced
+// graph's own device copy is not included.
+extern "C" int cent_reduced_workspace_estimate(const cent_reduced* r, int32_t what, int32_t k, int32_t sched,
+                                               const cent_opts* opts, int64_t* bytes) {
+    if (!r || !bytes || (what != CENT_TUNE_BC && what != CENT_TUNE_CC) || k < -1 ||
+        (sched != CENT_SCHED_PERSISTENT && sched != CENT_SCHED_PER_ITEM && sched != CENT_SCHED_SHARED))
+        return CENT_E_INVAL;
+    const size_t NP = (size_t)(r->np > 0 ? r->np : 1);
+    size_t nseg = 0;
+    for (int32_t x = 0; x < r->np; ++x) {
+        const size_t d = (size_t)(r->prp[x + 1] - r->prp[x]);
+        if (d > (size_t)kHubDeg) nseg += (d + kHubDeg - 1) / kHubDeg;
+    }
+    size_t nsmall = 0, nbig = 0, max_small = 1;
+    for (int32_t i = 0; i < r->nitems; ++i) {
+        const size_t R = (size_t)(r->items[i].hi - r->items[i].lo);
+        if (R <= (size_t)kItemLds) {
+            ++nsmall;
+            max_small = std::max(max_small, R);
+        } else {
+            ++nbig;
+        }
+    }
+    const int G = ws_grid(NP, opts ? opts->grid_waves : 0);
+    const size_t levels = ws_levels(-1);
+    const bool one = opts && (opts->flags & CENT_OPT_ONE_STREAM);
+    WsEst w;
+    w.add(8 * ensure_cap(2) + 4 * ensure_cap(2) + 8 * ensure_cap(4) + 4 * ensure_cap(kHostFlags));  // cnt, flags, lvstat, lvf
+    if (what == CENT_TUNE_CC) {
+        w.add(8 * ensure_cap(NP) + 4 * ensure_cap(1));  // red_D, red_head
+        if (sched == CENT_SCHED_PER_ITEM) {
+            ws_dense(w, NP, nseg, false, false, 1, G, levels);
+        } else if (nbig) {
+            // groups of up to kMultiBatch big items, alternating over two streams
+            const int nb = (int)std::min<size_t>(nbig, kMultiBatch);
+            ws_dense(w, NP, nseg, false, false, nb, G, levels);
+            if (nbig >= 2 && !one) {
+                WsEst p;
+                p.add(8 * ensure_cap(2) + 8 * ensure_cap(4));
+                ws_dense(p, NP, nseg, false, false, nb, G, levels);
+                w.add(p.peak);
+            }
+        }
+    } else {
+        // all vertices (cent_bc_reduced) or k sampled sources
+        // (cent_bc_reduced_sources: at most one listed copy a source, plus
+        // two a block for the farness terms; items of at most 64 copies)
+        const size_t L = k < 0 ? NP : std::min(NP, (size_t)k + 2 * (size_t)std::max(1, r->nblocks));
+        const size_t I = k < 0 ? (size_t)std::max(1, r->nitems) : L;
+        w.add(8 * ensure_cap(L) + 8 * ensure_cap(NP) + 4 * ensure_cap(2) + 4 * ensure_cap(I));  // red_D, red_bcp, red_head, red_deep
+        if (k >= 0) w.add(4 * ensure_cap(L) + 8 * ensure_cap(NP) + 16 * ensure_cap(I + 1) + 4 * ensure_cap(I + 1) +
+                          8 * ensure_cap(I + 1));  // red_src, red_swgt, red_items, red_order, red_poff
+        const size_t small = k < 0 ? nsmall : I;
+        if (sched == CENT_SCHED_PERSISTENT && small) {
+            const size_t grid = std::min<size_t>(small, kBcItemGrid);
+            // sampled items may join adjacent small blocks up to the LDS window
+            const size_t slot = (k < 0 ? max_small : (size_t)kItemLds) * kWave;
+            // per-workgroup path counts, coefficients and levels; partial sums
+            // over the items' ranges (a block's items each span the block)
+            const size_t part = k < 0 ? NP : NP + (size_t)kItemLds / kWave * L;
+            w.add(8 * ensure_cap(slot * grid) + 8 * ensure_cap(slot * grid) + ensure_cap(slot * grid) +
+                  8 * ensure_cap(part));
+        }
+        ws_dense(w, NP, nseg, true, false, 1, G, levels);  // big items and items too deep for the kernel
+    }
+    *bytes = (int64_t)w.peak;
     return CENT_OK;
 }

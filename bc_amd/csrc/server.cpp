@@ -4,11 +4,17 @@
 //
 // - Requests (graph, kind, source list) queue FIFO up to max_queue.
 // - `slots` worker threads, each with its own engine context and HIP stream,
-//   take the head request as soon as a slot is free AND its estimated device
-//   footprint (workspace + the graph's device copy if not cached) fits in the
-//   HBM budget next to everything already admitted.  Requests therefore run
-//   concurrently on the device, and a new one is admitted the moment a slot
-//   and memory free up, whatever the others are doing.
+//   take the head request as soon as a slot is free AND its device footprint
+//   (the engine's own workspace estimate for the call, and the graph's device
+//   copy if not cached) fits in the HBM budget next to everything else held.
+//   Requests therefore run concurrently on the device, and a new one is
+//   admitted the moment a slot and memory free up, whatever the others are
+//   doing.
+// - The budget counts what each slot's context holds: while a request runs,
+//   the larger of its estimate and the context's cached buffers; between
+//   requests, the cached buffers (cent_ctx_workspace_bytes).  A context
+//   above its share (budget / slots / 2) is trimmed after its request, and
+//   an admission that waits on idle slots' caches has them trimmed.
 // - Graphs are registered once (host copy) and uploaded on first use; device
 //   copies stay cached and are evicted least-recently-used, when no running
 //   request holds them, if the budget needs the room.
@@ -41,6 +47,7 @@ using clk = std::chrono::steady_clock;
 struct GraphEntry {
     struct CSR* host = nullptr;    // owned copy
     int64_t n = 0, nnz = 0;
+    int32_t depth = -1;            // BFS depth from the highest-degree vertex (what the upload probes)
     cent_graph* dev = nullptr;     // cached device CSR
     cent_reduced* red = nullptr;   // host reduction (closeness queries)
     int64_t dev_bytes = 0;         // estimated device bytes of dev + red
@@ -58,7 +65,7 @@ struct Request {
     int32_t status = CENT_OK;
     bool cancel = false;
     double t_submit = 0, t_start = 0, t_end = 0;
-    int64_t footprint = 0;     // admitted workspace bytes
+    int64_t footprint = 0;     // workspace bytes reserved for the request's slot
     int64_t graph_bytes = 0;   // graph bytes added by this request's upload (0 if cached)
     int32_t slot = -1;
     std::vector<double> bc;
@@ -76,7 +83,10 @@ struct cent_server {
     std::deque<Request*> queue;
     std::unordered_map<int64_t, Request*> reqs;
     std::vector<GraphEntry*> graphs;
-    int64_t used_ws = 0, used_graph = 0;
+    int64_t used_ws = 0, used_graph = 0;  // used_ws = sum of held
+    std::vector<int64_t> held;            // per slot: bytes its context may hold (see above)
+    std::vector<char> busy;               // per slot: running a request
+    bool trim_idle = false;               // an admission waits on idle slots' cached buffers
     int running = 0;
     bool stop = false;
     int64_t next_id = 1;
@@ -95,32 +105,36 @@ static double now_ms(const cent_server* s) {
 // on a 232k-vertex block tree).
 constexpr int32_t kSourceSlots = 256;
 
-// Device bytes a request needs besides the graph: the larger of the context
-// workspace of the 64-lane step (per vertex 512 B of path counts, which the
-// backward overwrites with coefficients, frontier masks, partials) and of the source-parallel step
-// (kSourceSlots slots, each with per vertex a 16-byte record, a BFS-order
-// entry and a BC partial plus a level table of up to 64 Ki entries for BC,
-// and a BFS-order entry and one map bit for closeness).
-static int64_t workspace_estimate(const GraphEntry* g, int32_t kind, bool all, int64_t k) {
-    const int64_t n = std::max<int64_t>(g->n, 1);
-    const int64_t slots = std::min<int64_t>(std::max<int64_t>(k, 1), kSourceSlots);
-    const int64_t lcap = std::min<int64_t>(n + 2, 65536);
-    if (kind == CENT_Q_BC) {
-        const int64_t dense = n * (512 + 8 + 17 * 8 + 8 + 64) + k * 16 + (64 << 20);
-        const int64_t source = slots * (n * (16 + 4 + 8) + lcap * 4) + n * 8 + k * 16 + (16 << 20);
-        return std::max(dense, source);
+// Options every request runs with: at most kSourceSlots per-source slots,
+// and one stream (the slots already run side by side; a second stream per
+// slot would double its workspace).
+static cent_opts request_opts() {
+    cent_opts o{};
+    o.max_slots = kSourceSlots;
+    o.flags = CENT_OPT_ONE_STREAM;
+    return o;
+}
+
+// Device bytes the request's call may hold in its context besides the graph
+// (the engine's estimates, which replay its allocation rules): the plain call
+// on the device graph (or, before the upload, on the host rows with the
+// graph's probe depth), and for requests that go through the reduction its
+// reduced call, or, before the reduction exists, the GPU reduction's scratch
+// (run_request raises the reservation once the reduced form is built).
+static int64_t workspace_estimate(const GraphEntry* g, const Request* r) {
+    const cent_opts o = request_opts();
+    const int32_t k = r->all ? -1 : (int32_t)r->src.size();
+    const int32_t what = r->kind == CENT_Q_BC ? CENT_TUNE_BC : CENT_TUNE_CC;
+    int64_t plain = 0, red = 0;
+    if (r->kind == CENT_Q_BC || !r->all) {
+        if (g->dev) (void)cent_graph_workspace_estimate(g->dev, what, k, &o, &plain);
+        else (void)cent_workspace_estimate(g->host->csrV, (int32_t)g->n, g->nnz, g->depth, 0, what, k, &o, &plain);
     }
-    // closeness: a launch group of up to 64 interleaved batches keeps, per
-    // vertex, 64 reached-masks and two planes of 64 frontier masks (1.5 KB)
-    constexpr int64_t kCcVertex = 8 * (3 * 64 + 1) + 8 + 64;
-    if (all && g->red) {
-        cent_reduced_view v;
-        cent_reduced_get_view(g->red, &v);
-        return (int64_t)v.np * kCcVertex + (16 << 20);
+    if (r->kind == CENT_Q_BC || r->all) {
+        if (g->red) (void)cent_reduced_workspace_estimate(g->red, what, k, CENT_SCHED_PERSISTENT, &o, &red);
+        else (void)cent_reduce_gpu_workspace_estimate((int32_t)g->n, g->nnz, &red);
     }
-    const int64_t dense = n * kCcVertex + k * 16 + (16 << 20);
-    const int64_t source = slots * (n * (4 + 1) + lcap * 4) + k * 16 + (16 << 20);
-    return std::max(dense, source);
+    return std::max(plain, red);
 }
 
 static int64_t graph_estimate(const GraphEntry* g) {
@@ -179,26 +193,56 @@ static bool reduced_bc_pays(const GraphEntry* g, const Request* r) {
     return red < 0.5 * plain;
 }
 
+// An admission on `slot` waits for memory: idle slots holding cached
+// buffers are asked to release them (the worker loop).  Called with s->mu held.
+static void ask_idle_trim(cent_server* s, int slot) {
+    for (size_t i = 0; i < s->held.size(); ++i)
+        if (i != (size_t)slot && !s->busy[i] && s->held[i] > 0) s->trim_idle = true;
+    if (s->trim_idle) s->cv_work.notify_all();
+}
+
 static void run_request(cent_server* s, int slot, Request* r, GraphEntry* g) {
     int st = CENT_OK;
     const bool dry = (s->o.flags & CENT_SERVER_DRYRUN) != 0;
     cent_ctx* ctx = dry ? nullptr : s->ctx[(size_t)slot];
+    bool reduced_now = false;
     {
         std::lock_guard<std::mutex> lk(g->build_mu);
         if (!dry && !g->dev) st = cent_graph_upload(ctx, g->host, &g->dev);
-        // the reduction runs on this slot's GPU (its scratch is transient,
-        // freed before the query runs); the dry run reduces on the host.
-        // All-vertex closeness always goes through it, betweenness when it
-        // is cheaper there (below)
-        if (!st && ((r->kind == CENT_Q_CC && r->all) || r->kind == CENT_Q_BC) && !g->red)
+        // the reduction runs on this slot's GPU; the dry run reduces on the
+        // host.  All-vertex closeness always goes through it, betweenness
+        // when it is cheaper there (below)
+        if (!st && ((r->kind == CENT_Q_CC && r->all) || r->kind == CENT_Q_BC) && !g->red) {
             st = dry ? cent_reduce(g->host, &g->red) : cent_reduce_gpu(ctx, g->host, &g->red);
+            reduced_now = !st;
+        }
+    }
+    if (reduced_now && !dry) {
+        // the reduction's scratch is released; the reservation becomes the
+        // reduced call's own estimate, waiting (holding nothing) if it grew
+        // beyond what is free
+        (void)cent_ctx_trim(ctx);
+        const int64_t want = workspace_estimate(g, r);
+        std::unique_lock<std::mutex> lk(s->mu);
+        if (want > s->held[(size_t)slot]) {
+            s->used_ws -= s->held[(size_t)slot];
+            s->held[(size_t)slot] = 0;
+            if (want + graph_estimate(g) + reduced_estimate(g->red) > s->o.hbm_budget) {
+                st = CENT_E_OOM;
+            } else {
+                while (!make_room(s, want, g)) {
+                    ask_idle_trim(s, slot);
+                    s->cv_done.wait(lk);
+                }
+                s->trim_idle = false;
+                s->used_ws += want;
+                s->held[(size_t)slot] = want;
+                r->footprint = want;
+            }
+        }
     }
     const int64_t k = r->all ? g->n : (int64_t)r->src.size();
-    cent_opts opts{};
-    opts.max_slots = kSourceSlots;  // the admission estimate counts this many
-    // slots already run side by side; a second stream per slot would also
-    // double the workspace the admission estimate counts
-    opts.flags = CENT_OPT_ONE_STREAM;
+    const cent_opts opts = request_opts();
     if (!st) {
         if (dry) {
             // modelled duration: sources x directed edges at 1e9 edge visits/s
@@ -225,7 +269,11 @@ static void run_request(cent_server* s, int slot, Request* r, GraphEntry* g) {
     }
     // keep a slot's cached workspace within its share of the budget
     if (!dry && cent_ctx_workspace_bytes(ctx) > s->o.hbm_budget / std::max(1, s->o.slots) / 2) cent_ctx_trim(ctx);
+    const int64_t keep = dry ? 0 : cent_ctx_workspace_bytes(ctx);
     std::lock_guard<std::mutex> lk(s->mu);
+    s->used_ws += keep - s->held[(size_t)slot];  // the slot now holds its cached buffers
+    s->held[(size_t)slot] = keep;
+    s->busy[(size_t)slot] = 0;
     if (!dry && g->red && (r->kind == CENT_Q_BC || r->all)) {
         // the reduced device copy now exists; account for it once
         const int64_t rb = reduced_estimate(g->red);
@@ -240,7 +288,6 @@ static void run_request(cent_server* s, int slot, Request* r, GraphEntry* g) {
     if (r->state == CENT_REQ_DONE) s->st.completed++;
     else if (r->state == CENT_REQ_FAILED) s->st.failed++;
     else s->st.cancelled++;
-    s->used_ws -= r->footprint;
     g->refs--;
     g->last_use = ++s->tick;
     s->running--;
@@ -254,12 +301,24 @@ static void worker(cent_server* s, int slot) {
         Request* r = nullptr;
         GraphEntry* g = nullptr;
         while (!s->stop) {
+            if (s->trim_idle && s->held[(size_t)slot] > 0 && s->ctx.size() > (size_t)slot) {
+                // another slot's admission waits on cached buffers: release ours
+                lk.unlock();
+                (void)cent_ctx_trim(s->ctx[(size_t)slot]);
+                lk.lock();
+                s->used_ws -= s->held[(size_t)slot];
+                s->held[(size_t)slot] = 0;
+                s->cv_work.notify_all();
+                s->cv_done.notify_all();
+                continue;
+            }
             if (!s->queue.empty()) {
                 Request* h = s->queue.front();
                 GraphEntry* hg = s->graphs[(size_t)h->gid];
-                const int64_t ws = workspace_estimate(hg, h->kind, h->all, h->all ? hg->n : (int64_t)h->src.size());
+                // this slot's cached buffers stay, so they count toward the reservation
+                const int64_t ws = std::max(workspace_estimate(hg, h), s->held[(size_t)slot]);
                 const int64_t gb = hg->dev_bytes > 0 ? 0 : graph_estimate(hg);
-                if (ws + graph_estimate(hg) > s->o.hbm_budget) {
+                if (workspace_estimate(hg, h) + graph_estimate(hg) > s->o.hbm_budget) {
                     // can never fit: fail it instead of blocking the queue
                     s->queue.pop_front();
                     h->state = CENT_REQ_FAILED;
@@ -269,13 +328,16 @@ static void worker(cent_server* s, int slot) {
                     s->cv_done.notify_all();
                     continue;
                 }
-                if (make_room(s, ws + gb, hg)) {
+                if (make_room(s, ws - s->held[(size_t)slot] + gb, hg)) {
                     s->queue.pop_front();
+                    s->trim_idle = false;
                     r = h;
                     g = hg;
                     r->footprint = ws;
                     r->graph_bytes = gb;
-                    s->used_ws += ws;
+                    s->used_ws += ws - s->held[(size_t)slot];
+                    s->held[(size_t)slot] = ws;
+                    s->busy[(size_t)slot] = 1;
                     if (gb) {
                         s->used_graph += gb;
                         g->dev_bytes = gb;
@@ -292,6 +354,7 @@ static void worker(cent_server* s, int slot) {
                     if (s->running > s->st.max_running) s->st.max_running = s->running;
                     break;
                 }
+                ask_idle_trim(s, slot);  // it waits
             }
             s->cv_work.wait(lk);
         }
@@ -309,6 +372,8 @@ extern "C" int cent_server_create(const cent_server_opts* opts, cent_server** ou
     cent_server* s = new (std::nothrow) cent_server();
     if (!s) return CENT_E_OOM;
     s->o = *opts;
+    s->held.assign((size_t)opts->slots, 0);
+    s->busy.assign((size_t)opts->slots, 0);
     if (!(opts->flags & CENT_SERVER_DRYRUN)) {
         for (int i = 0; i < opts->slots; ++i) {
             cent_ctx* c = nullptr;
@@ -345,6 +410,29 @@ extern "C" void cent_server_destroy(cent_server* s) {
     delete s;
 }
 
+// BFS depth from the highest-degree vertex: the probe cent_graph_create runs
+// at upload, which the workspace estimate takes before the graph is there.
+static int32_t probe_depth(const struct CSR* g) {
+    const int32_t n = g->csrVSize - 1;
+    if (n <= 0) return 0;
+    int32_t s0 = 0;
+    for (int32_t v = 1; v < n; ++v)
+        if (g->csrV[v + 1] - g->csrV[v] > g->csrV[s0 + 1] - g->csrV[s0]) s0 = v;
+    std::vector<int32_t> dist((size_t)n, -1), q((size_t)n);
+    size_t h = 0, t = 0;
+    q[t++] = s0;
+    dist[(size_t)s0] = 0;
+    while (h < t) {
+        const int32_t x = q[h++];
+        for (int32_t j = g->csrV[x]; j < g->csrV[x + 1]; ++j)
+            if (dist[(size_t)g->csrE[j]] < 0) {
+                dist[(size_t)g->csrE[j]] = dist[(size_t)x] + 1;
+                q[t++] = g->csrE[j];
+            }
+    }
+    return dist[(size_t)q[t - 1]];
+}
+
 extern "C" int cent_server_add_graph(cent_server* s, const struct CSR* csr, int32_t* gid) {
     if (!s || !csr || !gid) return CENT_E_INVAL;
     struct CSR* copy = nullptr;
@@ -359,6 +447,7 @@ extern "C" int cent_server_add_graph(cent_server* s, const struct CSR* csr, int3
     g->host = copy;
     g->n = copy->csrVSize - 1;
     g->nnz = copy->csrESize;
+    g->depth = probe_depth(copy);
     std::lock_guard<std::mutex> lk(s->mu);
     *gid = (int32_t)s->graphs.size();
     s->graphs.push_back(g);

@@ -210,7 +210,11 @@ int  cent_probe_copy_bandwidth(cent_ctx* ctx, int64_t bytes, int32_t iters, doub
  * smallest id).  Every input and output of the engine stays in the caller's
  * ids either way.
  * flags: CENT_GRAPH_KEEP_ORDER keeps the caller's ids on the device too;
- * CENT_GRAPH_DEGREE_ORDER / CENT_GRAPH_BFS_ORDER force either relabel. */
+ * CENT_GRAPH_DEGREE_ORDER / CENT_GRAPH_BFS_ORDER force either relabel.
+ * The device copy lives on ctx's GPU and does not depend on ctx itself: any
+ * context of that GPU may use it, after ctx is destroyed too.  The same
+ * holds for the device copy of a reduced graph (made by the first reduced
+ * call on it). */
 #define CENT_GRAPH_KEEP_ORDER 1
 #define CENT_GRAPH_DEGREE_ORDER 4
 #define CENT_GRAPH_BFS_ORDER 8
@@ -472,6 +476,40 @@ int  cent_bc_reduced(cent_ctx* ctx, const cent_reduced* r, double* bc, int64_t* 
  * counterpart.  Multigraph reductions: CENT_E_UNSUPPORTED. */
 int  cent_bc_reduced_sources(cent_ctx* ctx, const cent_reduced* r, const int32_t* src, int32_t k, double* bc,
                              int64_t* farness, int32_t sched, const cent_opts* opts);
+
+/* ------------------------------------------------ workspace estimates */
+/*
+ * Device memory a call will allocate in a context that holds nothing yet
+ * (no reference counterpart: for callers that budget device memory, such
+ * as the request server).  *bytes is the most the context holds at once
+ * during the call (cent_ctx_workspace_bytes afterwards is at most that),
+ * replayed from the engine's own allocation rules: buffer headroom, the
+ * per-level stores of the dense step and their growth, the peer context of
+ * the two-stream form, the per-source slots of the source-parallel step.
+ * what = CENT_TUNE_BC (cent_bc and cent_bc_device) or CENT_TUNE_CC
+ * (cent_cc, cent_cc_device); k sources, -1 = all; opts as given to the
+ * call.  Graph storage is not included.
+ *
+ * cent_workspace_estimate works from host rows before the graph is
+ * uploaded: depth is the BFS depth from the graph's highest-degree vertex
+ * (the upload probes the same), -1 if not known (the estimate then covers
+ * the dense step up to the depth it is chosen for by default, and the
+ * source-parallel step), gflags the cent_graph_create flags (tuning-table
+ * entries apply as they will at upload unless CENT_GRAPH_NO_TUNE; with
+ * depth -1 the engine defaults are assumed).  cent_graph_workspace_estimate
+ * uses an uploaded graph's own layout.  cent_reduced_workspace_estimate
+ * covers cent_cc_reduced (what = CC), cent_bc_reduced (BC, k = -1) and
+ * cent_bc_reduced_sources (BC, k sources) under schedule sched; for blocks
+ * deeper than 112 levels the dense level store may grow beyond it. */
+int  cent_workspace_estimate(const int32_t* row_ptr, int32_t n, int64_t nnz, int32_t depth, int32_t gflags,
+                             int32_t what, int32_t k, const cent_opts* opts, int64_t* bytes);
+int  cent_graph_workspace_estimate(const cent_graph* g, int32_t what, int32_t k, const cent_opts* opts,
+                                   int64_t* bytes);
+int  cent_reduced_workspace_estimate(const cent_reduced* r, int32_t what, int32_t k, int32_t sched,
+                                     const cent_opts* opts, int64_t* bytes);
+/* Device scratch cent_reduce_gpu takes for a graph of n vertices and nnz
+ * directed edges (kept by the context afterwards, until cent_ctx_trim). */
+int  cent_reduce_gpu_workspace_estimate(int32_t n, int64_t nnz, int64_t* bytes);
 
 /* ---------------------------------------------------------------- serving */
 /*

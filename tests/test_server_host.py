@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 import bc_amd
+from conftest import probe_depth
 
 
 def small_graph(n=200, seed=0):
@@ -31,17 +32,32 @@ def test_concurrency_bounded_by_slots():
 
 def test_budget_limits_admission_and_oversized_fails():
     g = small_graph()
-    # budget fits one BC workspace (~64 MiB fixed part + per-vertex) at a time
-    s = bc_amd.Server(slots=4, hbm_budget=100 << 20, dryrun=True)
+    src = [0, 1, 2]
+    # closeness from listed sources runs the plain call: its reservation is
+    # the engine's estimate for it (max_slots 256, one stream)
+    s = bc_amd.Server(slots=4, hbm_budget=1 << 40, dryrun=True)
     gid = s.add_graph(g)
-    rids = [s.submit(gid, bc_amd.CENT_Q_BC, [0]) for _ in range(6)]
+    F = s.wait(s.submit(gid, bc_amd.CENT_Q_CC, src))["footprint"]
+    graph_bytes = s.stats()["used_graph"]
+    assert F == bc_amd.workspace_estimate(g, bc_amd.CENT_TUNE_CC, len(src), probe_depth(g), max_slots=256,
+                                          one_stream=True) > 0
+    # betweenness goes through the reduction, whose GPU build reserves its scratch first
+    first = s.wait(s.submit(gid, bc_amd.CENT_Q_BC, [0]))["footprint"]
+    assert first >= bc_amd.reduce_gpu_workspace_estimate(g)
+    s.close()
+    # a budget that fits one reservation at a time serialises the requests
+    s = bc_amd.Server(slots=4, hbm_budget=graph_bytes + F + F // 2, dryrun=True)
+    gid = s.add_graph(g)
+    rids = [s.submit(gid, bc_amd.CENT_Q_CC, src) for _ in range(6)]
     for r in rids:
         assert s.wait(r)["state"] == "done"
-    assert s.stats()["max_running"] == 1
+    st = s.stats()
+    assert st["max_running"] == 1 and st["used_workspace"] == 0
     s.close()
-    s = bc_amd.Server(slots=2, hbm_budget=10 << 20, dryrun=True)
+    # one that never fits it fails the request instead of blocking the queue
+    s = bc_amd.Server(slots=2, hbm_budget=graph_bytes + F - 1, dryrun=True)
     gid = s.add_graph(g)
-    r = s.submit(gid, bc_amd.CENT_Q_BC, [0])
+    r = s.submit(gid, bc_amd.CENT_Q_CC, src)
     info = s.wait(r)
     assert info["state"] == "failed"
     status, _, _, _ = s.result(r)
@@ -73,11 +89,18 @@ def test_queue_full_and_cancel():
 
 def test_graph_cache_hits_and_evictions():
     gs = [small_graph(seed=i) for i in range(3)]
-    # room for one BC workspace plus ~one cached graph
-    s = bc_amd.Server(slots=1, hbm_budget=(64 << 20) + 2_000_000, dryrun=True)
+    src = [0, 1, 2]
+    s = bc_amd.Server(slots=1, hbm_budget=1 << 40, dryrun=True)
+    sizes, fp = [], []
+    for g in gs:
+        fp.append(s.wait(s.submit(s.add_graph(g), bc_amd.CENT_Q_CC, src))["footprint"])
+        sizes.append(s.stats()["used_graph"] - sum(sizes))
+    s.close()
+    # room for the largest closeness workspace plus about one cached graph
+    s = bc_amd.Server(slots=1, hbm_budget=max(fp) + max(sizes) * 3 // 2, dryrun=True)
     gids = [s.add_graph(g) for g in gs]
     for gid in [gids[0], gids[0], gids[1], gids[0], gids[2]]:
-        r = s.submit(gid, bc_amd.CENT_Q_BC, [0])
+        r = s.submit(gid, bc_amd.CENT_Q_CC, src)
         assert s.wait(r)["state"] == "done"
     st = s.stats()
     assert st["graph_hits"] >= 1 and st["graph_evictions"] >= 1
