@@ -1,11 +1,11 @@
 // Closeness level step over W side-by-side 64-source batches (W = 1, 2, 4,
-// 8, 16 or 32; the batches of a launch group are numbered b = 0 .. nb - 1 <=
-// W - 1).  The masks are interleaved by vertex:
+// 8, 16, 32 or 64; the batches of a launch group are numbered b = 0 .. nb - 1
+// <= W - 1).  The masks are interleaved by vertex:
 //   vis[v * W + b]  lanes of batch b that have reached v
 //   F[v * W + b]    lanes of batch b at the current distance
 // so one scan of v's row serves every batch, and a neighbour's masks for all
 // batches are one contiguous W x 8-byte read (one 128-byte cache line at
-// W = 16, two at W = 32) instead of W reads from W separate planes.
+// W = 16, four at W = 64) instead of W reads from W separate planes.
 //
 // Lane layout inside a wave: b = lane % W is the batch, e = lane / W the edge
 // slot, so a row is read E = 64 / W edges at a time and each lane ORs the

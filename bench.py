@@ -221,8 +221,8 @@ def extra_grid(ctx, args):
 
 def extra_closeness(ctx, args):
     """Closeness alone: the main workload's 4096 sources on R-MAT 2^20 through
-    the interleaved 8-batch step, and all-vertex closeness on the serving
-    pool's 148k-vertex R-MAT graph, reduced (D1 + AP, persistent items plus
+    the interleaved step, and all-vertex closeness on the serving pool's first
+    R-MAT graph (174k vertices), reduced (D1 + AP, persistent items plus
     bounded two-stream big items) and plain.  Farness must agree."""
     import numpy as np
     import bc_amd
@@ -246,7 +246,7 @@ def extra_closeness(ctx, args):
     dgp = ctx.upload(gp)
     tr, far_r = best_of(lambda: ctx.cc_reduced(red), 3)
     tp, (far_p, _) = best_of(lambda: ctx.cc(dgp, None), 2)
-    log(f"closeness: R-MAT 2^{args.scale} {len(src)} sources {t20 * 1e3:.1f} ms; 148k R-MAT all-vertex "
+    log(f"closeness: R-MAT 2^{args.scale} {len(src)} sources {t20 * 1e3:.1f} ms; pool R-MAT (n={gp.n}) all-vertex "
         f"reduced {tr * 1e3:.1f} ms, plain {tp * 1e3:.1f} ms")
     return {"workload": "closeness only: R-MAT 2^20 with the main workload's sources; all-vertex "
                         "closeness on the serving pool's R-MAT graph 0 (seed 5000)",
