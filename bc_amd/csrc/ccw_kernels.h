@@ -89,27 +89,95 @@ __device__ __forceinline__ uint64_t ccw_scan(const CcwArgs& a, int beg, int end,
     return ccw_batch_or<W>(orf);
 }
 
-// Per-block lane partials: s_far[b][s] += weight, for the new lanes nm of
-// batch b at vertex v; lane (b, e) covers sources e * W .. e * W + W - 1.
+// Weights reached per source, kept per lane as bit-sliced counters: plane k
+// holds bit k of the counter of each source the lane covers (lane (b, e):
+// sources e * W .. e * W + W - 1 of batch b, bit t for source e * W + t).
+// Adding weight w for the new lanes nm is a ripple-carry add of nm into the
+// planes at each set bit of w, a few register operations where one LDS
+// atomic per new (vertex, source) pair cost about a fifth of the closeness
+// step.  A carry out of the top plane, and the bits of w above it, go to
+// the block's LDS partials directly; the planes are added into them once,
+// at the end of the kernel (ccw_count_flush).
+constexpr int kCcwPlanes = 8;
+// Groups narrower than this keep one LDS atomic per new pair: a lane then
+// covers few sources (one at W = 1, the per-item schedule's width) and the
+// ripple over the planes costs more than its atomics.
+constexpr int kCcwSlicedFrom = 8;
+
+struct CcwCount {
+    uint64_t p[kCcwPlanes] = {};
+    bool used = false;  // anything counted (else the flush is skipped)
+};
+
 template <int W>
-__device__ __forceinline__ void ccw_count(long long (*s_far)[kWave], uint64_t nm, long long w, int lane) {
+__device__ __forceinline__ void ccw_count(CcwCount& c, long long (*s_far)[kWave], uint64_t nm, long long w,
+                                          int lane) {
     const int b = lane & (W - 1), e = lane / W;
     const uint64_t bits = W == 64 ? nm : (nm >> (e * W)) & ((1ull << (W & 63)) - 1ull);
-    for (uint64_t m = bits; m; m &= m - 1) atomicAdd((unsigned long long*)&s_far[b][e * W + __builtin_ctzll(m)], (unsigned long long)w);
+    if constexpr (W < kCcwSlicedFrom) {
+        for (uint64_t m = bits; m; m &= m - 1)
+            atomicAdd((unsigned long long*)&s_far[b][e * W + __builtin_ctzll(m)], (unsigned long long)w);
+        return;
+    }
+    c.used = true;
+    uint64_t carry_out = 0;
+    if (w == 1) {
+        uint64_t carry = bits;
+#pragma unroll
+        for (int k = 0; k < kCcwPlanes; ++k) {
+            const uint64_t t = c.p[k] & carry;
+            c.p[k] ^= carry;
+            carry = t;
+        }
+        carry_out = carry;
+    } else {
+#pragma unroll
+        for (int j = 0; j < kCcwPlanes; ++j) {
+            uint64_t carry = ((w >> j) & 1) ? bits : 0ull;
+#pragma unroll
+            for (int k = j; k < kCcwPlanes; ++k) {
+                const uint64_t t = c.p[k] & carry;
+                c.p[k] ^= carry;
+                carry = t;
+            }
+            // a carry out of plane j's ripple is worth 2^kCcwPlanes like any other
+            for (uint64_t m = carry; m; m &= m - 1)
+                atomicAdd((unsigned long long*)&s_far[b][e * W + __builtin_ctzll(m)], 1ull << kCcwPlanes);
+        }
+        const unsigned long long hi = (unsigned long long)(w >> kCcwPlanes) << kCcwPlanes;
+        if (hi)
+            for (uint64_t m = bits; m; m &= m - 1) atomicAdd((unsigned long long*)&s_far[b][e * W + __builtin_ctzll(m)], hi);
+    }
+    for (uint64_t m = carry_out; m; m &= m - 1)
+        atomicAdd((unsigned long long*)&s_far[b][e * W + __builtin_ctzll(m)], 1ull << kCcwPlanes);
+}
+
+// The lane's counters into the block's LDS partials.
+template <int W>
+__device__ __forceinline__ void ccw_count_flush(const CcwCount& c, long long (*s_far)[kWave], int lane) {
+    const int b = lane & (W - 1), e = lane / W;
+    if constexpr (W < kCcwSlicedFrom) return;
+    if (ballot(c.used) == 0) return;
+    for (int t = 0; t < (W == 64 ? 64 : W); ++t) {
+        unsigned long long cnt = 0;
+#pragma unroll
+        for (int k = 0; k < kCcwPlanes; ++k) cnt |= ((c.p[k] >> t) & 1ull) << k;
+        if (cnt) atomicAdd((unsigned long long*)&s_far[b][e * W + t], cnt);
+    }
 }
 
 // Finishes vertex v: writes the new masks, vis, the lane partials and the
 // distance sum; returns the number of new (source, v) pairs (lane 0).
 template <int W>
 __device__ __forceinline__ unsigned long long ccw_done(const CcwArgs& a, int v, uint64_t old, uint64_t nm,
-                                                       long long (*s_far)[kWave], int lane) {
+                                                       CcwCount& cnt, long long (*s_far)[kWave], int lane) {
     const int b = lane & (W - 1), e = lane / W;
     if (e == 0) {
         a.Fn[(size_t)v * W + b] = nm;  // every entry of Fn is written each level (no clear)
         if (nm) a.vis[(size_t)v * W + b] = old | nm;
     }
     if (ballot(nm != 0) == 0) return 0;
-    ccw_count<W>(s_far, nm, a.weight ? (long long)a.weight[v] : 1ll, lane);
+    ccw_count<W>(cnt, s_far, nm, a.weight ? (long long)a.weight[v] : 1ll, lane);
     const long long pc = (long long)wave_sum_i64(e == 0 ? (int64_t)__popcll(nm) : 0);
     if (lane == 0 && a.dist_sum) a.dist_sum[v] += (int64_t)a.dnext * pc;
     return (unsigned long long)pc;
@@ -151,6 +219,7 @@ __global__ __launch_bounds__(kBlock) void k_ccw(CcwArgs a) {
     const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const int nw = gridDim.x * kWavesPerBlock;
     unsigned long long nn = 0;
+    CcwCount cnt;
     for (int c = gw; c < a.nchunk; c += nw) {
         const int base = a.chunk_beg[c];
         const int vl = base + lane;
@@ -179,9 +248,10 @@ __global__ __launch_bounds__(kBlock) void k_ccw(CcwArgs a) {
             const int v = base + j;
             const uint64_t old = a.vis[(size_t)v * W + bl];
             const uint64_t nm = ccw_scan<W>(a, rl32(rbl, j), rl32(rel, j), Ab & ~old, lane);
-            nn += ccw_done<W>(a, v, old, nm, s_far, lane);
+            nn += ccw_done<W>(a, v, old, nm, cnt, s_far, lane);
         }
     }
+    ccw_count_flush<W>(cnt, s_far, lane);
     if (lane == 0 && nn) atomicAdd(&s_new, nn);
     ccw_flush<W>(a, s_far, &s_new, blockIdx.x);
 }
@@ -218,6 +288,7 @@ __global__ __launch_bounds__(kBlock) void k_ccw_hubfin(CcwArgs a) {
     const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const int nw = gridDim.x * kWavesPerBlock;
     unsigned long long nn = 0;
+    CcwCount cnt;
     for (int k = gw; k < a.nhub; k += nw) {
         const int v = a.hub_v[k];
         const uint64_t old = a.vis[(size_t)v * W + bl];
@@ -226,8 +297,9 @@ __global__ __launch_bounds__(kBlock) void k_ccw_hubfin(CcwArgs a) {
         for (int i = a.hub_seg_off[k] - a.seg_base + el; i < a.hub_seg_off[k + 1] - a.seg_base; i += E)
             orp |= a.part_or[(size_t)i * W + bl];
         const uint64_t nm = ccw_batch_or<W>(orp);
-        nn += ccw_done<W>(a, v, old, nm, s_far, lane);
+        nn += ccw_done<W>(a, v, old, nm, cnt, s_far, lane);
     }
+    ccw_count_flush<W>(cnt, s_far, lane);
     if (lane == 0 && nn) atomicAdd(&s_new, nn);
     ccw_flush<W>(a, s_far, &s_new, blockIdx.x);
 }
