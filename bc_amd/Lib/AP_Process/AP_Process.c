@@ -36,7 +36,39 @@ void cent_reduced_free(struct cent_reduced* r) {
     free(r->prp); free(r->pcol); free(r->pw); free(r->porig); free(r->pblock);
     free(r->boff); free(r->bhead); free(r->bparent); free(r->rep); free(r->vcomp);
     free(r->comp_w); free(r->items);
+    free(r->fin_x); free(r->fin_b); free(r->fin_base); free(r->fin_ffsum);
     free(r);
+}
+
+int cent_reduced_tables(struct cent_reduced* r) {
+    const int32_t n = r->n;
+    const size_t N = (size_t)(n > 0 ? n : 1);
+    free(r->fin_x); free(r->fin_b); free(r->fin_base); free(r->fin_ffsum);
+    r->fin_x = (int32_t*)malloc(N * sizeof(int32_t));
+    r->fin_b = (int32_t*)malloc(N * sizeof(int32_t));
+    r->fin_base = (int64_t*)malloc(N * sizeof(int64_t));
+    r->fin_ffsum = (int64_t*)calloc((size_t)r->nblocks + 1, sizeof(int64_t));
+    if (!r->fin_x || !r->fin_b || !r->fin_base || !r->fin_ffsum) return CENT_E_OOM;
+    for (int32_t b = 0; b < r->nblocks; ++b)
+        for (int32_t i = r->boff[b]; i < r->boff[b + 1]; ++i)
+            if (i != r->bhead[b]) r->fin_ffsum[b] += r->d1.ff[r->porig[i]];
+    for (int32_t v = 0; v < n; ++v) {
+        if (r->d1.parent[v] >= 0) continue;
+        const int32_t x = r->rep[v];
+        r->fin_x[v] = x;
+        r->fin_b[v] = x >= 0 ? r->pblock[x] : -1;
+        r->fin_base[v] = x >= 0 ? 0 : r->d1.ff[v];  /* lone: fully folded component */
+    }
+    /* folded vertices, reverse fold order (a parent before its children):
+     * farness(d) = farness(p) + N - 2 w[d] (Seq_CC/CC.cpp:2037-2049) */
+    for (int32_t i = r->d1.nfold - 1; i >= 0; --i) {
+        const int32_t d = r->d1.order[i];
+        const int32_t p = r->d1.parent[d];
+        r->fin_x[d] = r->fin_x[p];
+        r->fin_b[d] = r->fin_b[p];
+        r->fin_base[d] = r->fin_base[p] + r->comp_w[r->vcomp[d]] - 2 * r->d1.w[d];
+    }
+    return CENT_OK;
 }
 
 typedef struct { int32_t u, v; } edge_t;
@@ -319,8 +351,9 @@ int cent_reduced_build(const struct CSR* g, struct cent_reduced** out) {
         free(cnt);
     }
 
-    /* ---- work items */
+    /* ---- work items, finish tables */
     if ((st = cent_reduced_items(r))) goto fail;
+    if ((st = cent_reduced_tables(r))) goto fail;
     st = CENT_OK;
 fail:
     free(disc); free(low); free(dpar); free(it); free(fstack); free(estack); free(stamp); free(pblock_of);
@@ -347,40 +380,36 @@ fail:
  * valid only at each non-root block's head copy and at the head vertex's
  * copy in the parent block. */
 static int reduced_block_constants(const struct cent_reduced* r, const int64_t* D, int64_t* K) {
-    const int32_t n = r->n, nb = r->nblocks;
-    const size_t N = (size_t)(n > 0 ? n : 1);
+    /* Over the block-cut tree, blocks in index order (children before
+     * parents).  The copies of block b other than its head hang off
+     *   s_b = sum of ff over them (fin_ffsum) + down(C) of every child block C,
+     * down(C) = D at C's head + s_C being the weighted distance sum from C's
+     * head vertex to everything below it through C. */
+    const int32_t nb = r->nblocks;
     int64_t* down = (int64_t*)calloc((size_t)nb + 1, sizeof(int64_t));
-    int64_t* hang_ff = (int64_t*)malloc(N * sizeof(int64_t)); /* ff[v] + sum down(C) of child blocks at v */
-    if (!down || !hang_ff) { free(down); free(hang_ff); return CENT_E_OOM; }
-    for (int32_t v = 0; v < n; ++v) hang_ff[v] = r->d1.ff[v];
+    int64_t* child = (int64_t*)calloc((size_t)nb + 1, sizeof(int64_t));
+    if (!down || !child) { free(down); free(child); return CENT_E_OOM; }
     /* bottom-up */
     for (int32_t b = 0; b < nb; ++b) {
         if (r->bparent[b] < 0) continue;
-        const int32_t h = r->bhead[b];
-        int64_t s = D[h];
-        for (int32_t i = r->boff[b]; i < r->boff[b + 1]; ++i)
-            if (i != h) s += hang_ff[r->porig[i]];
-        down[b] = s;
-        hang_ff[r->porig[h]] += s;
+        down[b] = D[r->bhead[b]] + r->fin_ffsum[b] + child[b];
+        child[r->bparent[b]] += down[b];
     }
     /* top-down (parents before children) */
     for (int32_t b = nb - 1; b >= 0; --b) {
-        int64_t s = 0;
-        const int32_t h = r->bhead[b];
-        for (int32_t i = r->boff[b]; i < r->boff[b + 1]; ++i)
-            if (i != h) s += hang_ff[r->porig[i]];
+        const int64_t s = r->fin_ffsum[b] + child[b];
         if (r->bparent[b] < 0) {
             K[b] = s;
         } else {
             /* farness of the head vertex from its copy in the parent block */
             const int32_t P = r->bparent[b];
-            const int32_t xp = r->rep[r->porig[h]];  /* the head vertex's copy in P */
-            if (xp < 0 || r->pblock[xp] != P) { free(down); free(hang_ff); return CENT_E_INVAL; }
+            const int32_t xp = r->rep[r->porig[r->bhead[b]]];  /* the head vertex's copy in P */
+            if (xp < 0 || r->pblock[xp] != P) { free(down); free(child); return CENT_E_INVAL; }
             const int64_t far_h = D[xp] + K[P];
             K[b] = (far_h - down[b]) + s;
         }
     }
-    free(down); free(hang_ff);
+    free(down); free(child);
     return CENT_OK;
 }
 
@@ -391,17 +420,11 @@ int cent_reduced_finish(const struct cent_reduced* r, const int64_t* D, int64_t*
     if (!K) return CENT_E_OOM;
     const int st = reduced_block_constants(r, D, K);
     if (st) { free(K); return st; }
-    /* unfolded vertices */
+    /* farness(v) = D_B(x) + K_B at the copy x answering for v's unfolded
+     * ancestor, plus the fold chain's terms (tables: cent_reduced_tables) */
     for (int32_t v = 0; v < n; ++v) {
-        if (r->d1.parent[v] >= 0) continue;
-        const int32_t x = r->rep[v];
-        farness[v] = (x >= 0) ? D[x] + K[r->pblock[x]] : r->d1.ff[v];  /* lone: fully folded component */
-    }
-    /* folded vertices, reverse fold order (Seq_CC/CC.cpp:2037-2049) */
-    for (int32_t i = r->d1.nfold - 1; i >= 0; --i) {
-        const int32_t d = r->d1.order[i];
-        const int32_t p = r->d1.parent[d];
-        farness[d] = farness[p] + r->comp_w[r->vcomp[d]] - 2 * r->d1.w[d];
+        const int32_t x = r->fin_x[v];
+        farness[v] = r->fin_base[v] + (x >= 0 ? D[x] + K[r->fin_b[v]] : 0);
     }
     free(K);
     return CENT_OK;
@@ -421,14 +444,8 @@ int cent_reduced_farness_sources(const cent_reduced* r, const int64_t* D, const 
     const int st = reduced_block_constants(r, D, K);
     if (st) { free(K); return st; }
     for (int32_t i = 0; i < k; ++i) {
-        int32_t v = src[i];
-        int64_t add = 0;
-        while (r->d1.parent[v] >= 0) {  /* up the fold tree */
-            add += r->comp_w[r->vcomp[v]] - 2 * r->d1.w[v];
-            v = r->d1.parent[v];
-        }
-        const int32_t x = r->rep[v];
-        far[i] = add + ((x >= 0) ? D[x] + K[r->pblock[x]] : r->d1.ff[v]);
+        const int32_t v = src[i], x = r->fin_x[v];
+        far[i] = r->fin_base[v] + (x >= 0 ? D[x] + K[r->fin_b[v]] : 0);
     }
     free(K);
     return CENT_OK;
@@ -583,18 +600,35 @@ int cent_reduced_source_weights(const cent_reduced* r, const int32_t* src, int32
  * second what the GPU accumulates at v's copies (weighted Brandes from
  * every copy with WS > 0, source weight WS, target weight W).  With every
  * vertex once in S this is cent_reduced_finish_bc's formula. */
+int cent_reduced_source_weights_all(const cent_reduced* r, const int32_t* src, int32_t k, int64_t* WS, int64_t* wsf,
+                                    int64_t* Sc) {
+    return source_weights(r, src, k, WS, wsf, Sc);
+}
+
 int cent_reduced_bc_sources(const cent_reduced* r, const int32_t* src, int32_t k, const double* bcp, double* bc) {
     if (!r || (!bcp && r->np > 0) || !bc || k < 0 || (k > 0 && !src)) return CENT_E_INVAL;
     if (r->multi) return CENT_E_UNSUPPORTED;
-    const int32_t n = r->n;
-    const size_t N = (size_t)(n > 0 ? n : 1);
+    const size_t N = (size_t)(r->n > 0 ? r->n : 1);
     int64_t* WS = (int64_t*)malloc((size_t)(r->np > 0 ? r->np : 1) * sizeof(int64_t));
     int64_t* wsf = (int64_t*)malloc(N * sizeof(int64_t));
     int64_t* Sc = (int64_t*)malloc((size_t)(r->ncomp > 0 ? r->ncomp : 1) * sizeof(int64_t));
-    double* acc = (double*)calloc(N, sizeof(double));
-    if (!WS || !wsf || !Sc || !acc) { free(WS); free(wsf); free(Sc); free(acc); return CENT_E_OOM; }
+    if (!WS || !wsf || !Sc) { free(WS); free(wsf); free(Sc); return CENT_E_OOM; }
     int st = source_weights(r, src, k, WS, wsf, Sc);
-    if (!st) {
+    if (!st) st = cent_reduced_bc_sources_from(r, WS, wsf, Sc, bcp, bc);
+    free(WS); free(wsf); free(Sc);
+    return st;
+}
+
+int cent_reduced_bc_sources_from(const cent_reduced* r, const int64_t* WS, const int64_t* wsf, const int64_t* Sc,
+                                 const double* bcp, double* bc) {
+    if (!r || (!bcp && r->np > 0) || !bc || (r->np > 0 && !WS) || !wsf || !Sc) return CENT_E_INVAL;
+    if (r->multi) return CENT_E_UNSUPPORTED;
+    const int32_t n = r->n;
+    const size_t N = (size_t)(n > 0 ? n : 1);
+    double* acc = (double*)calloc(N, sizeof(double));
+    if (!acc) return CENT_E_OOM;
+    int st = CENT_OK;
+    {
         for (int32_t i = 0; i < r->d1.nfold; ++i) {
             const int32_t c = r->d1.order[i];
             const int32_t cc = r->vcomp[c];
@@ -608,7 +642,7 @@ int cent_reduced_bc_sources(const cent_reduced* r, const int32_t* src, int32_t k
         }
         for (int32_t v = 0; v < n; ++v) bc[v] += acc[v];
     }
-    free(WS); free(wsf); free(Sc); free(acc);
+    free(acc);
     return st;
 }
 

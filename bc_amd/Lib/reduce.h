@@ -77,6 +77,15 @@ struct cent_reduced {
     /* work items over the packed graph */
     int32_t  nitems;
     struct cent_item* items;
+    /* farness of every input vertex from D in one pass (cent_reduced_tables):
+     * farness[v] = fin_base[v] + (fin_x[v] >= 0 ? D[fin_x[v]] + K[fin_b[v]] : 0),
+     * fin_x the copy answering for v's unfolded ancestor a (-1: a lone or
+     * fully folded vertex), fin_b its block, fin_base the fold-chain terms
+     * sum (N - 2 w[d]) from v up to a, plus ff[a] when fin_x is -1 */
+    int32_t* fin_x;
+    int32_t* fin_b;
+    int64_t* fin_base;
+    int64_t* fin_ffsum;     /* [nblocks] sum of ff over the block's copies but its head */
     /* device-side cache owned by the engine, released by cent_reduced_free */
     void*    dev;
     void   (*dev_free)(void*);
@@ -88,6 +97,17 @@ void cent_reduced_free(struct cent_reduced* r);
 
 /* Fills r->items / r->nitems from r->boff (both builds share it). */
 int  cent_reduced_items(struct cent_reduced* r);
+/* Fills the fin_* tables (both builds share it). */
+int  cent_reduced_tables(struct cent_reduced* r);
+
+/* cent_reduced_source_weights with the per-vertex fold-subtree counts wsf
+ * ([n]) and the sources per component Sc ([ncomp]) as well, and
+ * cent_reduced_bc_sources from those three arrays (so a caller that needs
+ * the weights computes them once). */
+int  cent_reduced_source_weights_all(const struct cent_reduced* r, const int32_t* src, int32_t k, int64_t* WS,
+                                     int64_t* wsf, int64_t* Sc);
+int  cent_reduced_bc_sources_from(const struct cent_reduced* r, const int64_t* WS, const int64_t* wsf,
+                                  const int64_t* Sc, const double* bcp, double* bc);
 
 /* Given D[x] = sum_y d_B(x, y) W(y) for every packed vertex x, fills the
  * farness of every input vertex (assigned, int64), including folded

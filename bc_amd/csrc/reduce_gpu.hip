@@ -1147,6 +1147,7 @@ extern "C" int cent_reduce_gpu(cent_ctx* ctx, const struct CSR* csr, cent_reduce
     int st = reduce_gpu(ctx, simple ? simple : csr, r);
     cent_csr_free(simple);
     if (!st) st = cent_reduced_items(r);
+    if (!st) st = cent_reduced_tables(r);
     if (st) {
         cent_reduced_free(r);
         return st;

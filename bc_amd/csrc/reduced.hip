@@ -353,11 +353,13 @@ extern "C" int cent_cc_reduced(cent_ctx* ctx, const cent_reduced* rc, int64_t* f
         }
         TRY(run_big_groups(ctx, rd, sp, groups, Dd, opts, tm, &levels));
     }
-    std::vector<int64_t> D(NP);
-    HIPTRY(hipMemcpyAsync(D.data(), Dd, sizeof(int64_t) * NP, hipMemcpyDeviceToHost, s));
+    // the distance sums come back through the context's pinned staging
+    TRY(ensure_stage(ctx, sizeof(int64_t) * NP));
+    int64_t* D = static_cast<int64_t*>(ctx->h_stage);
+    HIPTRY(hipMemcpyAsync(D, Dd, sizeof(int64_t) * NP, hipMemcpyDeviceToHost, s));
     HIPTRY(hipStreamSynchronize(s));
     timer_collect(ctx, tm);
-    TRY(cent_reduced_finish(r, D.data(), farness));
+    TRY(cent_reduced_finish(r, D, farness));
     ctx->stats.levels += levels;
     ctx->stats.reached += (int64_t)r->n;  // every vertex is a source
     ctx->stats.edges_reached += 2 * r->m;
@@ -499,8 +501,8 @@ extern "C" int cent_bc_reduced_sources(cent_ctx* ctx, const cent_reduced* rc, co
     const size_t NP = (size_t)(r->np > 0 ? r->np : 1);
     // traversals: copies with sources behind them, plus (for farness) each
     // non-root block's head copy and the head vertex's copy in the parent
-    std::vector<int64_t> WS(NP, 0);
-    TRY(cent_reduced_source_weights(r, src, k, WS.data()));
+    std::vector<int64_t> WS(NP, 0), wsf((size_t)(r->n > 0 ? r->n : 1)), Sc((size_t)(r->ncomp > 0 ? r->ncomp : 1));
+    TRY(cent_reduced_source_weights_all(r, src, k, WS.data(), wsf.data(), Sc.data()));
     std::vector<char> need(NP, 0);
     for (int32_t x = 0; x < r->np; ++x) need[(size_t)x] = WS[(size_t)x] > 0;
     if (farness)
@@ -644,7 +646,7 @@ extern "C" int cent_bc_reduced_sources(cent_ctx* ctx, const cent_reduced* rc, co
             for (int x = it.x; x < it.y; ++x) bcp[(size_t)x] += part[(size_t)(poff[i] + x - it.x)];
         }
     }
-    TRY(cent_reduced_bc_sources(r, src, k, bcp.data(), bc));
+    TRY(cent_reduced_bc_sources_from(r, WS.data(), wsf.data(), Sc.data(), bcp.data(), bc));
     if (farness && k > 0) {
         std::vector<int64_t> D(NP, 0);
         for (size_t i = 0; i < list.size(); ++i) D[(size_t)list[i]] = Dl[i];
