@@ -246,6 +246,10 @@ struct cent_graph {
     std::vector<int32_t> h_rp;        // device-order row offsets (host), for re-scheduling
     std::vector<int32_t> probe_dist;  // BFS distance from the highest-degree vertex (caller ids, -1 unreached)
     std::vector<int32_t> probe_order; // that BFS's visit order (caller ids; dropped after the relabel)
+    // [2 n] batch grouping key per device id: 4-bit BFS distances to the
+    // kLandmarks highest-degree vertices (empty: deep graphs, which run the
+    // source-parallel step)
+    std::vector<uint64_t> lm_key;
     bool bfs_order = false;           // device ids are a BFS order (else degree order or the caller's)
     bool multi = false;               // parallel edges or self-loops (CENT_CSR_RAW input)
     cent_tune_sig sig{};              // autotuner signature
@@ -267,6 +271,7 @@ struct cent_graph {
 // frontier-queue step is chosen only on request (or by the autotuner).
 constexpr int kDeepLevels = 48;
 constexpr int kSmallBcVertices = 16384;
+constexpr int kLandmarks = 32;  // landmarks of the batch grouping key
 
 static inline int level_mode(const cent_graph* g, const cent_opts* o, bool with_bc) {
     const int m = o ? o->frontier : CENT_FRONTIER_AUTO;

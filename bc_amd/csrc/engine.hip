@@ -236,6 +236,127 @@ extern "C" void cent_graph_free(cent_graph* g) {
     delete g;
 }
 
+// ------------------------------------------------------------ grouping key
+// Batch grouping key of every vertex: its BFS distances to the kLandmarks
+// highest-degree vertices, 4 bits each (15: 15 or more, or unreached),
+// landmark 0 in the top bits of word 0 and landmark 16 in the top bits of
+// word 1, so comparing (word 0, word 1) compares the distance vectors in
+// landmark order.  Sources with equal or close keys tend to reach every
+// vertex at the same levels, so a 64-lane batch of them gathers fewer and
+// fuller slices of the level-sorted sigma / coef rows.  The keys come from
+// one 32-lane bit-parallel BFS over the device graph at upload.
+__global__ void k_lm_seed(const int32_t* __restrict__ lm, int L, uint32_t* __restrict__ vis,
+                          uint32_t* __restrict__ F, uint64_t* __restrict__ key) {
+    const int j = threadIdx.x;
+    if (j < L) {  // the landmarks are distinct vertices: one writer each
+        const int v = lm[j];
+        vis[v] = 1u << j;
+        F[v] = 1u << j;
+        key[2 * (size_t)v + (j >> 4)] &= ~(0xFull << (60 - 4 * (j & 15)));
+    }
+}
+
+// Level d: a wavefront per vertex ORs its neighbours' level-(d-1) masks,
+// lane-strided over the row, until every landmark it still lacks is found.
+__global__ __launch_bounds__(256) void k_lm_level(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                  int n, uint32_t full, int d, uint32_t* __restrict__ vis,
+                                                  const uint32_t* __restrict__ Fd, uint32_t* __restrict__ Fn,
+                                                  uint64_t* __restrict__ key, int* __restrict__ flag) {
+    const int lane = lane_id();
+    const int wpb = blockDim.x / kWave;
+    const int nw = gridDim.x * wpb;
+    const uint64_t dd = (uint64_t)(d < 15 ? d : 15);
+    for (int v = blockIdx.x * wpb + (threadIdx.x >> 6); v < n; v += nw) {
+        const uint32_t cand = full & ~vis[v];
+        uint32_t orf = 0;
+        if (cand) {
+            const int b = rp[v], e = rp[v + 1];
+            for (int j0 = b; j0 < e; j0 += kWave) {
+                const int j = j0 + lane;
+                orf |= (uint32_t)wave_or(j < e ? Fd[col[j]] : 0u);
+                if ((orf & cand) == cand) break;  // orf is wave-uniform
+            }
+        }
+        const uint32_t nm = orf & cand;
+        if (lane == 0) {
+            Fn[v] = nm;
+            if (nm) {
+                vis[v] |= nm;
+                uint64_t k[2] = {key[2 * (size_t)v], key[2 * (size_t)v + 1]};
+                for (uint32_t m = nm; m; m &= m - 1) {
+                    const int j = __builtin_ctz(m);
+                    const int sh = 60 - 4 * (j & 15);
+                    k[j >> 4] = (k[j >> 4] & ~(0xFull << sh)) | (dd << sh);
+                }
+                key[2 * (size_t)v] = k[0];
+                key[2 * (size_t)v + 1] = k[1];
+                *flag = 1;
+            }
+        }
+    }
+}
+
+// Fills g->lm_key from the device CSR g->d (device ids; rp: its host row
+// offsets).  Only shallow graphs get keys: deep ones run the source-parallel
+// step, which takes sources in caller order.
+static int lm_keys(cent_ctx* ctx, cent_graph* g, const int32_t* rp) {
+    const int32_t n = g->n;
+    if (n < 2 || g->ecc_probe > kDeepLevels) return CENT_OK;
+    const int L = n < kLandmarks ? (int)n : kLandmarks;
+    std::vector<int32_t> ids((size_t)n);
+    for (int32_t v = 0; v < n; ++v) ids[(size_t)v] = v;
+    std::partial_sort(ids.begin(), ids.begin() + L, ids.end(), [&](int32_t x, int32_t y) {
+        const int32_t dx = rp[x + 1] - rp[x], dy = rp[y + 1] - rp[y];
+        return dx != dy ? dx > dy : x < y;
+    });
+    // one scratch block: vis, two level planes (u32 per vertex), the keys
+    // (two u64 per vertex), the landmark list and the level flag
+    const size_t nn = (size_t)n;
+    const size_t off_key = (12 * nn + 15) / 16 * 16, off_lm = off_key + 16 * nn, off_flag = off_lm + 4 * kLandmarks;
+    char* buf = nullptr;
+    HIPTRY(hipMalloc((void**)&buf, off_flag + 4));
+    uint32_t* vis = reinterpret_cast<uint32_t*>(buf);
+    uint32_t* F[2] = {vis + nn, vis + 2 * nn};
+    uint64_t* key = reinterpret_cast<uint64_t*>(buf + off_key);
+    int32_t* lm = reinterpret_cast<int32_t*>(buf + off_lm);
+    int* flag = reinterpret_cast<int*>(buf + off_flag);
+    hipStream_t s = ctx->stream;
+    int rc = CENT_OK;
+    auto chk = [&](hipError_t e, const char* what) {
+        if (e != hipSuccess && rc == CENT_OK) {
+            set_error(ctx, e, what);
+            rc = e == hipErrorOutOfMemory ? CENT_E_OOM : CENT_E_DEVICE;
+        }
+        return rc == CENT_OK;
+    };
+    const uint32_t full = L == 32 ? 0xffffffffu : ((1u << L) - 1u);
+    if (chk(hipMemsetAsync(vis, 0, 8 * nn, s), "hipMemsetAsync(lm vis)") &&
+        chk(hipMemsetAsync(key, 0xff, 16 * nn, s), "hipMemsetAsync(lm key)") &&
+        chk(hipMemcpyAsync(lm, ids.data(), 4 * (size_t)L, hipMemcpyHostToDevice, s), "hipMemcpyAsync(lm)")) {
+        k_lm_seed<<<1, kLandmarks, 0, s>>>(lm, L, vis, F[0], key);
+        const int grid = (int)std::min<int64_t>(((int64_t)n + 3) / 4, 16384);
+        for (int d = 1; d <= n && rc == CENT_OK; ++d) {
+            int h = 0;
+            if (!chk(hipMemsetAsync(flag, 0, 4, s), "hipMemsetAsync(lm flag)")) break;
+            k_lm_level<<<grid, 256, 0, s>>>(g->d.rp, g->d.col, n, full, d, vis, F[(d - 1) & 1], F[d & 1], key, flag);
+            if (!chk(hipGetLastError(), "k_lm_level") ||
+                !chk(hipMemcpyAsync(&h, flag, 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(lm flag)") ||
+                !chk(hipStreamSynchronize(s), "hipStreamSynchronize(lm)"))
+                break;
+            if (!h) break;
+        }
+        if (rc == CENT_OK) {
+            g->lm_key.resize(2 * nn);
+            if (chk(hipMemcpyAsync(g->lm_key.data(), key, 16 * nn, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(lm key)"))
+                chk(hipStreamSynchronize(s), "hipStreamSynchronize(lm)");
+        }
+    }
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(buf);
+    if (rc) g->lm_key.clear();
+    return rc;
+}
+
 extern "C" int cent_graph_create(cent_ctx* ctx, const int32_t* row_ptr, const int32_t* col, int32_t n,
                                  int64_t nnz, int32_t flags, cent_graph** out) {
     if (!ctx || !out) return CENT_E_INVAL;
@@ -371,6 +492,11 @@ extern "C" int cent_graph_create(cent_ctx* ctx, const int32_t* row_ptr, const in
         return st;
     }
     g->d.multi = g->multi;
+    st = lm_keys(ctx, g, urp);
+    if (st) {
+        cent_graph_free(g);
+        return st;
+    }
     {   // inline rows for the source-parallel step: a vertex of degree <= 4 holds its neighbours in one
         // 16-byte row (-1 padded); a longer row is marked and read from the CSR
         std::vector<int4> ell((size_t)(n > 0 ? n : 1));

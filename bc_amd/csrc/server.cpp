@@ -138,8 +138,11 @@ static int64_t workspace_estimate(const GraphEntry* g, const Request* r) {
 }
 
 static int64_t graph_estimate(const GraphEntry* g) {
-    // CSR, the 16-byte inline rows, the id map and the chunk / hub tables
-    return 4 * (g->n + 1) + 4 * g->nnz + 16 * g->n + 4 * g->n + g->n / 4 + g->nnz / 16 + (1 << 20);
+    // CSR, then the larger of the upload's grouping-key scratch (28 bytes a
+    // vertex, released before the rest is built) and the 16-byte inline
+    // rows, the id map and the chunk / hub tables
+    return 4 * (g->n + 1) + 4 * g->nnz + std::max<int64_t>(28 * g->n, 20 * g->n + g->n / 4 + g->nnz / 16) +
+           (1 << 20);
 }
 
 static int64_t reduced_estimate(const cent_reduced* r) {
