@@ -628,6 +628,76 @@ extern "C" int cent_probe_copy_bandwidth(cent_ctx* ctx, int64_t bytes, int32_t i
     return rc;
 }
 
+// Random-gather probe: groups of L lanes read L consecutive doubles at a
+// random L*8-byte aligned position (a hash of the group and step), four
+// reads in flight per lane, so the rate is that of independent gathers of
+// L*8-byte pieces, like the dense step's slices of sigma / coef rows.
+__device__ __forceinline__ uint32_t probe_hash(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352dU;
+    x ^= x >> 15;
+    x *= 0x846ca68bU;
+    x ^= x >> 16;
+    return x;
+}
+
+template <int L>
+__global__ __launch_bounds__(256) void k_gather_probe(const double* __restrict__ a, uint32_t npieces, int steps,
+                                                      double* __restrict__ sink) {
+    constexpr int U = 4;
+    const uint32_t grp = (blockIdx.x * blockDim.x + threadIdx.x) / L;
+    const int sub = (int)(threadIdx.x % L);
+    double acc = 0.0;
+    for (int it = 0; it < steps; it += U) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t h = probe_hash(grp * 0x9E3779B9u + (uint32_t)(it + u) * 0x85EBCA6Bu);
+            v[u] = a[(size_t)(h % npieces) * L + sub];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc += v[u];
+    }
+    if (acc != 0.0) sink[0] = acc;  // the buffer is zero: never taken, keeps the loads
+}
+
+extern "C" int cent_probe_gather_bandwidth(cent_ctx* ctx, int64_t bytes, int32_t piece, int32_t iters, double* gbs) {
+    if (!ctx || !gbs || iters < 1 || (piece != 32 && piece != 64 && piece != 128 && piece != 256 && piece != 512) ||
+        bytes < 4 * piece || bytes / piece > (int64_t)UINT32_MAX)
+        return CENT_E_INVAL;
+    TRY(check_device(ctx));
+    double* a = nullptr;
+    HIPTRY(hipMalloc((void**)&a, (size_t)bytes + 64));
+    double* sink = a + bytes / 8;
+    ctx->ev_used = 0;
+    hipEvent_t e0 = next_event(ctx), e1 = next_event(ctx);
+    int rc = CENT_OK;
+    if (!e0 || !e1 || hipMemsetAsync(a, 0, (size_t)bytes + 64, ctx->stream) != hipSuccess) rc = CENT_E_DEVICE;
+    const int grid = 256 * 32, steps = 64;
+    const uint32_t np = (uint32_t)(bytes / piece);
+    auto launch = [&] {
+        switch (piece) {
+            case 32: k_gather_probe<4><<<grid, 256, 0, ctx->stream>>>(a, np, steps, sink); break;
+            case 64: k_gather_probe<8><<<grid, 256, 0, ctx->stream>>>(a, np, steps, sink); break;
+            case 128: k_gather_probe<16><<<grid, 256, 0, ctx->stream>>>(a, np, steps, sink); break;
+            case 256: k_gather_probe<32><<<grid, 256, 0, ctx->stream>>>(a, np, steps, sink); break;
+            default: k_gather_probe<64><<<grid, 256, 0, ctx->stream>>>(a, np, steps, sink); break;
+        }
+    };
+    if (!rc) {
+        launch();  // warm
+        (void)hipEventRecord(e0, ctx->stream);
+        for (int i = 0; i < iters; ++i) launch();
+        (void)hipEventRecord(e1, ctx->stream);
+        float ms = 0;
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = CENT_E_DEVICE;
+        else *gbs = (double)grid * 256 * steps * 8 * iters / (ms * 1e-3) / 1e9;
+    }
+    ctx->ev_used = 0;
+    (void)hipFree(a);
+    return rc;
+}
+
 // ------------------------------------------------------------ device arrays
 extern "C" int cent_device_alloc(cent_ctx* ctx, int64_t bytes, void** out) {
     if (!ctx || !out || bytes < 0) return CENT_E_INVAL;

@@ -558,13 +558,19 @@ EXTRA = {"rmat14": extra_rmat14, "blocktree": extra_blocktree, "blocktree-bc": e
          "closeness": extra_closeness, "reduce": extra_reduce}
 
 # Sources whose text decides the dense step's HBM traffic per launch: its
-# kernels, the level-loop section of levels.hip that launches them, and the
-# build flags.  The counter figures in profiles/rmat20_traffic.json are used
-# only while their key (this hash, the launch layout, the workload) matches
-# the tree.
+# kernels, the level-loop section of levels.hip that launches them, the
+# batch grouping (the sort of a call's sources and the landmark keys the
+# upload computes for it), and the build flags.  The counter figures in
+# profiles/rmat20_traffic.json are used only while their key (this hash, the
+# launch layout, the workload) matches the tree.
 TRAFFIC_SOURCES = ("bc_amd/csrc/msbfs_kernels.h", "bc_amd/csrc/wave.h", "bc_amd/Makefile")
-TRAFFIC_SECTION = ("bc_amd/csrc/levels.hip", "// ------------------------------------------------------------ level loop",
-                   "// ------------------------------------------------------------ frontier-queue mode")
+TRAFFIC_SECTIONS = (
+    ("bc_amd/csrc/levels.hip", "// ------------------------------------------------------------ level loop",
+     "// ------------------------------------------------------------ frontier-queue mode"),
+    ("bc_amd/csrc/levels.hip", "    // Batch order of the sources:", "    // pinned staging: device-order sources in"),
+    ("bc_amd/csrc/engine.hip", "// ------------------------------------------------------------ grouping key",
+     'extern "C" int cent_graph_create('),
+)
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "rmat20_traffic.json")
 
 
@@ -574,9 +580,9 @@ def kernel_sha16() -> str:
     for f in TRAFFIC_SOURCES:
         h.update(f.encode())
         h.update(open(os.path.join(ROOT, f), "rb").read())
-    f, begin, end = TRAFFIC_SECTION
-    text = open(os.path.join(ROOT, f)).read()
-    h.update(text[text.index(begin):text.index(end)].encode())
+    for f, begin, end in TRAFFIC_SECTIONS:
+        text = open(os.path.join(ROOT, f)).read()
+        h.update(text[text.index(begin):text.index(end)].encode())
     return h.hexdigest()[:16]
 
 
@@ -864,6 +870,14 @@ def main() -> int:
                              "basis": "lower-bound bytes model per launch (no counters for this tree)"})
     if args.copy_probe:
         roofline["copy_probe_GBs"] = round(ctx.probe_copy_bandwidth(1 << 30, 10), 1)
+    else:
+        # ceilings of this access pattern on this GPU, measured now: a plain
+        # copy, and random gathers of whole 128-byte lines and of 64-byte
+        # halves (the fabric moves whole lines, so a 64-byte slice of a
+        # sigma row costs a line)
+        roofline["ceilings_GBs"] = {"copy": round(ctx.probe_copy_bandwidth(1 << 30, 10), 1),
+                                    "gather_128B": round(ctx.probe_gather_bandwidth(128, 1 << 30, 10), 1),
+                                    "gather_64B": round(ctx.probe_gather_bandwidth(64, 1 << 30, 10), 1)}
 
     cpu = None
     if not args.no_cpu_baseline:

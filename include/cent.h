@@ -202,6 +202,12 @@ const char* cent_ctx_last_error(const cent_ctx* ctx);
  * plain device copy of `bytes` bytes and returns read+write GB/s.  Serves as
  * the calibration kernel for HBM counter readings. */
 int  cent_probe_copy_bandwidth(cent_ctx* ctx, int64_t bytes, int32_t iters, double* gbs);
+/* Measurement helper (no reference counterpart): times `iters` launches of
+ * random gathers from a zeroed buffer of `bytes` bytes, each gather reading
+ * one `piece`-byte aligned piece (32, 64, 128, 256 or 512), four in flight
+ * per lane, and returns the useful bytes read per second in GB/s: the
+ * ceiling for gathers of row slices of that size. */
+int  cent_probe_gather_bandwidth(cent_ctx* ctx, int64_t bytes, int32_t piece, int32_t iters, double* gbs);
 
 /* Uploads a CSR (validated; ids in range, offsets monotone).  By default the
  * device copy is relabelled for cache locality: by descending degree when
