@@ -11,18 +11,23 @@
 // Betweenness (k_ps_bc), per slot:
 //   R[v]      16-byte record, so a neighbour's level and path count share one
 //             cache line:
-//     .dist   int32   BFS level, -1 = unreached (restored after each source)
+//     .dist   int32   epoch-tagged BFS level: epoch of the slot's current
+//                     source << 16 | level; any other value = unreached,
+//                     so nothing is restored between sources
 //     .npred  uint32  predecessor count (per-predecessor formula only)
 //     .sig    double  path count; the backward overwrites it in place with
 //                     coef = (1 + delta) / sigma once v's delta is known
 //   Q[]       int32   BFS order; level l is Q[loff[l] .. loff[l+1])
 //   bcp[v]    double  this slot's BC partial over its sources
-// Forward, level d: each vertex u of the level scans its row; a neighbour w
-// with dist -1 is claimed by compare-and-swap (dist := d + 1, appended to
-// Q), and every w at level d + 1 gets sig[w] += sig[u] by an FP64 atomic
-// add.  Path counts are integers, exact in FP64 below 2^53, so the sum does
-// not depend on the order of the adds.  This is the level step of the
-// reference's queue BFS (Seq_Brandes/Brandes.c:274-298) for one source.
+// Forward, level d: each vertex u of the level scans its row once.  A
+// neighbour at level d - 1 adds its path count to u's (a pull, in row
+// order: sigma[u] = sum of sigma[pred], Seq_Brandes/Brandes.c:286-291), and
+// an unreached neighbour is claimed for level d + 1 by compare-and-swap on
+// its tag (appended to Q).  u's count is written once, by u's thread, and
+// read by level d + 1 after the level's barrier: no FP64 atomics, and the
+// record line of u is not read back before it is written.  This is the
+// level step of the reference's queue BFS (Brandes.c:274-298) for one
+// source.
 // Backward, level d from the deepest up: v sums coef[w] over its neighbours
 // w at level d + 1 in row order (delta = sigma * sum, Brandes.c:317-327 in
 // textbook form), then coef[v] replaces sig[v] and bcp[v] += delta.  The
@@ -51,12 +56,12 @@
 //
 // Path counts that grow past 2^512 (deep graphs with many parallel
 // shortest paths, e.g. chains of diamonds) would leave FP64 a few levels
-// later.  A source whose forward reads such a count is undone (its
-// records restored and its distance sums taken back) and run again in
-// scaled form: before each level is expanded, the level's largest count
-// exponent E is found and, when E >= 2^512's, every count read from the
-// level is multiplied by 2^(256 - E) (exact, a power of two), the shift
-// being kept per level in the slot's table; the backward multiplies that
+// later.  A source whose forward writes such a count is undone (its
+// distance sums taken back, its records left to the next epoch) and run
+// again in scaled form: once a level's counts are written, their largest
+// exponent E is known and, when E >= 512, the next level multiplies the
+// sums it reads from the level by 2^(256 - E) (exact, a power of two), the
+// shift being kept per level in the slot's table; the backward multiplies that
 // level's dependencies by the same factor (delta_d = sigma~_d * 2^-k_d *
 // sum coef~_{d+1}, with coef~ = (1 + delta) / sigma~ of the stored counts).
 // Sources whose counts stay below 2^512 run as before, bit for bit.
@@ -120,37 +125,50 @@ struct PsLocal {
 };
 
 // ============================================================ betweenness
-// Visit edge u -> w at forward level d.
-template <bool COMPAT>
-__device__ __forceinline__ void ps_visit(const PsArgs& a, PsRec* R, int* Q,
-                                         int w, double su, int d, int le, int* s_cnt, PsLocal& L) {
-    int lw = ld_dist(&R[w].dist);
-    if (lw < 0) {
-        const int old = atomicCAS(&R[w].dist, -1, d + 1);
-        if (old < 0) {
-            const int pos = le + atomicAdd(s_cnt, 1);
-            Q[pos] = w;
-            L.far += a.weight ? (int64_t)(d + 1) * a.weight[w] : (int64_t)(d + 1);
-            L.reached++;
-            if (a.dist_sum) atomicAdd((unsigned long long*)&a.dist_sum[w], (unsigned long long)(d + 1));
-            lw = d + 1;
-        } else {
-            lw = old;
-        }
-    }
-    if (lw == d + 1) {
-        atomicAdd(&R[w].sig, su);
-        if constexpr (COMPAT) atomicAdd(&R[w].npred, 1u);
-        L.dag++;
+// Levels carry the slot's source epoch: R[v].dist = ep << kPsLvBits | level
+// for a vertex the slot's current source has reached, any other value (an
+// older epoch, or -1 from the call's initial fill) for one it has not.  The
+// slot bumps ep for every source it takes, so no record is restored between
+// sources; before ep would pass kPsEpochMax the slot sets its levels back
+// to -1 and starts again at epoch 1.
+constexpr int kPsLvBits = 16;               // levels below 2^16 (the level table holds at most 65536)
+constexpr int kPsEpochMax = (1 << 15) - 1;  // keeps every tag non-negative
+constexpr int kNoEdge = INT32_MIN;          // tag of a staged slot with no edge: never stored
+constexpr int kNoLevel = INT32_MIN + 1;     // predecessor tag at level 0: never stored, never kNoEdge
+
+__device__ __forceinline__ int ps_tag(int ep, int l) { return (ep << kPsLvBits) | l; }
+__device__ __forceinline__ bool ps_seen(int t, int ep) { return t >= 0 && (t >> kPsLvBits) == ep; }
+
+// Bookkeeping of a vertex w claimed for level d + 1: appended to the next
+// level's part of Q, its level added to the source's farness (and to w's
+// distance sum when asked for).
+__device__ __forceinline__ void ps_claimed(const PsArgs& a, int* Q, int w, int d, int le, int* s_cnt, PsLocal& L) {
+    Q[le + atomicAdd(s_cnt, 1)] = w;
+    L.far += a.weight ? (int64_t)(d + 1) * a.weight[w] : (int64_t)(d + 1);
+    L.reached++;
+    if (a.dist_sum) atomicAdd((unsigned long long*)&a.dist_sum[w], (unsigned long long)(d + 1));
+}
+
+// Edge u -> w while u (level d) is expanded: w at level d - 1 (tag tp) adds
+// its path count to u's sum; an unreached w is claimed for level d + 1
+// (tag tn).
+__device__ __forceinline__ void ps_visit(const PsArgs& a, PsRec* R, int* Q, int w, int ep, int d, int tp, int tn,
+                                         int le, int* s_cnt, PsLocal& L, double& acc, uint32_t& np) {
+    const int t = ld_dist(&R[w].dist);
+    if (t == tp) {
+        acc += ld_sig(&R[w].sig);
+        ++np;
+    } else if (!ps_seen(t, ep) && atomicCAS(&R[w].dist, t, tn) == t) {
+        ps_claimed(a, Q, w, d, le, s_cnt, L);
     }
 }
 
-// Forward edges of up to kPsV rows at once (edges past the first kPsE of a
-// row finish in a plain loop): all column loads, then all level loads, then
-// all claims, then the bookkeeping, so one thread keeps up to kPsV * kPsE
-// independent chains in flight instead of walking its rows edge by edge.
-constexpr int kNoEdge = -2;
-constexpr int kClaimed = -3;
+// Expansion of up to kPsV rows at once (edges past the first kPsE of a row
+// finish in a plain loop): all column loads, then all level loads, then the
+// predecessors' path counts and the claims, then the bookkeeping, so one
+// thread keeps up to kPsV * kPsE independent chains in flight instead of
+// walking its rows edge by edge.  acc[m] / np[m]: row m's sum of its
+// predecessors' path counts (in row order) and their number.
 constexpr int kEllLong = INT32_MIN;  // inline-row marker: the row is read from rp / col
 static_assert(kPsE == 4, "inline rows hold kPsE = 4 neighbours");
 
@@ -181,11 +199,13 @@ __device__ __forceinline__ int ps_nbr(const PsArgs& a, const int* wi, int rb, in
     return wi[0] != kEllLong ? wi[e] : (e < dg ? a.col[rb + e] : -1);
 }
 
-template <bool COMPAT>
 __device__ __forceinline__ void ps_expand_rec(const PsArgs& a, PsRec* R, int* Q, const int (*wi)[kPsE],
-                                               const int* rb, const int* dg, const double* su, int d, int le,
-                                               int* s_cnt, PsLocal& L) {
-    int w[kPsV][kPsE], lev[kPsV][kPsE], old[kPsV][kPsE];
+                                              const int* rb, const int* dg, int ep, int d, int tp, int le,
+                                              int* s_cnt, PsLocal& L, double* acc, uint32_t* np) {
+    const int tn = ps_tag(ep, d + 1);
+    int w[kPsV][kPsE], t[kPsV][kPsE];
+    double x[kPsV][kPsE];
+    bool cl[kPsV][kPsE];
 #pragma unroll
     for (int m = 0; m < kPsV; ++m)
 #pragma unroll
@@ -193,84 +213,78 @@ __device__ __forceinline__ void ps_expand_rec(const PsArgs& a, PsRec* R, int* Q,
 #pragma unroll
     for (int m = 0; m < kPsV; ++m)
 #pragma unroll
-        for (int e = 0; e < kPsE; ++e) lev[m][e] = w[m][e] >= 0 ? ld_dist(&R[w[m][e]].dist) : kNoEdge;
+        for (int e = 0; e < kPsE; ++e) t[m][e] = w[m][e] >= 0 ? ld_dist(&R[w[m][e]].dist) : kNoEdge;
 #pragma unroll
     for (int m = 0; m < kPsV; ++m)
-#pragma unroll
-        for (int e = 0; e < kPsE; ++e) old[m][e] = lev[m][e] == -1 ? atomicCAS(&R[w[m][e]].dist, -1, d + 1) : 0;
-#pragma unroll
-    for (int m = 0; m < kPsV; ++m)
-#pragma unroll
-        for (int e = 0; e < kPsE; ++e)
-            if (lev[m][e] == -1) lev[m][e] = old[m][e] < 0 ? kClaimed : old[m][e];
-#pragma unroll
-    for (int m = 0; m < kPsV; ++m) {
 #pragma unroll
         for (int e = 0; e < kPsE; ++e) {
-            const int x = w[m][e];
-            int lv = lev[m][e];
-            if (lv == kClaimed) {
-                Q[le + atomicAdd(s_cnt, 1)] = x;
-                L.far += a.weight ? (int64_t)(d + 1) * a.weight[x] : (int64_t)(d + 1);
-                L.reached++;
-                if (a.dist_sum) atomicAdd((unsigned long long*)&a.dist_sum[x], (unsigned long long)(d + 1));
-                lv = d + 1;
-            }
-            if (lv == d + 1) {
-                atomicAdd(&R[x].sig, su[m]);
-                if constexpr (COMPAT) atomicAdd(&R[x].npred, 1u);
-                L.dag++;
+            const bool pred = w[m][e] >= 0 && t[m][e] == tp;
+            x[m][e] = pred ? ld_sig(&R[w[m][e]].sig) : 0.0;
+            cl[m][e] = w[m][e] >= 0 && !pred && !ps_seen(t[m][e], ep) &&
+                       atomicCAS(&R[w[m][e]].dist, t[m][e], tn) == t[m][e];
+        }
+#pragma unroll
+    for (int m = 0; m < kPsV; ++m) {
+        double s = 0.0;
+        uint32_t c = 0;
+#pragma unroll
+        for (int e = 0; e < kPsE; ++e) {
+            if (w[m][e] >= 0 && t[m][e] == tp) {
+                s += x[m][e];
+                ++c;
+            } else if (cl[m][e]) {
+                ps_claimed(a, Q, w[m][e], d, le, s_cnt, L);
             }
         }
-        for (int e = kPsE; e < dg[m]; ++e)
-            ps_visit<COMPAT>(a, R, Q, a.col[rb[m] + e], su[m], d, le, s_cnt, L);
+        for (int e = kPsE; e < dg[m]; ++e) ps_visit(a, R, Q, a.col[rb[m] + e], ep, d, tp, tn, le, s_cnt, L, s, c);
+        acc[m] = s;
+        np[m] = c;
     }
 }
 
-// Forward edges of a long row [rb, re) for one lane of a wavefront: edges
-// j, j + 64, ... taken kPsHubE at a time, staged like ps_expand_rec (column
-// loads, level loads, claims, bookkeeping), so a lane keeps kPsHubE
-// independent chains in flight instead of one.
-template <bool COMPAT>
-__device__ __forceinline__ void ps_expand_long(const PsArgs& a, PsRec* R, int* Q, int rb, int re, int lane,
-                                               double su, int d, int le, int* s_cnt, PsLocal& L) {
+// Expansion of a long row [rb, re) for one lane of a wavefront: edges j,
+// j + 64, ... taken kPsHubE at a time, staged like ps_expand_rec.  Returns
+// the lane's partial sum of predecessor path counts (its edges in order);
+// np counts them.
+__device__ __forceinline__ double ps_expand_long(const PsArgs& a, PsRec* R, int* Q, int rb, int re, int lane, int ep,
+                                                 int d, int tp, int le, int* s_cnt, PsLocal& L, uint32_t& np) {
+    const int tn = ps_tag(ep, d + 1);
+    double acc = 0.0;
     for (int j0 = rb + lane; j0 < re; j0 += kWave * kPsHubE) {
-        int w[kPsHubE], lev[kPsHubE], old[kPsHubE];
+        int w[kPsHubE], t[kPsHubE];
+        double x[kPsHubE];
+        bool cl[kPsHubE];
 #pragma unroll
         for (int e = 0; e < kPsHubE; ++e) {
             const int j = j0 + e * kWave;
             w[e] = j < re ? a.col[j] : -1;
         }
 #pragma unroll
-        for (int e = 0; e < kPsHubE; ++e) lev[e] = w[e] >= 0 ? ld_dist(&R[w[e]].dist) : kNoEdge;
-#pragma unroll
-        for (int e = 0; e < kPsHubE; ++e) old[e] = lev[e] == -1 ? atomicCAS(&R[w[e]].dist, -1, d + 1) : 0;
+        for (int e = 0; e < kPsHubE; ++e) t[e] = w[e] >= 0 ? ld_dist(&R[w[e]].dist) : kNoEdge;
 #pragma unroll
         for (int e = 0; e < kPsHubE; ++e) {
-            const int x = w[e];
-            int lv = lev[e];
-            if (lv == -1) lv = old[e] < 0 ? kClaimed : old[e];
-            if (lv == kClaimed) {
-                Q[le + atomicAdd(s_cnt, 1)] = x;
-                L.far += a.weight ? (int64_t)(d + 1) * a.weight[x] : (int64_t)(d + 1);
-                L.reached++;
-                if (a.dist_sum) atomicAdd((unsigned long long*)&a.dist_sum[x], (unsigned long long)(d + 1));
-                lv = d + 1;
-            }
-            if (lv == d + 1) {
-                atomicAdd(&R[x].sig, su);
-                if constexpr (COMPAT) atomicAdd(&R[x].npred, 1u);
-                L.dag++;
+            const bool pred = w[e] >= 0 && t[e] == tp;
+            x[e] = pred ? ld_sig(&R[w[e]].sig) : 0.0;
+            cl[e] = w[e] >= 0 && !pred && !ps_seen(t[e], ep) && atomicCAS(&R[w[e]].dist, t[e], tn) == t[e];
+        }
+#pragma unroll
+        for (int e = 0; e < kPsHubE; ++e) {
+            if (w[e] >= 0 && t[e] == tp) {
+                acc += x[e];
+                ++np;
+            } else if (cl[e]) {
+                ps_claimed(a, Q, w[e], d, le, s_cnt, L);
             }
         }
     }
+    return acc;
 }
 
 // Backward partial sum of a long row [rb, re) for one lane: coef[w] over the
-// lane's edges j, j + 64, ... to neighbours at level `lvl`, kPsHubE edges'
+// lane's edges j, j + 64, ... to neighbours with tag tl, kPsHubE edges'
 // loads in flight at a time; fixed order, so the wavefront sum that follows
 // is deterministic.
-__device__ __forceinline__ double ps_pull_long(const PsArgs& a, const PsRec* R, int rb, int re, int lane, int lvl) {
+__device__ __forceinline__ double ps_pull_long(const PsArgs& a, const PsRec* R, int rb, int re, int lane, int tl) {
     double acc = 0.0;
     for (int j0 = rb + lane; j0 < re; j0 += kWave * kPsHubE) {
         int w[kPsHubE], lev[kPsHubE];
@@ -283,7 +297,7 @@ __device__ __forceinline__ double ps_pull_long(const PsArgs& a, const PsRec* R, 
 #pragma unroll
         for (int e = 0; e < kPsHubE; ++e) lev[e] = w[e] >= 0 ? ld_dist(&R[w[e]].dist) : kNoEdge;
 #pragma unroll
-        for (int e = 0; e < kPsHubE; ++e) x[e] = lev[e] == lvl ? ld_sig(&R[w[e]].sig) : 0.0;
+        for (int e = 0; e < kPsHubE; ++e) x[e] = lev[e] == tl ? ld_sig(&R[w[e]].sig) : 0.0;
 #pragma unroll
         for (int e = 0; e < kPsHubE; ++e) acc += x[e];
     }
@@ -291,9 +305,9 @@ __device__ __forceinline__ double ps_pull_long(const PsArgs& a, const PsRec* R, 
 }
 
 // Backward sums of up to kPsV rows at once: acc[m] = sum over row m's
-// neighbours w at level `lvl` of coef[w] (held in sig), in row order.
+// neighbours w with tag tl of coef[w] (held in sig), in row order.
 __device__ __forceinline__ void ps_pull_rec(const PsArgs& a, const PsRec* R, const int (*wi)[kPsE], const int* rb,
-                                             const int* dg, int lvl, double* acc) {
+                                             const int* dg, int tl, double* acc) {
     int w[kPsV][kPsE], lev[kPsV][kPsE];
     double x[kPsV][kPsE];
 #pragma unroll
@@ -307,7 +321,7 @@ __device__ __forceinline__ void ps_pull_rec(const PsArgs& a, const PsRec* R, con
 #pragma unroll
     for (int m = 0; m < kPsV; ++m)
 #pragma unroll
-        for (int e = 0; e < kPsE; ++e) x[m][e] = lev[m][e] == lvl ? ld_sig(&R[w[m][e]].sig) : 0.0;
+        for (int e = 0; e < kPsE; ++e) x[m][e] = lev[m][e] == tl ? ld_sig(&R[w[m][e]].sig) : 0.0;
 #pragma unroll
     for (int m = 0; m < kPsV; ++m) {
         double t = 0.0;
@@ -315,20 +329,20 @@ __device__ __forceinline__ void ps_pull_rec(const PsArgs& a, const PsRec* R, con
         for (int e = 0; e < kPsE; ++e) t += x[m][e];
         for (int e = kPsE; e < dg[m]; ++e) {
             const int y = a.col[rb[m] + e];
-            if (ld_dist(&R[y].dist) == lvl) t += ld_sig(&R[y].sig);
+            if (ld_dist(&R[y].dist) == tl) t += ld_sig(&R[y].sig);
         }
         acc[m] = t;
     }
 }
 
 template <bool COMPAT>
-__global__ __launch_bounds__(kPsBlock) void k_ps_bc(PsArgs a) {
+__global__ __launch_bounds__(kPsBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_ps_bc(PsArgs a) {
     __shared__ int s_cnt[3];     // appends of level d go to s_cnt[d % 3]
     __shared__ int s_nbig[3];    // long rows of level d: s_big[..s_nbig[d % 3])
+    __shared__ int s_emax[3];    // scaled form: largest count exponent written at level d
     __shared__ int s_big[kPsBig];
     __shared__ int64_t s_red64[kPsWaves][3];
-    __shared__ int s_large;      // a count read by the forward reached 2^512
-    __shared__ int s_emax;       // scaled form: largest count exponent of the level
+    __shared__ int s_large;      // a count of the unscaled form reached 2^512
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
     const int wid = tid >> 6;
@@ -339,77 +353,98 @@ __global__ __launch_bounds__(kPsBlock) void k_ps_bc(PsArgs a) {
     int* lsh = loff + a.lcap;    // scaled form: shift k_d of level d's counts
     double* bcp = a.bcp + so;
     int scaled_si = -1;          // the source now being redone in scaled form
+    int ep = 0;                  // epoch of the slot's current source
     for (int si = blockIdx.x; si < a.k; si += gridDim.x) {
+        if (ep == kPsEpochMax) {  // epochs spent: every level back to unreached
+            for (int v = tid; v < a.n; v += kPsBlock) R[v].dist = -1;
+            ep = 0;
+            __syncthreads();
+        }
+        ++ep;
         const int s = a.src[si];
         const bool scaled = si == scaled_si;
         if (tid == 0) {
             s_large = 0;
             Q[0] = s;
-            R[s].dist = 0;
+            R[s].dist = ps_tag(ep, 0);
             R[s].sig = 1.0;
             loff[0] = 0;
             loff[1] = 1;
             s_cnt[0] = s_cnt[1] = s_cnt[2] = 0;
             s_nbig[0] = s_nbig[1] = s_nbig[2] = 0;
+            s_emax[0] = s_emax[1] = s_emax[2] = 0;
         }
         __syncthreads();
         PsLocal L;
         int lb = 0, le = 1, d = 0;
         bool deep = false;
+        double scp = 1.0;  // scale of the counts of level d - 1, which level d sums
         // ---------------- forward
+        // Level d: every vertex u of the level scans its row once; it sums
+        // the path counts of its neighbours at level d - 1 (all final: they
+        // were written during level d - 1) into its own count, and claims
+        // its unreached neighbours for level d + 1.  No count is added to
+        // by atomics, and u's count is written once, by u's thread.
         for (;;) {
             const int c = d % 3;
             if (tid == 0) {  // free the buffers level d + 1 will use (last read two levels ago)
                 s_cnt[(d + 1) % 3] = 0;
                 s_nbig[(d + 1) % 3] = 0;
+                s_emax[(d + 1) % 3] = 0;
             }
-            double sc = 1.0;  // scale of level d's counts
-            if (scaled) {
-                if (tid == 0) s_emax = 0;
-                __syncthreads();
-                int e = 0;
-                for (int i = lb + tid; i < le; i += kPsBlock) e = max(e, dexp(ld_sig(&R[Q[i]].sig)));
-                if (e >= kSigBig) atomicMax(&s_emax, e);
-                __syncthreads();
-                const int kd = s_emax >= kSigBig ? s_emax - kSigTo : 0;
-                if (tid == 0) lsh[d] = kd;
-                sc = __longlong_as_double((long long)(1023 - kd) << 52);
-            }
+            const int tp = d > 0 ? ps_tag(ep, d - 1) : kNoLevel;
+            auto finish = [&](int u, double acc, uint32_t np) {  // u's count, from its predecessors
+                const double sg = acc * scp;  // exact: scp is a power of two
+                R[u].sig = sg;
+                if constexpr (COMPAT) R[u].npred = np;
+                if (!scaled) {
+                    if (sg >= 0x1p512) s_large = 1;
+                } else if (dexp(sg) >= kSigBig) {
+                    atomicMax(&s_emax[c], dexp(sg));
+                }
+            };
             for (int i0 = lb + tid; i0 < le; i0 += kPsBlock * kPsV) {
                 int u[kPsV], rb[kPsV], dg[kPsV], wi[kPsV][kPsE];
-                double su[kPsV];
 #pragma unroll
                 for (int m = 0; m < kPsV; ++m) {
                     const int i = i0 + m * kPsBlock;
                     u[m] = i < le ? Q[i] : -1;
                 }
 #pragma unroll
-                for (int m = 0; m < kPsV; ++m) {
-                    ps_row(a, u[m], wi[m], rb[m], dg[m]);
-                    su[m] = 0.0;
-                    if (u[m] >= 0) {
-                        su[m] = ld_sig(&R[u[m]].sig) * sc;
-                        if (su[m] >= 0x1p512) s_large = 1;  // unscaled form only (scaled counts stay near 2^256)
-                    }
-                }
+                for (int m = 0; m < kPsV; ++m) ps_row(a, u[m], wi[m], rb[m], dg[m]);
+                bool big[kPsV];
 #pragma unroll
-                for (int m = 0; m < kPsV; ++m)
+                for (int m = 0; m < kPsV; ++m) {
+                    big[m] = false;
                     if (dg[m] > kPsBigDeg) {
                         const int slot = atomicAdd(&s_nbig[c], 1);
                         if (slot < kPsBig) {
                             s_big[slot] = u[m];
                             dg[m] = 0;
+                            big[m] = true;
                         }
                     }
-                ps_expand_rec<COMPAT>(a, R, Q, wi, rb, dg, su, d, le, &s_cnt[c], L);
+                }
+                double acc[kPsV];
+                uint32_t np[kPsV];
+                ps_expand_rec(a, R, Q, wi, rb, dg, ep, d, tp, le, &s_cnt[c], L, acc, np);
+#pragma unroll
+                for (int m = 0; m < kPsV; ++m) {
+                    if (u[m] < 0 || big[m]) continue;
+                    L.dag += np[m];
+                    if (d > 0) finish(u[m], acc[m], np[m]);
+                }
             }
             __syncthreads();
             const int nbig = min(s_nbig[c], kPsBig);
             for (int t = wid; t < nbig; t += kPsWaves) {
                 const int u = s_big[t];
-                const double su = ld_sig(&R[u].sig) * sc;
-                if (su >= 0x1p512) s_large = 1;
-                ps_expand_long<COMPAT>(a, R, Q, a.rp[u], a.rp[u + 1], lane, su, d, le, &s_cnt[c], L);
+                uint32_t np = 0;
+                const double acc = wave_sum(ps_expand_long(a, R, Q, a.rp[u], a.rp[u + 1], lane, ep, d, tp, le,
+                                                           &s_cnt[c], L, np));
+                L.dag += np;
+                const uint32_t npt = (uint32_t)wave_sum_i64((int64_t)np);
+                if (lane == 0 && d > 0) finish(u, acc, npt);
             }
             __syncthreads();
             const int nnew = s_cnt[c];
@@ -417,10 +452,15 @@ __global__ __launch_bounds__(kPsBlock) void k_ps_bc(PsArgs a) {
                 le += nnew;
                 break;
             }
+            if (scaled) {  // scale of level d's counts, for level d + 1's sums and the backward
+                const int kd = s_emax[c] >= kSigBig ? s_emax[c] - kSigTo : 0;
+                if (tid == 0) lsh[d] = kd;
+                scp = __longlong_as_double((long long)(1023 - kd) << 52);
+            }
             if (nnew == 0) break;
             if (d + 3 > a.lcap) {  // level offsets would not fit: the host reruns another way
                 deep = true;
-                le += nnew;        // so the restore below also covers the last level
+                le += nnew;
                 break;
             }
             lb = le;
@@ -431,14 +471,15 @@ __global__ __launch_bounds__(kPsBlock) void k_ps_bc(PsArgs a) {
         const int depth = d;
         const int tail = le;
         if (s_large) {
-            // records back to idle and this pass's distance sums taken back
-            // (integers, so the redo leaves them exactly as one pass would)
-            for (int i = tid; i < tail; i += kPsBlock) {
-                const int v = Q[i];
-                const int dv = R[v].dist;
-                if (a.dist_sum && dv > 0) atomicAdd((unsigned long long*)&a.dist_sum[v], (unsigned long long)(-(int64_t)dv));
-                R[v] = PsRec{-1, 0u, 0.0};
-            }
+            // this pass's distance sums taken back (integers, so the redo
+            // leaves them exactly as one pass would); its records are left
+            // to the next epoch
+            if (a.dist_sum)
+                for (int i = tid; i < tail; i += kPsBlock) {
+                    const int v = Q[i];
+                    const int dv = R[v].dist & ((1 << kPsLvBits) - 1);
+                    if (dv > 0) atomicAdd((unsigned long long*)&a.dist_sum[v], (unsigned long long)(-(int64_t)dv));
+                }
             __syncthreads();
             scaled_si = si;
             si -= gridDim.x;  // the same source again
@@ -453,6 +494,7 @@ __global__ __launch_bounds__(kPsBlock) void k_ps_bc(PsArgs a) {
                 const int c = dd % 3;
                 if (tid == 0) s_nbig[(dd + 2) % 3] = 0;  // the list level dd - 1 will use
                 const int b0 = loff[dd], b1 = loff[dd + 1];
+                const int tl = ps_tag(ep, dd + 1);
                 const double sc = scaled ? __longlong_as_double((long long)(1023 - lsh[dd]) << 52) : 1.0;
                 for (int i0 = b0 + tid; i0 < b1; i0 += kPsBlock * kPsV) {
                     int v[kPsV], rb[kPsV], dg[kPsV];
@@ -482,7 +524,7 @@ __global__ __launch_bounds__(kPsBlock) void k_ps_bc(PsArgs a) {
                         }
                     }
                     double acc[kPsV];
-                    ps_pull_rec(a, R, wi, rb, dg, dd + 1, acc);
+                    ps_pull_rec(a, R, wi, rb, dg, tl, acc);
 #pragma unroll
                     for (int m = 0; m < kPsV; ++m) {
                         if (v[m] < 0 || big[m]) continue;
@@ -494,7 +536,7 @@ __global__ __launch_bounds__(kPsBlock) void k_ps_bc(PsArgs a) {
                 }
                 __syncthreads();
                 auto long_row = [&](int v) {
-                    const double acc = wave_sum(ps_pull_long(a, R, a.rp[v], a.rp[v + 1], lane, dd + 1));
+                    const double acc = wave_sum(ps_pull_long(a, R, a.rp[v], a.rp[v + 1], lane, tl));
                     if (lane == 0) {
                         const double sv = ld_sig(&R[v].sig);
                         const double dl = (sv * acc) * sc;
@@ -515,7 +557,7 @@ __global__ __launch_bounds__(kPsBlock) void k_ps_bc(PsArgs a) {
                 __syncthreads();
             }
         }
-        // ---------------- per-source totals, then restore the slot
+        // ---------------- per-source totals
         int64_t far = L.far;
         int64_t rch = (int64_t)L.reached, dag = (int64_t)L.dag;
 #pragma unroll
@@ -528,9 +570,6 @@ __global__ __launch_bounds__(kPsBlock) void k_ps_bc(PsArgs a) {
             s_red64[wid][0] = far;
             s_red64[wid][1] = rch;
             s_red64[wid][2] = dag;
-        }
-        for (int i = tid; i < tail; i += kPsBlock) {
-            R[Q[i]] = PsRec{-1, 0u, 0.0};
         }
         __syncthreads();
         if (tid == 0) {
@@ -545,7 +584,7 @@ __global__ __launch_bounds__(kPsBlock) void k_ps_bc(PsArgs a) {
             atomicAdd(&a.cnt[1], (unsigned long long)g);
             atomicAdd(a.depth_sum, (unsigned long long)depth);
         }
-        __syncthreads();  // restores and totals done before the next source
+        __syncthreads();  // totals read before the next source writes s_red64
     }
 }
 
@@ -558,14 +597,6 @@ __device__ __forceinline__ bool bit_in(const uint32_t* M, int v) {
 __device__ __forceinline__ bool bit_claim(uint32_t* M, int v) {
     const uint32_t m = 1u << (v & 31);
     return (atomicOr(M + (v >> 5), m) & m) == 0u;
-}
-
-// w has just been claimed for level d + 1.
-__device__ __forceinline__ void ps_claimed(const PsArgs& a, int* Q, int w, int d, int le, int* s_cnt, PsLocal& L) {
-    Q[le + atomicAdd(s_cnt, 1)] = w;
-    L.far += a.weight ? (int64_t)(d + 1) * a.weight[w] : (int64_t)(d + 1);
-    L.reached++;
-    if (a.dist_sum) atomicAdd((unsigned long long*)&a.dist_sum[w], (unsigned long long)(d + 1));
 }
 
 // Expands up to kPsV rows at once, as ps_expand_rec: all column loads, then

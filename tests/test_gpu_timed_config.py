@@ -9,6 +9,11 @@
 - The block-tree workload at full size (2000 blocks, seed 3): all-source BC
   and farness through the D1/AP-reduced form against the oracle's plain
   Brandes over every vertex.
+- The high-diameter workload at full size: the 2048 x 2048 grid (5% of edges
+  removed, LCC, 1000 shortcuts, seed 4) with its 1024 seed-4 sources, on the
+  launch bench.py times (tuning table loaded, default level step): farness
+  and distance sums bit-exact and BC within 1e-9 of the oracle over all 1024
+  sources, and the closeness-only call's farness equal.
 - A reduced graph whose device copy failed half-way is rebuilt by the next
   call instead of being reused (the failure is injected on the host).
 """
@@ -81,6 +86,29 @@ def test_blocktree_2000_reduced_bc_matches_oracle(ctx):
     bc, far = ctx.bc_reduced(redg)
     assert np.array_equal(far, ofar)
     assert _bc_close(bc, obc)[1]
+
+
+@pytest.mark.timeout(900)
+def test_grid2048_1024_sources_matches_oracle():
+    g = bc_amd.gen_grid(2048, 2048, 0.05, 1000, seed=4)
+    src = bc_amd.sample_sources(g.n, 1024, 4)
+    print(f"[gpu] grid 2048^2 n={g.n} m={g.m}, 1024 sources, tuned, default step", flush=True)
+    bc_amd.tune_table_load()
+    try:
+        ctx = bc_amd.Context(0)
+        dg = ctx.upload(g)
+        bc, far, ds = ctx.bc(dg, src, dist_sum=True)
+        cfar, cds = ctx.cc(dg, src, dist_sum=True)
+    finally:
+        bc_amd.tune_table_clear()
+    print(f"[gpu] grid done; oracle Brandes on {_threads()} threads", flush=True)
+    obc, ofar, ods, _ = O.brandes(g, src, threads=_threads())
+    assert np.array_equal(far, ofar), np.nonzero(far != ofar)[0][:10]
+    assert np.array_equal(ds, ods), np.nonzero(ds != ods)[0][:10]
+    assert np.array_equal(cfar, ofar) and np.array_equal(cds, ods)
+    err, ok = _bc_close(bc, obc)
+    print(f"[gpu] grid: farness and distance sums exact, BC max rel err {err:.2e}", flush=True)
+    assert ok, err
 
 
 def test_reduced_device_copy_rebuilt_after_failed_upload(ctx):
