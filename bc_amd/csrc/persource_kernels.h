@@ -14,7 +14,10 @@
 //     .dist   int32   epoch-tagged BFS level: epoch of the slot's current
 //                     source << 16 | level; any other value = unreached,
 //                     so nothing is restored between sources
-//     .npred  uint32  predecessor count (per-predecessor formula only)
+//     .npred  uint32  low 28 bits: predecessor count (per-predecessor
+//                     formula); high 4 bits: which of a row of at most 4
+//                     edges lead to level + 1 (its successors), so the
+//                     backward reads only the successors' records
 //     .sig    double  path count; the backward overwrites it in place with
 //                     coef = (1 + delta) / sigma once v's delta is known
 //   Q[]       int32   BFS order; level l is Q[loff[l] .. loff[l+1])
@@ -137,6 +140,10 @@ constexpr int kNoEdge = INT32_MIN;          // tag of a staged slot with no edge
 constexpr int kNoLevel = INT32_MIN + 1;     // predecessor tag at level 0: never stored, never kNoEdge
 
 __device__ __forceinline__ int ps_tag(int ep, int l) { return (ep << kPsLvBits) | l; }
+
+// npred word: predecessor count and the successor bits of a short row.
+constexpr int kPsSuccShift = 28;
+constexpr uint32_t kPsCountMask = (1u << kPsSuccShift) - 1u;
 __device__ __forceinline__ bool ps_seen(int t, int ep) { return t >= 0 && (t >> kPsLvBits) == ep; }
 
 // Bookkeeping of a vertex w claimed for level d + 1: appended to the next
@@ -199,13 +206,17 @@ __device__ __forceinline__ int ps_nbr(const PsArgs& a, const int* wi, int rb, in
     return wi[0] != kEllLong ? wi[e] : (e < dg ? a.col[rb + e] : -1);
 }
 
+// True for a row staged by ps_row whose edges all sit in the staged slots
+// (an inline row, or a CSR row of at most kPsE edges): its successor bits
+// cover the whole row.
+__device__ __forceinline__ bool ps_short(const int* wi, int dg) { return wi[0] != kEllLong || dg <= kPsE; }
+
 __device__ __forceinline__ void ps_expand_rec(const PsArgs& a, PsRec* R, int* Q, const int (*wi)[kPsE],
                                               const int* rb, const int* dg, int ep, int d, int tp, int le,
-                                              int* s_cnt, PsLocal& L, double* acc, uint32_t* np) {
+                                              int* s_cnt, PsLocal& L, double* acc, uint32_t* np, uint32_t* sb) {
     const int tn = ps_tag(ep, d + 1);
     int w[kPsV][kPsE], t[kPsV][kPsE];
     double x[kPsV][kPsE];
-    bool cl[kPsV][kPsE];
 #pragma unroll
     for (int m = 0; m < kPsV; ++m)
 #pragma unroll
@@ -215,24 +226,32 @@ __device__ __forceinline__ void ps_expand_rec(const PsArgs& a, PsRec* R, int* Q,
 #pragma unroll
         for (int e = 0; e < kPsE; ++e) t[m][e] = w[m][e] >= 0 ? ld_dist(&R[w[m][e]].dist) : kNoEdge;
 #pragma unroll
-    for (int m = 0; m < kPsV; ++m)
+    for (int m = 0; m < kPsV; ++m) {
+        sb[m] = 0u;
 #pragma unroll
         for (int e = 0; e < kPsE; ++e) {
             const bool pred = w[m][e] >= 0 && t[m][e] == tp;
             x[m][e] = pred ? ld_sig(&R[w[m][e]].sig) : 0.0;
-            cl[m][e] = w[m][e] >= 0 && !pred && !ps_seen(t[m][e], ep) &&
-                       atomicCAS(&R[w[m][e]].dist, t[m][e], tn) == t[m][e];
+            // An unreached neighbour is claimed; whichever claim wins, it is
+            // at level d + 1 once the level is done, a successor, as is one
+            // already claimed.  t becomes the claim's old value: stale
+            // (not of this epoch) when this claim won, tn when another won.
+            const bool tr = w[m][e] >= 0 && !pred && !ps_seen(t[m][e], ep);
+            if (tr || t[m][e] == tn) sb[m] |= 1u << e;
+            if (tr) t[m][e] = atomicCAS(&R[w[m][e]].dist, t[m][e], tn);
         }
+    }
 #pragma unroll
     for (int m = 0; m < kPsV; ++m) {
         double s = 0.0;
         uint32_t c = 0;
 #pragma unroll
         for (int e = 0; e < kPsE; ++e) {
-            if (w[m][e] >= 0 && t[m][e] == tp) {
+            if (w[m][e] < 0) continue;
+            if (t[m][e] == tp) {
                 s += x[m][e];
                 ++c;
-            } else if (cl[m][e]) {
+            } else if (!ps_seen(t[m][e], ep)) {  // this claim won
                 ps_claimed(a, Q, w[m][e], d, le, s_cnt, L);
             }
         }
@@ -253,7 +272,6 @@ __device__ __forceinline__ double ps_expand_long(const PsArgs& a, PsRec* R, int*
     for (int j0 = rb + lane; j0 < re; j0 += kWave * kPsHubE) {
         int w[kPsHubE], t[kPsHubE];
         double x[kPsHubE];
-        bool cl[kPsHubE];
 #pragma unroll
         for (int e = 0; e < kPsHubE; ++e) {
             const int j = j0 + e * kWave;
@@ -265,14 +283,15 @@ __device__ __forceinline__ double ps_expand_long(const PsArgs& a, PsRec* R, int*
         for (int e = 0; e < kPsHubE; ++e) {
             const bool pred = w[e] >= 0 && t[e] == tp;
             x[e] = pred ? ld_sig(&R[w[e]].sig) : 0.0;
-            cl[e] = w[e] >= 0 && !pred && !ps_seen(t[e], ep) && atomicCAS(&R[w[e]].dist, t[e], tn) == t[e];
+            if (w[e] >= 0 && !pred && !ps_seen(t[e], ep)) t[e] = atomicCAS(&R[w[e]].dist, t[e], tn);  // as ps_expand_rec
         }
 #pragma unroll
         for (int e = 0; e < kPsHubE; ++e) {
-            if (w[e] >= 0 && t[e] == tp) {
+            if (w[e] < 0) continue;
+            if (t[e] == tp) {
                 acc += x[e];
                 ++np;
-            } else if (cl[e]) {
+            } else if (!ps_seen(t[e], ep)) {  // this claim won
                 ps_claimed(a, Q, w[e], d, le, s_cnt, L);
             }
         }
@@ -305,9 +324,11 @@ __device__ __forceinline__ double ps_pull_long(const PsArgs& a, const PsRec* R, 
 }
 
 // Backward sums of up to kPsV rows at once: acc[m] = sum over row m's
-// neighbours w with tag tl of coef[w] (held in sig), in row order.
+// neighbours w with tag tl of coef[w] (held in sig), in row order.  A short
+// row with its successor bits (mk[m] >= 0) reads only its successors'
+// counts; any other row (mk[m] < 0) tests every neighbour's level.
 __device__ __forceinline__ void ps_pull_rec(const PsArgs& a, const PsRec* R, const int (*wi)[kPsE], const int* rb,
-                                             const int* dg, int tl, double* acc) {
+                                             const int* dg, const int* mk, int tl, double* acc) {
     int w[kPsV][kPsE], lev[kPsV][kPsE];
     double x[kPsV][kPsE];
 #pragma unroll
@@ -317,11 +338,14 @@ __device__ __forceinline__ void ps_pull_rec(const PsArgs& a, const PsRec* R, con
 #pragma unroll
     for (int m = 0; m < kPsV; ++m)
 #pragma unroll
-        for (int e = 0; e < kPsE; ++e) lev[m][e] = w[m][e] >= 0 ? ld_dist(&R[w[m][e]].dist) : kNoEdge;
+        for (int e = 0; e < kPsE; ++e) lev[m][e] = mk[m] < 0 && w[m][e] >= 0 ? ld_dist(&R[w[m][e]].dist) : kNoEdge;
 #pragma unroll
     for (int m = 0; m < kPsV; ++m)
 #pragma unroll
-        for (int e = 0; e < kPsE; ++e) x[m][e] = lev[m][e] == tl ? ld_sig(&R[w[m][e]].sig) : 0.0;
+        for (int e = 0; e < kPsE; ++e) {
+            const bool on = mk[m] >= 0 ? w[m][e] >= 0 && ((mk[m] >> e) & 1) : lev[m][e] == tl;
+            x[m][e] = on ? ld_sig(&R[w[m][e]].sig) : 0.0;
+        }
 #pragma unroll
     for (int m = 0; m < kPsV; ++m) {
         double t = 0.0;
@@ -393,10 +417,12 @@ __global__ __launch_bounds__(kPsBlock) __attribute__((amdgpu_waves_per_eu(4))) v
                 s_emax[(d + 1) % 3] = 0;
             }
             const int tp = d > 0 ? ps_tag(ep, d - 1) : kNoLevel;
-            auto finish = [&](int u, double acc, uint32_t np) {  // u's count, from its predecessors
+            // u's count from its predecessors, with its predecessor count and
+            // successor bits
+            auto finish = [&](int u, double acc, uint32_t np, uint32_t sb) {
                 const double sg = acc * scp;  // exact: scp is a power of two
                 R[u].sig = sg;
-                if constexpr (COMPAT) R[u].npred = np;
+                R[u].npred = np | (sb << kPsSuccShift);
                 if (!scaled) {
                     if (sg >= 0x1p512) s_large = 1;
                 } else if (dexp(sg) >= kSigBig) {
@@ -426,13 +452,13 @@ __global__ __launch_bounds__(kPsBlock) __attribute__((amdgpu_waves_per_eu(4))) v
                     }
                 }
                 double acc[kPsV];
-                uint32_t np[kPsV];
-                ps_expand_rec(a, R, Q, wi, rb, dg, ep, d, tp, le, &s_cnt[c], L, acc, np);
+                uint32_t np[kPsV], sb[kPsV];
+                ps_expand_rec(a, R, Q, wi, rb, dg, ep, d, tp, le, &s_cnt[c], L, acc, np, sb);
 #pragma unroll
                 for (int m = 0; m < kPsV; ++m) {
                     if (u[m] < 0 || big[m]) continue;
                     L.dag += np[m];
-                    if (d > 0) finish(u[m], acc[m], np[m]);
+                    if (d > 0) finish(u[m], acc[m], np[m], sb[m]);
                 }
             }
             __syncthreads();
@@ -444,7 +470,7 @@ __global__ __launch_bounds__(kPsBlock) __attribute__((amdgpu_waves_per_eu(4))) v
                                                            &s_cnt[c], L, np));
                 L.dag += np;
                 const uint32_t npt = (uint32_t)wave_sum_i64((int64_t)np);
-                if (lane == 0 && d > 0) finish(u, acc, npt);
+                if (lane == 0 && d > 0) finish(u, acc, npt, 0u);
             }
             __syncthreads();
             const int nnew = s_cnt[c];
@@ -504,11 +530,14 @@ __global__ __launch_bounds__(kPsBlock) __attribute__((amdgpu_waves_per_eu(4))) v
                         const int i = i0 + m * kPsBlock;
                         v[m] = i < b1 ? Q[i] : -1;
                     }
-                    int wi[kPsV][kPsE];
+                    int wi[kPsV][kPsE], mk[kPsV];
+                    uint32_t pw[kPsV];
 #pragma unroll
                     for (int m = 0; m < kPsV; ++m) {
                         ps_row(a, v[m], wi[m], rb[m], dg[m]);
                         sv[m] = v[m] >= 0 ? ld_sig(&R[v[m]].sig) : 1.0;
+                        pw[m] = v[m] >= 0 ? ld_npred(&R[v[m]].npred) : 0u;
+                        mk[m] = v[m] >= 0 && ps_short(wi[m], dg[m]) ? (int)(pw[m] >> kPsSuccShift) : -1;
                     }
                     // long rows always go to the wavefronts (listed while the
                     // list has room), so each row's sum has one order whoever
@@ -524,14 +553,14 @@ __global__ __launch_bounds__(kPsBlock) __attribute__((amdgpu_waves_per_eu(4))) v
                         }
                     }
                     double acc[kPsV];
-                    ps_pull_rec(a, R, wi, rb, dg, tl, acc);
+                    ps_pull_rec(a, R, wi, rb, dg, mk, tl, acc);
 #pragma unroll
                     for (int m = 0; m < kPsV; ++m) {
                         if (v[m] < 0 || big[m]) continue;
                         const double dl = (sv[m] * acc[m]) * sc;
                         if (__builtin_isinf(sv[m])) atomicOr(a.overflow, 1);
                         R[v[m]].sig = (1.0 + dl) / sv[m];
-                        bcp[v[m]] += COMPAT ? dl * (double)ld_npred(&R[v[m]].npred) : dl;
+                        bcp[v[m]] += COMPAT ? dl * (double)(pw[m] & kPsCountMask) : dl;
                     }
                 }
                 __syncthreads();
@@ -542,7 +571,7 @@ __global__ __launch_bounds__(kPsBlock) __attribute__((amdgpu_waves_per_eu(4))) v
                         const double dl = (sv * acc) * sc;
                         if (__builtin_isinf(sv)) atomicOr(a.overflow, 1);
                         R[v].sig = (1.0 + dl) / sv;
-                        bcp[v] += COMPAT ? dl * (double)ld_npred(&R[v].npred) : dl;
+                        bcp[v] += COMPAT ? dl * (double)(ld_npred(&R[v].npred) & kPsCountMask) : dl;
                     }
                 };
                 const int nbig = s_nbig[c];
