@@ -75,10 +75,10 @@ struct cent_ctx {
     // largest request and reused by later calls
     DevBuf<char> red_arena;
     // source-parallel mode: per-slot arrays
-    DevBuf<PsRec> ps_rec;
+    DevBuf<char> ps_rec;     // PsRec or PsRecW records, by byte
+    DevBuf<double> ps_bcp;   // narrow records: the slots' BC partials
     DevBuf<uint32_t> ps_bits;
     DevBuf<int> ps_Q, ps_loff;
-    DevBuf<double> ps_bcp;
     DevBuf<unsigned long long> ps_depth;
     size_t F_levels = 0;     // levels the F buffer holds for the current N
     size_t F_n = 0;

@@ -178,7 +178,7 @@ extern "C" int64_t cent_ctx_workspace_bytes(const cent_ctx* ctx) {
                           ctx->bc.cap + ctx->hp_acc.cap + ctx->dist_sum.cap + ctx->far.cap + ctx->part_far.cap +
                           ctx->part_cnt.cap + ctx->cnt.cap + ctx->Fd2.cap + ctx->QF.cap + ctx->Adev.cap +
                           ctx->red_D.cap + ctx->red_part.cap + ctx->red_bcp.cap + ctx->it_sig.cap +
-                          ctx->it_coef.cap + 2 * ctx->ps_rec.cap + ctx->ps_bcp.cap + ctx->ps_depth.cap +
+                          ctx->it_coef.cap + ctx->ps_rec.cap / 8 + ctx->ps_bcp.cap + ctx->ps_depth.cap +
                           ctx->red_swgt.cap + ctx->red_poff.cap + 2 * ctx->red_items.cap) +
                      ctx->it_lvl.cap + ctx->red_arena.cap +
                      4 * (ctx->npred.cap + ctx->hp_cnt.cap + ctx->src.cap + ctx->flags.cap + ctx->ord.cap + ctx->Q.cap +

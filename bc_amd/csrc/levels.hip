@@ -505,7 +505,7 @@ static int level_batch_sparse(cent_ctx* ctx, SpPlan& pl, const BatchSpec& sp, co
 // else 256 — more concurrent traversals than that only contend for the
 // memory system — doubled up to 1024 while all slots' arrays stay within
 // kPsSlotSetBytes) and by a memory budget (40% of free device memory, at
-// most 32 GiB).  Betweenness returns CENT_E_UNSUPPORTED, with nothing
+// most 64 GiB).  Betweenness returns CENT_E_UNSUPPORTED, with nothing
 // accumulated into the BC output, if some source's BFS is deeper than the
 // level table: the caller then runs the batch form.  Closeness keeps no
 // level table and has no depth limit.
@@ -516,6 +516,41 @@ constexpr int kPsMaxCachedSlots = 1024;
 // 11k-113k vertices, all sources — and costs beyond (the 4.2M-vertex grid,
 // 117 MB per slot for BC, is fastest at 256).
 constexpr size_t kPsSlotSetBytes = (size_t)2 << 30;
+// Betweenness records go wide (PsRecW, the BC partial in the record) when
+// the slots' narrow records and partials together pass this size, far
+// beyond the 256 MB Infinity Cache; below it they nearly fit there and the
+// narrow form's smaller footprint wins.
+constexpr size_t kPsWideBytes = (size_t)512 << 20;
+
+// Slot plan of the source-parallel step: slot count, record form and the
+// bytes a slot holds.  budget: the most the slots may hold together
+// (SIZE_MAX: no limit).
+struct PsPlan {
+    size_t G;
+    bool wide;
+    size_t per_slot;
+};
+
+static PsPlan ps_plan(size_t n, bool bc, size_t K, int max_slots, int grid_waves, size_t budget) {
+    const size_t lcap = std::min<size_t>(n + 2, 65536);
+    const size_t nw = ((n + 31) / 32 + 3) / 4 * 4;
+    auto slot_bytes = [&](bool wide) {
+        return bc ? n * ((wide ? sizeof(PsRecW) : sizeof(PsRec) + 8) + 4) + lcap * 8 : n * 4 + nw * 4;
+    };
+    size_t per = slot_bytes(false);
+    size_t cap = kPsDefaultSlots;
+    if (max_slots > 0) cap = (size_t)max_slots;
+    else if (grid_waves > 0) cap = (size_t)std::max(1, grid_waves / kPsWaves);
+    else
+        while (cap < (size_t)kPsMaxCachedSlots && cap * 2 * per <= kPsSlotSetBytes) cap *= 2;
+    size_t G = std::max<size_t>(1, std::min({K, cap, budget / per}));
+    const bool wide = bc && G * n * (sizeof(PsRec) + 8) > kPsWideBytes;
+    if (wide) {
+        per = slot_bytes(true);
+        G = std::max<size_t>(1, std::min(G, budget / per));
+    }
+    return PsPlan{G, wide, per};
+}
 
 static int run_persource(cent_ctx* ctx, const cent_graph* g, const BatchSpec& sp, int32_t k, KernelTimer& tm,
                          int64_t* levels) {
@@ -526,24 +561,19 @@ static int run_persource(cent_ctx* ctx, const cent_graph* g, const BatchSpec& sp
     if (sp.bwgt || k < 1) return CENT_E_UNSUPPORTED;
     const int lcap = (int)std::min<size_t>(n + 2, 65536);
     const size_t nw = ((n + 31) / 32 + 3) / 4 * 4;  // closeness map words, padded for 16-byte clears
-    const size_t per_slot = bc ? n * (sizeof(PsRec) + 4 + 8) + (size_t)lcap * 8 : n * 4 + nw * 4;
-    size_t cap = kPsDefaultSlots;
-    if (sp.opts && sp.opts->max_slots > 0) cap = (size_t)sp.opts->max_slots;
-    else if (sp.opts && sp.opts->grid_waves > 0) cap = std::max(1, sp.opts->grid_waves / kPsWaves);
-    else
-        while (cap < (size_t)kPsMaxCachedSlots && cap * 2 * per_slot <= kPsSlotSetBytes) cap *= 2;
-    size_t G = std::min((size_t)k, cap);
     size_t freeb = 0, totalb = 0;
     HIPTRY(hipMemGetInfo(&freeb, &totalb));
-    const size_t have = sizeof(PsRec) * ctx->ps_rec.cap + 4 * (ctx->ps_Q.cap + ctx->ps_loff.cap + ctx->ps_bits.cap) +
-                        8 * ctx->ps_bcp.cap;
-    const size_t budget = std::min<size_t>((size_t)32 << 30, (freeb + have) / 5 * 2);
-    G = std::max<size_t>(1, std::min(G, budget / per_slot));
+    const size_t have = ctx->ps_rec.cap + 8 * ctx->ps_bcp.cap + 4 * (ctx->ps_Q.cap + ctx->ps_loff.cap + ctx->ps_bits.cap);
+    const size_t budget = std::min<size_t>((size_t)64 << 30, (freeb + have) / 5 * 2);
+    const PsPlan plan = ps_plan(n, bc, (size_t)k, sp.opts ? sp.opts->max_slots : 0,
+                                sp.opts ? sp.opts->grid_waves : 0, budget);
+    const size_t G = plan.G;
+    const bool wide = plan.wide;
     TRY(ensure_exact(ctx, ctx->ps_Q, n * G));
     if (bc) {
-        TRY(ensure_exact(ctx, ctx->ps_rec, n * G));
+        TRY(ensure_exact(ctx, ctx->ps_rec, n * G * (wide ? sizeof(PsRecW) : sizeof(PsRec))));
+        if (!wide) TRY(ensure_exact(ctx, ctx->ps_bcp, n * G));
         TRY(ensure_exact(ctx, ctx->ps_loff, (size_t)lcap * 2 * G));  // level offsets + count shifts
-        TRY(ensure_exact(ctx, ctx->ps_bcp, n * G));
     } else {
         TRY(ensure_exact(ctx, ctx->ps_bits, nw * G));
     }
@@ -552,7 +582,8 @@ static int run_persource(cent_ctx* ctx, const cent_graph* g, const BatchSpec& sp
     if (bc) {
         const size_t total = n * G;
         const int ig = (int)std::min<size_t>((total + 255) / 256, 16384);
-        k_ps_init<<<ig, 256, 0, s>>>(ctx->ps_rec.p, ctx->ps_bcp.p, total);
+        if (wide) k_ps_init<true><<<ig, 256, 0, s>>>(ctx->ps_rec.p, nullptr, total);
+        else k_ps_init<false><<<ig, 256, 0, s>>>(ctx->ps_rec.p, ctx->ps_bcp.p, total);
         HIPTRY(hipGetLastError());
     }
     HIPTRY(hipMemsetAsync(ctx->ps_depth.p, 0, sizeof(unsigned long long), s));
@@ -562,9 +593,9 @@ static int run_persource(cent_ctx* ctx, const cent_graph* g, const BatchSpec& sp
     a.Q = ctx->ps_Q.p;
     if (bc) {
         a.rec = ctx->ps_rec.p;
+        a.bcp = wide ? nullptr : ctx->ps_bcp.p;
         a.loff = ctx->ps_loff.p;
         a.lcap = lcap;
-        a.bcp = ctx->ps_bcp.p;
     } else {
         a.bits = ctx->ps_bits.p;
     }
@@ -573,8 +604,10 @@ static int run_persource(cent_ctx* ctx, const cent_graph* g, const BatchSpec& sp
     hipEvent_t e0;
     TRY(timer_begin(ctx, tm, &e0));
     if (!bc) k_ps_cc<<<(int)G, kPsBlock, 0, s>>>(a);
-    else if (compat) k_ps_bc<true><<<(int)G, kPsBlock, 0, s>>>(a);
-    else k_ps_bc<false><<<(int)G, kPsBlock, 0, s>>>(a);
+    else if (compat && wide) k_ps_bc<true, true><<<(int)G, kPsBlock, 0, s>>>(a);
+    else if (compat) k_ps_bc<true, false><<<(int)G, kPsBlock, 0, s>>>(a);
+    else if (wide) k_ps_bc<false, true><<<(int)G, kPsBlock, 0, s>>>(a);
+    else k_ps_bc<false, false><<<(int)G, kPsBlock, 0, s>>>(a);
     HIPTRY(hipGetLastError());
     TRY(timer_end(ctx, tm, e0, true));
     ctx->stats.launches_fwd++;
@@ -585,7 +618,8 @@ static int run_persource(cent_ctx* ctx, const cent_graph* g, const BatchSpec& sp
     if (ctx->h_flags[0] & 2) return CENT_E_UNSUPPORTED;
     if (ctx->h_flags[0] & 1) return CENT_E_OVERFLOW;
     if (bc) {
-        k_ps_reduce<<<(int)((n + 255) / 256), 256, 0, s>>>(ctx->ps_bcp.p, (int)G, d.n, sp.bc);
+        if (wide) k_ps_reduce<true><<<(int)((n + 255) / 256), 256, 0, s>>>(ctx->ps_rec.p, nullptr, (int)G, d.n, sp.bc);
+        else k_ps_reduce<false><<<(int)((n + 255) / 256), 256, 0, s>>>(ctx->ps_rec.p, ctx->ps_bcp.p, (int)G, d.n, sp.bc);
         HIPTRY(hipGetLastError());
     }
     *levels += (int64_t)depth;
@@ -1337,15 +1371,9 @@ static size_t ws_plain(int32_t n, size_t nseg, size_t nchunk, int32_t depth, int
         WsEst w = base;
         if (m == CENT_FRONTIER_SOURCE) {
             const size_t lcap = std::min<size_t>(N + 2, 65536);
-            const size_t nw = ((N + 31) / 32 + 3) / 4 * 4;
-            const size_t per_slot = bc ? N * (sizeof(PsRec) + 4 + 8) + lcap * 8 : N * 4 + nw * 4;
-            size_t cap = kPsDefaultSlots;
-            if (eff.max_slots > 0) cap = (size_t)eff.max_slots;
-            else if (eff.grid_waves > 0) cap = (size_t)std::max(1, eff.grid_waves / kPsWaves);
-            else
-                while (cap < (size_t)kPsMaxCachedSlots && cap * 2 * per_slot <= kPsSlotSetBytes) cap *= 2;
-            const size_t G2 = std::max<size_t>(1, std::min(K, cap));  // the free-memory cap only lowers it
-            w.add(per_slot * G2 + 8 * ensure_cap(1) + 4 * ensure_cap(2));
+            // the free-memory budget only lowers the slot count
+            const PsPlan plan = ps_plan(N, bc, K, eff.max_slots, eff.grid_waves, SIZE_MAX);
+            w.add(plan.per_slot * plan.G + 8 * ensure_cap(1) + 4 * ensure_cap(2));
             // a betweenness BFS deeper than the level table reruns as batches
             if (bc && (depth < 0 ? false : 2 * (size_t)depth + 2 >= lcap)) dense(w, true);
         } else if (m == CENT_FRONTIER_SPARSE) {

@@ -9,8 +9,9 @@
 // s + G, s + 2G, ... in turn.
 //
 // Betweenness (k_ps_bc), per slot:
-//   R[v]      16-byte record, so a neighbour's level and path count share one
-//             cache line:
+//   R[v]      record (PsRec, 16 bytes, or PsRecW, 32 bytes), so a
+//             neighbour's level and path count share one cache line (and,
+//             wide, a vertex's coef and BC partial):
 //     .dist   int32   epoch-tagged BFS level: epoch of the slot's current
 //                     source << 16 | level; any other value = unreached,
 //                     so nothing is restored between sources
@@ -20,8 +21,9 @@
 //                     backward reads only the successors' records
 //     .sig    double  path count; the backward overwrites it in place with
 //                     coef = (1 + delta) / sigma once v's delta is known
+//     .bc     double  wide form: this slot's BC partial over its sources
+//   bcp[v]    double  narrow form: the same partials, a separate array
 //   Q[]       int32   BFS order; level l is Q[loff[l] .. loff[l+1])
-//   bcp[v]    double  this slot's BC partial over its sources
 // Forward, level d: each vertex u of the level scans its row once.  A
 // neighbour at level d - 1 adds its path count to u's (a pull, in row
 // order: sigma[u] = sum of sigma[pred], Seq_Brandes/Brandes.c:286-291), and
@@ -33,7 +35,10 @@
 // source.
 // Backward, level d from the deepest up: v sums coef[w] over its neighbours
 // w at level d + 1 in row order (delta = sigma * sum, Brandes.c:317-327 in
-// textbook form), then coef[v] replaces sig[v] and bcp[v] += delta.  The
+// textbook form), then coef[v] replaces sig[v] and its BC partial gets
+// delta.  The wide form pays where the slots' arrays are far beyond the
+// Infinity Cache (a 2048^2 grid at 256 slots: BC kernel 995 -> 884 ms) and
+// costs where they nearly fit (R-MAT 2^14 at 1024 slots: 46 -> 51 ms).  The
 // host-side reduce adds the slots in slot order, so BC is deterministic for
 // a given G.
 //
@@ -69,6 +74,7 @@
 // sum coef~_{d+1}, with coef~ = (1 + delta) / sigma~ of the stored counts).
 // Sources whose counts stay below 2^512 run as before, bit for bit.
 #pragma once
+#include <type_traits>
 #include "records.h"
 #include "wave.h"
 
@@ -81,7 +87,7 @@ constexpr int kPsBlock = CENT_PS_BLOCK;  // 8 waves per workgroup
 constexpr int kPsWaves = kPsBlock / kWave;
 constexpr int kPsBigDeg = 64;   // longer rows are scanned by a wavefront
 constexpr int kPsBig = 1024;    // LDS capacity of the long-row list per level
-constexpr int kPsV = 4;         // rows a thread works on at once
+constexpr int kPsV = 3;         // rows a thread works on at once
 constexpr int kPsE = 4;         // edges per row issued together
 constexpr int kPsHubE = 8;      // edges per lane issued together on a long row
 
@@ -96,12 +102,12 @@ struct PsArgs {
     const int4* __restrict__ ell;
     const int* __restrict__ src;          // [k] device ids
     int k;
-    PsRec* __restrict__ rec;              // BC: [G][n]
+    void* __restrict__ rec;               // BC: [G][n] PsRec or PsRecW
+    double* __restrict__ bcp;             // BC, narrow records: [G][n] BC partials
     uint32_t* __restrict__ bits;          // closeness: [G][nw]
     int* __restrict__ Q;                  // [G][n]
     int* __restrict__ loff;               // BC: [G][2 lcap]: level offsets, then per-level count shifts
     int lcap;
-    double* __restrict__ bcp;             // BC: [G][n]
     int64_t* __restrict__ far;            // [k]
     int64_t* __restrict__ dist_sum;       // [n] or null
     const int64_t* __restrict__ weight;   // [n] or null
@@ -159,7 +165,8 @@ __device__ __forceinline__ void ps_claimed(const PsArgs& a, int* Q, int w, int d
 // Edge u -> w while u (level d) is expanded: w at level d - 1 (tag tp) adds
 // its path count to u's sum; an unreached w is claimed for level d + 1
 // (tag tn).
-__device__ __forceinline__ void ps_visit(const PsArgs& a, PsRec* R, int* Q, int w, int ep, int d, int tp, int tn,
+template <class Rec>
+__device__ __forceinline__ void ps_visit(const PsArgs& a, Rec* R, int* Q, int w, int ep, int d, int tp, int tn,
                                          int le, int* s_cnt, PsLocal& L, double& acc, uint32_t& np) {
     const int t = ld_dist(&R[w].dist);
     if (t == tp) {
@@ -211,7 +218,8 @@ __device__ __forceinline__ int ps_nbr(const PsArgs& a, const int* wi, int rb, in
 // cover the whole row.
 __device__ __forceinline__ bool ps_short(const int* wi, int dg) { return wi[0] != kEllLong || dg <= kPsE; }
 
-__device__ __forceinline__ void ps_expand_rec(const PsArgs& a, PsRec* R, int* Q, const int (*wi)[kPsE],
+template <class Rec>
+__device__ __forceinline__ void ps_expand_rec(const PsArgs& a, Rec* R, int* Q, const int (*wi)[kPsE],
                                               const int* rb, const int* dg, int ep, int d, int tp, int le,
                                               int* s_cnt, PsLocal& L, double* acc, uint32_t* np, uint32_t* sb) {
     const int tn = ps_tag(ep, d + 1);
@@ -265,7 +273,8 @@ __device__ __forceinline__ void ps_expand_rec(const PsArgs& a, PsRec* R, int* Q,
 // j + 64, ... taken kPsHubE at a time, staged like ps_expand_rec.  Returns
 // the lane's partial sum of predecessor path counts (its edges in order);
 // np counts them.
-__device__ __forceinline__ double ps_expand_long(const PsArgs& a, PsRec* R, int* Q, int rb, int re, int lane, int ep,
+template <class Rec>
+__device__ __forceinline__ double ps_expand_long(const PsArgs& a, Rec* R, int* Q, int rb, int re, int lane, int ep,
                                                  int d, int tp, int le, int* s_cnt, PsLocal& L, uint32_t& np) {
     const int tn = ps_tag(ep, d + 1);
     double acc = 0.0;
@@ -303,7 +312,8 @@ __device__ __forceinline__ double ps_expand_long(const PsArgs& a, PsRec* R, int*
 // lane's edges j, j + 64, ... to neighbours with tag tl, kPsHubE edges'
 // loads in flight at a time; fixed order, so the wavefront sum that follows
 // is deterministic.
-__device__ __forceinline__ double ps_pull_long(const PsArgs& a, const PsRec* R, int rb, int re, int lane, int tl) {
+template <class Rec>
+__device__ __forceinline__ double ps_pull_long(const PsArgs& a, const Rec* R, int rb, int re, int lane, int tl) {
     double acc = 0.0;
     for (int j0 = rb + lane; j0 < re; j0 += kWave * kPsHubE) {
         int w[kPsHubE], lev[kPsHubE];
@@ -327,7 +337,8 @@ __device__ __forceinline__ double ps_pull_long(const PsArgs& a, const PsRec* R, 
 // neighbours w with tag tl of coef[w] (held in sig), in row order.  A short
 // row with its successor bits (mk[m] >= 0) reads only its successors'
 // counts; any other row (mk[m] < 0) tests every neighbour's level.
-__device__ __forceinline__ void ps_pull_rec(const PsArgs& a, const PsRec* R, const int (*wi)[kPsE], const int* rb,
+template <class Rec>
+__device__ __forceinline__ void ps_pull_rec(const PsArgs& a, const Rec* R, const int (*wi)[kPsE], const int* rb,
                                              const int* dg, const int* mk, int tl, double* acc) {
     int w[kPsV][kPsE], lev[kPsV][kPsE];
     double x[kPsV][kPsE];
@@ -359,8 +370,9 @@ __device__ __forceinline__ void ps_pull_rec(const PsArgs& a, const PsRec* R, con
     }
 }
 
-template <bool COMPAT>
+template <bool COMPAT, bool WIDE>
 __global__ __launch_bounds__(kPsBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_ps_bc(PsArgs a) {
+    using Rec = typename std::conditional<WIDE, PsRecW, PsRec>::type;
     __shared__ int s_cnt[3];     // appends of level d go to s_cnt[d % 3]
     __shared__ int s_nbig[3];    // long rows of level d: s_big[..s_nbig[d % 3])
     __shared__ int s_emax[3];    // scaled form: largest count exponent written at level d
@@ -371,11 +383,15 @@ __global__ __launch_bounds__(kPsBlock) __attribute__((amdgpu_waves_per_eu(4))) v
     const int lane = tid & (kWave - 1);
     const int wid = tid >> 6;
     const size_t so = (size_t)blockIdx.x * (size_t)a.n;
-    PsRec* R = a.rec + so;
+    Rec* R = static_cast<Rec*>(a.rec) + so;
+    double* bcp = WIDE ? nullptr : a.bcp + so;
+    auto bc_add = [&](int v, double x) {
+        if constexpr (WIDE) R[v].bc += x;
+        else bcp[v] += x;
+    };
     int* Q = a.Q + so;
     int* loff = a.loff + (size_t)blockIdx.x * (size_t)a.lcap * 2;
     int* lsh = loff + a.lcap;    // scaled form: shift k_d of level d's counts
-    double* bcp = a.bcp + so;
     int scaled_si = -1;          // the source now being redone in scaled form
     int ep = 0;                  // epoch of the slot's current source
     for (int si = blockIdx.x; si < a.k; si += gridDim.x) {
@@ -560,7 +576,7 @@ __global__ __launch_bounds__(kPsBlock) __attribute__((amdgpu_waves_per_eu(4))) v
                         const double dl = (sv[m] * acc[m]) * sc;
                         if (__builtin_isinf(sv[m])) atomicOr(a.overflow, 1);
                         R[v[m]].sig = (1.0 + dl) / sv[m];
-                        bcp[v[m]] += COMPAT ? dl * (double)(pw[m] & kPsCountMask) : dl;
+                        bc_add(v[m], COMPAT ? dl * (double)(pw[m] & kPsCountMask) : dl);
                     }
                 }
                 __syncthreads();
@@ -571,7 +587,7 @@ __global__ __launch_bounds__(kPsBlock) __attribute__((amdgpu_waves_per_eu(4))) v
                         const double dl = (sv * acc) * sc;
                         if (__builtin_isinf(sv)) atomicOr(a.overflow, 1);
                         R[v].sig = (1.0 + dl) / sv;
-                        bcp[v] += COMPAT ? dl * (double)(ld_npred(&R[v].npred) & kPsCountMask) : dl;
+                        bc_add(v, COMPAT ? dl * (double)(ld_npred(&R[v].npred) & kPsCountMask) : dl);
                     }
                 };
                 const int nbig = s_nbig[c];
@@ -759,22 +775,32 @@ __global__ __launch_bounds__(kPsBlock) void k_ps_cc(PsArgs a) {
     }
 }
 
-// bc[v] = sum over slots g = 0 .. G-1 of bcp[g][v], in slot order.
-__global__ __launch_bounds__(256) void k_ps_reduce(const double* __restrict__ bcp, int G, int n,
-                                                   double* __restrict__ bc) {
+// bc[v] = sum over slots g = 0 .. G-1 of the slot's BC partial of v, in
+// slot order.
+template <bool WIDE>
+__global__ __launch_bounds__(256) void k_ps_reduce(const void* __restrict__ rec, const double* __restrict__ bcp,
+                                                   int G, int n, double* __restrict__ bc) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
     double t = 0.0;
-    for (int g = 0; g < G; ++g) t += bcp[(size_t)g * (size_t)n + v];
+    for (int g = 0; g < G; ++g) {
+        const size_t i = (size_t)g * (size_t)n + v;
+        t += WIDE ? static_cast<const PsRecW*>(rec)[i].bc : bcp[i];
+    }
     bc[v] += t;
 }
 
-// Betweenness slot arrays to their idle state: every record {-1, 0, 0}, bcp 0.
-__global__ __launch_bounds__(256) void k_ps_init(PsRec* __restrict__ rec, double* __restrict__ bcp, size_t total) {
+// Betweenness slot records to their idle state: never reached, BC partial 0.
+template <bool WIDE>
+__global__ __launch_bounds__(256) void k_ps_init(void* __restrict__ rec, double* __restrict__ bcp, size_t total) {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
-        rec[i] = PsRec{-1, 0u, 0.0};
-        bcp[i] = 0.0;
+        if constexpr (WIDE) {
+            static_cast<PsRecW*>(rec)[i] = PsRecW{-1, 0u, 0.0, 0.0, 0.0};
+        } else {
+            static_cast<PsRec*>(rec)[i] = PsRec{-1, 0u, 0.0};
+            bcp[i] = 0.0;
+        }
     }
 }
 
