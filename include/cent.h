@@ -521,7 +521,11 @@ int  cent_reduce_gpu_workspace_estimate(int32_t n, int64_t nnz, int64_t* bytes);
 /*
  * Query server: BC / closeness requests on registered graphs, continuously
  * batched on one GPU.  `slots` requests run at once, each on its own context
- * and HIP stream; the FIFO queue admits the next request as soon as a slot is
+ * and HIP stream (the HIP runtime spreads a process's streams over
+ * GPU_MAX_HW_QUEUES hardware queues, 4 unless the environment sets more, and
+ * streams sharing a queue run one after another: set it to at least
+ * `slots` before the process's first HIP call); the FIFO queue admits the
+ * next request as soon as a slot is
  * free and its estimated device footprint fits the HBM budget next to the
  * admitted requests and cached graphs (least-recently-used unreferenced
  * graphs are evicted to make room).  A request's result does not depend on
