@@ -34,11 +34,12 @@
 // level step of the reference's queue BFS (Brandes.c:274-298) for one
 // source.
 // Backward, level d from the deepest up: v sums coef[w] over its neighbours
-// w at level d + 1 in row order (delta = sigma * sum, Brandes.c:317-327 in
-// textbook form), then coef[v] replaces sig[v] and its BC partial gets
-// delta.  The wide form pays where the slots' arrays are far beyond the
-// Infinity Cache (a 2048^2 grid at 256 slots: BC kernel 995 -> 884 ms) and
-// costs where they nearly fit (R-MAT 2^14 at 1024 slots: 46 -> 51 ms).  The
+// w at level d + 1 in row order (only the records its successor bits name,
+// for a short row), delta = sigma * sum (Brandes.c:317-327 in textbook
+// form), then coef[v] replaces sig[v] and its BC partial gets delta.  The
+// wide form pays where the slots' arrays are far beyond the Infinity Cache
+// (a 2048^2 grid at 256 slots: BC kernel 995 -> 920 ms) and costs where
+// they nearly fit (R-MAT 2^14 at 1024 slots: 46 -> 51 ms).  The
 // host-side reduce adds the slots in slot order, so BC is deterministic for
 // a given G.
 //
