@@ -338,7 +338,8 @@ class Reduced:
 # ---------------------------------------------------------------- engine
 class _Opts(C.Structure):
     _fields_ = [("bc_mode", C.c_int32), ("flags", C.c_int32), ("block_size", C.c_int32),
-                ("grid_waves", C.c_int32), ("frontier", C.c_int32), ("max_slots", C.c_int32)]
+                ("grid_waves", C.c_int32), ("frontier", C.c_int32), ("max_slots", C.c_int32),
+                ("push_pct", C.c_int32)]
 
 
 CENT_FRONTIER_AUTO, CENT_FRONTIER_DENSE, CENT_FRONTIER_SPARSE = 0, 1, 2
@@ -456,7 +457,7 @@ def write_cc_answers(path: str, csr: "CSR", cc) -> None:
 class TuneParams(C.Structure):
     """Launch settings of the plain form (``cent_tune_params``)."""
     _fields_ = [("hub_deg", C.c_int32), ("chunk_edges", C.c_int32), ("grid_waves", C.c_int32),
-                ("frontier", C.c_int32), ("ms", C.c_double)]
+                ("frontier", C.c_int32), ("push_pct", C.c_int32), ("ms", C.c_double)]
 
     def as_dict(self) -> dict:
         d = {f: getattr(self, f) for f, _ in self._fields_}
@@ -502,8 +503,8 @@ def tune_signature(csr: "CSR", ecc: int) -> tuple:
     return sig.as_tuple()
 
 
-def tune_params(hub_deg=256, chunk_edges=1024, grid_waves=0, frontier="auto", ms=0.0) -> TuneParams:
-    return TuneParams(hub_deg, chunk_edges, grid_waves, FRONTIER[frontier], ms)
+def tune_params(hub_deg=256, chunk_edges=1024, grid_waves=0, frontier="auto", ms=0.0, push_pct=0) -> TuneParams:
+    return TuneParams(hub_deg, chunk_edges, grid_waves, FRONTIER[frontier], push_pct, ms)
 
 
 def tune_table_put(sig: tuple, p: TuneParams, what: int = CENT_TUNE_BC) -> None:
@@ -537,7 +538,7 @@ def tune_table_save(path: str) -> None:
 
 
 def _opts(mode=CENT_BC_TEXTBOOK, timing=False, grid_waves=0, frontier="auto", max_slots=0,
-          keep_source_order=False, one_stream=False) -> _Opts:
+          keep_source_order=False, one_stream=False, push_pct=0) -> _Opts:
     o = _Opts()
     o.bc_mode = mode
     o.flags = ((CENT_OPT_TIMING if timing else 0) | (CENT_OPT_SOURCE_ORDER if keep_source_order else 0) |
@@ -545,6 +546,7 @@ def _opts(mode=CENT_BC_TEXTBOOK, timing=False, grid_waves=0, frontier="auto", ma
     o.grid_waves = grid_waves
     o.frontier = FRONTIER[frontier]
     o.max_slots = max_slots
+    o.push_pct = push_pct
     return o
 
 
@@ -750,13 +752,14 @@ class Context:
     def bc(self, g: DeviceGraph, sources=None, mode: int = CENT_BC_TEXTBOOK, bc=None,
            farness: bool = True, dist_sum: bool = False, timing: bool = False,
            grid_waves: int = 0, frontier: str = "auto", max_slots: int = 0,
-           keep_source_order: bool = False, one_stream: bool = False):
-        """Brandes BC from ``sources`` (None: all).  Returns (bc, farness, dist_sum)."""
+           keep_source_order: bool = False, one_stream: bool = False, push_pct: int = 0):
+        """Brandes BC from ``sources`` (None: all).  Returns (bc, farness, dist_sum).
+        push_pct: level-2 push threshold of the dense step (0: tuned or default, -1: pull)."""
         s, k, kk = self._src(g, sources)
         out = np.zeros(g.n, np.float64) if bc is None else bc
         far = np.zeros(max(kk, 1), np.int64) if farness else None
         ds = np.zeros(max(g.n, 1), np.int64) if dist_sum else None
-        o = _opts(mode, timing, grid_waves, frontier, max_slots, keep_source_order, one_stream)
+        o = _opts(mode, timing, grid_waves, frontier, max_slots, keep_source_order, one_stream, push_pct)
         _check(_lib.cent_bc(self._h, g._h, _ptr(s, C.c_int32) if s is not None and k else None, k,
                             _ptr(out, C.c_double) if g.n else None,
                             _ptr(far, C.c_int64) if far is not None else None,
@@ -854,11 +857,12 @@ class Context:
                "cent_bc_reduced_sources", self.last_error())
         return bc[:red.n], far[:len(src)]
 
-    def bc_reduced(self, red: Reduced, sched: int = CENT_SCHED_PERSISTENT, timing: bool = False):
+    def bc_reduced(self, red: Reduced, sched: int = CENT_SCHED_PERSISTENT, timing: bool = False,
+                   push_pct: int = 0):
         """(bc, farness) of every vertex, all sources, through the reduced form."""
         bc = np.zeros(max(red.n, 1))
         far = np.zeros(max(red.n, 1), np.int64)
-        o = _opts(0, timing)
+        o = _opts(0, timing, push_pct=push_pct)
         _check(_lib.cent_bc_reduced(self._h, red._h, _ptr(bc, C.c_double), _ptr(far, C.c_int64),
                                     sched, C.byref(o)), "cent_bc_reduced", self.last_error())
         return bc[:red.n], far[:red.n]

@@ -86,6 +86,7 @@ struct cent_ctx {
     void* h_stage = nullptr; // pinned staging for a call's inputs and results
     size_t h_stage_bytes = 0;
     DevBuf<int> lexp;        // [levels][64] path-count exponents of a betweenness batch (scaling)
+    DevBuf<unsigned long long> push_e;  // edges of a betweenness batch's level-1 rows (level-2 push switch)
     DevBuf<int> lvf;         // per-level "non-empty" flags of an enqueued run of levels
     DevBuf<unsigned long long> lvstat;  // bounded batches: [0] summed depth, [1] bound failures
     int depth_hint = 0;      // depth of the last dense batch (length of the next run)
@@ -180,6 +181,7 @@ struct DevCSR {
     int* hub_buf = nullptr;
     int *seg_v = nullptr, *seg_beg = nullptr, *seg_end = nullptr, *hub_v = nullptr, *hub_seg_off = nullptr;
     std::vector<int> h_chunk_beg, h_hub_v, h_hub_seg_off;
+    std::vector<int64_t> h_chunk_rp;  // row offset at each chunk start: a view's edge count
     bool multi = false;  // rows may repeat a neighbour or list their own vertex
     int4* ell = nullptr; // [n] inline rows of the source-parallel step (see persource_kernels.h), or null
 };
@@ -257,6 +259,7 @@ struct cent_graph {
     // [1] closeness
     int32_t grid_waves[2] = {0, 0};
     int32_t frontier[2] = {CENT_FRONTIER_AUTO, CENT_FRONTIER_AUTO};
+    int32_t push_pct[2] = {0, 0};
 };
 
 // Level step for a plain call that neither the call nor the graph's tuned

@@ -66,6 +66,8 @@ int devcsr_schedule(cent_ctx* ctx, const int32_t* rp, const int32_t* bounds, int
     }
     if (n > 0) cb.push_back(n);
     d.nchunk = (int)cb.size() - 1;
+    d.h_chunk_rp.resize(cb.size());
+    for (size_t i = 0; i < cb.size(); ++i) d.h_chunk_rp[i] = rp[cb[i]];
     TRY(upload(ctx, &d.chunk_beg, cb.data(), cb.size()));
     // hub segments
     std::vector<int> sv, sb, se;
@@ -151,7 +153,7 @@ extern "C" void cent_ctx_destroy(cent_ctx* ctx) {
     release(ctx->it_lvl); release(ctx->red_deep);
     release(ctx->ps_rec); release(ctx->ps_bits); release(ctx->ps_Q); release(ctx->ps_loff); release(ctx->ps_bcp);
     release(ctx->ps_depth); release(ctx->lvf); release(ctx->lvstat); release(ctx->red_arena);
-    release(ctx->lexp); release(ctx->red_src); release(ctx->red_order); release(ctx->red_swgt);
+    release(ctx->lexp); release(ctx->push_e); release(ctx->red_src); release(ctx->red_order); release(ctx->red_swgt);
     release(ctx->red_poff); release(ctx->red_items);
     if (ctx->h_flags) (void)hipHostFree(ctx->h_flags);
     if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
@@ -184,7 +186,7 @@ extern "C" int64_t cent_ctx_workspace_bytes(const cent_ctx* ctx) {
                      4 * (ctx->npred.cap + ctx->hp_cnt.cap + ctx->src.cap + ctx->flags.cap + ctx->ord.cap + ctx->Q.cap +
                           ctx->qoff.cap + ctx->cand.cap + ctx->stamp.cap + ctx->ctl.cap + ctx->red_head.cap +
                           ctx->red_deep.cap + ctx->ps_Q.cap + ctx->ps_loff.cap + ctx->ps_bits.cap +
-                          ctx->lexp.cap + ctx->red_src.cap + ctx->red_order.cap));
+                          ctx->lexp.cap + 2 * ctx->push_e.cap + ctx->red_src.cap + ctx->red_order.cap));
 }
 
 extern "C" int cent_ctx_trim(cent_ctx* ctx) {
@@ -203,7 +205,7 @@ extern "C" int cent_ctx_trim(cent_ctx* ctx) {
     release(ctx->it_lvl); release(ctx->red_deep);
     release(ctx->ps_rec); release(ctx->ps_bits); release(ctx->ps_Q); release(ctx->ps_loff); release(ctx->ps_bcp);
     release(ctx->ps_depth); release(ctx->lvf); release(ctx->lvstat); release(ctx->red_arena);
-    release(ctx->lexp); release(ctx->red_src); release(ctx->red_order); release(ctx->red_swgt);
+    release(ctx->lexp); release(ctx->push_e); release(ctx->red_src); release(ctx->red_order); release(ctx->red_swgt);
     release(ctx->red_poff); release(ctx->red_items);
     if (ctx->h_stage) HIPTRY(hipHostFree(ctx->h_stage));
     ctx->h_stage = nullptr;
@@ -555,6 +557,7 @@ extern "C" int cent_graph_get_layout(const cent_graph* g, int32_t what, cent_tun
     out->chunk_edges = g->d.chunk_edges;
     out->grid_waves = g->grid_waves[i];
     out->frontier = g->frontier[i];
+    out->push_pct = g->push_pct[i];
     out->ms = 0.0;
     return CENT_OK;
 }
@@ -572,6 +575,7 @@ extern "C" int cent_graph_set_layout(cent_graph* g, int32_t what, const cent_tun
         if (what & (i == 0 ? CENT_TUNE_BC : CENT_TUNE_CC)) {
             g->grid_waves[i] = p->grid_waves;
             g->frontier[i] = p->frontier;
+            g->push_pct[i] = p->push_pct;
         }
     return CENT_OK;
 }

@@ -58,6 +58,7 @@ static int ensure_workspace(cent_ctx* ctx, const DevCSR& d, const BatchSpec& sp,
         TRY(ensure_lexp(ctx, ctx->F_levels, 0));
         TRY(ensure(ctx, ctx->sigma, N * kWave));  // coef is written over it in the backward
         if (sp.compat) TRY(ensure(ctx, ctx->npred, N * kWave));
+        TRY(ensure(ctx, ctx->push_e, 1));
     }
     TRY(ensure(ctx, ctx->part_far, (size_t)G * kWave * nb));
     TRY(ensure(ctx, ctx->part_cnt, (size_t)G * 2 * nb));
@@ -108,13 +109,23 @@ int level_flags_reset(cent_ctx* ctx) {
     return CENT_OK;
 }
 
-// Zeroes up to five ranges of 64-bit words in one launch (the clears that
+// Zeroes up to eight ranges of 64-bit words in one launch (the clears that
 // start a launch group; one launch instead of a fill per buffer).
 struct ClearRanges {
-    uint64_t* p[6];
-    size_t n[6];
+    uint64_t* p[8];
+    size_t n[8];
     int count;
 };
+
+// Level 2 of a betweenness batch is pushed from its level-1 frontier when
+// that frontier's rows hold under this share (percent) of the view's
+// edges, unless the call or the graph's tuned settings say otherwise.
+// Measured one-stream on 2^20-vertex graphs (push always / pulled): an
+// Erdos-Renyi graph of average degree 32 152 / 160 ms (its level-1 rows
+// hold about 0.1% of the edges), R-MAT with a = 0.45 102 / 105 ms, the
+// skewed R-MAT of a = 0.57 249 / 235 ms (level-1 rows about 15%: there
+// the push's per-edge atomics cost more than the pull's scan).
+constexpr int kPushPctDefault = 2;
 
 __global__ void k_clear(ClearRanges c) {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -179,6 +190,15 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
     const size_t fw = sp.with_bc ? 2 : (size_t)W;
     const size_t PL = N * fw;
     auto plane = [&](size_t p) { return ctx->F.p + p * PL; };
+    // Level 2 of a betweenness batch on a simple graph: pushed from the
+    // level-1 frontier while its rows hold fewer than push_lim edges (0:
+    // pulled); the batch's kernels decide on the device (push2_on).
+    unsigned long long push_lim = 0;
+    if (sp.with_bc && !d.multi && v.nchunk > 0) {
+        const int pct = sp.opts && sp.opts->push_pct ? sp.opts->push_pct : kPushPctDefault;
+        const int64_t ev = d.h_chunk_rp[(size_t)(v.c0 + v.nchunk)] - d.h_chunk_rp[(size_t)v.c0];
+        if (pct > 0 && ev > 0) push_lim = (unsigned long long)std::max<int64_t>(1, ev / 100 * pct + ev % 100 * pct / 100);
+    }
     {  // vis, the level-1 plane (and the level-0 one for betweenness), the partials
         ClearRanges cr{};
         auto add = [&](void* p, size_t n) { cr.p[cr.count] = static_cast<uint64_t*>(p); cr.n[cr.count++] = n; };
@@ -189,6 +209,10 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
         add(ctx->part_cnt.p, (size_t)G * 2 * nb);
         if (sp.with_bc)  // the exponent rows of every level this batch can reach: 64 ints, 32 words each
             add(ctx->lexp.p, std::min(ctx->lexp.cap, (ctx->F_levels + 2) * (size_t)kWave) / 2);
+        if (push_lim) {  // the level-1 rows' edge count, and the level-2 plane the push ORs into
+            add(ctx->push_e.p, 1);
+            if (ctx->F_levels >= 2) add(plane(2) + fw * v.lo, fw * R);
+        }
         size_t tot = 0;
         for (int i = 0; i < cr.count; ++i) tot = std::max(tot, cr.n[i]);
         const int cg = (int)std::min<size_t>((tot + 255) / 256, 4096);
@@ -230,7 +254,10 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
     int dl = 0;  // first level of the current run; on exit, the deepest non-empty level
     for (;;) {
         int run = bound > 0 ? bound : std::max(2, std::min(kHostFlags - 2, ctx->depth_hint + 2 - dl));
-        if (bc && (size_t)(dl + run + 1) > ctx->F_levels) TRY(grow_levels(ctx, PL, dl + 1));
+        // a grown store keeps the planes up to level dl + 2: levels 1 and 2
+        // are cleared when the batch starts (their pushes OR into them)
+        if (bc && (size_t)(dl + run + 1) > ctx->F_levels)
+            TRY(grow_levels(ctx, PL, (int)std::min<size_t>((size_t)dl + 3, ctx->F_levels + 1)));
         if (bc) run = std::min(run, (int)ctx->F_levels - 1 - dl);
         HIPTRY(hipMemsetAsync(ctx->lvf.p, 0, sizeof(int) * (size_t)(run + 1), s));
         HIPTRY(hipMemsetD32Async((hipDeviceptr_t)ctx->lvf.p, 1, 1, s));
@@ -280,8 +307,27 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
             fa.flag = ctx->lvf.p + i + 1; fa.overflow = ctx->flags.p + 1;
             fa.live = ctx->lvf.p + i;
             fa.lexp = ctx->lexp.p;
+            if (push_lim && lv <= 1) {  // level 1 counts its rows' edges, level 2 reads the count
+                fa.push_e = ctx->push_e.p;
+                fa.push_lim = lv == 1 ? push_lim : 0;
+            }
             hipEvent_t e0;
             TRY(timer_begin(ctx, tm, &e0));
+            if (lv == 1 && push_lim) {  // level 2 by push where the batch's level-1 rows are few (push2_on)
+                const int Gsh = std::min(Gseg, G);  // k_push2_sig_hub flushes into the G block slots
+                k_push2_or<<<G, kBlock, 0, s>>>(fa);
+                if (v.nhub) k_push2_or_hub<<<Gseg, kBlock, 0, s>>>(fa, ha);
+                if (compat) {
+                    k_push2_fin<true><<<G, kBlock, 0, s>>>(fa);
+                    k_push2_sig<true><<<G, kBlock, 0, s>>>(fa);
+                    if (v.nhub) k_push2_sig_hub<true><<<Gsh, kBlock, 0, s>>>(fa, ha);
+                } else {
+                    k_push2_fin<false><<<G, kBlock, 0, s>>>(fa);
+                    k_push2_sig<false><<<G, kBlock, 0, s>>>(fa);
+                    if (v.nhub) k_push2_sig_hub<false><<<Gsh, kBlock, 0, s>>>(fa, ha);
+                }
+                HIPTRY(hipGetLastError());
+            }
             if (lv == 0 && !d.multi) {  // level 1: push from the sources' rows (one path per neighbour)
                 const int Gp = std::min(G, kPushBlocks);
                 k_push1_or<<<Gp, kBlock, 0, s>>>(fa, d_src);
@@ -713,6 +759,7 @@ static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
     if (opts) eff = *opts;
     if (eff.grid_waves == 0) eff.grid_waves = g->grid_waves[with_bc ? 0 : 1];
     if (eff.frontier == CENT_FRONTIER_AUTO) eff.frontier = g->frontier[with_bc ? 0 : 1];
+    if (eff.push_pct == 0) eff.push_pct = g->push_pct[with_bc ? 0 : 1];
     opts = &eff;
     BatchSpec sp;
     sp.with_bc = with_bc;
@@ -1268,6 +1315,10 @@ extern "C" int cent_tune(cent_ctx* ctx, cent_graph* g, const int32_t* src, int32
             before.grid_waves == cur.grid_waves)
             break;  // a pass that moves nothing is a fixed point
     }
+    // the dense step's push/pull switch at level 2: always pulled, or pushed
+    // while the level-1 rows hold under 2 / 10 / 30 % of the edges
+    if (!st && what == CENT_TUNE_BC && cur.frontier == CENT_FRONTIER_DENSE)
+        sweep(&cent_tune_params::push_pct, {-1, 2, 10, 30});
     ctx->stats = saved;
     if (!st && best_ms >= 1e300) st = CENT_E_INVAL;  // budget too small to time anything
     if (st) {
@@ -1293,6 +1344,7 @@ void ws_dense(WsEst& w, size_t N, size_t nseg, bool bc, bool compat, int nb, int
         w.add(4 * (fl + 2) * kWave);                                 // lexp
         w.add(8 * ensure_cap(N * kWave));                            // sigma (coef written over it)
         if (compat) w.add(4 * ensure_cap(N * kWave));                // npred
+        w.add(8 * ensure_cap(1));                                    // push_e
     }
     w.add(8 * ensure_cap((size_t)G * kWave * nb) + 8 * ensure_cap((size_t)G * 2 * nb));  // part_far, part_cnt
     w.add(8 * ensure_cap(2) + 4 * ensure_cap(2));                    // cnt, flags

@@ -95,7 +95,16 @@ struct FwdArgs {
     // [levels][64] largest exponent of a stored path count per level and
     // lane, 0 below 2^512 (null: no scaling, closeness steps)
     int* __restrict__ lexp;
+    // Level 2 pushed from the level-1 frontier (k_push2_*) or pulled (k_fwd
+    // and its hub kernels): push_e counts the directed edges of the level-1
+    // vertices' rows (k_push1_fin adds them), and level 2 is pushed while
+    // that count is below push_lim.  push_lim 0: pulled.
+    unsigned long long* __restrict__ push_e;
+    unsigned long long push_lim;
 };
+
+// True where the batch's level 2 is pushed (see FwdArgs::push_lim).
+__device__ __forceinline__ bool push2_on(const FwdArgs& a) { return a.push_lim && *a.push_e < a.push_lim; }
 
 struct BwdArgs {
     int n;
@@ -357,7 +366,7 @@ __device__ __forceinline__ void bwd_vertex_done(const BwdArgs& a, int v, uint64_
 // step has its own interleaved kernels).
 template <bool COMPAT>
 __global__ __launch_bounds__(kBlock) void k_fwd(FwdArgs a) {
-    if (a.live && *a.live == 0) return;  // uniform over the grid
+    if ((a.live && *a.live == 0) || push2_on(a)) return;  // uniform over the grid
     const int lane = lane_id();
     const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const int nw = gridDim.x * kWavesPerBlock;
@@ -398,7 +407,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd(FwdArgs a) {
 }
 
 __global__ __launch_bounds__(kBlock) void k_fwd_hubseg(FwdArgs a, HubArgs h) {
-    if (a.live && *a.live == 0) return;  // uniform over the grid
+    if ((a.live && *a.live == 0) || push2_on(a)) return;  // uniform over the grid
     const int lane = lane_id();
     const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const int nw = gridDim.x * kWavesPerBlock;
@@ -417,7 +426,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_hubseg(FwdArgs a, HubArgs h) {
 
 template <bool COMPAT>
 __global__ __launch_bounds__(kBlock) void k_fwd_hubfin(FwdArgs a, HubArgs h) {
-    if (a.live && *a.live == 0) return;  // uniform over the grid
+    if ((a.live && *a.live == 0) || push2_on(a)) return;  // uniform over the grid
     const int lane = lane_id();
     const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     const int nw = gridDim.x * kWavesPerBlock;
@@ -507,11 +516,11 @@ __global__ __launch_bounds__(kBlock) void k_push1_fin(FwdArgs a, const int* __re
     __shared__ int s_pre[kWave + 1];
     __shared__ int s_beg[kWave];
     __shared__ unsigned long long s_far[kWave];
-    __shared__ unsigned long long s_new;
+    __shared__ unsigned long long s_new, s_edges;
     if (threadIdx.x < kWave) s_far[threadIdx.x] = 0;
-    if (threadIdx.x == 0) s_new = 0;
+    if (threadIdx.x == 0) s_new = s_edges = 0;
     const int E = push_rows(a.A, a.rp, src, s_pre, s_beg);  // synchronises the block
-    unsigned long long nn = 0;
+    unsigned long long nn = 0, ne = 0;
     FRec* Fn = reinterpret_cast<FRec*>(a.Fn);
     for (int e = blockIdx.x * kBlock + threadIdx.x; e < E; e += gridDim.x * kBlock) {
         const int s = push_lane(s_pre, e);
@@ -533,9 +542,12 @@ __global__ __launch_bounds__(kBlock) void k_push1_fin(FwdArgs a, const int* __re
         const int pc = __popcll(nm);
         if (a.dist_sum) a.dist_sum[u] += (int64_t)pc;  // u's finishing thread only
         nn += (unsigned long long)pc;
+        ne += (unsigned long long)(a.rp[u + 1] - a.rp[u]);  // u's row: level 2's push work
     }
     if (nn) atomicAdd(&s_new, nn);
+    if (ne) atomicAdd(&s_edges, ne);
     __syncthreads();
+    if (threadIdx.x == 0 && s_edges && a.push_e) atomicAdd(a.push_e, s_edges);
     const int slot = blockIdx.x;
     if (threadIdx.x < kWave && s_far[threadIdx.x])
         a.part_far[(size_t)slot * kWave + threadIdx.x] += (int64_t)s_far[threadIdx.x];
@@ -544,6 +556,197 @@ __global__ __launch_bounds__(kBlock) void k_push1_fin(FwdArgs a, const int* __re
         a.part_cnt[(size_t)slot * 2 + 1] += s_new;  // one shortest-path DAG edge per reached pair
         *a.flag = 1;
     }
+}
+
+// ---------------------------------------------------------------- level 2 push
+// Level 2 of a batch by push from the level-1 frontier, where the level-1
+// vertices' rows hold few of the graph's edges (push2_on; on R-MAT 2^20
+// about 15%: the pull would scan every row for them):
+//   k_push2_or (+ _hub)   ORs into the level-2 mask of every neighbour w of a
+//                         level-1 vertex u the lanes of u that have not
+//                         reached w (the level-2 plane starts zeroed);
+//   k_push2_fin           finishes every level-2 vertex as the pull would:
+//                         record base, vis, zeroed path-count slots,
+//                         distance sums and lane partials;
+//   k_push2_sig (+ _hub)  adds, over the same edges, the path count of lane
+//                         s at u into lane s's slot at w, for the lanes of u
+//                         that reach w at level 2.
+// Level-1 counts are 1 per lane (one path: the edge), so a level-2 count is
+// a small integer and its sum exact in any order: the results are the
+// pull's, bit for bit.  Multigraphs, whose level 1 is pulled, are pulled
+// here too.
+
+// ORs row [beg, end) of level-1 vertex u (lanes m1) into level 2.
+__device__ __forceinline__ void push2_or_row(const FwdArgs& a, int beg, int end, uint64_t m1, int lane) {
+    FRec* Fn = reinterpret_cast<FRec*>(a.Fn);
+    for (int j = beg + lane; j < end; j += kWave) {
+        const int w = a.col[j];
+        const uint64_t nm = m1 & ~a.vis[w];
+        if (nm) atomicOr((unsigned long long*)&Fn[w].mask, (unsigned long long)nm);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_push2_or(FwdArgs a) {
+    if ((a.live && *a.live == 0) || !push2_on(a)) return;  // uniform over the grid
+    const int lane = lane_id();
+    const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const int nw = gridDim.x * kWavesPerBlock;
+    const FRec* Fd = reinterpret_cast<const FRec*>(a.Fd);
+    for (int c = gw; c < a.nchunk; c += nw) {
+        const int v = a.chunk_beg[c] + lane;
+        uint64_t m1 = 0;
+        int rb = 0, re = 0;
+        if (v < a.chunk_beg[c + 1]) {
+            m1 = Fd[v].mask;
+            if (m1) {
+                rb = a.rp[v];
+                re = a.rp[v + 1];
+                if (re - rb > a.hub_deg) m1 = 0;  // hub rows: k_push2_or_hub
+            }
+        }
+        for (uint64_t work = ballot(m1 != 0); work; work &= work - 1) {
+            const int b = __builtin_ctzll(work);
+            push2_or_row(a, rl32(rb, b), rl32(re, b), rl64(m1, b), lane);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_push2_or_hub(FwdArgs a, HubArgs h) {
+    if ((a.live && *a.live == 0) || !push2_on(a)) return;
+    const int lane = lane_id();
+    const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const int nw = gridDim.x * kWavesPerBlock;
+    const FRec* Fd = reinterpret_cast<const FRec*>(a.Fd);
+    for (int i = gw; i < h.nseg; i += nw) {
+        const uint64_t m1 = Fd[h.seg_v[i]].mask;  // one address: the same in every lane
+        if (m1) push2_or_row(a, h.seg_beg[i], h.seg_end[i], m1, lane);
+    }
+}
+
+// Launched on at most the forward step's G workgroups (its lane partials go
+// to the block slots).
+template <bool COMPAT>
+__global__ __launch_bounds__(kBlock) void k_push2_fin(FwdArgs a) {
+    if ((a.live && *a.live == 0) || !push2_on(a)) return;
+    const int lane = lane_id();
+    const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const int nw = gridDim.x * kWavesPerBlock;
+    FRec* Fn = reinterpret_cast<FRec*>(a.Fn);
+    FwdAccum r;
+    for (int c = gw; c < a.nchunk; c += nw) {
+        const int v0 = a.chunk_beg[c];
+        const int vl = v0 + lane;
+        uint64_t nm = 0, old = 0;
+        if (vl < a.chunk_beg[c + 1]) {
+            nm = Fn[vl].mask;
+            if (nm) old = a.vis[vl];
+        }
+        for (uint64_t work = ballot(nm != 0); work; work &= work - 1) {
+            const int b = __builtin_ctzll(work);
+            const int v = v0 + b;
+            const uint64_t m = rl64(nm, b), o = rl64(old, b);
+            const int base = __popcll(o);  // lanes that reached v earlier
+            if ((m >> lane) & 1) {
+                const size_t idx = (size_t)v * kWave + base + lane_rank(m);
+                a.sigma[idx] = 0.0;
+                if constexpr (COMPAT) a.npred[idx] = 0u;
+                r.far += a.weight ? (int64_t)a.dnext * a.weight[v] : (int64_t)a.dnext;
+            }
+            if (lane == 0) {
+                Fn[v].base = (uint64_t)base;
+                a.vis[v] = o | m;
+                const int pc = __popcll(m);
+                if (a.dist_sum) a.dist_sum[v] += (int64_t)a.dnext * pc;
+                r.nnew += (unsigned long long)pc;
+            }
+        }
+    }
+    fwd_block_flush(a, r, blockIdx.x);
+}
+
+// Adds the path counts of level-1 vertex u (lanes m1 at row base b1) over
+// row [beg, end) into the level-2 slots; returns the DAG edges it adds (the
+// same in every lane).
+template <bool COMPAT>
+__device__ __forceinline__ unsigned long long push2_sig_row(const FwdArgs& a, int u, int beg, int end,
+                                                            uint64_t m1, int b1, int lane) {
+    const FRec* Fn = reinterpret_cast<const FRec*>(a.Fn);
+    const bool mine = (m1 >> lane) & 1;
+    const double su = mine ? a.sigma[(size_t)u * kWave + b1 + lane_rank(m1)] : 0.0;
+    unsigned long long dag = 0;
+    for (int j0 = beg; j0 < end; j0 += kWave) {
+        const int j = j0 + lane;
+        int w = 0, bw = 0;
+        uint64_t mw = 0;
+        if (j < end) {
+            w = a.col[j];
+            const FRec f = Fn[w];
+            mw = f.mask;
+            bw = (int)f.base;
+        }
+        for (uint64_t ball = ballot((mw & m1) != 0); ball; ball &= ball - 1) {
+            const int c = __builtin_ctzll(ball);
+            const int wc = rl32(w, c);
+            const uint64_t mc = rl64(mw, c);
+            if (mine && ((mc >> lane) & 1)) {
+                const size_t idx = (size_t)wc * kWave + rl32(bw, c) + lane_rank(mc);
+                unsafeAtomicAdd(&a.sigma[idx], su);
+                if constexpr (COMPAT) atomicAdd(&a.npred[idx], 1u);
+            }
+            dag += (unsigned long long)__popcll(mc & m1);
+        }
+    }
+    return dag;
+}
+
+// Launched on at most the forward step's G workgroups, like k_push2_fin.
+template <bool COMPAT>
+__global__ __launch_bounds__(kBlock) void k_push2_sig(FwdArgs a) {
+    if ((a.live && *a.live == 0) || !push2_on(a)) return;
+    const int lane = lane_id();
+    const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const int nw = gridDim.x * kWavesPerBlock;
+    const FRec* Fd = reinterpret_cast<const FRec*>(a.Fd);
+    FwdAccum r;
+    for (int c = gw; c < a.nchunk; c += nw) {
+        const int v = a.chunk_beg[c] + lane;
+        uint64_t m1 = 0;
+        int rb = 0, re = 0, b1 = 0;
+        if (v < a.chunk_beg[c + 1]) {
+            const FRec f = Fd[v];
+            m1 = f.mask;
+            b1 = (int)f.base;
+            if (m1) {
+                rb = a.rp[v];
+                re = a.rp[v + 1];
+                if (re - rb > a.hub_deg) m1 = 0;  // hub rows: k_push2_sig_hub
+            }
+        }
+        for (uint64_t work = ballot(m1 != 0); work; work &= work - 1) {
+            const int b = __builtin_ctzll(work);
+            r.dag += push2_sig_row<COMPAT>(a, a.chunk_beg[c] + b, rl32(rb, b), rl32(re, b), rl64(m1, b),
+                                           rl32(b1, b), lane);
+        }
+    }
+    if (lane != 0) r.dag = 0;  // one count per wave
+    fwd_block_flush(a, r, blockIdx.x);
+}
+
+template <bool COMPAT>
+__global__ __launch_bounds__(kBlock) void k_push2_sig_hub(FwdArgs a, HubArgs h) {
+    if ((a.live && *a.live == 0) || !push2_on(a)) return;
+    const int lane = lane_id();
+    const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const int nw = gridDim.x * kWavesPerBlock;
+    const FRec* Fd = reinterpret_cast<const FRec*>(a.Fd);
+    FwdAccum r;
+    for (int i = gw; i < h.nseg; i += nw) {
+        const int u = h.seg_v[i];
+        const FRec f = Fd[u];
+        if (f.mask) r.dag += push2_sig_row<COMPAT>(a, u, h.seg_beg[i], h.seg_end[i], f.mask, (int)f.base, lane);
+    }
+    if (lane != 0) r.dag = 0;
+    fwd_block_flush(a, r, blockIdx.x);
 }
 
 // ---------------------------------------------------------------- backward

@@ -13,8 +13,9 @@
 // bit-parallel step for closeness.
 //
 // File format, one entry per line, '#' starts a comment:
-//   lg_n deg_x2 skew lg_ecc what  hub_deg chunk_edges grid_waves frontier  ms
-// (a line without the `what` field is a betweenness entry).
+//   lg_n deg_x2 skew lg_ecc what  hub_deg chunk_edges grid_waves frontier push_pct  ms
+// (a line without the `what` field is a betweenness entry; a line without
+// push_pct leaves it 0, the engine default).
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -52,6 +53,7 @@ extern "C" int cent_tune_params_check(const cent_tune_params* p) {
     if (p->chunk_edges < 64 || p->chunk_edges > (1 << 22)) return CENT_E_INVAL;
     if (p->grid_waves < 0 || p->grid_waves > (1 << 22)) return CENT_E_INVAL;
     if (p->frontier < CENT_FRONTIER_AUTO || p->frontier > CENT_FRONTIER_SOURCE) return CENT_E_INVAL;
+    if (p->push_pct < -1 || p->push_pct > 1000) return CENT_E_INVAL;
     return CENT_OK;
 }
 
@@ -127,10 +129,10 @@ extern "C" int cent_tune_table_load(const char* path) {
         while (*p == ' ' || *p == '\t' || *p == '\r' || *p == '\n') ++p;
         if (!*p) continue;
         Entry e{};
-        int v[10] = {0};
+        int v[11] = {0};
         double ms = 0.0;
         int nf = 0;
-        {   // count the fields: 9 (betweenness entry) or 10 (with `what`)
+        {   // count the fields: 9 (betweenness entry), 10 (with `what`), 11 (and push_pct)
             char tmp[512];
             strncpy(tmp, p, sizeof tmp - 1);
             tmp[sizeof tmp - 1] = 0;
@@ -138,19 +140,26 @@ extern "C" int cent_tune_table_load(const char* path) {
             for (char* t = strtok_r(tmp, " \t\r\n", &save); t; t = strtok_r(nullptr, " \t\r\n", &save)) ++nf;
         }
         int got = -1;
-        if (nf == 10) {
+        if (nf == 11) {
+            got = sscanf(p, "%d %d %d %d %d %d %d %d %d %d %lf", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6],
+                         &v[7], &v[8], &v[9], &ms);
+            e.sig = cent_tune_sig{v[0], v[1], v[2], v[3]};
+            e.what = v[4];
+            e.p = cent_tune_params{v[5], v[6], v[7], v[8], v[9], ms};
+            got = got == 11 ? 0 : -1;
+        } else if (nf == 10) {
             got = sscanf(p, "%d %d %d %d %d %d %d %d %d %lf", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6],
                          &v[7], &v[8], &ms);
             e.sig = cent_tune_sig{v[0], v[1], v[2], v[3]};
             e.what = v[4];
-            e.p = cent_tune_params{v[5], v[6], v[7], v[8], ms};
+            e.p = cent_tune_params{v[5], v[6], v[7], v[8], 0, ms};
             got = got == 10 ? 0 : -1;
         } else if (nf == 9) {
             got = sscanf(p, "%d %d %d %d %d %d %d %d %lf", &v[0], &v[1], &v[2], &v[3], &v[5], &v[6], &v[7], &v[8],
                          &ms);
             e.sig = cent_tune_sig{v[0], v[1], v[2], v[3]};
             e.what = CENT_TUNE_BC;
-            e.p = cent_tune_params{v[5], v[6], v[7], v[8], ms};
+            e.p = cent_tune_params{v[5], v[6], v[7], v[8], 0, ms};
             got = got == 9 ? 0 : -1;
         }
         if (got != 0 || !what_ok(e.what) || cent_tune_params_check(&e.p) != CENT_OK) {
@@ -174,10 +183,10 @@ extern "C" int cent_tune_table_save(const char* path) {
     }
     FILE* f = fopen(path, "w");
     if (!f) return CENT_E_IO;
-    fprintf(f, "# cent autotuner table: lg_n deg_x2 skew lg_ecc what  hub_deg chunk_edges grid_waves frontier  ms\n");
+    fprintf(f, "# cent autotuner table: lg_n deg_x2 skew lg_ecc what  hub_deg chunk_edges grid_waves frontier push_pct  ms\n");
     for (const auto& e : snap)
-        fprintf(f, "%d %d %d %d %d  %d %d %d %d  %.4f\n", e.sig.lg_n, e.sig.deg_x2, e.sig.skew, e.sig.lg_ecc, e.what,
-                e.p.hub_deg, e.p.chunk_edges, e.p.grid_waves, e.p.frontier, e.p.ms);
+        fprintf(f, "%d %d %d %d %d  %d %d %d %d %d  %.4f\n", e.sig.lg_n, e.sig.deg_x2, e.sig.skew, e.sig.lg_ecc, e.what,
+                e.p.hub_deg, e.p.chunk_edges, e.p.grid_waves, e.p.frontier, e.p.push_pct, e.p.ms);
     const bool ok = fflush(f) == 0;
     fclose(f);
     return ok ? CENT_OK : CENT_E_IO;

@@ -172,6 +172,11 @@ typedef struct cent_opts {
     int32_t frontier;     /* CENT_FRONTIER_*                                 */
     int32_t max_slots;    /* SOURCE mode: cap on concurrent per-source slots
                              (0 = sized from free device memory)          */
+    int32_t push_pct;     /* DENSE betweenness: a batch's level 2 is pushed
+                             from its level-1 frontier when that frontier's
+                             rows hold under push_pct % of the traversed
+                             edges, else pulled (0 = the graph's tuned value
+                             or the engine default, -1 = always pulled)     */
 } cent_opts;
 
 typedef struct cent_stats {
@@ -327,6 +332,8 @@ typedef struct cent_tune_params {
     int32_t grid_waves;   /* 0 = engine default (2048 workgroups of 4 waves);
                              source-parallel step: grid_waves / 8 slots   */
     int32_t frontier;     /* CENT_FRONTIER_*; AUTO = upload-time heuristic   */
+    int32_t push_pct;     /* DENSE betweenness: level-2 push threshold, as
+                             cent_opts.push_pct (0 = engine default)        */
     double  ms;           /* measured trial time (cent_tune), else 0         */
 } cent_tune_params;
 

@@ -56,7 +56,7 @@ def test_table_exact_nearest_and_miss():
 
 
 def test_table_save_load_roundtrip(tmp_path):
-    p = bc_amd.tune_params(hub_deg=1024, chunk_edges=4096, grid_waves=32768, frontier="dense", ms=3.25)
+    p = bc_amd.tune_params(hub_deg=1024, chunk_edges=4096, grid_waves=32768, frontier="dense", ms=3.25, push_pct=-1)
     q = bc_amd.tune_params(hub_deg=512, chunk_edges=512, grid_waves=0, frontier="source", ms=1.5)
     bc_amd.tune_table_put((10, 6, 5, 2), p, bc_amd.CENT_TUNE_BC)
     bc_amd.tune_table_put((10, 6, 5, 2), q, bc_amd.CENT_TUNE_CC)
@@ -68,7 +68,7 @@ def test_table_save_load_roundtrip(tmp_path):
     bc_amd.tune_table_load(f)
     got = bc_amd.tune_table_get((10, 6, 5, 2))
     assert got.as_dict() == {"hub_deg": 1024, "chunk_edges": 4096, "grid_waves": 32768,
-                             "frontier": "dense", "ms": 3.25}
+                             "frontier": "dense", "push_pct": -1, "ms": 3.25}
     assert bc_amd.tune_table_get((10, 6, 5, 2), bc_amd.CENT_TUNE_CC).frontier == bc_amd.FRONTIER["source"]
 
 
@@ -89,7 +89,9 @@ def test_table_rejects_bad_input(tmp_path):
     with pytest.raises(bc_amd.CentError):
         bc_amd.tune_table_put((1, 1, 1, 1), bc_amd.tune_params(hub_deg=8))
     with pytest.raises(bc_amd.CentError):
-        bc_amd.tune_table_put((1, 1, 1, 1), bc_amd.TuneParams(256, 1024, 0, 7, 0.0))
+        bc_amd.tune_table_put((1, 1, 1, 1), bc_amd.TuneParams(256, 1024, 0, 7, 0, 0.0))
+    with pytest.raises(bc_amd.CentError):
+        bc_amd.tune_table_put((1, 1, 1, 1), bc_amd.tune_params(push_pct=-2))
     good = bc_amd.tune_params()
     bc_amd.tune_table_put((3, 3, 3, 3), good)
     f = tmp_path / "bad.txt"
@@ -99,6 +101,16 @@ def test_table_rejects_bad_input(tmp_path):
     assert bc_amd.tune_table_size() == 1        # nothing merged from the bad file
     with pytest.raises(bc_amd.CentError):
         bc_amd.tune_table_load(str(tmp_path / "absent.txt"))
+
+
+def test_table_reads_lines_without_push_pct(tmp_path):
+    f = tmp_path / "old.txt"
+    f.write_text("# older form: no push_pct column\n12 10 6 2 2  4096 256 0 1  0.6745\n"
+                 "12 10 6 2 512 512 0 1 0.8\n")
+    bc_amd.tune_table_load(str(f))
+    assert bc_amd.tune_table_get((12, 10, 6, 2), bc_amd.CENT_TUNE_CC).push_pct == 0
+    got = bc_amd.tune_table_get((12, 10, 6, 2), bc_amd.CENT_TUNE_BC)
+    assert (got.hub_deg, got.push_pct) == (512, 0)
 
 
 def test_bundled_table_parses():
