@@ -588,7 +588,7 @@ def kernel_sha16() -> str:
 
 def traffic_key(args, layout) -> dict:
     return {"scale": args.scale, "edge_factor": args.edge_factor, "sources": args.sources,
-            "launch": {k: layout[k] for k in ("hub_deg", "chunk_edges", "grid_waves", "frontier")},
+            "launch": {k: layout[k] for k in ("hub_deg", "chunk_edges", "grid_waves", "frontier", "push_pct")},
             "kernel_sha16": kernel_sha16()}
 
 
