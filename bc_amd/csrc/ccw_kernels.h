@@ -19,6 +19,19 @@ namespace cent {
 
 constexpr int kCcwUnroll = 4;  // edge groups whose loads are issued together
 
+// Workgroup size of the kernels that keep a per-workgroup LDS table of lane
+// partials (k_ccw, k_ccw_hubfin): W x 64 counters, 32 KB at W = 64.  Each
+// workgroup adds its table into its own slot of part_far at the end of every
+// level, and at W = 64 the table, not the registers, limits the waves a CU
+// holds (4 a SIMD at 256 threads).  There 512-thread workgroups write half
+// the partials and hold 6 waves a SIMD: all-vertex closeness on a 174k-vertex
+// R-MAT 48.8 -> 45.5 ms reduced, 83.6 -> 79.0 ms plain (1024 threads: no
+// gain).  At W = 32 the two sizes measure the same, and at W = 16 512
+// threads lose (4096 sources on R-MAT 2^20: 8.45 -> 8.9 ms): a workgroup
+// waits at its flush for its slowest wave.
+constexpr int kCcwBigBlock = 512;
+constexpr int ccw_block(int W) { return W >= 64 ? kCcwBigBlock : kBlock; }
+
 struct CcwArgs {
     int hub_deg;
     int nchunk;
@@ -208,16 +221,17 @@ __device__ __forceinline__ void ccw_block_init(long long (*s_far)[kWave], unsign
 }
 
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_ccw(CcwArgs a) {
+__global__ __launch_bounds__(ccw_block(W)) void k_ccw(CcwArgs a) {
     if (a.live && *a.live == 0) return;  // uniform over the grid
     __shared__ long long s_far[W][kWave];
     __shared__ unsigned long long s_new;
     ccw_block_init<W>(s_far, &s_new);
+    constexpr int WPB = ccw_block(W) / kWave;
     const int lane = lane_id();
     const int bl = lane & (W - 1);
     const uint64_t Ab = ccw_active(a, bl);
-    const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-    const int nw = gridDim.x * kWavesPerBlock;
+    const int gw = blockIdx.x * WPB + (threadIdx.x >> 6);
+    const int nw = gridDim.x * WPB;
     unsigned long long nn = 0;
     CcwCount cnt;
     for (int c = gw; c < a.nchunk; c += nw) {
@@ -276,17 +290,18 @@ __global__ __launch_bounds__(kBlock) void k_ccw_hubseg(CcwArgs a) {
 }
 
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_ccw_hubfin(CcwArgs a) {
+__global__ __launch_bounds__(ccw_block(W)) void k_ccw_hubfin(CcwArgs a) {
     if (a.live && *a.live == 0) return;
     __shared__ long long s_far[W][kWave];
     __shared__ unsigned long long s_new;
     ccw_block_init<W>(s_far, &s_new);
     constexpr int E = kWave / W;
+    constexpr int WPB = ccw_block(W) / kWave;
     const int lane = lane_id();
     const int bl = lane & (W - 1), el = lane / W;
     const uint64_t Ab = ccw_active(a, bl);
-    const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-    const int nw = gridDim.x * kWavesPerBlock;
+    const int gw = blockIdx.x * WPB + (threadIdx.x >> 6);
+    const int nw = gridDim.x * WPB;
     unsigned long long nn = 0;
     CcwCount cnt;
     for (int k = gw; k < a.nhub; k += nw) {

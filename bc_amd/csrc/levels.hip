@@ -151,10 +151,10 @@ static void ccw_level(const CcwArgs& ca, int lv, int G, int Gp, int Gseg, int Gh
         k_ccw_push_fin<W><<<dim3(Gp, nb), kBlock, 0, s>>>(ca, d_src);
         return;
     }
-    k_ccw<W><<<G, kBlock, 0, s>>>(ca);
+    k_ccw<W><<<G, ccw_block(W), 0, s>>>(ca);
     if (hubs) {
         k_ccw_hubseg<W><<<Gseg, kBlock, 0, s>>>(ca);
-        k_ccw_hubfin<W><<<Ghub, kBlock, 0, s>>>(ca);
+        k_ccw_hubfin<W><<<Ghub, ccw_block(W), 0, s>>>(ca);
     }
 }
 
@@ -175,7 +175,11 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
     const size_t R = (size_t)(v.hi - v.lo);
     const int nb = (k + kWave - 1) / kWave;
     if (nb < 1 || nb > kMultiBatch || (sp.with_bc && nb != 1)) return CENT_E_INVAL;
-    const int G = grid_for(v.nchunk, sp.opts);
+    const int W = sp.with_bc ? 1 : ccw_width(nb);
+    // workgroups of the level kernels (they hold the partial slots): the
+    // closeness kernels run the same waves in ccw_block(W)-thread workgroups
+    const int G = sp.with_bc ? grid_for(v.nchunk, sp.opts)
+                             : std::max(1, grid_for(v.nchunk, sp.opts) * kBlock / ccw_block(W));
     TRY(ensure_workspace(ctx, d, sp, G, nb));
     if (bound > kHostFlags - 2) bound = 0;  // the level flags of one run must fit
     if (bound > 0 && sp.with_bc)
@@ -186,7 +190,6 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
     // F planes: BC level lv at F + lv * PL ({mask, base} records, PL = 2N
     // words); closeness level parity p at F + p * PL, W interleaved masks a
     // vertex (PL = W N words)
-    const int W = sp.with_bc ? 1 : ccw_width(nb);
     const size_t fw = sp.with_bc ? 2 : (size_t)W;
     const size_t PL = N * fw;
     auto plane = [&](size_t p) { return ctx->F.p + p * PL; };
@@ -242,7 +245,8 @@ int level_batch(cent_ctx* ctx, const GView& v, const BatchSpec& sp, const int* d
         ha.nhub = v.nhub; ha.hub_v = d.hub_v + v.h0; ha.hub_seg_off = d.hub_seg_off + v.h0;
         ha.part_acc = ctx->hp_acc.p; ha.part_cnt = ctx->hp_cnt.p;
         Gseg = std::min((v.nseg + kWavesPerBlock - 1) / kWavesPerBlock, 4096);
-        Ghub = std::min((v.nhub + kWavesPerBlock - 1) / kWavesPerBlock, G);  // hubfin flushes into G slots
+        const int hw = sp.with_bc ? kWavesPerBlock : ccw_block(W) / kWave;
+        Ghub = std::min((v.nhub + hw - 1) / hw, G);  // hubfin flushes into G slots
     }
     const bool bc = sp.with_bc, compat = sp.compat;
     // Forward levels go in runs enqueued without waiting: level d0 + i of a
