@@ -18,6 +18,11 @@
 namespace cent {
 
 constexpr int kCcwUnroll = 4;  // edge groups whose loads are issued together
+// The same in the hub-segment kernel, whose long rows gain from more loads
+// in flight: all-vertex closeness on a 174k-vertex R-MAT 45.5 -> 44.7 ms
+// reduced, 79.3 -> 77.0 ms plain at 8 (12 and 16: the same as 8; 8 in the
+// main kernel too: slower).
+constexpr int kCcwUnrollHub = 8;
 
 // Workgroup size of the kernels that keep a per-workgroup LDS table of lane
 // partials (k_ccw, k_ccw_hubfin): W x 64 counters, 32 KB at W = 64.  Each
@@ -78,23 +83,23 @@ __device__ __forceinline__ uint64_t ccw_batch_or(uint64_t x) {
 
 // Lanes of batch b that reach some neighbour in edges [beg, end) of Fd,
 // restricted to cand; the result is in every lane of batch b.
-template <int W>
+template <int W, int U = kCcwUnroll>
 __device__ __forceinline__ uint64_t ccw_scan(const CcwArgs& a, int beg, int end, uint64_t cand, int lane) {
     constexpr int E = kWave / W;
     const int b = lane & (W - 1), e = lane / W;
     uint64_t orf = 0;
-    for (int j0 = beg; j0 < end; j0 += E * kCcwUnroll) {
-        int u[kCcwUnroll];
+    for (int j0 = beg; j0 < end; j0 += E * U) {
+        int u[U];
 #pragma unroll
-        for (int t = 0; t < kCcwUnroll; ++t) {
+        for (int t = 0; t < U; ++t) {
             const int j = j0 + t * E + e;
             u[t] = j < end ? a.col[j] : -1;
         }
 #pragma unroll
-        for (int t = 0; t < kCcwUnroll; ++t)
+        for (int t = 0; t < U; ++t)
             if (u[t] >= 0) orf |= a.Fd[(size_t)u[t] * W + b];
         orf &= cand;
-        if (j0 + E * kCcwUnroll < end) {
+        if (j0 + E * U < end) {
             const uint64_t g = ccw_batch_or<W>(orf);
             if (ballot(g != cand) == 0) return g;  // every candidate lane found
         }
@@ -284,7 +289,7 @@ __global__ __launch_bounds__(kBlock) void k_ccw_hubseg(CcwArgs a) {
         const int v = a.seg_v[i];
         const uint64_t cand = Ab & ~a.vis[(size_t)v * W + bl];
         if (ballot(cand != 0) == 0) continue;  // k_ccw_hubfin skips the vertex too
-        const uint64_t nm = ccw_scan<W>(a, a.seg_beg[i], a.seg_end[i], cand, lane);
+        const uint64_t nm = ccw_scan<W, kCcwUnrollHub>(a, a.seg_beg[i], a.seg_end[i], cand, lane);
         if (lane < W) a.part_or[(size_t)i * W + bl] = nm;
     }
 }
