@@ -188,39 +188,43 @@ constexpr int kEllLong = INT32_MIN;  // inline-row marker: the row is read from 
 static_assert(kPsE == 4, "inline rows hold kPsE = 4 neighbours");
 
 // Row of u for the staged expansions: wi = its neighbours from the inline
-// row (rb = dg = 0, no tail), or wi[0] = kEllLong and rb / dg its CSR row.
-// u < 0: an empty row.
-__device__ __forceinline__ void ps_row(const PsArgs& a, int u, int* wi, int& rb, int& dg) {
+// row (rb = dg = 0, no tail), or wi.x = kEllLong and rb / dg its CSR row.
+// u < 0: an empty row.  A staged row is one int4 register quadruple (an int
+// array here left dead stores to scratch memory behind).
+__device__ __forceinline__ void ps_row(const PsArgs& a, int u, int4& wi, int& rb, int& dg) {
     rb = 0;
     dg = 0;
-    if (u < 0) {
-        wi[0] = wi[1] = wi[2] = wi[3] = -1;
-        return;
-    }
+    wi = make_int4(-1, -1, -1, -1);
+    if (u < 0) return;
     if (a.ell) {
         const int4 e4 = a.ell[u];
         if (e4.x != kEllLong) {
-            wi[0] = e4.x; wi[1] = e4.y; wi[2] = e4.z; wi[3] = e4.w;
+            wi = e4;
             return;
         }
     }
-    wi[0] = kEllLong;
+    wi.x = kEllLong;
     rb = a.rp[u];
     dg = a.rp[u + 1] - rb;
 }
 
+// Component e (< 4) of v.
+__device__ __forceinline__ int ps_lane4(const int4& v, int e) {
+    return e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w;
+}
+
 // Neighbour e (< kPsE) of a row staged by ps_row, -1 past its end.
-__device__ __forceinline__ int ps_nbr(const PsArgs& a, const int* wi, int rb, int dg, int e) {
-    return wi[0] != kEllLong ? wi[e] : (e < dg ? a.col[rb + e] : -1);
+__device__ __forceinline__ int ps_nbr(const PsArgs& a, const int4& wi, int rb, int dg, int e) {
+    return wi.x != kEllLong ? ps_lane4(wi, e) : (e < dg ? a.col[rb + e] : -1);
 }
 
 // True for a row staged by ps_row whose edges all sit in the staged slots
 // (an inline row, or a CSR row of at most kPsE edges): its successor bits
 // cover the whole row.
-__device__ __forceinline__ bool ps_short(const int* wi, int dg) { return wi[0] != kEllLong || dg <= kPsE; }
+__device__ __forceinline__ bool ps_short(const int4& wi, int dg) { return wi.x != kEllLong || dg <= kPsE; }
 
 template <class Rec>
-__device__ __forceinline__ void ps_expand_rec(const PsArgs& a, Rec* R, int* Q, const int (*wi)[kPsE],
+__device__ __forceinline__ void ps_expand_rec(const PsArgs& a, Rec* R, int* Q, const int4* wi,
                                               const int* rb, const int* dg, int ep, int d, int tp, int le,
                                               int* s_cnt, PsLocal& L, double* acc, uint32_t* np, uint32_t* sb) {
     const int tn = ps_tag(ep, d + 1);
@@ -339,7 +343,7 @@ __device__ __forceinline__ double ps_pull_long(const PsArgs& a, const Rec* R, in
 // row with its successor bits (mk[m] >= 0) reads only its successors'
 // counts; any other row (mk[m] < 0) tests every neighbour's level.
 template <class Rec>
-__device__ __forceinline__ void ps_pull_rec(const PsArgs& a, const Rec* R, const int (*wi)[kPsE], const int* rb,
+__device__ __forceinline__ void ps_pull_rec(const PsArgs& a, const Rec* R, const int4* wi, const int* rb,
                                              const int* dg, const int* mk, int tl, double* acc) {
     int w[kPsV][kPsE], lev[kPsV][kPsE];
     double x[kPsV][kPsE];
@@ -447,7 +451,8 @@ __global__ __launch_bounds__(kPsBlock) __attribute__((amdgpu_waves_per_eu(4))) v
                 }
             };
             for (int i0 = lb + tid; i0 < le; i0 += kPsBlock * kPsV) {
-                int u[kPsV], rb[kPsV], dg[kPsV], wi[kPsV][kPsE];
+                int u[kPsV], rb[kPsV], dg[kPsV];
+                int4 wi[kPsV];
 #pragma unroll
                 for (int m = 0; m < kPsV; ++m) {
                     const int i = i0 + m * kPsBlock;
@@ -547,7 +552,8 @@ __global__ __launch_bounds__(kPsBlock) __attribute__((amdgpu_waves_per_eu(4))) v
                         const int i = i0 + m * kPsBlock;
                         v[m] = i < b1 ? Q[i] : -1;
                     }
-                    int wi[kPsV][kPsE], mk[kPsV];
+                    int4 wi[kPsV];
+                    int mk[kPsV];
                     uint32_t pw[kPsV];
 #pragma unroll
                     for (int m = 0; m < kPsV; ++m) {
@@ -647,7 +653,7 @@ __device__ __forceinline__ bool bit_claim(uint32_t* M, int v) {
 
 // Expands up to kPsV rows at once, as ps_expand_rec: all column loads, then
 // all map words, then all claims, then the bookkeeping.
-__device__ __forceinline__ void ps_expand_bits(const PsArgs& a, uint32_t* V, int* Q, const int (*wi)[kPsE],
+__device__ __forceinline__ void ps_expand_bits(const PsArgs& a, uint32_t* V, int* Q, const int4* wi,
                                                const int* rb, const int* dg, int d, int le, int* s_cnt, PsLocal& L) {
     int w[kPsV][kPsE];
     bool seen[kPsV][kPsE], got[kPsV][kPsE];
@@ -708,7 +714,8 @@ __global__ __launch_bounds__(kPsBlock) void k_ps_cc(PsArgs a) {
                 s_nbig[(d + 1) % 3] = 0;
             }
             for (int i0 = lb + tid; i0 < le; i0 += kPsBlock * kPsV) {
-                int rb[kPsV], dg[kPsV], wi[kPsV][kPsE];
+                int rb[kPsV], dg[kPsV];
+                int4 wi[kPsV];
 #pragma unroll
                 for (int m = 0; m < kPsV; ++m) {
                     const int i = i0 + m * kPsBlock;
