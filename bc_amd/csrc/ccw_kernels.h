@@ -226,8 +226,10 @@ __device__ __forceinline__ void ccw_block_init(long long (*s_far)[kWave], unsign
 }
 
 // Held to 64 VGPRs (8 waves a SIMD; a few registers of the count and flush
-// spill): all-vertex closeness on a 174k-vertex R-MAT 44.8 -> 43.9 ms
-// reduced, 77.4 -> 76.1 ms plain against the unconstrained 73 (6 waves).
+// spill) against the unconstrained 63-115: all-vertex closeness on a
+// 174k-vertex R-MAT 44.8 -> 43.9 ms reduced, 77.4 -> 76.1 ms plain; sampled
+// closeness on R-MAT 2^20 from 8192 sources (W = 32) 16.5-17.3 -> 13.9-14.3
+// ms, 16384 (W = 64) 26.7-28.5 -> 23.6-23.8 ms, 2048 and 4096 the same.
 template <int W>
 __global__ __launch_bounds__(ccw_block(W)) __attribute__((amdgpu_waves_per_eu(8))) void k_ccw(CcwArgs a) {
     if (a.live && *a.live == 0) return;  // uniform over the grid
