@@ -60,6 +60,17 @@ def timed(fn, reps: int):
     return sum(ts) / len(ts)
 
 
+def reduced_work_edges(red) -> int:
+    """Source-edge pairs the reduced traversal walks: every packed vertex of a
+    block is a BFS source over that block's edges, so sum over blocks of
+    n_b * m_b ("work TEPS" counts these; "effective TEPS" counts n * m of
+    the original graph)."""
+    import numpy as np
+    b = red.boff.astype(np.int64)
+    rp = red.prp.astype(np.int64)
+    return int((np.diff(b) * ((rp[b[1:]] - rp[b[:-1]]) // 2)).sum())
+
+
 def extra_blocktree(ctx, args):
     """Block-tree workload: 2000 blocks (Pareto(1.5) sizes in [8, 50000], ER avg
     degree 6 + Hamiltonian cycle), random tree of shared articulation vertices,
@@ -96,6 +107,7 @@ def extra_blocktree(ctx, args):
     tbi = timed(lambda: ctx.bc_reduced(red, bc_amd.CENT_SCHED_PER_ITEM), 1)
     log(f"blocktree BC+CC: persistent {tbp * 1e3:.2f} ms, per-item {tbi * 1e3:.1f} ms")
     n, m = g.n, g.m
+    work = reduced_work_edges(red)
     # the reference's own compute_D1_AP_CC_shareBased_DegreeOrder (Seq_CC/CC.cpp:1728-2052),
     # built from its sources into oracle/_ref, on the same graph, one host thread
     ref_cc = os.path.join(ROOT, "oracle", "_ref", "ref_cc")
@@ -128,9 +140,11 @@ def extra_blocktree(ctx, args):
         "queries_per_s_persistent": round(1.0 / tp, 2), "queries_per_s_per_item": round(1.0 / ti, 2),
         "speedup_vs_per_item": round(ti / tp, 2),
         "effective_TEPS_persistent": round(n * m / tp, 1),
+        "reduced_work_edges": work, "work_TEPS_persistent": round(work / tp, 1),
         "bc_cc_persistent_ms": round(tbp * 1e3, 3), "bc_cc_per_item_ms": round(tbi * 1e3, 3),
         "bc_cc_speedup_vs_per_item": round(tbi / tbp, 2),
         "bc_cc_effective_TEPS_persistent": round(n * m / tbp, 1),
+        "bc_cc_work_TEPS_persistent": round(work / tbp, 1),
     })
     return res
 
@@ -254,6 +268,9 @@ def extra_closeness(ctx, args):
             "rmat20_TEPS": round(len(src) * g.m / t20, 1),
             "pool_rmat_n": gp.n, "pool_rmat_m": gp.m,
             "reduced_ms": round(tr * 1e3, 2), "plain_ms": round(tp * 1e3, 2),
+            "reduced_effective_TEPS": round(gp.n * gp.m / tr, 1),
+            "reduced_work_TEPS": round(reduced_work_edges(red) / tr, 1),
+            "plain_TEPS": round(gp.n * gp.m / tp, 1),
             "farness_equal": bool(np.array_equal(far_r, far_p))}
 
 
