@@ -395,6 +395,7 @@ _lib.cent_ctx_workspace_bytes.restype = C.c_int64
 _lib.cent_ctx_trim.argtypes = [C.c_void_p]
 _lib.cent_probe_copy_bandwidth.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.POINTER(C.c_double)]
 _lib.cent_probe_gather_bandwidth.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.POINTER(C.c_double)]
+_lib.cent_probe_gather_lines.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.POINTER(C.c_double)]
 _lib.cent_cc_reduced.argtypes = [C.c_void_p, C.c_void_p, _I64P, C.c_int32, C.POINTER(_Opts)]
 _lib.cent_bc_reduced.argtypes = [C.c_void_p, C.c_void_p, _F64P, _I64P, C.c_int32, C.POINTER(_Opts)]
 _lib.cent_bc_reduced_sources.argtypes = [C.c_void_p, C.c_void_p, _I32P, C.c_int32, _F64P, _I64P, C.c_int32,
@@ -711,6 +712,13 @@ class Context:
         out = C.c_double()
         _check(_lib.cent_probe_gather_bandwidth(self._h, nbytes, piece, iters, C.byref(out)),
                "cent_probe_gather_bandwidth", self.last_error())
+        return out.value
+
+    def probe_gather_lines(self, lines: int, nbytes: int = 1 << 30, iters: int = 10) -> float:
+        """Line GB/s of loads that each read ``lines`` (1, 2, 4) random whole 128-byte lines."""
+        out = C.c_double()
+        _check(_lib.cent_probe_gather_lines(self._h, nbytes, lines, iters, C.byref(out)),
+               "cent_probe_gather_lines", self.last_error())
         return out.value
 
     def upload(self, csr: CSR, keep_order: bool = False, tuned: bool = True, order: str = "auto") -> DeviceGraph:

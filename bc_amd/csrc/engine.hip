@@ -702,6 +702,70 @@ extern "C" int cent_probe_gather_bandwidth(cent_ctx* ctx, int64_t bytes, int32_t
     return rc;
 }
 
+// Line-gather probe: each wave load instruction reads LC random whole
+// 128-byte lines (16 lanes a line, 8 bytes a lane; the wave's other lanes
+// issue nothing), eight loads in flight per lane.  The memory pipeline's
+// rate per load instruction, not per byte, limits loads that touch one or
+// two lines, which is what the dense step's slice gathers are.
+template <int LC>
+__global__ __launch_bounds__(256) void k_gather_lines_probe(const double* __restrict__ a, uint32_t nlines, int steps,
+                                                            double* __restrict__ sink) {
+    constexpr int U = 8;
+    const int lane = (int)(threadIdx.x & 63);
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    double acc = 0.0;
+    if (lane < 16 * LC) {
+        const uint32_t grp = (uint32_t)(lane >> 4);
+        const int sub = lane & 15;
+        for (int it = 0; it < steps; it += U) {
+            double v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t h = probe_hash(wave * 0x9E3779B9u + (uint32_t)(it + u) * 0x85EBCA6Bu + grp * 0x27D4EB2Fu);
+                v[u] = a[(size_t)(h % nlines) * 16 + sub];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) acc += v[u];
+        }
+    }
+    if (acc != 0.0) sink[0] = acc;  // the buffer is zero: never taken, keeps the loads
+}
+
+extern "C" int cent_probe_gather_lines(cent_ctx* ctx, int64_t bytes, int32_t lines, int32_t iters, double* gbs) {
+    if (!ctx || !gbs || iters < 1 || (lines != 1 && lines != 2 && lines != 4) || bytes < 128 * 4 ||
+        bytes / 128 > (int64_t)UINT32_MAX)
+        return CENT_E_INVAL;
+    TRY(check_device(ctx));
+    double* a = nullptr;
+    HIPTRY(hipMalloc((void**)&a, (size_t)bytes + 64));
+    double* sink = a + bytes / 8;
+    ctx->ev_used = 0;
+    hipEvent_t e0 = next_event(ctx), e1 = next_event(ctx);
+    int rc = CENT_OK;
+    if (!e0 || !e1 || hipMemsetAsync(a, 0, (size_t)bytes + 64, ctx->stream) != hipSuccess) rc = CENT_E_DEVICE;
+    const int grid = 256 * 32, steps = 64;
+    const uint32_t nl = (uint32_t)(bytes / 128);
+    auto launch = [&] {
+        switch (lines) {
+            case 1: k_gather_lines_probe<1><<<grid, 256, 0, ctx->stream>>>(a, nl, steps, sink); break;
+            case 2: k_gather_lines_probe<2><<<grid, 256, 0, ctx->stream>>>(a, nl, steps, sink); break;
+            default: k_gather_lines_probe<4><<<grid, 256, 0, ctx->stream>>>(a, nl, steps, sink); break;
+        }
+    };
+    if (!rc) {
+        launch();  // warm
+        (void)hipEventRecord(e0, ctx->stream);
+        for (int i = 0; i < iters; ++i) launch();
+        (void)hipEventRecord(e1, ctx->stream);
+        float ms = 0;
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = CENT_E_DEVICE;
+        else *gbs = (double)grid * 4 * steps * lines * 128.0 * iters / (ms * 1e-3) / 1e9;
+    }
+    ctx->ev_used = 0;
+    (void)hipFree(a);
+    return rc;
+}
+
 // ------------------------------------------------------------ device arrays
 extern "C" int cent_device_alloc(cent_ctx* ctx, int64_t bytes, void** out) {
     if (!ctx || !out || bytes < 0) return CENT_E_INVAL;

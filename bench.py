@@ -897,12 +897,16 @@ def main() -> int:
         roofline["copy_probe_GBs"] = round(ctx.probe_copy_bandwidth(1 << 30, 10), 1)
     else:
         # ceilings of this access pattern on this GPU, measured now: a plain
-        # copy, and random gathers of whole 128-byte lines and of 64-byte
-        # halves (the fabric moves whole lines, so a 64-byte slice of a
-        # sigma row costs a line)
+        # copy, random gathers of whole 128-byte lines and of 64-byte halves
+        # (the fabric moves whole lines, so a 64-byte slice of a sigma row
+        # costs a line), and whole lines read 1, 2 or 4 to a load
+        # instruction (the slice gathers touch 1.3 lines a load in the
+        # forward pass and 2.8 in the backward)
         roofline["ceilings_GBs"] = {"copy": round(ctx.probe_copy_bandwidth(1 << 30, 10), 1),
                                     "gather_128B": round(ctx.probe_gather_bandwidth(128, 1 << 30, 10), 1),
-                                    "gather_64B": round(ctx.probe_gather_bandwidth(64, 1 << 30, 10), 1)}
+                                    "gather_64B": round(ctx.probe_gather_bandwidth(64, 1 << 30, 10), 1),
+                                    "lines_per_load": {str(c): round(ctx.probe_gather_lines(c, 1 << 30, 10), 1)
+                                                       for c in (1, 2, 4)}}
 
     cpu = None
     if not args.no_cpu_baseline:

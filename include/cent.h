@@ -213,6 +213,12 @@ int  cent_probe_copy_bandwidth(cent_ctx* ctx, int64_t bytes, int32_t iters, doub
  * per lane, and returns the useful bytes read per second in GB/s: the
  * ceiling for gathers of row slices of that size. */
 int  cent_probe_gather_bandwidth(cent_ctx* ctx, int64_t bytes, int32_t piece, int32_t iters, double* gbs);
+/* Measurement helper (no reference counterpart): times `iters` launches in
+ * which every wave load instruction reads `lines` (1, 2 or 4) random whole
+ * 128-byte lines of a zeroed buffer of `bytes` bytes, and returns the line
+ * bytes read per second in GB/s: the ceiling for gathers whose loads touch
+ * that many lines each (the per-instruction rate limits 1 and 2). */
+int  cent_probe_gather_lines(cent_ctx* ctx, int64_t bytes, int32_t lines, int32_t iters, double* gbs);
 
 /* Uploads a CSR (validated; ids in range, offsets monotone).  By default the
  * device copy is relabelled for cache locality: by descending degree when
