@@ -252,6 +252,9 @@ struct cent_graph {
     // kLandmarks highest-degree vertices (empty: deep graphs, which run the
     // source-parallel step)
     std::vector<uint64_t> lm_key;
+    // [n] with lm_key: each device id's highest-degree neighbour (the lowest
+    // id among equals; -1 for none), the grouping key's tie-break
+    std::vector<int32_t> lm_top;
     bool bfs_order = false;           // device ids are a BFS order (else degree order or the caller's)
     bool multi = false;               // parallel edges or self-loops (CENT_CSR_RAW input)
     cent_tune_sig sig{};              // autotuner signature

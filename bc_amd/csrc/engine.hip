@@ -298,10 +298,10 @@ __global__ __launch_bounds__(256) void k_lm_level(const int32_t* __restrict__ rp
     }
 }
 
-// Fills g->lm_key from the device CSR g->d (device ids; rp: its host row
-// offsets).  Only shallow graphs get keys: deep ones run the source-parallel
-// step, which takes sources in caller order.
-static int lm_keys(cent_ctx* ctx, cent_graph* g, const int32_t* rp) {
+// Fills g->lm_key from the device CSR g->d (device ids; rp / col: its host
+// copy), and g->lm_top.  Only shallow graphs get keys: deep ones run the
+// source-parallel step, which takes sources in caller order.
+static int lm_keys(cent_ctx* ctx, cent_graph* g, const int32_t* rp, const int32_t* col) {
     const int32_t n = g->n;
     if (n < 2 || g->ecc_probe > kDeepLevels) return CENT_OK;
     const int L = n < kLandmarks ? (int)n : kLandmarks;
@@ -355,7 +355,25 @@ static int lm_keys(cent_ctx* ctx, cent_graph* g, const int32_t* rp) {
     }
     (void)hipStreamSynchronize(s);
     (void)hipFree(buf);
-    if (rc) g->lm_key.clear();
+    if (rc) {
+        g->lm_key.clear();
+        return rc;
+    }
+    // Sources hanging off the same high-degree vertex see the graph at
+    // nearly the same levels (one level further out than it): batched
+    // together they gather fuller slices.
+    g->lm_top.assign(nn, -1);
+    for (int32_t v = 0; v < n; ++v) {
+        int32_t best = -1, bd = -1;
+        for (int32_t j = rp[v]; j < rp[v + 1]; ++j) {
+            const int32_t w = col[j], dw = rp[w + 1] - rp[w];
+            if (dw > bd || (dw == bd && w < best)) {
+                bd = dw;
+                best = w;
+            }
+        }
+        g->lm_top[(size_t)v] = best;
+    }
     return rc;
 }
 
@@ -494,7 +512,7 @@ extern "C" int cent_graph_create(cent_ctx* ctx, const int32_t* row_ptr, const in
         return st;
     }
     g->d.multi = g->multi;
-    st = lm_keys(ctx, g, urp);
+    st = lm_keys(ctx, g, urp, ucol);
     if (st) {
         cent_graph_free(g);
         return st;

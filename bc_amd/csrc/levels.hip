@@ -787,11 +787,13 @@ static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
     // lanes tend to reach each vertex at the same levels, so a level's
     // gathers read fewer and fuller slices of the level-sorted rows.  Key:
     // the sum of the source's distances to the graph's landmarks (the
-    // highest-degree vertices), then those distances in landmark order,
-    // then degree (R-MAT 2^20, 4096 sources: about 11% fewer 128-byte row
-    // lines per step than grouping by the distance to the top vertex alone;
-    // sources in caller order: 30% more).  Without landmark keys: the BFS
-    // distance to the highest-degree vertex, then degree.
+    // highest-degree vertices), then its highest-degree neighbour (that
+    // neighbour's degree, then its id: sources hanging off one vertex see
+    // the graph at nearly the same levels), then the distances in landmark
+    // order, then degree (R-MAT 2^20, 4096 sources: about 12% fewer
+    // 128-byte row lines per step than grouping by the distance to the top
+    // vertex alone; sources in caller order: 30% more).  Without landmark
+    // keys: the BFS distance to the highest-degree vertex, then degree.
     // order[i] = caller index of slot i.
     const int lm = k > 0 ? level_mode(g, opts, with_bc) : CENT_FRONTIER_DENSE;
     std::vector<int32_t> order;
@@ -801,7 +803,7 @@ static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
         for (int32_t i = 0; i < k; ++i) order[(size_t)i] = i;
         struct Key {
             uint64_t k0, k1;
-            int32_t sum, deg;
+            int32_t sum, deg, tdeg, top;
         };
         std::vector<Key> key((size_t)k);
         for (int32_t i = 0; i < k; ++i) {
@@ -809,11 +811,16 @@ static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
             const int32_t x = g->new_id.empty() ? s0 : g->new_id[(size_t)s0];
             Key& q = key[(size_t)i];
             q.deg = g->h_rp[(size_t)x + 1] - g->h_rp[(size_t)x];
+            q.tdeg = q.top = 0;
             if (!g->lm_key.empty()) {
                 q.k0 = g->lm_key[2 * (size_t)x];
                 q.k1 = g->lm_key[2 * (size_t)x + 1];
                 q.sum = 0;
                 for (int b = 0; b < 64; b += 4) q.sum += (int32_t)((q.k0 >> b) & 15) + (int32_t)((q.k1 >> b) & 15);
+                if (!g->lm_top.empty() && g->lm_top[(size_t)x] >= 0) {
+                    q.top = g->lm_top[(size_t)x];
+                    q.tdeg = g->h_rp[(size_t)q.top + 1] - g->h_rp[(size_t)q.top];
+                }
             } else {
                 q.k0 = q.k1 = 0;
                 q.sum = g->probe_dist[(size_t)s0];
@@ -822,6 +829,8 @@ static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
         std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
             const Key &ka = key[(size_t)a], &kb = key[(size_t)b];
             if (ka.sum != kb.sum) return ka.sum < kb.sum;
+            if (ka.tdeg != kb.tdeg) return ka.tdeg > kb.tdeg;
+            if (ka.top != kb.top) return ka.top < kb.top;
             if (ka.k0 != kb.k0) return ka.k0 < kb.k0;
             if (ka.k1 != kb.k1) return ka.k1 < kb.k1;
             if (ka.deg != kb.deg) return ka.deg > kb.deg;
