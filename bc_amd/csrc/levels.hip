@@ -880,14 +880,15 @@ static int run_batches(cent_ctx* ctx, const cent_graph* g, const int32_t* src, i
     SpPlan plan;
     if (sparse) TRY(sp_prepare(ctx, g->d, sp, plan));
     // closeness on the dense step takes up to kMultiBatch batches per launch
-    // group, halved while that leaves fewer than four groups (two per stream)
-    // and a group keeps eight or more (R-MAT 2^20, 4096 sources: 16 batches a
-    // group 9.7 ms, 32 10.2 ms; all-vertex on a 148k-vertex R-MAT: 32 a group
-    // 68.7 ms, 16 85.3 ms, 8 108 ms)
+    // group, halved while that leaves fewer than two groups (one per stream)
+    // and a group keeps eight or more.  R-MAT 2^20 from 2048 / 4096 / 8192
+    // sources: 4.5 / 7.9 / 12.9-13.5 ms, against 5.0 / 8.15 / 13.9 ms when
+    // halving down to four groups; all-vertex on a 148k-vertex R-MAT: 32 a
+    // group 68.7 ms, 16 85.3 ms, 8 108 ms.
     int gb = kMultiBatch;
     {
         const int nbt = (int)((k + kWave - 1) / kWave);
-        while (gb > 8 && (nbt + gb - 1) / gb < 4) gb /= 2;
+        while (gb > 8 && (nbt + gb - 1) / gb < 2) gb /= 2;
     }
     const int32_t step = (!with_bc && !sparse) ? kWave * gb : kWave;
     // Level bound of a launch group on the dense step: a BFS from s ends
@@ -1419,7 +1420,7 @@ static size_t ws_plain(int32_t n, size_t nseg, size_t nchunk, int32_t depth, int
     auto dense = [&](WsEst& w, bool may_dual) {
         int gb = kMultiBatch;
         const int nbt = (int)((K + kWave - 1) / kWave);
-        while (gb > 8 && (nbt + gb - 1) / gb < 4) gb /= 2;
+        while (gb > 8 && (nbt + gb - 1) / gb < 2) gb /= 2;  // as run_batches
         const int nb = bc ? 1 : std::min(gb, nbt);
         ws_dense(w, N, nseg, bc, compat, nb, G, levels);
         const size_t step = bc ? (size_t)kWave : (size_t)kWave * gb;
